@@ -1,0 +1,117 @@
+"""Inputs and outputs of the plane-current PEEC solve, with the reference's names and meanings.
+
+Grid / PlaneLayer / Branch / TraceSegment / Solution / SolverError mirror `src/groundtruth/solver/peec.py`
+(Grid :197-223, PlaneLayer :987-993, Branch :996-1006, TraceSegment :1051-1058, via_impedance :1061-1072,
+Solution :1092-1110, SolverError :124-127) so a caller of the reference's solver passes the same objects.
+Only the fields the solve reads are kept on Grid (copper masks and pitch); rasterisation is not part of this path.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+
+MU0 = 4e-7 * math.pi
+SIGMA_CU = 5.8e7                # S/m, annealed copper at 20 C
+VIA_PLATING = 25e-6             # m, a via barrel's plating
+
+
+class SolverError(RuntimeError):
+    """A model that cannot be solved (no copper where current must go, unbalanced injections, GMRES not
+    converging): reported by the caller, not a bug (peec.py:124-127)."""
+
+
+class DeviceError(RuntimeError):
+    """The HIP library failed (missing library, out of device memory, a failed launch)."""
+
+
+@dataclass
+class Grid:
+    """One layer's copper on a square grid of pitch d (mm): copper nodes and the strips joining neighbours."""
+    x0: float
+    y0: float
+    d: float                    # mm
+    nx: int
+    ny: int
+    node: np.ndarray            # (nx, ny) bool
+    xb: np.ndarray              # (nx-1, ny) bool: x strip between (i, j) and (i+1, j)
+    yb: np.ndarray              # (nx, ny-1) bool
+
+    @classmethod
+    def from_mask(cls, node: np.ndarray, d_mm: float, x0: float = 0.0, y0: float = 0.0) -> "Grid":
+        """A grid whose strips join every pair of neighbouring copper nodes."""
+        node = np.asarray(node, bool)
+        return cls(x0, y0, d_mm, node.shape[0], node.shape[1], node,
+                   node[:-1, :] & node[1:, :], node[:, :-1] & node[:, 1:])
+
+
+@dataclass
+class PlaneLayer:
+    """A copper layer: name, height z (m), rasterised copper, copper thickness (m)."""
+    name: str
+    z: float
+    grid: Grid
+    thickness: float = 35e-6
+
+
+@dataclass
+class Branch:
+    """A lumped connection between two grid nodes (layer, i, j), impedance z(f)."""
+    a: tuple
+    b: tuple
+    z: Callable[[float], complex]
+    name: str = ""
+    kind: str = "part"
+    radius: float = 1.5e-4
+
+
+@dataclass
+class TraceSegment:
+    """A straight piece of the known current path from a to b (m), `i` amps per amp of loop current."""
+    a: np.ndarray
+    b: np.ndarray
+    width: float
+    i: float = 1.0
+
+
+def filament_self(length: float, rho: float) -> float:
+    """Partial self inductance of a straight filament of radius rho (the mutual of two parallel filaments rho
+    apart, Neumann's integral in closed form [Grover 1946])."""
+    def g(u):
+        return u * math.asinh(u / rho) - math.sqrt(u * u + rho * rho)
+    return MU0 / (4 * math.pi) * (g(length) - 2 * g(0.0) + g(-length))
+
+
+def via_impedance(length_m: float, drill_m: float, sigma: float = SIGMA_CU, filled: bool = False,
+                  plating: float = VIA_PLATING) -> Callable[[float], complex]:
+    """A plated barrel: R of its plating ring (or the whole hole when filled) + jw L of a filament at its radius
+    (peec.py:1061-1072)."""
+    r = max(drill_m / 2, 5e-5)
+    L = filament_self(max(length_m, 1e-6), r)
+    area = math.pi * r * r if filled else math.pi * 2 * r * plating
+    R = max(length_m, 1e-6) / (sigma * area)
+    return lambda f, R=R, L=L: complex(R, 2 * math.pi * f * L)
+
+
+@dataclass
+class Solution:
+    """One solve per amp of loop current (peec.py:1092-1110): strip currents, node potentials, branch currents,
+    loop impedance, dipole moment, iterations; `error` is the refinement's stated relative error bound where a
+    mixed-precision solve reports one."""
+    f: float
+    I_x: np.ndarray
+    I_y: np.ndarray
+    V: np.ndarray
+    I_branch: np.ndarray
+    Z_loop: complex
+    m: np.ndarray
+    iterations: int
+    x: Optional[np.ndarray] = field(default=None, repr=False)
+    residual: float = float("nan")
+
+    @property
+    def L_loop(self) -> float:
+        """Loop inductance (H): Im(Z_loop) / w."""
+        return self.Z_loop.imag / (2 * math.pi * self.f)
