@@ -1,0 +1,160 @@
+"""Benchmark of the plane-current PEEC solve on one MI355X.
+
+Workload (default): the two-layer board, 256 x 256 cells per layer at 0.25 mm, a 0.5 mm slot in the lower plane,
+20 stitching vias, one Manhattan route, solved at 10 frequencies log-spaced 1 MHz - 1 GHz, plain FP64, GMRES(50)
+to a relative residual of 1e-10. One step = the 10 solves (one route at every frequency) with the board resident.
+value = solves per second over the timed steps.
+
+Also reported: the L_p product's (the FFT matvec's) roofline, timed live with HIP events on the library's stream
+over repeated launches, its algorithmic bytes per launch (strip currents in and out, one read and one write of
+the padded spectra per transform, the kernel quadrant once), and the oracle's CPU time on a bounded sample of the
+same workload.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PEEC solves/sec + matvec HBM GB/s; p50/p99 what-if latency vs plain FP64 at 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+def lp_algorithmic_bytes(st) -> int:
+    """B = sum over orientations of [2c n_o + 4c S P0 P1 + r S^2 (P0/2+1)(P1/2+1)], c = 16, r = 8 (FP64)."""
+    c, r, S = 16, 8, st.n_sheets
+    total = 0
+    for P, n_o in ((st.Px, st.nxb), (st.Py, st.nyb)):
+        if n_o:
+            total += 2 * c * n_o + 4 * c * S * P[0] * P[1] + r * S * S * (P[0] // 2 + 1) * (P[1] // 2 + 1)
+    return total
+
+
+def time_lp(st, reps: int = 200):
+    """Average duration of one L_p product on the library's stream (HIP events on that stream)."""
+    import ctypes as C
+
+    import torch
+
+    from groundtruth_amd import _lib
+    dev = f"cuda:{st.device}"
+    rng = np.random.default_rng(4)
+    I = torch.from_numpy(rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)).to(dev)
+    out = torch.empty_like(I)
+    L = _lib.lib()
+    h = st.handle
+    stream = torch.cuda.ExternalStream(L.gt_ctx_stream(st.ctx.handle), device=dev)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        _lib.check(L.gt_lp_matvec(h, C.c_void_p(I.data_ptr()), C.c_void_p(out.data_ptr())))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        L.gt_lp_matvec(h, C.c_void_p(I.data_ptr()), C.c_void_p(out.data_ptr()))
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def cpu_baseline(board, route, f: float) -> dict:
+    """The oracle (numpy / scipy restatement of the reference's path) solving one frequency of the workload on
+    this host: its factorisation and its GMRES, timed once after a warm-up of the kernels."""
+    from oracle import peec as O
+    st = O.PlaneStack(board.layers, board.branches)
+    ex = O.RouteExcitation(st, route.trace, route.driver, route.receiver)
+    t0 = time.perf_counter()
+    st.operator(f)
+    t1 = time.perf_counter()
+    sol = ex.solve(f, tol=board.tol)
+    t2 = time.perf_counter()
+    return {"value": 1.0 / (t2 - t0), "unit": "solves/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"1 of the 10 solves (f = {f:.4g} Hz, {sol.iterations} iterations): factorisation "
+                      f"{t1 - t0:.2f} s + GMRES {t2 - t1:.2f} s; scipy FFT on all {os.cpu_count()} cores, "
+                      f"SuperLU 1 thread; {platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="two_layer")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported in this build "
+              "(independent boards and frequencies shard across GPUs without collectives; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from groundtruth_amd import boards, peec
+    board = boards.WORKLOADS[args.workload]()
+    route = board.routes[0]
+    t_setup = time.perf_counter()
+    st = peec.PlaneStack(board.layers, board.branches, device=0)
+    ex = peec.RouteExcitation(st, route.trace, route.driver, route.receiver)
+    st.handle
+    ops = [st.operator(f) for f in board.freqs]
+    setup_s = time.perf_counter() - t_setup
+
+    def step():
+        its = []
+        for f in board.freqs:
+            its.append(ex.solve(f, tol=board.tol, restart=board.restart).iterations)
+        return its
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(args.steps):
+        its = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    solves = args.steps * len(board.freqs)
+    t_lp = time_lp(st)
+    B = lp_algorithmic_bytes(st)
+    achieved = B / t_lp / 1e9
+    out = {
+        "metric": METRIC,
+        "value": solves / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded board generator, default_rng(1002))",
+        "config": {"workload": f"{args.workload}: {len(board.layers)} x {st.nx}x{st.ny} cells, "
+                               f"{len(board.freqs)} frequencies 1 MHz-1 GHz, GMRES({board.restart}) to "
+                               f"{board.tol:g}, plain FP64",
+                   "unknowns": st.nb + st.n_keep, "strips": st.ns, "vias": st.nbr,
+                   "iterations": its, "setup_s": round(setup_s, 3), "fft": [list(st.Px), list(st.Py)]},
+        "matvec_GBps": achieved,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "L_p product (lp_rows_fwd + lp_cols_mac + lp_rows_inv)",
+                     "bytes_per_launch": B, "us_per_launch": t_lp * 1e6},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

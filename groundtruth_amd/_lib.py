@@ -1,0 +1,159 @@
+"""ctypes binding of libgtpeec.so (include/gtpeec.h). The library is loaded from this package's directory; a
+missing library is an error (there is no fallback path)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .problem import DeviceError, SolverError
+
+LIB_PATH = Path(__file__).resolve().parent / "libgtpeec.so"
+
+GT_OK, GT_ERR_ARG, GT_ERR_OOM, GT_ERR_DEVICE, GT_ERR_NOCONV, GT_ERR_NUMERIC, GT_ERR_UNSUPPORTED = range(7)
+
+i32p = C.POINTER(C.c_int32)
+f64p = C.POINTER(C.c_double)
+
+
+class StackDesc(C.Structure):
+    _fields_ = [("n_sheets", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nxb", C.c_int32),
+                ("nyb", C.c_int32), ("nbr", C.c_int32), ("n_keep", C.c_int32), ("x_sij", i32p), ("y_sij", i32p),
+                ("a_keep", i32p), ("b_keep", i32p), ("node_sheet", i32p), ("px", C.c_int32 * 2),
+                ("py", C.c_int32 * 2), ("kx", f64p), ("ky", f64p)]
+
+
+class OpDesc(C.Structure):
+    _fields_ = [("w", C.c_double), ("r_sheet", f64p), ("z_tail", f64p), ("dinv_sheet", f64p), ("dinv_tail", f64p),
+                ("s_scale", C.c_double * 2), ("k", C.c_int32), ("q", f64p)]
+
+
+class GmresParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("bnorm", C.c_double), ("restart", C.c_int32), ("max_cycles", C.c_int32),
+                ("extra_cycles", C.c_int32), ("max_its", C.c_int32), ("check_misses", C.c_int32)]
+
+
+class GmresResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("products", C.c_int32),
+                ("residual", C.c_double)]
+
+
+# every symbol the header declares, with its ctypes signature
+SIGNATURES = {
+    "gt_last_error": (C.c_char_p, []),
+    "gt_abi_version": (C.c_int, []),
+    "gt_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "gt_ctx_destroy": (None, [C.c_void_p]),
+    "gt_ctx_sync": (C.c_int, [C.c_void_p]),
+    "gt_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "gt_stack_create": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(C.c_void_p)]),
+    "gt_stack_destroy": (None, [C.c_void_p]),
+    "gt_stack_device_bytes": (C.c_int64, [C.c_void_p]),
+    "gt_lp_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gt_schur_factor": (C.c_int, [C.c_void_p, i32p, f64p]),
+    "gt_schur_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), i32p, i32p, i32p]),
+    "gt_schur_solve_host": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "gt_schur_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gt_schur_host_factor": (C.c_int, [C.c_int32, C.c_int32, i32p, i32p, i32p, C.POINTER(C.c_void_p)]),
+    "gt_schur_host_solve": (C.c_int, [C.c_void_p, f64p, f64p, C.c_int32]),
+    "gt_schur_host_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), i32p, i32p, i32p]),
+    "gt_schur_host_free": (None, [C.c_void_p]),
+    "gt_op_create": (C.c_int, [C.c_void_p, C.POINTER(OpDesc), C.POINTER(C.c_void_p)]),
+    "gt_op_destroy": (None, [C.c_void_p]),
+    "gt_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gt_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gt_gmres": (C.c_int, [C.c_void_p, f64p, f64p, C.POINTER(GmresParams), C.POINTER(GmresResult)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded library (DeviceError when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise DeviceError(f"{LIB_PATH} is missing: build it first (python -m groundtruth_amd.build)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code: int, what: str = ""):
+    """Raise the exception a status code stands for (SolverError, MemoryError, DeviceError, ValueError)."""
+    if code == GT_OK:
+        return
+    msg = lib().gt_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if code in (GT_ERR_NOCONV, GT_ERR_NUMERIC):
+        raise SolverError(text)
+    if code == GT_ERR_OOM:
+        raise MemoryError(text)
+    if code == GT_ERR_ARG:
+        raise ValueError(text)
+    if code == GT_ERR_UNSUPPORTED:
+        raise NotImplementedError(text)
+    raise DeviceError(text)
+
+
+def ptr(a: np.ndarray, kind=f64p):
+    """A ctypes pointer to a contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(kind)
+
+
+def cplx(a) -> np.ndarray:
+    """A complex128 array as interleaved doubles (contiguous)."""
+    return np.ascontiguousarray(np.asarray(a, np.complex128)).view(np.float64)
+
+
+class Context:
+    """One HIP device and its stream (the library's gt_ctx)."""
+
+    _by_device: dict = {}
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().gt_ctx_create(int(device), C.byref(h)), "gt_ctx_create")
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def get(cls, device: int = 0) -> "Context":
+        ctx = cls._by_device.get(device)
+        if ctx is None:
+            ctx = cls._by_device[device] = Context(device)
+        return ctx
+
+    def sync(self):
+        check(lib().gt_ctx_sync(self.handle), "gt_ctx_sync")
+
+    def __del__(self):
+        try:
+            if self.handle and _LIB is not None:
+                _LIB.gt_ctx_destroy(self.handle)
+        except Exception:       # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+def exported_symbols() -> list:
+    """Names of the header's entry points that the library exports (every one, once built)."""
+    L = lib()
+    return [n for n in SIGNATURES if hasattr(L, n)]
+
+
+def device_available() -> bool:
+    """Whether a HIP device is visible (without touching the GPU when HIP_VISIBLE_DEVICES hides them all)."""
+    if os.environ.get("HIP_VISIBLE_DEVICES", None) == "":
+        return False
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:       # noqa: BLE001
+        return False

@@ -1,0 +1,472 @@
+// The C ABI of libgtpeec (include/gtpeec.h): argument checks, uploads, and error codes for everything below it.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "stack.hpp"
+
+namespace gt {
+void gmres_solve(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params& p, gt_gmres_result& res);
+}
+
+using namespace gt;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return GT_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return GT_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return GT_ERR_ARG;
+    }
+}
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
+
+void need(bool ok, const char* msg) {
+    if (!ok) fail(GT_ERR_ARG, msg);
+}
+
+const DevBuf<zd>& twiddles(gt_stack* st, int P) {
+    auto it = st->tw.find(P);
+    if (it != st->tw.end()) return it->second;
+    std::vector<zd> h(P);
+    std::vector<zf> hf(P);
+    for (int m = 0; m < P; ++m) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * m / P;
+        h[m] = zd((double)cosl(a), (double)-sinl(a));
+        hf[m] = zf((float)h[m].re, (float)h[m].im);
+    }
+    DevBuf<zd> d;
+    d.upload(h, st->ctx->stream);
+    DevBuf<zf> df;
+    df.upload(hf, st->ctx->stream);
+    st->twf.emplace(P, std::move(df));
+    return st->tw.emplace(P, std::move(d)).first->second;
+}
+
+void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int n0, int n1, const int32_t* P,
+                  const double* K) {
+    OrientHost& o = st->o[oi];
+    o.present = cnt > 0;
+    o.cnt = cnt;
+    o.lo = lo;
+    o.n0 = n0;
+    o.n1 = n1;
+    if (!o.present) return;
+    need(K != nullptr, "kernel spectrum missing");
+    need(pow2(P[0]) && pow2(P[1]), "FFT sizes must be powers of two");
+    need(P[0] >= 2 * n0 - 1 && P[1] >= 2 * n1 - 1, "FFT sizes below 2n - 1 would wrap the convolution");
+    need(n0 < 65536 && n1 < 65536, "grid above 65535 cells");
+    o.P0 = P[0];
+    o.P1 = P[1];
+    o.log0 = ilog2(P[0]);
+    o.log1 = ilog2(P[1]);
+    const int S = st->S;
+    if ((size_t)S * o.P0 > (size_t)LpPts<double>::cols)
+        fail(GT_ERR_UNSUPPORTED, "sheets x column FFT size above this build's LDS budget (S * P0 <= 4096)");
+    if (o.P1 + 1 > LpPts<double>::rows)
+        fail(GT_ERR_UNSUPPORTED, "row FFT size above this build's LDS budget (P1 <= 2048)");
+    o.TR = std::max(1, std::min(8, LpPts<double>::rows / (o.P1 + 1)));
+    // rows of the dense (sheet, i) grid as CSR over the sorted strips
+    std::vector<int> rowptr((size_t)S * n0 + 1, 0), rowcol(cnt);
+    int prev = -1;
+    for (int k = 0; k < cnt; ++k) {
+        const int s = sij[3 * k], i = sij[3 * k + 1], j = sij[3 * k + 2];
+        need(s >= 0 && s < S && i >= 0 && i < n0 && j >= 0 && j < n1, "strip outside its grid");
+        const int key = (s * n0 + i) * n1 + j;
+        need(key > prev, "strips must be sorted by (sheet, i, j) without repeats");
+        prev = key;
+        rowptr[s * n0 + i + 1]++;
+        rowcol[k] = (i << 16) | j;
+        st->h_strip_sheet.push_back((unsigned char)s);
+    }
+    for (size_t r = 0; r + 1 < rowptr.size(); ++r) rowptr[r + 1] += rowptr[r];
+    o.rowptr.upload(rowptr, st->ctx->stream);
+    o.rowcol.upload(rowcol, st->ctx->stream);
+    const size_t kq = (size_t)(o.P1 / 2 + 1) * (o.P0 / 2 + 1) * S * S;
+    std::vector<double> kd(K, K + kq);
+    std::vector<float> kf(kq);
+    for (size_t e = 0; e < kq; ++e) kf[e] = (float)kd[e];
+    o.K.upload(kd, st->ctx->stream);
+    o.Kf.upload(kf, st->ctx->stream);
+    twiddles(st, o.P0);
+    twiddles(st, o.P1);
+    st->t_elems = std::max(st->t_elems, (size_t)S * o.P1 * o.n0);
+    st->shape.lds_rows = std::max(st->shape.lds_rows, (size_t)o.TR * (o.P1 + 1) * sizeof(zd));
+    st->shape.lds_cols = std::max(st->shape.lds_cols, (size_t)S * o.P0 * sizeof(zd));
+}
+
+// B = Ar^T Ar from the branches' kept ends: its off-diagonal edges and its diagonal (degrees, edges to reference
+// nodes included)
+void laplacian(int nk, int nb, const int* a_keep, const int* b_keep, std::vector<int>& ea, std::vector<int>& eb,
+               std::vector<double>& diag) {
+    diag.assign(nk, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        const int ka = a_keep[b], kb = b_keep[b];
+        need(ka >= -1 && ka < nk && kb >= -1 && kb < nk, "kept node index out of range");
+        if (ka >= 0) diag[ka] += 1.0;
+        if (kb >= 0) diag[kb] += 1.0;
+        if (ka >= 0 && kb >= 0) {
+            need(ka != kb, "a branch joins a node to itself");
+            ea.push_back(ka);
+            eb.push_back(kb);
+        }
+    }
+}
+
+}  // namespace
+
+struct gt_schur_host_s {
+    SchurHost h;
+};
+
+LpArgs<double> gt_stack_s::lp_args(const zd* x, zd* y, int mode) const {
+    LpArgs<double> a{};
+    for (int oi = 0; oi < 2; ++oi) {
+        const OrientHost& h = o[oi];
+        OrientArgs<double>& d = a.o[oi];
+        d.n0 = h.n0;
+        d.n1 = h.n1;
+        d.P0 = h.P0;
+        d.P1 = h.P1;
+        d.log0 = h.log0;
+        d.log1 = h.log1;
+        d.TR = h.TR;
+        d.lo = h.lo;
+        d.cnt = h.cnt;
+        d.rowptr = h.rowptr.p;
+        d.rowcol = h.rowcol.p;
+        d.K = h.K.p;
+        d.T1 = work.T1.p;
+        d.T2 = work.T2.p;
+        d.tw0 = h.present ? tw.at(h.P0).p : nullptr;
+        d.tw1 = h.present ? tw.at(h.P1).p : nullptr;
+        a.nblk_rows[oi] = h.present ? S * ((h.n0 + h.TR - 1) / h.TR) : 0;
+        a.nblk_cols[oi] = h.present ? h.P1 : 0;
+    }
+    a.S = S;
+    a.mode = mode;
+    a.nb = nb;
+    a.x = x;
+    a.y = y;
+    a.a_keep = a_keep.p;
+    a.b_keep = b_keep.p;
+    a.r_sheet = nullptr;
+    a.w = 0;
+    return a;
+}
+
+int64_t gt_stack_s::device_bytes() const {
+    int64_t b = 0;
+    for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + h.K.bytes() + h.Kf.bytes();
+    for (const auto& t : tw) b += t.second.bytes();
+    for (const auto& t : twf) b += t.second.bytes();
+    b += a_keep.bytes() + b_keep.bytes() + adj_ptr.bytes() + adj.bytes() + node_sheet.bytes() + strip_sheet.bytes();
+    b += work.T1.bytes() + work.T2.bytes() + work.s.bytes() + work.z.bytes() + work.t.bytes() + W.bytes();
+    if (schur) b += schur->device_bytes();
+    return b;
+}
+
+extern "C" {
+
+const char* gt_last_error(void) { return g_err.c_str(); }
+
+int gt_abi_version(void) { return GT_ABI_VERSION; }
+
+int gt_ctx_create(int device, gt_ctx** out) {
+    return guard([&] {
+        need(out != nullptr, "out is NULL");
+        *out = nullptr;
+        int count = 0;
+        GT_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) fail(GT_ERR_DEVICE, "no HIP device " + std::to_string(device));
+        GT_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        GT_HIP(hipGetDeviceProperties(&prop, device));
+        auto* c = new gt_ctx_s;
+        c->device = device;
+        c->lds_max = prop.sharedMemPerBlock;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            fail(GT_ERR_DEVICE, "hipStreamCreate failed");
+        }
+        *out = c;
+    });
+}
+
+void gt_ctx_destroy(gt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int gt_ctx_sync(gt_ctx* ctx) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        GT_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void* gt_ctx_stream(gt_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
+    return guard([&] {
+        need(ctx && d && out, "NULL argument");
+        *out = nullptr;
+        need(d->n_sheets >= 1 && d->n_sheets <= LP_MAX_SHEETS, "1 to 8 current sheets");
+        need(d->nx >= 2 && d->ny >= 2, "grid below 2 x 2");
+        need(d->nxb >= 0 && d->nyb >= 0 && d->nbr >= 0 && d->n_keep >= 0, "negative count");
+        GT_HIP(hipSetDevice(ctx->device));
+        std::unique_ptr<gt_stack_s> st(new gt_stack_s);
+        st->ctx = ctx;
+        st->S = d->n_sheets;
+        st->nx = d->nx;
+        st->ny = d->ny;
+        st->nxb = d->nxb;
+        st->nyb = d->nyb;
+        st->nbr = d->nbr;
+        st->ns = d->nxb + d->nyb;
+        st->nb = st->ns + d->nbr;
+        st->nk = d->n_keep;
+        hipStream_t s = ctx->stream;
+        setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx);
+        setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky);
+        st->strip_sheet.upload(st->h_strip_sheet.empty() ? std::vector<unsigned char>{0} : st->h_strip_sheet, s);
+        const int nb = st->nb, nk = st->nk;
+        need(nb == 0 || (d->a_keep && d->b_keep), "incidence missing");
+        st->h_a_keep.assign(d->a_keep, d->a_keep + nb);
+        st->h_b_keep.assign(d->b_keep, d->b_keep + nb);
+        std::vector<int> aptr(nk + 1, 0);
+        for (int b = 0; b < nb; ++b) {
+            const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
+            need(ka >= -1 && ka < nk && kb >= -1 && kb < nk, "kept node index out of range");
+            if (ka >= 0) aptr[ka + 1]++;
+            if (kb >= 0) aptr[kb + 1]++;
+        }
+        for (int v = 0; v < nk; ++v) aptr[v + 1] += aptr[v];
+        std::vector<int> adj(std::max(1, aptr[nk]));
+        std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+        for (int b = 0; b < nb; ++b) {
+            if (st->h_a_keep[b] >= 0) adj[fill[st->h_a_keep[b]]++] = b << 1;
+            if (st->h_b_keep[b] >= 0) adj[fill[st->h_b_keep[b]]++] = (b << 1) | 1;
+        }
+        st->a_keep.upload(st->h_a_keep.empty() ? std::vector<int>{0} : st->h_a_keep, s);
+        st->b_keep.upload(st->h_b_keep.empty() ? std::vector<int>{0} : st->h_b_keep, s);
+        st->adj_ptr.upload(aptr, s);
+        st->adj.upload(adj, s);
+        std::vector<int> ns_(std::max(1, nk), 0);
+        if (nk) {
+            need(d->node_sheet != nullptr, "node_sheet missing");
+            for (int v = 0; v < nk; ++v) {
+                need(d->node_sheet[v] >= 0 && d->node_sheet[v] < st->S, "node sheet out of range");
+                ns_[v] = d->node_sheet[v];
+            }
+        }
+        st->node_sheet.upload(ns_, s);
+        st->work.T1.alloc(std::max<size_t>(1, st->t_elems));
+        st->work.T2.alloc(std::max<size_t>(1, st->t_elems));
+        st->work.s.alloc(std::max(1, nk));
+        st->work.z.alloc(std::max(1, nk));
+        st->work.t.alloc(std::max(1, d->nbr));
+        GT_HIP(hipStreamSynchronize(s));
+        *out = st.release();
+    });
+}
+
+void gt_stack_destroy(gt_stack* st) {
+    if (!st) return;
+    (void)hipSetDevice(st->ctx->device);
+    (void)hipStreamSynchronize(st->ctx->stream);
+    delete st;
+}
+
+int64_t gt_stack_device_bytes(const gt_stack* st) { return st ? st->device_bytes() : 0; }
+
+int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
+    return guard([&] {
+        need(st && I_dev && out_dev, "NULL argument");
+        if (st->ns == 0) return;
+        LpArgs<double> a = st->lp_args((const zd*)I_dev, (zd*)out_dev, 0);
+        launch_lp(a, st->shape, st->ctx->stream);
+    });
+}
+
+int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
+    return guard([&] {
+        need(st && node_xyz, "NULL argument");
+        const int nk = st->nk, nb = st->nb;
+        std::vector<int> ea, eb;
+        std::vector<double> diag;
+        laplacian(nk, nb, st->h_a_keep.data(), st->h_b_keep.data(), ea, eb, diag);
+        auto S = std::make_unique<Schur>();
+        S->h.build(nk, ea, eb, diag, node_xyz);
+        S->upload(st->ctx->stream);
+        // Woodbury columns W = B^-1 U (U: the branches' incidence), two real columns per complex solve
+        const int k = st->nbr;
+        std::vector<double> W((size_t)k * nk, 0.0);
+        std::vector<double> bb(2 * (size_t)nk), xx(2 * (size_t)nk);
+        for (int m = 0; m < k; m += 2) {
+            std::fill(bb.begin(), bb.end(), 0.0);
+            for (int c = 0; c < 2 && m + c < k; ++c) {
+                const int b = st->ns + m + c;
+                if (st->h_a_keep[b] >= 0) bb[2 * (size_t)st->h_a_keep[b] + c] += 1.0;
+                if (st->h_b_keep[b] >= 0) bb[2 * (size_t)st->h_b_keep[b] + c] -= 1.0;
+            }
+            S->h.solve_host(bb.data(), xx.data());
+            for (int c = 0; c < 2 && m + c < k; ++c)
+                for (int v = 0; v < nk; ++v) W[(size_t)(m + c) * nk + v] = xx[2 * (size_t)v + c];
+        }
+        if (G_out) {
+            for (int r = 0; r < k; ++r) {
+                const int b = st->ns + r;
+                const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
+                for (int c = 0; c < k; ++c) {
+                    double v = 0;
+                    if (ka >= 0) v += W[(size_t)c * nk + ka];
+                    if (kb >= 0) v -= W[(size_t)c * nk + kb];
+                    G_out[(size_t)r * k + c] = v;
+                }
+            }
+        }
+        st->W.upload(W.empty() ? std::vector<double>{0.0} : W, st->ctx->stream);
+        st->k = k;
+        GT_HIP(hipStreamSynchronize(st->ctx->stream));
+        st->schur = std::move(S);
+    });
+}
+
+int gt_schur_stats(const gt_stack* st, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels, int32_t* max_cols) {
+    return guard([&] {
+        need(st && st->schur, "no Schur factor");
+        const SchurHost& h = st->schur->h;
+        if (nnz_L) *nnz_L = h.nnz_L;
+        if (n_super) *n_super = (int32_t)h.c0.size();
+        if (n_levels) *n_levels = (int32_t)h.fwd_levels_ptr.size() - 1;
+        if (max_cols) *max_cols = h.max_cols;
+    });
+}
+
+int gt_schur_solve_host(const gt_stack* st, const double* b, double* x) {
+    return guard([&] {
+        need(st && st->schur && b && x, "NULL argument or no factor");
+        st->schur->h.solve_host(b, x);
+    });
+}
+
+int gt_schur_solve(gt_stack* st, const void* b_dev, void* x_dev) {
+    return guard([&] {
+        need(st && b_dev && x_dev, "NULL argument");
+        schur_apply(st, (const zd*)b_dev, (zd*)x_dev);
+    });
+}
+
+int gt_schur_host_factor(int32_t n_keep, int32_t nb, const int32_t* a_keep, const int32_t* b_keep,
+                         const int32_t* node_xyz, gt_schur_host** out) {
+    return guard([&] {
+        need(out && (nb == 0 || (a_keep && b_keep)) && (n_keep == 0 || node_xyz), "NULL argument");
+        *out = nullptr;
+        std::vector<int> ea, eb;
+        std::vector<double> diag;
+        laplacian(n_keep, nb, a_keep, b_keep, ea, eb, diag);
+        auto h = std::make_unique<gt_schur_host_s>();
+        h->h.build(n_keep, ea, eb, diag, node_xyz);
+        *out = h.release();
+    });
+}
+
+int gt_schur_host_solve(const gt_schur_host* h, const double* b, double* x, int32_t k) {
+    return guard([&] {
+        need(h && b && x && k >= 0, "bad argument");
+        const size_t n = (size_t)h->h.n;
+        for (int32_t c = 0; c < k; ++c) h->h.solve_host(b + 2 * n * c, x + 2 * n * c);
+    });
+}
+
+int gt_schur_host_stats(const gt_schur_host* hh, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels,
+                        int32_t* max_cols) {
+    return guard([&] {
+        need(hh != nullptr, "NULL handle");
+        const SchurHost& h = hh->h;
+        if (nnz_L) *nnz_L = h.nnz_L;
+        if (n_super) *n_super = (int32_t)h.c0.size();
+        if (n_levels) *n_levels = (int32_t)h.fwd_levels_ptr.size() - 1;
+        if (max_cols) *max_cols = h.max_cols;
+    });
+}
+
+void gt_schur_host_free(gt_schur_host* h) { delete h; }
+
+int gt_op_create(gt_stack* st, const gt_op_desc* d, gt_op** out) {
+    return guard([&] {
+        need(st && d && out, "NULL argument");
+        *out = nullptr;
+        need(d->r_sheet && d->dinv_sheet, "per-sheet impedances missing");
+        need(st->nbr == 0 || (d->z_tail && d->dinv_tail), "branch impedances missing");
+        need(d->k == 0 || (d->k == st->k && d->q != nullptr), "Woodbury rank must be the stack's branch count");
+        need(d->k <= 256, "Woodbury rank above 256");
+        std::unique_ptr<gt_op_s> op(new gt_op_s);
+        op->st = st;
+        op->w = d->w;
+        hipStream_t s = st->ctx->stream;
+        auto up = [&](DevBuf<zd>& dst, const double* src, size_t cnt) {
+            std::vector<zd> v(std::max<size_t>(1, cnt));
+            for (size_t i = 0; i < cnt; ++i) v[i] = zd(src[2 * i], src[2 * i + 1]);
+            dst.upload(v, s);
+        };
+        up(op->r_sheet, d->r_sheet, st->S);
+        up(op->dinv_sheet, d->dinv_sheet, st->S);
+        up(op->z_tail, d->z_tail, st->nbr);
+        up(op->dinv_tail, d->dinv_tail, st->nbr);
+        up(op->q, d->q, (size_t)d->k * d->k);
+        op->s_scale = zd(d->s_scale[0], d->s_scale[1]);
+        op->k = d->k;
+        GT_HIP(hipStreamSynchronize(s));
+        *out = op.release();
+    });
+}
+
+void gt_op_destroy(gt_op* op) { delete op; }
+
+int gt_matvec(gt_op* op, const void* x_dev, void* y_dev) {
+    return guard([&] {
+        need(op && x_dev && y_dev && x_dev != y_dev, "NULL or aliased argument");
+        op_matvec(op, (const zd*)x_dev, (zd*)y_dev);
+    });
+}
+
+int gt_precond(gt_op* op, const void* r_dev, void* z_dev) {
+    return guard([&] {
+        need(op && r_dev && z_dev && r_dev != z_dev, "NULL or aliased argument");
+        need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        op_precond(op, (const zd*)r_dev, (zd*)z_dev);
+    });
+}
+
+int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params* p, gt_gmres_result* res) {
+    return guard([&] {
+        need(op && b_host && x_host && p && res, "NULL argument");
+        need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
+        GT_HIP(hipSetDevice(op->st->ctx->device));
+        gmres_solve(op, b_host, x_host, *p, *res);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+// The FFT-convolved partial-inductance product L_p I (peec.py:2054-2135 `_conv` / `Lp_matvec`), fused with the
+// system product's strip rows (peec.py:3043-3062) and its incidence rows.
+//
+// Per strip orientation the currents live on a dense (S, n0, n1) grid (n0 along the strips' axis i, n1 across);
+// the kernel spectrum is real and even, stored as its quadrant. Three passes, each one HBM round trip of the
+// padded spectrum's valid rows (rows i >= n0 of the padded array are zero on the way in and not needed on the way
+// out, so they never touch HBM):
+//   A  rows_fwd : per (sheet, tile of rows): scatter the strips into LDS rows, FFT along j (P1), write T1[s][j][i]
+//                 (transposed, so that B reads whole columns contiguously).
+//   B  cols_mac : per column j: read T1[*][j][0:n0] for every sheet, FFT along i (P0), multiply by the kernel
+//                 quadrant and sum over source sheets per pixel (peec.py:2066-2074, _mac.py:20-34), inverse FFT,
+//                 write rows 0:n0 to T2[t][j][i].
+//   C  rows_inv : per (sheet, tile of rows): read T2, inverse FFT along j, gather at the strips and finish the
+//                 product's strip rows: R I + jw (L_p I) - Ar V.
+// The incidence rows (branch tail and Ar^T I) run in `tail_bottom`.
+#include "fft_lds.hpp"
+#include "lp.hpp"
+
+namespace gt {
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ const OrientArgs<T>& pick(const LpArgs<T>& a, int oi) {
+    return oi ? a.o[1] : a.o[0];
+}
+
+template <class T, int NTH, int MAXPTS>
+__global__ __launch_bounds__(NTH) void lp_rows_fwd(LpArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    int blk = blockIdx.x, oi = 0;
+    if (blk >= a.nblk_rows[0]) { blk -= a.nblk_rows[0]; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int ntiles = (o.n0 + o.TR - 1) / o.TR;
+    const int s = blk / ntiles, i0 = (blk - s * ntiles) * o.TR;
+    const int nr = min(o.TR, o.n0 - i0);
+    const int P1s = o.P1 + 1;
+    for (int e = threadIdx.x; e < nr * P1s; e += NTH) buf[e] = cpx<T>();
+    __syncthreads();
+    const int kb = o.rowptr[s * o.n0 + i0], ke = o.rowptr[s * o.n0 + i0 + nr];
+    const cpx<T>* xin = a.x + o.lo;
+    for (int k = kb + threadIdx.x; k < ke; k += NTH) {
+        const int ij = o.rowcol[k];
+        buf[((ij >> 16) - i0) * P1s + (ij & 0xffff)] = xin[k];
+    }
+    __syncthreads();
+    fft_lds<T, false, NTH, MAXPTS>(buf, o.P1, o.log1, nr, P1s, o.tw1);
+    cpx<T>* dst = o.T1 + (size_t)s * o.P1 * o.n0 + i0;
+    for (int e = threadIdx.x; e < nr * o.P1; e += NTH) {
+        const int r = e % nr, j = e / nr;
+        dst[(size_t)j * o.n0 + r] = buf[r * P1s + j];
+    }
+}
+
+template <class T, int NTH, int MAXPTS, int MAXS>
+__global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    int j = blockIdx.x, oi = 0;
+    if (j >= a.nblk_cols[0]) { j -= a.nblk_cols[0]; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int S = a.S, P0 = o.P0, n0 = o.n0;
+    for (int e = threadIdx.x; e < S * P0; e += NTH) {
+        const int s = e / P0, i = e - s * P0;
+        buf[e] = i < n0 ? o.T1[((size_t)s * o.P1 + j) * n0 + i] : cpx<T>();
+    }
+    __syncthreads();
+    fft_lds<T, false, NTH, MAXPTS>(buf, P0, o.log0, S, P0, o.tw0);
+    // pixel (i, j) of the full spectrum reads quadrant (min(i, P0 - i), min(j, P1 - j)): the spectrum is even
+    const int qj = min(j, o.P1 - j);
+    const int h0 = P0 / 2 + 1;
+    for (int i = threadIdx.x; i < P0; i += NTH) {
+        const int qi = min(i, P0 - i);
+        const T* Kp = o.K + ((size_t)qj * h0 + qi) * S * S;
+        cpx<T> v[MAXS];
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s)
+            if (s < S) v[s] = buf[s * P0 + i];
+#pragma unroll
+        for (int t = 0; t < MAXS; ++t) {
+            if (t < S) {
+                cpx<T> acc;
+#pragma unroll
+                for (int s = 0; s < MAXS; ++s) {
+                    if (s < S) {
+                        const T k = Kp[t * S + s];
+                        acc.re += k * v[s].re;
+                        acc.im += k * v[s].im;
+                    }
+                }
+                buf[t * P0 + i] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    fft_lds<T, true, NTH, MAXPTS>(buf, P0, o.log0, S, P0, o.tw0);
+    for (int e = threadIdx.x; e < S * n0; e += NTH) {
+        const int t = e / n0, i = e - t * n0;
+        o.T2[((size_t)t * o.P1 + j) * n0 + i] = buf[t * P0 + i];
+    }
+}
+
+template <class T, int NTH, int MAXPTS>
+__global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    int blk = blockIdx.x, oi = 0;
+    if (blk >= a.nblk_rows[0]) { blk -= a.nblk_rows[0]; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int ntiles = (o.n0 + o.TR - 1) / o.TR;
+    const int t = blk / ntiles, i0 = (blk - t * ntiles) * o.TR;
+    const int nr = min(o.TR, o.n0 - i0);
+    const int P1s = o.P1 + 1;
+    const cpx<T>* src = o.T2 + (size_t)t * o.P1 * o.n0 + i0;
+    for (int e = threadIdx.x; e < nr * o.P1; e += NTH) {
+        const int r = e % nr, j = e / nr;
+        buf[r * P1s + j] = src[(size_t)j * o.n0 + r];
+    }
+    __syncthreads();
+    fft_lds<T, true, NTH, MAXPTS>(buf, o.P1, o.log1, nr, P1s, o.tw1);
+    const int kb = o.rowptr[t * o.n0 + i0], ke = o.rowptr[t * o.n0 + i0 + nr];
+    for (int k = kb + threadIdx.x; k < ke; k += NTH) {
+        const int ij = o.rowcol[k];
+        const cpx<T> val = buf[((ij >> 16) - i0) * P1s + (ij & 0xffff)];
+        const int b = o.lo + k;
+        if (a.mode == 0) {
+            a.y[b] = val;
+        } else {
+            const cpx<T> xb = a.x[b];
+            const cpx<T> R = a.r_sheet[t];
+            cpx<T> v = R * xb;
+            v.re -= a.w * val.im;
+            v.im += a.w * val.re;
+            const int ka = a.a_keep[b], kk = a.b_keep[b];
+            if (ka >= 0) v = v - a.x[a.nb + ka];
+            if (kk >= 0) v = v + a.x[a.nb + kk];
+            a.y[b] = v;
+        }
+    }
+}
+
+// Branch rows (z I - Ar V) and node rows (Ar^T Ib) of the system product.
+template <class T>
+__global__ __launch_bounds__(256) void tail_bottom(TailArgs<T> a) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx < a.nbr) {
+        const int b = a.ns + idx;
+        cpx<T> v = a.z_tail[idx] * a.x[b];
+        const int ka = a.a_keep[b], kk = a.b_keep[b];
+        if (ka >= 0) v = v - a.x[a.nb + ka];
+        if (kk >= 0) v = v + a.x[a.nb + kk];
+        a.y[b] = v;
+    } else if (idx < a.nbr + a.nk) {
+        const int n = idx - a.nbr;
+        cpx<T> acc;
+        for (int e = a.adj_ptr[n]; e < a.adj_ptr[n + 1]; ++e) {
+            const int code = a.adj[e];
+            const cpx<T> v = a.x[code >> 1];
+            if (code & 1) acc = acc - v;
+            else acc = acc + v;
+        }
+        a.y[a.nb + n] = acc;
+    }
+}
+
+}  // namespace
+
+template <class T>
+void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
+    const int nrow = a.nblk_rows[0] + a.nblk_rows[1];
+    const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
+    constexpr int NTH = 256;
+    hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow), dim3(NTH), sh.lds_rows, st, a);
+    GT_LAUNCH("lp_rows_fwd");
+    hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol), dim3(NTH), sh.lds_cols, st, a);
+    GT_LAUNCH("lp_cols_mac");
+    hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow), dim3(NTH), sh.lds_rows, st, a);
+    GT_LAUNCH("lp_rows_inv");
+}
+
+template <class T>
+void launch_tail_bottom(const TailArgs<T>& a, hipStream_t st) {
+    const int n = a.nbr + a.nk;
+    if (n == 0) return;
+    hipLaunchKernelGGL((tail_bottom<T>), dim3(blocks_for(n, 256)), dim3(256), 0, st, a);
+    GT_LAUNCH("tail_bottom");
+}
+
+template void launch_lp<double>(const LpArgs<double>&, const LpShape&, hipStream_t);
+template void launch_lp<float>(const LpArgs<float>&, const LpShape&, hipStream_t);
+template void launch_tail_bottom<double>(const TailArgs<double>&, hipStream_t);
+template void launch_tail_bottom<float>(const TailArgs<float>&, hipStream_t);
+
+}  // namespace gt

@@ -1,0 +1,71 @@
+// Launch arguments of the L_p product kernels (lp.hip).
+#pragma once
+
+#include "gt_common.hpp"
+
+namespace gt {
+
+constexpr int LP_MAX_SHEETS = 8;
+
+// Points one workgroup's LDS holds in the row passes (a tile of rows) and in the column pass (every sheet's
+// column): 64 KiB of complex values.
+template <class T>
+struct LpPts;
+template <>
+struct LpPts<double> { static constexpr int rows = 4096, cols = 4096; };
+template <>
+struct LpPts<float> { static constexpr int rows = 8192, cols = 8192; };
+
+template <class T>
+struct OrientArgs {
+    int n0, n1;               // dense strip grid: rows (axis i) and columns (axis j)
+    int P0, P1, log0, log1;   // FFT sizes
+    int TR;                   // rows per workgroup in the row passes
+    int lo, cnt;              // the orientation's strips: x[lo, lo + cnt)
+    const int* rowptr;        // S * n0 + 1: first strip of each (sheet, row), relative to lo
+    const int* rowcol;        // cnt: (i << 16) | j of each strip
+    const T* K;               // kernel quadrant [qj][qi][t][s], scaled by 1 / (P0 P1)
+    cpx<T>* T1;               // [s][j][i] after the row FFT (S * P1 * n0)
+    cpx<T>* T2;               // [t][j][i] after the column pass
+    const cpx<T>* tw0;        // twiddles of P0
+    const cpx<T>* tw1;        // twiddles of P1
+};
+
+template <class T>
+struct LpArgs {
+    OrientArgs<T> o[2];
+    int nblk_rows[2];         // workgroups of each orientation in the row passes
+    int nblk_cols[2];         // workgroups of each orientation in the column pass (= its P1)
+    int S;
+    int mode;                 // 0: y = L_p x (strip vectors); 1: the system product's strip rows
+    int nb;                   // offset of the node potentials in x (mode 1)
+    const cpx<T>* x;
+    cpx<T>* y;
+    const int* a_keep;
+    const int* b_keep;
+    const cpx<T>* r_sheet;    // S strip impedances (mode 1)
+    T w;                      // angular frequency (mode 1)
+};
+
+template <class T>
+struct TailArgs {
+    int ns, nb, nbr, nk;
+    const cpx<T>* x;
+    cpx<T>* y;
+    const cpx<T>* z_tail;
+    const int* a_keep;
+    const int* b_keep;
+    const int* adj_ptr;       // nk + 1
+    const int* adj;           // (branch << 1) | (1 if the node is the branch's b end)
+};
+
+struct LpShape {
+    size_t lds_rows = 0, lds_cols = 0;
+};
+
+template <class T>
+void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st);
+template <class T>
+void launch_tail_bottom(const TailArgs<T>& a, hipStream_t st);
+
+}  // namespace gt

@@ -1,0 +1,73 @@
+// Host-side objects behind the C ABI: the context (device + stream), the resident stack and the per-frequency
+// operator, and the device work each owns.
+#pragma once
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "gt_common.hpp"
+#include "lp.hpp"
+#include "schur.hpp"
+
+struct gt_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t lds_max = 65536;
+};
+
+namespace gt {
+
+struct OrientHost {
+    bool present = false;
+    int n0 = 0, n1 = 0, P0 = 0, P1 = 0, log0 = 0, log1 = 0, TR = 1, lo = 0, cnt = 0;
+    DevBuf<int> rowptr, rowcol;
+    DevBuf<double> K;
+    DevBuf<float> Kf;
+};
+
+// Per-vector scratch of one matvec / preconditioner application.
+struct Work {
+    DevBuf<zd> T1, T2;        // L_p intermediates
+    DevBuf<zd> s, z;          // Schur right-hand side and solution (n_keep)
+    DevBuf<zd> t;             // Woodbury: U^T z, then Q U^T z (k)
+};
+
+}  // namespace gt
+
+struct gt_stack_s {
+    gt_ctx* ctx = nullptr;
+    int S = 0, nx = 0, ny = 0, nxb = 0, nyb = 0, nbr = 0, ns = 0, nb = 0, nk = 0;
+    gt::OrientHost o[2];
+    std::map<int, gt::DevBuf<gt::zd>> tw;
+    std::map<int, gt::DevBuf<gt::zf>> twf;
+    gt::DevBuf<int> a_keep, b_keep, adj_ptr, adj, node_sheet;
+    gt::DevBuf<unsigned char> strip_sheet;    // ns: sheet of each strip
+    std::vector<unsigned char> h_strip_sheet;
+    std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
+    gt::LpShape shape;
+    size_t t_elems = 0;                       // complex elements of one L_p intermediate
+    gt::Work work;
+    std::unique_ptr<gt::Schur> schur;
+    gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
+    int k = 0;
+
+    int64_t device_bytes() const;
+    gt::LpArgs<double> lp_args(const gt::cpx<double>* x, gt::cpx<double>* y, int mode) const;
+};
+
+struct gt_op_s {
+    gt_stack* st = nullptr;
+    double w = 0;
+    gt::DevBuf<gt::zd> r_sheet, z_tail, dinv_sheet, dinv_tail, q;
+    gt::zd s_scale;
+    int k = 0;
+};
+
+namespace gt {
+// device work of the operator (ops.hip)
+void op_matvec(gt_op* op, const zd* x, zd* y);
+void op_precond(gt_op* op, const zd* r, zd* z);
+void schur_apply(gt_stack* st, const zd* b, zd* x);   // B^-1 b on the device
+}  // namespace gt

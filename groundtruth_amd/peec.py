@@ -1,0 +1,547 @@
+"""The plane-current PEEC solve behind the reference's own call signatures, on an MI355X.
+
+Drop-in for `src/groundtruth/solver/peec.py`:
+  * `PlaneStack(layers, branches, capacitance, sigma, precond, screen)` (peec.py:1555-1594) numbers the unknowns as
+    the reference does (x strips, y strips, branches; nodes in (layer, i, j) order; one reference node per connected
+    piece, peec.py:1596-1656 / 1845-1862), computes the inductance kernels, and makes the board resident on the
+    device: kernel spectra, strip layout, incidence and the Schur factor (libgtpeec, include/gtpeec.h).
+  * `PlaneStack.solve(f, rhs_top, J, tol=1e-5, max_cycles=8, guess=None) -> (Ib, V, iterations, x)`
+    (peec.py:2513-2607): the same system, right-preconditioned GMRES with the same control flow, on the device.
+  * `RouteExcitation(stack, trace, inject_driver, inject_receiver, ...)` and `.solve(f, guess) -> Solution`
+    (peec.py:2703-2817): psi, injections, loop impedance and moment as the reference.
+Errors are the reference's: SolverError for models and non-convergence, MemoryError for allocation failures,
+DeviceError for a failed device (the reference then redoes the solve on its CPU path, peec.py:2549-2558).
+
+Scope of this build: one current sheet per layer (screen=False, the Simulate default), the "diag" preconditioner,
+full-width strips of one copper thickness, lumped branches (vias). Plane-pair capacitance, face sheets, the
+"sheet" preconditioner and layers of different thickness raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from collections import OrderedDict
+from typing import Optional
+
+import numpy as np
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+
+from . import _lib
+from . import greens as G
+from .problem import SIGMA_CU, Solution, SolverError
+
+GMRES_EXTRA_CYCLES = 24         # peec.py:98
+GMRES_CHECK_MISSES = 4          # peec.py:100
+SCREEN_MIN_NODES = 16           # peec.py:112
+OPERATORS_KEPT = 32             # per-frequency operators kept per stack (small device arrays each)
+
+_KERNELS: dict = {}
+
+
+def _kernel_pair(nx: int, ny: int, d: float, dz: float, px: tuple, py: tuple):
+    """(x quadrant, y quadrant, zero-offset mutual) for the device FFT sizes, cached by the 12-digit keys of d, dz
+    (peec.py:3276-3306)."""
+    d, dz = G.key(d), G.key(dz)
+    k = (nx, ny, d, dz, px, py)
+    hit = _KERNELS.get(k)
+    if hit is None:
+        kx = G.strip_kernel(nx - 1, ny, d, dz)
+        ky = G.strip_kernel(ny - 1, nx, d, dz).T
+        hit = (G.quadrant(kx, *px), G.quadrant(ky, *py), float(kx[nx - 2, ny - 1]))
+        _KERNELS[k] = hit
+    return hit
+
+
+class Operator:
+    """One frequency's device operator (gt_op) and the host values it was made from (peec.py:1505-1516)."""
+
+    def __init__(self, stack: "PlaneStack", f: float):
+        self.stack = stack
+        stack.handle                    # resident first: its factor gives G
+        self.f = G.key(f)
+        self.w = 2 * math.pi * self.f
+        zs = np.array([G.slab_impedance(l.thickness, self.f, stack.sigma) for l in stack.layers])
+        self.r_sheet = zs
+        self.z_tail = np.array([complex(b.z(self.f)) for b in stack.branches], complex)
+        self.dinv_sheet = 1.0 / (self.r_sheet + 1j * self.w * stack.L_self)
+        self.dinv_tail = 1.0 / self.z_tail if len(self.z_tail) else np.zeros(0, complex)
+        ds = self.dinv_sheet[0]
+        k = stack.nbr
+        if k:
+            c = self.dinv_tail / ds - 1.0
+            Q = np.diag(c) @ np.linalg.inv(np.eye(k) + stack.G @ np.diag(c))
+        else:
+            Q = np.zeros((0, 0), complex)
+        self.Q = Q
+        rs, zt, dsh, dt, q = (_lib.cplx(a) for a in (self.r_sheet, self.z_tail if k else np.zeros(1, complex),
+                                                        self.dinv_sheet, self.dinv_tail if k else np.zeros(1, complex),
+                                                        Q if k else np.zeros(1, complex)))
+        desc = _lib.OpDesc()
+        desc.w = self.w
+        desc.r_sheet, desc.z_tail, desc.dinv_sheet, desc.dinv_tail = (_lib.ptr(a) for a in (rs, zt, dsh, dt))
+        sc = 1.0 / ds
+        desc.s_scale[0], desc.s_scale[1] = sc.real, sc.imag
+        desc.k = k
+        desc.q = _lib.ptr(q)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().gt_op_create(stack.handle, C.byref(desc), C.byref(h)), "gt_op_create")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and _lib._LIB is not None:
+                _lib._LIB.gt_op_destroy(self.handle)
+        except Exception:       # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+class PlaneStack:
+    """Plane layers and lumped branches resident on one device (peec.py:1555). Unknown order:
+    [x strips | y strips | branches]."""
+
+    def __init__(self, layers: list, branches: Optional[list] = None, capacitance: Optional[list] = None,
+                 sigma: float = SIGMA_CU, precond: str = "diag", screen: bool = False, device: int = 0):
+        if not layers:
+            raise SolverError("no plane layers")
+        if precond not in ("diag", "sheet"):
+            raise ValueError("precond must be 'diag' or 'sheet'")
+        if precond == "sheet":
+            raise NotImplementedError("the 'sheet' preconditioner is not part of this build")
+        if capacitance:
+            raise NotImplementedError("plane-pair capacitance is not part of this build")
+        g0 = layers[0].grid
+        for l in layers:
+            if (l.grid.nx, l.grid.ny, l.grid.d) != (g0.nx, g0.ny, g0.d):
+                raise SolverError("all layers must share one grid")
+        if len({float(l.thickness) for l in layers}) != 1:
+            raise NotImplementedError("layers of different copper thickness are not part of this build")
+        self.precond_mode = precond
+        self.layers = list(layers)
+        self.N = self.n_sheets = len(layers)
+        self.nx, self.ny = g0.nx, g0.ny
+        self.d = g0.d * 1e-3
+        self.zs = np.array([l.z for l in layers], float)
+        self.sigma = sigma
+        self.sheet_z = self.zs.copy()
+        self.sheet_t = np.array([l.thickness for l in layers], float)
+        self.device = device
+        self._topology(list(branches or []))
+        if screen:
+            self._check_no_split()
+        self._kernels()
+        self._handle = None
+        self._ops: OrderedDict = OrderedDict()
+
+    @property
+    def handle(self):
+        """The resident device stack (gt_stack), made on first use: spectra, layout, incidence, Schur factor."""
+        if self._handle is None:
+            self._upload()
+        return self._handle
+
+    # ---------------------------------------------------------------- topology (peec.py:1596-1656, 1845-1862)
+    def _topology(self, branches: list):
+        node = np.stack([np.asarray(l.grid.node, bool) for l in self.layers])
+        self.xb = np.stack([np.asarray(l.grid.xb, bool) for l in self.layers]) & node[:, :-1, :] & node[:, 1:, :]
+        self.yb = np.stack([np.asarray(l.grid.yb, bool) for l in self.layers]) & node[:, :, :-1] & node[:, :, 1:]
+        self.node = node
+        at = np.nonzero(node)
+        self.nid = -np.ones(node.shape, np.int64)
+        self.nid[at] = np.arange(len(at[0]))
+        self.n_grid = self.n_nodes = len(at[0])
+        self.node_layer, self.node_i, self.node_j = (a.astype(np.int32) for a in at)
+        self.xa, self.ya = np.argwhere(self.xb), np.argwhere(self.yb)
+        self.nxb, self.nyb = len(self.xa), len(self.ya)
+        self.ns = self.nxb + self.nyb
+        self.branches = [b for b in branches if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0
+                         and tuple(b.a) != tuple(b.b)]
+        self.nbr = len(self.branches)
+        self.ncap = 0
+        self.nb = self.ns + self.nbr
+        xa, ya = self.xa, self.ya
+        ba = np.array([self.node_id(b.a) for b in self.branches], np.int64)
+        bb = np.array([self.node_id(b.b) for b in self.branches], np.int64)
+        self.a_nodes = np.concatenate([self.nid[tuple(xa.T)], self.nid[tuple(ya.T)], ba]).astype(np.int64)
+        self.b_nodes = np.concatenate([self.nid[xa[:, 0], xa[:, 1] + 1, xa[:, 2]],
+                                       self.nid[ya[:, 0], ya[:, 1], ya[:, 2] + 1], bb]).astype(np.int64)
+        n = self.n_nodes
+        g = sp.coo_matrix((np.ones(self.nb), (self.a_nodes, self.b_nodes)), shape=(n, n))
+        _, self.full_label = connected_components(g, directed=False)
+        self.cond_label = self.full_label
+        _, first = np.unique(self.full_label, return_index=True)
+        self.keep = np.ones(n, bool)
+        self.keep[first] = False
+        self.n_keep = int(self.keep.sum())
+        kidx = np.cumsum(self.keep) - 1
+        self.a_keep = np.where(self.keep[self.a_nodes], kidx[self.a_nodes], -1).astype(np.int32)
+        self.b_keep = np.where(self.keep[self.b_nodes], kidx[self.b_nodes], -1).astype(np.int32)
+
+    def _check_no_split(self):
+        """screen=True splits a layer with copper above and below it at >= 16 of its nodes into two face sheets
+        (peec.py:1716-1765); this build has one sheet per layer, so such a stack is refused."""
+        if self.N < 3:
+            return
+        for k in range(self.N):
+            above = np.zeros(self.node.shape[1:], bool)
+            below = np.zeros(self.node.shape[1:], bool)
+            for m in range(self.N):
+                if self.zs[m] > self.zs[k]:
+                    above |= self.node[m]
+                elif self.zs[m] < self.zs[k]:
+                    below |= self.node[m]
+            if np.count_nonzero(self.node[k] & above & below) >= SCREEN_MIN_NODES:
+                raise NotImplementedError("face sheets (screen=True with an inner plane) are not part of this build")
+
+    def node_id(self, node) -> int:
+        """Node number of (layer, i, j), -1 off copper (peec.py:1864-1869)."""
+        return int(self.nid[tuple(node)])
+
+    def face_gap(self, s: int, t: int) -> float:
+        """Distance between the faces sheets s and t turn to each other, at least 1 um (peec.py:1925-1932)."""
+        if s == t:
+            return 0.0
+        gap = abs(self.sheet_z[s] - self.sheet_z[t]) - 0.5 * (self.sheet_t[s] + self.sheet_t[t])
+        return max(gap, 1e-6)
+
+    # ---------------------------------------------------------------- kernels (peec.py:1941-1988)
+    def _kernels(self):
+        nx, ny, S = self.nx, self.ny, self.n_sheets
+        self.Px = (G.fft_size(nx - 1), G.fft_size(ny))
+        self.Py = (G.fft_size(nx), G.fft_size(ny - 1))
+        hx = (self.Px[1] // 2 + 1, self.Px[0] // 2 + 1)
+        hy = (self.Py[1] // 2 + 1, self.Py[0] // 2 + 1)
+        Kx = np.empty((*hx, S, S))
+        Ky = np.empty((*hy, S, S))
+        self.L_self = _kernel_pair(nx, ny, self.d, 0.0, self.Px, self.Py)[2]
+        for s in range(S):
+            for t in range(s, S):
+                qx, qy, _ = _kernel_pair(nx, ny, self.d, self.face_gap(s, t), self.Px, self.Py)
+                Kx[:, :, s, t] = Kx[:, :, t, s] = qx.T / (self.Px[0] * self.Px[1])
+                Ky[:, :, s, t] = Ky[:, :, t, s] = qy.T / (self.Py[0] * self.Py[1])
+        self._kx, self._ky = np.ascontiguousarray(Kx), np.ascontiguousarray(Ky)
+
+    # ---------------------------------------------------------------- the device copy
+    def _upload(self):
+        L = _lib.lib()
+        self.ctx = _lib.Context.get(self.device)
+        xs = np.ascontiguousarray(self.xa, np.int32)
+        ys = np.ascontiguousarray(self.ya, np.int32)
+        keep_idx = np.nonzero(self.keep)[0]
+        node_sheet = np.ascontiguousarray(self.node_layer[keep_idx], np.int32)
+        d = _lib.StackDesc()
+        d.n_sheets, d.nx, d.ny = self.n_sheets, self.nx, self.ny
+        d.nxb, d.nyb, d.nbr, d.n_keep = self.nxb, self.nyb, self.nbr, self.n_keep
+        d.x_sij, d.y_sij = _lib.ptr(xs.reshape(-1) if xs.size else np.zeros(3, np.int32), _lib.i32p), \
+            _lib.ptr(ys.reshape(-1) if ys.size else np.zeros(3, np.int32), _lib.i32p)
+        a_keep = self.a_keep if len(self.a_keep) else np.zeros(1, np.int32)
+        b_keep = self.b_keep if len(self.b_keep) else np.zeros(1, np.int32)
+        d.a_keep, d.b_keep = _lib.ptr(a_keep, _lib.i32p), _lib.ptr(b_keep, _lib.i32p)
+        d.node_sheet = _lib.ptr(node_sheet if len(node_sheet) else np.zeros(1, np.int32), _lib.i32p)
+        d.px[0], d.px[1] = self.Px
+        d.py[0], d.py[1] = self.Py
+        d.kx, d.ky = _lib.ptr(self._kx), _lib.ptr(self._ky)
+        h = C.c_void_p()
+        _lib.check(L.gt_stack_create(self.ctx.handle, C.byref(d), C.byref(h)), "gt_stack_create")
+        xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
+                                             self.node_j[keep_idx]], axis=1).astype(np.int32))
+        Gm = np.zeros((max(self.nbr, 1), max(self.nbr, 1)))
+        code = L.gt_schur_factor(h, _lib.ptr(xyz.reshape(-1) if xyz.size else np.zeros(3, np.int32), _lib.i32p),
+                                 _lib.ptr(Gm))
+        if code != _lib.GT_OK:
+            L.gt_stack_destroy(h)
+            _lib.check(code, "gt_schur_factor")
+        self.G = Gm[: self.nbr, : self.nbr].copy()
+        self._handle = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_handle", None) and _lib._LIB is not None:
+                self._ops.clear()
+                _lib._LIB.gt_stack_destroy(self._handle)
+        except Exception:       # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def device_bytes(self) -> int:
+        """Device memory this stack holds (its resident spectra, layout, factor and scratch)."""
+        return int(_lib.lib().gt_stack_device_bytes(self.handle))
+
+    def schur_stats(self) -> dict:
+        nnz, nsup, lev, mc = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().gt_schur_stats(self.handle, C.byref(nnz), C.byref(nsup), C.byref(lev),
+                                             C.byref(mc)), "gt_schur_stats")
+        return {"nnz_L": nnz.value, "supernodes": nsup.value, "levels": lev.value, "max_cols": mc.value}
+
+    # ---------------------------------------------------------------- operators (peec.py:2227-2345)
+    @staticmethod
+    def freq_key(f: float) -> float:
+        return G.key(f)
+
+    def operator(self, f: float) -> Operator:
+        """The device operator at f's 12-digit key (kept for later solves at the same key)."""
+        k = G.key(f)
+        op = self._ops.get(k)
+        if op is None:
+            op = self._ops[k] = Operator(self, k)
+            while len(self._ops) > OPERATORS_KEPT:
+                self._ops.popitem(last=False)
+        else:
+            self._ops.move_to_end(k)
+        return op
+
+    def solve(self, f: float, rhs_top: np.ndarray, J: np.ndarray, tol: float = 1e-5, max_cycles: int = 8,
+              guess: Optional[list] = None, restart: Optional[int] = None):
+        """Solve K [I; V] = [rhs_top; J] (peec.py:2513-2607). Returns (Ib, V, iterations, x). `restart` sets the
+        GMRES restart length (None: the reference's min(200, max(100, 4e8 / 16n)))."""
+        op = self.operator(f)
+        nb = self.nb
+        rhs = np.concatenate([np.asarray(rhs_top, complex), np.asarray(J, float)[self.keep].astype(complex)])
+        bnorm = float(np.linalg.norm(rhs))
+        x0, r0 = None, rhs
+        Gs = [g for g in (guess or []) if g is not None and len(g) == len(rhs) and np.any(g)]
+        if Gs:
+            KG = np.stack([self.matvec(op, np.asarray(g, complex)) for g in Gs], axis=1)
+            c, *_ = np.linalg.lstsq(KG, rhs, rcond=None)
+            r = rhs - KG @ c
+            if np.linalg.norm(r) < 0.9 * bnorm:
+                x0, r0 = np.stack(Gs, axis=1) @ c, r
+        x, its, ok, _ = self.gmres(op, r0, tol, bnorm=bnorm, max_cycles=max_cycles, restart=restart)
+        if not ok:
+            raise SolverError(f"GMRES did not converge at {f:.4g} Hz ({its} iterations)")
+        if x0 is not None:
+            x = x + x0
+        V = np.zeros(self.n_nodes, complex)
+        V[self.keep] = x[nb:]
+        return x[:nb], V, its, x
+
+    def gmres(self, op: Operator, b: np.ndarray, tol: float, bnorm: Optional[float] = None, max_cycles: int = 8,
+              restart: Optional[int] = None, extra_cycles: int = GMRES_EXTRA_CYCLES, max_its: Optional[int] = None):
+        """x = M^-1 y with (K M^-1) y = b by the device GMRES (peec.py:3093-3224). (x, its, converged, result)."""
+        n = self.nb + self.n_keep
+        bb = _lib.cplx(b)
+        assert bb.size == 2 * n
+        out = np.zeros(2 * n)
+        p = _lib.GmresParams(float(tol), float(bnorm or 0.0), int(restart or 0), int(max_cycles),
+                             int(extra_cycles), int(max_its or 0), GMRES_CHECK_MISSES)
+        res = _lib.GmresResult()
+        _lib.check(_lib.lib().gt_gmres(op.handle, _lib.ptr(bb), _lib.ptr(out), C.byref(p), C.byref(res)), "gt_gmres")
+        return out.view(np.complex128).copy(), int(res.iterations), bool(res.converged), res
+
+    # ---------------------------------------------------------------- device products on host vectors
+    def _dev(self, v: np.ndarray):
+        import torch
+        self.handle
+        return torch.from_numpy(np.ascontiguousarray(v, np.complex128)).to(f"cuda:{self.device}")
+
+    def _run(self, fn, x: np.ndarray, n_out: int) -> np.ndarray:
+        import torch
+        xd = self._dev(x)
+        yd = torch.zeros(n_out, dtype=torch.complex128, device=f"cuda:{self.device}")
+        torch.cuda.synchronize(self.device)
+        _lib.check(fn(C.c_void_p(xd.data_ptr()), C.c_void_p(yd.data_ptr())))
+        self.ctx.sync()
+        return yd.cpu().numpy()
+
+    def Lp_matvec(self, I_s: np.ndarray) -> np.ndarray:
+        """L_p I on the device (peec.py:2115-2135)."""
+        return self._run(lambda a, b: _lib.lib().gt_lp_matvec(self.handle, a, b), I_s, self.ns)
+
+    def matvec(self, op: Operator, x: np.ndarray) -> np.ndarray:
+        """K x on the device (peec.py:3043-3062)."""
+        return self._run(lambda a, b: _lib.lib().gt_matvec(op.handle, a, b), x, self.nb + self.n_keep)
+
+    def precond(self, op: Operator, r: np.ndarray) -> np.ndarray:
+        """M^-1 r on the device (peec.py:3064-3083)."""
+        return self._run(lambda a, b: _lib.lib().gt_precond(op.handle, a, b), r, self.nb + self.n_keep)
+
+    def schur_solve(self, b: np.ndarray) -> np.ndarray:
+        """B^-1 b on the device (B = Ar^T Ar, the factorised Schur base)."""
+        return self._run(lambda a, c: _lib.lib().gt_schur_solve(self.handle, a, c), b, self.n_keep)
+
+    def schur_solve_host(self, b: np.ndarray) -> np.ndarray:
+        """B^-1 b with the same factor on the host (a device-free check of the factorisation)."""
+        bb = _lib.cplx(b)
+        out = np.zeros_like(bb)
+        _lib.check(_lib.lib().gt_schur_solve_host(self.handle, _lib.ptr(bb), _lib.ptr(out)), "gt_schur_solve_host")
+        return out.view(np.complex128).copy()
+
+    # ---------------------------------------------------------------- coupling to the signal (peec.py:2609-2700)
+    def pos(self, node) -> np.ndarray:
+        k, i, j = node
+        return np.array([i * self.d, j * self.d, self.zs[k]])
+
+    def strip_coupling(self, trace: list) -> np.ndarray:
+        """psi = int A.dl along every strip per amp of signal (peec.py:2609-2656): the segment's vector potential
+        at the strip's centre on its face toward the segment, or its mean over 3 x 3 Gauss points of the strip's
+        square within 3 pitches (+ 2 widths) of the segment."""
+        d = self.d
+        x, w = np.polynomial.legendre.leggauss(3)
+        q = 0.5 * x
+        W = (np.outer(w, w) / 4.0).ravel()
+        qa, ql = (g.ravel() for g in np.meshgrid(q, q, indexing="ij"))
+        psi = np.zeros(self.ns)
+        for axis, idx, off in ((0, self.xa, 0), (1, self.ya, self.nxb)):
+            if len(idx) == 0:
+                continue
+            k, i, j = idx[:, 0], idx[:, 1], idx[:, 2]
+            cx = i * d + (d / 2 if axis == 0 else 0.0)
+            cy = j * d + (d / 2 if axis == 1 else 0.0)
+            half = 0.5 * self.sheet_t[k]
+            acc = np.zeros(len(idx))
+            for s in trace:
+                ab = s.b - s.a
+                if np.linalg.norm(ab[:2]) == 0:
+                    continue
+                L = float(np.linalg.norm(ab))
+                if abs(ab[axis]) < 1e-15 * L:
+                    continue
+                zseg = 0.5 * (s.a[2] + s.b[2])
+                cz = self.sheet_z[k] + np.where(zseg > self.sheet_z[k], half, -half)
+                centres = np.stack([cx, cy, cz], axis=1)
+                rho_min = max(0.5 * s.width, 1e-7)
+                val = G.segment_potential(s.a, s.b, centres, rho_min)
+                near = _dist_to_segment(centres, s.a, s.b) < 3 * d + 2 * s.width
+                if near.any():
+                    c = centres[near]
+                    da, dl = (qa, ql) if axis == 0 else (ql, qa)
+                    pts = np.stack([c[:, 0, None] + da * d, c[:, 1, None] + dl * d,
+                                    np.broadcast_to(c[:, 2, None], (len(c), len(W)))], axis=2)
+                    vq = G.segment_potential(s.a, s.b, pts.reshape(-1, 3), rho_min).reshape(len(c), len(W))
+                    val[near] = (vq * W).sum(1)
+                acc += (s.i * ab[axis] / L) * val
+            psi[off: off + len(idx)] = d * acc
+        return psi
+
+    def branch_coupling(self, trace: list, start: int = 0) -> np.ndarray:
+        """psi of via branches from the vertical trace segments (peec.py:2658-2666)."""
+        vert = [s for s in trace if abs((s.b - s.a)[2]) > 0 and np.linalg.norm((s.b - s.a)[:2]) < 1e-12]
+        out = np.zeros(self.nbr - start)
+        for n, b in enumerate(self.branches[start:]):
+            if b.kind == "via" and vert:
+                pa, pb = self.pos(b.a), self.pos(b.b)
+                out[n] = sum(s.i * G.segment_mutual(s.a, s.b, pa, pb, b.radius) for s in vert)
+        return out
+
+    def moment(self, trace: list, Ib: np.ndarray) -> np.ndarray:
+        """m = 1/2 sum r x I dl over the closed loop (peec.py:2668-2700)."""
+        m = np.zeros(3, complex)
+        for s in trace:
+            m += 0.5 * s.i * np.cross(s.a, s.b)
+        d = self.d
+        Ix, Iy = Ib[: self.nxb], Ib[self.nxb: self.ns]
+        if self.nxb:
+            k, _, j = self.xa.T
+            m[1] += 0.5 * np.sum(self.sheet_z[k] * d * Ix)
+            m[2] += 0.5 * np.sum(-(j * d) * d * Ix)
+        if self.nyb:
+            k, i, _ = self.ya.T
+            m[0] += 0.5 * np.sum(-self.sheet_z[k] * d * Iy)
+            m[2] += 0.5 * np.sum((i * d) * d * Iy)
+        for b, cur in zip(self.branches, Ib[self.ns:]):
+            m += 0.5 * np.cross(self.pos(b.a), self.pos(b.b)) * cur
+        return m
+
+
+def _dist_to_segment(P: np.ndarray, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    ab = b - a
+    L2 = float(ab @ ab)
+    if L2 == 0:
+        return np.linalg.norm(P - a, axis=1)
+    t = np.clip((P - a) @ ab / L2, 0, 1)
+    return np.linalg.norm(P - (a + t[:, None] * ab), axis=1)
+
+
+def trace_self(trace: list, trace_t: float) -> float:
+    """Partial self inductance of the signal path, sum_ij I_i I_j M_ij (peec.py:3227-3237)."""
+    segs = [s for s in trace if np.linalg.norm(s.b - s.a) > 0]
+    L = 0.0
+    for a, sa in enumerate(segs):
+        L += sa.i * sa.i * G.strip_self_gmd(float(np.linalg.norm(sa.b - sa.a)), sa.width, trace_t)
+        for sb in segs[a + 1:]:
+            L += 2 * sa.i * sb.i * G.segment_mutual(sa.a, sa.b, sb.a, sb.b, G.GMD_FLAT * (sa.width + trace_t))
+    return L
+
+
+def trace_resistance(trace: list, f: float, sigma: float, trace_t: float) -> float:
+    """I^2-weighted skin-effect resistance of the horizontal segments (peec.py:3240-3250)."""
+    R = 0.0
+    for s in trace:
+        L = float(np.linalg.norm(s.b - s.a))
+        if L == 0 or abs(s.b[2] - s.a[2]) > 0:
+            continue
+        R += s.i * s.i * L / (sigma * s.width * G.skin_thickness(trace_t, f, sigma))
+    return R
+
+
+class RouteExcitation:
+    """A loop on a PlaneStack: the known trace current plus return injections (peec.py:2703-2817)."""
+
+    def __init__(self, stack: PlaneStack, trace: list, inject_driver=None, inject_receiver=None,
+                 trace_thickness: float = 35e-6, strip_psi: Optional[np.ndarray] = None,
+                 branch_psi: Optional[np.ndarray] = None, injections: Optional[list] = None):
+        self.stack = stack
+        self.trace = trace
+        self.trace_t = trace_thickness
+        self.inject = (inject_driver, inject_receiver)
+        self.extra = list(injections or [])
+        terms = []
+        if inject_driver is not None:
+            terms.append((inject_driver, -1.0, "driver return node"))
+        if inject_receiver is not None:
+            terms.append((inject_receiver, 1.0, "receiver return node"))
+        terms += [(node, float(a), f"current injection at {node}") for node, a in self.extra]
+        if not terms:
+            raise SolverError("no current is injected")
+        ids = []
+        for node, amps, what in terms:
+            n = stack.node_id(node)
+            if n < 0:
+                raise SolverError(f"{what} is not on plane copper")
+            ids.append((int(n), amps))
+        bal: dict = {}
+        for n, amps in ids:
+            c = int(stack.full_label[n])
+            bal[c] = bal.get(c, 0.0) + amps
+        if any(abs(v) > 1e-9 for v in bal.values()):
+            if inject_driver is not None and inject_receiver is not None and not self.extra:
+                raise SolverError("receiver return node is not connected to the driver's return node")
+            raise SolverError("the injected currents do not balance on every connected piece of copper")
+        self.injections = ids
+        self.strip_psi = stack.strip_coupling(trace) if strip_psi is None else strip_psi
+        bp = branch_psi if branch_psi is not None else np.zeros(0)
+        if len(bp) < stack.nbr:
+            bp = np.concatenate([bp, stack.branch_coupling(trace, start=len(bp))])
+        self.branch_psi = bp
+        self.psi = np.concatenate([self.strip_psi, self.branch_psi])
+        self.L_trace = trace_self(trace, trace_thickness) if trace else 0.0
+
+    def rhs(self, f: float):
+        """(rhs_top = -jw psi, J) at f (peec.py:2789-2795)."""
+        w = 2 * math.pi * f
+        J = np.zeros(self.stack.n_nodes)
+        for n, amps in self.injections:
+            J[n] += amps
+        return -1j * w * self.psi, J
+
+    def solve(self, f: float, guess: Optional[list] = None, tol: float = 1e-5, max_cycles: int = 8,
+              restart: Optional[int] = None) -> Solution:
+        """The loop's solution at f (peec.py:2787-2817). tol / max_cycles / restart go to PlaneStack.solve
+        (the reference uses its defaults)."""
+        rhs_top, J = self.rhs(f)
+        Ib, V, its, x = self.stack.solve(f, rhs_top, J, tol=tol, max_cycles=max_cycles, guess=guess,
+                                         restart=restart)
+        return self.unpack(f, Ib, V, its, x)
+
+    def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
+        st = self.stack
+        w = 2 * math.pi * f
+        I_x = np.zeros((st.N, st.nx - 1, st.ny), complex)
+        I_y = np.zeros((st.N, st.nx, st.ny - 1), complex)
+        I_x[tuple(st.xa.T)] = Ib[:st.nxb]
+        I_y[tuple(st.ya.T)] = Ib[st.nxb: st.ns]
+        Vg = np.zeros((st.N, st.nx, st.ny), complex)
+        Vg[np.nonzero(st.node)] = V[:st.n_grid]
+        v_inj = sum(amps * V[n] for n, amps in self.injections)
+        Z = complex(trace_resistance(self.trace, f, st.sigma, self.trace_t) + v_inj
+                    + 1j * w * (self.L_trace + np.dot(self.psi, Ib)))
+        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, st.moment(self.trace, Ib), its, x)

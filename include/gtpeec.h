@@ -1,0 +1,168 @@
+/*
+ * gtpeec.h - C ABI of libgtpeec.so, the MI355X (gfx950) plane-current PEEC solve.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary. Every call returns GT_OK or an error code, and
+ * gt_last_error() gives the calling thread's last message. Complex numbers are interleaved (re, im) doubles.
+ *
+ * Object model (the lifetimes of the reference's device seam, src/groundtruth/solver/gpu.py:617-666):
+ *   gt_ctx    one HIP device and the stream every call on it is ordered on (gpu.py:247 `use`, one process per GPU).
+ *   gt_stack  a board resident on the device: strip layout, incidence, kernel spectra, Schur factor
+ *             (peec.py:1555 PlaneStack; uploaded once, as gpu.py:617-645 `_stack_data`).
+ *   gt_op     one frequency's data on a stack (peec.py:1505 _Operator; gpu.py:504-520 `_op`).
+ *
+ * Unknowns (peec.py:1596-1656): x = [Ib (nb) ; V_keep (n_keep)], nb = nxb + nyb + nbr, strips sorted by
+ * (sheet, i, j) per orientation (x strips first), lumped branches after the strips.
+ */
+#ifndef GTPEEC_H
+#define GTPEEC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_ABI_VERSION 1
+
+#define GT_OK 0
+#define GT_ERR_ARG 1          /* a bad argument (the caller's bug): ValueError                                   */
+#define GT_ERR_OOM 2          /* device or host allocation failed: MemoryError (peec.py:2428-2429)               */
+#define GT_ERR_DEVICE 3       /* a HIP call failed (no device, launch failure)                                    */
+#define GT_ERR_NOCONV 4       /* GMRES did not converge: SolverError (peec.py:2599-2600)                          */
+#define GT_ERR_NUMERIC 5      /* the Schur factorisation met a zero or non-finite pivot: SolverError             */
+#define GT_ERR_UNSUPPORTED 6  /* a shape this build has no kernel for (FFT size, LDS budget)                      */
+
+typedef struct gt_ctx_s gt_ctx;
+typedef struct gt_stack_s gt_stack;
+typedef struct gt_op_s gt_op;
+
+/* The calling thread's last error message ("" after success). */
+const char* gt_last_error(void);
+int gt_abi_version(void);
+
+/* ---------------------------------------------------------------- context
+ * Replaces the device selection of gpu.py:247 `use` / :120 `probe`: binds HIP device `device` and creates the
+ * stream all work of this context is ordered on. */
+int gt_ctx_create(int device, gt_ctx** out);
+void gt_ctx_destroy(gt_ctx* ctx);
+int gt_ctx_sync(gt_ctx* ctx);
+/* The context's hipStream_t (for callers that order their own device work with it). */
+void* gt_ctx_stream(gt_ctx* ctx);
+
+/* ---------------------------------------------------------------- stack
+ * Replaces gpu.py:617-666 `_stack_data` / `_orients` (kernels, strip indices and incidence uploaded once per stack
+ * and device) for PlaneStack (peec.py:1555-1988). The kernel spectra are the real, even spectra of
+ * peec.py:3276-3371 `_kernel_pair` / `_kernel_spectrum` for the FFT sizes given here (any size >= 2n-1 embeds the
+ * linear convolution), stored as their [0, P1/2] x [0, P0/2] quadrant, pixel-major over sheet pairs: element
+ * ((qj * (P0/2 + 1) + qi) * S + t) * S + s, already scaled by 1 / (P0 * P1). */
+typedef struct {
+    int32_t n_sheets;         /* S: current sheets (one per layer)                                               */
+    int32_t nx, ny;           /* nodes per layer                                                                  */
+    int32_t nxb, nyb;         /* x and y strips                                                                   */
+    int32_t nbr;              /* lumped branches (vias) after the strips                                          */
+    int32_t n_keep;           /* kept node potentials (one reference node per connected piece dropped)           */
+    const int32_t* x_sij;     /* nxb x 3 (sheet, i, j) of the x strips, sorted                                    */
+    const int32_t* y_sij;     /* nyb x 3                                                                          */
+    const int32_t* a_keep;    /* nb: kept index of each branch's node a (+1 end), -1 for a reference node        */
+    const int32_t* b_keep;    /* nb: kept index of node b (-1 end)                                                */
+    const int32_t* node_sheet;/* n_keep: sheet of each kept node                                                  */
+    int32_t px[2];            /* FFT size of the x strips (P0 along i, P1 along j), powers of two                */
+    int32_t py[2];            /* FFT size of the y strips                                                         */
+    const double* kx;         /* x kernel quadrant, (px[1]/2+1)(px[0]/2+1) S S doubles                            */
+    const double* ky;         /* y kernel quadrant                                                                */
+} gt_stack_desc;
+
+int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* desc, gt_stack** out);
+void gt_stack_destroy(gt_stack* stack);
+/* Device bytes the stack holds (the footprint gpu.py:593-605 `device_bytes` estimates). */
+int64_t gt_stack_device_bytes(const gt_stack* stack);
+
+/* L_p I (peec.py:2115-2135 Lp_matvec) on device vectors of nxb + nyb complex values: scatter, row FFT, column
+ * FFT + kernel multiply-add + inverse column FFT, inverse row FFT, gather. */
+int gt_lp_matvec(gt_stack* stack, const void* I_dev, void* out_dev);
+
+/* ---------------------------------------------------------------- Schur complement preconditioner
+ * Replaces peec.py:2345-2445 `_factorise` (SuperLU of S = Ar^T D^-1 Ar in a host thread) and the C shim
+ * solver/accelerate_shim.c:21-59 (gt_factor / gt_solve / gt_free). With every strip of a stack alike, D^-1 is one
+ * admittance d_s per strip and S = d_s (B + U C U^T): B = Ar^T Ar is the real graph Laplacian of the kept nodes
+ * (strips and branches, weight 1), U = the branches' incidence columns, C = diag(d_br / d_s - 1). B is factorised
+ * once per stack (nested dissection, supernodal Cholesky on the host), its triangular factors are solved on the
+ * device level by level, and each frequency only changes the k x k Woodbury matrix.
+ *
+ * node_xyz: n_keep x 3 (sheet, i, j) of the kept nodes (the nested-dissection geometry).
+ * G_out: k x k (k = nbr) real, row-major: U^T B^-1 U, from which the caller forms each frequency's
+ *        Q = C (I + G C)^-1 (peec.py:2357-2377's Woodbury identity). May be NULL when nbr == 0. */
+int gt_schur_factor(gt_stack* stack, const int32_t* node_xyz, double* G_out);
+/* Statistics of the factor: entries of L, supernodes, tree levels, the largest supernode's columns. */
+int gt_schur_stats(const gt_stack* stack, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels, int32_t* max_cols);
+/* B^-1 b on the host with the same factor (a check of the factorisation that needs no device). b, x: n_keep
+ * complex values. */
+int gt_schur_solve_host(const gt_stack* stack, const double* b, double* x);
+/* B^-1 b on the device (level-scheduled triangular solves). b_dev, x_dev: n_keep complex values. */
+int gt_schur_solve(gt_stack* stack, const void* b_dev, void* x_dev);
+
+/* The same factorisation without a device: the role of solver/accelerate_shim.c:21-59 (gt_factor -> handle,
+ * gt_solve(handle, x, k), gt_free) for B = Ar^T Ar given by the branches' kept end nodes (-1: a reference node).
+ * gt_schur_host_solve solves k complex right-hand sides (n_keep each, consecutive) in place of b into x. */
+typedef struct gt_schur_host_s gt_schur_host;
+int gt_schur_host_factor(int32_t n_keep, int32_t nb, const int32_t* a_keep, const int32_t* b_keep,
+                         const int32_t* node_xyz, gt_schur_host** out);
+int gt_schur_host_solve(const gt_schur_host* h, const double* b, double* x, int32_t k);
+int gt_schur_host_stats(const gt_schur_host* h, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels,
+                        int32_t* max_cols);
+void gt_schur_host_free(gt_schur_host* h);
+
+/* ---------------------------------------------------------------- per-frequency operator
+ * Replaces peec.py:2150-2178 `_impedances` + :2465-2511 `_preconditioner_admittance` as uploaded by gpu.py:504-520. */
+typedef struct {
+    double w;                 /* angular frequency 2 pi f                                                         */
+    const double* r_sheet;    /* 2 S: strip impedance per square of each sheet (surface impedance)               */
+    const double* z_tail;     /* 2 nbr: branch impedances                                                         */
+    const double* dinv_sheet; /* 2 S: preconditioner admittance of a strip on each sheet, 1/(R + jw L_self)      */
+    const double* dinv_tail;  /* 2 nbr: 1 / z_tail                                                                */
+    double s_scale[2];        /* 1 / d_s: scales B^-1 to S^-1                                                      */
+    int32_t k;                /* Woodbury rank (nbr, or 0)                                                        */
+    const double* q;          /* 2 k k: Q = C (I + G C)^-1, row-major                                             */
+} gt_op_desc;
+
+int gt_op_create(gt_stack* stack, const gt_op_desc* desc, gt_op** out);
+void gt_op_destroy(gt_op* op);
+
+/* The system product y = K x = [Z Ib - Ar V ; Ar^T Ib] (peec.py:3043-3062 _HostMath.matvec, gpu.py:551-564). */
+int gt_matvec(gt_op* op, const void* x_dev, void* y_dev);
+/* The preconditioner z = M^-1 r by block elimination (peec.py:3064-3083 _HostMath.precond, gpu.py:566-573),
+ * with the Schur solve on the device. */
+int gt_precond(gt_op* op, const void* r_dev, void* z_dev);
+
+/* ---------------------------------------------------------------- GMRES
+ * Replaces peec.py:3093-3224 `_gmres` for the right-preconditioned system (K M^-1) y = r0, x = M^-1 y
+ * (peec.py:2581): CGS with a DGKS re-pass below 0.7, Givens rotations, the true-residual check with its aim and
+ * miss budget, restarts and extra cycles while each falls to 0.7. The basis and every vector stay on the device;
+ * only the Hessenberg column and three norms cross to the host per iteration. */
+typedef struct {
+    double tol;               /* stop at ||b - K x|| <= tol * bnorm                                               */
+    double bnorm;             /* <= 0: ||b||                                                                      */
+    int32_t restart;          /* <= 0: min(200, max(100, 4e8 / (16 n))) (peec.py:3087-3090)                      */
+    int32_t max_cycles;       /* restarts (peec.py default 8)                                                     */
+    int32_t extra_cycles;     /* further restarts while each still falls to 0.7 (24 for an own factorisation)   */
+    int32_t max_its;          /* <= 0: no budget                                                                  */
+    int32_t check_misses;     /* true-residual misses before a cycle restarts from its check (4)                 */
+} gt_gmres_params;
+
+typedef struct {
+    int32_t iterations;
+    int32_t converged;
+    int32_t products;         /* system products made (K M^-1 applications)                                      */
+    double residual;          /* ||b - K x|| / bnorm at the end                                                    */
+} gt_gmres_result;
+
+/* Solve K x = b (b, x: n = nb + n_keep complex values on the host). x holds the answer (also when unconverged:
+ * the best so far, as the reference). Returns GT_OK; convergence is in result->converged. */
+int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params* params,
+             gt_gmres_result* result);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GTPEEC_H */
