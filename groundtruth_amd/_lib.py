@@ -65,6 +65,7 @@ SIGNATURES = {
     "gt_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_gmres": (C.c_int, [C.c_void_p, f64p, f64p, C.POINTER(GmresParams), C.POINTER(GmresResult)]),
+    "gt_gmres_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GmresParams), C.POINTER(GmresResult)]),
 }
 
 _LIB = None

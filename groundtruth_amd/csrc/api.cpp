@@ -8,6 +8,7 @@
 
 namespace gt {
 void gmres_solve(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params& p, gt_gmres_result& res);
+void gmres_solve_dev(gt_op* op, const zd* b_dev, zd* x_dev, const gt_gmres_params& p, gt_gmres_result& res);
 }
 
 using namespace gt;
@@ -106,7 +107,8 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.Kf.upload(kf, st->ctx->stream);
     twiddles(st, o.P0);
     twiddles(st, o.P1);
-    st->t_elems = std::max(st->t_elems, (size_t)S * o.P1 * o.n0);
+    o.t_off = st->t_elems;                      // each orientation its own region: they run in one launch
+    st->t_elems += (size_t)S * o.P1 * o.n0;
     st->shape.lds_rows = std::max(st->shape.lds_rows, (size_t)o.TR * (o.P1 + 1) * sizeof(zd));
     st->shape.lds_cols = std::max(st->shape.lds_cols, (size_t)S * o.P0 * sizeof(zd));
 }
@@ -152,8 +154,8 @@ LpArgs<double> gt_stack_s::lp_args(const zd* x, zd* y, int mode) const {
         d.rowptr = h.rowptr.p;
         d.rowcol = h.rowcol.p;
         d.K = h.K.p;
-        d.T1 = work.T1.p;
-        d.T2 = work.T2.p;
+        d.T1 = work.T1.p + h.t_off;
+        d.T2 = work.T2.p + h.t_off;
         d.tw0 = h.present ? tw.at(h.P0).p : nullptr;
         d.tw1 = h.present ? tw.at(h.P1).p : nullptr;
         a.nblk_rows[oi] = h.present ? S * ((h.n0 + h.TR - 1) / h.TR) : 0;
@@ -466,6 +468,16 @@ int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_par
         need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
         GT_HIP(hipSetDevice(op->st->ctx->device));
         gmres_solve(op, b_host, x_host, *p, *res);
+    });
+}
+
+int gt_gmres_dev(gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* p, gt_gmres_result* res) {
+    return guard([&] {
+        need(op && b_dev && x_dev && p && res && b_dev != x_dev, "NULL or aliased argument");
+        need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
+        GT_HIP(hipSetDevice(op->st->ctx->device));
+        gmres_solve_dev(op, (const zd*)b_dev, (zd*)x_dev, *p, *res);
     });
 }
 

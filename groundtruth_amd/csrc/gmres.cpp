@@ -22,7 +22,8 @@ struct Solver {
     gt_op* op;
     hipStream_t s;
     int n, restart;
-    DevBuf<zd> V, w, x, xt, r, b, t, u, ydev;
+    DevBuf<zd> V, w, x, xt, r, t, u, ydev;
+    const zd* bp = nullptr;            // the right-hand side (device, caller's)
     Krylov kr;
     std::vector<double> red;
     std::vector<cd> ybuf;             // host source of the last combination's coefficients (kept until copied)
@@ -34,7 +35,6 @@ struct Solver {
         x.alloc(n);
         xt.alloc(n);
         r.alloc(n);
-        b.alloc(n);
         t.alloc(n);
         u.alloc(n);
         ydev.alloc(restart + 1);
@@ -87,7 +87,7 @@ struct Solver {
         const int max_its = p.max_its > 0 ? p.max_its : std::numeric_limits<int>::max();
         const int check_misses = p.check_misses > 0 ? p.check_misses : 4;
         GT_HIP(hipMemsetAsync(x.p, 0, sizeof(zd) * n, s));
-        const double b0 = norm(b.p);
+        const double b0 = norm(bp);
         const double bnorm = p.bnorm > 0 ? p.bnorm : b0;
         res.iterations = 0;
         res.converged = 0;
@@ -98,7 +98,7 @@ struct Solver {
             return;
         }
         int its = 0;
-        GT_HIP(hipMemcpyAsync(r.p, b.p, sizeof(zd) * n, hipMemcpyDeviceToDevice, s));
+        GT_HIP(hipMemcpyAsync(r.p, bp, sizeof(zd) * n, hipMemcpyDeviceToDevice, s));
         double last = std::numeric_limits<double>::infinity();
         const int ld = restart;
         std::vector<cd> H((size_t)(restart + 1) * restart), sn(restart), g(restart + 1);
@@ -178,7 +178,7 @@ struct Solver {
                     GT_HIP(hipMemcpyAsync(xt.p, x.p, sizeof(zd) * n, hipMemcpyDeviceToDevice, s));
                     combine(xt.p, y, k);
                     apply(xt.p, u.p);
-                    k_sub(r.p, b.p, u.p, n, s);
+                    k_sub(r.p, bp, u.p, n, s);
                     const double rt = norm(r.p);
                     rnorm = rt;
                     if (rt <= tol * bnorm) {
@@ -201,7 +201,7 @@ struct Solver {
             std::vector<cd> y = k ? back_substitute(H, ld, g, k) : std::vector<cd>();
             combine(x.p, y, k);
             apply(x.p, u.p);
-            k_sub(r.p, b.p, u.p, n, s);
+            k_sub(r.p, bp, u.p, n, s);
         }
         rnorm = norm(r.p);
         finish(res, its, rnorm <= tol * bnorm, rnorm / bnorm);
@@ -217,18 +217,27 @@ struct Solver {
 
 }  // namespace
 
+int default_restart(int n) { return (int)std::min(200.0, std::max(100.0, std::floor(4e8 / (16.0 * n)))); }
+
+void gmres_solve_dev(gt_op* op, const zd* b_dev, zd* x_dev, const gt_gmres_params& p, gt_gmres_result& res) {
+    gt_stack* st = op->st;
+    const int n = st->nb + st->nk;
+    Solver sv(op, p.restart > 0 ? p.restart : default_restart(n));
+    sv.bp = b_dev;
+    sv.run(p, res);
+    // x = M^-1 y (zero when y is)
+    op_precond(op, sv.x.p, x_dev);
+    GT_HIP(hipStreamSynchronize(sv.s));
+}
+
 void gmres_solve(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params& p, gt_gmres_result& res) {
     gt_stack* st = op->st;
     const int n = st->nb + st->nk;
-    int restart = p.restart;
-    if (restart <= 0) restart = (int)std::min(200.0, std::max(100.0, std::floor(4e8 / (16.0 * n))));
-    Solver sv(op, restart);
-    hipStream_t s = sv.s;
-    GT_HIP(hipMemcpyAsync(sv.b.p, b_host, sizeof(zd) * n, hipMemcpyHostToDevice, s));
-    sv.run(p, res);
-    // x = M^-1 y (zero when y is)
-    op_precond(op, sv.x.p, sv.t.p);
-    GT_HIP(hipMemcpyAsync(x_host, sv.t.p, sizeof(zd) * n, hipMemcpyDeviceToHost, s));
+    hipStream_t s = st->ctx->stream;
+    DevBuf<zd> b(n), x(n);
+    GT_HIP(hipMemcpyAsync(b.p, b_host, sizeof(zd) * n, hipMemcpyHostToDevice, s));
+    gmres_solve_dev(op, b.p, x.p, p, res);
+    GT_HIP(hipMemcpyAsync(x_host, x.p, sizeof(zd) * n, hipMemcpyDeviceToHost, s));
     GT_HIP(hipStreamSynchronize(s));
 }
 

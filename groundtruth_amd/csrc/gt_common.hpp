@@ -55,9 +55,13 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // Set-up copies wait for completion: their host sources are often temporaries.
     void upload(const T* host, size_t count, hipStream_t s) {
         if (count > n) fail(GT_ERR_ARG, "upload larger than the buffer");
-        if (count) GT_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) {
+            GT_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+            GT_HIP(hipStreamSynchronize(s));
+        }
     }
     void upload(const std::vector<T>& v, hipStream_t s) {
         if (v.size() != n) alloc(v.size());

@@ -22,6 +22,7 @@ namespace gt {
 struct OrientHost {
     bool present = false;
     int n0 = 0, n1 = 0, P0 = 0, P1 = 0, log0 = 0, log1 = 0, TR = 1, lo = 0, cnt = 0;
+    size_t t_off = 0;          // offset of this orientation's L_p intermediates in T1 / T2
     DevBuf<int> rowptr, rowcol;
     DevBuf<double> K;
     DevBuf<float> Kf;
@@ -47,7 +48,7 @@ struct gt_stack_s {
     std::vector<unsigned char> h_strip_sheet;
     std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
     gt::LpShape shape;
-    size_t t_elems = 0;                       // complex elements of one L_p intermediate
+    size_t t_elems = 0;                       // complex elements of the L_p intermediates (both orientations)
     gt::Work work;
     std::unique_ptr<gt::Schur> schur;
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)

@@ -327,6 +327,17 @@ class PlaneStack:
         _lib.check(_lib.lib().gt_gmres(op.handle, _lib.ptr(bb), _lib.ptr(out), C.byref(p), C.byref(res)), "gt_gmres")
         return out.view(np.complex128).copy(), int(res.iterations), bool(res.converged), res
 
+    def gmres_dev(self, op: Operator, b_dev, x_dev, tol: float, bnorm: Optional[float] = None, max_cycles: int = 8,
+                  restart: Optional[int] = None, extra_cycles: int = GMRES_EXTRA_CYCLES):
+        """The device GMRES on device-resident vectors (torch complex128 tensors of nb + n_keep values, their work
+        finished): x_dev = M^-1 y. Returns (iterations, converged, result)."""
+        p = _lib.GmresParams(float(tol), float(bnorm or 0.0), int(restart or 0), int(max_cycles),
+                             int(extra_cycles), 0, GMRES_CHECK_MISSES)
+        res = _lib.GmresResult()
+        _lib.check(_lib.lib().gt_gmres_dev(op.handle, C.c_void_p(b_dev.data_ptr()), C.c_void_p(x_dev.data_ptr()),
+                                           C.byref(p), C.byref(res)), "gt_gmres_dev")
+        return int(res.iterations), bool(res.converged), res
+
     # ---------------------------------------------------------------- device products on host vectors
     def _dev(self, v: np.ndarray):
         import torch

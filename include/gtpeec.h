@@ -160,6 +160,10 @@ typedef struct {
  * the best so far, as the reference). Returns GT_OK; convergence is in result->converged. */
 int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params* params,
              gt_gmres_result* result);
+/* The same with b and x resident on the device (n complex values each, ordered on the context's stream; the
+ * call returns when x is written). */
+int gt_gmres_dev(gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* params,
+                 gt_gmres_result* result);
 
 #ifdef __cplusplus
 }
