@@ -75,10 +75,10 @@ def cpu_baseline(board, route, f: float) -> dict:
     t0 = time.perf_counter()
     st.operator(f)
     t1 = time.perf_counter()
-    sol = ex.solve(f, tol=board.tol)
+    sol = ex.solve(f, tol=board.tol, restart=board.restart)
     t2 = time.perf_counter()
     return {"value": 1.0 / (t2 - t0), "unit": "solves/s", "cores": os.cpu_count(), "kind": "port",
-            "sample": f"1 of the 10 solves (f = {f:.4g} Hz, {sol.iterations} iterations): factorisation "
+            "sample": f"1 of the 10 solves (f = {f:.4g} Hz, GMRES({board.restart}), {sol.iterations} iterations): factorisation "
                       f"{t1 - t0:.2f} s + GMRES {t2 - t1:.2f} s; scipy FFT on all {os.cpu_count()} cores, "
                       f"SuperLU 1 thread; {platform.processor() or platform.machine()}"}
 

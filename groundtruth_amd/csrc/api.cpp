@@ -360,7 +360,7 @@ int gt_schur_stats(const gt_stack* st, int64_t* nnz_L, int32_t* n_super, int32_t
         const SchurHost& h = st->schur->h;
         if (nnz_L) *nnz_L = h.nnz_L;
         if (n_super) *n_super = (int32_t)h.c0.size();
-        if (n_levels) *n_levels = (int32_t)h.fwd_levels_ptr.size() - 1;
+        if (n_levels) *n_levels = h.levels;
         if (max_cols) *max_cols = h.max_cols;
     });
 }
@@ -408,7 +408,7 @@ int gt_schur_host_stats(const gt_schur_host* hh, int64_t* nnz_L, int32_t* n_supe
         const SchurHost& h = hh->h;
         if (nnz_L) *nnz_L = h.nnz_L;
         if (n_super) *n_super = (int32_t)h.c0.size();
-        if (n_levels) *n_levels = (int32_t)h.fwd_levels_ptr.size() - 1;
+        if (n_levels) *n_levels = h.levels;
         if (max_cols) *max_cols = h.max_cols;
     });
 }

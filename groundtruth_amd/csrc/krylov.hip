@@ -22,10 +22,18 @@ __device__ __forceinline__ double block_sum(double val, double* lds) {
     return r;
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk of the vectors: G basis rows per sweep over
+// w, every row's sum reduced within each wave by shuffles and across the four waves in LDS (two barriers a sweep).
 __global__ __launch_bounds__(NTH) void kr_dots(const zd* __restrict__ V, int m, const zd* __restrict__ w, int n,
                                                int chunk, double* part, int P, int wslot) {
-    __shared__ double lds[NTH / 64];
+    __shared__ double red[NTH / 64][2 * G + 1];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double* out = part + (size_t)blockIdx.x * P;
     for (int g0 = 0; g0 < m; g0 += G) {
         const int gm = min(G, m - g0);
@@ -35,7 +43,7 @@ __global__ __launch_bounds__(NTH) void kr_dots(const zd* __restrict__ V, int m, 
         double ww = 0;
         for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
             const zd wi = w[i];
-            if (g0 == 0) ww += wi.re * wi.re + wi.im * wi.im;
+            ww += wi.re * wi.re + wi.im * wi.im;
 #pragma unroll
             for (int r = 0; r < G; ++r) {
                 if (r < gm) {
@@ -48,18 +56,24 @@ __global__ __launch_bounds__(NTH) void kr_dots(const zd* __restrict__ V, int m, 
 #pragma unroll
         for (int r = 0; r < G; ++r) {
             if (r < gm) {
-                const double sr = block_sum(ar[r], lds);
-                const double si = block_sum(ai[r], lds);
-                if (threadIdx.x == 0) {
-                    out[2 * (g0 + r)] = sr;
-                    out[2 * (g0 + r) + 1] = si;
-                }
+                ar[r] = wave_sum(ar[r]);
+                ai[r] = wave_sum(ai[r]);
             }
         }
-        if (g0 == 0) {
-            const double s = block_sum(ww, lds);
-            if (threadIdx.x == 0) out[wslot] = s;
+        ww = wave_sum(ww);
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                red[wave][2 * r] = ar[r];
+                red[wave][2 * r + 1] = ai[r];
+            }
+            red[wave][2 * G] = ww;
         }
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (t < 2 * gm) out[2 * g0 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        if (g0 == 0 && t == 2 * G) out[wslot] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        __syncthreads();
     }
 }
 
@@ -145,7 +159,7 @@ void Krylov::init(int n_, int rows) {
     if (rows > 512) fail(GT_ERR_ARG, "GMRES restart above 511");
     n = n_;
     rows_cap = rows;
-    nblk = (int)std::min<long>(1024, std::max<long>(1, (n + 4095) / 4096));
+    nblk = (int)std::min<long>(2048, std::max<long>(1, (n + 1023) / 1024));
     part.alloc((size_t)nblk * (2 * rows + 8));
     red.alloc(2 * rows + 8);
 }

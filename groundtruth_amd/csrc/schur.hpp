@@ -19,15 +19,20 @@ struct SchurHost {
     std::vector<int64_t> roff;          // offset of s's off-diagonal rows in `rows`
     std::vector<int> rows;              // off-diagonal row indices (new numbering), ascending per supernode
     std::vector<int64_t> poff;          // offset of s's panel in `panel`: (nc + nr) x nc, column-major
+    // the solve panel Q = [L_CC^-1 ; -L_RC L_CC^-1] (column-major, m x nc): the forward solve of a supernode is
+    // [x_C ; u_R] = Q f_C + [0 ; f_R] and the backward x_C = Q^T [z_C ; x_R]: dense products, no serial chain
     std::vector<double> panel;
     std::vector<int> parent;            // assembly-tree parent (-1 at a root)
     std::vector<int> child_ptr, child;  // children per supernode (CSR, ascending)
-    std::vector<int64_t> rel_off;       // offset of s's row map in `rel` (nr[s] entries)
-    std::vector<int> rel;               // position of each off-diagonal row of s in its parent's front
-    std::vector<int64_t> foff;          // offset of s's front vector (nc + nr) in the device scratch
-    int64_t front_total = 0;
-    std::vector<int> fwd_levels_ptr, fwd_levels;   // supernodes by height (leaves first)
-    std::vector<int> bwd_levels_ptr, bwd_levels;   // supernodes by depth (roots first)
+    std::vector<int64_t> uoff;          // offset of s's update vector (nr entries) in the update buffer
+    int64_t upd_total = 0;
+    std::vector<int64_t> fpos;          // offset of s's front positions (nc + nr) in the contributor CSR
+    std::vector<int64_t> cptr;          // contributors of each front position: update-buffer indices, children
+    std::vector<int64_t> cidx;          //   in ascending order (a fixed summation order)
+    int levels = 0;
+    std::vector<int> fwd_ptr, bwd_ptr;  // work items per level (forward: leaves first; backward: roots first)
+    std::vector<int> fwd_s, fwd_r0;     // forward items: supernode, first row of its block
+    std::vector<int> bwd_s, bwd_q0, bwd_qn;   // backward items: supernode, first column, columns
     int64_t nnz_L = 0;
     int max_cols = 0;
 
@@ -35,23 +40,27 @@ struct SchurHost {
     // diagonal (degree incl. edges to reference nodes); xyz: (sheet, i, j) per node
     void build(int n, const std::vector<int>& ea, const std::vector<int>& eb, const std::vector<double>& diag,
                const int32_t* xyz);
-    // x = B^-1 b on the host (complex values interleaved)
+    // x = B^-1 b on the host (complex values interleaved), with the same panels as the device
     void solve_host(const double* b, double* x) const;
 };
 
+constexpr int SCHUR_ROWS = 64;         // panel rows per forward work item
+constexpr int SCHUR_COLS = 16;         // panel columns per backward work item of a wide supernode
+constexpr int SCHUR_COLS_MAX = 64;     // a supernode of at most this many columns is one backward item
+
 struct Schur {
     SchurHost h;
-    DevBuf<int> perm, c0, nc, nr, child_ptr, child, rows, rel, fwd_levels, bwd_levels;
-    DevBuf<int64_t> roff, poff, rel_off, foff;
+    DevBuf<int> perm, c0, nc, nr, rows, fwd_s, fwd_r0, bwd_s, bwd_q0, bwd_qn;
+    DevBuf<int64_t> roff, poff, uoff, fpos, cptr, cidx;
     DevBuf<double> panel;
-    DevBuf<zd> fvec;                    // front vectors of one solve
-    DevBuf<zd> xp;                      // the solution in the new numbering
+    DevBuf<zd> ubuf;                    // update vectors of one forward solve
+    DevBuf<zd> zp, xp;                  // forward and backward solutions in the new numbering
 
     void upload(hipStream_t s);
     int64_t device_bytes() const;
 };
 
-// x = B^-1 b on the device; b and x hold n complex values in the old numbering (may alias? no)
+// x = B^-1 b on the device; b and x hold n complex values in the old numbering (distinct buffers)
 void schur_solve_device(Schur& S, const zd* b, zd* x, hipStream_t s);
 
 }  // namespace gt

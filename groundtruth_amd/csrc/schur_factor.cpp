@@ -6,6 +6,7 @@
 // stack; the frequency enters through the scalar d_s and the Woodbury matrix (the caller's part).
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <numeric>
 
@@ -232,15 +233,47 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
         for (int q = 0; q < C; ++q) pos[c0[s] + q] = -1;
         for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = -1;
     }
-    // ---- each supernode's rows in its parent's front
-    rel_off.assign(ns + 1, 0);
-    for (int s = 0; s < ns; ++s) rel_off[s + 1] = rel_off[s] + nr[s];
-    rel.assign(rel_off[ns], 0);
-    for (int s = 0; s < ns; ++s) {
-        const int p = parent[s];
+    // ---- solve panels: Q = [L_CC^-1 ; -L_RC L_CC^-1] in place of each L panel
+    for (int sn = 0; sn < ns; ++sn) {
+        const int C = nc[sn], m = nc[sn] + nr[sn];
+        double* P = &panel[poff[sn]];
+        std::vector<double> inv((size_t)C * C, 0.0);      // column-major, lower
+        for (int j = 0; j < C; ++j) {
+            inv[(size_t)j * C + j] = 1.0 / P[(size_t)j * m + j];
+            for (int i = j + 1; i < C; ++i) {
+                double acc = 0.0;
+                for (int k = j; k < i; ++k) acc += P[(size_t)k * m + i] * inv[(size_t)j * C + k];
+                inv[(size_t)j * C + i] = -acc / P[(size_t)i * m + i];
+            }
+        }
+        // -L_RC L_CC^-1, column j = -sum_{k >= j} L_RC[:, k] inv[k, j]
+        const int R = nr[sn];
+        std::vector<double> M((size_t)R * C, 0.0);
+        for (int j = 0; j < C; ++j)
+            for (int k = j; k < C; ++k) {
+                const double f = inv[(size_t)j * C + k];
+                if (f == 0.0) continue;
+                const double* Lk = P + (size_t)k * m + C;
+                double* Mj = &M[(size_t)j * R];
+                for (int a = 0; a < R; ++a) Mj[a] -= Lk[a] * f;
+            }
+        for (int j = 0; j < C; ++j) {
+            for (int i = 0; i < C; ++i) P[(size_t)j * m + i] = i >= j ? inv[(size_t)j * C + i] : 0.0;
+            for (int a = 0; a < R; ++a) P[(size_t)j * m + C + a] = M[(size_t)j * R + a];
+        }
+    }
+    // ---- update vectors and the contributors of each front position (children ascending)
+    uoff.assign(ns + 1, 0);
+    for (int sn = 0; sn < ns; ++sn) uoff[sn + 1] = uoff[sn] + nr[sn];
+    upd_total = uoff[ns];
+    fpos.assign(ns + 1, 0);
+    for (int sn = 0; sn < ns; ++sn) fpos[sn + 1] = fpos[sn] + nc[sn] + nr[sn];
+    std::vector<int64_t> at_of(upd_total, 0);          // front position (global) each update entry adds into
+    for (int sn = 0; sn < ns; ++sn) {
+        const int p = parent[sn];
         if (p < 0) continue;
-        for (int a = 0; a < nr[s]; ++a) {
-            const int c = rows[roff[s] + a];
+        for (int a = 0; a < nr[sn]; ++a) {
+            const int c = rows[roff[sn] + a];
             int at;
             if (c < c0[p] + nc[p]) {
                 at = c - c0[p];
@@ -252,75 +285,96 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
                 at = nc[p] + (int)(f - b);
             }
             if (at < 0) fail(GT_ERR_ARG, "a child's row lies below its parent");
-            rel[rel_off[s] + a] = at;
+            at_of[uoff[sn] + a] = fpos[p] + at;
         }
     }
-    // ---- front vectors and levels
-    foff.assign(ns + 1, 0);
-    for (int s = 0; s < ns; ++s) foff[s + 1] = foff[s] + nc[s] + nr[s];
-    front_total = foff[ns];
+    cptr.assign(fpos[ns] + 1, 0);
+    for (int sn = 0; sn < ns; ++sn)
+        if (parent[sn] >= 0)
+            for (int a = 0; a < nr[sn]; ++a) cptr[at_of[uoff[sn] + a] + 1]++;
+    for (int64_t q = 0; q < fpos[ns]; ++q) cptr[q + 1] += cptr[q];
+    cidx.assign(cptr[fpos[ns]], 0);
+    {
+        std::vector<int64_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int sn = 0; sn < ns; ++sn)         // ascending: each position's contributors in child order
+            if (parent[sn] >= 0)
+                for (int a = 0; a < nr[sn]; ++a) cidx[fill[at_of[uoff[sn] + a]]++] = uoff[sn] + a;
+    }
+    // ---- levels and their work items
     std::vector<int> h(ns, 0), dep(ns, 0);
     int hmax = 0, dmax = 0;
-    for (int s = 0; s < ns; ++s) {
-        for (int e = child_ptr[s]; e < child_ptr[s + 1]; ++e) h[s] = std::max(h[s], h[child[e]] + 1);
-        hmax = std::max(hmax, h[s]);
+    for (int sn = 0; sn < ns; ++sn) {
+        for (int e = child_ptr[sn]; e < child_ptr[sn + 1]; ++e) h[sn] = std::max(h[sn], h[child[e]] + 1);
+        hmax = std::max(hmax, h[sn]);
     }
-    for (int s = ns - 1; s >= 0; --s) {
-        dep[s] = parent[s] < 0 ? 0 : dep[parent[s]] + 1;
-        dmax = std::max(dmax, dep[s]);
+    for (int sn = ns - 1; sn >= 0; --sn) {
+        dep[sn] = parent[sn] < 0 ? 0 : dep[parent[sn]] + 1;
+        dmax = std::max(dmax, dep[sn]);
     }
-    auto group = [&](const std::vector<int>& key, int kmax, std::vector<int>& ptr, std::vector<int>& lst) {
-        ptr.assign(kmax + 2, 0);
-        for (int s = 0; s < ns; ++s) ptr[key[s] + 1]++;
-        for (int k = 0; k <= kmax; ++k) ptr[k + 1] += ptr[k];
-        lst.assign(ns, 0);
-        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-        for (int s = 0; s < ns; ++s) lst[fill[key[s]]++] = s;
-    };
-    group(h, hmax, fwd_levels_ptr, fwd_levels);
-    group(dep, dmax, bwd_levels_ptr, bwd_levels);
+    levels = ns ? hmax + 1 : 0;
+    fwd_ptr.assign(1, 0);
+    fwd_s.clear();
+    fwd_r0.clear();
+    for (int l = 0; l <= hmax && ns; ++l) {
+        for (int sn = 0; sn < ns; ++sn)
+            if (h[sn] == l)
+                for (int r0 = 0; r0 < nc[sn] + nr[sn]; r0 += SCHUR_ROWS) { fwd_s.push_back(sn); fwd_r0.push_back(r0); }
+        fwd_ptr.push_back((int)fwd_s.size());
+    }
+    bwd_ptr.assign(1, 0);
+    bwd_s.clear();
+    bwd_q0.clear();
+    bwd_qn.clear();
+    for (int l = 0; l <= dmax && ns; ++l) {
+        for (int sn = 0; sn < ns; ++sn) {
+            if (dep[sn] != l) continue;
+            const int w = nc[sn] <= SCHUR_COLS_MAX ? nc[sn] : SCHUR_COLS;
+            for (int q0 = 0; q0 < nc[sn]; q0 += w) {
+                bwd_s.push_back(sn);
+                bwd_q0.push_back(q0);
+                bwd_qn.push_back(std::min(w, nc[sn] - q0));
+            }
+        }
+        bwd_ptr.push_back((int)bwd_s.size());
+    }
     (void)roots;
 }
 
 void SchurHost::solve_host(const double* b, double* x) const {
-    std::vector<double> w(2 * (size_t)n);
-    for (int c = 0; c < n; ++c) { w[2 * c] = b[2 * perm[c]]; w[2 * c + 1] = b[2 * perm[c] + 1]; }
+    typedef std::complex<double> cd;
     const int ns = (int)c0.size();
-    for (int s = 0; s < ns; ++s) {
-        const int C = nc[s], m = nc[s] + nr[s];
-        const double* L = &panel[poff[s]];
+    std::vector<cd> zp(n), xp(n), ub(upd_total);
+    for (int sn = 0; sn < ns; ++sn) {
+        const int C = nc[sn], m = nc[sn] + nr[sn];
+        const double* Q = &panel[poff[sn]];
+        std::vector<cd> f(m);
+        for (int q = 0; q < m; ++q) {
+            cd acc = q < C ? cd(b[2 * perm[c0[sn] + q]], b[2 * perm[c0[sn] + q] + 1]) : cd(0);
+            const int64_t pos = fpos[sn] + q;
+            for (int64_t e = cptr[pos]; e < cptr[pos + 1]; ++e) acc += ub[cidx[e]];
+            f[q] = acc;
+        }
+        for (int r = 0; r < m; ++r) {
+            cd acc = r < C ? cd(0) : f[r];
+            const int qmax = r < C ? r + 1 : C;
+            for (int q = 0; q < qmax; ++q) acc += Q[(size_t)q * m + r] * f[q];
+            if (r < C) zp[c0[sn] + r] = acc;
+            else ub[uoff[sn] + r - C] = acc;
+        }
+    }
+    for (int sn = ns - 1; sn >= 0; --sn) {
+        const int C = nc[sn], m = nc[sn] + nr[sn];
+        const double* Q = &panel[poff[sn]];
         for (int q = 0; q < C; ++q) {
-            const int col = c0[s] + q;
-            const double d = L[(size_t)q * m + q];
-            w[2 * col] /= d;
-            w[2 * col + 1] /= d;
-            const double xr = w[2 * col], xi = w[2 * col + 1];
-            for (int i = q + 1; i < m; ++i) {
-                const int row = i < C ? c0[s] + i : rows[roff[s] + i - C];
-                const double l = L[(size_t)q * m + i];
-                w[2 * row] -= l * xr;
-                w[2 * row + 1] -= l * xi;
+            cd acc = 0;
+            for (int r = q; r < m; ++r) {
+                const cd v = r < C ? zp[c0[sn] + r] : xp[rows[roff[sn] + r - C]];
+                acc += Q[(size_t)q * m + r] * v;
             }
+            xp[c0[sn] + q] = acc;
         }
     }
-    for (int s = ns - 1; s >= 0; --s) {
-        const int C = nc[s], m = nc[s] + nr[s];
-        const double* L = &panel[poff[s]];
-        for (int q = C - 1; q >= 0; --q) {
-            const int col = c0[s] + q;
-            double ar = w[2 * col], ai = w[2 * col + 1];
-            for (int i = q + 1; i < m; ++i) {
-                const int row = i < C ? c0[s] + i : rows[roff[s] + i - C];
-                const double l = L[(size_t)q * m + i];
-                ar -= l * w[2 * row];
-                ai -= l * w[2 * row + 1];
-            }
-            const double d = L[(size_t)q * m + q];
-            w[2 * col] = ar / d;
-            w[2 * col + 1] = ai / d;
-        }
-    }
-    for (int c = 0; c < n; ++c) { x[2 * perm[c]] = w[2 * c]; x[2 * perm[c] + 1] = w[2 * c + 1]; }
+    for (int c = 0; c < n; ++c) { x[2 * perm[c]] = xp[c].real(); x[2 * perm[c] + 1] = xp[c].imag(); }
 }
 
 }  // namespace gt

@@ -264,9 +264,10 @@ class PlaneStack:
         return op
 
     def solve(self, f: float, rhs_top: np.ndarray, J: np.ndarray, tol: float = 1e-5, max_cycles: int = 8,
-              guess: Optional[list] = None):
+              guess: Optional[list] = None, restart: Optional[int] = None):
         """K [I; V] = [rhs_top; J[keep]] by right-preconditioned GMRES, with the least-squares warm start over
-        `guess` (peec.py:2513-2607, own factorisation: no borrow branch)."""
+        `guess` (peec.py:2513-2607, own factorisation: no borrow branch). `restart` sets the GMRES restart length
+        (None: the reference's gmres_restart(n))."""
         op = self.operator(f)
         m = HostMath(self)
         nb = self.nb
@@ -281,7 +282,7 @@ class PlaneStack:
             if np.linalg.norm(r) < 0.9 * bnorm:
                 x0, r0 = np.stack(G, axis=1) @ c, r
         y, its, ok = gmres(lambda v: m.matvec(op, m.precond(op, v)), r0, tol, max_cycles=max_cycles, bnorm=bnorm,
-                           extra_cycles=GMRES_EXTRA_CYCLES)
+                           restart=restart, extra_cycles=GMRES_EXTRA_CYCLES)
         if not ok:
             raise SolverError(f"GMRES did not converge at {f:.4g} Hz ({its} iterations)")
         x = m.precond(op, y) if y.any() else np.zeros(len(rhs), complex)
@@ -645,13 +646,13 @@ class RouteExcitation:
             J[n] += amps
         return -1j * w * self.psi, J
 
-    def solve(self, f: float, guess: Optional[list] = None, tol: float = 1e-5, max_cycles: int = 8) -> Solution:
-        """Solve and unpack I_x, I_y, V, branch currents, Z_loop and the moment (peec.py:2787-2817). `tol` and
-        `max_cycles` are passed to PlaneStack.solve (the reference uses its defaults there)."""
+    def solve(self, f: float, guess: Optional[list] = None, tol: float = 1e-5, max_cycles: int = 8,
+              restart: Optional[int] = None) -> Solution:
+        """Solve and unpack I_x, I_y, V, branch currents, Z_loop and the moment (peec.py:2787-2817). `tol`,
+        `max_cycles` and `restart` are passed to PlaneStack.solve (the reference uses its defaults there)."""
         st = self.stack
-        w = 2 * math.pi * f
         rhs_top, J = self.rhs(f)
-        Ib, V, its, x = st.solve(f, rhs_top, J, tol=tol, max_cycles=max_cycles, guess=guess)
+        Ib, V, its, x = st.solve(f, rhs_top, J, tol=tol, max_cycles=max_cycles, guess=guess, restart=restart)
         return self.unpack(f, Ib, V, its, x)
 
     def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
