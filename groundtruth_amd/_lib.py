@@ -66,6 +66,8 @@ SIGNATURES = {
     "gt_precond": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_gmres": (C.c_int, [C.c_void_p, f64p, f64p, C.POINTER(GmresParams), C.POINTER(GmresResult)]),
     "gt_gmres_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GmresParams), C.POINTER(GmresResult)]),
+    "gt_gmres_batch": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p), C.POINTER(GmresParams), C.POINTER(GmresResult)]),
 }
 
 _LIB = None

@@ -7,8 +7,8 @@
 #include "stack.hpp"
 
 namespace gt {
-void gmres_solve(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params& p, gt_gmres_result& res);
-void gmres_solve_dev(gt_op* op, const zd* b_dev, zd* x_dev, const gt_gmres_params& p, gt_gmres_result& res);
+void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x, const gt_gmres_params& p,
+                 gt_gmres_result* res);
 }
 
 using namespace gt;
@@ -137,7 +137,7 @@ struct gt_schur_host_s {
     SchurHost h;
 };
 
-LpArgs<double> gt_stack_s::lp_args(const zd* x, zd* y, int mode) const {
+LpArgs<double> gt_stack_s::lp_args(int mode) const {
     LpArgs<double> a{};
     for (int oi = 0; oi < 2; ++oi) {
         const OrientHost& h = o[oi];
@@ -164,12 +164,9 @@ LpArgs<double> gt_stack_s::lp_args(const zd* x, zd* y, int mode) const {
     a.S = S;
     a.mode = mode;
     a.nb = nb;
-    a.x = x;
-    a.y = y;
+    a.t_stride = t_elems;
     a.a_keep = a_keep.p;
     a.b_keep = b_keep.p;
-    a.r_sheet = nullptr;
-    a.w = 0;
     return a;
 }
 
@@ -282,11 +279,7 @@ int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
             }
         }
         st->node_sheet.upload(ns_, s);
-        st->work.T1.alloc(std::max<size_t>(1, st->t_elems));
-        st->work.T2.alloc(std::max<size_t>(1, st->t_elems));
-        st->work.s.alloc(std::max(1, nk));
-        st->work.z.alloc(std::max(1, nk));
-        st->work.t.alloc(std::max(1, d->nbr));
+        st->work.ensure(st.get(), 1);
         GT_HIP(hipStreamSynchronize(s));
         *out = st.release();
     });
@@ -305,7 +298,10 @@ int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
     return guard([&] {
         need(st && I_dev && out_dev, "NULL argument");
         if (st->ns == 0) return;
-        LpArgs<double> a = st->lp_args((const zd*)I_dev, (zd*)out_dev, 0);
+        LpArgs<double> a = st->lp_args(0);
+        a.rhs.nr = 1;
+        a.rhs.x[0] = (const zd*)I_dev;
+        a.rhs.y[0] = (zd*)out_dev;
         launch_lp(a, st->shape, st->ctx->stream);
     });
 }
@@ -374,8 +370,8 @@ int gt_schur_solve_host(const gt_stack* st, const double* b, double* x) {
 
 int gt_schur_solve(gt_stack* st, const void* b_dev, void* x_dev) {
     return guard([&] {
-        need(st && b_dev && x_dev, "NULL argument");
-        schur_apply(st, (const zd*)b_dev, (zd*)x_dev);
+        need(st && b_dev && x_dev && b_dev != x_dev, "NULL or aliased argument");
+        schur_apply(st, (const zd*)b_dev, (zd*)x_dev, 1);
     });
 }
 
@@ -449,7 +445,9 @@ void gt_op_destroy(gt_op* op) { delete op; }
 int gt_matvec(gt_op* op, const void* x_dev, void* y_dev) {
     return guard([&] {
         need(op && x_dev && y_dev && x_dev != y_dev, "NULL or aliased argument");
-        op_matvec(op, (const zd*)x_dev, (zd*)y_dev);
+        const zd* x = (const zd*)x_dev;
+        zd* y = (zd*)y_dev;
+        ops_matvec(op->st, 1, &op, &x, &y);
     });
 }
 
@@ -457,7 +455,9 @@ int gt_precond(gt_op* op, const void* r_dev, void* z_dev) {
     return guard([&] {
         need(op && r_dev && z_dev && r_dev != z_dev, "NULL or aliased argument");
         need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
-        op_precond(op, (const zd*)r_dev, (zd*)z_dev);
+        const zd* r = (const zd*)r_dev;
+        zd* z = (zd*)z_dev;
+        ops_precond(op->st, 1, &op, &r, &z);
     });
 }
 
@@ -467,7 +467,15 @@ int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_par
         need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
         GT_HIP(hipSetDevice(op->st->ctx->device));
-        gmres_solve(op, b_host, x_host, *p, *res);
+        const int n = op->st->nb + op->st->nk;
+        hipStream_t s = op->st->ctx->stream;
+        DevBuf<zd> b(n), x(n);
+        GT_HIP(hipMemcpyAsync(b.p, b_host, sizeof(zd) * n, hipMemcpyHostToDevice, s));
+        const zd* bp = b.p;
+        zd* xp = x.p;
+        gmres_batch(op->st, 1, &op, &bp, &xp, *p, res);
+        GT_HIP(hipMemcpyAsync(x_host, x.p, sizeof(zd) * n, hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
     });
 }
 
@@ -477,7 +485,29 @@ int gt_gmres_dev(gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_param
         need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
         GT_HIP(hipSetDevice(op->st->ctx->device));
-        gmres_solve_dev(op, (const zd*)b_dev, (zd*)x_dev, *p, *res);
+        const zd* bp = (const zd*)b_dev;
+        zd* xp = (zd*)x_dev;
+        gmres_batch(op->st, 1, &op, &bp, &xp, *p, res);
+    });
+}
+
+int gt_gmres_batch(gt_stack* st, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
+                   const gt_gmres_params* p, gt_gmres_result* res) {
+    return guard([&] {
+        need(st && ops && b_dev && x_dev && p && res, "NULL argument");
+        need(nr >= 1 && nr <= KMAX, "1 to 16 right-hand sides per batch");
+        need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
+        std::vector<const zd*> b(nr);
+        std::vector<zd*> x(nr);
+        for (int i = 0; i < nr; ++i) {
+            need(ops[i] && ops[i]->st == st, "every operator must belong to the stack");
+            need(b_dev[i] && x_dev[i] && b_dev[i] != x_dev[i], "NULL or aliased vector");
+            b[i] = (const zd*)b_dev[i];
+            x[i] = (zd*)x_dev[i];
+        }
+        GT_HIP(hipSetDevice(st->ctx->device));
+        gmres_batch(st, nr, ops, b.data(), x.data(), *p, res);
     });
 }
 

@@ -1,5 +1,6 @@
-// GMRES vector work on the device. Reductions are two-stage and in a fixed order (block partial sums, then one
-// tree per value), so a solve gives the same bits on every run.
+// GMRES vector work on the device for a batch of right-hand sides (blockIdx.y = right-hand side). Reductions are
+// two-stage and in a fixed order (block partial sums, then one tree per value), so a solve gives the same bits on
+// every run and whatever else is in its batch.
 #include "krylov.hpp"
 
 namespace gt {
@@ -7,11 +8,16 @@ namespace gt {
 namespace {
 
 constexpr int NTH = 256;
-constexpr int G = 16;             // basis rows per pass of the multi-dot
+constexpr int G = 16;             // basis rows per sweep of the multi-dot
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
 
 // Sum of `val` over the workgroup; the result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double val, double* lds) {
-    for (int off = 32; off > 0; off >>= 1) val += __shfl_down(val, off, 64);
+    val = wave_sum(val);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) lds[wave] = val;
@@ -22,19 +28,17 @@ __device__ __forceinline__ double block_sum(double val, double* lds) {
     return r;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk of the vectors: G basis rows per sweep over
-// w, every row's sum reduced within each wave by shuffles and across the four waves in LDS (two barriers a sweep).
-__global__ __launch_bounds__(NTH) void kr_dots(const zd* __restrict__ V, int m, const zd* __restrict__ w, int n,
-                                               int chunk, double* part, int P, int wslot) {
+// Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk: G basis rows per sweep over w, every row's sum
+// reduced within each wave by shuffles and across the four waves in LDS (two barriers a sweep).
+__global__ __launch_bounds__(NTH) void kr_dots(KrSet k, int n, int chunk, double* part, int P, int nblk, int wslot) {
     __shared__ double red[NTH / 64][2 * G + 1];
+    const int p = blockIdx.y;
+    const zd* __restrict__ V = k.V[p];
+    const zd* __restrict__ w = k.w[p];
+    const int m = k.m[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double* out = part + (size_t)blockIdx.x * P;
+    double* out = part + ((size_t)p * nblk + blockIdx.x) * P;
     for (int g0 = 0; g0 < m; g0 += G) {
         const int gm = min(G, m - g0);
         double ar[G], ai[G];
@@ -77,11 +81,16 @@ __global__ __launch_bounds__(NTH) void kr_dots(const zd* __restrict__ V, int m, 
     }
 }
 
-__global__ __launch_bounds__(NTH) void kr_update(const zd* __restrict__ V, int m, zd* w, int n, int chunk,
-                                                 const double* __restrict__ h, double* part, int P, int slot) {
+__global__ __launch_bounds__(NTH) void kr_update(KrSet k, int n, int chunk, const double* red, double* part, int P,
+                                                 int nblk, int slot) {
     __shared__ double lds[NTH / 64];
     __shared__ double hs_raw[1024];
     zd* hs = reinterpret_cast<zd*>(hs_raw);
+    const int p = blockIdx.y;
+    const zd* __restrict__ V = k.V[p];
+    zd* w = k.w[p];
+    const int m = k.m[p];
+    const double* h = red + (size_t)p * P;
     for (int r = threadIdx.x; r < m; r += NTH) hs[r] = zd(h[2 * r], h[2 * r + 1]);
     __syncthreads();
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
@@ -98,12 +107,13 @@ __global__ __launch_bounds__(NTH) void kr_update(const zd* __restrict__ V, int m
         nn += acc.re * acc.re + acc.im * acc.im;
     }
     const double s = block_sum(nn, lds);
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * P + slot] = s;
+    if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot] = s;
 }
 
-__global__ __launch_bounds__(NTH) void kr_norm(const zd* __restrict__ v, int n, int chunk, double* part, int P,
-                                               int slot) {
+__global__ __launch_bounds__(NTH) void kr_norm(KrSet k, int n, int chunk, double* part, int P, int nblk, int slot) {
     __shared__ double lds[NTH / 64];
+    const int p = blockIdx.y;
+    const zd* __restrict__ v = k.w[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double nn = 0;
     for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
@@ -111,46 +121,51 @@ __global__ __launch_bounds__(NTH) void kr_norm(const zd* __restrict__ v, int n, 
         nn += a.re * a.re + a.im * a.im;
     }
     const double s = block_sum(nn, lds);
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * P + slot] = s;
+    if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot] = s;
 }
 
-// red[c] = sum over blocks of part[b][c] for c in [c0, c1): one workgroup per value
-__global__ __launch_bounds__(NTH) void kr_reduce(const double* __restrict__ part, int nblk, int P, int c0,
-                                                 double* red) {
+// red[p][c0 + blockIdx.x] = sum over blocks of part[p][b][c0 + blockIdx.x] (one workgroup per value)
+__global__ __launch_bounds__(NTH) void kr_reduce(const double* __restrict__ part, int nblk, int P, int c0, double* red) {
     __shared__ double lds[NTH / 64];
-    const int c = c0 + blockIdx.x;
+    const int p = blockIdx.y, c = c0 + blockIdx.x;
     double a = 0;
-    for (int b = threadIdx.x; b < nblk; b += NTH) a += part[(size_t)b * P + c];
+    for (int b = threadIdx.x; b < nblk; b += NTH) a += part[((size_t)p * nblk + b) * P + c];
     const double s = block_sum(a, lds);
-    if (threadIdx.x == 0) red[c] = s;
+    if (threadIdx.x == 0) red[(size_t)p * P + c] = s;
 }
 
-__global__ __launch_bounds__(NTH) void kr_scale(zd* dst, const zd* src, double f, int n) {
-    const int i = blockIdx.x * NTH + threadIdx.x;
-    if (i < n) dst[i] = f * src[i];
+__global__ __launch_bounds__(NTH) void kr_scale(VecOps v, int n) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (i < n) v.dst[p][i] = v.f[p] * v.a[p][i];
 }
 
-__global__ __launch_bounds__(NTH) void kr_combine(zd* x, const zd* __restrict__ V, const zd* __restrict__ y, int k,
-                                                  int n) {
+__global__ __launch_bounds__(NTH) void kr_sub(VecOps v, int n) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (i < n) v.dst[p][i] = v.a[p][i] - v.b[p][i];
+}
+
+__global__ __launch_bounds__(NTH) void kr_copy(VecOps v, int n) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (i < n) v.dst[p][i] = v.a[p][i];
+}
+
+__global__ __launch_bounds__(NTH) void kr_zero(VecOps v, int n) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (i < n) v.dst[p][i] = zd();
+}
+
+__global__ __launch_bounds__(NTH) void kr_combine(VecOps v, const zd* __restrict__ y, int rows, int n) {
     __shared__ double ys_raw[1024];
     zd* ys = reinterpret_cast<zd*>(ys_raw);
-    for (int r = threadIdx.x; r < k; r += NTH) ys[r] = y[r];
+    const int p = blockIdx.y, k = v.k[p];
+    for (int r = threadIdx.x; r < k; r += NTH) ys[r] = y[(size_t)p * rows + r];
     __syncthreads();
     const int i = blockIdx.x * NTH + threadIdx.x;
     if (i >= n) return;
-    zd acc = x[i];
+    const zd* V = v.a[p];
+    zd acc = v.dst[p][i];
     for (int r = 0; r < k; ++r) acc = acc + V[(size_t)r * n + i] * ys[r];
-    x[i] = acc;
-}
-
-__global__ __launch_bounds__(NTH) void kr_sub(zd* r, const zd* b, const zd* t, int n) {
-    const int i = blockIdx.x * NTH + threadIdx.x;
-    if (i < n) r[i] = b[i] - t[i];
-}
-
-__global__ __launch_bounds__(NTH) void kr_add(zd* x, const zd* t, int n) {
-    const int i = blockIdx.x * NTH + threadIdx.x;
-    if (i < n) x[i] = x[i] + t[i];
+    v.dst[p][i] = acc;
 }
 
 }  // namespace
@@ -159,59 +174,66 @@ void Krylov::init(int n_, int rows) {
     if (rows > 512) fail(GT_ERR_ARG, "GMRES restart above 511");
     n = n_;
     rows_cap = rows;
+    P = 2 * rows + 8;
     nblk = (int)std::min<long>(2048, std::max<long>(1, (n + 1023) / 1024));
-    part.alloc((size_t)nblk * (2 * rows + 8));
-    red.alloc(2 * rows + 8);
+    part.alloc((size_t)KMAX * nblk * P);
+    red.alloc((size_t)KMAX * P);
 }
 
-void Krylov::dots(const zd* V, int m, const zd* w, hipStream_t s) {
-    const int P = 2 * rows_cap + 8, chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_dots, dim3(nblk), dim3(NTH), 0, s, V, m, w, n, chunk, part.p, P, 2 * rows_cap);
+void Krylov::dots(const KrSet& k, hipStream_t s) {
+    const int chunk = (n + nblk - 1) / nblk;
+    int mmax = 0;
+    for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
+    hipLaunchKernelGGL(kr_dots, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_w());
     GT_LAUNCH("kr_dots");
-    hipLaunchKernelGGL(kr_reduce, dim3(2 * m), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
     GT_LAUNCH("kr_reduce");
-    hipLaunchKernelGGL(kr_reduce, dim3(1), dim3(NTH), 0, s, part.p, nblk, P, 2 * rows_cap, red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::project_update(const zd* V, int m, zd* w, hipStream_t s) {
-    const int P = 2 * rows_cap + 8, chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_update, dim3(nblk), dim3(NTH), 0, s, V, m, w, n, chunk, red.p, part.p, P,
-                       2 * rows_cap + 1);
+void Krylov::project_update(const KrSet& k, hipStream_t s) {
+    const int chunk = (n + nblk - 1) / nblk;
+    hipLaunchKernelGGL(kr_update, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
+                       slot_w_after());
     GT_LAUNCH("kr_update");
-    hipLaunchKernelGGL(kr_reduce, dim3(1), dim3(NTH), 0, s, part.p, nblk, P, 2 * rows_cap + 1, red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::norm2(const zd* v, int slot, hipStream_t s) {
-    const int P = 2 * rows_cap + 8, chunk = (n + nblk - 1) / nblk;
-    const int c = 2 * rows_cap + 2 + slot;
-    hipLaunchKernelGGL(kr_norm, dim3(nblk), dim3(NTH), 0, s, v, n, chunk, part.p, P, c);
+void Krylov::norm2(const KrSet& k, hipStream_t s) {
+    const int chunk = (n + nblk - 1) / nblk;
+    hipLaunchKernelGGL(kr_norm, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
     GT_LAUNCH("kr_norm");
-    hipLaunchKernelGGL(kr_reduce, dim3(1), dim3(NTH), 0, s, part.p, nblk, P, c, red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_norm(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-void k_scale(zd* dst, const zd* src, double f, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_scale, dim3(blocks_for(n, NTH)), dim3(NTH), 0, s, dst, src, f, n);
+void k_scale(const VecOps& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_scale, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_scale");
 }
 
-void k_combine(zd* x, const zd* V, const zd* y, int k, int n, hipStream_t s) {
-    if (k <= 0) return;
-    if (k > 512) fail(GT_ERR_ARG, "combination of more than 512 basis vectors");
-    hipLaunchKernelGGL(kr_combine, dim3(blocks_for(n, NTH)), dim3(NTH), 0, s, x, V, y, k, n);
-    GT_LAUNCH("kr_combine");
-}
-
-void k_sub(zd* r, const zd* b, const zd* t, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_sub, dim3(blocks_for(n, NTH)), dim3(NTH), 0, s, r, b, t, n);
+void k_sub(const VecOps& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_sub, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_sub");
 }
 
-void k_add(zd* x, const zd* t, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_add, dim3(blocks_for(n, NTH)), dim3(NTH), 0, s, x, t, n);
-    GT_LAUNCH("kr_add");
+void k_copy(const VecOps& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_copy, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+    GT_LAUNCH("kr_copy");
+}
+
+void k_zero(const VecOps& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_zero, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+    GT_LAUNCH("kr_zero");
+}
+
+void k_combine(const VecOps& v, const zd* y, int rows, int n, hipStream_t s) {
+    for (int p = 0; p < v.nr; ++p)
+        if (v.k[p] > 512) fail(GT_ERR_ARG, "combination of more than 512 basis vectors");
+    hipLaunchKernelGGL(kr_combine, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n);
+    GT_LAUNCH("kr_combine");
 }
 
 }  // namespace gt

@@ -1,30 +1,55 @@
-// Device vector work of GMRES (krylov.hip): the Gram-Schmidt projections, norms, combinations and residuals of
-// peec.py:2841-2897 `_HostVectors` / gpu.py:409-468 `TorchVectors`, with deterministic two-stage reductions.
+// Device vector work of GMRES for a batch of right-hand sides (krylov.hip): the Gram-Schmidt projections, norms,
+// combinations and residuals of peec.py:2841-2897 `_HostVectors` / gpu.py:409-468 `TorchVectors`, each right-hand
+// side with its own basis. Reductions are two-stage and in a fixed order (the same bits every run, and the same
+// bits for a right-hand side alone or in any batch).
 #pragma once
 
 #include "gt_common.hpp"
+#include "lp.hpp"
 
 namespace gt {
 
-struct Krylov {
-    int n = 0;
-    int nblk = 0;                 // partial-sum blocks of the reductions
-    DevBuf<double> part;          // nblk x (2 * (rows + 1)) partial sums
-    DevBuf<double> red;           // reduced values: h (2 * rows), then |w|^2, |w_after|^2
-    int rows_cap = 0;
-
-    void init(int n, int rows);
-    // h = V[0:m]^H w and |w|^2 -> red[0 : 2m], red[2m]
-    void dots(const zd* V, int m, const zd* w, hipStream_t s);
-    // w -= V[0:m]^T h (h = red[0:2m]); |w|^2 -> red[2m + 1]
-    void project_update(const zd* V, int m, zd* w, hipStream_t s);
-    // |v|^2 -> red[slot]
-    void norm2(const zd* v, int slot, hipStream_t s);
+struct KrSet {
+    int nr = 0;
+    const zd* V[KMAX];        // basis rows 0.. of each right-hand side (row stride n)
+    int m[KMAX];              // rows taken
+    zd* w[KMAX];              // the vector projected / updated / measured
 };
 
-void k_scale(zd* dst, const zd* src, double f, int n, hipStream_t s);                 // dst = f * src
-void k_combine(zd* x, const zd* V, const zd* y_dev, int k, int n, hipStream_t s);     // x += V[0:k]^T y
-void k_sub(zd* r, const zd* b, const zd* t, int n, hipStream_t s);                    // r = b - t
-void k_add(zd* x, const zd* t, int n, hipStream_t s);                                 // x += t
+struct Krylov {
+    int n = 0;
+    int nblk = 0;             // partial-sum blocks of the reductions
+    int rows_cap = 0;
+    int P = 0;                // values per (right-hand side, block): 2 rows_cap (h) + 8 (norms)
+    DevBuf<double> part;      // KMAX x nblk x P partial sums
+    DevBuf<double> red;       // KMAX x P reduced values
+
+    void init(int n, int rows);
+    int slot_w() const { return 2 * rows_cap; }          // |w|^2 before the projection
+    int slot_w_after() const { return 2 * rows_cap + 1; } // |w|^2 after it
+    int slot_norm() const { return 2 * rows_cap + 2; }    // norm2's value
+    // h_p = V_p[0:m_p]^H w_p and |w_p|^2
+    void dots(const KrSet& k, hipStream_t s);
+    // w_p -= V_p[0:m_p]^T h_p (h_p as reduced by dots), |w_p|^2 after
+    void project_update(const KrSet& k, hipStream_t s);
+    // |w_p|^2
+    void norm2(const KrSet& k, hipStream_t s);
+};
+
+struct VecOps {
+    int nr = 0;
+    zd* dst[KMAX];
+    const zd* a[KMAX];
+    const zd* b[KMAX];
+    double f[KMAX];
+    int k[KMAX];
+};
+
+void k_scale(const VecOps& v, int n, hipStream_t s);                    // dst = f * a
+void k_sub(const VecOps& v, int n, hipStream_t s);                      // dst = a - b
+void k_copy(const VecOps& v, int n, hipStream_t s);                     // dst = a
+void k_zero(const VecOps& v, int n, hipStream_t s);                     // dst = 0
+// dst_p += sum_{r < k_p} a_p[r n : (r+1) n] * y[p rows + r]  (y on the device, `rows` values per right-hand side)
+void k_combine(const VecOps& v, const zd* y, int rows, int n, hipStream_t s);
 
 }  // namespace gt

@@ -36,17 +36,18 @@ __global__ __launch_bounds__(NTH) void lp_rows_fwd(LpArgs<T> a) {
     const int s = blk / ntiles, i0 = (blk - s * ntiles) * o.TR;
     const int nr = min(o.TR, o.n0 - i0);
     const int P1s = o.P1 + 1;
+    const int p = blockIdx.y;
     for (int e = threadIdx.x; e < nr * P1s; e += NTH) buf[e] = cpx<T>();
     __syncthreads();
     const int kb = o.rowptr[s * o.n0 + i0], ke = o.rowptr[s * o.n0 + i0 + nr];
-    const cpx<T>* xin = a.x + o.lo;
+    const cpx<T>* xin = a.rhs.x[p] + o.lo;
     for (int k = kb + threadIdx.x; k < ke; k += NTH) {
         const int ij = o.rowcol[k];
         buf[((ij >> 16) - i0) * P1s + (ij & 0xffff)] = xin[k];
     }
     __syncthreads();
     fft_lds<T, false, NTH, MAXPTS>(buf, o.P1, o.log1, nr, P1s, o.tw1);
-    cpx<T>* dst = o.T1 + (size_t)s * o.P1 * o.n0 + i0;
+    cpx<T>* dst = o.T1 + p * a.t_stride + (size_t)s * o.P1 * o.n0 + i0;
     for (int e = threadIdx.x; e < nr * o.P1; e += NTH) {
         const int r = e % nr, j = e / nr;
         dst[(size_t)j * o.n0 + r] = buf[r * P1s + j];
@@ -61,9 +62,11 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     if (j >= a.nblk_cols[0]) { j -= a.nblk_cols[0]; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
     const int S = a.S, P0 = o.P0, n0 = o.n0;
+    const cpx<T>* T1 = o.T1 + blockIdx.y * a.t_stride;
+    cpx<T>* T2 = o.T2 + blockIdx.y * a.t_stride;
     for (int e = threadIdx.x; e < S * P0; e += NTH) {
         const int s = e / P0, i = e - s * P0;
-        buf[e] = i < n0 ? o.T1[((size_t)s * o.P1 + j) * n0 + i] : cpx<T>();
+        buf[e] = i < n0 ? T1[((size_t)s * o.P1 + j) * n0 + i] : cpx<T>();
     }
     __syncthreads();
     fft_lds<T, false, NTH, MAXPTS>(buf, P0, o.log0, S, P0, o.tw0);
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     fft_lds<T, true, NTH, MAXPTS>(buf, P0, o.log0, S, P0, o.tw0);
     for (int e = threadIdx.x; e < S * n0; e += NTH) {
         const int t = e / n0, i = e - t * n0;
-        o.T2[((size_t)t * o.P1 + j) * n0 + i] = buf[t * P0 + i];
+        T2[((size_t)t * o.P1 + j) * n0 + i] = buf[t * P0 + i];
     }
 }
 
@@ -112,7 +115,8 @@ __global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
     const int t = blk / ntiles, i0 = (blk - t * ntiles) * o.TR;
     const int nr = min(o.TR, o.n0 - i0);
     const int P1s = o.P1 + 1;
-    const cpx<T>* src = o.T2 + (size_t)t * o.P1 * o.n0 + i0;
+    const int p = blockIdx.y;
+    const cpx<T>* src = o.T2 + p * a.t_stride + (size_t)t * o.P1 * o.n0 + i0;
     for (int e = threadIdx.x; e < nr * o.P1; e += NTH) {
         const int r = e % nr, j = e / nr;
         buf[r * P1s + j] = src[(size_t)j * o.n0 + r];
@@ -124,43 +128,49 @@ __global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
         const int ij = o.rowcol[k];
         const cpx<T> val = buf[((ij >> 16) - i0) * P1s + (ij & 0xffff)];
         const int b = o.lo + k;
+        cpx<T>* y = a.rhs.y[p];
         if (a.mode == 0) {
-            a.y[b] = val;
+            y[b] = val;
         } else {
-            const cpx<T> xb = a.x[b];
-            const cpx<T> R = a.r_sheet[t];
+            const cpx<T>* x = a.rhs.x[p];
+            const cpx<T> xb = x[b];
+            const cpx<T> R = a.rhs.r_sheet[p][t];
+            const T w = a.rhs.w[p];
             cpx<T> v = R * xb;
-            v.re -= a.w * val.im;
-            v.im += a.w * val.re;
+            v.re -= w * val.im;
+            v.im += w * val.re;
             const int ka = a.a_keep[b], kk = a.b_keep[b];
-            if (ka >= 0) v = v - a.x[a.nb + ka];
-            if (kk >= 0) v = v + a.x[a.nb + kk];
-            a.y[b] = v;
+            if (ka >= 0) v = v - x[a.nb + ka];
+            if (kk >= 0) v = v + x[a.nb + kk];
+            y[b] = v;
         }
     }
 }
 
-// Branch rows (z I - Ar V) and node rows (Ar^T Ib) of the system product.
+// Branch rows (z I - Ar V) and node rows (Ar^T Ib) of the system product, one right-hand side per blockIdx.y.
 template <class T>
 __global__ __launch_bounds__(256) void tail_bottom(TailArgs<T> a) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    const cpx<T>* x = a.rhs.x[p];
+    cpx<T>* y = a.rhs.y[p];
     if (idx < a.nbr) {
         const int b = a.ns + idx;
-        cpx<T> v = a.z_tail[idx] * a.x[b];
+        cpx<T> v = a.rhs.z_tail[p][idx] * x[b];
         const int ka = a.a_keep[b], kk = a.b_keep[b];
-        if (ka >= 0) v = v - a.x[a.nb + ka];
-        if (kk >= 0) v = v + a.x[a.nb + kk];
-        a.y[b] = v;
+        if (ka >= 0) v = v - x[a.nb + ka];
+        if (kk >= 0) v = v + x[a.nb + kk];
+        y[b] = v;
     } else if (idx < a.nbr + a.nk) {
         const int n = idx - a.nbr;
         cpx<T> acc;
         for (int e = a.adj_ptr[n]; e < a.adj_ptr[n + 1]; ++e) {
             const int code = a.adj[e];
-            const cpx<T> v = a.x[code >> 1];
+            const cpx<T> v = x[code >> 1];
             if (code & 1) acc = acc - v;
             else acc = acc + v;
         }
-        a.y[a.nb + n] = acc;
+        y[a.nb + n] = acc;
     }
 }
 
@@ -171,11 +181,13 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int nrow = a.nblk_rows[0] + a.nblk_rows[1];
     const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
     constexpr int NTH = 256;
-    hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow), dim3(NTH), sh.lds_rows, st, a);
+    const int nr = a.rhs.nr;
+    hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_fwd");
-    hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol), dim3(NTH), sh.lds_cols, st, a);
+    hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols, st,
+                       a);
     GT_LAUNCH("lp_cols_mac");
-    hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow), dim3(NTH), sh.lds_rows, st, a);
+    hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_inv");
 }
 
@@ -183,7 +195,7 @@ template <class T>
 void launch_tail_bottom(const TailArgs<T>& a, hipStream_t st) {
     const int n = a.nbr + a.nk;
     if (n == 0) return;
-    hipLaunchKernelGGL((tail_bottom<T>), dim3(blocks_for(n, 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((tail_bottom<T>), dim3(blocks_for(n, 256), a.rhs.nr), dim3(256), 0, st, a);
     GT_LAUNCH("tail_bottom");
 }
 

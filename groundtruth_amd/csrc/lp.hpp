@@ -1,4 +1,4 @@
-// Launch arguments of the L_p product kernels (lp.hip).
+// Launch arguments of the L_p product kernels (lp.hip), for a batch of right-hand sides.
 #pragma once
 
 #include "gt_common.hpp"
@@ -6,6 +6,7 @@
 namespace gt {
 
 constexpr int LP_MAX_SHEETS = 8;
+constexpr int KMAX = 16;              // right-hand sides one batched launch takes
 
 // Points one workgroup's LDS holds in the row passes (a tile of rows) and in the column pass (every sheet's
 // column): 64 KiB of complex values.
@@ -25,10 +26,21 @@ struct OrientArgs {
     const int* rowptr;        // S * n0 + 1: first strip of each (sheet, row), relative to lo
     const int* rowcol;        // cnt: (i << 16) | j of each strip
     const T* K;               // kernel quadrant [qj][qi][t][s], scaled by 1 / (P0 P1)
-    cpx<T>* T1;               // [s][j][i] after the row FFT (S * P1 * n0)
+    cpx<T>* T1;               // [s][j][i] after the row FFT (S * P1 * n0), right-hand side 0
     cpx<T>* T2;               // [t][j][i] after the column pass
     const cpx<T>* tw0;        // twiddles of P0
     const cpx<T>* tw1;        // twiddles of P1
+};
+
+// The right-hand sides of one batched product: input and output vectors, and each one's operator data.
+template <class T>
+struct RhsSet {
+    int nr;
+    const cpx<T>* x[KMAX];
+    cpx<T>* y[KMAX];
+    T w[KMAX];                        // angular frequency
+    const cpx<T>* r_sheet[KMAX];      // S strip impedances
+    const cpx<T>* z_tail[KMAX];       // branch impedances
 };
 
 template <class T>
@@ -39,24 +51,20 @@ struct LpArgs {
     int S;
     int mode;                 // 0: y = L_p x (strip vectors); 1: the system product's strip rows
     int nb;                   // offset of the node potentials in x (mode 1)
-    const cpx<T>* x;
-    cpx<T>* y;
+    size_t t_stride;          // elements of T1 / T2 per right-hand side
     const int* a_keep;
     const int* b_keep;
-    const cpx<T>* r_sheet;    // S strip impedances (mode 1)
-    T w;                      // angular frequency (mode 1)
+    RhsSet<T> rhs;
 };
 
 template <class T>
 struct TailArgs {
     int ns, nb, nbr, nk;
-    const cpx<T>* x;
-    cpx<T>* y;
-    const cpx<T>* z_tail;
     const int* a_keep;
     const int* b_keep;
     const int* adj_ptr;       // nk + 1
     const int* adj;           // (branch << 1) | (1 if the node is the branch's b end)
+    RhsSet<T> rhs;
 };
 
 struct LpShape {

@@ -1,7 +1,9 @@
-// The operator's device work: the system product K x (peec.py:3043-3062) and the preconditioner M^-1 r
-// (peec.py:3064-3083): y = S^-1 (r2 - Ar^T D^-1 r1), x1 = D^-1 (r1 + Ar y), with
-// S^-1 = (1/d_s) (B + U C U^T)^-1 applied as B^-1 (device triangular solves) plus the Woodbury correction
-// z - W Q (U^T z) (W = B^-1 U resident, Q the frequency's k x k matrix).
+// The operator's device work for a batch of right-hand sides, each with its own frequency (gt_op) on one stack:
+// the system product K x (peec.py:3043-3062) and the preconditioner M^-1 r (peec.py:3064-3083):
+// y = S^-1 (r2 - Ar^T D^-1 r1), x1 = D^-1 (r1 + Ar y), with S^-1 = (1/d_s) (B + U C U^T)^-1 applied as B^-1 (the
+// device triangular solves, all right-hand sides in one pass over the panels) plus the Woodbury correction
+// z - W Q (U^T z) (W = B^-1 U resident, Q each frequency's k x k matrix). Loops over the batch sit inside the
+// kernels, so the incidence and W are read once per batch.
 #include "stack.hpp"
 
 namespace gt {
@@ -9,17 +11,17 @@ namespace gt {
 namespace {
 
 struct PreArgs {
-    int ns, nb, nk, k;
-    const zd* r;              // n: [r1 ; r2]
-    zd* out;                  // n: [x1 ; y]
-    zd* s;                    // nk: Schur right-hand side
-    const zd* z;              // nk: B^-1 s
-    zd* t;                    // k: U^T z, then Q U^T z
-    const zd* dinv_sheet;
-    const zd* dinv_tail;
-    const zd* q;              // k x k row-major
+    int ns, nb, nk, k, nr;
+    const zd* r[KMAX];        // n: [r1 ; r2]
+    zd* out[KMAX];            // n: [x1 ; y]
+    const zd* dinv_sheet[KMAX];
+    const zd* dinv_tail[KMAX];
+    const zd* q[KMAX];        // k x k row-major
+    zd s_scale[KMAX];
+    zd* s;                    // nk x nr interleaved: Schur right-hand sides
+    const zd* z;              // nk x nr interleaved: B^-1 s
+    zd* t;                    // nr x k: Q (U^T z)
     const double* W;          // k columns of nk
-    zd s_scale;
     const int* a_keep;
     const int* b_keep;
     const int* adj_ptr;
@@ -33,97 +35,153 @@ struct PreArgs {
 __global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs a) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= a.nk) return;
-    zd acc = a.r[a.nb + n];
-    const zd ds = a.dinv_sheet[a.node_sheet[n]];
-    for (int e = a.adj_ptr[n]; e < a.adj_ptr[n + 1]; ++e) {
-        const int code = a.adj[e], b = code >> 1;
-        const zd v = (b < a.ns ? ds : a.dinv_tail[b - a.ns]) * a.r[b];
-        if (code & 1) acc = acc + v;
-        else acc = acc - v;
+    const int e0 = a.adj_ptr[n], e1 = a.adj_ptr[n + 1];
+    const int sh = a.node_sheet[n];
+    for (int p = 0; p < a.nr; ++p) {
+        const zd* r = a.r[p];
+        zd acc = r[a.nb + n];
+        const zd ds = a.dinv_sheet[p][sh];
+        for (int e = e0; e < e1; ++e) {
+            const int code = a.adj[e], b = code >> 1;
+            const zd v = (b < a.ns ? ds : a.dinv_tail[p][b - a.ns]) * r[b];
+            if (code & 1) acc = acc + v;
+            else acc = acc - v;
+        }
+        a.s[(size_t)n * a.nr + p] = acc;
     }
-    a.s[n] = acc;
 }
 
-// t = Q (U^T z): one workgroup (k is the number of branches, small)
+// t_p = Q_p (U^T z_p): one workgroup per right-hand side (k, the branch count, is small)
 __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs a) {
     __shared__ double ut_raw[512];
     zd* ut = reinterpret_cast<zd*>(ut_raw);
+    const int p = blockIdx.x;
     for (int m = threadIdx.x; m < a.k; m += 256) {
         const int b = a.ns + m;
         const int ka = a.a_keep[b], kb = a.b_keep[b];
         zd v;
-        if (ka >= 0) v = v + a.z[ka];
-        if (kb >= 0) v = v - a.z[kb];
+        if (ka >= 0) v = v + a.z[(size_t)ka * a.nr + p];
+        if (kb >= 0) v = v - a.z[(size_t)kb * a.nr + p];
         ut[m] = v;
     }
     __syncthreads();
+    const zd* q = a.q[p];
     for (int i = threadIdx.x; i < a.k; i += 256) {
         zd acc;
-        for (int m = 0; m < a.k; ++m) acc = acc + a.q[i * a.k + m] * ut[m];
-        a.t[i] = acc;
+        for (int m = 0; m < a.k; ++m) acc = acc + q[i * a.k + m] * ut[m];
+        a.t[p * a.k + i] = acc;
     }
 }
 
-// y = s_scale (z - W t), written to the node part of the output
+// y_p = s_scale_p (z_p - W t_p), written to the node part of each output; W is read once for the batch
 __global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs a) {
-    __shared__ double ts_raw[512];
+    __shared__ double ts_raw[2 * 256];
     zd* ts = reinterpret_cast<zd*>(ts_raw);
-    for (int m = threadIdx.x; m < a.k; m += 256) ts[m] = a.t[m];
-    __syncthreads();
     const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= a.nk) return;
-    zd v = a.z[n];
-    for (int m = 0; m < a.k; ++m) {
-        const double w = a.W[(size_t)m * a.nk + n];
-        v.re -= w * ts[m].re;
-        v.im -= w * ts[m].im;
+    for (int p = 0; p < a.nr; ++p) {
+        __syncthreads();
+        for (int m = threadIdx.x; m < a.k; m += 256) ts[m] = a.t[p * a.k + m];
+        __syncthreads();
+        if (n < a.nk) {
+            zd v = a.z[(size_t)n * a.nr + p];
+            for (int m = 0; m < a.k; ++m) {
+                const double w = a.W[(size_t)m * a.nk + n];
+                v.re -= w * ts[m].re;
+                v.im -= w * ts[m].im;
+            }
+            a.out[p][a.nb + n] = a.s_scale[p] * v;
+        }
     }
-    a.out[a.nb + n] = a.s_scale * v;
 }
 
 // x1 = D^-1 (r1 + Ar y)
 __global__ __launch_bounds__(256) void pre_branches(PreArgs a) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= a.nb) return;
-    zd v = a.r[b];
     const int ka = a.a_keep[b], kb = a.b_keep[b];
-    if (ka >= 0) v = v + a.out[a.nb + ka];
-    if (kb >= 0) v = v - a.out[a.nb + kb];
-    const zd d = b < a.ns ? a.dinv_sheet[a.strip_sheet[b]] : a.dinv_tail[b - a.ns];
-    a.out[b] = d * v;
+    const int sh = b < a.ns ? a.strip_sheet[b] : -1;
+    for (int p = 0; p < a.nr; ++p) {
+        zd v = a.r[p][b];
+        const zd* y = a.out[p] + a.nb;
+        if (ka >= 0) v = v + y[ka];
+        if (kb >= 0) v = v - y[kb];
+        const zd d = sh >= 0 ? a.dinv_sheet[p][sh] : a.dinv_tail[p][b - a.ns];
+        a.out[p][b] = d * v;
+    }
 }
 
 }  // namespace
 
-void op_matvec(gt_op* op, const zd* x, zd* y) {
-    gt_stack* st = op->st;
+void Work::ensure(const gt_stack* st, int nr) {
+    if (nr <= cap) return;
+    T1.alloc(std::max<size_t>(1, st->t_elems) * nr);
+    T2.alloc(std::max<size_t>(1, st->t_elems) * nr);
+    s.alloc((size_t)std::max(1, st->nk) * nr);
+    z.alloc((size_t)std::max(1, st->nk) * nr);
+    t.alloc((size_t)std::max(1, st->nbr) * nr);
+    cap = nr;
+}
+
+void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const zd* const* x, zd* const* y) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     hipStream_t s = st->ctx->stream;
-    LpArgs<double> la = st->lp_args(x, y, 1);
-    la.r_sheet = op->r_sheet.p;
-    la.w = op->w;
+    st->work.ensure(st, nr);
+    RhsSet<double> rs{};
+    rs.nr = nr;
+    for (int p = 0; p < nr; ++p) {
+        rs.x[p] = x[p];
+        rs.y[p] = y[p];
+        rs.w[p] = ops[p]->w;
+        rs.r_sheet[p] = ops[p]->r_sheet.p;
+        rs.z_tail[p] = ops[p]->z_tail.p;
+    }
+    LpArgs<double> la = st->lp_args(1);
+    la.rhs = rs;
     if (st->ns > 0) launch_lp(la, st->shape, s);
-    TailArgs<double> ta{st->ns, st->nb, st->nbr, st->nk, x, y, op->z_tail.p, st->a_keep.p, st->b_keep.p,
-                        st->adj_ptr.p, st->adj.p};
+    TailArgs<double> ta{st->ns, st->nb, st->nbr, st->nk, st->a_keep.p, st->b_keep.p, st->adj_ptr.p, st->adj.p, rs};
     launch_tail_bottom(ta, s);
 }
 
-void schur_apply(gt_stack* st, const zd* b, zd* x) {
+void schur_apply(gt_stack* st, const zd* b, zd* x, int nr) {
     if (!st->schur) fail(GT_ERR_ARG, "the stack has no Schur factor (gt_schur_factor first)");
-    schur_solve_device(*st->schur, b, x, st->ctx->stream);
+    schur_solve_device(*st->schur, b, x, nr, st->ctx->stream);
 }
 
-void op_precond(gt_op* op, const zd* r, zd* z) {
-    gt_stack* st = op->st;
+void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const zd* const* r, zd* const* z) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     hipStream_t s = st->ctx->stream;
-    PreArgs a{st->ns, st->nb, st->nk, op->k, r, z, st->work.s.p, st->work.z.p, st->work.t.p, op->dinv_sheet.p,
-              op->dinv_tail.p, op->q.p, st->W.p, op->s_scale, st->a_keep.p, st->b_keep.p, st->adj_ptr.p,
-              st->adj.p, st->node_sheet.p, st->strip_sheet.p};
+    st->work.ensure(st, nr);
+    PreArgs a{};
+    a.ns = st->ns;
+    a.nb = st->nb;
+    a.nk = st->nk;
+    a.nr = nr;
+    a.k = ops[0]->k;
+    for (int p = 0; p < nr; ++p) {
+        if (ops[p]->st != st || ops[p]->k != a.k) fail(GT_ERR_ARG, "a batch's operators must share one stack");
+        a.r[p] = r[p];
+        a.out[p] = z[p];
+        a.dinv_sheet[p] = ops[p]->dinv_sheet.p;
+        a.dinv_tail[p] = ops[p]->dinv_tail.p;
+        a.q[p] = ops[p]->q.p;
+        a.s_scale[p] = ops[p]->s_scale;
+    }
+    a.s = st->work.s.p;
+    a.z = st->work.z.p;
+    a.t = st->work.t.p;
+    a.W = st->W.p;
+    a.a_keep = st->a_keep.p;
+    a.b_keep = st->b_keep.p;
+    a.adj_ptr = st->adj_ptr.p;
+    a.adj = st->adj.p;
+    a.node_sheet = st->node_sheet.p;
+    a.strip_sheet = st->strip_sheet.p;
     if (st->nk > 0) {
         hipLaunchKernelGGL(pre_schur_rhs, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_schur_rhs");
-        schur_apply(st, st->work.s.p, st->work.z.p);
-        if (op->k > 0) {
-            hipLaunchKernelGGL(pre_woodbury_small, dim3(1), dim3(256), 0, s, a);
+        schur_apply(st, st->work.s.p, st->work.z.p, nr);
+        if (a.k > 0) {
+            hipLaunchKernelGGL(pre_woodbury_small, dim3(nr), dim3(256), 0, s, a);
             GT_LAUNCH("pre_woodbury_small");
         }
         hipLaunchKernelGGL(pre_woodbury_apply, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);

@@ -53,14 +53,15 @@ struct Schur {
     DevBuf<int> perm, c0, nc, nr, rows, fwd_s, fwd_r0, bwd_s, bwd_q0, bwd_qn;
     DevBuf<int64_t> roff, poff, uoff, fpos, cptr, cidx;
     DevBuf<double> panel;
-    DevBuf<zd> ubuf;                    // update vectors of one forward solve
+    DevBuf<zd> ubuf;                    // update vectors of one forward solve (up to 16 right-hand sides)
     DevBuf<zd> zp, xp;                  // forward and backward solutions in the new numbering
 
     void upload(hipStream_t s);
     int64_t device_bytes() const;
 };
 
-// x = B^-1 b on the device; b and x hold n complex values in the old numbering (distinct buffers)
-void schur_solve_device(Schur& S, const zd* b, zd* x, hipStream_t s);
+// x = B^-1 b on the device for nr <= 16 right-hand sides, interleaved: entry (node, p) at node * nr + p, in the old
+// numbering (b and x distinct)
+void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t s);
 
 }  // namespace gt

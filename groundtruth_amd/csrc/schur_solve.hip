@@ -6,6 +6,7 @@
 //   backward (roots first):  x_C = Q^T [z_C ; x_R]                     -- a dense product over its columns
 // One launch per level; a large supernode is split over many workgroups (blocks of 64 rows forward, 16 columns
 // backward), so the wide top separators run on many CUs. The real panel multiplies complex vectors.
+#include "lp.hpp"
 #include "schur.hpp"
 
 namespace gt {
@@ -13,7 +14,6 @@ namespace gt {
 namespace {
 
 constexpr int NTH = 256;
-constexpr int CH = 1024;          // vector entries staged in LDS at a time
 
 struct SolveArgs {
     const int* perm;
@@ -38,18 +38,25 @@ struct SolveArgs {
     zd* x;
 };
 
-// f at front position q of supernode s: its right-hand side (a column) plus the children's update entries
-__device__ __forceinline__ zd front(const SolveArgs& a, int s, int q, int C, int c0) {
-    zd acc = q < C ? a.b[a.perm[c0 + q]] : zd();
+// Entries staged in LDS per pass for a batch of up to KB right-hand sides (16 KiB).
+template <int KB>
+constexpr int chunk_rows() { return KB == 1 ? 1024 : (KB <= 4 ? 256 : 64); }
+
+// f at front position q of supernode s for right-hand side p of nr (interleaved vectors): its right-hand side
+// (a column) plus the children's update entries, in the contributors' fixed order
+__device__ __forceinline__ zd front(const SolveArgs& a, int s, int q, int C, int c0, int p, int nr) {
+    zd acc = q < C ? a.b[(size_t)a.perm[c0 + q] * nr + p] : zd();
     const int64_t pos = a.fpos[s] + q;
-    for (int64_t e = a.cptr[pos]; e < a.cptr[pos + 1]; ++e) acc = acc + a.ubuf[a.cidx[e]];
+    for (int64_t e = a.cptr[pos]; e < a.cptr[pos + 1]; ++e) acc = acc + a.ubuf[(size_t)a.cidx[e] * nr + p];
     return acc;
 }
 
-// One work item: rows [r0, r0 + 64) of supernode s. Lane = row, the four waves split the columns q (stride 4),
-// their partial sums meet in LDS.
-__global__ __launch_bounds__(NTH) void schur_fwd(SolveArgs a, int item0) {
-    __shared__ double f_raw[2 * CH];
+// One work item: rows [r0, r0 + 64) of supernode s, for nr <= KB right-hand sides. Lane = row; the four waves
+// split the columns q (stride 4); their partial sums meet in LDS.
+template <int KB>
+__global__ __launch_bounds__(NTH) void schur_fwd(SolveArgs a, int item0, int nr) {
+    constexpr int CB = chunk_rows<KB>();
+    __shared__ double f_raw[2 * CB * KB];
     __shared__ double part_raw[2 * NTH];
     zd* f = reinterpret_cast<zd*>(f_raw);
     zd* part = reinterpret_cast<zd*>(part_raw);
@@ -59,51 +66,69 @@ __global__ __launch_bounds__(NTH) void schur_fwd(SolveArgs a, int item0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = r0 + lane;
     const int qlim = r < C ? r + 1 : C;
-    zd acc;
-    for (int cs = 0; cs < C; cs += CH) {
-        const int ce = min(C, cs + CH);
-        for (int q = cs + threadIdx.x; q < ce; q += NTH) f[q - cs] = front(a, s, q, C, c0);
+    zd acc[KB];
+    for (int cs = 0; cs < C; cs += CB) {
+        const int ce = min(C, cs + CB);
+        for (int e = threadIdx.x; e < (ce - cs) * nr; e += NTH) {
+            const int q = cs + e / nr, p = e - (q - cs) * nr;
+            f[(q - cs) * KB + p] = front(a, s, q, C, c0, p, nr);
+        }
         __syncthreads();
         if (r < m) {
             const int qe = min(ce, qlim);
-            zd a1, a2, a3;
             int q = cs + wave;
-            for (; q + 12 < qe; q += 16) {          // four independent loads in flight per lane
+            for (; q + 4 < qe; q += 8) {            // two independent panel loads in flight per lane
                 const double w0 = Q[(size_t)q * m + r], w1 = Q[(size_t)(q + 4) * m + r];
-                const double w2 = Q[(size_t)(q + 8) * m + r], w3 = Q[(size_t)(q + 12) * m + r];
-                const zd v0 = f[q - cs], v1 = f[q + 4 - cs], v2 = f[q + 8 - cs], v3 = f[q + 12 - cs];
-                acc.re += w0 * v0.re; acc.im += w0 * v0.im;
-                a1.re += w1 * v1.re; a1.im += w1 * v1.im;
-                a2.re += w2 * v2.re; a2.im += w2 * v2.im;
-                a3.re += w3 * v3.re; a3.im += w3 * v3.im;
+                const zd* f0 = f + (q - cs) * KB;
+                const zd* f1 = f + (q + 4 - cs) * KB;
+#pragma unroll
+                for (int p = 0; p < KB; ++p) {
+                    if (p < nr) {
+                        acc[p].re += w0 * f0[p].re + w1 * f1[p].re;
+                        acc[p].im += w0 * f0[p].im + w1 * f1[p].im;
+                    }
+                }
             }
             for (; q < qe; q += 4) {
-                const double w = Q[(size_t)q * m + r];
-                const zd v = f[q - cs];
-                acc.re += w * v.re;
-                acc.im += w * v.im;
+                const double w0 = Q[(size_t)q * m + r];
+                const zd* f0 = f + (q - cs) * KB;
+#pragma unroll
+                for (int p = 0; p < KB; ++p) {
+                    if (p < nr) {
+                        acc[p].re += w0 * f0[p].re;
+                        acc[p].im += w0 * f0[p].im;
+                    }
+                }
             }
-            acc = acc + a1 + a2 + a3;
         }
         __syncthreads();
     }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (wave == 0 && r < m) {
-        zd t = part[lane] + part[64 + lane] + part[128 + lane] + part[192 + lane];
-        if (r < C) {
-            a.zp[c0 + r] = t;
-        } else {
-            t = t + front(a, s, r, C, c0);
-            a.ubuf[a.uoff[s] + r - C] = t;
+#pragma unroll
+    for (int p = 0; p < KB; ++p) {
+        if (p < nr) {
+            part[threadIdx.x] = acc[p];
+            __syncthreads();
+            if (wave == 0 && r < m) {
+                zd t = part[lane] + part[64 + lane] + part[128 + lane] + part[192 + lane];
+                if (r < C) {
+                    a.zp[(size_t)(c0 + r) * nr + p] = t;
+                } else {
+                    t = t + front(a, s, r, C, c0, p, nr);
+                    a.ubuf[(size_t)(a.uoff[s] + r - C) * nr + p] = t;
+                }
+            }
+            __syncthreads();
         }
     }
 }
 
-// One work item: columns [q0, q0 + qn) of supernode s (qn <= 64); wave w takes columns q0 + w, q0 + w + 4, ...,
-// its lanes stride the rows.
-__global__ __launch_bounds__(NTH) void schur_bwd(SolveArgs a, int item0) {
-    __shared__ double v_raw[2 * CH];
+// One work item: columns [q0, q0 + qn) of supernode s, taken 4 K at a time: wave w holds columns qc + w + 4k,
+// its lanes stride the rows; every panel load feeds all nr <= KB right-hand sides.
+template <int KB>
+__global__ __launch_bounds__(NTH) void schur_bwd(SolveArgs a, int item0, int nr) {
+    constexpr int CB = chunk_rows<KB>();
+    constexpr int K = KB == 1 ? 16 : (KB <= 4 ? 4 : 1);
+    __shared__ double v_raw[2 * CB * KB];
     zd* v = reinterpret_cast<zd*>(v_raw);
     const int s = a.item_s[item0 + blockIdx.x], q0 = a.item_0[item0 + blockIdx.x];
     const int C = a.nc[s], m = C + a.nr[s], c0 = a.c0[s];
@@ -111,49 +136,76 @@ __global__ __launch_bounds__(NTH) void schur_bwd(SolveArgs a, int item0) {
     const double* Q = a.panel + a.poff[s];
     const int* rows = a.rows + a.roff[s];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int K = 16;
-    zd acc[K];
-    for (int rs = q0; rs < m; rs += CH) {           // rows below the block's first column only
-        const int re = min(m, rs + CH);
-        for (int r = rs + threadIdx.x; r < re; r += NTH) v[r - rs] = r < C ? a.zp[c0 + r] : a.xp[rows[r - C]];
-        __syncthreads();
+    for (int qc = q0; qc < qend; qc += 4 * K) {
+        zd acc[K][KB];
+        for (int rs = qc; rs < m; rs += CB) {       // rows from the chunk's first column down
+            const int re = min(m, rs + CB);
+            for (int e = threadIdx.x; e < (re - rs) * nr; e += NTH) {
+                const int r = rs + e / nr, p = e - (r - rs) * nr;
+                v[(r - rs) * KB + p] = r < C ? a.zp[(size_t)(c0 + r) * nr + p] : a.xp[(size_t)rows[r - C] * nr + p];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int q = qc + wave + 4 * k;
+                if (q < qend) {
+                    const double* Qq = Q + (size_t)q * m;
+                    for (int r = max(q, rs) + lane; r < re; r += 64) {
+                        const double w0 = Qq[r];
+                        const zd* u = v + (r - rs) * KB;
+#pragma unroll
+                        for (int p = 0; p < KB; ++p) {
+                            if (p < nr) {
+                                acc[k][p].re += w0 * u[p].re;
+                                acc[k][p].im += w0 * u[p].im;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int q = q0 + wave + 4 * k;
-            if (q < qend) {
-                const double* Qq = Q + (size_t)q * m;
-                int r = max(q, rs) + lane;
-                zd a1;
-                for (; r + 64 < re; r += 128) {
-                    const double w0 = Qq[r], w1 = Qq[r + 64];
-                    const zd u0 = v[r - rs], u1 = v[r + 64 - rs];
-                    acc[k].re += w0 * u0.re; acc[k].im += w0 * u0.im;
-                    a1.re += w1 * u1.re; a1.im += w1 * u1.im;
-                }
-                if (r < re) {
-                    const double w0 = Qq[r];
-                    const zd u0 = v[r - rs];
-                    acc[k].re += w0 * u0.re; acc[k].im += w0 * u0.im;
-                }
-                acc[k] = acc[k] + a1;
-            }
-        }
-        __syncthreads();
-    }
+            const int q = qc + wave + 4 * k;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int q = q0 + wave + 4 * k;
-        if (q < qend) {
-            double re = acc[k].re, im = acc[k].im;
-            for (int off = 32; off > 0; off >>= 1) {
-                re += __shfl_down(re, off, 64);
-                im += __shfl_down(im, off, 64);
-            }
-            if (lane == 0) {
-                a.xp[c0 + q] = zd(re, im);
-                a.x[a.perm[c0 + q]] = zd(re, im);
+            for (int p = 0; p < KB; ++p) {
+                if (p < nr) {
+                    double re = acc[k][p].re, im = acc[k][p].im;
+                    for (int off = 32; off > 0; off >>= 1) {
+                        re += __shfl_down(re, off, 64);
+                        im += __shfl_down(im, off, 64);
+                    }
+                    if (lane == 0 && q < qend) {
+                        a.xp[(size_t)(c0 + q) * nr + p] = zd(re, im);
+                        a.x[(size_t)a.perm[c0 + q] * nr + p] = zd(re, im);
+                    }
+                }
             }
         }
+    }
+}
+
+template <int KB>
+void run_levels(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
+    const std::vector<int>& fp = S.h.fwd_ptr;
+    a.item_s = S.fwd_s.p;
+    a.item_0 = S.fwd_r0.p;
+    for (size_t l = 0; l + 1 < fp.size(); ++l) {
+        const int cnt = fp[l + 1] - fp[l];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(schur_fwd<KB>, dim3(cnt), dim3(NTH), 0, st, a, fp[l], nr);
+        GT_LAUNCH("schur_fwd");
+    }
+    const std::vector<int>& bp = S.h.bwd_ptr;
+    a.item_s = S.bwd_s.p;
+    a.item_0 = S.bwd_q0.p;
+    a.item_n = S.bwd_qn.p;
+    for (size_t l = 0; l + 1 < bp.size(); ++l) {
+        const int cnt = bp[l + 1] - bp[l];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(schur_bwd<KB>, dim3(cnt), dim3(NTH), 0, st, a, bp[l], nr);
+        GT_LAUNCH("schur_bwd");
     }
 }
 
@@ -179,9 +231,9 @@ void Schur::upload(hipStream_t st) {
     cptr.upload(nz64(h.cptr), st);
     cidx.upload(nz64(h.cidx), st);
     panel.upload(h.panel.empty() ? std::vector<double>{0.0} : h.panel, st);
-    ubuf.alloc(h.upd_total > 0 ? h.upd_total : 1);
-    zp.alloc(h.n > 0 ? h.n : 1);
-    xp.alloc(h.n > 0 ? h.n : 1);
+    ubuf.alloc((size_t)(h.upd_total > 0 ? h.upd_total : 1) * KMAX);
+    zp.alloc((size_t)(h.n > 0 ? h.n : 1) * KMAX);
+    xp.alloc((size_t)(h.n > 0 ? h.n : 1) * KMAX);
     GT_HIP(hipStreamSynchronize(st));
 }
 
@@ -192,28 +244,13 @@ int64_t Schur::device_bytes() const {
                      xp.bytes());
 }
 
-void schur_solve_device(Schur& S, const zd* b, zd* x, hipStream_t st) {
+void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
     SolveArgs a{S.perm.p, S.c0.p, S.nc.p, S.nr.p, S.rows.p, nullptr, nullptr, nullptr, S.roff.p, S.poff.p, S.uoff.p,
                 S.fpos.p, S.cptr.p, S.cidx.p, S.panel.p, S.ubuf.p, S.zp.p, S.xp.p, b, x};
-    const std::vector<int>& fp = S.h.fwd_ptr;
-    a.item_s = S.fwd_s.p;
-    a.item_0 = S.fwd_r0.p;
-    for (size_t l = 0; l + 1 < fp.size(); ++l) {
-        const int cnt = fp[l + 1] - fp[l];
-        if (cnt == 0) continue;
-        hipLaunchKernelGGL(schur_fwd, dim3(cnt), dim3(NTH), 0, st, a, fp[l]);
-        GT_LAUNCH("schur_fwd");
-    }
-    const std::vector<int>& bp = S.h.bwd_ptr;
-    a.item_s = S.bwd_s.p;
-    a.item_0 = S.bwd_q0.p;
-    a.item_n = S.bwd_qn.p;
-    for (size_t l = 0; l + 1 < bp.size(); ++l) {
-        const int cnt = bp[l + 1] - bp[l];
-        if (cnt == 0) continue;
-        hipLaunchKernelGGL(schur_bwd, dim3(cnt), dim3(NTH), 0, st, a, bp[l]);
-        GT_LAUNCH("schur_bwd");
-    }
+    if (nr == 1) run_levels<1>(S, a, nr, st);
+    else if (nr <= 4) run_levels<4>(S, a, nr, st);
+    else run_levels<16>(S, a, nr, st);
 }
 
 }  // namespace gt

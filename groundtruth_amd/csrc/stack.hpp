@@ -28,11 +28,13 @@ struct OrientHost {
     DevBuf<float> Kf;
 };
 
-// Per-vector scratch of one matvec / preconditioner application.
+// Scratch of one batched product / preconditioner application (grown to the largest batch used).
 struct Work {
-    DevBuf<zd> T1, T2;        // L_p intermediates
-    DevBuf<zd> s, z;          // Schur right-hand side and solution (n_keep)
-    DevBuf<zd> t;             // Woodbury: U^T z, then Q U^T z (k)
+    int cap = 0;              // right-hand sides the buffers hold
+    DevBuf<zd> T1, T2;        // L_p intermediates, t_elems per right-hand side
+    DevBuf<zd> s, z;          // Schur right-hand sides and solutions, interleaved (n_keep x batch)
+    DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (k each)
+    void ensure(const gt_stack_s* st, int nr);
 };
 
 }  // namespace gt
@@ -55,7 +57,7 @@ struct gt_stack_s {
     int k = 0;
 
     int64_t device_bytes() const;
-    gt::LpArgs<double> lp_args(const gt::cpx<double>* x, gt::cpx<double>* y, int mode) const;
+    gt::LpArgs<double> lp_args(int mode) const;
 };
 
 struct gt_op_s {
@@ -67,8 +69,8 @@ struct gt_op_s {
 };
 
 namespace gt {
-// device work of the operator (ops.hip)
-void op_matvec(gt_op* op, const zd* x, zd* y);
-void op_precond(gt_op* op, const zd* r, zd* z);
-void schur_apply(gt_stack* st, const zd* b, zd* x);   // B^-1 b on the device
+// device work of a batch of operators on one stack (ops.hip): y_p = K_p x_p, z_p = M_p^-1 r_p for p < nr <= 16
+void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const zd* const* x, zd* const* y);
+void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const zd* const* r, zd* const* z);
+void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides
 }  // namespace gt

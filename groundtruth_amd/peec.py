@@ -35,6 +35,7 @@ GMRES_EXTRA_CYCLES = 24         # peec.py:98
 GMRES_CHECK_MISSES = 4          # peec.py:100
 SCREEN_MIN_NODES = 16           # peec.py:112
 OPERATORS_KEPT = 32             # per-frequency operators kept per stack (small device arrays each)
+BATCH = 16                      # right-hand sides per batched device launch
 
 _KERNELS: dict = {}
 
@@ -327,6 +328,51 @@ class PlaneStack:
         _lib.check(_lib.lib().gt_gmres(op.handle, _lib.ptr(bb), _lib.ptr(out), C.byref(p), C.byref(res)), "gt_gmres")
         return out.view(np.complex128).copy(), int(res.iterations), bool(res.converged), res
 
+    def gmres_batch_dev(self, ops: list, b_devs: list, x_devs: list, tol: float, max_cycles: int = 8,
+                        restart: Optional[int] = None, extra_cycles: int = GMRES_EXTRA_CYCLES,
+                        bnorm: Optional[float] = None) -> list:
+        """Batched device GMRES: x_i = M_i^-1 y_i with (K_i M_i^-1) y_i = b_i for every operator i of this stack, on
+        device-resident vectors (torch complex128 tensors, their work finished), at most 16 per launch group. Each
+        follows the reference's GMRES; the batch shares every launch. Returns one GmresResult per solve."""
+        out = []
+        for g0 in range(0, len(ops), BATCH):
+            grp = slice(g0, g0 + BATCH)
+            o, b, x = ops[grp], b_devs[grp], x_devs[grp]
+            nr = len(o)
+            H = (C.c_void_p * nr)(*[op.handle for op in o])
+            B = (C.c_void_p * nr)(*[t.data_ptr() for t in b])
+            X = (C.c_void_p * nr)(*[t.data_ptr() for t in x])
+            p = _lib.GmresParams(float(tol), float(bnorm or 0.0), int(restart or 0), int(max_cycles),
+                                 int(extra_cycles), 0, GMRES_CHECK_MISSES)
+            res = (_lib.GmresResult * nr)()
+            _lib.check(_lib.lib().gt_gmres_batch(self.handle, nr, H, B, X, C.byref(p), res), "gt_gmres_batch")
+            out += list(res)
+        return out
+
+    def solve_many(self, fs: list, rhs_tops: list, Js: list, tol: float = 1e-5, max_cycles: int = 8,
+                   restart: Optional[int] = None) -> list:
+        """PlaneStack.solve for many (f, rhs_top, J) on this stack at once (batched on the device): the list of
+        (Ib, V, iterations, x), each the same as a solve of its own. SolverError if any does not converge."""
+        import torch
+        dev = f"cuda:{self.device}"
+        ops = [self.operator(f) for f in fs]
+        rhs = [np.concatenate([np.asarray(r, complex), np.asarray(J, float)[self.keep].astype(complex)])
+               for r, J in zip(rhs_tops, Js)]
+        b = [torch.from_numpy(r).to(dev) for r in rhs]
+        x = [torch.empty_like(t) for t in b]
+        torch.cuda.synchronize(self.device)
+        res = self.gmres_batch_dev(ops, b, x, tol, max_cycles=max_cycles, restart=restart)
+        self.ctx.sync()
+        out = []
+        for f, r, xd in zip(fs, res, x):
+            if not r.converged:
+                raise SolverError(f"GMRES did not converge at {f:.4g} Hz ({r.iterations} iterations)")
+            xv = xd.cpu().numpy()
+            V = np.zeros(self.n_nodes, complex)
+            V[self.keep] = xv[self.nb:]
+            out.append((xv[:self.nb], V, int(r.iterations), xv))
+        return out
+
     def gmres_dev(self, op: Operator, b_dev, x_dev, tol: float, bnorm: Optional[float] = None, max_cycles: int = 8,
                   restart: Optional[int] = None, extra_cycles: int = GMRES_EXTRA_CYCLES):
         """The device GMRES on device-resident vectors (torch complex128 tensors of nb + n_keep values, their work
@@ -542,6 +588,14 @@ class RouteExcitation:
         Ib, V, its, x = self.stack.solve(f, rhs_top, J, tol=tol, max_cycles=max_cycles, guess=guess,
                                          restart=restart)
         return self.unpack(f, Ib, V, its, x)
+
+    def solve_many(self, fs: list, tol: float = 1e-5, max_cycles: int = 8, restart: Optional[int] = None) -> list:
+        """The loop's solutions at every frequency of `fs`, solved as one batch on the device (each the same as
+        solve(f) alone)."""
+        rhs = [self.rhs(f) for f in fs]
+        sols = self.stack.solve_many(fs, [r[0] for r in rhs], [r[1] for r in rhs], tol=tol, max_cycles=max_cycles,
+                                     restart=restart)
+        return [self.unpack(f, *s) for f, s in zip(fs, sols)]
 
     def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
         st = self.stack

@@ -164,6 +164,13 @@ int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_par
  * call returns when x is written). */
 int gt_gmres_dev(gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* params,
                  gt_gmres_result* result);
+/* A batch of nr <= 16 solves on one stack, each with its own operator (frequency) and right-hand side: the many
+ * frequencies and nets of one board (peec.py:2787 RouteExcitation.solve per (route, f); simulate.py:3964-4013 runs
+ * them as parallel threads). Each follows the reference's GMRES exactly; the batch shares every launch, so the
+ * spectra, incidence and Schur panels are read once per step. A solve's result is bit-identical alone or batched.
+ * b_dev[i], x_dev[i]: device vectors of n complex values; results[i]: that solve's outcome. */
+int gt_gmres_batch(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
+                   const gt_gmres_params* params, gt_gmres_result* results);
 
 #ifdef __cplusplus
 }

@@ -134,3 +134,35 @@ def test_an_iteration_budget_too_small_is_a_solver_error():
     ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
     with pytest.raises(peec.SolverError):
         ex.solve(1e8, tol=1e-14, max_cycles=1, restart=2)
+
+
+def test_batched_solves_are_bit_identical_to_single_solves_and_match_the_oracle():
+    """The frequencies of one board solved as one batch: every result bit-identical to its solve alone (the batch
+    shares launches, never arithmetic), and within the parity bar of the oracle."""
+    b = CASES["two_layer_20"]()
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    fs = [1e6, 3e7, 2e8, 1e9, 5e5]
+    many = ex.solve_many(fs, tol=1e-12)
+    for f, s in zip(fs, many):
+        one = ex.solve(f, tol=1e-12)
+        assert np.array_equal(s.x, one.x) and s.iterations == one.iterations
+    ox = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver)
+    for f, s in zip(fs[:3], many[:3]):
+        ref = ox.solve(f, tol=1e-12)
+        assert l2(_currents(s), _currents(ref)) < 1e-9
+        assert abs(s.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9
+
+
+def test_a_mixed_batch_of_routes_and_frequencies_gives_each_its_own_answer():
+    """Different nets at different frequencies in one batch (the serving mix): each result equals its own solve."""
+    b = CASES["four_layer_24"]()
+    st = peec.PlaneStack(b.layers, b.branches)
+    exs = [peec.RouteExcitation(st, r.trace, r.driver, r.receiver) for r in b.routes]
+    jobs = [(exs[0], 1e7), (exs[1], 4e8), (exs[0], 9e8), (exs[1], 2e6)]
+    rhs = [ex.rhs(f) for ex, f in jobs]
+    many = st.solve_many([f for _, f in jobs], [r[0] for r in rhs], [r[1] for r in rhs], tol=1e-10)
+    for (ex, f), (Ib, V, its, x) in zip(jobs, many):
+        one = ex.solve(f, tol=1e-10)
+        assert np.array_equal(x, one.x) and its == one.iterations
