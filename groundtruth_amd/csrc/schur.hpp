@@ -11,6 +11,28 @@
 
 namespace gt {
 
+// One work item of a device solve with everything it needs about its supernode (one load). An item covers one chunk
+// [k0, k1) of its tile's reduction (columns forward, rows backward); a tile whose reduction spans several chunks is a
+// group of items that write partial sums to slots slot0 + kidx, added in order by a finalize item (one per group,
+// launched after the level).
+struct alignas(16) SnItem {
+    int32_t s;          // supernode
+    int32_t t0;         // first row of the tile (forward) or first column (backward)
+    int32_t kw;         // forward: column phases per wave x 1 (64-row tile) or x 4 (16-row tile); backward: row phases
+    int32_t C;          // columns
+    int32_t m;          // rows (columns + off-diagonal rows)
+    int32_t c0;         // first column (new numbering)
+    int32_t k0, k1;     // the chunk of the reduction
+    int32_t slot0;      // first partial-sum slot of the group, -1 for a tile of one chunk
+    int32_t kidx;       // chunk index in the group
+    int32_t ks;         // chunks in the group
+    int32_t pad0;
+    int64_t poff;       // panel offset
+    int64_t aux;        // forward: update-vector offset; backward: offset of the off-diagonal row list
+    int64_t fpos;       // first front position
+    int64_t pad;
+};
+
 struct SchurHost {
     int n = 0;                          // kept nodes
     std::vector<int> perm;              // new -> old
@@ -29,10 +51,16 @@ struct SchurHost {
     std::vector<int64_t> fpos;          // offset of s's front positions (nc + nr) in the contributor CSR
     std::vector<int64_t> cptr;          // contributors of each front position: update-buffer indices, children
     std::vector<int64_t> cidx;          //   in ascending order (a fixed summation order)
+    // the same lists for the device, two per front position: {a, b} with -1 for none, or, for a position with more
+    // than two contributors, {-2 - cptr[pos], count} (the list in cidx)
+    std::vector<int32_t> ctab;
     int levels = 0;
     std::vector<int> fwd_ptr, bwd_ptr;  // work items per level (forward: leaves first; backward: roots first)
-    std::vector<int> fwd_s, fwd_r0;     // forward items: supernode, first row of its block
-    std::vector<int> bwd_s, bwd_q0, bwd_qn;   // backward items: supernode, first column, columns
+    std::vector<SnItem> fwd, bwd;       // the work items
+    std::vector<int> fwd_cols, bwd_rows;      // per level: right-hand-side rows staged (LDS sizing)
+    std::vector<int> ffin_ptr, bfin_ptr;      // per level: finalize items of its split tiles
+    std::vector<SnItem> ffin, bfin;
+    int64_t slots = 0;                  // partial-sum slots (SCHUR_TILE x 16 right-hand sides each)
     int64_t nnz_L = 0;
     int max_cols = 0;
 
@@ -44,14 +72,21 @@ struct SchurHost {
     void solve_host(const double* b, double* x) const;
 };
 
-constexpr int SCHUR_ROWS = 64;         // panel rows per forward work item
-constexpr int SCHUR_COLS = 16;         // panel columns per backward work item of a wide supernode
-constexpr int SCHUR_COLS_MAX = 64;     // a supernode of at most this many columns is one backward item
+// Work items of the device solves. Forward: a tile of 64 panel rows whose columns are split 4 ways inside each wave
+// (at most 32 columns), or, for a wider supernode, a tile of 16 rows with the columns split 16 ways over the
+// workgroup. Backward: a tile of 16 columns, each column's rows split 16 ways inside a wave, or, for a supernode of
+// 17 to 32 columns and at most 64 rows, one tile of 32 columns with the rows split 8 ways. Every lane takes at most
+// SCHUR_PV panel values per item (all loaded at once), which sets the chunk sizes.
+constexpr int SCHUR_PV = 8;
+constexpr int SCHUR_NARROW = 32;       // forward: the widest supernode of 64-row tiles
+constexpr int SCHUR_TILE = 16;         // rows (forward, wide) or columns (backward) of a split tile
 
 struct Schur {
     SchurHost h;
-    DevBuf<int> perm, c0, nc, nr, rows, fwd_s, fwd_r0, bwd_s, bwd_q0, bwd_qn;
-    DevBuf<int64_t> roff, poff, uoff, fpos, cptr, cidx;
+    DevBuf<int> perm, rows, ctab;
+    DevBuf<int64_t> cidx;
+    DevBuf<SnItem> fwd, bwd, ffin, bfin;
+    DevBuf<zd> part;                    // partial sums of split tiles
     DevBuf<double> panel;
     DevBuf<zd> ubuf;                    // update vectors of one forward solve (up to 16 right-hand sides)
     DevBuf<zd> zp, xp;                  // forward and backward solutions in the new numbering

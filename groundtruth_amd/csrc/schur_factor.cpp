@@ -5,6 +5,7 @@
 // symmetric mode). Here, with every strip alike, S = d_s (B + U C U^T) and only the real B is factorised, once per
 // stack; the frequency enters through the scalar d_s and the Woodbury matrix (the caller's part).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -312,30 +313,88 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
         dmax = std::max(dmax, dep[sn]);
     }
     levels = ns ? hmax + 1 : 0;
-    fwd_ptr.assign(1, 0);
-    fwd_s.clear();
-    fwd_r0.clear();
-    for (int l = 0; l <= hmax && ns; ++l) {
-        for (int sn = 0; sn < ns; ++sn)
-            if (h[sn] == l)
-                for (int r0 = 0; r0 < nc[sn] + nr[sn]; r0 += SCHUR_ROWS) { fwd_s.push_back(sn); fwd_r0.push_back(r0); }
-        fwd_ptr.push_back((int)fwd_s.size());
-    }
-    bwd_ptr.assign(1, 0);
-    bwd_s.clear();
-    bwd_q0.clear();
-    bwd_qn.clear();
-    for (int l = 0; l <= dmax && ns; ++l) {
-        for (int sn = 0; sn < ns; ++sn) {
-            if (dep[sn] != l) continue;
-            const int w = nc[sn] <= SCHUR_COLS_MAX ? nc[sn] : SCHUR_COLS;
-            for (int q0 = 0; q0 < nc[sn]; q0 += w) {
-                bwd_s.push_back(sn);
-                bwd_q0.push_back(q0);
-                bwd_qn.push_back(std::min(w, nc[sn] - q0));
+    ctab.assign(2 * (size_t)fpos[ns], -1);
+    for (int64_t q = 0; q < fpos[ns]; ++q) {
+        const int64_t k = cptr[q + 1] - cptr[q];
+        if (k > 2) {
+            if (cptr[q] > INT32_MAX - 2) fail(GT_ERR_UNSUPPORTED, "contributor lists too long");
+            ctab[2 * q] = -2 - (int32_t)cptr[q];
+            ctab[2 * q + 1] = (int32_t)k;
+        } else {
+            for (int64_t e = 0; e < k; ++e) {
+                if (cidx[cptr[q] + e] > INT32_MAX) fail(GT_ERR_UNSUPPORTED, "update buffer too long");
+                ctab[2 * q + e] = (int32_t)cidx[cptr[q] + e];
             }
         }
-        bwd_ptr.push_back((int)bwd_s.size());
+    }
+    slots = 0;
+    // the items of one tile: chunks [lo, hi) of `step` over its reduction, split when more than one (then with a
+    // finalize item in `fin`)
+    auto tile_items = [&](std::vector<SnItem>& out, std::vector<SnItem>& fin, int sn, int t0, int kw, int lo, int hi,
+                          int step, bool forward) {
+        const int ks = std::max(1, (hi - lo + step - 1) / step);
+        for (int k = 0; k < ks; ++k) {
+            SnItem it{};
+            it.s = sn;
+            it.t0 = t0;
+            it.kw = kw;
+            it.C = nc[sn];
+            it.m = nc[sn] + nr[sn];
+            it.c0 = c0[sn];
+            it.k0 = lo + k * step;
+            it.k1 = std::min(hi, lo + (k + 1) * step);
+            it.slot0 = ks > 1 ? (int32_t)slots : -1;
+            it.kidx = k;
+            it.ks = ks;
+            it.poff = poff[sn];
+            it.aux = forward ? uoff[sn] : roff[sn];
+            it.fpos = fpos[sn];
+            out.push_back(it);
+        }
+        if (ks > 1) {
+            fin.push_back(out.back());
+            slots += ks;
+        }
+        return std::min(step, hi - lo);
+    };
+    fwd_ptr.assign(1, 0);
+    fwd.clear();
+    fwd_cols.clear();
+    ffin_ptr.assign(1, 0);
+    ffin.clear();
+    for (int l = 0; l <= hmax && ns; ++l) {
+        int cols = SCHUR_TILE;          // the 16-row cross-wave reduction reuses the staging buffer
+        for (int sn = 0; sn < ns; ++sn) {
+            if (h[sn] != l) continue;
+            const int m = nc[sn] + nr[sn], narrow = nc[sn] <= SCHUR_NARROW;
+            const int tile = narrow ? 64 : SCHUR_TILE, step = SCHUR_PV * (narrow ? 4 : 16);
+            for (int r0 = 0; r0 < m; r0 += tile) {
+                const int qend = std::min(nc[sn], std::min(m, r0 + tile));  // zeros above L^-1 beyond
+                cols = std::max(cols, tile_items(fwd, ffin, sn, r0, narrow ? 1 : 4, 0, qend, step, true));
+            }
+        }
+        fwd_ptr.push_back((int)fwd.size());
+        ffin_ptr.push_back((int)ffin.size());
+        fwd_cols.push_back(cols);
+    }
+    bwd_ptr.assign(1, 0);
+    bwd.clear();
+    bwd_rows.clear();
+    bfin_ptr.assign(1, 0);
+    bfin.clear();
+    for (int l = 0; l <= dmax && ns; ++l) {
+        int rws = 1;
+        for (int sn = 0; sn < ns; ++sn) {
+            if (dep[sn] != l) continue;
+            const int m = nc[sn] + nr[sn];
+            const bool wide_tile = nc[sn] > 16 && nc[sn] <= 32 && m <= SCHUR_PV * 8;
+            const int rp = wide_tile ? 8 : 16, tile = wide_tile ? 32 : SCHUR_TILE;
+            for (int q0 = 0; q0 < nc[sn]; q0 += tile)   // rows above q0 meet the zeros above L^-1
+                rws = std::max(rws, tile_items(bwd, bfin, sn, q0, rp, q0, m, SCHUR_PV * rp, false));
+        }
+        bwd_ptr.push_back((int)bwd.size());
+        bfin_ptr.push_back((int)bfin.size());
+        bwd_rows.push_back(rws);
     }
     (void)roots;
 }
