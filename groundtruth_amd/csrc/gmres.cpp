@@ -47,8 +47,8 @@ struct Batch {
     std::vector<Rhs> R;
     Krylov kr;
     DevBuf<zd> ydev;                       // combination coefficients, (restart + 1) per right-hand side
-    std::vector<cd> ybuf;
-    std::vector<double> red;
+    HostBuf<cd> ybuf;
+    HostBuf<double> red;
 
     Batch(gt_stack* st_, int nr_, int restart_, const gt_gmres_params& p_)
         : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), nr(nr_), p(p_), R(nr_) {
@@ -67,15 +67,15 @@ struct Batch {
         }
         kr.init(n, restart + 1);
         ydev.alloc((size_t)KMAX * (restart + 1));
-        ybuf.assign((size_t)KMAX * (restart + 1), 0);
-        red.resize((size_t)KMAX * kr.P);
+        ybuf.alloc((size_t)KMAX * (restart + 1));
+        red.alloc((size_t)KMAX * kr.P);
     }
 
     zd* row(Rhs& q, int j) { return q.V.p + (size_t)j * n; }
 
     void fetch(int cnt) {
-        GT_HIP(hipMemcpyAsync(red.data(), kr.red.p, sizeof(double) * kr.P * cnt, hipMemcpyDeviceToHost, s));
-        GT_HIP(hipStreamSynchronize(s));
+        GT_HIP(hipMemcpyAsync(red.p, kr.red.p, sizeof(double) * kr.P * cnt, hipMemcpyDeviceToHost, s));
+        spin_sync(s);
     }
     double val(int slot_in_batch, int c) const { return red[(size_t)slot_in_batch * kr.P + c]; }
 
@@ -116,13 +116,13 @@ struct Batch {
             if (from[i] != dst[i]) any = true;
         }
         if (any) k_copy(c, n, s);
-        GT_HIP(hipStreamSynchronize(s));     // the previous copy out of ybuf has finished
+        spin_sync(s);                        // the previous copy out of ybuf has finished
         for (int i = 0; i < c.nr; ++i) {
             c.a[i] = R[who[i]].V.p;
             c.k[i] = (int)ys[i].size();
             for (size_t r = 0; r < ys[i].size(); ++r) ybuf[(size_t)i * (restart + 1) + r] = ys[i][r];
         }
-        GT_HIP(hipMemcpyAsync(ydev.p, ybuf.data(), sizeof(zd) * c.nr * (restart + 1), hipMemcpyHostToDevice, s));
+        GT_HIP(hipMemcpyAsync(ydev.p, ybuf.p, sizeof(zd) * c.nr * (restart + 1), hipMemcpyHostToDevice, s));
         k_combine(c, ydev.p, restart + 1, n, s);
     }
 
@@ -241,6 +241,10 @@ struct Batch {
                 }
                 apply(A, in, out);
             }
+            // the fused pass also forms the second pass's coefficients V^H w', used when the re-pass is taken
+            int jmax = 0;
+            for (int q : A) jmax = std::max(jmax, R[q].j);
+            const bool fuse = Krylov::fused(jmax + 1);
             auto project = [&](const std::vector<int>& who) {
                 KrSet k{};
                 k.nr = (int)who.size();
@@ -251,10 +255,17 @@ struct Batch {
                     k.w[i] = q.w.p;
                 }
                 kr.dots(k, s);
-                kr.project_update(k, s);
+                if (fuse) {
+                    kr.update_dots(k, s);
+                    kr.repass(k, s);
+                } else {
+                    kr.project_update(k, s);
+                }
                 fetch(k.nr);
             };
             project(A);
+            // CGS, then the DGKS re-pass where |w| fell below 0.7 of its value before the projection (on the fused
+            // path the device took the test and the re-pass already; its decision is read back)
             std::vector<int> D;
             for (size_t i = 0; i < A.size(); ++i) {
                 Rhs& q = R[A[i]];
@@ -262,7 +273,15 @@ struct Batch {
                 for (int r = 0; r <= q.j; ++r) q.h[r] = cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
                 const double w_norm = std::sqrt(val((int)i, kr.slot_w()));
                 q.hn = std::sqrt(val((int)i, kr.slot_w_after()));
-                if (q.hn < 0.7 * w_norm) D.push_back(A[i]);
+                if (fuse) {
+                    if (val((int)i, kr.slot_repass()) != 0.0) {
+                        for (int r = 0; r <= q.j; ++r)
+                            q.h[r] += cd(val((int)i, kr.slot_h2() + 2 * r), val((int)i, kr.slot_h2() + 2 * r + 1));
+                        q.hn = std::sqrt(val((int)i, kr.slot_w_after2()));
+                    }
+                } else if (q.hn < 0.7 * w_norm) {
+                    D.push_back(A[i]);
+                }
             }
             if (!D.empty()) {                 // DGKS: a second pass only after cancellation
                 project(D);

@@ -70,6 +70,40 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Page-locked host memory (hipHostMalloc): the small per-iteration copies of the solver go straight to and from it.
+template <class T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) GT_HIP(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+        for (size_t i = 0; i < count; ++i) p[i] = T();
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
+
+// Waits for the stream by polling: the solver's host steps sit between short device phases, where a sleeping wait
+// adds its wake-up time to every step.
+inline void spin_sync(hipStream_t s) {
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) GT_HIP(e);
+    }
+}
+
 // Interleaved complex number for device and host code.
 template <class T>
 struct cpx {

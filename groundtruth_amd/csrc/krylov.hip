@@ -81,12 +81,106 @@ __global__ __launch_bounds__(NTH) void kr_dots(KrSet k, int n, int chunk, double
     }
 }
 
+// The update w' = w - V h fused with the products V^H w' (the second Gram-Schmidt pass's coefficients). Lane =
+// (element e = lane & 7, row group g = lane >> 3): a lane holds rows g, g + 8, .. (at most RPL) of its element's basis
+// column, so each basis value is loaded once for both; the row groups' parts of V h meet by butterfly sums, and the
+// products are summed over the block's elements per row.
+template <int RPL>
+__global__ __launch_bounds__(NTH) void kr_update_dots(KrSet k, int n, int chunk, const double* red, double* part,
+                                                      int P, int nblk, int slot_after, int slot_h2) {
+    __shared__ double hs_raw[2 * 8 * RPL];
+    __shared__ double wsum[NTH / 64][2 * 8 * RPL + 1];
+    zd* hs = reinterpret_cast<zd*>(hs_raw);
+    const int p = blockIdx.y;
+    const zd* __restrict__ V = k.V[p];
+    zd* w = k.w[p];
+    const int m = k.m[p];
+    const double* h = red + (size_t)p * P;
+    for (int r = threadIdx.x; r < 8 * RPL; r += NTH) hs[r] = r < m ? zd(h[2 * r], h[2 * r + 1]) : zd();
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, e = lane & 7, g = lane >> 3;
+    zd hr[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) hr[t] = hs[g + 8 * t];
+    double dre[RPL], dim[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) dre[t] = dim[t] = 0.0;
+    double nn = 0.0;
+    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int ib = i0; ib < i1; ib += 32) {            // 4 waves x 8 elements per step
+        const int i = ib + 8 * wave + e;
+        const bool in = i < i1;
+        zd vv[RPL];
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int r = g + 8 * t;
+            vv[t] = in && r < m ? V[(size_t)r * n + i] : zd();
+        }
+        const zd wi0 = in ? w[i] : zd();
+        double sre = 0.0, sim = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            sre += vv[t].re * hr[t].re - vv[t].im * hr[t].im;
+            sim += vv[t].re * hr[t].im + vv[t].im * hr[t].re;
+        }
+#pragma unroll
+        for (int off = 8; off < 64; off <<= 1) {
+            sre += __shfl_xor(sre, off);
+            sim += __shfl_xor(sim, off);
+        }
+        const zd wi(wi0.re - sre, wi0.im - sim);
+        if (in && g == 0) {
+            w[i] = wi;
+            nn += wi.re * wi.re + wi.im * wi.im;
+        }
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            dre[t] += vv[t].re * wi.re + vv[t].im * wi.im;
+            dim[t] += vv[t].re * wi.im - vv[t].im * wi.re;
+        }
+    }
+    // the eight element lanes of each row group, then the four waves in order
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+            dre[t] += __shfl_xor(dre[t], off);
+            dim[t] += __shfl_xor(dim[t], off);
+        }
+    }
+    nn = wave_sum(nn);
+    if (e == 0) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            wsum[wave][2 * (g + 8 * t)] = dre[t];
+            wsum[wave][2 * (g + 8 * t) + 1] = dim[t];
+        }
+    }
+    if (lane == 0) wsum[wave][2 * 8 * RPL] = nn;
+    __syncthreads();
+    double* out = part + ((size_t)p * nblk + blockIdx.x) * P;
+    for (int c = threadIdx.x; c <= 2 * 8 * RPL; c += NTH) {
+        const double v = wsum[0][c] + wsum[1][c] + wsum[2][c] + wsum[3][c];
+        if (c == 2 * 8 * RPL) out[slot_after] = v;
+        else if (c < 2 * m) out[slot_h2 + c] = v;
+    }
+}
+
+// w -= V h, |w|^2 after. With test_w >= 0 (the re-pass), only where sqrt(red[test_after]) < 0.7 sqrt(red[test_w]):
+// block 0 records the decision at red[flag], and a right-hand side without the re-pass leaves its vector alone.
 __global__ __launch_bounds__(NTH) void kr_update(KrSet k, int n, int chunk, const double* red, double* part, int P,
-                                                 int nblk, int slot) {
+                                                 int nblk, int slot, double* red_out, int test_w, int test_after,
+                                                 int flag) {
     __shared__ double lds[NTH / 64];
     __shared__ double hs_raw[1024];
     zd* hs = reinterpret_cast<zd*>(hs_raw);
     const int p = blockIdx.y;
+    if (test_w >= 0) {
+        const double* rp = red_out + (size_t)p * P;
+        const bool take = sqrt(rp[test_after]) < 0.7 * sqrt(rp[test_w]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) red_out[(size_t)p * P + flag] = take ? 1.0 : 0.0;
+        if (!take) return;
+    }
     const zd* __restrict__ V = k.V[p];
     zd* w = k.w[p];
     const int m = k.m[p];
@@ -174,7 +268,7 @@ void Krylov::init(int n_, int rows) {
     if (rows > 512) fail(GT_ERR_ARG, "GMRES restart above 511");
     n = n_;
     rows_cap = rows;
-    P = 2 * rows + 8;
+    P = 4 * rows + 8;
     nblk = (int)std::min<long>(2048, std::max<long>(1, (n + 1023) / 1024));
     part.alloc((size_t)KMAX * nblk * P);
     red.alloc((size_t)KMAX * P);
@@ -192,12 +286,42 @@ void Krylov::dots(const KrSet& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::project_update(const KrSet& k, hipStream_t s) {
+void Krylov::project_update(const KrSet& k, hipStream_t s, bool second) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_update, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
-                       slot_w_after());
+    hipLaunchKernelGGL(kr_update, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p + (second ? slot_h2() : 0),
+                       part.p, P, nblk, slot_w_after(), red.p, -1, -1, 0);
     GT_LAUNCH("kr_update");
     hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after(), red.p);
+    GT_LAUNCH("kr_reduce");
+}
+
+void Krylov::update_dots(const KrSet& k, hipStream_t s) {
+    const int chunk = (n + nblk - 1) / nblk;
+    int mmax = 0;
+    for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
+    if (!fused(mmax)) fail(GT_ERR_ARG, "the fused Gram-Schmidt pass serves at most 128 basis rows");
+    if (mmax <= 32)
+        hipLaunchKernelGGL(kr_update_dots<4>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
+                           slot_w_after(), slot_h2());
+    else if (mmax <= 64)
+        hipLaunchKernelGGL(kr_update_dots<8>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
+                           slot_w_after(), slot_h2());
+    else
+        hipLaunchKernelGGL(kr_update_dots<16>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
+                           slot_w_after(), slot_h2());
+    GT_LAUNCH("kr_update_dots");
+    hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_h2(), red.p);
+    GT_LAUNCH("kr_reduce");
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after(), red.p);
+    GT_LAUNCH("kr_reduce");
+}
+
+void Krylov::repass(const KrSet& k, hipStream_t s) {
+    const int chunk = (n + nblk - 1) / nblk;
+    hipLaunchKernelGGL(kr_update, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p + slot_h2(), part.p, P, nblk,
+                       slot_w_after2(), red.p, slot_w(), slot_w_after(), slot_repass());
+    GT_LAUNCH("kr_update");
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after2(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 

@@ -20,7 +20,7 @@ struct Krylov {
     int n = 0;
     int nblk = 0;             // partial-sum blocks of the reductions
     int rows_cap = 0;
-    int P = 0;                // values per (right-hand side, block): 2 rows_cap (h) + 8 (norms)
+    int P = 0;                // values per (right-hand side, block): 2 rows_cap (h) + 8 (norms) + 2 rows_cap (h2)
     DevBuf<double> part;      // KMAX x nblk x P partial sums
     DevBuf<double> red;       // KMAX x P reduced values
 
@@ -28,10 +28,21 @@ struct Krylov {
     int slot_w() const { return 2 * rows_cap; }          // |w|^2 before the projection
     int slot_w_after() const { return 2 * rows_cap + 1; } // |w|^2 after it
     int slot_norm() const { return 2 * rows_cap + 2; }    // norm2's value
+    int slot_w_after2() const { return 2 * rows_cap + 3; } // |w|^2 after the re-pass
+    int slot_repass() const { return 2 * rows_cap + 4; }   // 1 where the re-pass was taken, else 0
+    int slot_h2() const { return 2 * rows_cap + 8; }      // the re-pass coefficients V^H w' (2 rows_cap values)
     // h_p = V_p[0:m_p]^H w_p and |w_p|^2
     void dots(const KrSet& k, hipStream_t s);
-    // w_p -= V_p[0:m_p]^T h_p (h_p as reduced by dots), |w_p|^2 after
-    void project_update(const KrSet& k, hipStream_t s);
+    // w_p -= V_p[0:m_p]^T h_p (h_p as reduced by dots, or the re-pass coefficients when `second`), |w_p|^2 after
+    void project_update(const KrSet& k, hipStream_t s, bool second = false);
+    // whether update_dots serves m rows
+    static bool fused(int m) { return m <= 128; }
+    // w_p -= V_p[0:m_p]^T h_p, |w_p|^2 after, and the re-pass coefficients V_p^H w_p of the updated vector: one
+    // read of the basis for the update and the second classical Gram-Schmidt pass's products
+    void update_dots(const KrSet& k, hipStream_t s);
+    // after update_dots: the second pass w_p -= V_p^T h2_p where |w'_p| < 0.7 |w_p| (the DGKS test, decided on the
+    // device and recorded in slot_repass), |w_p|^2 after it in slot_w_after2
+    void repass(const KrSet& k, hipStream_t s);
     // |w_p|^2
     void norm2(const KrSet& k, hipStream_t s);
 };
