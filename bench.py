@@ -12,6 +12,7 @@ the padded spectra per transform, the kernel quadrant once), and the oracle's CP
 same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+python bench.py --lp-only [--lp-reps R]   (the counter runs: L_p products, then a 64 MiB device copy as the scale)
 """
 from __future__ import annotations
 
@@ -67,6 +68,39 @@ def time_lp(st, reps: int = 200):
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per L_p product from the committed FETCH_SIZE / WRITE_SIZE runs of this workload (a line
+    `traffic_bytes_per_launch: N` in profiles/lp_traffic_<workload>.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"lp_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
+
+
+def lp_only(args):
+    """Set-up, a few L_p products, then a plain device copy of 64 MiB (the scale for the memory counters)."""
+    import torch
+
+    from groundtruth_amd import boards, peec
+    board = boards.WORKLOADS[args.workload]()
+    st = peec.PlaneStack(board.layers, board.branches, device=0)
+    st.handle
+    t_lp = time_lp(st, reps=args.lp_reps)
+    n = 64 << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": args.workload, "lp_us": t_lp * 1e6, "bytes_per_launch": lp_algorithmic_bytes(st),
+                      "copy_bytes": n}))
+
+
 def cpu_baseline(board, route, f: float) -> dict:
     """The oracle (numpy / scipy restatement of the reference's path) solving one frequency of the workload on
     this host: its factorisation and its GMRES, timed once after a warm-up of the kernels."""
@@ -91,12 +125,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="two_layer")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lp-only", action="store_true")
+    ap.add_argument("--lp-reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported in this build "
               "(independent boards and frequencies shard across GPUs without collectives; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.lp_only:
+        lp_only(args)
+        return
     import torch
 
     from groundtruth_amd import boards, peec
@@ -165,7 +204,7 @@ def main():
         "matvec_GBps": achieved,
         "value_pcie_inclusive": len(board.freqs) / dt_host,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "L_p product (lp_rows_fwd + lp_cols_mac + lp_rows_inv)",
                      "bytes_per_launch": B, "us_per_launch": t_lp * 1e6},
     }
