@@ -138,7 +138,8 @@ int gt_precond(gt_op* op, const void* r_dev, void* z_dev);
  * Replaces peec.py:3093-3224 `_gmres` for the right-preconditioned system (K M^-1) y = r0, x = M^-1 y
  * (peec.py:2581): CGS with a DGKS re-pass below 0.7, Givens rotations, the true-residual check with its aim and
  * miss budget, restarts and extra cycles while each falls to 0.7. The basis and every vector stay on the device;
- * only the Hessenberg column and three norms cross to the host per iteration. */
+ * per iteration only the Hessenberg column, the re-pass coefficients, the norms and the re-pass decision (taken on
+ * the device) cross to the host. */
 typedef struct {
     double tol;               /* stop at ||b - K x|| <= tol * bnorm                                               */
     double bnorm;             /* <= 0: ||b||                                                                      */
