@@ -53,6 +53,7 @@ SIGNATURES = {
     "gt_stack_device_bytes": (C.c_int64, [C.c_void_p]),
     "gt_lp_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_schur_factor": (C.c_int, [C.c_void_p, i32p, f64p]),
+    "gt_stack_derive": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, C.POINTER(C.c_void_p)]),
     "gt_schur_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), i32p, i32p, i32p]),
     "gt_schur_solve_host": (C.c_int, [C.c_void_p, f64p, f64p]),
     "gt_schur_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -170,6 +170,8 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
     return a;
 }
 
+// The stack's own device memory plus its core's (a derived stack counts the shared core too: it is what the
+// stack needs resident)
 int64_t gt_stack_s::device_bytes() const {
     int64_t b = 0;
     for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + h.K.bytes() + h.Kf.bytes();
@@ -180,6 +182,50 @@ int64_t gt_stack_s::device_bytes() const {
     if (schur) b += schur->device_bytes();
     return b;
 }
+
+namespace {
+
+// The incidence of a stack's branches on the device: a_keep, b_keep and the node adjacency (CSR of
+// (branch << 1 | end)).
+void upload_incidence(gt_stack* st) {
+    const int nb = st->nb, nk = st->nk;
+    hipStream_t s = st->ctx->stream;
+    std::vector<int> aptr(nk + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
+        need(ka >= -1 && ka < nk && kb >= -1 && kb < nk, "kept node index out of range");
+        if (ka >= 0) aptr[ka + 1]++;
+        if (kb >= 0) aptr[kb + 1]++;
+    }
+    for (int v = 0; v < nk; ++v) aptr[v + 1] += aptr[v];
+    std::vector<int> adj(std::max(1, aptr[nk]));
+    std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+    for (int b = 0; b < nb; ++b) {
+        if (st->h_a_keep[b] >= 0) adj[fill[st->h_a_keep[b]]++] = b << 1;
+        if (st->h_b_keep[b] >= 0) adj[fill[st->h_b_keep[b]]++] = (b << 1) | 1;
+    }
+    st->a_keep.upload(st->h_a_keep.empty() ? std::vector<int>{0} : st->h_a_keep, s);
+    st->b_keep.upload(st->h_b_keep.empty() ? std::vector<int>{0} : st->h_b_keep, s);
+    st->adj_ptr.upload(aptr, s);
+    st->adj.upload(adj, s);
+}
+
+// G = U^T W for the stack's k branches (row-major k x k)
+void woodbury_gram(const gt_stack* st, double* G) {
+    const int k = st->k, nk = st->nk;
+    for (int r = 0; r < k; ++r) {
+        const int b = st->ns + r;
+        const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
+        for (int c = 0; c < k; ++c) {
+            double v = 0;
+            if (ka >= 0) v += st->h_W[(size_t)c * nk + ka];
+            if (kb >= 0) v -= st->h_W[(size_t)c * nk + kb];
+            G[(size_t)r * k + c] = v;
+        }
+    }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -252,24 +298,7 @@ int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
         need(nb == 0 || (d->a_keep && d->b_keep), "incidence missing");
         st->h_a_keep.assign(d->a_keep, d->a_keep + nb);
         st->h_b_keep.assign(d->b_keep, d->b_keep + nb);
-        std::vector<int> aptr(nk + 1, 0);
-        for (int b = 0; b < nb; ++b) {
-            const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
-            need(ka >= -1 && ka < nk && kb >= -1 && kb < nk, "kept node index out of range");
-            if (ka >= 0) aptr[ka + 1]++;
-            if (kb >= 0) aptr[kb + 1]++;
-        }
-        for (int v = 0; v < nk; ++v) aptr[v + 1] += aptr[v];
-        std::vector<int> adj(std::max(1, aptr[nk]));
-        std::vector<int> fill(aptr.begin(), aptr.end() - 1);
-        for (int b = 0; b < nb; ++b) {
-            if (st->h_a_keep[b] >= 0) adj[fill[st->h_a_keep[b]]++] = b << 1;
-            if (st->h_b_keep[b] >= 0) adj[fill[st->h_b_keep[b]]++] = (b << 1) | 1;
-        }
-        st->a_keep.upload(st->h_a_keep.empty() ? std::vector<int>{0} : st->h_a_keep, s);
-        st->b_keep.upload(st->h_b_keep.empty() ? std::vector<int>{0} : st->h_b_keep, s);
-        st->adj_ptr.upload(aptr, s);
-        st->adj.upload(adj, s);
+        upload_incidence(st.get());
         std::vector<int> ns_(std::max(1, nk), 0);
         if (nk) {
             need(d->node_sheet != nullptr, "node_sheet missing");
@@ -331,22 +360,75 @@ int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
             for (int c = 0; c < 2 && m + c < k; ++c)
                 for (int v = 0; v < nk; ++v) W[(size_t)(m + c) * nk + v] = xx[2 * (size_t)v + c];
         }
-        if (G_out) {
-            for (int r = 0; r < k; ++r) {
-                const int b = st->ns + r;
-                const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
-                for (int c = 0; c < k; ++c) {
-                    double v = 0;
-                    if (ka >= 0) v += W[(size_t)c * nk + ka];
-                    if (kb >= 0) v -= W[(size_t)c * nk + kb];
-                    G_out[(size_t)r * k + c] = v;
-                }
-            }
-        }
         st->W.upload(W.empty() ? std::vector<double>{0.0} : W, st->ctx->stream);
+        st->h_W = std::move(W);
         st->k = k;
+        if (G_out) woodbury_gram(st, G_out);
         GT_HIP(hipStreamSynchronize(st->ctx->stream));
         st->schur = std::move(S);
+    });
+}
+
+int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra, const int32_t* b_keep_extra,
+                    double* G_out, gt_stack** out) {
+    return guard([&] {
+        need(base && out && n_extra >= 0 && (n_extra == 0 || (a_keep_extra && b_keep_extra)), "bad argument");
+        need(base->schur != nullptr, "the base stack has no Schur factor (gt_schur_factor first)");
+        *out = nullptr;
+        GT_HIP(hipSetDevice(base->ctx->device));
+        std::unique_ptr<gt_stack_s> st(new gt_stack_s(base->core));
+        st->ctx = base->ctx;
+        st->S = base->S;
+        st->nx = base->nx;
+        st->ny = base->ny;
+        st->nxb = base->nxb;
+        st->nyb = base->nyb;
+        st->ns = base->ns;
+        st->nk = base->nk;
+        st->nbr = base->nbr + n_extra;
+        st->nb = st->ns + st->nbr;
+        const int nk = st->nk;
+        st->h_a_keep = base->h_a_keep;
+        st->h_b_keep = base->h_b_keep;
+        for (int e = 0; e < n_extra; ++e) {
+            need(a_keep_extra[e] >= -1 && a_keep_extra[e] < nk && b_keep_extra[e] >= -1 && b_keep_extra[e] < nk,
+                 "kept node index out of range");
+            need(a_keep_extra[e] != b_keep_extra[e], "a branch joins a node to itself");
+            st->h_a_keep.push_back(a_keep_extra[e]);
+            st->h_b_keep.push_back(b_keep_extra[e]);
+        }
+        upload_incidence(st.get());
+        // W' = B^-1 U' for the added branches by the device solves (up to 16 columns at a time), after the base's
+        hipStream_t s = st->ctx->stream;
+        const int k0 = base->k, k = k0 + n_extra;
+        std::vector<double> W(base->h_W);
+        W.resize((size_t)k * nk, 0.0);
+        if (n_extra > 0 && nk > 0) {
+            DevBuf<zd> bd((size_t)nk * KMAX), xd((size_t)nk * KMAX);
+            std::vector<zd> bh((size_t)nk * KMAX), xh((size_t)nk * KMAX);
+            for (int m0 = 0; m0 < n_extra; m0 += KMAX) {
+                const int nr = std::min(KMAX, n_extra - m0);
+                std::fill(bh.begin(), bh.end(), zd());
+                for (int c = 0; c < nr; ++c) {
+                    const int ka = a_keep_extra[m0 + c], kb = b_keep_extra[m0 + c];
+                    if (ka >= 0) bh[(size_t)ka * nr + c].re += 1.0;
+                    if (kb >= 0) bh[(size_t)kb * nr + c].re -= 1.0;
+                }
+                bd.upload(bh.data(), (size_t)nk * nr, s);
+                schur_apply(st.get(), bd.p, xd.p, nr);
+                GT_HIP(hipMemcpyAsync(xh.data(), xd.p, sizeof(zd) * (size_t)nk * nr, hipMemcpyDeviceToHost, s));
+                GT_HIP(hipStreamSynchronize(s));
+                for (int c = 0; c < nr; ++c)
+                    for (int v = 0; v < nk; ++v) W[(size_t)(k0 + m0 + c) * nk + v] = xh[(size_t)v * nr + c].re;
+            }
+        }
+        st->W.upload(W.empty() ? std::vector<double>{0.0} : W, s);
+        st->h_W = std::move(W);
+        st->k = k;
+        if (G_out) woodbury_gram(st.get(), G_out);
+        st->work.ensure(st.get(), 1);
+        GT_HIP(hipStreamSynchronize(s));
+        *out = st.release();
     });
 }
 

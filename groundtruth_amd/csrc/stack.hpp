@@ -39,22 +39,53 @@ struct Work {
 
 }  // namespace gt
 
+namespace gt {
+
+// What a stack shares with the what-if stacks derived from it (peec.py:1871-1909 `with_branches`: "shares grids and
+// kernels"): the strip layout, the kernel spectra and twiddles, and the Schur factor of the Laplacian of its own
+// branches.
+struct StackCore {
+    OrientHost o[2];
+    std::map<int, DevBuf<zd>> tw;
+    std::map<int, DevBuf<zf>> twf;
+    DevBuf<int> node_sheet;
+    DevBuf<unsigned char> strip_sheet;        // ns: sheet of each strip
+    std::vector<unsigned char> h_strip_sheet;
+    LpShape shape;
+    size_t t_elems = 0;                       // complex elements of the L_p intermediates (both orientations)
+    std::unique_ptr<Schur> schur;
+};
+
+}  // namespace gt
+
 struct gt_stack_s {
+    std::shared_ptr<gt::StackCore> core;
+    // the core's members under their own names
+    gt::OrientHost (&o)[2];
+    std::map<int, gt::DevBuf<gt::zd>>& tw;
+    std::map<int, gt::DevBuf<gt::zf>>& twf;
+    gt::DevBuf<int>& node_sheet;
+    gt::DevBuf<unsigned char>& strip_sheet;
+    std::vector<unsigned char>& h_strip_sheet;
+    gt::LpShape& shape;
+    size_t& t_elems;
+    std::unique_ptr<gt::Schur>& schur;
+
     gt_ctx* ctx = nullptr;
     int S = 0, nx = 0, ny = 0, nxb = 0, nyb = 0, nbr = 0, ns = 0, nb = 0, nk = 0;
-    gt::OrientHost o[2];
-    std::map<int, gt::DevBuf<gt::zd>> tw;
-    std::map<int, gt::DevBuf<gt::zf>> twf;
-    gt::DevBuf<int> a_keep, b_keep, adj_ptr, adj, node_sheet;
-    gt::DevBuf<unsigned char> strip_sheet;    // ns: sheet of each strip
-    std::vector<unsigned char> h_strip_sheet;
+    gt::DevBuf<int> a_keep, b_keep, adj_ptr, adj;
     std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
-    gt::LpShape shape;
-    size_t t_elems = 0;                       // complex elements of the L_p intermediates (both orientations)
     gt::Work work;
-    std::unique_ptr<gt::Schur> schur;
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
+    std::vector<double> h_W;                  // its host copy (the columns a derived stack extends)
     int k = 0;
+
+    explicit gt_stack_s(std::shared_ptr<gt::StackCore> c = std::make_shared<gt::StackCore>())
+        : core(std::move(c)), o(core->o), tw(core->tw), twf(core->twf), node_sheet(core->node_sheet),
+          strip_sheet(core->strip_sheet), h_strip_sheet(core->h_strip_sheet), shape(core->shape),
+          t_elems(core->t_elems), schur(core->schur) {}
+    gt_stack_s(const gt_stack_s&) = delete;
+    gt_stack_s& operator=(const gt_stack_s&) = delete;
 
     int64_t device_bytes() const;
     gt::LpArgs<double> lp_args(int mode) const;

@@ -19,6 +19,7 @@ full-width strips of one copper thickness, lumped branches (vias). Plane-pair ca
 from __future__ import annotations
 
 import ctypes as C
+import copy
 import math
 from collections import OrderedDict
 from typing import Optional
@@ -70,7 +71,10 @@ class Operator:
         ds = self.dinv_sheet[0]
         k = stack.nbr
         if k:
-            c = self.dinv_tail / ds - 1.0
+            # S = d_s (B + U C U^T): B holds the factored branches at weight 1 (C = d/d_s - 1); branches a what-if
+            # stack added (with_branches) are not in B (C = d/d_s)
+            in_B = np.arange(k) < stack.nbr_factored
+            c = self.dinv_tail / ds - in_B
             Q = np.diag(c) @ np.linalg.inv(np.eye(k) + stack.G @ np.diag(c))
         else:
             Q = np.zeros((0, 0), complex)
@@ -133,6 +137,9 @@ class PlaneStack:
         self._kernels()
         self._handle = None
         self._ops: OrderedDict = OrderedDict()
+        self._base: Optional["PlaneStack"] = None
+        self._parent: Optional["PlaneStack"] = None
+        self.nbr_factored = self.nbr       # leading branches in the factored Laplacian B (all, for a base stack)
 
     @property
     def handle(self):
@@ -222,8 +229,53 @@ class PlaneStack:
                 Ky[:, :, s, t] = Ky[:, :, t, s] = qy.T / (self.Py[0] * self.Py[1])
         self._kx, self._ky = np.ascontiguousarray(Kx), np.ascontiguousarray(Ky)
 
+    # ---------------------------------------------------------------- what-if stacks (peec.py:1871-1923)
+    def with_branches(self, extra: list) -> "PlaneStack":
+        """This stack plus a few branches (candidate vias): shares the grids, kernels, numbering and the device's
+        spectra and Schur factor; the added branches enter the preconditioner as Woodbury columns of the base's
+        factor (peec.py:1871-1909). A branch that joins two islands changes the reference nodes: then a stack of its
+        own."""
+        extra = [b for b in extra if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0 and tuple(b.a) != tuple(b.b)]
+        if not extra:
+            return self
+        a = np.array([self.node_id(b.a) for b in extra], np.int64)
+        b = np.array([self.node_id(b.b) for b in extra], np.int64)
+        if np.any(self.full_label[a] != self.full_label[b]):
+            return PlaneStack(self.layers, self.branches + extra, sigma=self.sigma, precond=self.precond_mode,
+                              device=self.device)
+        base = self if self._base is None else self._base
+        new = copy.copy(self)
+        new.branches = self.branches + extra
+        new.nbr = self.nbr + len(extra)
+        new.nb = self.nb + len(extra)
+        new.a_nodes = np.concatenate([self.a_nodes, a])
+        new.b_nodes = np.concatenate([self.b_nodes, b])
+        kidx = np.cumsum(self.keep) - 1
+        new.a_keep = np.concatenate([self.a_keep, np.where(self.keep[a], kidx[a], -1)]).astype(np.int32)
+        new.b_keep = np.concatenate([self.b_keep, np.where(self.keep[b], kidx[b], -1)]).astype(np.int32)
+        new._base = base                    # the stack whose factor serves (the first of a chain of what-ifs)
+        new._parent = self                  # the stack map_state maps from
+        new.nbr_factored = base.nbr_factored
+        new._handle = None
+        new._ops = OrderedDict()
+        return new
+
+    def map_state(self, x: Optional[np.ndarray], stack: "PlaneStack") -> Optional[np.ndarray]:
+        """A solution vector of `stack` in this stack's unknowns (a warm start): the same stack, or this stack's
+        base, whose added branches start at zero current (peec.py:1911-1923)."""
+        if x is None:
+            return None
+        if stack is self:
+            return x
+        if self._parent is stack and len(x) == stack.nb + stack.n_keep:
+            return np.concatenate([x[:stack.nb], np.zeros(self.nb - stack.nb, complex), x[stack.nb:]])
+        return None
+
     # ---------------------------------------------------------------- the device copy
     def _upload(self):
+        if self._base is not None:
+            self._derive()
+            return
         L = _lib.lib()
         self.ctx = _lib.Context.get(self.device)
         xs = np.ascontiguousarray(self.xa, np.int32)
@@ -252,6 +304,23 @@ class PlaneStack:
         if code != _lib.GT_OK:
             L.gt_stack_destroy(h)
             _lib.check(code, "gt_schur_factor")
+        self.G = Gm[: self.nbr, : self.nbr].copy()
+        self._handle = h
+
+    def _derive(self):
+        """The device stack of a what-if stack: the base's (made first) plus the added branches' incidence and
+        Woodbury columns (gt_stack_derive)."""
+        base = self._base
+        base_h = base.handle
+        self.ctx = base.ctx
+        nx = self.nbr - base.nbr
+        ea = np.ascontiguousarray(self.a_keep[base.nb:], np.int32)
+        eb = np.ascontiguousarray(self.b_keep[base.nb:], np.int32)
+        Gm = np.zeros((max(self.nbr, 1), max(self.nbr, 1)))
+        h = C.c_void_p()
+        _lib.check(_lib.lib().gt_stack_derive(base_h, nx, _lib.ptr(ea if nx else np.zeros(1, np.int32), _lib.i32p),
+                                              _lib.ptr(eb if nx else np.zeros(1, np.int32), _lib.i32p),
+                                              _lib.ptr(Gm), C.byref(h)), "gt_stack_derive")
         self.G = Gm[: self.nbr, : self.nbr].copy()
         self._handle = h
 
@@ -571,6 +640,13 @@ class RouteExcitation:
         self.branch_psi = bp
         self.psi = np.concatenate([self.strip_psi, self.branch_psi])
         self.L_trace = trace_self(trace, trace_thickness) if trace else 0.0
+
+    def on(self, stack: PlaneStack) -> "RouteExcitation":
+        """The same loop on a what-if stack (with_branches): the strip coupling is reused, the added branches'
+        coupling computed (peec.py:2771-2774)."""
+        drv, rcv = self.inject
+        return RouteExcitation(stack, self.trace, drv, rcv, self.trace_t, self.strip_psi, self.branch_psi,
+                               self.extra)
 
     def rhs(self, f: float):
         """(rhs_top = -jw psi, J) at f (peec.py:2789-2795)."""

@@ -111,6 +111,16 @@ class Solution:
     x: Optional[np.ndarray] = field(default=None, repr=False)
     residual: float = float("nan")
 
+    def state_for(self, stack) -> np.ndarray:
+        """The solver's unknown vector [branch currents; kept node potentials] of this solution on `stack`, the
+        stack it was solved on (a warm start), rebuilt from the currents and potentials held here
+        (peec.py:1127-1144)."""
+        if self.x is not None and len(self.x) == stack.nb + stack.n_keep:
+            return self.x
+        strips = np.concatenate([self.I_x[tuple(stack.xa.T)], self.I_y[tuple(stack.ya.T)]])
+        Ib = np.concatenate([strips, np.asarray(self.I_branch, complex)])
+        return np.concatenate([Ib, self.V[np.nonzero(stack.node)][stack.keep]])
+
     @property
     def L_loop(self) -> float:
         """Loop inductance (H): Im(Z_loop) / w."""

@@ -93,6 +93,15 @@ int gt_lp_matvec(gt_stack* stack, const void* I_dev, void* out_dev);
  * G_out: k x k (k = nbr) real, row-major: U^T B^-1 U, from which the caller forms each frequency's
  *        Q = C (I + G C)^-1 (peec.py:2357-2377's Woodbury identity). May be NULL when nbr == 0. */
 int gt_schur_factor(gt_stack* stack, const int32_t* node_xyz, double* G_out);
+/* A what-if stack (peec.py:1871-1909 `with_branches`): `base` plus n_extra branches (candidate vias) whose kept end
+ * nodes are a_keep_extra / b_keep_extra (-1: a reference node), appended after the base's branches. It shares the
+ * base's spectra, strip layout and Schur factor (held while any stack uses them; both order their work on the
+ * base's context); its own are the incidence and the Woodbury columns W = B^-1 [U U'] of the base's factor, the added ones
+ * made by device solves. The added branches are not in B: a caller's C for them is d_br / d_s, not d_br / d_s - 1.
+ * G_out: k x k (k = base branches + n_extra), row-major U^T B^-1 U, or NULL. A branch that joins two pieces of
+ * copper the base keeps apart changes the reference nodes; such a what-if needs a stack of its own. */
+int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra, const int32_t* b_keep_extra,
+                    double* G_out, gt_stack** out);
 /* Statistics of the factor: entries of L, supernodes, tree levels, the largest supernode's columns. */
 int gt_schur_stats(const gt_stack* stack, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels, int32_t* max_cols);
 /* B^-1 b on the host with the same factor (a check of the factorisation that needs no device). b, x: n_keep

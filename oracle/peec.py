@@ -15,6 +15,7 @@ Each function cites the reference lines it follows.
 """
 from __future__ import annotations
 
+import copy
 import math
 from dataclasses import dataclass, field
 from typing import Callable, Optional
@@ -169,6 +170,46 @@ class PlaneStack:
     def node_id(self, node) -> int:
         """Node number of (layer, i, j), -1 off copper (peec.py:1864-1869)."""
         return int(self.nid[tuple(node)])
+
+    # ---------------------------------------------------------------- what-if stacks (peec.py:1871-1923)
+    def with_branches(self, extra: list) -> "PlaneStack":
+        """This stack plus a few branches (peec.py:1871-1909): a shallow copy with the branch list and incidence
+        extended, the kernels shared; a branch joining two islands makes a stack of its own. Its operators factorise
+        the extended S afresh (the reference reuses the base factor through Woodbury: the same S^-1)."""
+        extra = [b for b in extra if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0 and tuple(b.a) != tuple(b.b)]
+        if not extra:
+            return self
+        a = np.array([self.node_id(b.a) for b in extra])
+        b = np.array([self.node_id(b.b) for b in extra])
+        if np.any(self.full_label[a] != self.full_label[b]):
+            return PlaneStack(self.layers, self.branches + extra, self.sigma)
+        new = copy.copy(self)
+        new.branches = self.branches + extra
+        new.nbr = self.nbr + len(extra)
+        new.nb = self.nb + len(extra)
+        new.a_nodes = np.concatenate([self.a_nodes, a])
+        new.b_nodes = np.concatenate([self.b_nodes, b])
+        r = np.arange(self.nb, new.nb)
+        rows = sp.csr_matrix((np.concatenate([np.ones(len(r)), -np.ones(len(r))]),
+                              (np.concatenate([r - self.nb] * 2), np.concatenate([a, b]))),
+                             shape=(len(r), self.n_nodes))
+        new.A = sp.vstack([self.A, rows]).tocsr()
+        new.Ar = new.A[:, self.keep].tocsc()
+        new.ArT = new.Ar.T.tocsr()
+        new._base = self
+        new._ops = {}
+        return new
+
+    def map_state(self, x, stack: "PlaneStack"):
+        """A solution vector of `stack` in this stack's unknowns: the same stack, or this stack's base, whose added
+        branches start at zero current (peec.py:1911-1923)."""
+        if x is None:
+            return None
+        if stack is self:
+            return x
+        if getattr(self, "_base", None) is stack and len(x) == stack.nb + stack.Ar.shape[1]:
+            return np.concatenate([x[:stack.nb], np.zeros(self.nb - stack.nb, complex), x[stack.nb:]])
+        return None
 
     # ---------------------------------------------------------------- inductance (peec.py:1925-1988, 2054-2135)
     def face_gap(self, s: int, t: int) -> float:
@@ -603,13 +644,15 @@ def trace_resistance(trace: list, f: float, sigma: float, trace_t: float) -> flo
 
 
 class RouteExcitation:
-    """A known trace current plus injections on a stack (peec.py:2703-2817, without derived stacks)."""
+    """A known trace current plus injections on a stack (peec.py:2703-2817)."""
 
     def __init__(self, stack: PlaneStack, trace: list, inject_driver=None, inject_receiver=None,
                  trace_thickness: float = 35e-6, injections: Optional[list] = None):
         self.stack = stack
         self.trace = trace
         self.trace_t = trace_thickness
+        self.inject = (inject_driver, inject_receiver)
+        self.extra = list(injections or [])
         terms = []
         if inject_driver is not None:
             terms.append((inject_driver, -1.0, "driver return node"))
@@ -637,6 +680,10 @@ class RouteExcitation:
         self.branch_psi = stack.branch_coupling(trace)
         self.psi = np.concatenate([self.strip_psi, self.branch_psi])
         self.L_trace = trace_self(trace, trace_thickness) if trace else 0.0
+
+    def on(self, stack: PlaneStack) -> "RouteExcitation":
+        """The same loop on a what-if stack (peec.py:2771-2774); the coupling is recomputed (the same strip values)."""
+        return RouteExcitation(stack, self.trace, *self.inject, self.trace_t, self.extra)
 
     def rhs(self, f: float):
         """(rhs_top = -jw psi, J) of this loop at f (peec.py:2789-2795)."""
