@@ -6,10 +6,12 @@ to a relative residual of 1e-10. One step = the 10 solves (one route at every fr
 right-hand sides resident, run as one batch (each solve follows the reference's GMRES and is bit-identical to its
 solve alone). value = solves per second over the timed steps.
 
-Also reported: the L_p product's (the FFT matvec's) roofline, timed live with HIP events on the library's stream
-over repeated launches, its algorithmic bytes per launch (strip currents in and out, one read and one write of
-the padded spectra per transform, the kernel quadrant once), and the oracle's CPU time on a bounded sample of the
-same workload.
+Also reported: the L_p product's (the FFT matvec's) roofline. Every L_p launch of the timed steps is bracketed by
+HIP events on the library's stream (gt_ctx_lp_timing); achieved = the right-hand sides per launch x the algorithmic
+bytes of one product (strip currents in and out, one read and one write of the padded spectra per transform, the
+kernel quadrant once) / the average launch. traffic = the same launch's HBM bytes from the FETCH_SIZE / WRITE_SIZE
+runs (per right-hand side, committed under profiles/). Beside it, a single-right-hand-side product timed over 200
+launches, and the oracle's CPU time on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 python bench.py --lp-only [--lp-reps R]   (the counter runs: L_p products, then a 64 MiB device copy as the scale)
@@ -165,15 +167,26 @@ def main():
                 raise RuntimeError(f"GMRES did not converge at {op.f:.4g} Hz")
         return [int(r.iterations) for r in res]
 
+    import ctypes as C
+
+    from groundtruth_amd import _lib
+    L = _lib.lib()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # every L_p launch of the timed steps bracketed by HIP events on the library's stream
+    _lib.check(L.gt_ctx_lp_timing(st.ctx.handle, 1))
     t0 = time.perf_counter()
     its = []
     for _ in range(args.steps):
         its = step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    n_lp, n_rhs, lp_ms = C.c_int64(), C.c_int64(), C.c_double()
+    _lib.check(L.gt_ctx_lp_time(st.ctx.handle, C.byref(n_lp), C.byref(n_rhs), C.byref(lp_ms)))
+    _lib.check(L.gt_ctx_lp_timing(st.ctx.handle, 0))
+    lp_us = lp_ms.value / max(1, n_lp.value) * 1e3           # average L_p launch in the timed steps
+    lp_rhs = n_rhs.value / max(1, n_lp.value)                # right-hand sides per launch
     solves = args.steps * len(board.freqs)
     # the drop-in path (host vectors in and out, Solution unpacked) for the PCIe-inclusive rate
     t1 = time.perf_counter()
@@ -182,7 +195,8 @@ def main():
     dt_host = time.perf_counter() - t1
     t_lp = time_lp(st)
     B = lp_algorithmic_bytes(st)
-    achieved = B / t_lp / 1e9
+    achieved = lp_rhs * B / (lp_us * 1e-6) / 1e9
+    traffic = measured_traffic(args.workload)
     out = {
         "metric": METRIC,
         "value": solves / dt,
@@ -204,9 +218,14 @@ def main():
         "matvec_GBps": achieved,
         "value_pcie_inclusive": len(board.freqs) / dt_host,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
-                     "kernel": "L_p product (lp_rows_fwd + lp_cols_mac + lp_rows_inv)",
-                     "bytes_per_launch": B, "us_per_launch": t_lp * 1e6},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": round(lp_rhs * traffic) if traffic else None,
+                     "kernel": "L_p product (lp_rows_fwd + lp_cols_mac + lp_rows_inv), as launched by the timed "
+                               "batched solves",
+                     "launches": n_lp.value, "rhs_per_launch": lp_rhs, "us_per_launch": lp_us,
+                     "bytes_per_launch": round(lp_rhs * B),
+                     "single_rhs": {"us_per_launch": t_lp * 1e6, "achieved": B / t_lp / 1e9,
+                                    "bytes_per_launch": B}},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])

@@ -265,15 +265,44 @@ void gt_ctx_destroy(gt_ctx* ctx) {
 int gt_ctx_sync(gt_ctx* ctx) {
     return guard([&] {
         need(ctx != nullptr, "ctx is NULL");
+        std::lock_guard<std::mutex> lk(ctx->mu);
         GT_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 
 void* gt_ctx_stream(gt_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int gt_ctx_lp_timing(gt_ctx* ctx, int enable) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->lp_timing = enable != 0;
+        ctx->lp_used = 0;
+        ctx->lp_rhs = 0;
+    });
+}
+
+int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms) {
+    return guard([&] {
+        need(ctx && launches && rhs && ms, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipStreamSynchronize(ctx->stream));
+        double total = 0;
+        for (size_t i = 0; i < ctx->lp_used; ++i) {
+            float t = 0;
+            GT_HIP(hipEventElapsedTime(&t, ctx->lp_events[2 * i], ctx->lp_events[2 * i + 1]));
+            total += t;
+        }
+        *launches = (int64_t)ctx->lp_used;
+        *rhs = ctx->lp_rhs;
+        *ms = total;
+    });
+}
+
 int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
     return guard([&] {
         need(ctx && d && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
         *out = nullptr;
         need(d->n_sheets >= 1 && d->n_sheets <= LP_MAX_SHEETS, "1 to 8 current sheets");
         need(d->nx >= 2 && d->ny >= 2, "grid below 2 x 2");
@@ -316,6 +345,7 @@ int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
 
 void gt_stack_destroy(gt_stack* st) {
     if (!st) return;
+    std::lock_guard<std::mutex> lk(st->ctx->mu);
     (void)hipSetDevice(st->ctx->device);
     (void)hipStreamSynchronize(st->ctx->stream);
     delete st;
@@ -326,6 +356,7 @@ int64_t gt_stack_device_bytes(const gt_stack* st) { return st ? st->device_bytes
 int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
     return guard([&] {
         need(st && I_dev && out_dev, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
         if (st->ns == 0) return;
         LpArgs<double> a = st->lp_args(0);
         a.rhs.nr = 1;
@@ -338,6 +369,7 @@ int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
 int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
     return guard([&] {
         need(st && node_xyz, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
         const int nk = st->nk, nb = st->nb;
         std::vector<int> ea, eb;
         std::vector<double> diag;
@@ -373,6 +405,7 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
                     double* G_out, gt_stack** out) {
     return guard([&] {
         need(base && out && n_extra >= 0 && (n_extra == 0 || (a_keep_extra && b_keep_extra)), "bad argument");
+        std::lock_guard<std::mutex> lk(base->ctx->mu);
         need(base->schur != nullptr, "the base stack has no Schur factor (gt_schur_factor first)");
         *out = nullptr;
         GT_HIP(hipSetDevice(base->ctx->device));
@@ -453,6 +486,7 @@ int gt_schur_solve_host(const gt_stack* st, const double* b, double* x) {
 int gt_schur_solve(gt_stack* st, const void* b_dev, void* x_dev) {
     return guard([&] {
         need(st && b_dev && x_dev && b_dev != x_dev, "NULL or aliased argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
         schur_apply(st, (const zd*)b_dev, (zd*)x_dev, 1);
     });
 }
@@ -496,6 +530,7 @@ void gt_schur_host_free(gt_schur_host* h) { delete h; }
 int gt_op_create(gt_stack* st, const gt_op_desc* d, gt_op** out) {
     return guard([&] {
         need(st && d && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
         *out = nullptr;
         need(d->r_sheet && d->dinv_sheet, "per-sheet impedances missing");
         need(st->nbr == 0 || (d->z_tail && d->dinv_tail), "branch impedances missing");
@@ -522,11 +557,16 @@ int gt_op_create(gt_stack* st, const gt_op_desc* d, gt_op** out) {
     });
 }
 
-void gt_op_destroy(gt_op* op) { delete op; }
+void gt_op_destroy(gt_op* op) {
+    if (!op) return;
+    std::lock_guard<std::mutex> lk(op->st->ctx->mu);
+    delete op;
+}
 
 int gt_matvec(gt_op* op, const void* x_dev, void* y_dev) {
     return guard([&] {
         need(op && x_dev && y_dev && x_dev != y_dev, "NULL or aliased argument");
+        std::lock_guard<std::mutex> lk(op->st->ctx->mu);
         const zd* x = (const zd*)x_dev;
         zd* y = (zd*)y_dev;
         ops_matvec(op->st, 1, &op, &x, &y);
@@ -536,6 +576,7 @@ int gt_matvec(gt_op* op, const void* x_dev, void* y_dev) {
 int gt_precond(gt_op* op, const void* r_dev, void* z_dev) {
     return guard([&] {
         need(op && r_dev && z_dev && r_dev != z_dev, "NULL or aliased argument");
+        std::lock_guard<std::mutex> lk(op->st->ctx->mu);
         need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         const zd* r = (const zd*)r_dev;
         zd* z = (zd*)z_dev;
@@ -546,6 +587,7 @@ int gt_precond(gt_op* op, const void* r_dev, void* z_dev) {
 int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_params* p, gt_gmres_result* res) {
     return guard([&] {
         need(op && b_host && x_host && p && res, "NULL argument");
+        std::lock_guard<std::mutex> lk(op->st->ctx->mu);
         need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
         GT_HIP(hipSetDevice(op->st->ctx->device));
@@ -564,6 +606,7 @@ int gt_gmres(gt_op* op, const double* b_host, double* x_host, const gt_gmres_par
 int gt_gmres_dev(gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* p, gt_gmres_result* res) {
     return guard([&] {
         need(op && b_dev && x_dev && p && res && b_dev != x_dev, "NULL or aliased argument");
+        std::lock_guard<std::mutex> lk(op->st->ctx->mu);
         need(op->st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
         GT_HIP(hipSetDevice(op->st->ctx->device));
@@ -577,6 +620,7 @@ int gt_gmres_batch(gt_stack* st, int32_t nr, gt_op* const* ops, const void* cons
                    const gt_gmres_params* p, gt_gmres_result* res) {
     return guard([&] {
         need(st && ops && b_dev && x_dev && p && res, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
         need(nr >= 1 && nr <= KMAX, "1 to 16 right-hand sides per batch");
         need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");

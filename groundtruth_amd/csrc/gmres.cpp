@@ -27,10 +27,15 @@ constexpr double PROGRESS = 0.7;
 
 enum State { START, ARNOLDI, DONE };
 
+// A device pointer in the role of the buffers it replaces (`.p`)
+struct Vec {
+    zd* p = nullptr;
+};
+
 struct Rhs {
     gt_op* op = nullptr;
     const zd* b = nullptr;                 // right-hand side (device)
-    DevBuf<zd> V, w, x, xt, r, t, u;       // basis (restart + 1 rows) and vectors
+    Vec V, w, x, xt, r, t, u;              // basis (restart + 1 rows) and vectors, in the context's workspace
     State state = START;
     bool conv = false;
     double bnorm = 0, last = 0, aim = 0, resid = 0, hn = 0;
@@ -39,36 +44,80 @@ struct Rhs {
     std::vector<double> cs;
 };
 
+}  // namespace
+
+// The device (and pinned host) memory of a GMRES batch. The context keeps it between calls, grown to the largest
+// vector length, basis and batch it has served: a solve allocates nothing once its size has been seen.
+struct GmresWork {
+    size_t n = 0;                          // vector length the slots hold
+    int rows = 0;                          // basis rows the slots hold
+    int slots = 0;                         // right-hand-side slots allocated at that size
+    DevBuf<zd> V[KMAX], vec[KMAX];         // per slot: the basis, and six vectors of n
+    Krylov kr;
+    int kr_rows = 0;
+    size_t kr_n = 0;
+    DevBuf<zd> ydev;                       // combination coefficients, rows per right-hand side
+    HostBuf<cd> ybuf;
+    HostBuf<double> red;
+
+    void ensure(size_t n_, int rows_, int nr) {
+        if (n_ > n || rows_ > rows) {
+            for (int i = 0; i < slots; ++i) {
+                V[i].release();
+                vec[i].release();
+            }
+            slots = 0;
+            n = std::max(n, n_);
+            rows = std::max(rows, rows_);
+            ydev.alloc((size_t)KMAX * rows);
+            ybuf.alloc((size_t)KMAX * rows);
+        }
+        for (; slots < nr; ++slots) {
+            V[slots].alloc((size_t)rows * n);
+            vec[slots].alloc(6 * n);
+        }
+        if (kr_n != n_ || kr_rows != rows_) {
+            kr.init((int)n_, rows_);
+            red.alloc((size_t)KMAX * kr.P);
+            kr_n = n_;
+            kr_rows = rows_;
+        }
+    }
+};
+
+namespace {
+
 struct Batch {
     gt_stack* st;
     hipStream_t s;
     int n, restart, nr;
     gt_gmres_params p;
     std::vector<Rhs> R;
-    Krylov kr;
-    DevBuf<zd> ydev;                       // combination coefficients, (restart + 1) per right-hand side
-    HostBuf<cd> ybuf;
-    HostBuf<double> red;
+    GmresWork& gw;
+    Krylov& kr;
+    DevBuf<zd>& ydev;
+    HostBuf<cd>& ybuf;
+    HostBuf<double>& red;
 
-    Batch(gt_stack* st_, int nr_, int restart_, const gt_gmres_params& p_)
-        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), nr(nr_), p(p_), R(nr_) {
-        for (Rhs& q : R) {
-            q.V.alloc((size_t)(restart + 1) * n);
-            q.w.alloc(n);
-            q.x.alloc(n);
-            q.xt.alloc(n);
-            q.r.alloc(n);
-            q.t.alloc(n);
-            q.u.alloc(n);
+    Batch(gt_stack* st_, int nr_, int restart_, const gt_gmres_params& p_, GmresWork& gw_)
+        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), nr(nr_), p(p_), R(nr_), gw(gw_),
+          kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
+        gw.ensure((size_t)n, restart + 1, nr);
+        for (int i = 0; i < nr; ++i) {
+            Rhs& q = R[i];
+            q.V.p = gw.V[i].p;
+            zd* v = gw.vec[i].p;
+            q.w.p = v;
+            q.x.p = v + (size_t)n;
+            q.xt.p = v + 2 * (size_t)n;
+            q.r.p = v + 3 * (size_t)n;
+            q.t.p = v + 4 * (size_t)n;
+            q.u.p = v + 5 * (size_t)n;
             q.H.assign((size_t)(restart + 1) * restart, 0);
             q.sn.assign(restart, 0);
             q.g.assign(restart + 1, 0);
             q.cs.assign(restart, 0);
         }
-        kr.init(n, restart + 1);
-        ydev.alloc((size_t)KMAX * (restart + 1));
-        ybuf.alloc((size_t)KMAX * (restart + 1));
-        red.alloc((size_t)KMAX * kr.P);
     }
 
     zd* row(Rhs& q, int j) { return q.V.p + (size_t)j * n; }
@@ -416,7 +465,8 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
                  gt_gmres_result* res) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     const int n = st->nb + st->nk;
-    Batch bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), p);
+    if (!st->ctx->gmres) st->ctx->gmres.reset(new GmresWork);
+    Batch bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), p, *st->ctx->gmres);
     for (int i = 0; i < nr; ++i) {
         if (ops[i]->st != st) fail(GT_ERR_ARG, "a batch's operators must share one stack");
         bt.R[i].op = ops[i];
@@ -437,3 +487,8 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
 }
 
 }  // namespace gt
+
+gt_ctx_s::gt_ctx_s() = default;
+gt_ctx_s::~gt_ctx_s() {
+    for (hipEvent_t e : lp_events) (void)hipEventDestroy(e);
+}

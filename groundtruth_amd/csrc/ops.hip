@@ -137,7 +137,25 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const zd* const* x, zd*
     }
     LpArgs<double> la = st->lp_args(1);
     la.rhs = rs;
-    if (st->ns > 0) launch_lp(la, st->shape, s);
+    if (st->ns > 0) {
+        gt_ctx* c = st->ctx;
+        if (c->lp_timing) {
+            if (c->lp_used * 2 + 2 > c->lp_events.size()) {
+                for (int e = 0; e < 2; ++e) {
+                    hipEvent_t ev;
+                    GT_HIP(hipEventCreate(&ev));
+                    c->lp_events.push_back(ev);
+                }
+            }
+            GT_HIP(hipEventRecord(c->lp_events[2 * c->lp_used], s));
+            launch_lp(la, st->shape, s);
+            GT_HIP(hipEventRecord(c->lp_events[2 * c->lp_used + 1], s));
+            c->lp_used++;
+            c->lp_rhs += nr;
+        } else {
+            launch_lp(la, st->shape, s);
+        }
+    }
     TailArgs<double> ta{st->ns, st->nb, st->nbr, st->nk, st->a_keep.p, st->b_keep.p, st->adj_ptr.p, st->adj.p, rs};
     launch_tail_bottom(ta, s);
 }

@@ -5,16 +5,32 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "gt_common.hpp"
 #include "lp.hpp"
 #include "schur.hpp"
 
+namespace gt {
+struct GmresWork;
+}
+
 struct gt_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t lds_max = 65536;
+    // every call that works on this context's stream and scratch holds it (calls from several host threads run one
+    // at a time)
+    std::mutex mu;
+    std::unique_ptr<gt::GmresWork> gmres;     // GMRES device memory kept between calls (gmres.cpp)
+    // L_p launch timing (gt_ctx_lp_timing): an event pair on the stream around every L_p launch while enabled
+    bool lp_timing = false;
+    std::vector<hipEvent_t> lp_events;
+    size_t lp_used = 0;                       // pairs recorded
+    int64_t lp_rhs = 0;                       // right-hand sides those launches carried
+    gt_ctx_s();
+    ~gt_ctx_s();
 };
 
 namespace gt {

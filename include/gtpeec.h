@@ -48,6 +48,12 @@ void gt_ctx_destroy(gt_ctx* ctx);
 int gt_ctx_sync(gt_ctx* ctx);
 /* The context's hipStream_t (for callers that order their own device work with it). */
 void* gt_ctx_stream(gt_ctx* ctx);
+/* Measurement (no counterpart in the reference: the per-kernel HBM figures the north star asks for). While enabled,
+ * every L_p launch of the system products on this context is bracketed by a HIP event pair on its stream;
+ * enabling (or disabling) clears the record. gt_ctx_lp_time waits for the stream and gives the launches recorded,
+ * the right-hand sides they carried and their summed duration in ms. */
+int gt_ctx_lp_timing(gt_ctx* ctx, int enable);
+int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms);
 
 /* ---------------------------------------------------------------- stack
  * Replaces gpu.py:617-666 `_stack_data` / `_orients` (kernels, strip indices and incidence uploaded once per stack
