@@ -14,6 +14,7 @@
 //                 product's strip rows: R I + jw (L_p I) - Ar V.
 // The incidence rows (branch tail and Ar^T I) run in `tail_bottom`.
 #include "fft_lds.hpp"
+#include "fft_wave.hpp"
 #include "lp.hpp"
 
 namespace gt {
@@ -104,6 +105,89 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     }
 }
 
+// Pass B with the transforms in registers (column length P0 = 64 E, S <= 4 sheets): one wave per column j. Lane l
+// holds rows l + 64 m of every sheet, loaded straight from T1 (whole contiguous columns); the forward FFT, the
+// kernel multiply-add over source sheets per pixel (both sheets of a pixel sit in the same lane) and the inverse
+// FFT run in registers with the wave's LDS exchanges between passes; rows 0:n0 go straight to T2.
+template <class T, int E, int S>
+__global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* ex = reinterpret_cast<cpx<T>*>(smem_raw);
+    constexpr int N = 64 * E;
+    int j = blockIdx.x, oi = 0;
+    if (j >= a.nblk_cols[0]) { j -= a.nblk_cols[0]; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int lane = threadIdx.x, n0 = o.n0, P1 = o.P1;
+    const cpx<T>* T1 = o.T1 + blockIdx.y * a.t_stride;
+    cpx<T>* T2 = o.T2 + blockIdx.y * a.t_stride;
+    cpx<T> v[S][E];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int i = lane + 64 * m;
+            v[s][m] = i < n0 ? T1[((size_t)s * P1 + j) * n0 + i] : cpx<T>();
+        }
+    wave_fft<T, E, S, false>(v, ex, o.tw0, lane);
+    // pixel (i, j) of the full spectrum reads quadrant (min(i, P0 - i), min(j, P1 - j)): the spectrum is even
+    const int qj = min(j, P1 - j);
+    constexpr int h0 = N / 2 + 1;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int i = lane + 64 * m;
+        const T* Kp = o.K + ((size_t)qj * h0 + min(i, N - i)) * S * S;
+        cpx<T> out[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const T k = Kp[t * S + s];
+                out[t].re += k * v[s][m].re;
+                out[t].im += k * v[s][m].im;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < S; ++t) v[t][m] = out[t];
+    }
+    wave_fft<T, E, S, true>(v, ex, o.tw0, lane);
+#pragma unroll
+    for (int t = 0; t < S; ++t)
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int i = lane + 64 * m;
+            if (i < n0) T2[((size_t)t * P1 + j) * n0 + i] = v[t][m];
+        }
+}
+
+template <class T, int E>
+bool launch_cols_w_e(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
+    const size_t lds = (64 * E + 8 * E) * sizeof(cpx<T>);
+    switch (a.S) {
+    case 1: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 1>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
+    case 2: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 2>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
+    case 3: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 3>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
+    case 4: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 4>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
+    default: return false;
+    }
+}
+
+// The register form of pass B when both orientations share a column length it has (256, 512, 1024) and S <= 4.
+template <class T>
+bool launch_cols_w(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
+    int P0 = 0;
+    for (int oi = 0; oi < 2; ++oi) {
+        if (a.nblk_cols[oi] == 0) continue;
+        if (P0 != 0 && a.o[oi].P0 != P0) return false;
+        P0 = a.o[oi].P0;
+    }
+    switch (P0) {
+    case 256: return launch_cols_w_e<T, 4>(a, ncol, nr, st);
+    case 512: return launch_cols_w_e<T, 8>(a, ncol, nr, st);
+    case 1024: return launch_cols_w_e<T, 16>(a, ncol, nr, st);
+    default: return false;
+    }
+}
+
 template <class T, int NTH, int MAXPTS>
 __global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -184,8 +268,9 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int nr = a.rhs.nr;
     hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_fwd");
-    hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols, st,
-                       a);
+    if (!launch_cols_w(a, ncol, nr, st))
+        hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols,
+                           st, a);
     GT_LAUNCH("lp_cols_mac");
     hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_inv");

@@ -166,3 +166,28 @@ def test_a_mixed_batch_of_routes_and_frequencies_gives_each_its_own_answer():
     for (ex, f), (Ib, V, its, x) in zip(jobs, many):
         one = ex.solve(f, tol=1e-10)
         assert np.array_equal(x, one.x) and its == one.iterations
+
+
+@pytest.mark.parametrize("name,n,kw", [("single_plane", 200, {}), ("two_layer", 100, {"n_vias": 4}),
+                                       ("two_layer", 200, {"n_vias": 4}), ("two_layer", 300, {"n_vias": 4}),
+                                       ("four_layer", 100, {"n_nets": 1, "n_stitch": 4})])
+def test_Lp_matvec_with_register_transforms_matches_the_oracle(name, n, kw):
+    """Column lengths 256 / 512 / 1024 and 1, 2, 4 sheets: the L_p product's column pass with its transforms in
+    registers (one wave per column) against the oracle's scipy FFTs."""
+    b = boards.scaled(name, n, n_freqs=1, **kw) if name != "single_plane" else boards.single_plane(n=n)
+    st, ref = peec.PlaneStack(b.layers, b.branches), O.PlaneStack(b.layers, b.branches)
+    assert st.Px[0] in (256, 512, 1024)
+    rng = np.random.default_rng(8)
+    I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)
+    assert rel(st.Lp_matvec(I), ref.Lp_matvec(I)) < 1e-13
+
+
+def test_Lp_matvec_with_register_transforms_three_sheets():
+    b = boards.scaled("four_layer", 100, n_freqs=1, n_nets=1, n_stitch=4)
+    layers = b.layers[:3]
+    branches = [br for br in b.branches if br.a[0] < 3 and br.b[0] < 3]
+    st, ref = peec.PlaneStack(layers, branches), O.PlaneStack(layers, branches)
+    assert st.n_sheets == 3 and st.Px[0] == 256
+    rng = np.random.default_rng(9)
+    I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)
+    assert rel(st.Lp_matvec(I), ref.Lp_matvec(I)) < 1e-13
