@@ -159,6 +159,126 @@ __global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
         }
 }
 
+// Passes A and C with the row transforms in registers (row length P1 = 64 E): a workgroup is RW waves, one row
+// each, of one sheet's tile of rows [i0, i0 + RW). Each row lives in LDS as a padded strip (xpad) that is also its
+// wave's exchange buffer. Between LDS and HBM the tile moves transposed (T1 / T2 are [sheet][j][i]): RW-row segments.
+constexpr int RW = 4;
+
+template <class T, int E>
+__global__ __launch_bounds__(64 * RW) void lp_rows_fwd_w(LpArgs<T> a, int nb0) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int N = 64 * E, NS = N + N / 8;
+    int blk = blockIdx.x, oi = 0;
+    if (blk >= nb0) { blk -= nb0; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int ntiles = (o.n0 + RW - 1) / RW;
+    const int s = blk / ntiles, i0 = (blk - s * ntiles) * RW;
+    const int nr = min(RW, o.n0 - i0);
+    const int p = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    cpx<T>* row = reinterpret_cast<cpx<T>*>(smem_raw) + w * NS;
+    for (int e = lane; e < NS; e += 64) row[e] = cpx<T>();
+    __syncthreads();
+    if (w < nr) {
+        const int kb = o.rowptr[s * o.n0 + i0 + w], ke = o.rowptr[s * o.n0 + i0 + w + 1];
+        const cpx<T>* xin = a.rhs.x[p] + o.lo;
+        for (int k = kb + lane; k < ke; k += 64) row[xpad(o.rowcol[k] & 0xffff)] = xin[k];
+    }
+    __syncthreads();
+    cpx<T> v[1][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[0][m] = row[xpad(lane + 64 * m)];
+    __syncthreads();
+    wave_fft<T, E, 1, false>(v, row, o.tw1, lane);
+#pragma unroll
+    for (int m = 0; m < E; ++m) row[xpad(lane + 64 * m)] = v[0][m];
+    __syncthreads();
+    cpx<T>* dst = o.T1 + p * a.t_stride + (size_t)s * N * o.n0 + i0;
+    const cpx<T>* tile = reinterpret_cast<const cpx<T>*>(smem_raw);
+    for (int e = threadIdx.x; e < nr * N; e += 64 * RW) {
+        const int r = e % nr, j = e / nr;
+        dst[(size_t)j * o.n0 + r] = tile[r * NS + xpad(j)];
+    }
+}
+
+template <class T, int E>
+__global__ __launch_bounds__(64 * RW) void lp_rows_inv_w(LpArgs<T> a, int nb0) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int N = 64 * E, NS = N + N / 8;
+    int blk = blockIdx.x, oi = 0;
+    if (blk >= nb0) { blk -= nb0; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int ntiles = (o.n0 + RW - 1) / RW;
+    const int t = blk / ntiles, i0 = (blk - t * ntiles) * RW;
+    const int nr = min(RW, o.n0 - i0);
+    const int p = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+    const cpx<T>* src = o.T2 + p * a.t_stride + (size_t)t * N * o.n0 + i0;
+    for (int e = threadIdx.x; e < nr * N; e += 64 * RW) {
+        const int r = e % nr, j = e / nr;
+        tile[r * NS + xpad(j)] = src[(size_t)j * o.n0 + r];
+    }
+    __syncthreads();
+    cpx<T>* row = tile + w * NS;
+    cpx<T> v[1][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[0][m] = w < nr ? row[xpad(lane + 64 * m)] : cpx<T>();
+    __syncthreads();
+    wave_fft<T, E, 1, true>(v, row, o.tw1, lane);
+#pragma unroll
+    for (int m = 0; m < E; ++m) row[xpad(lane + 64 * m)] = v[0][m];
+    __syncthreads();
+    if (w >= nr) return;
+    const int i = i0 + w;
+    const int kb = o.rowptr[t * o.n0 + i], ke = o.rowptr[t * o.n0 + i + 1];
+    cpx<T>* y = a.rhs.y[p];
+    const cpx<T>* x = a.rhs.x[p];
+    for (int k = kb + lane; k < ke; k += 64) {
+        const cpx<T> val = row[xpad(o.rowcol[k] & 0xffff)];
+        const int b = o.lo + k;
+        if (a.mode == 0) {
+            y[b] = val;
+        } else {
+            const cpx<T> xb = x[b];
+            const cpx<T> R = a.rhs.r_sheet[p][t];
+            const T wf = a.rhs.w[p];
+            cpx<T> vv = R * xb;
+            vv.re -= wf * val.im;
+            vv.im += wf * val.re;
+            const int ka = a.a_keep[b], kk = a.b_keep[b];
+            if (ka >= 0) vv = vv - x[a.nb + ka];
+            if (kk >= 0) vv = vv + x[a.nb + kk];
+            y[b] = vv;
+        }
+    }
+}
+
+// The register forms of passes A and C when both orientations share a row length they have (256, 512, 1024).
+template <class T>
+bool launch_rows_w(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st) {
+    int P1 = 0, nb[2] = {0, 0};
+    for (int oi = 0; oi < 2; ++oi) {
+        if (a.nblk_cols[oi] == 0) continue;
+        if (P1 != 0 && a.o[oi].P1 != P1) return false;
+        P1 = a.o[oi].P1;
+        nb[oi] = a.S * ((a.o[oi].n0 + RW - 1) / RW);
+    }
+    const int blocks = nb[0] + nb[1];
+    if (blocks == 0) return false;
+    const size_t lds = (size_t)RW * (P1 + P1 / 8) * sizeof(cpx<T>);
+    if (lds > 65536) return false;            // (P1 = 1024 in FP64) the LDS kernels take it
+#define GT_ROWS(E)                                                                                                \
+    if (inverse) hipLaunchKernelGGL((lp_rows_inv_w<T, E>), dim3(blocks, nr_rhs), dim3(64 * RW), lds, st, a, nb[0]); \
+    else hipLaunchKernelGGL((lp_rows_fwd_w<T, E>), dim3(blocks, nr_rhs), dim3(64 * RW), lds, st, a, nb[0]);       \
+    return true;
+    switch (P1) {
+    case 256: GT_ROWS(4)
+    case 512: GT_ROWS(8)
+    case 1024: GT_ROWS(16)
+    default: return false;
+    }
+#undef GT_ROWS
+}
+
 template <class T, int E>
 bool launch_cols_w_e(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
     const size_t lds = (64 * E + 8 * E) * sizeof(cpx<T>);
@@ -266,13 +386,15 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
     constexpr int NTH = 256;
     const int nr = a.rhs.nr;
-    hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+    if (!launch_rows_w(a, nr, false, st))
+        hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_fwd");
     if (!launch_cols_w(a, ncol, nr, st))
         hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols,
                            st, a);
     GT_LAUNCH("lp_cols_mac");
-    hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+    if (!launch_rows_w(a, nr, true, st))
+        hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_inv");
 }
 
