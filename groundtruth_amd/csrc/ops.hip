@@ -73,25 +73,46 @@ __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs a) {
     }
 }
 
-// y_p = s_scale_p (z_p - W t_p), written to the node part of each output; W is read once for the batch
+// y_p = s_scale_p (z_p - W t_p), written to the node part of each output. A thread takes one node for the whole
+// batch: each W value is read once and applied to every right-hand side (t of the batch staged in LDS, in chunks
+// of WB branches). The sums over m run in the same order for every batch size.
+constexpr int WB = 64;
+
 __global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs a) {
-    __shared__ double ts_raw[2 * 256];
+    __shared__ double ts_raw[2 * WB * KMAX];
     zd* ts = reinterpret_cast<zd*>(ts_raw);
     const int n = blockIdx.x * 256 + threadIdx.x;
-    for (int p = 0; p < a.nr; ++p) {
+    double vre[KMAX], vim[KMAX];
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p) {
+        const zd z = n < a.nk && p < a.nr ? a.z[(size_t)n * a.nr + p] : zd();
+        vre[p] = z.re;
+        vim[p] = z.im;
+    }
+    for (int m0 = 0; m0 < a.k; m0 += WB) {
+        const int mc = min(WB, a.k - m0);
         __syncthreads();
-        for (int m = threadIdx.x; m < a.k; m += 256) ts[m] = a.t[p * a.k + m];
+        for (int e = threadIdx.x; e < mc * KMAX; e += 256) {      // right-hand sides past the batch: zeros
+            const int p = e / mc, m = e - p * mc;
+            ts[m * KMAX + p] = p < a.nr ? a.t[p * a.k + m0 + m] : zd();
+        }
         __syncthreads();
         if (n < a.nk) {
-            zd v = a.z[(size_t)n * a.nr + p];
-            for (int m = 0; m < a.k; ++m) {
-                const double w = a.W[(size_t)m * a.nk + n];
-                v.re -= w * ts[m].re;
-                v.im -= w * ts[m].im;
+            for (int m = 0; m < mc; ++m) {
+                const double w = a.W[(size_t)(m0 + m) * a.nk + n];
+                const zd* tm = ts + m * KMAX;
+#pragma unroll
+                for (int p = 0; p < KMAX; ++p) {
+                    vre[p] -= w * tm[p].re;
+                    vim[p] -= w * tm[p].im;
+                }
             }
-            a.out[p][a.nb + n] = a.s_scale[p] * v;
         }
     }
+    if (n >= a.nk) return;
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p)
+        if (p < a.nr) a.out[p][a.nb + n] = a.s_scale[p] * zd(vre[p], vim[p]);
 }
 
 // x1 = D^-1 (r1 + Ar y)
