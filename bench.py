@@ -4,7 +4,9 @@ Workload (default): the two-layer board, 256 x 256 cells per layer at 0.25 mm, a
 20 stitching vias, one Manhattan route, solved at 10 frequencies log-spaced 1 MHz - 1 GHz, plain FP64, GMRES(50)
 to a relative residual of 1e-10. One step = the 10 solves (one route at every frequency) with the board and the
 right-hand sides resident, run as one batch (each solve follows the reference's GMRES and is bit-identical to its
-solve alone). value = solves per second over the timed steps.
+solve alone). value = solves per second over the timed steps. The same batch is then timed in mixed precision
+(FP32 FFT / Krylov inner solves, FP64 Arnoldi, Schur and iterative refinement to 1e-8) and reported beside it
+with each solution's relative error (and the plain run's) against an FP64 solve of the same system at tol 1e-13.
 
 Also reported: the L_p product's (the FFT matvec's) roofline. Every L_p launch of the timed steps is bracketed by
 HIP events on the library's stream (gt_ctx_lp_timing); achieved = the right-hand sides per launch x the algorithmic
@@ -31,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "PEEC solves/sec + matvec HBM GB/s; p50/p99 what-if latency vs plain FP64 at 1e-8"
+MIXED_TOL = 1e-8                # the mixed-precision run's FP64 refinement target (the workload's 1e-8)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -185,6 +188,31 @@ def main():
     n_lp, n_rhs, lp_ms = C.c_int64(), C.c_int64(), C.c_double()
     _lib.check(L.gt_ctx_lp_time(st.ctx.handle, C.byref(n_lp), C.byref(n_rhs), C.byref(lp_ms)))
     _lib.check(L.gt_ctx_lp_timing(st.ctx.handle, 0))
+
+    # the same batch in mixed precision: FP32 FFT / Krylov inner solves, FP64 Arnoldi, Schur and refinement to 1e-8
+    x_mix = [torch.zeros_like(r) for r in rhs_dev]
+
+    def step_mixed():
+        res = st.gmres_batch_dev(ops, rhs_dev, x_mix, MIXED_TOL, restart=board.restart, precision="mixed")
+        for op, r in zip(ops, res):
+            if not r.converged:
+                raise RuntimeError(f"mixed-precision refinement did not reach {MIXED_TOL:g} at {op.f:.4g} Hz")
+        return [int(r.iterations) for r in res], [float(r.residual) for r in res]
+
+    for _ in range(args.warmup):
+        step_mixed()
+    torch.cuda.synchronize()
+    t0m = time.perf_counter()
+    for _ in range(args.steps):
+        its_mix, res_mix = step_mixed()
+    torch.cuda.synchronize()
+    dt_mix = time.perf_counter() - t0m
+    # an FP64 reference solution at tol 1e-13 (untimed) to state both runs' errors against
+    x_ref = [torch.zeros_like(r) for r in rhs_dev]
+    st.gmres_batch_dev(ops, rhs_dev, x_ref, 1e-13, restart=board.restart)
+    torch.cuda.synchronize()
+    err_mix = [float((xm - xr).norm() / xr.norm()) for xm, xr in zip(x_mix, x_ref)]
+    err_fp64 = [float((xp - xr).norm() / xr.norm()) for xp, xr in zip(x_dev, x_ref)]
     lp_us = lp_ms.value / max(1, n_lp.value) * 1e3           # average L_p launch in the timed steps
     lp_rhs = n_rhs.value / max(1, n_lp.value)                # right-hand sides per launch
     solves = args.steps * len(board.freqs)
@@ -227,6 +255,13 @@ def main():
                      "single_rhs": {"us_per_launch": t_lp * 1e6, "achieved": B / t_lp / 1e9,
                                     "bytes_per_launch": B}},
     }
+    out["mixed"] = {"value": solves / dt_mix, "unit": "solves/s", "ms_per_step": dt_mix / args.steps * 1e3,
+                    "speedup_vs_fp64": (solves / dt_mix) / (solves / dt), "tol": MIXED_TOL,
+                    "iterations": its_mix, "residual": res_mix,
+                    "rel_error": err_mix, "rel_error_fp64_run": err_fp64,
+                    "error_reference": "the plain FP64 solve of the same system at tol 1e-13",
+                    "precision": "FP32 L_p FFT convolution and Krylov vectors; FP64 Arnoldi dot products, Givens, "
+                                 "Schur preconditioner and iterative refinement"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])
     print(json.dumps(out))

@@ -9,6 +9,8 @@
 namespace gt {
 void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x, const gt_gmres_params& p,
                  gt_gmres_result* res);
+void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
+                       const gt_gmres_params& p, double inner_tol, int max_refine, gt_gmres_result* res);
 }
 
 using namespace gt;
@@ -550,6 +552,13 @@ int gt_op_create(gt_stack* st, const gt_op_desc* d, gt_op** out) {
         up(op->z_tail, d->z_tail, st->nbr);
         up(op->dinv_tail, d->dinv_tail, st->nbr);
         up(op->q, d->q, (size_t)d->k * d->k);
+        auto upf = [&](DevBuf<zf>& dst, const double* src, size_t cnt) {
+            std::vector<zf> v(std::max<size_t>(1, cnt));
+            for (size_t i = 0; i < cnt; ++i) v[i] = zf((float)src[2 * i], (float)src[2 * i + 1]);
+            dst.upload(v, s);
+        };
+        upf(op->r_sheet_f, d->r_sheet, st->S);
+        upf(op->z_tail_f, d->z_tail, st->nbr);
         op->s_scale = zd(d->s_scale[0], d->s_scale[1]);
         op->k = d->k;
         GT_HIP(hipStreamSynchronize(s));
@@ -634,6 +643,28 @@ int gt_gmres_batch(gt_stack* st, int32_t nr, gt_op* const* ops, const void* cons
         }
         GT_HIP(hipSetDevice(st->ctx->device));
         gmres_batch(st, nr, ops, b.data(), x.data(), *p, res);
+    });
+}
+
+int gt_gmres_batch_mixed(gt_stack* st, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
+                         const gt_gmres_params* p, double inner_tol, int32_t max_refine, gt_gmres_result* res) {
+    return guard([&] {
+        need(st && ops && b_dev && x_dev && p && res, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        need(nr >= 1 && nr <= KMAX, "1 to 16 right-hand sides per batch");
+        need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
+        need(max_refine >= 1 && max_refine <= 64, "1 to 64 refinement rounds");
+        std::vector<const zd*> b(nr);
+        std::vector<zd*> x(nr);
+        for (int i = 0; i < nr; ++i) {
+            need(ops[i] && ops[i]->st == st, "every operator must belong to the stack");
+            need(b_dev[i] && x_dev[i] && b_dev[i] != x_dev[i], "NULL or aliased vector");
+            b[i] = (const zd*)b_dev[i];
+            x[i] = (zd*)x_dev[i];
+        }
+        GT_HIP(hipSetDevice(st->ctx->device));
+        gmres_batch_mixed(st, nr, ops, b.data(), x.data(), *p, inner_tol, max_refine, res);
     });
 }
 

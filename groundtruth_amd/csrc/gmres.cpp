@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "krylov.hpp"
@@ -28,14 +29,16 @@ constexpr double PROGRESS = 0.7;
 enum State { START, ARNOLDI, DONE };
 
 // A device pointer in the role of the buffers it replaces (`.p`)
+template <class T>
 struct Vec {
-    zd* p = nullptr;
+    cpx<T>* p = nullptr;
 };
 
+template <class T>
 struct Rhs {
     gt_op* op = nullptr;
-    const zd* b = nullptr;                 // right-hand side (device)
-    Vec V, w, x, xt, r, t, u;              // basis (restart + 1 rows) and vectors, in the context's workspace
+    const cpx<T>* b = nullptr;             // right-hand side (device)
+    Vec<T> V, w, x, xt, r, t, u;              // basis (restart + 1 rows) and vectors, in the context's workspace
     State state = START;
     bool conv = false;
     double bnorm = 0, last = 0, aim = 0, resid = 0, hn = 0;
@@ -59,6 +62,18 @@ struct GmresWork {
     DevBuf<zd> ydev;                       // combination coefficients, rows per right-hand side
     HostBuf<cd> ybuf;
     HostBuf<double> red;
+    DevBuf<zd> ref[KMAX];                  // mixed precision: the FP64 refinement vectors of each slot (4 n)
+    size_t ref_n = 0;
+    int ref_slots = 0;
+
+    void ensure_ref(size_t n_, int nr) {
+        if (n_ > ref_n) {
+            for (int i = 0; i < ref_slots; ++i) ref[i].release();
+            ref_slots = 0;
+            ref_n = n_;
+        }
+        for (; ref_slots < nr; ++ref_slots) ref[ref_slots].alloc(4 * ref_n);
+    }
 
     void ensure(size_t n_, int rows_, int nr) {
         if (n_ > n || rows_ > rows) {
@@ -87,12 +102,16 @@ struct GmresWork {
 
 namespace {
 
+// T: the storage of the vectors and basis (double; float for the mixed-precision inner solve). The Hessenberg
+// matrix, the rotations and every reduction are double.
+template <class T>
 struct Batch {
+    using cz = cpx<T>;
     gt_stack* st;
     hipStream_t s;
     int n, restart, nr;
     gt_gmres_params p;
-    std::vector<Rhs> R;
+    std::vector<Rhs<T>> R;
     GmresWork& gw;
     Krylov& kr;
     DevBuf<zd>& ydev;
@@ -104,9 +123,9 @@ struct Batch {
           kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
         gw.ensure((size_t)n, restart + 1, nr);
         for (int i = 0; i < nr; ++i) {
-            Rhs& q = R[i];
-            q.V.p = gw.V[i].p;
-            zd* v = gw.vec[i].p;
+            Rhs<T>& q = R[i];
+            q.V.p = reinterpret_cast<cz*>(gw.V[i].p);
+            cz* v = reinterpret_cast<cz*>(gw.vec[i].p);
             q.w.p = v;
             q.x.p = v + (size_t)n;
             q.xt.p = v + 2 * (size_t)n;
@@ -120,7 +139,7 @@ struct Batch {
         }
     }
 
-    zd* row(Rhs& q, int j) { return q.V.p + (size_t)j * n; }
+    cz* row(Rhs<T>& q, int j) { return q.V.p + (size_t)j * n; }
 
     void fetch(int cnt) {
         GT_HIP(hipMemcpyAsync(red.p, kr.red.p, sizeof(double) * kr.P * cnt, hipMemcpyDeviceToHost, s));
@@ -129,24 +148,24 @@ struct Batch {
     double val(int slot_in_batch, int c) const { return red[(size_t)slot_in_batch * kr.P + c]; }
 
     // out_q = K_q M_q^-1 v_q for the listed right-hand sides (t_q the intermediate)
-    void apply(const std::vector<int>& who, const std::vector<const zd*>& in, const std::vector<zd*>& out) {
+    void apply(const std::vector<int>& who, const std::vector<const cz*>& in, const std::vector<cz*>& out) {
         const int c = (int)who.size();
         std::vector<gt_op*> ops(c);
-        std::vector<zd*> tmp(c);
+        std::vector<cz*> tmp(c);
         for (int i = 0; i < c; ++i) {
             ops[i] = R[who[i]].op;
             tmp[i] = R[who[i]].t.p;
             R[who[i]].products++;
         }
-        ops_precond(st, c, ops.data(), in.data(), tmp.data());
-        std::vector<const zd*> tin(tmp.begin(), tmp.end());
-        ops_matvec(st, c, ops.data(), tin.data(), out.data());
+        ops_precond<T>(st, c, ops.data(), in.data(), tmp.data());
+        std::vector<const cz*> tin(tmp.begin(), tmp.end());
+        ops_matvec<T>(st, c, ops.data(), tin.data(), out.data());
     }
 
-    void norms(const std::vector<int>& who, const std::vector<const zd*>& v, std::vector<double>& out) {
-        KrSet k{};
+    void norms(const std::vector<int>& who, const std::vector<const cz*>& v, std::vector<double>& out) {
+        KrSetT<T> k{};
         k.nr = (int)who.size();
-        for (int i = 0; i < k.nr; ++i) k.w[i] = const_cast<zd*>(v[i]);
+        for (int i = 0; i < k.nr; ++i) k.w[i] = const_cast<cz*>(v[i]);
         kr.norm2(k, s);
         fetch(k.nr);
         out.resize(who.size());
@@ -154,9 +173,9 @@ struct Batch {
     }
 
     // dst_q = from_q + V_q[0:k_q]^T y_q   (from_q copied first when distinct)
-    void combine(const std::vector<int>& who, const std::vector<std::vector<cd>>& ys, const std::vector<zd*>& dst,
-                 const std::vector<const zd*>& from) {
-        VecOps c{};
+    void combine(const std::vector<int>& who, const std::vector<std::vector<cd>>& ys, const std::vector<cz*>& dst,
+                 const std::vector<const cz*>& from) {
+        VecOpsT<T> c{};
         c.nr = (int)who.size();
         bool any = false;
         for (int i = 0; i < c.nr; ++i) {
@@ -175,7 +194,7 @@ struct Batch {
         k_combine(c, ydev.p, restart + 1, n, s);
     }
 
-    std::vector<cd> back_substitute(const Rhs& q, int k) const {
+    std::vector<cd> back_substitute(const Rhs<T>& q, int k) const {
         std::vector<cd> y(k);
         const int ld = restart;
         for (int i = k - 1; i >= 0; --i) {
@@ -187,12 +206,12 @@ struct Batch {
     }
 
     // r_q = b_q - K_q M_q^-1 x_q for the listed right-hand sides
-    void residuals(const std::vector<int>& who, const std::vector<zd*>& xs) {
-        std::vector<const zd*> in(xs.begin(), xs.end());
-        std::vector<zd*> out;
+    void residuals(const std::vector<int>& who, const std::vector<cz*>& xs) {
+        std::vector<const cz*> in(xs.begin(), xs.end());
+        std::vector<cz*> out;
         for (int q : who) out.push_back(R[q].u.p);
         apply(who, in, out);
-        VecOps v{};
+        VecOpsT<T> v{};
         v.nr = (int)who.size();
         for (int i = 0; i < v.nr; ++i) {
             v.dst[i] = R[who[i]].r.p;
@@ -211,7 +230,7 @@ struct Batch {
         for (int i = 0; i < nr; ++i) all[i] = i;
         // x = 0, r = b, |b|
         {
-            VecOps v{};
+            VecOpsT<T> v{};
             v.nr = nr;
             for (int i = 0; i < nr; ++i) v.dst[i] = R[i].x.p;
             k_zero(v, n, s);
@@ -220,12 +239,12 @@ struct Batch {
                 v.a[i] = R[i].b;
             }
             k_copy(v, n, s);
-            std::vector<const zd*> bs;
+            std::vector<const cz*> bs;
             for (int i = 0; i < nr; ++i) bs.push_back(R[i].b);
             std::vector<double> b0;
             norms(all, bs, b0);
             for (int i = 0; i < nr; ++i) {
-                Rhs& q = R[i];
+                Rhs<T>& q = R[i];
                 q.bnorm = p.bnorm > 0 ? p.bnorm : b0[i];
                 q.last = std::numeric_limits<double>::infinity();
                 if (q.bnorm == 0 || b0[i] <= tol * q.bnorm) {
@@ -241,13 +260,13 @@ struct Batch {
             for (int i = 0; i < nr; ++i)
                 if (R[i].state == START) S.push_back(i);
             if (!S.empty()) {
-                std::vector<const zd*> rs;
+                std::vector<const cz*> rs;
                 for (int q : S) rs.push_back(R[q].r.p);
                 std::vector<double> beta;
                 norms(S, rs, beta);
-                VecOps v{};
+                VecOpsT<T> v{};
                 for (size_t i = 0; i < S.size(); ++i) {
-                    Rhs& q = R[S[i]];
+                    Rhs<T>& q = R[S[i]];
                     const double b = beta[i];
                     q.resid = b / q.bnorm;
                     if (b <= tol * q.bnorm) { q.state = DONE; q.conv = true; continue; }
@@ -282,8 +301,8 @@ struct Batch {
             if (A.empty()) break;
             // ---- one Arnoldi step each: w = K M^-1 V[j], one CGS pass, |w| before and after
             {
-                std::vector<const zd*> in;
-                std::vector<zd*> out;
+                std::vector<const cz*> in;
+                std::vector<cz*> out;
                 for (int q : A) {
                     in.push_back(row(R[q], R[q].j));
                     out.push_back(R[q].w.p);
@@ -295,10 +314,10 @@ struct Batch {
             for (int q : A) jmax = std::max(jmax, R[q].j);
             const bool fuse = Krylov::fused(jmax + 1);
             auto project = [&](const std::vector<int>& who) {
-                KrSet k{};
+                KrSetT<T> k{};
                 k.nr = (int)who.size();
                 for (int i = 0; i < k.nr; ++i) {
-                    Rhs& q = R[who[i]];
+                    Rhs<T>& q = R[who[i]];
                     k.V[i] = q.V.p;
                     k.m[i] = q.j + 1;
                     k.w[i] = q.w.p;
@@ -317,7 +336,7 @@ struct Batch {
             // path the device took the test and the re-pass already; its decision is read back)
             std::vector<int> D;
             for (size_t i = 0; i < A.size(); ++i) {
-                Rhs& q = R[A[i]];
+                Rhs<T>& q = R[A[i]];
                 q.h.assign(q.j + 1, 0);
                 for (int r = 0; r <= q.j; ++r) q.h[r] = cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
                 const double w_norm = std::sqrt(val((int)i, kr.slot_w()));
@@ -335,7 +354,7 @@ struct Batch {
             if (!D.empty()) {                 // DGKS: a second pass only after cancellation
                 project(D);
                 for (size_t i = 0; i < D.size(); ++i) {
-                    Rhs& q = R[D[i]];
+                    Rhs<T>& q = R[D[i]];
                     for (int r = 0; r <= q.j; ++r) q.h[r] += cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
                     q.hn = std::sqrt(val((int)i, kr.slot_w_after()));
                 }
@@ -343,7 +362,7 @@ struct Batch {
             // ---- Givens rotations and the decisions
             std::vector<int> C, E, Nx;        // true-residual checks, cycle ends, next basis vector
             for (int qi : A) {
-                Rhs& q = R[qi];
+                Rhs<T>& q = R[qi];
                 const int j = q.j, ld = restart;
                 std::vector<cd> col(j + 2);
                 for (int i = 0; i <= j; ++i) col[i] = q.h[i];
@@ -384,8 +403,8 @@ struct Batch {
             // ---- true-residual checks: xt = x + V y, r = b - K M^-1 xt
             if (!C.empty()) {
                 std::vector<std::vector<cd>> ys;
-                std::vector<zd*> dst;
-                std::vector<const zd*> from;
+                std::vector<cz*> dst;
+                std::vector<const cz*> from;
                 for (int qi : C) {
                     ys.push_back(back_substitute(R[qi], R[qi].k));
                     dst.push_back(R[qi].xt.p);
@@ -393,13 +412,13 @@ struct Batch {
                 }
                 combine(C, ys, dst, from);
                 residuals(C, dst);
-                std::vector<const zd*> rs;
+                std::vector<const cz*> rs;
                 for (int qi : C) rs.push_back(R[qi].r.p);
                 std::vector<double> rt;
                 norms(C, rs, rt);
-                VecOps cp{};
+                VecOpsT<T> cp{};
                 for (size_t i = 0; i < C.size(); ++i) {
-                    Rhs& q = R[C[i]];
+                    Rhs<T>& q = R[C[i]];
                     q.resid = rt[i] / q.bnorm;
                     if (rt[i] <= tol * q.bnorm) {
                         cp.dst[cp.nr] = q.x.p;
@@ -422,16 +441,16 @@ struct Batch {
             }
             // ---- the next basis vector, V[j + 1] = w / hn; a full cycle ends
             if (!Nx.empty()) {
-                VecOps v{};
+                VecOpsT<T> v{};
                 for (int qi : Nx) {
-                    Rhs& q = R[qi];
+                    Rhs<T>& q = R[qi];
                     v.dst[v.nr] = row(q, q.j + 1);
                     v.a[v.nr] = q.w.p;
                     v.f[v.nr++] = 1.0 / q.hn;
                 }
                 k_scale(v, n, s);
                 for (int qi : Nx) {
-                    Rhs& q = R[qi];
+                    Rhs<T>& q = R[qi];
                     if (q.j + 1 >= restart) E.push_back(qi);
                     else q.j++;
                 }
@@ -439,10 +458,10 @@ struct Batch {
             // ---- cycle ends: x += V y (least squares by back substitution), r = b - K M^-1 x
             if (!E.empty()) {
                 std::vector<std::vector<cd>> ys;
-                std::vector<zd*> dst;
-                std::vector<const zd*> from;
+                std::vector<cz*> dst;
+                std::vector<const cz*> from;
                 for (int qi : E) {
-                    Rhs& q = R[qi];
+                    Rhs<T>& q = R[qi];
                     int k = q.k;
                     while (k > 0 && q.H[(size_t)(k - 1) * restart + (k - 1)] == cd(0)) --k;
                     ys.push_back(k ? back_substitute(q, k) : std::vector<cd>());
@@ -466,7 +485,7 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     const int n = st->nb + st->nk;
     if (!st->ctx->gmres) st->ctx->gmres.reset(new GmresWork);
-    Batch bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), p, *st->ctx->gmres);
+    Batch<double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), p, *st->ctx->gmres);
     for (int i = 0; i < nr; ++i) {
         if (ops[i]->st != st) fail(GT_ERR_ARG, "a batch's operators must share one stack");
         bt.R[i].op = ops[i];
@@ -476,13 +495,161 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
     // x = M^-1 y (zero when y is)
     std::vector<const zd*> ys;
     for (int i = 0; i < nr; ++i) ys.push_back(bt.R[i].x.p);
-    ops_precond(st, nr, ops, ys.data(), x);
+    ops_precond<double>(st, nr, ops, ys.data(), x);
     GT_HIP(hipStreamSynchronize(bt.s));
     for (int i = 0; i < nr; ++i) {
         res[i].iterations = bt.R[i].its;
         res[i].converged = bt.R[i].conv ? 1 : 0;
         res[i].products = bt.R[i].products;
         res[i].residual = bt.R[i].resid;
+    }
+}
+
+void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
+                       const gt_gmres_params& p, double inner_tol, int max_refine, gt_gmres_result* res) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
+    if (!(inner_tol > 0 && inner_tol < 1)) fail(GT_ERR_ARG, "the inner tolerance must lie in (0, 1)");
+    for (int i = 0; i < nr; ++i)
+        if (ops[i]->st != st) fail(GT_ERR_ARG, "a batch's operators must share one stack");
+    const int n = st->nb + st->nk;
+    hipStream_t s = st->ctx->stream;
+    if (!st->ctx->gmres) st->ctx->gmres.reset(new GmresWork);
+    GmresWork& gw = *st->ctx->gmres;
+    const int restart = p.restart > 0 ? p.restart : default_restart(n);
+    gw.ensure((size_t)n, restart + 1, nr);
+    gw.ensure_ref((size_t)n, nr);
+    // per right-hand side, in its FP64 refinement buffer: r (residual), d (correction), u (K x), and r in FP32
+    std::vector<zd*> r(nr), d(nr), u(nr);
+    std::vector<zf*> r32(nr);
+    for (int i = 0; i < nr; ++i) {
+        zd* base = gw.ref[i].p;
+        r[i] = base;
+        d[i] = base + (size_t)n;
+        u[i] = base + 2 * (size_t)n;
+        r32[i] = reinterpret_cast<zf*>(base + 3 * (size_t)n);
+    }
+    Krylov& kr = gw.kr;
+    auto norms = [&](const std::vector<int>& who, const std::vector<zd*>& v) {
+        KrSet k{};
+        k.nr = (int)who.size();
+        for (int i = 0; i < k.nr; ++i) k.w[i] = v[who[i]];
+        kr.norm2(k, s);
+        GT_HIP(hipMemcpyAsync(gw.red.p, kr.red.p, sizeof(double) * kr.P * k.nr, hipMemcpyDeviceToHost, s));
+        spin_sync(s);
+        std::vector<double> out(k.nr);
+        for (int i = 0; i < k.nr; ++i) out[i] = std::sqrt(gw.red[(size_t)i * kr.P + kr.slot_norm()]);
+        return out;
+    };
+    auto subset = [&](const std::vector<int>& who, auto&& vec) {
+        using E = typename std::decay<decltype(vec[0])>::type;
+        std::vector<E> out;
+        for (int q : who) out.push_back(vec[q]);
+        return out;
+    };
+    std::vector<int> all(nr);
+    for (int i = 0; i < nr; ++i) all[i] = i;
+    // x = 0, r = b
+    {
+        VecOps v{};
+        v.nr = nr;
+        for (int i = 0; i < nr; ++i) {
+            v.dst[i] = x[i];
+        }
+        k_zero(v, n, s);
+        for (int i = 0; i < nr; ++i) {
+            v.dst[i] = r[i];
+            v.a[i] = b[i];
+        }
+        k_copy(v, n, s);
+    }
+    const std::vector<double> b0 = norms(all, r);
+    std::vector<double> bnorm(nr), last(nr, std::numeric_limits<double>::infinity());
+    std::vector<int> its(nr, 0), prods(nr, 0), conv(nr, 0);
+    std::vector<double> resid(nr, 0.0);
+    std::vector<int> act;
+    for (int i = 0; i < nr; ++i) {
+        bnorm[i] = p.bnorm > 0 ? p.bnorm : b0[i];
+        resid[i] = bnorm[i] > 0 ? b0[i] / bnorm[i] : 0.0;
+        if (bnorm[i] == 0 || b0[i] <= p.tol * bnorm[i]) conv[i] = 1;
+        else act.push_back(i);
+    }
+    last = b0;
+    for (int round = 0; round < max_refine && !act.empty(); ++round) {
+        const int na = (int)act.size();
+        const std::vector<gt_op*> aop = subset(act, std::vector<gt_op*>(ops, ops + nr));
+        // the inner solve in FP32: (K M^-1) y = r to inner_tol, y in the inner batch's x
+        k_to_float(na, subset(act, r32).data(), subset(act, r).data(), n, s);
+        gt_gmres_params ip = p;
+        ip.tol = inner_tol;
+        ip.bnorm = 0;
+        ip.restart = restart;
+        Batch<float> bt(st, na, restart, ip, gw);
+        for (int i = 0; i < na; ++i) {
+            bt.R[i].op = aop[i];
+            bt.R[i].b = r32[act[i]];
+        }
+        bt.run();
+        // d = M^-1 y in FP64 (y widened), x += d, u = K x, r = b - u
+        {
+            VecOps v{};
+            v.nr = na;
+            for (int i = 0; i < na; ++i) v.dst[i] = u[act[i]];
+            k_zero(v, n, s);
+            std::vector<const zf*> ys;
+            for (int i = 0; i < na; ++i) ys.push_back(bt.R[i].x.p);
+            k_add_float(na, subset(act, u).data(), ys.data(), n, s);
+            std::vector<const zd*> uin;
+            for (int q : act) uin.push_back(u[q]);
+            ops_precond<double>(st, na, aop.data(), uin.data(), subset(act, d).data());
+            for (int i = 0; i < na; ++i) {
+                v.dst[i] = x[act[i]];
+                v.a[i] = x[act[i]];
+                v.b[i] = d[act[i]];
+            }
+            // x + d as a - (-d): the correction enters through the sum kernel
+            VecOps w{};
+            w.nr = na;
+            for (int i = 0; i < na; ++i) {
+                w.dst[i] = d[act[i]];
+                w.a[i] = d[act[i]];
+                w.f[i] = -1.0;
+            }
+            k_scale(w, n, s);
+            k_sub(v, n, s);
+            std::vector<const zd*> xin;
+            for (int q : act) xin.push_back(x[q]);
+            ops_matvec<double>(st, na, aop.data(), xin.data(), subset(act, u).data());
+            for (int i = 0; i < na; ++i) {
+                v.dst[i] = r[act[i]];
+                v.a[i] = b[act[i]];
+                v.b[i] = u[act[i]];
+            }
+            k_sub(v, n, s);
+        }
+        const std::vector<double> rn = norms(act, r);
+        std::vector<int> next;
+        for (int i = 0; i < na; ++i) {
+            const int q = act[i];
+            its[q] += bt.R[i].its;
+            prods[q] += bt.R[i].products + 2;
+            resid[q] = rn[i] / bnorm[q];
+            if (rn[i] <= p.tol * bnorm[q]) {
+                conv[q] = 1;
+            } else if (rn[i] > 0.5 * last[q]) {
+                conv[q] = 0;                   // the refinement stalled
+            } else {
+                next.push_back(q);
+            }
+            last[q] = rn[i];
+        }
+        act = next;
+    }
+    GT_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < nr; ++i) {
+        res[i].iterations = its[i];
+        res[i].converged = conv[i];
+        res[i].products = prods[i];
+        res[i].residual = resid[i];
     }
 }
 

@@ -10,6 +10,12 @@ namespace {
 constexpr int NTH = 256;
 constexpr int G = 16;             // basis rows per sweep of the multi-dot
 
+// a stored value as double
+__device__ __forceinline__ zd ldz(const zd& v) { return v; }
+__device__ __forceinline__ zd ldz(const cpx<float>& v) { return zd(v.re, v.im); }
+template <class T>
+__device__ __forceinline__ cpx<T> stz(double re, double im) { return cpx<T>((T)re, (T)im); }
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
@@ -30,11 +36,13 @@ __device__ __forceinline__ double block_sum(double val, double* lds) {
 
 // Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk: G basis rows per sweep over w, every row's sum
 // reduced within each wave by shuffles and across the four waves in LDS (two barriers a sweep).
-__global__ __launch_bounds__(NTH) void kr_dots(KrSet k, int n, int chunk, double* part, int P, int nblk, int wslot) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_dots(KrSetT<T> k, int n, int chunk, double* part, int P, int nblk,
+                                               int wslot) {
     __shared__ double red[NTH / 64][2 * G + 1];
     const int p = blockIdx.y;
-    const zd* __restrict__ V = k.V[p];
-    const zd* __restrict__ w = k.w[p];
+    const cpx<T>* __restrict__ V = k.V[p];
+    const cpx<T>* __restrict__ w = k.w[p];
     const int m = k.m[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -46,12 +54,12 @@ __global__ __launch_bounds__(NTH) void kr_dots(KrSet k, int n, int chunk, double
         for (int r = 0; r < G; ++r) ar[r] = ai[r] = 0;
         double ww = 0;
         for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
-            const zd wi = w[i];
+            const zd wi = ldz(w[i]);
             ww += wi.re * wi.re + wi.im * wi.im;
 #pragma unroll
             for (int r = 0; r < G; ++r) {
                 if (r < gm) {
-                    const zd v = V[(size_t)(g0 + r) * n + i];
+                    const zd v = ldz(V[(size_t)(g0 + r) * n + i]);
                     ar[r] += v.re * wi.re + v.im * wi.im;
                     ai[r] += v.re * wi.im - v.im * wi.re;
                 }
@@ -85,15 +93,15 @@ __global__ __launch_bounds__(NTH) void kr_dots(KrSet k, int n, int chunk, double
 // (element e = lane & 7, row group g = lane >> 3): a lane holds rows g, g + 8, .. (at most RPL) of its element's basis
 // column, so each basis value is loaded once for both; the row groups' parts of V h meet by butterfly sums, and the
 // products are summed over the block's elements per row.
-template <int RPL>
-__global__ __launch_bounds__(NTH) void kr_update_dots(KrSet k, int n, int chunk, const double* red, double* part,
+template <class T, int RPL>
+__global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<T> k, int n, int chunk, const double* red, double* part,
                                                       int P, int nblk, int slot_after, int slot_h2) {
     __shared__ double hs_raw[2 * 8 * RPL];
     __shared__ double wsum[NTH / 64][2 * 8 * RPL + 1];
     zd* hs = reinterpret_cast<zd*>(hs_raw);
     const int p = blockIdx.y;
-    const zd* __restrict__ V = k.V[p];
-    zd* w = k.w[p];
+    const cpx<T>* __restrict__ V = k.V[p];
+    cpx<T>* w = k.w[p];
     const int m = k.m[p];
     const double* h = red + (size_t)p * P;
     for (int r = threadIdx.x; r < 8 * RPL; r += NTH) hs[r] = r < m ? zd(h[2 * r], h[2 * r + 1]) : zd();
@@ -114,9 +122,9 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSet k, int n, int chunk,
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int r = g + 8 * t;
-            vv[t] = in && r < m ? V[(size_t)r * n + i] : zd();
+            vv[t] = in && r < m ? ldz(V[(size_t)r * n + i]) : zd();
         }
-        const zd wi0 = in ? w[i] : zd();
+        const zd wi0 = in ? ldz(w[i]) : zd();
         double sre = 0.0, sim = 0.0;
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
@@ -128,9 +136,11 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSet k, int n, int chunk,
             sre += __shfl_xor(sre, off);
             sim += __shfl_xor(sim, off);
         }
-        const zd wi(wi0.re - sre, wi0.im - sim);
+        // the stored value (rounded to T) is what the products and the norm see
+        const cpx<T> ws = stz<T>(wi0.re - sre, wi0.im - sim);
+        const zd wi = ldz(ws);
         if (in && g == 0) {
-            w[i] = wi;
+            w[i] = ws;
             nn += wi.re * wi.re + wi.im * wi.im;
         }
 #pragma unroll
@@ -168,9 +178,10 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSet k, int n, int chunk,
 
 // w -= V h, |w|^2 after. With test_w >= 0 (the re-pass), only where sqrt(red[test_after]) < 0.7 sqrt(red[test_w]):
 // block 0 records the decision at red[flag], and a right-hand side without the re-pass leaves its vector alone.
-__global__ __launch_bounds__(NTH) void kr_update(KrSet k, int n, int chunk, const double* red, double* part, int P,
-                                                 int nblk, int slot, double* red_out, int test_w, int test_after,
-                                                 int flag) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_update(KrSetT<T> k, int n, int chunk, const double* red, double* part,
+                                                 int P, int nblk, int slot, double* red_out, int test_w,
+                                                 int test_after, int flag) {
     __shared__ double lds[NTH / 64];
     __shared__ double hs_raw[1024];
     zd* hs = reinterpret_cast<zd*>(hs_raw);
@@ -181,8 +192,8 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSet k, int n, int chunk, cons
         if (blockIdx.x == 0 && threadIdx.x == 0) red_out[(size_t)p * P + flag] = take ? 1.0 : 0.0;
         if (!take) return;
     }
-    const zd* __restrict__ V = k.V[p];
-    zd* w = k.w[p];
+    const cpx<T>* __restrict__ V = k.V[p];
+    cpx<T>* w = k.w[p];
     const int m = k.m[p];
     const double* h = red + (size_t)p * P;
     for (int r = threadIdx.x; r < m; r += NTH) hs[r] = zd(h[2 * r], h[2 * r + 1]);
@@ -190,28 +201,32 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSet k, int n, int chunk, cons
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double nn = 0;
     for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
-        zd acc = w[i];
+        zd acc = ldz(w[i]);
         for (int r = 0; r < m; ++r) {
-            const zd v = V[(size_t)r * n + i];
+            const zd v = ldz(V[(size_t)r * n + i]);
             const zd hr = hs[r];
             acc.re -= v.re * hr.re - v.im * hr.im;
             acc.im -= v.re * hr.im + v.im * hr.re;
         }
-        w[i] = acc;
-        nn += acc.re * acc.re + acc.im * acc.im;
+        const cpx<T> ws = stz<T>(acc.re, acc.im);
+        w[i] = ws;
+        const zd a = ldz(ws);
+        nn += a.re * a.re + a.im * a.im;
     }
     const double s = block_sum(nn, lds);
     if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot] = s;
 }
 
-__global__ __launch_bounds__(NTH) void kr_norm(KrSet k, int n, int chunk, double* part, int P, int nblk, int slot) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_norm(KrSetT<T> k, int n, int chunk, double* part, int P, int nblk,
+                                               int slot) {
     __shared__ double lds[NTH / 64];
     const int p = blockIdx.y;
-    const zd* __restrict__ v = k.w[p];
+    const cpx<T>* __restrict__ v = k.w[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double nn = 0;
     for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
-        const zd a = v[i];
+        const zd a = ldz(v[i]);
         nn += a.re * a.re + a.im * a.im;
     }
     const double s = block_sum(nn, lds);
@@ -228,27 +243,38 @@ __global__ __launch_bounds__(NTH) void kr_reduce(const double* __restrict__ part
     if (threadIdx.x == 0) red[(size_t)p * P + c] = s;
 }
 
-__global__ __launch_bounds__(NTH) void kr_scale(VecOps v, int n) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_scale(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) v.dst[p][i] = v.f[p] * v.a[p][i];
+    if (i < n) {
+        const zd a = ldz(v.a[p][i]);
+        v.dst[p][i] = stz<T>(v.f[p] * a.re, v.f[p] * a.im);
+    }
 }
 
-__global__ __launch_bounds__(NTH) void kr_sub(VecOps v, int n) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_sub(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) v.dst[p][i] = v.a[p][i] - v.b[p][i];
+    if (i < n) {
+        const zd a = ldz(v.a[p][i]), b = ldz(v.b[p][i]);
+        v.dst[p][i] = stz<T>(a.re - b.re, a.im - b.im);
+    }
 }
 
-__global__ __launch_bounds__(NTH) void kr_copy(VecOps v, int n) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_copy(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
     if (i < n) v.dst[p][i] = v.a[p][i];
 }
 
-__global__ __launch_bounds__(NTH) void kr_zero(VecOps v, int n) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_zero(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) v.dst[p][i] = zd();
+    if (i < n) v.dst[p][i] = cpx<T>();
 }
 
-__global__ __launch_bounds__(NTH) void kr_combine(VecOps v, const zd* __restrict__ y, int rows, int n) {
+template <class T>
+__global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T> v, const zd* __restrict__ y, int rows, int n) {
     __shared__ double ys_raw[1024];
     zd* ys = reinterpret_cast<zd*>(ys_raw);
     const int p = blockIdx.y, k = v.k[p];
@@ -256,10 +282,25 @@ __global__ __launch_bounds__(NTH) void kr_combine(VecOps v, const zd* __restrict
     __syncthreads();
     const int i = blockIdx.x * NTH + threadIdx.x;
     if (i >= n) return;
-    const zd* V = v.a[p];
-    zd acc = v.dst[p][i];
-    for (int r = 0; r < k; ++r) acc = acc + V[(size_t)r * n + i] * ys[r];
-    v.dst[p][i] = acc;
+    const cpx<T>* V = v.a[p];
+    zd acc = ldz(v.dst[p][i]);
+    for (int r = 0; r < k; ++r) acc = acc + ldz(V[(size_t)r * n + i]) * ys[r];
+    v.dst[p][i] = stz<T>(acc.re, acc.im);
+}
+
+struct Conv {
+    cpx<float>* f[KMAX];
+    zd* d[KMAX];
+};
+
+__global__ __launch_bounds__(NTH) void kr_to_float(Conv c, int n) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (i < n) c.f[p][i] = cpx<float>((float)c.d[p][i].re, (float)c.d[p][i].im);
+}
+
+__global__ __launch_bounds__(NTH) void kr_add_float(Conv c, int n) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (i < n) c.d[p][i] = c.d[p][i] + zd(c.f[p][i].re, c.f[p][i].im);
 }
 
 }  // namespace
@@ -274,11 +315,12 @@ void Krylov::init(int n_, int rows) {
     red.alloc((size_t)KMAX * P);
 }
 
-void Krylov::dots(const KrSet& k, hipStream_t s) {
+template <class T>
+void Krylov::dots(const KrSetT<T>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
     int mmax = 0;
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
-    hipLaunchKernelGGL(kr_dots, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_w());
+    hipLaunchKernelGGL(kr_dots<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_w());
     GT_LAUNCH("kr_dots");
     hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
     GT_LAUNCH("kr_reduce");
@@ -286,29 +328,33 @@ void Krylov::dots(const KrSet& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::project_update(const KrSet& k, hipStream_t s, bool second) {
+template <class T>
+void Krylov::project_update(const KrSetT<T>& k, hipStream_t s, bool second) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_update, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p + (second ? slot_h2() : 0),
-                       part.p, P, nblk, slot_w_after(), red.p, -1, -1, 0);
+    hipLaunchKernelGGL(kr_update<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
+                       (const double*)(red.p + (second ? slot_h2() : 0)), part.p, P, nblk, slot_w_after(), red.p, -1,
+                       -1, 0);
     GT_LAUNCH("kr_update");
     hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::update_dots(const KrSet& k, hipStream_t s) {
+template <class T>
+void Krylov::update_dots(const KrSetT<T>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
     int mmax = 0;
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
     if (!fused(mmax)) fail(GT_ERR_ARG, "the fused Gram-Schmidt pass serves at most 128 basis rows");
+    const double* rd = red.p;
     if (mmax <= 32)
-        hipLaunchKernelGGL(kr_update_dots<4>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
+        hipLaunchKernelGGL((kr_update_dots<T, 4>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
                            slot_w_after(), slot_h2());
     else if (mmax <= 64)
-        hipLaunchKernelGGL(kr_update_dots<8>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
+        hipLaunchKernelGGL((kr_update_dots<T, 8>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
                            slot_w_after(), slot_h2());
     else
-        hipLaunchKernelGGL(kr_update_dots<16>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p, part.p, P, nblk,
-                           slot_w_after(), slot_h2());
+        hipLaunchKernelGGL((kr_update_dots<T, 16>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+                           nblk, slot_w_after(), slot_h2());
     GT_LAUNCH("kr_update_dots");
     hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_h2(), red.p);
     GT_LAUNCH("kr_reduce");
@@ -316,48 +362,91 @@ void Krylov::update_dots(const KrSet& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::repass(const KrSet& k, hipStream_t s) {
+template <class T>
+void Krylov::repass(const KrSetT<T>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_update, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, red.p + slot_h2(), part.p, P, nblk,
-                       slot_w_after2(), red.p, slot_w(), slot_w_after(), slot_repass());
+    hipLaunchKernelGGL(kr_update<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
+                       (const double*)(red.p + slot_h2()), part.p, P, nblk, slot_w_after2(), red.p, slot_w(),
+                       slot_w_after(), slot_repass());
     GT_LAUNCH("kr_update");
     hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after2(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-void Krylov::norm2(const KrSet& k, hipStream_t s) {
+template <class T>
+void Krylov::norm2(const KrSetT<T>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_norm, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
+    hipLaunchKernelGGL(kr_norm<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
     GT_LAUNCH("kr_norm");
     hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_norm(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-void k_scale(const VecOps& v, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_scale, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+template <class T>
+void k_scale(const VecOpsT<T>& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_scale<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_scale");
 }
 
-void k_sub(const VecOps& v, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_sub, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+template <class T>
+void k_sub(const VecOpsT<T>& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_sub<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_sub");
 }
 
-void k_copy(const VecOps& v, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_copy, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+template <class T>
+void k_copy(const VecOpsT<T>& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_copy<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_copy");
 }
 
-void k_zero(const VecOps& v, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_zero, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+template <class T>
+void k_zero(const VecOpsT<T>& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(kr_zero<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_zero");
 }
 
-void k_combine(const VecOps& v, const zd* y, int rows, int n, hipStream_t s) {
+template <class T>
+void k_combine(const VecOpsT<T>& v, const zd* y, int rows, int n, hipStream_t s) {
     for (int p = 0; p < v.nr; ++p)
         if (v.k[p] > 512) fail(GT_ERR_ARG, "combination of more than 512 basis vectors");
-    hipLaunchKernelGGL(kr_combine, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n);
+    hipLaunchKernelGGL(kr_combine<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n);
     GT_LAUNCH("kr_combine");
 }
+
+void k_to_float(int nr, cpx<float>* const* dst, const zd* const* a, int n, hipStream_t s) {
+    Conv c{};
+    for (int p = 0; p < nr; ++p) {
+        c.f[p] = dst[p];
+        c.d[p] = const_cast<zd*>(a[p]);
+    }
+    hipLaunchKernelGGL(kr_to_float, dim3(blocks_for(n, NTH), nr), dim3(NTH), 0, s, c, n);
+    GT_LAUNCH("kr_to_float");
+}
+
+void k_add_float(int nr, zd* const* dst, const cpx<float>* const* a, int n, hipStream_t s) {
+    Conv c{};
+    for (int p = 0; p < nr; ++p) {
+        c.f[p] = const_cast<cpx<float>*>(a[p]);
+        c.d[p] = dst[p];
+    }
+    hipLaunchKernelGGL(kr_add_float, dim3(blocks_for(n, NTH), nr), dim3(NTH), 0, s, c, n);
+    GT_LAUNCH("kr_add_float");
+}
+
+#define GT_KRYLOV(T)                                                                   \
+    template void Krylov::dots<T>(const KrSetT<T>&, hipStream_t);                    \
+    template void Krylov::project_update<T>(const KrSetT<T>&, hipStream_t, bool);    \
+    template void Krylov::update_dots<T>(const KrSetT<T>&, hipStream_t);             \
+    template void Krylov::repass<T>(const KrSetT<T>&, hipStream_t);                  \
+    template void Krylov::norm2<T>(const KrSetT<T>&, hipStream_t);                   \
+    template void k_scale<T>(const VecOpsT<T>&, int, hipStream_t);                   \
+    template void k_sub<T>(const VecOpsT<T>&, int, hipStream_t);                     \
+    template void k_copy<T>(const VecOpsT<T>&, int, hipStream_t);                    \
+    template void k_zero<T>(const VecOpsT<T>&, int, hipStream_t);                    \
+    template void k_combine<T>(const VecOpsT<T>&, const zd*, int, int, hipStream_t);
+GT_KRYLOV(double)
+GT_KRYLOV(float)
+#undef GT_KRYLOV
 
 }  // namespace gt

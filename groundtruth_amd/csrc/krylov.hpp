@@ -9,12 +9,16 @@
 
 namespace gt {
 
-struct KrSet {
+// The vectors of a batch: basis rows V_p[0:m_p] (row stride n) and the vector w_p each pass works on. T is the
+// storage type (double, or float for the mixed-precision inner solve); every product and sum is in double.
+template <class T>
+struct KrSetT {
     int nr = 0;
-    const zd* V[KMAX];        // basis rows 0.. of each right-hand side (row stride n)
-    int m[KMAX];              // rows taken
-    zd* w[KMAX];              // the vector projected / updated / measured
+    const cpx<T>* V[KMAX];
+    int m[KMAX];
+    cpx<T>* w[KMAX];
 };
+using KrSet = KrSetT<double>;
 
 struct Krylov {
     int n = 0;
@@ -32,35 +36,50 @@ struct Krylov {
     int slot_repass() const { return 2 * rows_cap + 4; }   // 1 where the re-pass was taken, else 0
     int slot_h2() const { return 2 * rows_cap + 8; }      // the re-pass coefficients V^H w' (2 rows_cap values)
     // h_p = V_p[0:m_p]^H w_p and |w_p|^2
-    void dots(const KrSet& k, hipStream_t s);
+    template <class T>
+    void dots(const KrSetT<T>& k, hipStream_t s);
     // w_p -= V_p[0:m_p]^T h_p (h_p as reduced by dots, or the re-pass coefficients when `second`), |w_p|^2 after
-    void project_update(const KrSet& k, hipStream_t s, bool second = false);
+    template <class T>
+    void project_update(const KrSetT<T>& k, hipStream_t s, bool second = false);
     // whether update_dots serves m rows
     static bool fused(int m) { return m <= 128; }
     // w_p -= V_p[0:m_p]^T h_p, |w_p|^2 after, and the re-pass coefficients V_p^H w_p of the updated vector: one
     // read of the basis for the update and the second classical Gram-Schmidt pass's products
-    void update_dots(const KrSet& k, hipStream_t s);
+    template <class T>
+    void update_dots(const KrSetT<T>& k, hipStream_t s);
     // after update_dots: the second pass w_p -= V_p^T h2_p where |w'_p| < 0.7 |w_p| (the DGKS test, decided on the
     // device and recorded in slot_repass), |w_p|^2 after it in slot_w_after2
-    void repass(const KrSet& k, hipStream_t s);
+    template <class T>
+    void repass(const KrSetT<T>& k, hipStream_t s);
     // |w_p|^2
-    void norm2(const KrSet& k, hipStream_t s);
+    template <class T>
+    void norm2(const KrSetT<T>& k, hipStream_t s);
 };
 
-struct VecOps {
+template <class T>
+struct VecOpsT {
     int nr = 0;
-    zd* dst[KMAX];
-    const zd* a[KMAX];
-    const zd* b[KMAX];
+    cpx<T>* dst[KMAX];
+    const cpx<T>* a[KMAX];
+    const cpx<T>* b[KMAX];
     double f[KMAX];
     int k[KMAX];
 };
+using VecOps = VecOpsT<double>;
 
-void k_scale(const VecOps& v, int n, hipStream_t s);                    // dst = f * a
-void k_sub(const VecOps& v, int n, hipStream_t s);                      // dst = a - b
-void k_copy(const VecOps& v, int n, hipStream_t s);                     // dst = a
-void k_zero(const VecOps& v, int n, hipStream_t s);                     // dst = 0
+template <class T>
+void k_scale(const VecOpsT<T>& v, int n, hipStream_t s);            // dst = f * a
+template <class T>
+void k_sub(const VecOpsT<T>& v, int n, hipStream_t s);              // dst = a - b
+template <class T>
+void k_copy(const VecOpsT<T>& v, int n, hipStream_t s);             // dst = a
+template <class T>
+void k_zero(const VecOpsT<T>& v, int n, hipStream_t s);             // dst = 0
 // dst_p += sum_{r < k_p} a_p[r n : (r+1) n] * y[p rows + r]  (y on the device, `rows` values per right-hand side)
-void k_combine(const VecOps& v, const zd* y, int rows, int n, hipStream_t s);
+template <class T>
+void k_combine(const VecOpsT<T>& v, const zd* y, int rows, int n, hipStream_t s);
+// precision changes of whole vectors: dst_p = (float) a_p, and dst_p += (double) a_p
+void k_to_float(int nr, cpx<float>* const* dst, const zd* const* a, int n, hipStream_t s);
+void k_add_float(int nr, zd* const* dst, const cpx<float>* const* a, int n, hipStream_t s);
 
 }  // namespace gt

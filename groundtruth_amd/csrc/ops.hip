@@ -4,16 +4,26 @@
 // device triangular solves, all right-hand sides in one pass over the panels) plus the Woodbury correction
 // z - W Q (U^T z) (W = B^-1 U resident, Q each frequency's k x k matrix). Loops over the batch sit inside the
 // kernels, so the incidence and W are read once per batch.
+#include <type_traits>
+
 #include "stack.hpp"
 
 namespace gt {
 
 namespace {
 
+__device__ __forceinline__ zd ldz(const zd& v) { return v; }
+__device__ __forceinline__ zd ldz(const cpx<float>& v) { return zd(v.re, v.im); }
+template <class T>
+__device__ __forceinline__ cpx<T> stz(const zd& v) { return cpx<T>((T)v.re, (T)v.im); }
+
+// T: the storage of the input and output vectors (double, or float for the mixed-precision inner solve); the
+// arithmetic and the Schur solve are in double
+template <class T>
 struct PreArgs {
     int ns, nb, nk, k, nr;
-    const zd* r[KMAX];        // n: [r1 ; r2]
-    zd* out[KMAX];            // n: [x1 ; y]
+    const cpx<T>* r[KMAX];    // n: [r1 ; r2]
+    cpx<T>* out[KMAX];        // n: [x1 ; y]
     const zd* dinv_sheet[KMAX];
     const zd* dinv_tail[KMAX];
     const zd* q[KMAX];        // k x k row-major
@@ -32,18 +42,19 @@ struct PreArgs {
 
 // s = r2 - Ar^T (D^-1 r1): a kept node's incident strips lie on its own sheet (one admittance), its branches carry
 // their own
-__global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs a) {
+template <class T>
+__global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs<T> a) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= a.nk) return;
     const int e0 = a.adj_ptr[n], e1 = a.adj_ptr[n + 1];
     const int sh = a.node_sheet[n];
     for (int p = 0; p < a.nr; ++p) {
-        const zd* r = a.r[p];
-        zd acc = r[a.nb + n];
+        const cpx<T>* r = a.r[p];
+        zd acc = ldz(r[a.nb + n]);
         const zd ds = a.dinv_sheet[p][sh];
         for (int e = e0; e < e1; ++e) {
             const int code = a.adj[e], b = code >> 1;
-            const zd v = (b < a.ns ? ds : a.dinv_tail[p][b - a.ns]) * r[b];
+            const zd v = (b < a.ns ? ds : a.dinv_tail[p][b - a.ns]) * ldz(r[b]);
             if (code & 1) acc = acc + v;
             else acc = acc - v;
         }
@@ -52,7 +63,8 @@ __global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs a) {
 }
 
 // t_p = Q_p (U^T z_p): one workgroup per right-hand side (k, the branch count, is small)
-__global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs a) {
+template <class T>
+__global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs<T> a) {
     __shared__ double ut_raw[512];
     zd* ut = reinterpret_cast<zd*>(ut_raw);
     const int p = blockIdx.x;
@@ -78,7 +90,8 @@ __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs a) {
 // of WB branches). The sums over m run in the same order for every batch size.
 constexpr int WB = 64;
 
-__global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs a) {
+template <class T>
+__global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs<T> a) {
     __shared__ double ts_raw[2 * WB * KMAX];
     zd* ts = reinterpret_cast<zd*>(ts_raw);
     const int n = blockIdx.x * 256 + threadIdx.x;
@@ -112,22 +125,23 @@ __global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs a) {
     if (n >= a.nk) return;
 #pragma unroll
     for (int p = 0; p < KMAX; ++p)
-        if (p < a.nr) a.out[p][a.nb + n] = a.s_scale[p] * zd(vre[p], vim[p]);
+        if (p < a.nr) a.out[p][a.nb + n] = stz<T>(a.s_scale[p] * zd(vre[p], vim[p]));
 }
 
 // x1 = D^-1 (r1 + Ar y)
-__global__ __launch_bounds__(256) void pre_branches(PreArgs a) {
+template <class T>
+__global__ __launch_bounds__(256) void pre_branches(PreArgs<T> a) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= a.nb) return;
     const int ka = a.a_keep[b], kb = a.b_keep[b];
     const int sh = b < a.ns ? a.strip_sheet[b] : -1;
     for (int p = 0; p < a.nr; ++p) {
-        zd v = a.r[p][b];
-        const zd* y = a.out[p] + a.nb;
-        if (ka >= 0) v = v + y[ka];
-        if (kb >= 0) v = v - y[kb];
+        zd v = ldz(a.r[p][b]);
+        const cpx<T>* y = a.out[p] + a.nb;
+        if (ka >= 0) v = v + ldz(y[ka]);
+        if (kb >= 0) v = v - ldz(y[kb]);
         const zd d = sh >= 0 ? a.dinv_sheet[p][sh] : a.dinv_tail[p][b - a.ns];
-        a.out[p][b] = d * v;
+        a.out[p][b] = stz<T>(d * v);
     }
 }
 
@@ -143,20 +157,72 @@ void Work::ensure(const gt_stack* st, int nr) {
     cap = nr;
 }
 
-void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const zd* const* x, zd* const* y) {
+namespace {
+
+// The L_p arguments of a product in storage type T (the double spectra and twiddles, or their float copies; the
+// scratch T1 / T2 viewed as T)
+template <class T>
+LpArgs<T> lp_args_t(const gt_stack* st) {
+    if constexpr (std::is_same<T, double>::value) {
+        return st->lp_args(1);
+    } else {
+        const LpArgs<double> d = st->lp_args(1);
+        LpArgs<float> a{};
+        for (int oi = 0; oi < 2; ++oi) {
+            const OrientHost& h = st->o[oi];
+            const OrientArgs<double>& od = d.o[oi];
+            OrientArgs<float>& o = a.o[oi];
+            o.n0 = od.n0;
+            o.n1 = od.n1;
+            o.P0 = od.P0;
+            o.P1 = od.P1;
+            o.log0 = od.log0;
+            o.log1 = od.log1;
+            o.TR = od.TR;
+            o.lo = od.lo;
+            o.cnt = od.cnt;
+            o.rowptr = od.rowptr;
+            o.rowcol = od.rowcol;
+            o.K = h.Kf.p;
+            o.T1 = reinterpret_cast<zf*>(st->work.T1.p) + h.t_off;
+            o.T2 = reinterpret_cast<zf*>(st->work.T2.p) + h.t_off;
+            o.tw0 = h.present ? st->twf.at(h.P0).p : nullptr;
+            o.tw1 = h.present ? st->twf.at(h.P1).p : nullptr;
+            a.nblk_rows[oi] = d.nblk_rows[oi];
+            a.nblk_cols[oi] = d.nblk_cols[oi];
+        }
+        a.S = d.S;
+        a.mode = 1;
+        a.nb = d.nb;
+        a.t_stride = d.t_stride;
+        a.a_keep = d.a_keep;
+        a.b_keep = d.b_keep;
+        return a;
+    }
+}
+
+}  // namespace
+
+template <class T>
+void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     hipStream_t s = st->ctx->stream;
     st->work.ensure(st, nr);
-    RhsSet<double> rs{};
+    RhsSet<T> rs{};
     rs.nr = nr;
     for (int p = 0; p < nr; ++p) {
         rs.x[p] = x[p];
         rs.y[p] = y[p];
-        rs.w[p] = ops[p]->w;
-        rs.r_sheet[p] = ops[p]->r_sheet.p;
-        rs.z_tail[p] = ops[p]->z_tail.p;
+        rs.w[p] = (T)ops[p]->w;
+        if constexpr (std::is_same<T, double>::value) {
+            rs.r_sheet[p] = ops[p]->r_sheet.p;
+            rs.z_tail[p] = ops[p]->z_tail.p;
+        } else {
+            rs.r_sheet[p] = ops[p]->r_sheet_f.p;
+            rs.z_tail[p] = ops[p]->z_tail_f.p;
+        }
     }
-    LpArgs<double> la = st->lp_args(1);
+    LpArgs<T> la = lp_args_t<T>(st);
     la.rhs = rs;
     if (st->ns > 0) {
         gt_ctx* c = st->ctx;
@@ -177,7 +243,7 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const zd* const* x, zd*
             launch_lp(la, st->shape, s);
         }
     }
-    TailArgs<double> ta{st->ns, st->nb, st->nbr, st->nk, st->a_keep.p, st->b_keep.p, st->adj_ptr.p, st->adj.p, rs};
+    TailArgs<T> ta{st->ns, st->nb, st->nbr, st->nk, st->a_keep.p, st->b_keep.p, st->adj_ptr.p, st->adj.p, rs};
     launch_tail_bottom(ta, s);
 }
 
@@ -186,11 +252,12 @@ void schur_apply(gt_stack* st, const zd* b, zd* x, int nr) {
     schur_solve_device(*st->schur, b, x, nr, st->ctx->stream);
 }
 
-void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const zd* const* r, zd* const* z) {
+template <class T>
+void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     hipStream_t s = st->ctx->stream;
     st->work.ensure(st, nr);
-    PreArgs a{};
+    PreArgs<T> a{};
     a.ns = st->ns;
     a.nb = st->nb;
     a.nk = st->nk;
@@ -216,20 +283,25 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const zd* const* r, zd
     a.node_sheet = st->node_sheet.p;
     a.strip_sheet = st->strip_sheet.p;
     if (st->nk > 0) {
-        hipLaunchKernelGGL(pre_schur_rhs, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(pre_schur_rhs<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_schur_rhs");
         schur_apply(st, st->work.s.p, st->work.z.p, nr);
         if (a.k > 0) {
-            hipLaunchKernelGGL(pre_woodbury_small, dim3(nr), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(pre_woodbury_small<T>, dim3(nr), dim3(256), 0, s, a);
             GT_LAUNCH("pre_woodbury_small");
         }
-        hipLaunchKernelGGL(pre_woodbury_apply, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(pre_woodbury_apply<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_woodbury_apply");
     }
     if (st->nb > 0) {
-        hipLaunchKernelGGL(pre_branches, dim3(blocks_for(st->nb, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(pre_branches<T>, dim3(blocks_for(st->nb, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_branches");
     }
 }
+
+template void ops_matvec<double>(gt_stack*, int, gt_op* const*, const zd* const*, zd* const*);
+template void ops_matvec<float>(gt_stack*, int, gt_op* const*, const zf* const*, zf* const*);
+template void ops_precond<double>(gt_stack*, int, gt_op* const*, const zd* const*, zd* const*);
+template void ops_precond<float>(gt_stack*, int, gt_op* const*, const zf* const*, zf* const*);
 
 }  // namespace gt

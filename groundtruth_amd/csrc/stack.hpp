@@ -111,13 +111,17 @@ struct gt_op_s {
     gt_stack* st = nullptr;
     double w = 0;
     gt::DevBuf<gt::zd> r_sheet, z_tail, dinv_sheet, dinv_tail, q;
+    gt::DevBuf<gt::zf> r_sheet_f, z_tail_f;   // the product's impedances for single-precision products
     gt::zd s_scale;
     int k = 0;
 };
 
 namespace gt {
 // device work of a batch of operators on one stack (ops.hip): y_p = K_p x_p, z_p = M_p^-1 r_p for p < nr <= 16
-void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const zd* const* x, zd* const* y);
-void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const zd* const* r, zd* const* z);
+// (T: the storage of the vectors, double or float; the preconditioner's arithmetic and Schur solve are double)
+template <class T>
+void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y);
+template <class T>
+void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z);
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides
 }  // namespace gt

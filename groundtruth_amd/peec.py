@@ -37,6 +37,8 @@ GMRES_CHECK_MISSES = 4          # peec.py:100
 SCREEN_MIN_NODES = 16           # peec.py:112
 OPERATORS_KEPT = 32             # per-frequency operators kept per stack (small device arrays each)
 BATCH = 16                      # right-hand sides per batched device launch
+MIXED_INNER_TOL = 1e-5          # mixed precision: each FP32 inner solve's relative residual
+MIXED_MAX_REFINE = 10           # mixed precision: FP64 refinement rounds at most
 
 _KERNELS: dict = {}
 
@@ -399,10 +401,14 @@ class PlaneStack:
 
     def gmres_batch_dev(self, ops: list, b_devs: list, x_devs: list, tol: float, max_cycles: int = 8,
                         restart: Optional[int] = None, extra_cycles: int = GMRES_EXTRA_CYCLES,
-                        bnorm: Optional[float] = None) -> list:
+                        bnorm: Optional[float] = None, precision: str = "fp64",
+                        inner_tol: float = MIXED_INNER_TOL, max_refine: int = MIXED_MAX_REFINE) -> list:
         """Batched device GMRES: x_i = M_i^-1 y_i with (K_i M_i^-1) y_i = b_i for every operator i of this stack, on
         device-resident vectors (torch complex128 tensors, their work finished), at most 16 per launch group. Each
-        follows the reference's GMRES; the batch shares every launch. Returns one GmresResult per solve."""
+        follows the reference's GMRES; the batch shares every launch. precision "mixed": FP64 iterative refinement
+        to tol around FP32 inner solves to inner_tol (gt_gmres_batch_mixed). Returns one GmresResult per solve."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision is 'fp64' or 'mixed'")
         out = []
         for g0 in range(0, len(ops), BATCH):
             grp = slice(g0, g0 + BATCH)
@@ -414,12 +420,16 @@ class PlaneStack:
             p = _lib.GmresParams(float(tol), float(bnorm or 0.0), int(restart or 0), int(max_cycles),
                                  int(extra_cycles), 0, GMRES_CHECK_MISSES)
             res = (_lib.GmresResult * nr)()
-            _lib.check(_lib.lib().gt_gmres_batch(self.handle, nr, H, B, X, C.byref(p), res), "gt_gmres_batch")
+            if precision == "fp64":
+                _lib.check(_lib.lib().gt_gmres_batch(self.handle, nr, H, B, X, C.byref(p), res), "gt_gmres_batch")
+            else:
+                _lib.check(_lib.lib().gt_gmres_batch_mixed(self.handle, nr, H, B, X, C.byref(p), float(inner_tol),
+                                                           int(max_refine), res), "gt_gmres_batch_mixed")
             out += list(res)
         return out
 
     def solve_many(self, fs: list, rhs_tops: list, Js: list, tol: float = 1e-5, max_cycles: int = 8,
-                   restart: Optional[int] = None) -> list:
+                   restart: Optional[int] = None, precision: str = "fp64") -> list:
         """PlaneStack.solve for many (f, rhs_top, J) on this stack at once (batched on the device): the list of
         (Ib, V, iterations, x), each the same as a solve of its own. SolverError if any does not converge."""
         import torch
@@ -430,7 +440,7 @@ class PlaneStack:
         b = [torch.from_numpy(r).to(dev) for r in rhs]
         x = [torch.empty_like(t) for t in b]
         torch.cuda.synchronize(self.device)
-        res = self.gmres_batch_dev(ops, b, x, tol, max_cycles=max_cycles, restart=restart)
+        res = self.gmres_batch_dev(ops, b, x, tol, max_cycles=max_cycles, restart=restart, precision=precision)
         self.ctx.sync()
         out = []
         for f, r, xd in zip(fs, res, x):
@@ -665,12 +675,13 @@ class RouteExcitation:
                                          restart=restart)
         return self.unpack(f, Ib, V, its, x)
 
-    def solve_many(self, fs: list, tol: float = 1e-5, max_cycles: int = 8, restart: Optional[int] = None) -> list:
+    def solve_many(self, fs: list, tol: float = 1e-5, max_cycles: int = 8, restart: Optional[int] = None,
+                   precision: str = "fp64") -> list:
         """The loop's solutions at every frequency of `fs`, solved as one batch on the device (each the same as
-        solve(f) alone)."""
+        solve(f) alone); precision "mixed": FP32 inner solves under FP64 refinement to tol."""
         rhs = [self.rhs(f) for f in fs]
         sols = self.stack.solve_many(fs, [r[0] for r in rhs], [r[1] for r in rhs], tol=tol, max_cycles=max_cycles,
-                                     restart=restart)
+                                     restart=restart, precision=precision)
         return [self.unpack(f, *s) for f, s in zip(fs, sols)]
 
     def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
