@@ -74,6 +74,13 @@ SIGNATURES = {
     "gt_gmres_batch_mixed": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p), C.POINTER(GmresParams), C.c_double, C.c_int32,
                                        C.POINTER(GmresResult)]),
+    "gt_session_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gt_session_destroy": (None, [C.c_void_p]),
+    "gt_session_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GmresParams), i32p]),
+    "gt_session_step": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p]),
+    "gt_session_poll": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(GmresResult), i32p]),
+    "gt_session_device_bytes": (C.c_int64, [C.c_void_p]),
+    "gt_session_plan_bytes": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32]),
 }
 
 _LIB = None

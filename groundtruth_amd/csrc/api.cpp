@@ -11,6 +11,14 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
                  gt_gmres_result* res);
 void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
                        const gt_gmres_params& p, double inner_tol, int max_refine, gt_gmres_result* res);
+int64_t session_plan_bytes(const gt_stack* st, int cap, int restart);
+gt_session* session_create(gt_stack* st, int cap, int restart);
+int64_t session_bytes(const gt_session* ss);
+int session_submit(gt_session* ss, gt_op* op, const zd* b, zd* x, const gt_gmres_params& p);
+int session_step(gt_session* ss, int max_steps, int* ready);
+bool session_poll(gt_session* ss, int slot, gt_gmres_result* res);
+void session_destroy(gt_session* ss);
+gt_stack* session_stack(gt_session* ss);
 }
 
 using namespace gt;
@@ -175,12 +183,13 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
 // The stack's own device memory plus its core's (a derived stack counts the shared core too: it is what the
 // stack needs resident)
 int64_t gt_stack_s::device_bytes() const {
-    int64_t b = 0;
+    int64_t b = a_keep.bytes() + b_keep.bytes() + adj_ptr.bytes() + adj.bytes();
+    b += work.T1.bytes() + work.T2.bytes() + work.s.bytes() + work.z.bytes() + work.t.bytes() + W.bytes();
+    if (derived) return b;                 // the core is its base's
     for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + h.K.bytes() + h.Kf.bytes();
     for (const auto& t : tw) b += t.second.bytes();
     for (const auto& t : twf) b += t.second.bytes();
-    b += a_keep.bytes() + b_keep.bytes() + adj_ptr.bytes() + adj.bytes() + node_sheet.bytes() + strip_sheet.bytes();
-    b += work.T1.bytes() + work.T2.bytes() + work.s.bytes() + work.z.bytes() + work.t.bytes() + W.bytes();
+    b += node_sheet.bytes() + strip_sheet.bytes();
     if (schur) b += schur->device_bytes();
     return b;
 }
@@ -412,6 +421,7 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
         *out = nullptr;
         GT_HIP(hipSetDevice(base->ctx->device));
         std::unique_ptr<gt_stack_s> st(new gt_stack_s(base->core));
+        st->derived = true;
         st->ctx = base->ctx;
         st->S = base->S;
         st->nx = base->nx;
@@ -666,6 +676,64 @@ int gt_gmres_batch_mixed(gt_stack* st, int32_t nr, gt_op* const* ops, const void
         GT_HIP(hipSetDevice(st->ctx->device));
         gmres_batch_mixed(st, nr, ops, b.data(), x.data(), *p, inner_tol, max_refine, res);
     });
+}
+
+int gt_session_create(gt_stack* st, int32_t capacity, int32_t restart, gt_session** out) {
+    return guard([&] {
+        need(st && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        *out = nullptr;
+        need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        GT_HIP(hipSetDevice(st->ctx->device));
+        *out = session_create(st, capacity, restart);
+    });
+}
+
+void gt_session_destroy(gt_session* ss) {
+    if (!ss) return;
+    gt_ctx* ctx = session_stack(ss)->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    session_destroy(ss);
+}
+
+int gt_session_submit(gt_session* ss, gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* p,
+                      int32_t* slot) {
+    return guard([&] {
+        need(ss && op && b_dev && x_dev && p && slot && b_dev != x_dev, "NULL or aliased argument");
+        std::lock_guard<std::mutex> lk(session_stack(ss)->ctx->mu);
+        need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
+        *slot = session_submit(ss, op, (const zd*)b_dev, (zd*)x_dev, *p);
+    });
+}
+
+int gt_session_step(gt_session* ss, int32_t max_steps, int32_t* running, int32_t* ready) {
+    return guard([&] {
+        need(ss && running && ready, "NULL argument");
+        need(max_steps >= 0, "negative step count");
+        gt_ctx* ctx = session_stack(ss)->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        int rd = 0;
+        *running = session_step(ss, max_steps, &rd);
+        *ready = rd;
+    });
+}
+
+int gt_session_poll(gt_session* ss, int32_t slot, gt_gmres_result* res, int32_t* done) {
+    return guard([&] {
+        need(ss && res && done, "NULL argument");
+        std::lock_guard<std::mutex> lk(session_stack(ss)->ctx->mu);
+        *done = session_poll(ss, slot, res) ? 1 : 0;
+    });
+}
+
+int64_t gt_session_device_bytes(const gt_session* ss) { return ss ? session_bytes(ss) : 0; }
+
+int64_t gt_session_plan_bytes(const gt_stack* st, int32_t capacity, int32_t restart) {
+    if (!st || capacity < 1 || capacity > KMAX || restart < 0) return -1;
+    return session_plan_bytes(st, capacity, restart);
 }
 
 }  // extern "C"

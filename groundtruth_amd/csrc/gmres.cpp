@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <limits>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -26,7 +27,9 @@ using cd = std::complex<double>;
 
 constexpr double PROGRESS = 0.7;
 
-enum State { START, ARNOLDI, DONE };
+// FREE: no work in the slot; PENDING: submitted, its set-up (x = 0, r = b, |b|) not yet launched; START: at a cycle
+// start; ARNOLDI: inside a cycle; DONE: stopped, x = M^-1 y not yet formed; READY: finished, results held.
+enum State { FREE, PENDING, START, ARNOLDI, DONE, READY };
 
 // A device pointer in the role of the buffers it replaces (`.p`)
 template <class T>
@@ -38,8 +41,9 @@ template <class T>
 struct Rhs {
     gt_op* op = nullptr;
     const cpx<T>* b = nullptr;             // right-hand side (device)
-    Vec<T> V, w, x, xt, r, t, u;              // basis (restart + 1 rows) and vectors, in the context's workspace
-    State state = START;
+    gt_gmres_params p{};                   // its tolerance and budgets
+    Vec<T> V, w, x, xt, r, t, u;              // basis (restart + 1 rows) and vectors, in the workspace
+    State state = FREE;
     bool conv = false;
     double bnorm = 0, last = 0, aim = 0, resid = 0, hn = 0;
     int cycle = 0, its = 0, j = 0, k = 0, misses = 0, products = 0;
@@ -73,6 +77,20 @@ struct GmresWork {
             ref_n = n_;
         }
         for (; ref_slots < nr; ++ref_slots) ref[ref_slots].alloc(4 * ref_n);
+    }
+
+    // device bytes of a workspace of `slots` slots of `rows` basis rows over vectors of n
+    static int64_t plan_bytes(size_t n_, int rows_, int slots_) {
+        const int64_t z = sizeof(zd);
+        const int64_t P = 4 * (int64_t)rows_ + 8;
+        const int64_t nblk = std::min<int64_t>(2048, std::max<int64_t>(1, ((int64_t)n_ + 1023) / 1024));
+        return (int64_t)slots_ * ((int64_t)rows_ + 6) * (int64_t)n_ * z + (int64_t)KMAX * rows_ * z +
+               (int64_t)KMAX * nblk * P * 8 + (int64_t)KMAX * P * 8;
+    }
+    int64_t device_bytes() const {
+        int64_t b = ydev.bytes() + kr.part.bytes() + kr.red.bytes();
+        for (int i = 0; i < KMAX; ++i) b += V[i].bytes() + vec[i].bytes() + ref[i].bytes();
+        return b;
     }
 
     void ensure(size_t n_, int rows_, int nr) {
@@ -109,8 +127,7 @@ struct Batch {
     using cz = cpx<T>;
     gt_stack* st;
     hipStream_t s;
-    int n, restart, nr;
-    gt_gmres_params p;
+    int n, restart, cap;
     std::vector<Rhs<T>> R;
     GmresWork& gw;
     Krylov& kr;
@@ -118,11 +135,12 @@ struct Batch {
     HostBuf<cd>& ybuf;
     HostBuf<double>& red;
 
-    Batch(gt_stack* st_, int nr_, int restart_, const gt_gmres_params& p_, GmresWork& gw_)
-        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), nr(nr_), p(p_), R(nr_), gw(gw_),
+    // cap slots over the workspace gw, every one FREE
+    Batch(gt_stack* st_, int cap_, int restart_, GmresWork& gw_)
+        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), cap(cap_), R(cap_), gw(gw_),
           kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
-        gw.ensure((size_t)n, restart + 1, nr);
-        for (int i = 0; i < nr; ++i) {
+        gw.ensure((size_t)n, restart + 1, cap);
+        for (int i = 0; i < cap; ++i) {
             Rhs<T>& q = R[i];
             q.V.p = reinterpret_cast<cz*>(gw.V[i].p);
             cz* v = reinterpret_cast<cz*>(gw.vec[i].p);
@@ -137,6 +155,24 @@ struct Batch {
             q.g.assign(restart + 1, 0);
             q.cs.assign(restart, 0);
         }
+    }
+
+    // A solve enters slot i (FREE or READY): its set-up runs at the next init().
+    void submit(int i, gt_op* op, const cz* b, const gt_gmres_params& p) {
+        Rhs<T>& q = R[i];
+        q.op = op;
+        q.b = b;
+        q.p = p;
+        q.state = PENDING;
+        q.conv = false;
+        q.bnorm = q.last = q.aim = q.resid = q.hn = 0;
+        q.cycle = q.its = q.j = q.k = q.misses = q.products = 0;
+    }
+
+    bool running() const {
+        for (const Rhs<T>& q : R)
+            if (q.state == PENDING || q.state == START || q.state == ARNOLDI) return true;
+        return false;
     }
 
     cz* row(Rhs<T>& q, int j) { return q.V.p + (size_t)j * n; }
@@ -221,122 +257,131 @@ struct Batch {
         k_sub(v, n, s);
     }
 
-    void run() {
-        const double tol = p.tol;
-        const int max_cycles = p.max_cycles, extra = p.extra_cycles;
-        const int max_its = p.max_its > 0 ? p.max_its : std::numeric_limits<int>::max();
-        const int check_misses = p.check_misses > 0 ? p.check_misses : 4;
-        std::vector<int> all(nr);
-        for (int i = 0; i < nr; ++i) all[i] = i;
-        // x = 0, r = b, |b|
-        {
-            VecOpsT<T> v{};
-            v.nr = nr;
-            for (int i = 0; i < nr; ++i) v.dst[i] = R[i].x.p;
-            k_zero(v, n, s);
-            for (int i = 0; i < nr; ++i) {
-                v.dst[i] = R[i].r.p;
-                v.a[i] = R[i].b;
-            }
-            k_copy(v, n, s);
-            std::vector<const cz*> bs;
-            for (int i = 0; i < nr; ++i) bs.push_back(R[i].b);
-            std::vector<double> b0;
-            norms(all, bs, b0);
-            for (int i = 0; i < nr; ++i) {
-                Rhs<T>& q = R[i];
-                q.bnorm = p.bnorm > 0 ? p.bnorm : b0[i];
-                q.last = std::numeric_limits<double>::infinity();
-                if (q.bnorm == 0 || b0[i] <= tol * q.bnorm) {
-                    q.state = DONE;
-                    q.conv = true;
-                    q.resid = q.bnorm > 0 ? b0[i] / q.bnorm : 0.0;
-                }
+    static int max_its_of(const Rhs<T>& q) {
+        return q.p.max_its > 0 ? q.p.max_its : std::numeric_limits<int>::max();
+    }
+
+    // The PENDING slots' set-up: x = 0, r = b, |b|; a right-hand side already within its tolerance is DONE.
+    void init() {
+        std::vector<int> P;
+        for (int i = 0; i < cap; ++i)
+            if (R[i].state == PENDING) P.push_back(i);
+        if (P.empty()) return;
+        VecOpsT<T> v{};
+        v.nr = (int)P.size();
+        for (int i = 0; i < v.nr; ++i) v.dst[i] = R[P[i]].x.p;
+        k_zero(v, n, s);
+        for (int i = 0; i < v.nr; ++i) {
+            v.dst[i] = R[P[i]].r.p;
+            v.a[i] = R[P[i]].b;
+        }
+        k_copy(v, n, s);
+        std::vector<const cz*> bs;
+        for (int q : P) bs.push_back(R[q].b);
+        std::vector<double> b0;
+        norms(P, bs, b0);
+        for (size_t i = 0; i < P.size(); ++i) {
+            Rhs<T>& q = R[P[i]];
+            q.bnorm = q.p.bnorm > 0 ? q.p.bnorm : b0[i];
+            q.last = std::numeric_limits<double>::infinity();
+            q.state = START;
+            if (q.bnorm == 0 || b0[i] <= q.p.tol * q.bnorm) {
+                q.state = DONE;
+                q.conv = true;
+                q.resid = q.bnorm > 0 ? b0[i] / q.bnorm : 0.0;
             }
         }
-        for (;;) {
-            // ---- cycle starts: |r|, the stopping tests, V[0] = r / |r|
-            std::vector<int> S;
-            for (int i = 0; i < nr; ++i)
-                if (R[i].state == START) S.push_back(i);
-            if (!S.empty()) {
-                std::vector<const cz*> rs;
-                for (int q : S) rs.push_back(R[q].r.p);
-                std::vector<double> beta;
-                norms(S, rs, beta);
-                VecOpsT<T> v{};
-                for (size_t i = 0; i < S.size(); ++i) {
-                    Rhs<T>& q = R[S[i]];
-                    const double b = beta[i];
-                    q.resid = b / q.bnorm;
-                    if (b <= tol * q.bnorm) { q.state = DONE; q.conv = true; continue; }
-                    if (q.its >= max_its || q.cycle >= max_cycles + extra ||
-                        (q.cycle >= max_cycles && b > PROGRESS * q.last)) {
-                        q.state = DONE;
-                        q.conv = false;
-                        continue;
-                    }
-                    q.last = b;
-                    std::fill(q.H.begin(), q.H.end(), cd(0));
-                    std::fill(q.cs.begin(), q.cs.end(), 0.0);
-                    std::fill(q.sn.begin(), q.sn.end(), cd(0));
-                    std::fill(q.g.begin(), q.g.end(), cd(0));
-                    q.g[0] = b;
-                    q.j = 0;
-                    q.k = 0;
-                    q.aim = tol * q.bnorm;
-                    q.misses = 0;
-                    q.cycle++;
-                    q.state = ARNOLDI;
-                    v.dst[v.nr] = row(q, 0);
-                    v.a[v.nr] = q.r.p;
-                    v.f[v.nr] = 1.0 / b;
-                    v.nr++;
+    }
+
+    // Gram-Schmidt for the slots `who` (all on the fused pass or all not), the values fetched to the host
+    void project(const std::vector<int>& who, bool fuse) {
+        KrSetT<T> k{};
+        k.nr = (int)who.size();
+        for (int i = 0; i < k.nr; ++i) {
+            Rhs<T>& q = R[who[i]];
+            k.V[i] = q.V.p;
+            k.m[i] = q.j + 1;
+            k.w[i] = q.w.p;
+        }
+        kr.dots(k, s);
+        if (fuse) {
+            kr.update_dots(k, s);
+            kr.repass(k, s);
+        } else {
+            kr.project_update(k, s);
+        }
+        fetch(k.nr);
+    }
+
+    // One Arnoldi step of every slot inside a cycle, after the cycle starts; false when no slot is left in one.
+    bool step() {
+        // ---- cycle starts: |r|, the stopping tests, V[0] = r / |r|
+        std::vector<int> S;
+        for (int i = 0; i < cap; ++i)
+            if (R[i].state == START) S.push_back(i);
+        if (!S.empty()) {
+            std::vector<const cz*> rs;
+            for (int q : S) rs.push_back(R[q].r.p);
+            std::vector<double> beta;
+            norms(S, rs, beta);
+            VecOpsT<T> v{};
+            for (size_t i = 0; i < S.size(); ++i) {
+                Rhs<T>& q = R[S[i]];
+                const double b = beta[i];
+                q.resid = b / q.bnorm;
+                if (b <= q.p.tol * q.bnorm) { q.state = DONE; q.conv = true; continue; }
+                if (q.its >= max_its_of(q) || q.cycle >= q.p.max_cycles + q.p.extra_cycles ||
+                    (q.cycle >= q.p.max_cycles && b > PROGRESS * q.last)) {
+                    q.state = DONE;
+                    q.conv = false;
+                    continue;
                 }
-                if (v.nr) k_scale(v, n, s);
+                q.last = b;
+                std::fill(q.H.begin(), q.H.end(), cd(0));
+                std::fill(q.cs.begin(), q.cs.end(), 0.0);
+                std::fill(q.sn.begin(), q.sn.end(), cd(0));
+                std::fill(q.g.begin(), q.g.end(), cd(0));
+                q.g[0] = b;
+                q.j = 0;
+                q.k = 0;
+                q.aim = q.p.tol * q.bnorm;
+                q.misses = 0;
+                q.cycle++;
+                q.state = ARNOLDI;
+                v.dst[v.nr] = row(q, 0);
+                v.a[v.nr] = q.r.p;
+                v.f[v.nr] = 1.0 / b;
+                v.nr++;
             }
-            std::vector<int> A;
-            for (int i = 0; i < nr; ++i)
-                if (R[i].state == ARNOLDI) A.push_back(i);
-            if (A.empty()) break;
-            // ---- one Arnoldi step each: w = K M^-1 V[j], one CGS pass, |w| before and after
-            {
-                std::vector<const cz*> in;
-                std::vector<cz*> out;
-                for (int q : A) {
-                    in.push_back(row(R[q], R[q].j));
-                    out.push_back(R[q].w.p);
-                }
-                apply(A, in, out);
+            if (v.nr) k_scale(v, n, s);
+        }
+        std::vector<int> A;
+        for (int i = 0; i < cap; ++i)
+            if (R[i].state == ARNOLDI) A.push_back(i);
+        if (A.empty()) return false;
+        // ---- one Arnoldi step each: w = K M^-1 V[j], one CGS pass, |w| before and after
+        {
+            std::vector<const cz*> in;
+            std::vector<cz*> out;
+            for (int q : A) {
+                in.push_back(row(R[q], R[q].j));
+                out.push_back(R[q].w.p);
             }
-            // the fused pass also forms the second pass's coefficients V^H w', used when the re-pass is taken
-            int jmax = 0;
-            for (int q : A) jmax = std::max(jmax, R[q].j);
-            const bool fuse = Krylov::fused(jmax + 1);
-            auto project = [&](const std::vector<int>& who) {
-                KrSetT<T> k{};
-                k.nr = (int)who.size();
-                for (int i = 0; i < k.nr; ++i) {
-                    Rhs<T>& q = R[who[i]];
-                    k.V[i] = q.V.p;
-                    k.m[i] = q.j + 1;
-                    k.w[i] = q.w.p;
-                }
-                kr.dots(k, s);
-                if (fuse) {
-                    kr.update_dots(k, s);
-                    kr.repass(k, s);
-                } else {
-                    kr.project_update(k, s);
-                }
-                fetch(k.nr);
-            };
-            project(A);
-            // CGS, then the DGKS re-pass where |w| fell below 0.7 of its value before the projection (on the fused
-            // path the device took the test and the re-pass already; its decision is read back)
-            std::vector<int> D;
-            for (size_t i = 0; i < A.size(); ++i) {
-                Rhs<T>& q = R[A[i]];
+            apply(A, in, out);
+        }
+        // CGS with the DGKS re-pass where |w| fell below 0.7 of its value before the projection. Up to 128 basis rows
+        // the update pass also forms the re-pass's coefficients V^H w' and the device takes the test and the re-pass
+        // (its decision read back); longer bases test on the host. The path depends on the slot's own j only, so a
+        // right-hand side's arithmetic does not depend on the rest of the batch.
+        std::vector<int> D;
+        for (const bool fuse : {true, false}) {
+            std::vector<int> G;
+            for (int q : A)
+                if (Krylov::fused(R[q].j + 1) == fuse) G.push_back(q);
+            if (G.empty()) continue;
+            project(G, fuse);
+            for (size_t i = 0; i < G.size(); ++i) {
+                Rhs<T>& q = R[G[i]];
                 q.h.assign(q.j + 1, 0);
                 for (int r = 0; r <= q.j; ++r) q.h[r] = cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
                 const double w_norm = std::sqrt(val((int)i, kr.slot_w()));
@@ -348,130 +393,137 @@ struct Batch {
                         q.hn = std::sqrt(val((int)i, kr.slot_w_after2()));
                     }
                 } else if (q.hn < 0.7 * w_norm) {
-                    D.push_back(A[i]);
+                    D.push_back(G[i]);
                 }
             }
-            if (!D.empty()) {                 // DGKS: a second pass only after cancellation
-                project(D);
-                for (size_t i = 0; i < D.size(); ++i) {
-                    Rhs<T>& q = R[D[i]];
-                    for (int r = 0; r <= q.j; ++r) q.h[r] += cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
-                    q.hn = std::sqrt(val((int)i, kr.slot_w_after()));
-                }
+        }
+        if (!D.empty()) {                     // DGKS on the host's test: the second pass
+            project(D, false);
+            for (size_t i = 0; i < D.size(); ++i) {
+                Rhs<T>& q = R[D[i]];
+                for (int r = 0; r <= q.j; ++r) q.h[r] += cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
+                q.hn = std::sqrt(val((int)i, kr.slot_w_after()));
             }
-            // ---- Givens rotations and the decisions
-            std::vector<int> C, E, Nx;        // true-residual checks, cycle ends, next basis vector
-            for (int qi : A) {
-                Rhs<T>& q = R[qi];
-                const int j = q.j, ld = restart;
-                std::vector<cd> col(j + 2);
-                for (int i = 0; i <= j; ++i) col[i] = q.h[i];
-                col[j + 1] = q.hn;
-                for (int i = 0; i < j; ++i) {
-                    const cd a = col[i], c = col[i + 1];
-                    col[i] = q.cs[i] * a + q.sn[i] * c;
-                    col[i + 1] = -std::conj(q.sn[i]) * a + q.cs[i] * c;
-                }
-                const cd a = col[j];
-                const double rho = std::hypot(std::abs(a), q.hn);
-                if (rho == 0.0) {
-                    q.cs[j] = 1.0;
-                    q.sn[j] = 0.0;
-                } else if (std::abs(a) == 0.0) {
-                    q.cs[j] = 0.0;
-                    q.sn[j] = 1.0;
-                    col[j] = q.hn;
-                } else {
-                    const cd phase = a / std::abs(a);
-                    q.cs[j] = std::abs(a) / rho;
-                    q.sn[j] = phase * q.hn / rho;
-                    col[j] = phase * rho;
-                }
-                col[j + 1] = 0.0;
-                for (int i = 0; i <= j + 1; ++i) q.H[(size_t)i * ld + j] = col[i];
-                q.g[j + 1] = -std::conj(q.sn[j]) * q.g[j];
-                q.g[j] = q.cs[j] * q.g[j];
-                q.its++;
-                q.k = j + 1;
-                double hnorm = 0;
-                for (int i = 0; i <= j; ++i) hnorm += std::norm(q.h[i]);
-                hnorm = std::sqrt(hnorm);
-                if (q.hn <= 1e-14 * hnorm || q.its >= max_its) E.push_back(qi);
-                else if (std::abs(q.g[j + 1]) <= q.aim) C.push_back(qi);
-                else Nx.push_back(qi);
+        }
+        // ---- Givens rotations and the decisions
+        std::vector<int> C, E, Nx;        // true-residual checks, cycle ends, next basis vector
+        for (int qi : A) {
+            Rhs<T>& q = R[qi];
+            const int j = q.j, ld = restart;
+            std::vector<cd> col(j + 2);
+            for (int i = 0; i <= j; ++i) col[i] = q.h[i];
+            col[j + 1] = q.hn;
+            for (int i = 0; i < j; ++i) {
+                const cd a = col[i], c = col[i + 1];
+                col[i] = q.cs[i] * a + q.sn[i] * c;
+                col[i + 1] = -std::conj(q.sn[i]) * a + q.cs[i] * c;
             }
-            // ---- true-residual checks: xt = x + V y, r = b - K M^-1 xt
-            if (!C.empty()) {
-                std::vector<std::vector<cd>> ys;
-                std::vector<cz*> dst;
-                std::vector<const cz*> from;
-                for (int qi : C) {
-                    ys.push_back(back_substitute(R[qi], R[qi].k));
-                    dst.push_back(R[qi].xt.p);
-                    from.push_back(R[qi].x.p);
-                }
-                combine(C, ys, dst, from);
-                residuals(C, dst);
-                std::vector<const cz*> rs;
-                for (int qi : C) rs.push_back(R[qi].r.p);
-                std::vector<double> rt;
-                norms(C, rs, rt);
-                VecOpsT<T> cp{};
-                for (size_t i = 0; i < C.size(); ++i) {
-                    Rhs<T>& q = R[C[i]];
-                    q.resid = rt[i] / q.bnorm;
-                    if (rt[i] <= tol * q.bnorm) {
-                        cp.dst[cp.nr] = q.x.p;
-                        cp.a[cp.nr++] = q.xt.p;
-                        q.state = DONE;
-                        q.conv = true;
-                        continue;
-                    }
-                    q.misses++;
-                    if (q.misses >= check_misses) {     // restart from the check: x = xt, r = rt
-                        cp.dst[cp.nr] = q.x.p;
-                        cp.a[cp.nr++] = q.xt.p;
-                        q.state = START;
-                        continue;
-                    }
-                    q.aim = 0.5 * std::abs(q.g[q.j + 1]);
-                    Nx.push_back(C[i]);
-                }
-                if (cp.nr) k_copy(cp, n, s);
+            const cd a = col[j];
+            const double rho = std::hypot(std::abs(a), q.hn);
+            if (rho == 0.0) {
+                q.cs[j] = 1.0;
+                q.sn[j] = 0.0;
+            } else if (std::abs(a) == 0.0) {
+                q.cs[j] = 0.0;
+                q.sn[j] = 1.0;
+                col[j] = q.hn;
+            } else {
+                const cd phase = a / std::abs(a);
+                q.cs[j] = std::abs(a) / rho;
+                q.sn[j] = phase * q.hn / rho;
+                col[j] = phase * rho;
             }
-            // ---- the next basis vector, V[j + 1] = w / hn; a full cycle ends
-            if (!Nx.empty()) {
-                VecOpsT<T> v{};
-                for (int qi : Nx) {
-                    Rhs<T>& q = R[qi];
-                    v.dst[v.nr] = row(q, q.j + 1);
-                    v.a[v.nr] = q.w.p;
-                    v.f[v.nr++] = 1.0 / q.hn;
-                }
-                k_scale(v, n, s);
-                for (int qi : Nx) {
-                    Rhs<T>& q = R[qi];
-                    if (q.j + 1 >= restart) E.push_back(qi);
-                    else q.j++;
-                }
+            col[j + 1] = 0.0;
+            for (int i = 0; i <= j + 1; ++i) q.H[(size_t)i * ld + j] = col[i];
+            q.g[j + 1] = -std::conj(q.sn[j]) * q.g[j];
+            q.g[j] = q.cs[j] * q.g[j];
+            q.its++;
+            q.k = j + 1;
+            double hnorm = 0;
+            for (int i = 0; i <= j; ++i) hnorm += std::norm(q.h[i]);
+            hnorm = std::sqrt(hnorm);
+            if (q.hn <= 1e-14 * hnorm || q.its >= max_its_of(q)) E.push_back(qi);
+            else if (std::abs(q.g[j + 1]) <= q.aim) C.push_back(qi);
+            else Nx.push_back(qi);
+        }
+        // ---- true-residual checks: xt = x + V y, r = b - K M^-1 xt
+        if (!C.empty()) {
+            std::vector<std::vector<cd>> ys;
+            std::vector<cz*> dst;
+            std::vector<const cz*> from;
+            for (int qi : C) {
+                ys.push_back(back_substitute(R[qi], R[qi].k));
+                dst.push_back(R[qi].xt.p);
+                from.push_back(R[qi].x.p);
             }
-            // ---- cycle ends: x += V y (least squares by back substitution), r = b - K M^-1 x
-            if (!E.empty()) {
-                std::vector<std::vector<cd>> ys;
-                std::vector<cz*> dst;
-                std::vector<const cz*> from;
-                for (int qi : E) {
-                    Rhs<T>& q = R[qi];
-                    int k = q.k;
-                    while (k > 0 && q.H[(size_t)(k - 1) * restart + (k - 1)] == cd(0)) --k;
-                    ys.push_back(k ? back_substitute(q, k) : std::vector<cd>());
-                    dst.push_back(q.x.p);
-                    from.push_back(q.x.p);
+            combine(C, ys, dst, from);
+            residuals(C, dst);
+            std::vector<const cz*> rs;
+            for (int qi : C) rs.push_back(R[qi].r.p);
+            std::vector<double> rt;
+            norms(C, rs, rt);
+            VecOpsT<T> cp{};
+            for (size_t i = 0; i < C.size(); ++i) {
+                Rhs<T>& q = R[C[i]];
+                q.resid = rt[i] / q.bnorm;
+                if (rt[i] <= q.p.tol * q.bnorm) {
+                    cp.dst[cp.nr] = q.x.p;
+                    cp.a[cp.nr++] = q.xt.p;
+                    q.state = DONE;
+                    q.conv = true;
+                    continue;
+                }
+                q.misses++;
+                if (q.misses >= (q.p.check_misses > 0 ? q.p.check_misses : 4)) {     // restart from the check: x = xt, r = rt
+                    cp.dst[cp.nr] = q.x.p;
+                    cp.a[cp.nr++] = q.xt.p;
                     q.state = START;
+                    continue;
                 }
-                combine(E, ys, dst, from);
-                residuals(E, dst);
+                q.aim = 0.5 * std::abs(q.g[q.j + 1]);
+                Nx.push_back(C[i]);
             }
+            if (cp.nr) k_copy(cp, n, s);
+        }
+        // ---- the next basis vector, V[j + 1] = w / hn; a full cycle ends
+        if (!Nx.empty()) {
+            VecOpsT<T> v{};
+            for (int qi : Nx) {
+                Rhs<T>& q = R[qi];
+                v.dst[v.nr] = row(q, q.j + 1);
+                v.a[v.nr] = q.w.p;
+                v.f[v.nr++] = 1.0 / q.hn;
+            }
+            k_scale(v, n, s);
+            for (int qi : Nx) {
+                Rhs<T>& q = R[qi];
+                if (q.j + 1 >= restart) E.push_back(qi);
+                else q.j++;
+            }
+        }
+        // ---- cycle ends: x += V y (least squares by back substitution), r = b - K M^-1 x
+        if (!E.empty()) {
+            std::vector<std::vector<cd>> ys;
+            std::vector<cz*> dst;
+            std::vector<const cz*> from;
+            for (int qi : E) {
+                Rhs<T>& q = R[qi];
+                int k = q.k;
+                while (k > 0 && q.H[(size_t)(k - 1) * restart + (k - 1)] == cd(0)) --k;
+                ys.push_back(k ? back_substitute(q, k) : std::vector<cd>());
+                dst.push_back(q.x.p);
+                from.push_back(q.x.p);
+                q.state = START;
+            }
+            combine(E, ys, dst, from);
+            residuals(E, dst);
+        }
+        return true;
+    }
+
+    void run() {
+        init();
+        while (step()) {
         }
     }
 };
@@ -485,11 +537,10 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     const int n = st->nb + st->nk;
     if (!st->ctx->gmres) st->ctx->gmres.reset(new GmresWork);
-    Batch<double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), p, *st->ctx->gmres);
+    Batch<double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), *st->ctx->gmres);
     for (int i = 0; i < nr; ++i) {
         if (ops[i]->st != st) fail(GT_ERR_ARG, "a batch's operators must share one stack");
-        bt.R[i].op = ops[i];
-        bt.R[i].b = b[i];
+        bt.submit(i, ops[i], b[i], p);
     }
     bt.run();
     // x = M^-1 y (zero when y is)
@@ -583,11 +634,8 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
         ip.tol = inner_tol;
         ip.bnorm = 0;
         ip.restart = restart;
-        Batch<float> bt(st, na, restart, ip, gw);
-        for (int i = 0; i < na; ++i) {
-            bt.R[i].op = aop[i];
-            bt.R[i].b = r32[act[i]];
-        }
+        Batch<float> bt(st, na, restart, gw);
+        for (int i = 0; i < na; ++i) bt.submit(i, aop[i], r32[act[i]], ip);
         bt.run();
         // d = M^-1 y in FP64 (y widened), x += d, u = K x, r = b - u
         {
@@ -652,6 +700,113 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
         res[i].residual = resid[i];
     }
 }
+
+}  // namespace gt
+
+// A serving session: up to 16 slots over one stack and a GMRES workspace of its own, so sessions of several stacks
+// interleave on one context. A solve enters a free slot at any time and joins the running batch at its next step;
+// every slot runs the same state machine as in a batch, so its result is bit-identical to the same solve alone.
+struct gt_session_s {
+    gt_stack* st = nullptr;
+    int cap = 0, restart = 0;
+    gt::GmresWork gw;
+    std::unique_ptr<gt::Batch<double>> bt;
+    std::vector<gt::zd*> xout;             // the caller's solution vector of each slot
+};
+
+namespace gt {
+
+int64_t session_plan_bytes(const gt_stack* st, int cap, int restart) {
+    const int n = st->nb + st->nk;
+    const int rows = (restart > 0 ? restart : default_restart(n)) + 1;
+    int64_t b = GmresWork::plan_bytes((size_t)n, rows, cap);
+    if (cap > st->work.cap) {              // the stack's product scratch grows to the session's batch
+        const int64_t per = 2 * (int64_t)std::max<size_t>(1, st->t_elems) + 2 * (int64_t)std::max(1, st->nk) +
+                            std::max(1, st->nbr);
+        b += per * (cap - st->work.cap) * (int64_t)sizeof(zd);
+    }
+    return b;
+}
+
+gt_session* session_create(gt_stack* st, int cap, int restart) {
+    if (cap < 1 || cap > KMAX) fail(GT_ERR_ARG, "1 to 16 slots per session");
+    if (restart < 0) fail(GT_ERR_ARG, "negative restart");
+    const int n = st->nb + st->nk;
+    std::unique_ptr<gt_session_s> ss(new gt_session_s);
+    ss->st = st;
+    ss->cap = cap;
+    ss->restart = restart > 0 ? restart : default_restart(n);
+    ss->bt.reset(new Batch<double>(st, cap, ss->restart, ss->gw));
+    ss->xout.assign(cap, nullptr);
+    st->work.ensure(st, cap);
+    return ss.release();
+}
+
+int64_t session_bytes(const gt_session* ss) { return ss->gw.device_bytes(); }
+
+int session_submit(gt_session* ss, gt_op* op, const zd* b, zd* x, const gt_gmres_params& p) {
+    if (op->st != ss->st) fail(GT_ERR_ARG, "the operator belongs to another stack");
+    if (p.restart > 0 && p.restart != ss->restart) fail(GT_ERR_ARG, "the session's restart differs");
+    Batch<double>& bt = *ss->bt;
+    for (int i = 0; i < ss->cap; ++i) {
+        if (bt.R[i].state != FREE) continue;
+        bt.submit(i, op, b, p);
+        ss->xout[i] = x;
+        return i;
+    }
+    return -1;
+}
+
+// Up to max_steps Arnoldi steps of the running slots (the newly submitted set up first); the solves that stopped
+// get x = M^-1 y and wait, READY, for their poll. Returns the slots still running.
+int session_step(gt_session* ss, int max_steps, int* ready) {
+    Batch<double>& bt = *ss->bt;
+    bt.init();
+    for (int k = 0; k < max_steps; ++k)
+        if (!bt.step()) break;
+    std::vector<int> fin;
+    for (int i = 0; i < ss->cap; ++i)
+        if (bt.R[i].state == DONE) fin.push_back(i);
+    if (!fin.empty()) {
+        const int c = (int)fin.size();
+        std::vector<gt_op*> ops(c);
+        std::vector<const zd*> ys(c);
+        std::vector<zd*> xs(c);
+        for (int i = 0; i < c; ++i) {
+            ops[i] = bt.R[fin[i]].op;
+            ys[i] = bt.R[fin[i]].x.p;
+            xs[i] = ss->xout[fin[i]];
+        }
+        ops_precond<double>(ss->st, c, ops.data(), ys.data(), xs.data());
+        spin_sync(bt.s);
+        for (int i : fin) bt.R[i].state = READY;
+    }
+    int run = 0, rd = 0;
+    for (int i = 0; i < ss->cap; ++i) {
+        const State q = bt.R[i].state;
+        if (q == READY) rd++;
+        else if (q != FREE) run++;
+    }
+    if (ready) *ready = rd;
+    return run;
+}
+
+bool session_poll(gt_session* ss, int slot, gt_gmres_result* res) {
+    if (slot < 0 || slot >= ss->cap) fail(GT_ERR_ARG, "slot out of range");
+    Rhs<double>& q = ss->bt->R[slot];
+    if (q.state != READY) return false;
+    res->iterations = q.its;
+    res->converged = q.conv ? 1 : 0;
+    res->products = q.products;
+    res->residual = q.resid;
+    q.state = FREE;
+    ss->xout[slot] = nullptr;
+    return true;
+}
+
+void session_destroy(gt_session* ss) { delete ss; }
+
+gt_stack* session_stack(gt_session* ss) { return ss->st; }
 
 }  // namespace gt
 

@@ -95,6 +95,7 @@ struct gt_stack_s {
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
     std::vector<double> h_W;                  // its host copy (the columns a derived stack extends)
     int k = 0;
+    bool derived = false;                     // a what-if stack: its core belongs to the stack it came from
 
     explicit gt_stack_s(std::shared_ptr<gt::StackCore> c = std::make_shared<gt::StackCore>())
         : core(std::move(c)), o(core->o), tw(core->tw), twf(core->twf), node_sheet(core->node_sheet),

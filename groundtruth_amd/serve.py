@@ -1,0 +1,385 @@
+"""Serving what-if solves on one device: boards stay resident, requests are batched continuously, admission is by
+device-memory footprint, and every request's latency is recorded.
+
+The reference solves a board's (model, frequency) pairs as independent tasks: `run_parallel` (simulate.py:3964-4013)
+runs them on a thread pool, and `Admission` (simulate.py:3930-3961) lets a task start only while the memory it
+plans to use fits the budget. Here the tasks are GMRES solves on the device, and they share launches instead of
+threads: a `Session` (gt_session) runs up to 16 solves of one stack as one batch, a new solve joins it between two
+Arnoldi steps, and a finished one leaves without waiting for the rest. `Server` keeps one session per stack in use
+(a board, or a what-if stack derived from it, peec.py:1871-1909), turns them round-robin, admits the sessions by
+their exact device bytes (gt_session_plan_bytes, the footprint gpu.py:593-605 `device_bytes` estimates), and
+reports p50/p99 latency and solves per second.
+
+A served solve is the same solve as `RouteExcitation.solve(f)`: the same right-hand side, norm and GMRES state
+machine, so its solution is bit-identical alone and in any mix (the library's batches keep each right-hand side's
+arithmetic to itself).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .peec import BATCH, GMRES_CHECK_MISSES, GMRES_EXTRA_CYCLES, PlaneStack, RouteExcitation
+from .problem import SolverError
+
+
+def session_plan_bytes(stack: PlaneStack, capacity: int = BATCH, restart: Optional[int] = None) -> int:
+    """Device bytes a session of `capacity` slots on `stack` adds: its GMRES workspace and the growth of the stack's
+    product scratch to that batch."""
+    b = int(_lib.lib().gt_session_plan_bytes(stack.handle, int(capacity), int(restart or 0)))
+    if b < 0:
+        raise ValueError("1 to 16 slots and a non-negative restart")
+    return b
+
+
+class Session:
+    """Continuous batching of the solves of one stack (gt_session): `submit` puts a solve into a free slot, `step`
+    advances the running ones, `poll` collects a finished one."""
+
+    def __init__(self, stack: PlaneStack, capacity: int = BATCH, restart: Optional[int] = None):
+        self.stack = stack
+        self.capacity = int(capacity)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().gt_session_create(stack.handle, self.capacity, int(restart or 0), C.byref(h)),
+                   "gt_session_create")
+        self._h = h
+
+    def submit(self, op, b_dev, x_dev, tol: float, bnorm: float = 0.0, max_cycles: int = 8,
+               extra_cycles: int = GMRES_EXTRA_CYCLES, max_its: int = 0) -> int:
+        """The slot the solve x = M^-1 y, (K M^-1) y = b took, or -1 when every slot is taken. b_dev, x_dev: torch
+        complex128 device vectors of nb + n_keep values, b's work finished; b stays unchanged until the poll."""
+        p = _lib.GmresParams(float(tol), float(bnorm), 0, int(max_cycles), int(extra_cycles), int(max_its),
+                             GMRES_CHECK_MISSES)
+        slot = C.c_int32()
+        _lib.check(_lib.lib().gt_session_submit(self._h, op.handle, C.c_void_p(b_dev.data_ptr()),
+                                                C.c_void_p(x_dev.data_ptr()), C.byref(p), C.byref(slot)),
+                   "gt_session_submit")
+        return int(slot.value)
+
+    def step(self, max_steps: int = 1):
+        """Set up the new solves, advance the running ones by up to max_steps Arnoldi steps, finish the stopped
+        ones. (running, ready)."""
+        run, rd = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().gt_session_step(self._h, int(max_steps), C.byref(run), C.byref(rd)),
+                   "gt_session_step")
+        return int(run.value), int(rd.value)
+
+    def poll(self, slot: int):
+        """The finished solve's GmresResult (its slot then free), or None while it runs."""
+        res, done = _lib.GmresResult(), C.c_int32()
+        _lib.check(_lib.lib().gt_session_poll(self._h, int(slot), C.byref(res), C.byref(done)), "gt_session_poll")
+        return res if done.value else None
+
+    def device_bytes(self) -> int:
+        return int(_lib.lib().gt_session_device_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            _lib.lib().gt_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib._LIB is not None:
+                _lib._LIB.gt_session_destroy(self._h)
+        except Exception:       # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+class Admission:
+    """Device-memory admission after simulate.py:3930-3961 `Admission`: a claim is granted while the bytes held
+    plus its own fit the budget, and always when nothing is held (a claim is never left waiting for memory that
+    nothing will free). Non-blocking: the serving loop asks again on its next turn. `reserve` takes bytes that stay
+    held for good (the resident boards) and count against the budget without counting as a claim."""
+
+    def __init__(self, budget: float):
+        self.budget = float(budget)
+        self.used = 0.0
+        self.used_peak = 0.0
+        self.held = 0
+        self.peak = 0
+
+    def reserve(self, nbytes: float) -> bool:
+        if self.used + nbytes > self.budget:
+            return False
+        self.used += nbytes
+        self.used_peak = max(self.used_peak, self.used)
+        return True
+
+    def try_hold(self, need: float) -> bool:
+        if self.held and self.used + need > self.budget:
+            return False
+        self.used += need
+        self.used_peak = max(self.used_peak, self.used)
+        self.held += 1
+        self.peak = max(self.peak, self.held)
+        return True
+
+    def release(self, need: float):
+        self.used -= need
+        self.held -= 1
+
+
+@dataclass
+class Request:
+    """One what-if request: a net (route) of a resident board, the frequencies to solve it at and, for a what-if,
+    the branches (candidate vias) added to the board. `solutions` (one Solution per frequency, in order) and
+    `latency` (s, from submission to the last solve's end) are filled when it completes."""
+    board: str
+    route: object                    # .trace, .driver, .receiver (a boards.Route), as RouteExcitation takes them
+    fs: list
+    extra: list = field(default_factory=list)
+    tol: float = 1e-8
+    max_cycles: int = 8
+    tag: object = None
+    # filled by the server
+    arrival: float = 0.0
+    done_at: float = 0.0
+    solutions: list = field(default_factory=list)
+    error: Optional[str] = None
+
+    @property
+    def latency(self) -> float:
+        return self.done_at - self.arrival
+
+
+class _Solve:
+    """One (request, frequency) solve in flight."""
+    __slots__ = ("req", "k", "f", "ex", "op", "b", "x", "bnorm", "slot")
+
+    def __init__(self, req, k, f, ex, op, b, x, bnorm):
+        self.req, self.k, self.f, self.ex, self.op, self.b, self.x, self.bnorm = req, k, f, ex, op, b, x, bnorm
+        self.slot = -1
+
+
+class _Served:
+    """A stack the server runs solves on, its session and the solves waiting for or holding its slots."""
+
+    def __init__(self, stack: PlaneStack, need: float):
+        self.stack = stack
+        self.need = need
+        self.session: Optional[Session] = None
+        self.whatif = False
+        self.waiting: deque = deque()
+        self.slots: dict = {}
+        self.last_used = 0.0
+
+
+class Server:
+    """The serving loop of one device. `add_board` makes a board resident (its spectra and Schur factor stay on the
+    device); `submit` queues a request; `turn` admits what fits, steps every session once and collects the finished
+    solves; `drain` turns until everything submitted has completed.
+
+    Per turn, each session with work gets `steps_per_turn` Arnoldi steps; a new solve enters its stack's session as
+    soon as a slot is free, so solves of different requests and frequencies share launches while they overlap. A
+    what-if request gets the stack derived from its board with the added branches (its own session: a batch serves
+    one stack). Sessions are admitted by their planned device bytes against `budget` (default: 90 % of the device
+    memory free when the server starts); idle sessions are closed, oldest first, when a new one needs their room."""
+
+    def __init__(self, device: int = 0, budget: Optional[float] = None, capacity: int = BATCH,
+                 steps_per_turn: int = 1, restart: Optional[int] = None):
+        import torch
+        self.device = device
+        self.dev = f"cuda:{device}"
+        self.capacity = int(capacity)
+        self.steps_per_turn = int(steps_per_turn)
+        self.restart = restart
+        if budget is None:
+            free, _ = torch.cuda.mem_get_info(device)
+            budget = 0.9 * free
+        self.admission = Admission(budget)
+        self.boards: dict = {}
+        self._excitations: dict = {}
+        self._served: OrderedDict = OrderedDict()   # id(stack) -> _Served
+        self._pending: deque = deque()              # requests not yet split into solves (their stack not admitted)
+        self._open: dict = {}                       # id(request) -> solves left
+        self.completed: list = []
+        self.solves_done = 0
+        self.t0: Optional[float] = None
+
+    # ---------------------------------------------------------------- boards and requests
+    def add_board(self, name: str, stack: PlaneStack):
+        """Make `stack` resident and servable as `name` (its device bytes are held for the server's life)."""
+        stack.handle
+        if not self.admission.reserve(stack.device_bytes()):
+            raise MemoryError(f"board {name} does not fit the device budget")
+        self.boards[name] = stack
+
+    def submit(self, req: Request, now: Optional[float] = None):
+        if req.board not in self.boards:
+            raise ValueError(f"unknown board {req.board!r}")
+        req.arrival = time.perf_counter() if now is None else now
+        if self.t0 is None:
+            self.t0 = req.arrival
+        self._pending.append(req)
+
+    def busy(self) -> bool:
+        return bool(self._pending or self._open)
+
+    # ---------------------------------------------------------------- the loop
+    def _stack_for(self, req: Request) -> PlaneStack:
+        base = self.boards[req.board]
+        return base.with_branches(req.extra) if req.extra else base
+
+    def _excitation(self, req: Request, stack: PlaneStack) -> RouteExcitation:
+        """The route's excitation on the board (its strip coupling computed once per board and route), moved onto
+        a what-if stack by RouteExcitation.on."""
+        key = (req.board, id(req.route))
+        hit = self._excitations.get(key)
+        if hit is None:
+            r = req.route
+            hit = self._excitations[key] = (r, RouteExcitation(self.boards[req.board], r.trace, r.driver,
+                                                                r.receiver))
+        ex = hit[1]
+        return ex if stack is self.boards[req.board] else ex.on(stack)
+
+    def _admit_served(self, stack: PlaneStack, n_solves: int) -> Optional[_Served]:
+        """The stack's session, made when its bytes are admitted (None: not now). A board's session has every
+        slot; a what-if stack's, made for one request, as many as that request's solves."""
+        sv = self._served.get(id(stack))
+        if sv is not None:
+            return sv
+        whatif = stack is not None and all(stack is not b for b in self.boards.values())
+        cap = min(self.capacity, max(1, n_solves)) if whatif else self.capacity
+        need = session_plan_bytes(stack, cap, self.restart)
+        if whatif:
+            need += stack.device_bytes()           # the what-if stack's own incidence and Woodbury columns
+        while not self.admission.try_hold(need):
+            idle = [k for k, s in self._served.items() if not s.slots and not s.waiting]
+            if not idle:
+                return None
+            self._close(idle[0])
+        sv = _Served(stack, need)
+        sv.whatif = whatif
+        sv.session = Session(stack, cap, self.restart)
+        self._served[id(stack)] = sv
+        return sv
+
+    def _close(self, key):
+        sv = self._served.pop(key)
+        if sv.session is not None:
+            sv.session.close()
+        self.admission.release(sv.need)
+
+    def _split(self, req: Request, sv: _Served):
+        """The request's solves, queued on its stack's session: the right-hand sides of RouteExcitation.solve."""
+        import torch
+        stack = sv.stack
+        ex = self._excitation(req, stack)
+        req.solutions = [None] * len(req.fs)
+        for k, f in enumerate(req.fs):
+            rhs_top, J = ex.rhs(f)
+            rhs = np.concatenate([np.asarray(rhs_top, complex), np.asarray(J, float)[stack.keep].astype(complex)])
+            b = torch.from_numpy(rhs).to(self.dev)
+            x = torch.empty_like(b)
+            sv.waiting.append(_Solve(req, k, f, ex, stack.operator(f), b, x, float(np.linalg.norm(rhs))))
+        self._open[id(req)] = len(req.fs)
+
+    def turn(self) -> list:
+        """Admit, step every session with work once, collect. The requests completed in this turn."""
+        import torch
+        # requests whose stack's session is (or can be) admitted
+        still = deque()
+        split = False
+        while self._pending:
+            req = self._pending.popleft()
+            try:
+                stack = self._stack_for(req)
+                sv = self._admit_served(stack, len(req.fs))
+            except (SolverError, ValueError, NotImplementedError) as e:
+                req.error = str(e)
+                req.done_at = time.perf_counter()
+                self.completed.append(req)
+                continue
+            if sv is None:
+                still.append(req)
+                continue
+            self._split(req, sv)
+            split = True
+        self._pending = still
+        if split:
+            torch.cuda.synchronize(self.device)        # the new right-hand sides are on the device
+        finished = []
+        for key in list(self._served):
+            sv = self._served[key]
+            if not sv.waiting and not sv.slots:
+                continue
+            while sv.waiting:
+                s = sv.waiting[0]
+                slot = sv.session.submit(s.op, s.b, s.x, s.req.tol, bnorm=s.bnorm, max_cycles=s.req.max_cycles)
+                if slot < 0:
+                    break
+                s.slot = slot
+                sv.slots[slot] = sv.waiting.popleft()
+            _, ready = sv.session.step(self.steps_per_turn)
+            if ready:
+                finished += self._collect(sv)
+            sv.last_used = time.perf_counter()
+            self._served.move_to_end(key)
+            if sv.whatif and not sv.slots and not sv.waiting:
+                self._close(key)                       # a what-if stack serves its one request
+        return finished
+
+    def _collect(self, sv: _Served) -> list:
+        done = []
+        for slot in list(sv.slots):
+            res = sv.session.poll(slot)
+            if res is None:
+                continue
+            s = sv.slots.pop(slot)
+            self.solves_done += 1
+            req = s.req
+            if not res.converged:
+                req.error = f"GMRES did not converge at {s.f:.4g} Hz ({res.iterations} iterations)"
+            else:
+                st = sv.stack
+                xv = s.x.cpu().numpy()
+                V = np.zeros(st.n_nodes, complex)
+                V[st.keep] = xv[st.nb:]
+                sol = s.ex.unpack(s.f, xv[:st.nb], V, int(res.iterations), xv)
+                sol.residual = float(res.residual)
+                req.solutions[s.k] = sol
+            self._open[id(req)] -= 1
+            if self._open[id(req)] == 0:
+                del self._open[id(req)]
+                req.done_at = time.perf_counter()
+                self.completed.append(req)
+                done.append(req)
+        return done
+
+    def drain(self, limit_s: float = 600.0):
+        t_end = time.perf_counter() + limit_s
+        while self.busy():
+            self.turn()
+            if time.perf_counter() > t_end:
+                raise TimeoutError("requests still open after the time limit")
+
+    def close(self):
+        for key in list(self._served):
+            self._close(key)
+
+    # ---------------------------------------------------------------- statistics
+    def stats(self) -> dict:
+        """p50 / p99 latency (ms) of the completed requests, solves per second since the first submission, and the
+        admission's peaks."""
+        lat = np.array([r.latency for r in self.completed if r.error is None]) * 1e3
+        span = (max(r.done_at for r in self.completed) - self.t0) if self.completed else 0.0
+        return {
+            "requests": len(self.completed),
+            "failed": sum(r.error is not None for r in self.completed),
+            "solves": self.solves_done,
+            "p50_ms": float(np.percentile(lat, 50)) if lat.size else math.nan,
+            "p99_ms": float(np.percentile(lat, 99)) if lat.size else math.nan,
+            "max_ms": float(lat.max()) if lat.size else math.nan,
+            "solves_per_s": self.solves_done / span if span > 0 else math.nan,
+            "peak_sessions": self.admission.peak,
+            "peak_bytes": self.admission.used_peak,
+            "budget_bytes": self.admission.budget,
+        }

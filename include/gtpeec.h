@@ -80,7 +80,8 @@ typedef struct {
 
 int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* desc, gt_stack** out);
 void gt_stack_destroy(gt_stack* stack);
-/* Device bytes the stack holds (the footprint gpu.py:593-605 `device_bytes` estimates). */
+/* Device bytes the stack holds (the footprint gpu.py:593-605 `device_bytes` estimates); for a what-if stack
+ * (gt_stack_derive), only what it adds to its base's. */
 int64_t gt_stack_device_bytes(const gt_stack* stack);
 
 /* L_p I (peec.py:2115-2135 Lp_matvec) on device vectors of nxb + nyb complex values: scatter, row FFT, column
@@ -199,6 +200,32 @@ int gt_gmres_batch(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* c
 int gt_gmres_batch_mixed(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
                          const gt_gmres_params* params, double inner_tol, int32_t max_refine,
                          gt_gmres_result* results);
+
+/* ---------------------------------------------------------------- serving sessions
+ * Continuous batching of the solves of one stack (the device form of simulate.py:3964-4013 `run_parallel`, whose
+ * threads each run a whole solve: here the solves share every launch and join or leave the batch between steps).
+ * A session holds `capacity` <= 16 slots and a GMRES workspace of its own (restart: 0 for the default of
+ * peec.py:3082-3090 `gmres_restart`), so sessions of several stacks, a board and its what-if stacks, interleave on
+ * one context. gt_session_submit puts a solve into a free slot (*slot = -1 when all are taken); each
+ * gt_session_step sets up the newly submitted solves, advances every running one by up to max_steps Arnoldi steps,
+ * and writes x_dev of the solves that stopped; gt_session_poll gives a finished solve's outcome and frees its slot.
+ * Each solve follows gt_gmres_batch's state machine, so its x and result are bit-identical to the same solve by
+ * gt_gmres_dev, whatever shares the session with it. b_dev must stay unchanged until the poll; x_dev is written
+ * by the step that finishes the solve. */
+typedef struct gt_session_s gt_session;
+int gt_session_create(gt_stack* stack, int32_t capacity, int32_t restart, gt_session** out);
+void gt_session_destroy(gt_session* session);
+int gt_session_submit(gt_session* session, gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* params,
+                      int32_t* slot);
+/* *running: slots with a solve under way; *ready: finished solves waiting for their poll. */
+int gt_session_step(gt_session* session, int32_t max_steps, int32_t* running, int32_t* ready);
+/* *done = 1 and *result filled when the slot's solve has finished (the slot is then free), else *done = 0. */
+int gt_session_poll(gt_session* session, int32_t slot, gt_gmres_result* result, int32_t* done);
+/* Device bytes (the footprint admission plans with: gpu.py:593-605 `device_bytes`, made exact): what the session
+ * holds, and what a session of `capacity` slots on `stack` would add, its workspace and the growth of the stack's
+ * product scratch, before it is made. */
+int64_t gt_session_device_bytes(const gt_session* session);
+int64_t gt_session_plan_bytes(const gt_stack* stack, int32_t capacity, int32_t restart);
 
 #ifdef __cplusplus
 }

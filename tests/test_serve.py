@@ -1,0 +1,176 @@
+"""Serving: continuous batching of solves (gt_session), admission by device bytes, and the request loop.
+
+CPU: the admission rule of simulate.py:3930-3961 `Admission` (grant while the budget holds, always when nothing is
+held), and the library's session entry points are exported.
+GPU: a session's solves, joining and leaving a running batch at different steps, give the same bits as each solve
+alone (gt_gmres); the planned bytes are the bytes a session holds; the server's answers for a stream of board and
+what-if requests equal each request solved alone, with a roomy budget and with one that admits a session at a time.
+"""
+import numpy as np
+import pytest
+
+from groundtruth_amd import _lib, boards, peec
+from groundtruth_amd.serve import Admission, Request, Server, Session, session_plan_bytes
+from test_whatif import _extra_vias
+
+
+def test_admission_grants_while_the_budget_holds_and_always_when_empty():
+    a = Admission(100)
+    assert a.try_hold(150)                  # nothing held: granted however large
+    assert not a.try_hold(1)
+    a.release(150)
+    assert a.try_hold(60) and a.try_hold(40) and not a.try_hold(1)
+    assert a.peak == 2 and a.used_peak == 150
+    a.release(60)
+    assert a.try_hold(50) and a.used == 90 and a.held == 2
+    b = Admission(100)
+    assert b.reserve(80) and not b.reserve(30) and b.held == 0
+    assert b.try_hold(50)                   # no claim held: granted past the budget
+    assert not b.try_hold(1)
+
+
+def test_session_entry_points_are_exported():
+    L = _lib.lib()
+    for name in ("gt_session_create", "gt_session_destroy", "gt_session_submit", "gt_session_step",
+                 "gt_session_poll", "gt_session_device_bytes", "gt_session_plan_bytes"):
+        assert hasattr(L, name)
+
+
+def _board():
+    return boards.scaled("two_layer", 24, n_vias=5, n_freqs=6)
+
+
+def _rhs(stack, ex, f):
+    import torch
+    rhs_top, J = ex.rhs(f)
+    rhs = np.concatenate([np.asarray(rhs_top, complex), np.asarray(J, float)[stack.keep].astype(complex)])
+    b = torch.from_numpy(rhs).to("cuda:0")
+    return b, torch.empty_like(b), float(np.linalg.norm(rhs))
+
+
+@pytest.mark.gpu
+def test_session_solves_joining_a_running_batch_are_bit_identical_to_solves_alone():
+    import torch
+    b = _board()
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    fs = list(b.freqs)
+    alone = [ex.solve(f, tol=1e-10) for f in fs]
+    ses = Session(st, capacity=4)
+    assert ses.device_bytes() > 0
+    vec = {}
+    queue = list(range(len(fs)))
+    slots = {}
+    out = {}
+    steps = 0
+    while queue or slots:
+        # one new solve every third step while a slot is free: solves join the batch at different Arnoldi steps
+        if queue and steps % 3 == 0:
+            k = queue[0]
+            if k not in vec:
+                vec[k] = _rhs(st, ex, fs[k])
+                torch.cuda.synchronize()
+            bd, xd, bn = vec[k]
+            slot = ses.submit(st.operator(fs[k]), bd, xd, 1e-10, bnorm=bn)
+            if slot >= 0:
+                slots[slot] = queue.pop(0)
+        ses.step(1)
+        steps += 1
+        for slot in list(slots):
+            res = ses.poll(slot)
+            if res is not None:
+                out[slots.pop(slot)] = res
+        assert steps < 5000
+    assert len(out) == len(fs)
+    for k, one in enumerate(alone):
+        assert out[k].converged and out[k].iterations == one.iterations
+        assert np.array_equal(vec[k][1].cpu().numpy(), one.x)
+    ses.close()
+
+
+@pytest.mark.gpu
+def test_session_plan_bytes_are_the_bytes_it_holds():
+    b = _board()
+    st = peec.PlaneStack(b.layers, b.branches)
+    st.handle
+    for cap in (1, 5, 16):
+        plan = session_plan_bytes(st, cap)
+        before = st.device_bytes()
+        ses = Session(st, capacity=cap)
+        assert plan == ses.device_bytes() + (st.device_bytes() - before)
+        ses.close()
+
+
+@pytest.mark.gpu
+def test_a_full_session_refuses_a_solve_and_a_foreign_operator_is_an_error():
+    b = _board()
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    other = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    ses = Session(st, capacity=1)
+    bd, xd, bn = _rhs(st, ex, 1e8)
+    import torch
+    torch.cuda.synchronize()
+    assert ses.submit(st.operator(1e8), bd, xd, 1e-8, bnorm=bn) == 0
+    bd2, xd2, _ = _rhs(st, ex, 2e8)
+    assert ses.submit(st.operator(2e8), bd2, xd2, 1e-8) == -1
+    with pytest.raises(ValueError):
+        ses.submit(other.operator(1e8), bd2, xd2, 1e-8)
+    while ses.poll(0) is None:
+        ses.step(4)
+    ses.close()
+
+
+def _requests(b, n):
+    """Board and what-if requests over the board's frequencies (seeded)."""
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(n):
+        fs = [float(f) for f in rng.choice(b.freqs, size=int(rng.integers(1, 4)), replace=False)]
+        extra = _extra_vias(b, int(rng.integers(1, 3)), 100 + i) if i % 2 else []
+        reqs.append(Request("b", b.routes[0], fs, extra=extra, tol=1e-10, tag=i))
+    return reqs
+
+
+def _alone(b, req):
+    st = peec.PlaneStack(b.layers, b.branches)
+    r = req.route
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    if req.extra:
+        ex = ex.on(st.with_branches(req.extra))
+    return [ex.solve(f, tol=req.tol) for f in req.fs]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("budget", ["roomy", "tight"])
+def test_served_requests_equal_each_request_solved_alone(budget):
+    b = _board()
+    st = peec.PlaneStack(b.layers, b.branches)
+    st.handle
+    cap_bytes = None if budget == "roomy" else st.device_bytes() + 1     # one session at a time
+    srv = Server(budget=cap_bytes, capacity=8)
+    srv.add_board("b", st)
+    reqs = _requests(b, 6)
+    # staggered arrivals: two requests, a few turns, the rest one per turn
+    srv.submit(reqs[0])
+    srv.submit(reqs[1])
+    for _ in range(3):
+        srv.turn()
+    for q in reqs[2:]:
+        srv.submit(q)
+        srv.turn()
+    srv.drain(limit_s=300)
+    stats = srv.stats()
+    assert stats["requests"] == 6 and stats["failed"] == 0
+    assert stats["solves"] == sum(len(q.fs) for q in reqs)
+    assert stats["p99_ms"] >= stats["p50_ms"] > 0
+    if budget == "tight":
+        assert srv.admission.peak == 1      # one session at a time
+    for q in reqs:
+        ref = _alone(b, q)
+        for got, one in zip(q.solutions, ref):
+            assert np.array_equal(got.x, one.x) and got.iterations == one.iterations
+            assert got.Z_loop == one.Z_loop
+    srv.close()
