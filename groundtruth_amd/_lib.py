@@ -92,6 +92,13 @@ def lib():
     if _LIB is None:
         if not LIB_PATH.exists():
             raise DeviceError(f"{LIB_PATH} is missing: build it first (python -m groundtruth_amd.build)")
+        # torch first: its wheel carries its own HIP and HSA runtimes under the same soname as ROCm's. Loaded
+        # after this library, torch would bring a second runtime into the process, which then finds no device;
+        # loaded before it, this library binds to torch's, and the two share one runtime and device memory.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

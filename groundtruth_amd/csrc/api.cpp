@@ -443,34 +443,10 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
             st->h_b_keep.push_back(b_keep_extra[e]);
         }
         upload_incidence(st.get());
-        // W' = B^-1 U' for the added branches by the device solves (up to 16 columns at a time), after the base's
+        // W' = [W  B^-1 U']: the base's columns copied on the device, the added ones by device solves
         hipStream_t s = st->ctx->stream;
-        const int k0 = base->k, k = k0 + n_extra;
-        std::vector<double> W(base->h_W);
-        W.resize((size_t)k * nk, 0.0);
-        if (n_extra > 0 && nk > 0) {
-            DevBuf<zd> bd((size_t)nk * KMAX), xd((size_t)nk * KMAX);
-            std::vector<zd> bh((size_t)nk * KMAX), xh((size_t)nk * KMAX);
-            for (int m0 = 0; m0 < n_extra; m0 += KMAX) {
-                const int nr = std::min(KMAX, n_extra - m0);
-                std::fill(bh.begin(), bh.end(), zd());
-                for (int c = 0; c < nr; ++c) {
-                    const int ka = a_keep_extra[m0 + c], kb = b_keep_extra[m0 + c];
-                    if (ka >= 0) bh[(size_t)ka * nr + c].re += 1.0;
-                    if (kb >= 0) bh[(size_t)kb * nr + c].re -= 1.0;
-                }
-                bd.upload(bh.data(), (size_t)nk * nr, s);
-                schur_apply(st.get(), bd.p, xd.p, nr);
-                GT_HIP(hipMemcpyAsync(xh.data(), xd.p, sizeof(zd) * (size_t)nk * nr, hipMemcpyDeviceToHost, s));
-                GT_HIP(hipStreamSynchronize(s));
-                for (int c = 0; c < nr; ++c)
-                    for (int v = 0; v < nk; ++v) W[(size_t)(k0 + m0 + c) * nk + v] = xh[(size_t)v * nr + c].re;
-            }
-        }
-        st->W.upload(W.empty() ? std::vector<double>{0.0} : W, s);
-        st->h_W = std::move(W);
-        st->k = k;
-        if (G_out) woodbury_gram(st.get(), G_out);
+        woodbury_extend(st.get(), base, n_extra, a_keep_extra, b_keep_extra);
+        if (G_out) woodbury_gram_device(st.get(), G_out);
         st->work.ensure(st.get(), 1);
         GT_HIP(hipStreamSynchronize(s));
         *out = st.release();

@@ -252,6 +252,83 @@ void schur_apply(gt_stack* st, const zd* b, zd* x, int nr) {
     schur_solve_device(*st->schur, b, x, nr, st->ctx->stream);
 }
 
+namespace {
+
+struct UnitCols {
+    int ka[KMAX], kb[KMAX];   // kept end nodes of the added branches (-1: a reference node)
+};
+
+// b (n_keep x nr interleaved) = the incidence columns U' e_c of nr added branches: +1 at node a, -1 at node b
+__global__ __launch_bounds__(256) void wb_units(zd* __restrict__ b, int nk, int nr, UnitCols u) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)nk * nr) return;
+    const int v = (int)(i / nr), c = (int)(i % nr);
+    const double val = (v == u.ka[c] ? 1.0 : 0.0) - (v == u.kb[c] ? 1.0 : 0.0);
+    b[i] = zd(val, 0.0);
+}
+
+// W[c] (a column of n_keep, c < nr) = Re x[:, c] of the interleaved solutions
+__global__ __launch_bounds__(256) void wb_take(const zd* __restrict__ x, int nk, int nr, double* __restrict__ W) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)nk * nr) return;
+    const int v = (int)(i / nr), c = (int)(i % nr);
+    W[(size_t)c * nk + v] = x[i].re;
+}
+
+// G = U^T W (k x k, row-major): G[r][c] = W[c][a_r] - W[c][b_r] over the branches r (kept ends; -1 a reference node)
+__global__ __launch_bounds__(256) void wb_gram(const double* __restrict__ W, int nk, int k, const int* __restrict__ ka,
+                                               const int* __restrict__ kb, double* __restrict__ G) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= k * k) return;
+    const int r = i / k, c = i % k;
+    double v = 0;
+    if (ka[r] >= 0) v += W[(size_t)c * nk + ka[r]];
+    if (kb[r] >= 0) v -= W[(size_t)c * nk + kb[r]];
+    G[i] = v;
+}
+
+}  // namespace
+
+void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra) {
+    hipStream_t s = st->ctx->stream;
+    const int nk = st->nk, k0 = base->k, k = k0 + n_extra;
+    st->W.alloc(std::max<size_t>(1, (size_t)k * nk));
+    if (k0 > 0 && nk > 0)
+        GT_HIP(hipMemcpyAsync(st->W.p, base->W.p, sizeof(double) * (size_t)k0 * nk, hipMemcpyDeviceToDevice, s));
+    if (n_extra > 0 && nk > 0) {
+        const int chunk = std::min(n_extra, KMAX);
+        st->work.ensure(st, chunk);
+        for (int m0 = 0; m0 < n_extra; m0 += KMAX) {
+            const int nr = std::min(KMAX, n_extra - m0);
+            UnitCols u{};
+            for (int c = 0; c < nr; ++c) {
+                u.ka[c] = a_extra[m0 + c];
+                u.kb[c] = b_extra[m0 + c];
+            }
+            const size_t cnt = (size_t)nk * nr;
+            hipLaunchKernelGGL(wb_units, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, st->work.s.p, nk, nr, u);
+            GT_LAUNCH("wb_units");
+            schur_apply(st, st->work.s.p, st->work.z.p, nr);
+            hipLaunchKernelGGL(wb_take, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, st->work.z.p, nk, nr,
+                               st->W.p + (size_t)(k0 + m0) * nk);
+            GT_LAUNCH("wb_take");
+        }
+    }
+    st->k = k;
+}
+
+void woodbury_gram_device(const gt_stack* st, double* G_host) {
+    const int k = st->k;
+    if (k == 0) return;
+    hipStream_t s = st->ctx->stream;
+    DevBuf<double> G((size_t)k * k);
+    hipLaunchKernelGGL(wb_gram, dim3(blocks_for((size_t)k * k, 256)), dim3(256), 0, s, st->W.p, st->nk, k,
+                       st->a_keep.p + st->ns, st->b_keep.p + st->ns, G.p);
+    GT_LAUNCH("wb_gram");
+    GT_HIP(hipMemcpyAsync(G_host, G.p, sizeof(double) * (size_t)k * k, hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+}
+
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");

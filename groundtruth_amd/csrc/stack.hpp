@@ -93,7 +93,7 @@ struct gt_stack_s {
     std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
     gt::Work work;
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
-    std::vector<double> h_W;                  // its host copy (the columns a derived stack extends)
+    std::vector<double> h_W;                  // its host copy (a factored stack's; a what-if stack's stay on the device)
     int k = 0;
     bool derived = false;                     // a what-if stack: its core belongs to the stack it came from
 
@@ -125,4 +125,8 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z);
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides
+// A what-if stack's Woodbury columns on the device: the base's W copied, then B^-1 U' for the added branches
+void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra);
+// G = U^T W (k x k row-major) of the stack's branches, from its device W, into host memory
+void woodbury_gram_device(const gt_stack* st, double* G_host);
 }  // namespace gt

@@ -369,22 +369,80 @@ class PlaneStack:
         nb = self.nb
         rhs = np.concatenate([np.asarray(rhs_top, complex), np.asarray(J, float)[self.keep].astype(complex)])
         bnorm = float(np.linalg.norm(rhs))
-        x0, r0 = None, rhs
-        Gs = [g for g in (guess or []) if g is not None and len(g) == len(rhs) and np.any(g)]
+        Gs = [g for g in (guess or []) if g is not None and len(g) == len(rhs)]
         if Gs:
-            KG = np.stack([self.matvec(op, np.asarray(g, complex)) for g in Gs], axis=1)
-            c, *_ = np.linalg.lstsq(KG, rhs, rcond=None)
-            r = rhs - KG @ c
-            if np.linalg.norm(r) < 0.9 * bnorm:
-                x0, r0 = np.stack(Gs, axis=1) @ c, r
-        x, its, ok, _ = self.gmres(op, r0, tol, bnorm=bnorm, max_cycles=max_cycles, restart=restart)
+            # the warm start and the solve on device vectors
+            import torch
+            dev = f"cuda:{self.device}"
+            rhs_d = torch.from_numpy(rhs).to(dev)
+            G_d = [torch.from_numpy(np.ascontiguousarray(g, np.complex128)).to(dev) for g in Gs]
+            torch.cuda.synchronize(self.device)
+            x0, r0 = self.warm_start_dev(op, rhs_d, G_d)
+            xd = torch.empty_like(rhs_d)
+            its, ok, _ = self.gmres_dev(op, r0, xd, tol, bnorm=bnorm, max_cycles=max_cycles, restart=restart)
+            with torch.cuda.stream(self.torch_stream()):
+                x = (xd + x0 if x0 is not None else xd).cpu().numpy()
+        else:
+            x, its, ok, _ = self.gmres(op, rhs, tol, bnorm=bnorm, max_cycles=max_cycles, restart=restart)
         if not ok:
             raise SolverError(f"GMRES did not converge at {f:.4g} Hz ({its} iterations)")
-        if x0 is not None:
-            x = x + x0
         V = np.zeros(self.n_nodes, complex)
         V[self.keep] = x[nb:]
         return x[:nb], V, its, x
+
+    def torch_stream(self):
+        """The library's stream as a torch stream: torch work in its context is ordered with the library's."""
+        import torch
+        self.handle
+        ts = getattr(self.ctx, "torch_stream", None)
+        if ts is None:
+            ts = self.ctx.torch_stream = torch.cuda.ExternalStream(_lib.lib().gt_ctx_stream(self.ctx.handle),
+                                                                   device=f"cuda:{self.device}")
+        return ts
+
+    def warm_start_dev(self, op: Operator, rhs, guess: list):
+        """(x0, r0) on device vectors (torch complex128, their work finished; the results ordered on the library's
+        stream): x0 = G c with c minimising |K G c - rhs| over the guesses, r0 = rhs - K x0 the residual left to
+        GMRES, taken when it removes at least a tenth of |rhs|; else (None, rhs) (peec.py:2568-2576). The products
+        K g and the m x m normal equations stay on the device; c is solved on the host."""
+        import torch
+        n = self.nb + self.n_keep
+        Gs = [g for g in guess if g is not None and g.numel() == n]
+        with torch.cuda.stream(self.torch_stream()):
+            Gs = [g for g in Gs if bool(torch.any(g != 0))]
+            if not Gs:
+                return None, rhs
+            KG = torch.empty((len(Gs), n), dtype=torch.complex128, device=rhs.device)
+            for i, g in enumerate(Gs):
+                _lib.check(_lib.lib().gt_matvec(op.handle, C.c_void_p(g.data_ptr()), C.c_void_p(KG[i].data_ptr())),
+                           "gt_matvec")
+            m = len(Gs)
+            # the normal equations (KG^H KG) c = KG^H rhs by dot products over the n unknowns
+            dots = [torch.vdot(KG[i], KG[j]) for i in range(m) for j in range(m)]
+            dots += [torch.vdot(KG[i], rhs) for i in range(m)]
+            v = torch.stack(dots).cpu().numpy()
+            A, y = v[: m * m].reshape(m, m), v[m * m:]
+            c, *_ = np.linalg.lstsq(A, y, rcond=None)
+            r = rhs.clone()
+            x0 = torch.zeros_like(rhs)
+            for i in range(m):
+                r.sub_(KG[i], alpha=complex(c[i]))
+                x0.add_(Gs[i], alpha=complex(c[i]))
+            if float(torch.linalg.vector_norm(r)) < 0.9 * float(torch.linalg.vector_norm(rhs)):
+                return x0, r
+        return None, rhs
+
+    def map_state_dev(self, x, stack: "PlaneStack"):
+        """map_state on a device vector (torch)."""
+        import torch
+        if x is None:
+            return None
+        if stack is self:
+            return x
+        if self._parent is stack and x.numel() == stack.nb + stack.n_keep:
+            z = torch.zeros(self.nb - stack.nb, dtype=x.dtype, device=x.device)
+            return torch.cat([x[:stack.nb], z, x[stack.nb:]])
+        return None
 
     def gmres(self, op: Operator, b: np.ndarray, tol: float, bnorm: Optional[float] = None, max_cycles: int = 8,
               restart: Optional[int] = None, extra_cycles: int = GMRES_EXTRA_CYCLES, max_its: Optional[int] = None):

@@ -138,6 +138,7 @@ class Request:
     extra: list = field(default_factory=list)
     tol: float = 1e-8
     max_cycles: int = 8
+    warm: bool = True                # a what-if starts from the board's solution at f when the server holds one
     tag: object = None
     # filled by the server
     arrival: float = 0.0
@@ -151,11 +152,12 @@ class Request:
 
 
 class _Solve:
-    """One (request, frequency) solve in flight."""
-    __slots__ = ("req", "k", "f", "ex", "op", "b", "x", "bnorm", "slot")
+    """One (request, frequency) solve in flight: GMRES on b = rhs - K x0 from the warm start x0 (None: cold)."""
+    __slots__ = ("req", "k", "f", "ex", "op", "b", "x", "bnorm", "x0", "slot")
 
-    def __init__(self, req, k, f, ex, op, b, x, bnorm):
+    def __init__(self, req, k, f, ex, op, b, x, bnorm, x0):
         self.req, self.k, self.f, self.ex, self.op, self.b, self.x, self.bnorm = req, k, f, ex, op, b, x, bnorm
+        self.x0 = x0
         self.slot = -1
 
 
@@ -202,15 +204,42 @@ class Server:
         self._open: dict = {}                       # id(request) -> solves left
         self.completed: list = []
         self.solves_done = 0
+        self.warm_starts = 0
         self.t0: Optional[float] = None
+        self._board_x: dict = {}                    # (board, route id, f key) -> the board's latest x (device)
 
     # ---------------------------------------------------------------- boards and requests
-    def add_board(self, name: str, stack: PlaneStack):
-        """Make `stack` resident and servable as `name` (its device bytes are held for the server's life)."""
+    def add_board(self, name: str, stack: PlaneStack, routes=()):
+        """Make `stack` resident and servable as `name` (its device bytes are held for the server's life), with the
+        excitations of `routes` (their coupling to the board's strips) computed ahead of the requests."""
         stack.handle
         if not self.admission.reserve(stack.device_bytes()):
             raise MemoryError(f"board {name} does not fit the device budget")
         self.boards[name] = stack
+        for r in routes:
+            self.excitation(name, r)
+
+    def prime(self, name: str, freqs, routes=None, tol: float = 1e-8):
+        """Solve the board's nets at `freqs` (all its registered routes by default) and keep the solutions: the
+        analysis of the board a what-if edits, whose solutions the what-ifs then start from."""
+        rts = routes if routes is not None else [r for (b, _), (r, _) in self._excitations.items() if b == name]
+        for r in rts:
+            self.submit(Request(name, r, [float(f) for f in freqs], tol=tol))
+        self.drain()
+
+    def board_solution(self, name: str, route, f: float) -> Optional[np.ndarray]:
+        """The board's latest solution x of the route at f (None when it has not been solved here)."""
+        x = self._board_x.get((name, id(route), PlaneStack.freq_key(f)))
+        return None if x is None else x.cpu().numpy()
+
+    def reset_stats(self):
+        """Forget the completed requests and the peaks (after a warm-up)."""
+        self.completed = []
+        self.solves_done = 0
+        self.warm_starts = 0
+        self.t0 = None
+        self.admission.used_peak = self.admission.used
+        self.admission.peak = self.admission.held
 
     def submit(self, req: Request, now: Optional[float] = None):
         if req.board not in self.boards:
@@ -228,16 +257,18 @@ class Server:
         base = self.boards[req.board]
         return base.with_branches(req.extra) if req.extra else base
 
-    def _excitation(self, req: Request, stack: PlaneStack) -> RouteExcitation:
-        """The route's excitation on the board (its strip coupling computed once per board and route), moved onto
-        a what-if stack by RouteExcitation.on."""
-        key = (req.board, id(req.route))
+    def excitation(self, board: str, route) -> RouteExcitation:
+        """The route's excitation on the board, its strip coupling computed once per board and route."""
+        key = (board, id(route))
         hit = self._excitations.get(key)
         if hit is None:
-            r = req.route
-            hit = self._excitations[key] = (r, RouteExcitation(self.boards[req.board], r.trace, r.driver,
-                                                                r.receiver))
-        ex = hit[1]
+            hit = self._excitations[key] = (route, RouteExcitation(self.boards[board], route.trace, route.driver,
+                                                                    route.receiver))
+        return hit[1]
+
+    def _excitation(self, req: Request, stack: PlaneStack) -> RouteExcitation:
+        """The request's excitation on its stack (moved onto a what-if stack by RouteExcitation.on)."""
+        ex = self.excitation(req.board, req.route)
         return ex if stack is self.boards[req.board] else ex.on(stack)
 
     def _admit_served(self, stack: PlaneStack, n_solves: int) -> Optional[_Served]:
@@ -269,17 +300,28 @@ class Server:
         self.admission.release(sv.need)
 
     def _split(self, req: Request, sv: _Served):
-        """The request's solves, queued on its stack's session: the right-hand sides of RouteExcitation.solve."""
+        """The request's solves, queued on its stack's session: the right-hand sides of RouteExcitation.solve, and
+        for a what-if the warm start of PlaneStack.solve from the board's solution at f (peec.py:2568-2576)."""
         import torch
         stack = sv.stack
+        board = self.boards[req.board]
         ex = self._excitation(req, stack)
         req.solutions = [None] * len(req.fs)
         for k, f in enumerate(req.fs):
             rhs_top, J = ex.rhs(f)
             rhs = np.concatenate([np.asarray(rhs_top, complex), np.asarray(J, float)[stack.keep].astype(complex)])
+            op = stack.operator(f)
             b = torch.from_numpy(rhs).to(self.dev)
+            x0 = None
+            if sv.whatif and req.warm:
+                xb = self._board_x.get((req.board, id(req.route), PlaneStack.freq_key(f)))
+                g = stack.map_state_dev(xb, board)
+                if g is not None:
+                    torch.cuda.synchronize(self.device)
+                    x0, b = stack.warm_start_dev(op, b, [g])
+                    self.warm_starts += x0 is not None
             x = torch.empty_like(b)
-            sv.waiting.append(_Solve(req, k, f, ex, stack.operator(f), b, x, float(np.linalg.norm(rhs))))
+            sv.waiting.append(_Solve(req, k, f, ex, op, b, x, float(np.linalg.norm(rhs)), x0))
         self._open[id(req)] = len(req.fs)
 
     def turn(self) -> list:
@@ -328,6 +370,7 @@ class Server:
         return finished
 
     def _collect(self, sv: _Served) -> list:
+        import torch
         done = []
         for slot in list(sv.slots):
             res = sv.session.poll(slot)
@@ -340,7 +383,14 @@ class Server:
                 req.error = f"GMRES did not converge at {s.f:.4g} Hz ({res.iterations} iterations)"
             else:
                 st = sv.stack
-                xv = s.x.cpu().numpy()
+                xd = s.x
+                if s.x0 is not None:
+                    with torch.cuda.stream(st.torch_stream()):
+                        xd = s.x + s.x0
+                if not sv.whatif:
+                    self._board_x[(req.board, id(req.route), PlaneStack.freq_key(s.f))] = xd
+                with torch.cuda.stream(st.torch_stream()):
+                    xv = xd.cpu().numpy()
                 V = np.zeros(st.n_nodes, complex)
                 V[st.keep] = xv[st.nb:]
                 sol = s.ex.unpack(s.f, xv[:st.nb], V, int(res.iterations), xv)
@@ -375,6 +425,7 @@ class Server:
             "requests": len(self.completed),
             "failed": sum(r.error is not None for r in self.completed),
             "solves": self.solves_done,
+            "warm_starts": self.warm_starts,
             "p50_ms": float(np.percentile(lat, 50)) if lat.size else math.nan,
             "p99_ms": float(np.percentile(lat, 99)) if lat.size else math.nan,
             "max_ms": float(lat.max()) if lat.size else math.nan,

@@ -130,7 +130,7 @@ def _requests(b, n):
     for i in range(n):
         fs = [float(f) for f in rng.choice(b.freqs, size=int(rng.integers(1, 4)), replace=False)]
         extra = _extra_vias(b, int(rng.integers(1, 3)), 100 + i) if i % 2 else []
-        reqs.append(Request("b", b.routes[0], fs, extra=extra, tol=1e-10, tag=i))
+        reqs.append(Request("b", b.routes[0], fs, extra=extra, tol=1e-10, warm=False, tag=i))
     return reqs
 
 
@@ -173,4 +173,38 @@ def test_served_requests_equal_each_request_solved_alone(budget):
         for got, one in zip(q.solutions, ref):
             assert np.array_equal(got.x, one.x) and got.iterations == one.iterations
             assert got.Z_loop == one.Z_loop
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_whatif_requests_start_from_the_primed_board_solutions():
+    """A primed board (its net solved at its frequencies) serves what-ifs warm: each equals
+    RouteExcitation.on(what-if).solve(f, guess=[map_state(board solution)]) alone, bit for bit."""
+    b = _board()
+    st = peec.PlaneStack(b.layers, b.branches)
+    srv = Server(capacity=8)
+    srv.add_board("b", st, routes=b.routes)
+    fs = [float(f) for f in b.freqs[:3]]
+    srv.prime("b", fs, tol=1e-10)
+    reqs = [Request("b", b.routes[0], fs[i:i + 2], extra=_extra_vias(b, 1, 200 + i), tol=1e-10) for i in range(2)]
+    for q in reqs:
+        srv.submit(q)
+    srv.drain(limit_s=300)
+    assert srv.stats()["warm_starts"] == 4
+    r = b.routes[0]
+    st1 = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st1, r.trace, r.driver, r.receiver)
+    base = {f: ex.solve(f, tol=1e-10) for f in fs}
+    for q in reqs:
+        d = st1.with_branches(q.extra)
+        for f, got in zip(q.fs, q.solutions):
+            one = ex.on(d).solve(f, tol=1e-10, guess=[d.map_state(base[f].x, st1)])
+            cold = ex.on(d).solve(f, tol=1e-10)
+            assert np.array_equal(got.x, one.x) and got.iterations == one.iterations
+            assert got.iterations < cold.iterations
+            # the warm solution meets the tolerance on the system itself: |rhs - K x| <= tol |rhs|
+            rhs_top, J = ex.on(d).rhs(f)
+            rhs = np.concatenate([rhs_top, np.asarray(J, float)[d.keep].astype(complex)])
+            Kx = d.matvec(d.operator(f), got.x)
+            assert np.linalg.norm(rhs - Kx) / np.linalg.norm(rhs) < 2e-10
     srv.close()
