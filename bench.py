@@ -166,6 +166,14 @@ def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
     for name, b, st in bds:
         srv.prime(name, b.freqs, tol=SERVE_TOL)
     prime_s = time.perf_counter() - t_pr
+    # one what-if per multi-layer board (untimed): the first use of the warm start's device routines
+    from groundtruth_amd import boards as B
+    from groundtruth_amd.serve import Request
+    for name, b, st in bds:
+        if len(b.layers) > 1:
+            srv.submit(Request(name, b.routes[0], [b.freqs[0]], extra=B._stitch(b.layers, 3, 3, "warm-up"),
+                               tol=SERVE_TOL))
+    srv.drain()
     srv.reset_stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
