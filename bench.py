@@ -11,12 +11,15 @@ with each solution's relative error (and the plain run's) against an FP64 solve 
 Also reported: the L_p product's (the FFT matvec's) roofline. Every L_p launch of the timed steps is bracketed by
 HIP events on the library's stream (gt_ctx_lp_timing); achieved = the right-hand sides per launch x the algorithmic
 bytes of one product (strip currents in and out, one read and one write of the padded spectra per transform, the
-kernel quadrant once) / the average launch. traffic = the same launch's HBM bytes from the FETCH_SIZE / WRITE_SIZE
-runs (per right-hand side, committed under profiles/). Beside it, a single-right-hand-side product timed over 200
-launches, and the oracle's CPU time on a bounded sample of the same workload.
+kernel quadrant once) / the average launch. traffic = the HBM bytes of an L_p launch from the FETCH_SIZE /
+WRITE_SIZE runs committed under profiles/ (launches of 8 right-hand sides), scaled to the timed launches' average
+right-hand sides. Beside it, a single-right-hand-side product timed over 200 launches, and the oracle's CPU time on a
+bounded sample of the same workload. Then the served what-if stream (serve_stream): p50/p99 latency and solves/s of
+the continuous-batching server, beside the same stream solved one solve at a time.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-python bench.py --lp-only [--lp-reps R]   (the counter runs: L_p products, then a 64 MiB device copy as the scale)
+python bench.py --lp-only [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N right-hand sides,
+                                                         then a 64 MiB device copy as the scale)
 """
 from __future__ import annotations
 
@@ -47,8 +50,9 @@ def lp_algorithmic_bytes(st) -> int:
     return total
 
 
-def time_lp(st, reps: int = 200):
-    """Average duration of one L_p product on the library's stream (HIP events on that stream)."""
+def time_lp(st, reps: int = 200, nr: int = 1):
+    """Average duration of one L_p launch of nr right-hand sides on the library's stream (HIP events on that
+    stream)."""
     import ctypes as C
 
     import torch
@@ -56,34 +60,41 @@ def time_lp(st, reps: int = 200):
     from groundtruth_amd import _lib
     dev = f"cuda:{st.device}"
     rng = np.random.default_rng(4)
-    I = torch.from_numpy(rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)).to(dev)
-    out = torch.empty_like(I)
+    I = [torch.from_numpy(rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)).to(dev) for _ in range(nr)]
+    out = [torch.empty_like(v) for v in I]
     L = _lib.lib()
     h = st.handle
+    X = (C.c_void_p * nr)(*[v.data_ptr() for v in I])
+    Y = (C.c_void_p * nr)(*[v.data_ptr() for v in out])
     stream = torch.cuda.ExternalStream(L.gt_ctx_stream(st.ctx.handle), device=dev)
     torch.cuda.synchronize()
     for _ in range(5):
-        _lib.check(L.gt_lp_matvec(h, C.c_void_p(I.data_ptr()), C.c_void_p(out.data_ptr())))
+        _lib.check(L.gt_lp_matvec_batch(h, nr, X, Y))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        L.gt_lp_matvec(h, C.c_void_p(I.data_ptr()), C.c_void_p(out.data_ptr()))
+        L.gt_lp_matvec_batch(h, nr, X, Y)
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
 def measured_traffic(workload: str):
-    """HBM bytes per L_p product from the committed FETCH_SIZE / WRITE_SIZE runs of this workload (a line
-    `traffic_bytes_per_launch: N` in profiles/lp_traffic_<workload>.md), or None."""
+    """(HBM bytes per L_p launch, right-hand sides of that launch) from the committed FETCH_SIZE / WRITE_SIZE runs
+    of this workload (lines `traffic_bytes_per_launch: N` and `traffic_rhs_per_launch: R` in
+    profiles/lp_traffic_<workload>.md), or None."""
     path = os.path.join(ROOT, "profiles", f"lp_traffic_{workload}.md")
     if not os.path.exists(path):
         return None
+    vals = {}
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
-    return None
+            for k in ("traffic_bytes_per_launch", "traffic_rhs_per_launch"):
+                if line.startswith(k + ":"):
+                    vals[k] = int(line.split(":")[1])
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals["traffic_bytes_per_launch"], vals.get("traffic_rhs_per_launch", 1)
 
 
 def lp_only(args):
@@ -94,7 +105,7 @@ def lp_only(args):
     board = boards.WORKLOADS[args.workload]()
     st = peec.PlaneStack(board.layers, board.branches, device=0)
     st.handle
-    t_lp = time_lp(st, reps=args.lp_reps)
+    t_lp = time_lp(st, reps=args.lp_reps, nr=args.lp_batch)
     n = 64 << 20
     a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
     b = torch.empty_like(a)
@@ -102,8 +113,8 @@ def lp_only(args):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"workload": args.workload, "lp_us": t_lp * 1e6, "bytes_per_launch": lp_algorithmic_bytes(st),
-                      "copy_bytes": n}))
+    print(json.dumps({"workload": args.workload, "rhs_per_launch": args.lp_batch, "lp_us": t_lp * 1e6,
+                      "bytes_per_launch": args.lp_batch * lp_algorithmic_bytes(st), "copy_bytes": n}))
 
 
 SERVE_TOL = 1e-8                # the served stream's tolerance (plain FP64 GMRES)
@@ -257,6 +268,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lp-only", action="store_true")
     ap.add_argument("--lp-reps", type=int, default=20)
+    ap.add_argument("--lp-batch", type=int, default=8, help="--lp-only: right-hand sides per L_p launch")
     ap.add_argument("--serve-rate", type=float, default=SERVE_RATE, help="served stream: requests per second")
     ap.add_argument("--serve-seconds", type=float, default=SERVE_SECONDS, help="served stream: arrival window")
     ap.add_argument("--no-serve", action="store_true")
@@ -380,7 +392,8 @@ def main():
         "value_pcie_inclusive": len(board.freqs) / dt_host,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": round(lp_rhs * traffic) if traffic else None,
+                     "traffic": round(traffic[0] * lp_rhs / traffic[1]) if traffic else None,
+                     "traffic_measured": {"bytes": traffic[0], "rhs_per_launch": traffic[1]} if traffic else None,
                      "kernel": "L_p product (lp_rows_fwd + lp_cols_mac + lp_rows_inv), as launched by the timed "
                                "batched solves",
                      "launches": n_lp.value, "rhs_per_launch": lp_rhs, "us_per_launch": lp_us,

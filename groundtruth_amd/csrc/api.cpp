@@ -377,6 +377,24 @@ int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
     });
 }
 
+int gt_lp_matvec_batch(gt_stack* st, int32_t nr, const void* const* I_dev, void* const* out_dev) {
+    return guard([&] {
+        need(st && I_dev && out_dev, "NULL argument");
+        need(nr >= 1 && nr <= KMAX, "1 to 16 right-hand sides per batch");
+        for (int i = 0; i < nr; ++i) need(I_dev[i] && out_dev[i] && I_dev[i] != out_dev[i], "NULL or aliased vector");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        if (st->ns == 0) return;
+        st->work.ensure(st, nr);
+        LpArgs<double> a = st->lp_args(0);
+        a.rhs.nr = nr;
+        for (int i = 0; i < nr; ++i) {
+            a.rhs.x[i] = (const zd*)I_dev[i];
+            a.rhs.y[i] = (zd*)out_dev[i];
+        }
+        launch_lp(a, st->shape, st->ctx->stream);
+    });
+}
+
 int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
     return guard([&] {
         need(st && node_xyz, "NULL argument");

@@ -105,6 +105,12 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     }
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w mod 8), each XCD with its own L2. Row tiles i0 and
+// i0 + RW share the 128-byte lines of the transposed intermediates ([t][j][i]: 8 complex doubles of consecutive i),
+// so the tile order is remapped for consecutive tiles to run on one XCD: tile = (w mod 8) (G / 8) + w / 8 over a
+// grid of G = a multiple of 8 workgroups (those past the last tile exit).
+__device__ __forceinline__ int xcd_tile(int bx, int G) { return (bx & 7) * (G >> 3) + (bx >> 3); }
+
 // Pass B with the transforms in registers (column length P0 = 64 E, S <= 4 sheets): one wave per column j. Lane l
 // holds rows l + 64 m of every sheet, loaded straight from T1 (whole contiguous columns); the forward FFT, the
 // kernel multiply-add over source sheets per pixel (both sheets of a pixel sit in the same lane) and the inverse
@@ -114,12 +120,19 @@ __global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* ex = reinterpret_cast<cpx<T>*>(smem_raw);
     constexpr int N = 64 * E;
-    int j = blockIdx.x, oi = 0;
-    if (j >= a.nblk_cols[0]) { j -= a.nblk_cols[0]; oi = 1; }
+    // one (column, right-hand side) per workgroup, in an order that keeps a column's right-hand sides, and the
+    // columns j and P1 - j that read the same kernel row, next to each other on one XCD (xcd_tile)
+    const int nr = a.rhs.nr;
+    const int item = xcd_tile(blockIdx.x, gridDim.x);
+    if (item >= (a.nblk_cols[0] + a.nblk_cols[1]) * nr) return;
+    int k = item / nr, oi = 0;
+    const int p = item - k * nr;
+    if (k >= a.nblk_cols[0]) { k -= a.nblk_cols[0]; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
     const int lane = threadIdx.x, n0 = o.n0, P1 = o.P1;
-    const cpx<T>* T1 = o.T1 + blockIdx.y * a.t_stride;
-    cpx<T>* T2 = o.T2 + blockIdx.y * a.t_stride;
+    const int j = (k & 1) ? P1 - 1 - (k >> 1) : (k >> 1);
+    const cpx<T>* T1 = o.T1 + p * a.t_stride;
+    cpx<T>* T2 = o.T2 + p * a.t_stride;
     cpx<T> v[S][E];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -165,10 +178,11 @@ __global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
 constexpr int RW = 4;
 
 template <class T, int E>
-__global__ __launch_bounds__(64 * RW) void lp_rows_fwd_w(LpArgs<T> a, int nb0) {
+__global__ __launch_bounds__(64 * RW) void lp_rows_fwd_w(LpArgs<T> a, int nb0, int nblk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int N = 64 * E, NS = N + N / 8;
-    int blk = blockIdx.x, oi = 0;
+    int blk = xcd_tile(blockIdx.x, gridDim.x), oi = 0;
+    if (blk >= nblk) return;
     if (blk >= nb0) { blk -= nb0; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
     const int ntiles = (o.n0 + RW - 1) / RW;
@@ -201,10 +215,11 @@ __global__ __launch_bounds__(64 * RW) void lp_rows_fwd_w(LpArgs<T> a, int nb0) {
 }
 
 template <class T, int E>
-__global__ __launch_bounds__(64 * RW) void lp_rows_inv_w(LpArgs<T> a, int nb0) {
+__global__ __launch_bounds__(64 * RW) void lp_rows_inv_w(LpArgs<T> a, int nb0, int nblk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int N = 64 * E, NS = N + N / 8;
-    int blk = blockIdx.x, oi = 0;
+    int blk = xcd_tile(blockIdx.x, gridDim.x), oi = 0;
+    if (blk >= nblk) return;
     if (blk >= nb0) { blk -= nb0; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
     const int ntiles = (o.n0 + RW - 1) / RW;
@@ -266,9 +281,12 @@ bool launch_rows_w(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st)
     if (blocks == 0) return false;
     const size_t lds = (size_t)RW * (P1 + P1 / 8) * sizeof(cpx<T>);
     if (lds > 65536) return false;            // (P1 = 1024 in FP64) the LDS kernels take it
-#define GT_ROWS(E)                                                                                                \
-    if (inverse) hipLaunchKernelGGL((lp_rows_inv_w<T, E>), dim3(blocks, nr_rhs), dim3(64 * RW), lds, st, a, nb[0]); \
-    else hipLaunchKernelGGL((lp_rows_fwd_w<T, E>), dim3(blocks, nr_rhs), dim3(64 * RW), lds, st, a, nb[0]);       \
+    const int grid = (blocks + 7) / 8 * 8;
+#define GT_ROWS(E)                                                                                                 \
+    if (inverse)                                                                                                   \
+        hipLaunchKernelGGL((lp_rows_inv_w<T, E>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);  \
+    else                                                                                                           \
+        hipLaunchKernelGGL((lp_rows_fwd_w<T, E>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);  \
     return true;
     switch (P1) {
     case 256: GT_ROWS(4)
@@ -282,11 +300,12 @@ bool launch_rows_w(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st)
 template <class T, int E>
 bool launch_cols_w_e(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
     const size_t lds = (64 * E + 8 * E) * sizeof(cpx<T>);
+    const int grid = (ncol * nr + 7) / 8 * 8;
     switch (a.S) {
-    case 1: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 1>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
-    case 2: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 2>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
-    case 3: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 3>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
-    case 4: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 4>), dim3(ncol, nr), dim3(64), lds, st, a); return true;
+    case 1: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 1>), dim3(grid), dim3(64), lds, st, a); return true;
+    case 2: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 2>), dim3(grid), dim3(64), lds, st, a); return true;
+    case 3: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 3>), dim3(grid), dim3(64), lds, st, a); return true;
+    case 4: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 4>), dim3(grid), dim3(64), lds, st, a); return true;
     default: return false;
     }
 }

@@ -87,6 +87,9 @@ int64_t gt_stack_device_bytes(const gt_stack* stack);
 /* L_p I (peec.py:2115-2135 Lp_matvec) on device vectors of nxb + nyb complex values: scatter, row FFT, column
  * FFT + kernel multiply-add + inverse column FFT, inverse row FFT, gather. */
 int gt_lp_matvec(gt_stack* stack, const void* I_dev, void* out_dev);
+/* The same for nr <= 16 vectors in one launch (peec.py:2115-2135 applied to each): the kernel spectra are read once
+ * for the batch. */
+int gt_lp_matvec_batch(gt_stack* stack, int32_t nr, const void* const* I_dev, void* const* out_dev);
 
 /* ---------------------------------------------------------------- Schur complement preconditioner
  * Replaces peec.py:2345-2445 `_factorise` (SuperLU of S = Ar^T D^-1 Ar in a host thread) and the C shim

@@ -43,6 +43,26 @@ def test_Lp_matvec_matches_the_oracle(pair):
     assert rel(st.Lp_matvec(I), ref.Lp_matvec(I)) < 1e-13
 
 
+def test_batched_Lp_launch_gives_each_vector_its_single_product(pair):
+    """gt_lp_matvec_batch: one launch of 5 vectors equals 5 single products, bit for bit."""
+    import ctypes as C
+
+    import torch
+
+    from groundtruth_amd import _lib
+    b, st, ref = pair
+    rng = np.random.default_rng(9)
+    Is = [rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns) for _ in range(5)]
+    X = [torch.from_numpy(v).to("cuda:0") for v in Is]
+    Y = [torch.empty_like(v) for v in X]
+    torch.cuda.synchronize()
+    _lib.check(_lib.lib().gt_lp_matvec_batch(st.handle, 5, (C.c_void_p * 5)(*[v.data_ptr() for v in X]),
+                                             (C.c_void_p * 5)(*[v.data_ptr() for v in Y])))
+    st.ctx.sync()
+    for v, y in zip(Is, Y):
+        assert np.array_equal(y.cpu().numpy(), st.Lp_matvec(v))
+
+
 def test_device_schur_solve_matches_the_host_factor(pair):
     b, st, ref = pair
     rng = np.random.default_rng(5)
