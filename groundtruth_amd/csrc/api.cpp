@@ -71,7 +71,7 @@ const DevBuf<zd>& twiddles(gt_stack* st, int P) {
 }
 
 void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int n0, int n1, const int32_t* P,
-                  const double* K) {
+                  const double* K, const KernelGen* kg) {
     OrientHost& o = st->o[oi];
     o.present = cnt > 0;
     o.cnt = cnt;
@@ -79,7 +79,7 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.n0 = n0;
     o.n1 = n1;
     if (!o.present) return;
-    need(K != nullptr, "kernel spectrum missing");
+    need(K != nullptr || kg != nullptr, "kernel spectrum missing");
     need(pow2(P[0]) && pow2(P[1]), "FFT sizes must be powers of two");
     need(P[0] >= 2 * n0 - 1 && P[1] >= 2 * n1 - 1, "FFT sizes below 2n - 1 would wrap the convolution");
     need(n0 < 65536 && n1 < 65536, "grid above 65535 cells");
@@ -109,14 +109,18 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     for (size_t r = 0; r + 1 < rowptr.size(); ++r) rowptr[r + 1] += rowptr[r];
     o.rowptr.upload(rowptr, st->ctx->stream);
     o.rowcol.upload(rowcol, st->ctx->stream);
-    const size_t kq = (size_t)(o.P1 / 2 + 1) * (o.P0 / 2 + 1) * S * S;
-    std::vector<double> kd(K, K + kq);
-    std::vector<float> kf(kq);
-    for (size_t e = 0; e < kq; ++e) kf[e] = (float)kd[e];
-    o.K.upload(kd, st->ctx->stream);
-    o.Kf.upload(kf, st->ctx->stream);
     twiddles(st, o.P0);
     twiddles(st, o.P1);
+    if (kg) {
+        kernel_quadrant_device(st, oi, *kg);
+    } else {
+        const size_t kq = (size_t)(o.P1 / 2 + 1) * (o.P0 / 2 + 1) * S * S;
+        std::vector<double> kd(K, K + kq);
+        std::vector<float> kf(kq);
+        for (size_t e = 0; e < kq; ++e) kf[e] = (float)kd[e];
+        o.K.upload(kd, st->ctx->stream);
+        o.Kf.upload(kf, st->ctx->stream);
+    }
     o.t_off = st->t_elems;                      // each orientation its own region: they run in one launch
     st->t_elems += (size_t)S * o.P1 * o.n0;
     st->shape.lds_rows = std::max(st->shape.lds_rows, (size_t)o.TR * (o.P1 + 1) * sizeof(zd));
@@ -310,8 +314,11 @@ int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms) {
     });
 }
 
-int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
-    return guard([&] {
+}  // extern "C"
+
+namespace {
+
+void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd, gt_stack** out) {
         need(ctx && d && out, "NULL argument");
         std::lock_guard<std::mutex> lk(ctx->mu);
         *out = nullptr;
@@ -331,8 +338,16 @@ int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
         st->nb = st->ns + d->nbr;
         st->nk = d->n_keep;
         hipStream_t s = ctx->stream;
-        setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx);
-        setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky);
+        KernelGen kg;
+        if (kd) {
+            need(kd->d > 0 && kd->dz && kd->near_field, "kernel description incomplete");
+            kg.d = kd->d;
+            kg.mu = kd->mu;
+            kg.dz = kd->dz;
+            kg.nearv = kd->near_field;
+        }
+        setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx, kd ? &kg : nullptr);
+        setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky, kd ? &kg : nullptr);
         st->strip_sheet.upload(st->h_strip_sheet.empty() ? std::vector<unsigned char>{0} : st->h_strip_sheet, s);
         const int nb = st->nb, nk = st->nk;
         need(nb == 0 || (d->a_keep && d->b_keep), "incidence missing");
@@ -351,6 +366,31 @@ int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
         st->work.ensure(st.get(), 1);
         GT_HIP(hipStreamSynchronize(s));
         *out = st.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* d, gt_stack** out) {
+    return guard([&] { create_stack(ctx, d, nullptr, out); });
+}
+
+int gt_stack_create_gen(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd, gt_stack** out) {
+    return guard([&] {
+        need(kd != nullptr, "NULL argument");
+        create_stack(ctx, d, kd, out);
+    });
+}
+
+int gt_stack_kernel(const gt_stack* st, int32_t orient, double* out) {
+    return guard([&] {
+        need(st && out && (orient == 0 || orient == 1), "bad argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        const OrientHost& o = st->o[orient];
+        if (!o.present) return;
+        GT_HIP(hipMemcpyAsync(out, o.K.p, o.K.bytes(), hipMemcpyDeviceToHost, st->ctx->stream));
+        GT_HIP(hipStreamSynchronize(st->ctx->stream));
     });
 }
 

@@ -125,6 +125,14 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z);
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides
+// What the device kernel generation needs beyond the stack's own sizes (kernel_gen.hip)
+struct KernelGen {
+    double d = 0, mu = 0;       // pitch (m), mu0 / (4 pi)
+    const double* dz = nullptr;     // S x S face gaps
+    const double* nearv = nullptr;  // S x S x 5 x 5 near-field mutuals [s][t][axial][lateral]
+};
+// The kernel quadrant of orientation oi (its FFT sizes and twiddles set) made on the device, FP64 and FP32 copies
+void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg);
 // A what-if stack's Woodbury columns on the device: the base's W copied, then B^-1 U' for the added branches
 void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra);
 // G = U^T W (k x k row-major) of the stack's branches, from its device W, into host memory

@@ -57,6 +57,29 @@ def _kernel_pair(nx: int, ny: int, d: float, dz: float, px: tuple, py: tuple):
     return hit
 
 
+_NEAR: dict = {}
+
+
+def _near_table(d: float, dz: float) -> np.ndarray:
+    """The near-field mutuals of two square strips dz apart: tape(d, a d, l d, dz) at axial offsets a and lateral
+    offsets l of 0..4 cells, as strip_kernel sets them (kernel.py:268-294), cached by the keyed d and dz."""
+    k = (d, dz)
+    hit = _NEAR.get(k)
+    if hit is None:
+        hit = _NEAR[k] = np.array([[G.tape(d, a * d, l * d, dz) for l in range(G.NEAR + 1)]
+                                   for a in range(G.NEAR + 1)])
+    return hit
+
+
+def kernel_quadrant(stack: "PlaneStack", orient: int) -> np.ndarray:
+    """The stack's resident kernel quadrant of orientation 0 (x strips) or 1 (y strips), (P1/2+1, P0/2+1, S, S)."""
+    P = stack.Px if orient == 0 else stack.Py
+    S = stack.n_sheets
+    out = np.zeros((P[1] // 2 + 1, P[0] // 2 + 1, S, S))
+    _lib.check(_lib.lib().gt_stack_kernel(stack.handle, int(orient), _lib.ptr(out)), "gt_stack_kernel")
+    return out
+
+
 class Operator:
     """One frequency's device operator (gt_op) and the host values it was made from (peec.py:1505-1516)."""
 
@@ -108,7 +131,8 @@ class PlaneStack:
     [x strips | y strips | branches]."""
 
     def __init__(self, layers: list, branches: Optional[list] = None, capacitance: Optional[list] = None,
-                 sigma: float = SIGMA_CU, precond: str = "diag", screen: bool = False, device: int = 0):
+                 sigma: float = SIGMA_CU, precond: str = "diag", screen: bool = False, device: int = 0,
+                 kernels: str = "device"):
         if not layers:
             raise SolverError("no plane layers")
         if precond not in ("diag", "sheet"):
@@ -133,6 +157,9 @@ class PlaneStack:
         self.sheet_z = self.zs.copy()
         self.sheet_t = np.array([l.thickness for l in layers], float)
         self.device = device
+        if kernels not in ("device", "host"):
+            raise ValueError("kernels is 'device' or 'host'")
+        self.kernel_mode = kernels
         self._topology(list(branches or []))
         if screen:
             self._check_no_split()
@@ -219,6 +246,15 @@ class PlaneStack:
         nx, ny, S = self.nx, self.ny, self.n_sheets
         self.Px = (G.fft_size(nx - 1), G.fft_size(ny))
         self.Py = (G.fft_size(nx), G.fft_size(ny - 1))
+        if self.kernel_mode == "device":
+            # the spectra are made on the device at upload; the host keeps the near field (the reference's graded
+            # quadrature, 5 x 5 offsets per sheet pair) and the self term
+            d = G.key(self.d)
+            self.L_self = _near_table(d, 0.0)[0, 0]
+            self._dz = np.array([[G.key(self.face_gap(s, t)) for t in range(S)] for s in range(S)])
+            self._near = np.ascontiguousarray(np.stack([np.stack([_near_table(d, self._dz[s, t]) for t in range(S)])
+                                                        for s in range(S)]))
+            return
         hx = (self.Px[1] // 2 + 1, self.Px[0] // 2 + 1)
         hy = (self.Py[1] // 2 + 1, self.Py[0] // 2 + 1)
         Kx = np.empty((*hx, S, S))
@@ -244,7 +280,7 @@ class PlaneStack:
         b = np.array([self.node_id(b.b) for b in extra], np.int64)
         if np.any(self.full_label[a] != self.full_label[b]):
             return PlaneStack(self.layers, self.branches + extra, sigma=self.sigma, precond=self.precond_mode,
-                              device=self.device)
+                              device=self.device, kernels=self.kernel_mode)
         base = self if self._base is None else self._base
         new = copy.copy(self)
         new.branches = self.branches + extra
@@ -295,9 +331,14 @@ class PlaneStack:
         d.node_sheet = _lib.ptr(node_sheet if len(node_sheet) else np.zeros(1, np.int32), _lib.i32p)
         d.px[0], d.px[1] = self.Px
         d.py[0], d.py[1] = self.Py
-        d.kx, d.ky = _lib.ptr(self._kx), _lib.ptr(self._ky)
         h = C.c_void_p()
-        _lib.check(L.gt_stack_create(self.ctx.handle, C.byref(d), C.byref(h)), "gt_stack_create")
+        if self.kernel_mode == "device":
+            kd = _lib.KernelDesc(G.key(self.d), G.MU0 / (4 * math.pi), _lib.ptr(self._dz), _lib.ptr(self._near))
+            _lib.check(L.gt_stack_create_gen(self.ctx.handle, C.byref(d), C.byref(kd), C.byref(h)),
+                       "gt_stack_create_gen")
+        else:
+            d.kx, d.ky = _lib.ptr(self._kx), _lib.ptr(self._ky)
+            _lib.check(L.gt_stack_create(self.ctx.handle, C.byref(d), C.byref(h)), "gt_stack_create")
         xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
                                              self.node_j[keep_idx]], axis=1).astype(np.int32))
         Gm = np.zeros((max(self.nbr, 1), max(self.nbr, 1)))

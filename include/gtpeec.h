@@ -79,6 +79,21 @@ typedef struct {
 } gt_stack_desc;
 
 int gt_stack_create(gt_ctx* ctx, const gt_stack_desc* desc, gt_stack** out);
+
+/* The same with the kernel spectra made on the device (peec.py:3276-3371 `_kernel_pair`, kernel.py:268-294
+ * `strip_kernel`): desc->kx / desc->ky are ignored. The far field is the filament closed form at every offset; the
+ * near field (offsets below 5 cells along and across) comes from the caller, the reference's graded quadrature of
+ * the tape mutual (kernel.py:222-265); the spectrum is two cosine sums over the even kernel, made into the same
+ * quadrant layout. */
+typedef struct {
+    double d;                 /* grid pitch (m)                                                                   */
+    double mu;                /* mu0 / (4 pi) as the caller computes it                                          */
+    const double* dz;         /* S x S: face gap between sheets s and t (m), symmetric                          */
+    const double* near_field; /* S x S x 5 x 5: [s][t][axial][lateral] tape mutuals at offsets of 0..4 cells (H) */
+} gt_kernel_desc;
+int gt_stack_create_gen(gt_ctx* ctx, const gt_stack_desc* desc, const gt_kernel_desc* kernels, gt_stack** out);
+/* The resident kernel quadrant of orientation 0 (x strips) or 1 (y strips), in the layout of gt_stack_desc. */
+int gt_stack_kernel(const gt_stack* stack, int32_t orient, double* out);
 void gt_stack_destroy(gt_stack* stack);
 /* Device bytes the stack holds (the footprint gpu.py:593-605 `device_bytes` estimates); for a what-if stack
  * (gt_stack_derive), only what it adds to its base's. */

@@ -1,0 +1,145 @@
+"""Kernel spectra made on the device (gt_stack_create_gen, csrc/kernel_gen.hip) against the host's
+(greens.strip_kernel + quadrant, the restatement of peec.py:3276-3371 the oracle pins to the reference's kernel.py).
+
+CPU: the near-field table the host hands over is strip_kernel's own near entries.
+GPU: the device quadrant of both orientations against the host's on boards of 1, 2 and 4 sheets and on the full
+two-layer board, and the L_p product and a whole solve of a device-kernel stack against a host-kernel stack and the
+oracle. The device sums the even spectrum directly (two cosine products) where the host runs numpy's FFT, and it
+takes the far-field filament mutual without the closed form's cancellation: on the small boards the two agree to
+1e-13 of the largest entry; at full size the device is checked against an extended-precision spectrum.
+"""
+import numpy as np
+import pytest
+
+from groundtruth_amd import boards, greens as G, peec
+from oracle import peec as O
+
+
+def test_near_table_is_strip_kernels_near_field():
+    d, dz = G.key(0.25e-3), G.key(0.235e-3)
+    K = G.strip_kernel(9, 9, d, dz)
+    c = 8
+    T = peec._near_table(d, dz)
+    for a in range(5):
+        for l in range(5):
+            assert T[a, l] == K[c + a, c + l] == K[c - a, c - l]
+
+
+def _host_quadrants(st):
+    S = st.n_sheets
+    qs = []
+    for oi, P, (na, nl) in ((0, st.Px, (st.nx - 1, st.ny)), (1, st.Py, (st.ny - 1, st.nx))):
+        Q = np.empty((P[1] // 2 + 1, P[0] // 2 + 1, S, S))
+        for s in range(S):
+            for t in range(S):
+                dz = G.key(st.face_gap(s, t))
+                k = G.strip_kernel(na, nl, G.key(st.d), dz)
+                k = k if oi == 0 else k.T
+                Q[:, :, s, t] = G.quadrant(k, *P).T / (P[0] * P[1])
+        qs.append(Q)
+    return qs
+
+
+CASES = {
+    "single_plane_16": lambda: boards.single_plane(n=16),
+    "two_layer_20": lambda: boards.scaled("two_layer", 20, n_vias=5, n_freqs=1),
+    "four_layer_24": lambda: boards.scaled("four_layer", 24, n_nets=2, n_freqs=1, n_stitch=3),
+    "two_layer_37": lambda: boards.scaled("two_layer", 37, n_vias=7, n_freqs=1),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_quadrants_match_the_host_spectra(name):
+    b = CASES[name]()
+    st = peec.PlaneStack(b.layers, b.branches)
+    assert st.kernel_mode == "device"
+    for oi, Q in enumerate(_host_quadrants(st)):
+        D = peec.kernel_quadrant(st, oi)
+        assert np.max(np.abs(D - Q)) <= 1e-13 * np.max(np.abs(Q))
+
+
+def _axis_mp(x_cells, d, rho):
+    """The filament mutual on the strips' axis (lateral offset 0) at 40 digits: the closed form's cancellation is
+    worst there when the sheets coincide (rho at its 1e-12 d floor)."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    L, R = mp.mpf(d), mp.mpf(rho)
+    g = lambda u: u * mp.asinh(u / R) - mp.sqrt(u * u + R * R)
+    mu = mp.mpf(G.MU0) / (4 * mp.pi)
+    return [float(mu * (g(mp.mpf(int(x)) * L + L) - 2 * g(mp.mpf(int(x)) * L) + g(mp.mpf(int(x)) * L - L)))
+            for x in x_cells]
+
+
+def _extended_spectrum(st, s, t):
+    """The x strips' spectrum of sheet pair (s, t) in extended precision (numpy longdouble, 64-bit mantissa): the
+    closed-form far field, whose (L/x)^2 cancellation costs at most ~1e-14 there (its axis at 40 digits when the sheets
+    coincide), the near field from the host's table, and the two cosine sums."""
+    d, dz = G.key(st.d), G.key(st.face_gap(s, t))
+    P = st.Px
+    n0, n1 = st.nx - 1, st.ny
+    L = np.longdouble(d)
+    a = np.arange(n0).astype(np.longdouble)[:, None] * L
+    l = np.arange(n1).astype(np.longdouble)[None, :] * L
+    rho = np.maximum(np.sqrt(l * l + np.longdouble(dz) ** 2), np.longdouble(1e-12) * L)
+    g = lambda u: u * np.arcsinh(u / rho) - np.sqrt(u * u + rho * rho)
+    K = np.longdouble(G.MU0) / (4 * np.longdouble(np.pi)) * (g(a + L) - 2 * g(a) + g(a - L))
+    if dz == 0.0:
+        K[5:, 0] = _axis_mp(range(5, n0), d, max(float(dz), 1e-12 * d))
+    K[:5, :5] = peec._near_table(d, dz)
+    h0, h1 = P[0] // 2 + 1, P[1] // 2 + 1
+    pi = np.longdouble(np.pi)
+    wb = np.where(np.arange(n1) == 0, 1, 2).astype(np.longdouble)
+    wa = np.where(np.arange(n0) == 0, 1, 2).astype(np.longdouble)
+    C1 = wb[:, None] * np.cos(2 * pi * (np.outer(np.arange(n1), np.arange(h1)) % P[1]).astype(np.longdouble) / P[1])
+    C0 = wa[:, None] * np.cos(2 * pi * (np.outer(np.arange(n0), np.arange(h0)) % P[0]).astype(np.longdouble) / P[0])
+    return (((K @ C1).T @ C0) / (P[0] * P[1])).astype(np.float64)
+
+
+@pytest.mark.gpu
+def test_device_quadrant_of_a_single_plane_against_extended_precision():
+    """One sheet (dz = 0) at 200 x 200: the far field along the strips' axis is where the reference's closed form
+    cancels most (its double-precision axis values are ~1e-10 off); the device's integral form against 40-digit
+    axis values and a longdouble spectrum."""
+    b = boards.single_plane(n=200)
+    st = peec.PlaneStack(b.layers, b.branches)
+    D = peec.kernel_quadrant(st, 0)
+    X = _extended_spectrum(st, 0, 0)
+    assert np.max(np.abs(D[:, :, 0, 0] - X)) <= 1e-14 * np.max(np.abs(X))
+
+
+@pytest.mark.gpu
+def test_device_quadrants_at_full_two_layer_size():
+    """On the 256 x 256 two-layer board: the device spectrum against an extended-precision one to 1e-14 of the
+    largest entry, and against the host's within the host's own rounding. The reference's closed form of the
+    filament mutual cancels to (L/x)^2 of its terms, which puts its double-precision spectrum 1.6e-13 of the largest
+    entry from the exact one here; the device takes the same quantity as an integral of positive terms."""
+    b = boards.two_layer()
+    st = peec.PlaneStack(b.layers, b.branches)
+    D = peec.kernel_quadrant(st, 0)
+    for s, t in ((0, 0), (0, 1)):
+        X = _extended_spectrum(st, s, t)
+        assert np.max(np.abs(D[:, :, t, s] - X)) <= 1e-14 * np.max(np.abs(X))
+    for oi, Q in enumerate(_host_quadrants(st)):
+        D = peec.kernel_quadrant(st, oi)
+        assert np.max(np.abs(D - Q)) <= 5e-13 * np.max(np.abs(Q))
+
+
+@pytest.mark.gpu
+def test_device_kernel_stack_products_and_solve_match_the_host_kernel_stack():
+    b = CASES["two_layer_37"]()
+    dev = peec.PlaneStack(b.layers, b.branches)
+    host = peec.PlaneStack(b.layers, b.branches, kernels="host")
+    ref = O.PlaneStack(b.layers, b.branches)
+    rng = np.random.default_rng(3)
+    I = rng.standard_normal(dev.ns) + 1j * rng.standard_normal(dev.ns)
+    a, h, o = dev.Lp_matvec(I), host.Lp_matvec(I), ref.Lp_matvec(I)
+    # the host's (and the oracle's) far-field values carry the closed form's cancellation, ~1e-13 of the spectrum
+    assert np.max(np.abs(a - h)) <= 5e-13 * np.max(np.abs(h))
+    assert np.max(np.abs(a - o)) <= 5e-13 * np.max(np.abs(o))
+    assert dev.L_self == host.L_self
+    r = b.routes[0]
+    s1 = peec.RouteExcitation(dev, r.trace, r.driver, r.receiver).solve(2e8, tol=1e-12)
+    s2 = peec.RouteExcitation(host, r.trace, r.driver, r.receiver).solve(2e8, tol=1e-12)
+    assert np.linalg.norm(s1.x - s2.x) / np.linalg.norm(s2.x) < 1e-10
+    assert abs(s1.Z_loop - s2.Z_loop) / abs(s2.Z_loop) < 1e-10
