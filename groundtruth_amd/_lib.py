@@ -34,6 +34,10 @@ class KernelDesc(C.Structure):
     _fields_ = [("d", C.c_double), ("mu", C.c_double), ("dz", f64p), ("near_field", f64p)]
 
 
+class Segment(C.Structure):
+    _fields_ = [("a", C.c_double * 3), ("b", C.c_double * 3), ("width", C.c_double), ("current", C.c_double)]
+
+
 class GmresParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("bnorm", C.c_double), ("restart", C.c_int32), ("max_cycles", C.c_int32),
                 ("extra_cycles", C.c_int32), ("max_its", C.c_int32), ("check_misses", C.c_int32)]
@@ -55,6 +59,8 @@ SIGNATURES = {
     "gt_stack_create": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(C.c_void_p)]),
     "gt_stack_create_gen": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(KernelDesc), C.POINTER(C.c_void_p)]),
     "gt_stack_kernel": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
+    "gt_strip_coupling": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.POINTER(Segment), f64p, f64p,
+                                    C.c_void_p]),
     "gt_stack_destroy": (None, [C.c_void_p]),
     "gt_stack_device_bytes": (C.c_int64, [C.c_void_p]),
     "gt_lp_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -383,6 +383,16 @@ int gt_stack_create_gen(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_des
     });
 }
 
+int gt_strip_coupling(gt_stack* st, double d, double mu, int32_t nseg, const gt_segment* segs, const double* sheet_z,
+                      const double* sheet_t, void* psi_dev) {
+    return guard([&] {
+        need(st && sheet_z && sheet_t && psi_dev && nseg >= 0 && (nseg == 0 || segs), "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        GT_HIP(hipSetDevice(st->ctx->device));
+        strip_coupling_device(st, d, mu, nseg, segs, sheet_z, sheet_t, (double*)psi_dev);
+    });
+}
+
 int gt_stack_kernel(const gt_stack* st, int32_t orient, double* out) {
     return guard([&] {
         need(st && out && (orient == 0 || orient == 1), "bad argument");

@@ -132,7 +132,7 @@ class PlaneStack:
 
     def __init__(self, layers: list, branches: Optional[list] = None, capacitance: Optional[list] = None,
                  sigma: float = SIGMA_CU, precond: str = "diag", screen: bool = False, device: int = 0,
-                 kernels: str = "device"):
+                 setup: str = "device"):
         if not layers:
             raise SolverError("no plane layers")
         if precond not in ("diag", "sheet"):
@@ -157,9 +157,9 @@ class PlaneStack:
         self.sheet_z = self.zs.copy()
         self.sheet_t = np.array([l.thickness for l in layers], float)
         self.device = device
-        if kernels not in ("device", "host"):
-            raise ValueError("kernels is 'device' or 'host'")
-        self.kernel_mode = kernels
+        if setup not in ("device", "host"):
+            raise ValueError("setup is 'device' or 'host'")
+        self.setup_mode = setup             # where the kernel spectra and the strips' psi are made
         self._topology(list(branches or []))
         if screen:
             self._check_no_split()
@@ -246,7 +246,7 @@ class PlaneStack:
         nx, ny, S = self.nx, self.ny, self.n_sheets
         self.Px = (G.fft_size(nx - 1), G.fft_size(ny))
         self.Py = (G.fft_size(nx), G.fft_size(ny - 1))
-        if self.kernel_mode == "device":
+        if self.setup_mode == "device":
             # the spectra are made on the device at upload; the host keeps the near field (the reference's graded
             # quadrature, 5 x 5 offsets per sheet pair) and the self term
             d = G.key(self.d)
@@ -280,7 +280,7 @@ class PlaneStack:
         b = np.array([self.node_id(b.b) for b in extra], np.int64)
         if np.any(self.full_label[a] != self.full_label[b]):
             return PlaneStack(self.layers, self.branches + extra, sigma=self.sigma, precond=self.precond_mode,
-                              device=self.device, kernels=self.kernel_mode)
+                              device=self.device, setup=self.setup_mode)
         base = self if self._base is None else self._base
         new = copy.copy(self)
         new.branches = self.branches + extra
@@ -314,6 +314,10 @@ class PlaneStack:
         if self._base is not None:
             self._derive()
             return
+        self._factor(self._create())
+
+    def _create(self):
+        """The device stack without its Schur factor: layout, incidence, kernel spectra (gt_stack_create[_gen])."""
         L = _lib.lib()
         self.ctx = _lib.Context.get(self.device)
         xs = np.ascontiguousarray(self.xa, np.int32)
@@ -332,13 +336,19 @@ class PlaneStack:
         d.px[0], d.px[1] = self.Px
         d.py[0], d.py[1] = self.Py
         h = C.c_void_p()
-        if self.kernel_mode == "device":
+        if self.setup_mode == "device":
             kd = _lib.KernelDesc(G.key(self.d), G.MU0 / (4 * math.pi), _lib.ptr(self._dz), _lib.ptr(self._near))
             _lib.check(L.gt_stack_create_gen(self.ctx.handle, C.byref(d), C.byref(kd), C.byref(h)),
                        "gt_stack_create_gen")
         else:
             d.kx, d.ky = _lib.ptr(self._kx), _lib.ptr(self._ky)
             _lib.check(L.gt_stack_create(self.ctx.handle, C.byref(d), C.byref(h)), "gt_stack_create")
+        return h
+
+    def _factor(self, h):
+        """The Schur factor of the stack `h` (gt_schur_factor), then the stack is this PlaneStack's."""
+        L = _lib.lib()
+        keep_idx = np.nonzero(self.keep)[0]
         xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
                                              self.node_j[keep_idx]], axis=1).astype(np.int32))
         Gm = np.zeros((max(self.nbr, 1), max(self.nbr, 1)))
@@ -608,7 +618,27 @@ class PlaneStack:
     def strip_coupling(self, trace: list) -> np.ndarray:
         """psi = int A.dl along every strip per amp of signal (peec.py:2609-2656): the segment's vector potential
         at the strip's centre on its face toward the segment, or its mean over 3 x 3 Gauss points of the strip's
-        square within 3 pitches (+ 2 widths) of the segment."""
+        square within 3 pitches (+ 2 widths) of the segment. On the device (gt_strip_coupling) unless the stack's
+        set-up is the host's."""
+        if self.setup_mode == "host" or self.ns == 0:
+            return self.strip_coupling_host(trace)
+        import torch
+        segs = (_lib.Segment * max(1, len(trace)))()
+        for k, s in enumerate(trace):
+            segs[k].a[:] = [float(v) for v in s.a]
+            segs[k].b[:] = [float(v) for v in s.b]
+            segs[k].width, segs[k].current = float(s.width), float(s.i)
+        psi = torch.zeros(self.ns, dtype=torch.float64, device=f"cuda:{self.device}")
+        torch.cuda.synchronize(self.device)
+        z = np.ascontiguousarray(self.sheet_z, float)
+        t = np.ascontiguousarray(self.sheet_t, float)
+        _lib.check(_lib.lib().gt_strip_coupling(self.handle, float(self.d), G.MU0 / (4 * math.pi), len(trace), segs,
+                                                _lib.ptr(z), _lib.ptr(t), C.c_void_p(psi.data_ptr())),
+                   "gt_strip_coupling")
+        return psi.cpu().numpy()
+
+    def strip_coupling_host(self, trace: list) -> np.ndarray:
+        """strip_coupling in numpy."""
         d = self.d
         x, w = np.polynomial.legendre.leggauss(3)
         q = 0.5 * x

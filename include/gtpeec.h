@@ -106,6 +106,18 @@ int gt_lp_matvec(gt_stack* stack, const void* I_dev, void* out_dev);
  * for the batch. */
 int gt_lp_matvec_batch(gt_stack* stack, int32_t nr, const void* const* I_dev, void* const* out_dev);
 
+/* psi of every strip (peec.py:2609-2656 `strip_coupling`): the integral of the trace's vector potential along each
+ * strip per amp of loop current, the strips' right-hand side (rhs_top = -j w psi). One trace segment from a to b (m),
+ * `width` wide (m), carrying `current` amps per amp of loop current. psi_dev: ns = nxb + nyb doubles (device).
+ * sheet_z, sheet_t: each sheet's height and copper thickness (m); d: the pitch (m); mu: mu0 / (4 pi). */
+typedef struct {
+    double a[3], b[3];
+    double width;
+    double current;
+} gt_segment;
+int gt_strip_coupling(gt_stack* stack, double d, double mu, int32_t nseg, const gt_segment* segs,
+                      const double* sheet_z, const double* sheet_t, void* psi_dev);
+
 /* ---------------------------------------------------------------- Schur complement preconditioner
  * Replaces peec.py:2345-2445 `_factorise` (SuperLU of S = Ar^T D^-1 Ar in a host thread) and the C shim
  * solver/accelerate_shim.c:21-59 (gt_factor / gt_solve / gt_free). With every strip of a stack alike, D^-1 is one

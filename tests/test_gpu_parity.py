@@ -2,7 +2,7 @@
 
 Tolerances: the L_p product, the system product and the Schur solve are one FP64 computation in a different order
 (FFT sizes, reduction order): <= 1e-13 / 1e-12 of the largest entry. The product checks build their stacks with the
-host's kernel values (kernels="host"), the oracle's own, so they compare the device's FFTs and rows alone; the
+host's kernel values (setup="host"), the oracle's own, so they compare the device's FFTs and rows alone; the
 device-made kernels are pinned in test_kernel_gen.py. The whole solves use the device-made kernels. The preconditioner uses a different exact
 factorisation (Cholesky of B with the Woodbury update, against SuperLU of S): <= 1e-10. Whole solves at tol 1e-12:
 strip currents and Z_loop within 1e-9 relative (the north star's bar), iterations within 2 of the oracle's.
@@ -35,7 +35,7 @@ CASES = {
 @pytest.fixture(scope="module", params=sorted(CASES))
 def pair(request):
     b = CASES[request.param]()
-    return b, peec.PlaneStack(b.layers, b.branches, kernels="host"), O.PlaneStack(b.layers, b.branches)
+    return b, peec.PlaneStack(b.layers, b.branches, setup="host"), O.PlaneStack(b.layers, b.branches)
 
 
 def test_Lp_matvec_matches_the_oracle(pair):
@@ -197,7 +197,7 @@ def test_Lp_matvec_with_register_transforms_matches_the_oracle(name, n, kw):
     """Column lengths 256 / 512 / 1024 and 1, 2, 4 sheets: the L_p product's column pass with its transforms in
     registers (one wave per column) against the oracle's scipy FFTs."""
     b = boards.scaled(name, n, n_freqs=1, **kw) if name != "single_plane" else boards.single_plane(n=n)
-    st, ref = peec.PlaneStack(b.layers, b.branches, kernels="host"), O.PlaneStack(b.layers, b.branches)
+    st, ref = peec.PlaneStack(b.layers, b.branches, setup="host"), O.PlaneStack(b.layers, b.branches)
     assert st.Px[0] in (256, 512, 1024)
     rng = np.random.default_rng(8)
     I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)
@@ -208,7 +208,7 @@ def test_Lp_matvec_with_register_transforms_three_sheets():
     b = boards.scaled("four_layer", 100, n_freqs=1, n_nets=1, n_stitch=4)
     layers = b.layers[:3]
     branches = [br for br in b.branches if br.a[0] < 3 and br.b[0] < 3]
-    st, ref = peec.PlaneStack(layers, branches, kernels="host"), O.PlaneStack(layers, branches)
+    st, ref = peec.PlaneStack(layers, branches, setup="host"), O.PlaneStack(layers, branches)
     assert st.n_sheets == 3 and st.Px[0] == 256
     rng = np.random.default_rng(9)
     I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)

@@ -71,7 +71,7 @@ def _host_lp(st, I):
 @pytest.mark.parametrize("k", range(3))
 def test_uploaded_kernels_give_the_oracles_Lp_product(k):
     b = _boards()[k]
-    st = peec.PlaneStack(b.layers, b.branches, kernels="host")
+    st = peec.PlaneStack(b.layers, b.branches, setup="host")
     ref = O.PlaneStack(b.layers, b.branches)
     rng = np.random.default_rng(4)
     I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)
@@ -185,7 +185,7 @@ def test_woodbury_form_equals_the_schur_inverse():
 
 def test_route_coupling_and_trace_terms_match_the_oracle():
     b = boards.scaled("four_layer", 24, n_nets=3, n_freqs=1, n_stitch=3)
-    st, ref = peec.PlaneStack(b.layers, b.branches), O.PlaneStack(b.layers, b.branches)
+    st, ref = peec.PlaneStack(b.layers, b.branches, setup="host"), O.PlaneStack(b.layers, b.branches)
     for r in b.routes:
         ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
         ox = O.RouteExcitation(ref, r.trace, r.driver, r.receiver)

@@ -53,7 +53,7 @@ CASES = {
 def test_device_quadrants_match_the_host_spectra(name):
     b = CASES[name]()
     st = peec.PlaneStack(b.layers, b.branches)
-    assert st.kernel_mode == "device"
+    assert st.setup_mode == "device"
     for oi, Q in enumerate(_host_quadrants(st)):
         D = peec.kernel_quadrant(st, oi)
         assert np.max(np.abs(D - Q)) <= 1e-13 * np.max(np.abs(Q))
@@ -129,7 +129,7 @@ def test_device_quadrants_at_full_two_layer_size():
 def test_device_kernel_stack_products_and_solve_match_the_host_kernel_stack():
     b = CASES["two_layer_37"]()
     dev = peec.PlaneStack(b.layers, b.branches)
-    host = peec.PlaneStack(b.layers, b.branches, kernels="host")
+    host = peec.PlaneStack(b.layers, b.branches, setup="host")
     ref = O.PlaneStack(b.layers, b.branches)
     rng = np.random.default_rng(3)
     I = rng.standard_normal(dev.ns) + 1j * rng.standard_normal(dev.ns)

@@ -56,7 +56,7 @@ def test_derived_topology_equals_a_fresh_stack():
 
 def test_map_state_and_state_for():
     b = _board()
-    st = peec.PlaneStack(b.layers, b.branches)
+    st = peec.PlaneStack(b.layers, b.branches, setup="host")
     d = st.with_branches(_extra_vias(b, 2, 3))
     rng = np.random.default_rng(1)
     x = rng.standard_normal(st.nb + st.n_keep) + 1j * rng.standard_normal(st.nb + st.n_keep)
