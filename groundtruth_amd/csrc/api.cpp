@@ -225,20 +225,6 @@ void upload_incidence(gt_stack* st) {
     st->adj.upload(adj, s);
 }
 
-// G = U^T W for the stack's k branches (row-major k x k)
-void woodbury_gram(const gt_stack* st, double* G) {
-    const int k = st->k, nk = st->nk;
-    for (int r = 0; r < k; ++r) {
-        const int b = st->ns + r;
-        const int ka = st->h_a_keep[b], kb = st->h_b_keep[b];
-        for (int c = 0; c < k; ++c) {
-            double v = 0;
-            if (ka >= 0) v += st->h_W[(size_t)c * nk + ka];
-            if (kb >= 0) v -= st->h_W[(size_t)c * nk + kb];
-            G[(size_t)r * k + c] = v;
-        }
-    }
-}
 
 }  // namespace
 
@@ -286,6 +272,15 @@ int gt_ctx_sync(gt_ctx* ctx) {
 }
 
 void* gt_ctx_stream(gt_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        need(m_min >= 1, "the smallest device front must be at least one row");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->schur_dev_min = m_min;
+    });
+}
 
 int gt_ctx_lp_timing(gt_ctx* ctx, int enable) {
     return guard([&] {
@@ -454,29 +449,21 @@ int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
         std::vector<double> diag;
         laplacian(nk, nb, st->h_a_keep.data(), st->h_b_keep.data(), ea, eb, diag);
         auto S = std::make_unique<Schur>();
+        S->h.dev_min = st->ctx->schur_dev_min;
         S->h.build(nk, ea, eb, diag, node_xyz);
         S->upload(st->ctx->stream);
-        // Woodbury columns W = B^-1 U (U: the branches' incidence), two real columns per complex solve
-        const int k = st->nbr;
-        std::vector<double> W((size_t)k * nk, 0.0);
-        std::vector<double> bb(2 * (size_t)nk), xx(2 * (size_t)nk);
-        for (int m = 0; m < k; m += 2) {
-            std::fill(bb.begin(), bb.end(), 0.0);
-            for (int c = 0; c < 2 && m + c < k; ++c) {
-                const int b = st->ns + m + c;
-                if (st->h_a_keep[b] >= 0) bb[2 * (size_t)st->h_a_keep[b] + c] += 1.0;
-                if (st->h_b_keep[b] >= 0) bb[2 * (size_t)st->h_b_keep[b] + c] -= 1.0;
-            }
-            S->h.solve_host(bb.data(), xx.data());
-            for (int c = 0; c < 2 && m + c < k; ++c)
-                for (int v = 0; v < nk; ++v) W[(size_t)(m + c) * nk + v] = xx[2 * (size_t)v + c];
-        }
-        st->W.upload(W.empty() ? std::vector<double>{0.0} : W, st->ctx->stream);
-        st->h_W = std::move(W);
-        st->k = k;
-        if (G_out) woodbury_gram(st, G_out);
-        GT_HIP(hipStreamSynchronize(st->ctx->stream));
+        S->factor_device(st->ctx->stream);
         st->schur = std::move(S);
+        // Woodbury columns W = B^-1 U (U: the branches' incidence) by device solves, and G = U^T W
+        const int k = st->nbr;
+        std::vector<int> ka(k), kb(k);
+        for (int m = 0; m < k; ++m) {
+            ka[m] = st->h_a_keep[st->ns + m];
+            kb[m] = st->h_b_keep[st->ns + m];
+        }
+        woodbury_extend(st, nullptr, k, ka.data(), kb.data());
+        if (G_out) woodbury_gram_device(st, G_out);
+        GT_HIP(hipStreamSynchronize(st->ctx->stream));
     });
 }
 
@@ -535,6 +522,8 @@ int gt_schur_stats(const gt_stack* st, int64_t* nnz_L, int32_t* n_super, int32_t
 int gt_schur_solve_host(const gt_stack* st, const double* b, double* x) {
     return guard([&] {
         need(st && st->schur && b && x, "NULL argument or no factor");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        st->schur->fetch_panels(st->ctx->stream);     // the device fronts' panels, once
         st->schur->h.solve_host(b, x);
     });
 }

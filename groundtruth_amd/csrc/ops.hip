@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void wb_gram(const double* __restrict__ W, int
 
 void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra) {
     hipStream_t s = st->ctx->stream;
-    const int nk = st->nk, k0 = base->k, k = k0 + n_extra;
+    const int nk = st->nk, k0 = base ? base->k : 0, k = k0 + n_extra;
     st->W.alloc(std::max<size_t>(1, (size_t)k * nk));
     if (k0 > 0 && nk > 0)
         GT_HIP(hipMemcpyAsync(st->W.p, base->W.p, sizeof(double) * (size_t)k0 * nk, hipMemcpyDeviceToDevice, s));

@@ -5,6 +5,8 @@
 // The device keeps the panels and solves L L^T x = b level by level (schur_solve.hip).
 #pragma once
 
+#include <cstdint>
+
 #include <vector>
 
 #include "gt_common.hpp"
@@ -63,6 +65,18 @@ struct SchurHost {
     int64_t slots = 0;                  // partial-sum slots (SCHUR_TILE x 16 right-hand sides each)
     int64_t nnz_L = 0;
     int max_cols = 0;
+    // Fronts factorised on the device (schur_dense.hip): those of at least dev_min rows and every ancestor of one.
+    // Their panels are made on the device; `build` leaves them out of the host's numeric and panel work and hands
+    // over what their assembly needs: the original entries (front row, front column, value; duplicates summed) and
+    // the update matrices of their host children (nr x nr, column-major).
+    int dev_min = INT32_MAX;
+    std::vector<char> on_dev;
+    std::vector<int> dev_order;                 // device fronts in postorder
+    std::vector<int64_t> trip_off;              // per device front (index into dev_order): its original entries
+    std::vector<int32_t> trip_r, trip_c;
+    std::vector<double> trip_v;
+    std::vector<std::vector<double>> upd_host;  // update matrices of host fronts whose parent is on the device
+    bool host_panels = true;                    // false while device fronts' panels exist only on the device
 
     // factorise B given by its strictly-off-diagonal edges (a, b pairs, weight 1 each, repeated edges add) and the
     // diagonal (degree incl. edges to reference nodes); xyz: (sheet, i, j) per node
@@ -92,6 +106,11 @@ struct Schur {
     DevBuf<zd> zp, xp;                  // forward and backward solutions in the new numbering
 
     void upload(hipStream_t s);
+    // the panels of the device fronts (schur_dense.hip), after upload: assembly, blocked partial Cholesky and solve
+    // panels, straight into `panel`
+    void factor_device(hipStream_t s);
+    // the device fronts' panels copied into h.panel (for solve_host)
+    void fetch_panels(hipStream_t s);
     int64_t device_bytes() const;
 };
 

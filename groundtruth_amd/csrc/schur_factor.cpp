@@ -171,6 +171,21 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     }
     for (int s = 0; s < ns; ++s)
         if (parent[s] < 0 && nr[s] > 0) fail(GT_ERR_UNSUPPORTED, "an assembly-tree root has rows above it");
+    // ---- which fronts the device takes: the large ones and their ancestors (children come first in postorder)
+    on_dev.assign(ns, 0);
+    dev_order.clear();
+    for (int s = 0; s < ns; ++s) {
+        bool d = (int64_t)nc[s] + nr[s] >= (int64_t)dev_min;
+        for (int e = child_ptr[s]; e < child_ptr[s + 1]; ++e) d = d || on_dev[child[e]];
+        on_dev[s] = d ? 1 : 0;
+        if (d) dev_order.push_back(s);
+    }
+    host_panels = dev_order.empty();
+    trip_off.assign(1, 0);
+    trip_r.clear();
+    trip_c.clear();
+    trip_v.clear();
+    upd_host.assign(ns, std::vector<double>());
     // ---- numeric: multifrontal partial Cholesky per supernode
     poff.assign(ns + 1, 0);
     for (int s = 0; s < ns; ++s) poff[s + 1] = poff[s] + (int64_t)(nc[s] + nr[s]) * nc[s];
@@ -185,6 +200,37 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     std::vector<int> pos(n, -1);
     for (int s = 0; s < ns; ++s) {
         const int C = nc[s], m = nc[s] + nr[s];
+        if (on_dev[s]) {
+            // the front's original entries for the device: (front row, front column) in the lower triangle
+            for (int q = 0; q < C; ++q) pos[c0[s] + q] = q;
+            for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = C + a;
+            std::vector<std::pair<int64_t, double>> ent;
+            for (int q = 0; q < C; ++q) {
+                const int c = c0[s] + q, old = perm[c];
+                ent.push_back({(int64_t)q * m + q, diag[old]});
+                for (int e = aptr[old]; e < aptr[old + 1]; ++e) {
+                    const int c2 = iperm[adj[e]];
+                    if (c2 > c) ent.push_back({(int64_t)q * m + pos[c2], -1.0});
+                }
+            }
+            std::sort(ent.begin(), ent.end(),
+                      [](const std::pair<int64_t, double>& x, const std::pair<int64_t, double>& y) {
+                          return x.first < y.first;
+                      });
+            for (size_t e = 0; e < ent.size(); ++e) {
+                if (e > 0 && ent[e].first == ent[e - 1].first) {
+                    trip_v.back() += ent[e].second;
+                    continue;
+                }
+                trip_c.push_back((int32_t)(ent[e].first / m));
+                trip_r.push_back((int32_t)(ent[e].first % m));
+                trip_v.push_back(ent[e].second);
+            }
+            trip_off.push_back((int64_t)trip_v.size());
+            for (int q = 0; q < C; ++q) pos[c0[s] + q] = -1;
+            for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = -1;
+            continue;
+        }
         std::vector<double> F((size_t)m * m, 0.0);
         for (int q = 0; q < C; ++q) pos[c0[s] + q] = q;
         for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = C + a;
@@ -229,13 +275,15 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
             std::vector<double> U((size_t)r * r);
             for (int b = 0; b < r; ++b)
                 for (int a = 0; a < r; ++a) U[(size_t)b * r + a] = F[(size_t)(C + b) * m + C + a];
-            upd[s] = std::move(U);
+            if (parent[s] >= 0 && on_dev[parent[s]]) upd_host[s] = std::move(U);
+            else upd[s] = std::move(U);
         }
         for (int q = 0; q < C; ++q) pos[c0[s] + q] = -1;
         for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = -1;
     }
     // ---- solve panels: Q = [L_CC^-1 ; -L_RC L_CC^-1] in place of each L panel
     for (int sn = 0; sn < ns; ++sn) {
+        if (on_dev[sn]) continue;
         const int C = nc[sn], m = nc[sn] + nr[sn];
         double* P = &panel[poff[sn]];
         std::vector<double> inv((size_t)C * C, 0.0);      // column-major, lower

@@ -23,6 +23,7 @@ struct gt_ctx_s {
     // every call that works on this context's stream and scratch holds it (calls from several host threads run one
     // at a time)
     std::mutex mu;
+    int schur_dev_min = 256;                  // fronts of at least this many rows are factorised on the device
     std::unique_ptr<gt::GmresWork> gmres;     // GMRES device memory kept between calls (gmres.cpp)
     // L_p launch timing (gt_ctx_lp_timing): an event pair on the stream around every L_p launch while enabled
     bool lp_timing = false;
@@ -93,7 +94,6 @@ struct gt_stack_s {
     std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
     gt::Work work;
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
-    std::vector<double> h_W;                  // its host copy (a factored stack's; a what-if stack's stay on the device)
     int k = 0;
     bool derived = false;                     // a what-if stack: its core belongs to the stack it came from
 
@@ -136,7 +136,7 @@ struct KernelGen {
 };
 // The kernel quadrant of orientation oi (its FFT sizes and twiddles set) made on the device, FP64 and FP32 copies
 void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg);
-// A what-if stack's Woodbury columns on the device: the base's W copied, then B^-1 U' for the added branches
+// Woodbury columns on the device: the base's W copied (none without a base), then B^-1 U' for the added branches
 void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra);
 // G = U^T W (k x k row-major) of the stack's branches, from its device W, into host memory
 void woodbury_gram_device(const gt_stack* st, double* G_host);

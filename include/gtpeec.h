@@ -53,6 +53,11 @@ void* gt_ctx_stream(gt_ctx* ctx);
  * enabling (or disabling) clears the record. gt_ctx_lp_time waits for the stream and gives the launches recorded,
  * the right-hand sides they carried and their summed duration in ms. */
 int gt_ctx_lp_timing(gt_ctx* ctx, int enable);
+/* The factorisations made on this context (gt_schur_factor) take every front of at least m_min rows, and its
+ * ancestors, on the device (assembly, blocked partial Cholesky, solve panels); smaller fronts stay on the host.
+ * Default 256; INT32_MAX keeps the whole factorisation on the host. (No counterpart in the reference, which
+ * factorises with SuperLU on the host, peec.py:2379-2424.) */
+int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min);
 int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms);
 
 /* ---------------------------------------------------------------- stack
@@ -123,8 +128,9 @@ int gt_strip_coupling(gt_stack* stack, double d, double mu, int32_t nseg, const 
  * solver/accelerate_shim.c:21-59 (gt_factor / gt_solve / gt_free). With every strip of a stack alike, D^-1 is one
  * admittance d_s per strip and S = d_s (B + U C U^T): B = Ar^T Ar is the real graph Laplacian of the kept nodes
  * (strips and branches, weight 1), U = the branches' incidence columns, C = diag(d_br / d_s - 1). B is factorised
- * once per stack (nested dissection, supernodal Cholesky on the host), its triangular factors are solved on the
- * device level by level, and each frequency only changes the k x k Woodbury matrix.
+ * once per stack (nested dissection, multifrontal supernodal Cholesky: the small fronts on the host, the large ones
+ * on the device, gt_ctx_schur_device_fronts), its triangular factors are solved on the device level by level, and
+ * each frequency only changes the k x k Woodbury matrix, whose columns W = B^-1 U are device solves.
  *
  * node_xyz: n_keep x 3 (sheet, i, j) of the kept nodes (the nested-dissection geometry).
  * G_out: k x k (k = nbr) real, row-major: U^T B^-1 U, from which the caller forms each frequency's
