@@ -387,7 +387,8 @@ def main():
                                f"{len(board.freqs)} frequencies 1 MHz-1 GHz, GMRES({board.restart}) to "
                                f"{board.tol:g}, plain FP64",
                    "unknowns": st.nb + st.n_keep, "strips": st.ns, "vias": st.nbr,
-                   "iterations": its, "setup_s": round(setup_s, 3), "fft": [list(st.Px), list(st.Py)]},
+                   "iterations": its, "setup_s": round(setup_s, 3), "fft": [list(st.Px), list(st.Py)],
+                   "lp_plan": getattr(st, "lp_plan", None)},
         "matvec_GBps": achieved,
         "value_pcie_inclusive": len(board.freqs) / dt_host,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

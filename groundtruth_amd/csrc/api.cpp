@@ -440,6 +440,78 @@ int gt_lp_matvec_batch(gt_stack* st, int32_t nr, const void* const* I_dev, void*
     });
 }
 
+int gt_stack_lp_autotune(gt_stack* st, int32_t reps, int32_t* rw_out, int32_t* cw_out) {
+    return guard([&] {
+        need(st && rw_out && cw_out && reps >= 1, "bad argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        GT_HIP(hipSetDevice(st->ctx->device));
+        hipStream_t s = st->ctx->stream;
+        const int nrs[3] = {1, 4, 8};
+        if (st->ns > 0) {
+            st->work.ensure(st, 8);
+            DevBuf<zd> X((size_t)st->ns * 8), Y((size_t)st->ns * 8);
+            GT_HIP(hipMemsetAsync(X.p, 0, X.bytes(), s));
+            LpArgs<double> a = st->lp_args(0);
+            for (int i = 0; i < 8; ++i) {
+                a.rhs.x[i] = X.p + (size_t)i * st->ns;
+                a.rhs.y[i] = Y.p + (size_t)i * st->ns;
+            }
+            hipEvent_t e0, e1;
+            GT_HIP(hipEventCreate(&e0));
+            GT_HIP(hipEventCreate(&e1));
+            const int shapes[3] = {1, 2, 4};
+            for (int b = 0; b < 3; ++b) {
+                a.rhs.nr = nrs[b];
+                float best = 0;
+                int brw = st->shape.rw[b], bcw = st->shape.cw[b];
+                for (int rw : shapes)
+                    for (int cw : shapes) {
+                        LpShape sh = st->shape;
+                        sh.rw[b] = rw;
+                        sh.cw[b] = cw;
+                        launch_lp(a, sh, s);
+                        launch_lp(a, sh, s);
+                        GT_HIP(hipEventRecord(e0, s));
+                        for (int r = 0; r < reps; ++r) launch_lp(a, sh, s);
+                        GT_HIP(hipEventRecord(e1, s));
+                        GT_HIP(hipEventSynchronize(e1));
+                        float ms = 0;
+                        GT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (best == 0 || ms < best) {
+                            best = ms;
+                            brw = rw;
+                            bcw = cw;
+                        }
+                    }
+                st->shape.rw[b] = brw;
+                st->shape.cw[b] = bcw;
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            GT_HIP(hipStreamSynchronize(s));
+        }
+        for (int b = 0; b < 3; ++b) {
+            rw_out[b] = st->shape.rw[b];
+            cw_out[b] = st->shape.cw[b];
+        }
+    });
+}
+
+int gt_stack_set_lp_plan(gt_stack* st, const int32_t* rw, const int32_t* cw) {
+    return guard([&] {
+        need(st && rw && cw, "NULL argument");
+        for (int b = 0; b < 3; ++b) {
+            need(rw[b] == 1 || rw[b] == 2 || rw[b] == 4, "rows per workgroup: 1, 2 or 4");
+            need(cw[b] == 1 || cw[b] == 2 || cw[b] == 4, "waves per workgroup: 1, 2 or 4");
+        }
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        for (int b = 0; b < 3; ++b) {
+            st->shape.rw[b] = rw[b];
+            st->shape.cw[b] = cw[b];
+        }
+    });
+}
+
 int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
     return guard([&] {
         need(st && node_xyz, "NULL argument");

@@ -115,21 +115,25 @@ __device__ __forceinline__ int xcd_tile(int bx, int G) { return (bx & 7) * (G >>
 // holds rows l + 64 m of every sheet, loaded straight from T1 (whole contiguous columns); the forward FFT, the
 // kernel multiply-add over source sheets per pixel (both sheets of a pixel sit in the same lane) and the inverse
 // FFT run in registers with the wave's LDS exchanges between passes; rows 0:n0 go straight to T2.
-template <class T, int E, int S>
-__global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
+template <class T, int E, int S, int CW>
+__global__ __launch_bounds__(64 * CW) void lp_cols_mac_w(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cpx<T>* ex = reinterpret_cast<cpx<T>*>(smem_raw);
     constexpr int N = 64 * E;
-    // one (column, right-hand side) per workgroup, in an order that keeps a column's right-hand sides, and the
-    // columns j and P1 - j that read the same kernel row, next to each other on one XCD (xcd_tile)
+    const int wave = threadIdx.x >> 6;
+    cpx<T>* ex = reinterpret_cast<cpx<T>*>(smem_raw) + wave * (N + N / 8);
+    // one (column, right-hand side) per wave, CW consecutive ones per workgroup, in an order that keeps a column's
+    // right-hand sides, and the columns j and P1 - j that read the same kernel row, next to each other on one XCD
+    // (xcd_tile). A wave past the last item runs item 0 without storing it, so every wave meets every barrier.
     const int nr = a.rhs.nr;
-    const int item = xcd_tile(blockIdx.x, gridDim.x);
-    if (item >= (a.nblk_cols[0] + a.nblk_cols[1]) * nr) return;
+    const int items = (a.nblk_cols[0] + a.nblk_cols[1]) * nr;
+    int item = xcd_tile(blockIdx.x, gridDim.x) * CW + wave;
+    const bool live = item < items;
+    if (!live) item = 0;
     int k = item / nr, oi = 0;
     const int p = item - k * nr;
     if (k >= a.nblk_cols[0]) { k -= a.nblk_cols[0]; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
-    const int lane = threadIdx.x, n0 = o.n0, P1 = o.P1;
+    const int lane = threadIdx.x & 63, n0 = o.n0, P1 = o.P1;
     const int j = (k & 1) ? P1 - 1 - (k >> 1) : (k >> 1);
     const cpx<T>* T1 = o.T1 + p * a.t_stride;
     cpx<T>* T2 = o.T2 + p * a.t_stride;
@@ -163,6 +167,7 @@ __global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
         for (int t = 0; t < S; ++t) v[t][m] = out[t];
     }
     wave_fft<T, E, S, true>(v, ex, o.tw0, lane);
+    if (!live) return;
 #pragma unroll
     for (int t = 0; t < S; ++t)
 #pragma unroll
@@ -175,9 +180,8 @@ __global__ __launch_bounds__(64) void lp_cols_mac_w(LpArgs<T> a) {
 // Passes A and C with the row transforms in registers (row length P1 = 64 E): a workgroup is RW waves, one row
 // each, of one sheet's tile of rows [i0, i0 + RW). Each row lives in LDS as a padded strip (xpad) that is also its
 // wave's exchange buffer. Between LDS and HBM the tile moves transposed (T1 / T2 are [sheet][j][i]): RW-row segments.
-constexpr int RW = 4;
 
-template <class T, int E>
+template <class T, int E, int RW>
 __global__ __launch_bounds__(64 * RW) void lp_rows_fwd_w(LpArgs<T> a, int nb0, int nblk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int N = 64 * E, NS = N + N / 8;
@@ -214,7 +218,7 @@ __global__ __launch_bounds__(64 * RW) void lp_rows_fwd_w(LpArgs<T> a, int nb0, i
     }
 }
 
-template <class T, int E>
+template <class T, int E, int RW>
 __global__ __launch_bounds__(64 * RW) void lp_rows_inv_w(LpArgs<T> a, int nb0, int nblk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int N = 64 * E, NS = N + N / 8;
@@ -267,52 +271,77 @@ __global__ __launch_bounds__(64 * RW) void lp_rows_inv_w(LpArgs<T> a, int nb0, i
     }
 }
 
-// The register forms of passes A and C when both orientations share a row length they have (256, 512, 1024).
+// The register forms of passes A and C when both orientations share a row length they have (256, 512, 1024), with
+// RW rows per workgroup.
+template <class T, int E, int RW>
+bool launch_rows_w_rw(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st, int P1) {
+    int nb[2] = {0, 0};
+    for (int oi = 0; oi < 2; ++oi)
+        if (a.nblk_cols[oi] != 0) nb[oi] = a.S * ((a.o[oi].n0 + RW - 1) / RW);
+    const int blocks = nb[0] + nb[1];
+    if (blocks == 0) return false;
+    const size_t lds = (size_t)RW * (P1 + P1 / 8) * sizeof(cpx<T>);
+    if (lds > 65536) return false;            // (P1 = 1024 in FP64 with 4 rows) the LDS kernels take it
+    const int grid = (blocks + 7) / 8 * 8;
+    if (inverse)
+        hipLaunchKernelGGL((lp_rows_inv_w<T, E, RW>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);
+    else
+        hipLaunchKernelGGL((lp_rows_fwd_w<T, E, RW>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);
+    return true;
+}
+
+template <class T, int E>
+bool launch_rows_w_e(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st, int P1, int rw) {
+    switch (rw) {
+    case 1: return launch_rows_w_rw<T, E, 1>(a, nr_rhs, inverse, st, P1);
+    case 2: return launch_rows_w_rw<T, E, 2>(a, nr_rhs, inverse, st, P1);
+    default: return launch_rows_w_rw<T, E, 4>(a, nr_rhs, inverse, st, P1);
+    }
+}
+
 template <class T>
-bool launch_rows_w(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st) {
-    int P1 = 0, nb[2] = {0, 0};
+bool launch_rows_w(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st, int rw) {
+    int P1 = 0;
     for (int oi = 0; oi < 2; ++oi) {
         if (a.nblk_cols[oi] == 0) continue;
         if (P1 != 0 && a.o[oi].P1 != P1) return false;
         P1 = a.o[oi].P1;
-        nb[oi] = a.S * ((a.o[oi].n0 + RW - 1) / RW);
     }
-    const int blocks = nb[0] + nb[1];
-    if (blocks == 0) return false;
-    const size_t lds = (size_t)RW * (P1 + P1 / 8) * sizeof(cpx<T>);
-    if (lds > 65536) return false;            // (P1 = 1024 in FP64) the LDS kernels take it
-    const int grid = (blocks + 7) / 8 * 8;
-#define GT_ROWS(E)                                                                                                 \
-    if (inverse)                                                                                                   \
-        hipLaunchKernelGGL((lp_rows_inv_w<T, E>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);  \
-    else                                                                                                           \
-        hipLaunchKernelGGL((lp_rows_fwd_w<T, E>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);  \
-    return true;
     switch (P1) {
-    case 256: GT_ROWS(4)
-    case 512: GT_ROWS(8)
-    case 1024: GT_ROWS(16)
+    case 256: return launch_rows_w_e<T, 4>(a, nr_rhs, inverse, st, P1, rw);
+    case 512: return launch_rows_w_e<T, 8>(a, nr_rhs, inverse, st, P1, rw);
+    case 1024: return launch_rows_w_e<T, 16>(a, nr_rhs, inverse, st, P1, rw);
     default: return false;
     }
-#undef GT_ROWS
+}
+
+template <class T, int E, int CW>
+bool launch_cols_w_cw(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
+    const size_t lds = (size_t)CW * (64 * E + 8 * E) * sizeof(cpx<T>);
+    if (lds > 65536) return false;
+    const int groups = (ncol * nr + CW - 1) / CW;
+    const int grid = (groups + 7) / 8 * 8;
+    switch (a.S) {
+    case 1: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 1, CW>), dim3(grid), dim3(64 * CW), lds, st, a); return true;
+    case 2: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 2, CW>), dim3(grid), dim3(64 * CW), lds, st, a); return true;
+    case 3: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 3, CW>), dim3(grid), dim3(64 * CW), lds, st, a); return true;
+    case 4: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 4, CW>), dim3(grid), dim3(64 * CW), lds, st, a); return true;
+    default: return false;
+    }
 }
 
 template <class T, int E>
-bool launch_cols_w_e(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
-    const size_t lds = (64 * E + 8 * E) * sizeof(cpx<T>);
-    const int grid = (ncol * nr + 7) / 8 * 8;
-    switch (a.S) {
-    case 1: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 1>), dim3(grid), dim3(64), lds, st, a); return true;
-    case 2: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 2>), dim3(grid), dim3(64), lds, st, a); return true;
-    case 3: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 3>), dim3(grid), dim3(64), lds, st, a); return true;
-    case 4: hipLaunchKernelGGL((lp_cols_mac_w<T, E, 4>), dim3(grid), dim3(64), lds, st, a); return true;
-    default: return false;
+bool launch_cols_w_e(const LpArgs<T>& a, int ncol, int nr, hipStream_t st, int cw) {
+    switch (cw) {
+    case 2: return launch_cols_w_cw<T, E, 2>(a, ncol, nr, st) || launch_cols_w_cw<T, E, 1>(a, ncol, nr, st);
+    case 4: return launch_cols_w_cw<T, E, 4>(a, ncol, nr, st) || launch_cols_w_cw<T, E, 1>(a, ncol, nr, st);
+    default: return launch_cols_w_cw<T, E, 1>(a, ncol, nr, st);
     }
 }
 
 // The register form of pass B when both orientations share a column length it has (256, 512, 1024) and S <= 4.
 template <class T>
-bool launch_cols_w(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
+bool launch_cols_w(const LpArgs<T>& a, int ncol, int nr, hipStream_t st, int cw) {
     int P0 = 0;
     for (int oi = 0; oi < 2; ++oi) {
         if (a.nblk_cols[oi] == 0) continue;
@@ -320,9 +349,9 @@ bool launch_cols_w(const LpArgs<T>& a, int ncol, int nr, hipStream_t st) {
         P0 = a.o[oi].P0;
     }
     switch (P0) {
-    case 256: return launch_cols_w_e<T, 4>(a, ncol, nr, st);
-    case 512: return launch_cols_w_e<T, 8>(a, ncol, nr, st);
-    case 1024: return launch_cols_w_e<T, 16>(a, ncol, nr, st);
+    case 256: return launch_cols_w_e<T, 4>(a, ncol, nr, st, cw);
+    case 512: return launch_cols_w_e<T, 8>(a, ncol, nr, st, cw);
+    case 1024: return launch_cols_w_e<T, 16>(a, ncol, nr, st, cw);
     default: return false;
     }
 }
@@ -405,14 +434,16 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
     constexpr int NTH = 256;
     const int nr = a.rhs.nr;
-    if (!launch_rows_w(a, nr, false, st))
+    const int bucket = nr == 1 ? 0 : nr <= 4 ? 1 : 2;
+    const int rw = sh.rw[bucket], cw = sh.cw[bucket];
+    if (!launch_rows_w(a, nr, false, st, rw))
         hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_fwd");
-    if (!launch_cols_w(a, ncol, nr, st))
+    if (!launch_cols_w(a, ncol, nr, st, cw))
         hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols,
                            st, a);
     GT_LAUNCH("lp_cols_mac");
-    if (!launch_rows_w(a, nr, true, st))
+    if (!launch_rows_w(a, nr, true, st, rw))
         hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     GT_LAUNCH("lp_rows_inv");
 }

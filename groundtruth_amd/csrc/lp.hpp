@@ -69,6 +69,9 @@ struct TailArgs {
 
 struct LpShape {
     size_t lds_rows = 0, lds_cols = 0;
+    // launch shapes of the register passes per batch-size class (1; 2-4; 5-16 right-hand sides): rows per row-pass
+    // workgroup and waves per column-pass workgroup. They change no arithmetic; an autotuner may set them.
+    int rw[3] = {4, 4, 4}, cw[3] = {1, 1, 1};
 };
 
 template <class T>

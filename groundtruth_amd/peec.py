@@ -58,6 +58,7 @@ def _kernel_pair(nx: int, ny: int, d: float, dz: float, px: tuple, py: tuple):
 
 
 _NEAR: dict = {}
+_LP_PLANS: dict = {}            # (sheets, nx, ny, Px, Py) -> the tuned L_p launch shapes (rows, waves per class)
 
 
 def _near_table(d: float, dz: float) -> np.ndarray:
@@ -315,7 +316,9 @@ class PlaneStack:
         if self._base is not None:
             self._derive()
             return
-        self._factor(self._create())
+        h = self._create()
+        self._tune(h)
+        self._factor(h)
 
     def _create(self):
         """The device stack without its Schur factor: layout, incidence, kernel spectra (gt_stack_create[_gen])."""
@@ -345,6 +348,22 @@ class PlaneStack:
             d.kx, d.ky = _lib.ptr(self._kx), _lib.ptr(self._ky)
             _lib.check(L.gt_stack_create(self.ctx.handle, C.byref(d), C.byref(h)), "gt_stack_create")
         return h
+
+    def _tune(self, h):
+        """The L_p product's launch shapes for this board's shape: tuned on the first stack of the shape in this
+        process (gt_stack_lp_autotune), the same plan applied to every later one."""
+        L = _lib.lib()
+        key = (self.n_sheets, self.nx, self.ny, tuple(self.Px), tuple(self.Py))
+        plan = _LP_PLANS.get(key)
+        if plan is None:
+            rw, cw = np.zeros(3, np.int32), np.zeros(3, np.int32)
+            _lib.check(L.gt_stack_lp_autotune(h, 5, _lib.ptr(rw, _lib.i32p), _lib.ptr(cw, _lib.i32p)),
+                       "gt_stack_lp_autotune")
+            plan = _LP_PLANS[key] = (rw, cw)
+        else:
+            _lib.check(L.gt_stack_set_lp_plan(h, _lib.ptr(plan[0], _lib.i32p), _lib.ptr(plan[1], _lib.i32p)),
+                       "gt_stack_set_lp_plan")
+        self.lp_plan = {"rows_per_workgroup": plan[0].tolist(), "waves_per_workgroup": plan[1].tolist()}
 
     def _factor(self, h):
         """The Schur factor of the stack `h` (gt_schur_factor), then the stack is this PlaneStack's."""

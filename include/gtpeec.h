@@ -123,6 +123,14 @@ typedef struct {
 int gt_strip_coupling(gt_stack* stack, double d, double mu, int32_t nseg, const gt_segment* segs,
                       const double* sheet_z, const double* sheet_t, void* psi_dev);
 
+/* The L_p product's launch shapes, tuned (no counterpart in the reference): for each batch-size class (1, 2-4, 5-16
+ * right-hand sides) the rows per row-pass workgroup (1, 2, 4) and waves per column-pass workgroup (1, 2, 4) that
+ * ran `reps` products of 1, 4 and 8 right-hand sides fastest on this stack. The shapes change no arithmetic, so a
+ * product's bits do not depend on them. rw_out, cw_out: 3 values each, the plan kept. gt_stack_set_lp_plan applies
+ * a plan found before (the same for every stack of one shape). Both hold for the what-if stacks of the stack. */
+int gt_stack_lp_autotune(gt_stack* stack, int32_t reps, int32_t* rw_out, int32_t* cw_out);
+int gt_stack_set_lp_plan(gt_stack* stack, const int32_t* rw, const int32_t* cw);
+
 /* ---------------------------------------------------------------- Schur complement preconditioner
  * Replaces peec.py:2345-2445 `_factorise` (SuperLU of S = Ar^T D^-1 Ar in a host thread) and the C shim
  * solver/accelerate_shim.c:21-59 (gt_factor / gt_solve / gt_free). With every strip of a stack alike, D^-1 is one
