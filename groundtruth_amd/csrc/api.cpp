@@ -282,6 +282,14 @@ int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min) {
     });
 }
 
+int gt_ctx_graphs(gt_ctx* ctx, int enable) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->graphs = enable != 0;
+    });
+}
+
 int gt_ctx_lp_timing(gt_ctx* ctx, int enable) {
     return guard([&] {
         need(ctx != nullptr, "ctx is NULL");

@@ -249,7 +249,7 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
 
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr) {
     if (!st->schur) fail(GT_ERR_ARG, "the stack has no Schur factor (gt_schur_factor first)");
-    schur_solve_device(*st->schur, b, x, nr, st->ctx->stream);
+    schur_solve_device(*st->schur, b, x, nr, st->ctx->stream, st->ctx->graphs);
 }
 
 namespace {

@@ -95,8 +95,22 @@ constexpr int SCHUR_PV = 8;
 constexpr int SCHUR_NARROW = 32;       // forward: the widest supernode of 64-row tiles
 constexpr int SCHUR_TILE = 16;         // rows (forward, wide) or columns (backward) of a split tile
 
+// A captured Schur solve (the level launches of one batch size, input and output)
+struct SchurGraph {
+    int nr;
+    const zd* b;
+    zd* x;
+    hipGraphExec_t exec;
+};
+constexpr size_t SCHUR_GRAPHS = 32;
+
 struct Schur {
     SchurHost h;
+    std::vector<SchurGraph> graphs;     // captured solves, replayed when the same (nr, b, x) comes again
+    Schur() = default;
+    Schur(const Schur&) = delete;
+    Schur& operator=(const Schur&) = delete;
+    ~Schur();
     DevBuf<int> perm, rows, ctab;
     DevBuf<int64_t> cidx;
     DevBuf<SnItem> fwd, bwd, ffin, bfin;
@@ -116,6 +130,6 @@ struct Schur {
 
 // x = B^-1 b on the device for nr <= 16 right-hand sides, interleaved: entry (node, p) at node * nr + p, in the old
 // numbering (b and x distinct)
-void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t s);
+void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t s, bool graph);
 
 }  // namespace gt

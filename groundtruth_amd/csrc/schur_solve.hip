@@ -350,8 +350,9 @@ int64_t Schur::device_bytes() const {
                      panel.bytes() + part.bytes() + ffin.bytes() + bfin.bytes() + ubuf.bytes() + zp.bytes() + xp.bytes());
 }
 
-void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) {
-    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
+namespace {
+
+void launch_levels(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) {
     SolveArgs a{nullptr, S.perm.p, S.rows.p, reinterpret_cast<const int2*>(S.ctab.p), S.cidx.p, S.panel.p, S.ubuf.p,
                 S.zp.p, S.xp.p, b, x, S.part.p};
     if (nr == 1) run_levels<1>(S, a, nr, st);
@@ -359,6 +360,48 @@ void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) {
     else if (nr <= 4) run_levels<4>(S, a, nr, st);
     else if (nr <= 8) run_levels<8>(S, a, nr, st);
     else run_levels<16>(S, a, nr, st);
+}
+
+}  // namespace
+
+// With graphs, the level launches of a (nr, b, x) are captured once into a hipGraph and replayed: the same kernels
+// with the same arguments, so the same bits, and one submission from the host instead of one per level. The
+// sequence holds kernels only (no memset nodes). At most SCHUR_GRAPHS are kept, the oldest dropped first.
+void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t st, bool graph) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
+    if (!graph) {
+        launch_levels(S, b, x, nr, st);
+        return;
+    }
+    for (SchurGraph& g : S.graphs)
+        if (g.nr == nr && g.b == b && g.x == x) {
+            GT_HIP(hipGraphLaunch(g.exec, st));
+            return;
+        }
+    hipGraph_t gr = nullptr;
+    GT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_levels(S, b, x, nr, st);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+    }
+    GT_HIP(hipStreamEndCapture(st, &gr));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    GT_HIP(e);
+    if (S.graphs.size() >= SCHUR_GRAPHS) {
+        (void)hipGraphExecDestroy(S.graphs.front().exec);
+        S.graphs.erase(S.graphs.begin());
+    }
+    S.graphs.push_back({nr, b, x, exec});
+    GT_HIP(hipGraphLaunch(exec, st));
+}
+
+Schur::~Schur() {
+    for (SchurGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
 }
 
 }  // namespace gt

@@ -58,6 +58,11 @@ int gt_ctx_lp_timing(gt_ctx* ctx, int enable);
  * Default 256; INT32_MAX keeps the whole factorisation on the host. (No counterpart in the reference, which
  * factorises with SuperLU on the host, peec.py:2379-2424.) */
 int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min);
+/* Schur solves on this context captured in hipGraphs: the level launches of each batch size are recorded once and
+ * replayed with one submission; the same kernels and arguments, so the same bits. Off by default: on the MI355X a
+ * replay takes as long as the launches it replaces (the levels' time is their own latency on the device, not the
+ * host's submissions). (No counterpart in the reference.) */
+int gt_ctx_graphs(gt_ctx* ctx, int enable);
 int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms);
 
 /* ---------------------------------------------------------------- stack
