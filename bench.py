@@ -15,7 +15,8 @@ kernel quadrant once) / the average launch. traffic = the HBM bytes of an L_p la
 WRITE_SIZE runs committed under profiles/ (launches of 8 right-hand sides), scaled to the timed launches' average
 right-hand sides. Beside it, a single-right-hand-side product timed over 200 launches, and the oracle's CPU time on a
 bounded sample of the same workload. Then the served what-if stream (serve_stream): p50/p99 latency and solves/s of
-the continuous-batching server, beside the same stream solved one solve at a time.
+the continuous-batching server, beside the same stream solved one solve at a time; and the what-if latency at 512^2
+cells per layer (serve_whatif_512), the north star's p99 < 200 ms case.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 python bench.py --lp-only [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N right-hand sides,
@@ -242,6 +243,61 @@ def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
     return out
 
 
+WHATIF_RATE = 4.0              # the 512^2 what-if stream: Poisson arrivals per second
+WHATIF_SECONDS = 12.0
+
+
+def serve_whatif_512(rate: float = WHATIF_RATE, seconds: float = WHATIF_SECONDS, seed: int = 3) -> dict:
+    """The north star's latency case: what-if requests on a resident two-layer board of 512 x 512 cells per layer
+    (1.56 M unknowns; 0.125 mm pitch, the 256^2 board's size), each one added stitching via at a random copper cell
+    and one of the board's frequencies, Poisson arrivals, FP64 GMRES to 1e-8 from the board's solution (primed before
+    the stream, untimed)."""
+    from groundtruth_amd import boards, peec
+    from groundtruth_amd.serve import Request, Server
+    b = boards.two_layer(n=512, pitch_mm=0.125)
+    t0 = time.perf_counter()
+    st = peec.PlaneStack(b.layers, b.branches, device=0)
+    srv = Server(capacity=16)
+    srv.add_board("two_layer_512", st, routes=b.routes)
+    setup_s = time.perf_counter() - t0
+    srv.prime("two_layer_512", b.freqs, tol=SERVE_TOL)
+    rng = np.random.default_rng(seed)
+
+    def whatif(k):
+        i, j = (int(v) for v in rng.integers(1, 511, 2))
+        return Request("two_layer_512", b.routes[0], [float(b.freqs[int(rng.integers(len(b.freqs)))])],
+                       extra=boards._stitch(b.layers, i, j, f"whatif{k}"), tol=SERVE_TOL)
+    srv.submit(whatif(-1))            # the warm start's first use, untimed
+    srv.drain()
+    srv.reset_stats()
+    arr, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        arr.append(t)
+    reqs = [whatif(k) for k in range(len(arr))]
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(arr) or srv.busy():
+        now = time.perf_counter() - t0
+        while i < len(arr) and arr[i] <= now:
+            srv.submit(reqs[i], now=t0 + arr[i])
+            i += 1
+        if srv.busy():
+            srv.turn()
+        elif i < len(arr):
+            time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
+    s = srv.stats()
+    its = [q.solutions[0].iterations for q in srv.completed if q.error is None]
+    srv.close()
+    return {"requests": s["requests"], "failed": s["failed"], "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"],
+            "max_ms": s["max_ms"], "solves_per_s": s["solves_per_s"], "rate_per_s": rate,
+            "unknowns": st.nb + st.n_keep, "setup_s": round(setup_s, 2), "warm_starts": s["warm_starts"],
+            "iterations_median": float(np.median(its)) if its else None, "tol": SERVE_TOL,
+            "target_p99_ms": 200.0}
+
+
 def cpu_baseline(board, route, f: float) -> dict:
     """The oracle (numpy / scipy restatement of the reference's path) solving one frequency of the workload on
     this host: its factorisation and its GMRES, timed once after a warm-up of the kernels."""
@@ -421,6 +477,7 @@ def main():
                                     "the resident factor); plain FP64 GMRES to 1e-8; one_at_a_time: the same "
                                     "stream solved in arrival order one solve at a time; what-ifs start from the "
                                     "board solution at f (the boards primed before the stream, untimed)")
+        out["whatif_512"] = serve_whatif_512()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])
     print(json.dumps(out))
