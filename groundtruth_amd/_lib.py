@@ -88,6 +88,8 @@ SIGNATURES = {
     "gt_gmres_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GmresParams), C.POINTER(GmresResult)]),
     "gt_gmres_batch": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                  C.POINTER(C.c_void_p), C.POINTER(GmresParams), C.POINTER(GmresResult)]),
+    "gt_gmres_batch_fp32_basis": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                            C.POINTER(C.c_void_p), C.POINTER(GmresParams), C.POINTER(GmresResult)]),
     "gt_gmres_batch_mixed": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p), C.POINTER(GmresParams), C.c_double, C.c_int32,
                                        C.POINTER(GmresResult)]),

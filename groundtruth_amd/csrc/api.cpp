@@ -11,6 +11,8 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
                  gt_gmres_result* res);
 void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
                        const gt_gmres_params& p, double inner_tol, int max_refine, gt_gmres_result* res);
+void gmres_batch_fp32_basis(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
+                            const gt_gmres_params& p, gt_gmres_result* res);
 int64_t session_plan_bytes(const gt_stack* st, int cap, int restart);
 gt_session* session_create(gt_stack* st, int cap, int restart);
 int64_t session_bytes(const gt_session* ss);
@@ -766,6 +768,27 @@ int gt_gmres_batch(gt_stack* st, int32_t nr, gt_op* const* ops, const void* cons
         }
         GT_HIP(hipSetDevice(st->ctx->device));
         gmres_batch(st, nr, ops, b.data(), x.data(), *p, res);
+    });
+}
+
+int gt_gmres_batch_fp32_basis(gt_stack* st, int32_t nr, gt_op* const* ops, const void* const* b_dev,
+                              void* const* x_dev, const gt_gmres_params* p, gt_gmres_result* res) {
+    return guard([&] {
+        need(st && ops && b_dev && x_dev && p && res, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        need(nr >= 1 && nr <= KMAX, "1 to 16 right-hand sides per batch");
+        need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        need(p->tol > 0 && p->max_cycles >= 0 && p->extra_cycles >= 0, "bad GMRES parameters");
+        std::vector<const zd*> b(nr);
+        std::vector<zd*> x(nr);
+        for (int i = 0; i < nr; ++i) {
+            need(ops[i] && ops[i]->st == st, "every operator must belong to the stack");
+            need(b_dev[i] && x_dev[i] && b_dev[i] != x_dev[i], "NULL or aliased vector");
+            b[i] = (const zd*)b_dev[i];
+            x[i] = (zd*)x_dev[i];
+        }
+        GT_HIP(hipSetDevice(st->ctx->device));
+        gmres_batch_fp32_basis(st, nr, ops, b.data(), x.data(), *p, res);
     });
 }
 

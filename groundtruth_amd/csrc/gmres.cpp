@@ -37,12 +37,13 @@ struct Vec {
     cpx<T>* p = nullptr;
 };
 
-template <class T>
+template <class TV, class TW>
 struct Rhs {
     gt_op* op = nullptr;
-    const cpx<T>* b = nullptr;             // right-hand side (device)
+    const cpx<TW>* b = nullptr;            // right-hand side (device)
     gt_gmres_params p{};                   // its tolerance and budgets
-    Vec<T> V, w, x, xt, r, t, u;              // basis (restart + 1 rows) and vectors, in the workspace
+    Vec<TV> V;                             // basis (restart + 1 rows)
+    Vec<TW> w, x, xt, r, t, u;             // vectors, in the workspace
     State state = FREE;
     bool conv = false;
     double bnorm = 0, last = 0, aim = 0, resid = 0, hn = 0;
@@ -120,15 +121,18 @@ struct GmresWork {
 
 namespace {
 
-// T: the storage of the vectors and basis (double; float for the mixed-precision inner solve). The Hessenberg
-// matrix, the rotations and every reduction are double.
-template <class T>
+// TV: the storage of the basis, TW: of the vectors (double; float and float for the mixed-precision inner solve;
+// float and double for the solve whose basis alone is single precision). The Hessenberg matrix, the rotations and
+// every reduction are double.
+template <class TV, class TW>
 struct Batch {
-    using cz = cpx<T>;
+    using cz = cpx<TW>;
+    using cv = cpx<TV>;
+    using RhsT = Rhs<TV, TW>;
     gt_stack* st;
     hipStream_t s;
     int n, restart, cap;
-    std::vector<Rhs<T>> R;
+    std::vector<RhsT> R;
     GmresWork& gw;
     Krylov& kr;
     DevBuf<zd>& ydev;
@@ -141,8 +145,8 @@ struct Batch {
           kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
         gw.ensure((size_t)n, restart + 1, cap);
         for (int i = 0; i < cap; ++i) {
-            Rhs<T>& q = R[i];
-            q.V.p = reinterpret_cast<cz*>(gw.V[i].p);
+            RhsT& q = R[i];
+            q.V.p = reinterpret_cast<cv*>(gw.V[i].p);
             cz* v = reinterpret_cast<cz*>(gw.vec[i].p);
             q.w.p = v;
             q.x.p = v + (size_t)n;
@@ -159,7 +163,7 @@ struct Batch {
 
     // A solve enters slot i (FREE or READY): its set-up runs at the next init().
     void submit(int i, gt_op* op, const cz* b, const gt_gmres_params& p) {
-        Rhs<T>& q = R[i];
+        RhsT& q = R[i];
         q.op = op;
         q.b = b;
         q.p = p;
@@ -170,12 +174,12 @@ struct Batch {
     }
 
     bool running() const {
-        for (const Rhs<T>& q : R)
+        for (const RhsT& q : R)
             if (q.state == PENDING || q.state == START || q.state == ARNOLDI) return true;
         return false;
     }
 
-    cz* row(Rhs<T>& q, int j) { return q.V.p + (size_t)j * n; }
+    cv* row(RhsT& q, int j) { return q.V.p + (size_t)j * n; }
 
     void fetch(int cnt) {
         GT_HIP(hipMemcpyAsync(red.p, kr.red.p, sizeof(double) * kr.P * cnt, hipMemcpyDeviceToHost, s));
@@ -193,13 +197,13 @@ struct Batch {
             tmp[i] = R[who[i]].t.p;
             R[who[i]].products++;
         }
-        ops_precond<T>(st, c, ops.data(), in.data(), tmp.data());
+        ops_precond<TW>(st, c, ops.data(), in.data(), tmp.data());
         std::vector<const cz*> tin(tmp.begin(), tmp.end());
-        ops_matvec<T>(st, c, ops.data(), tin.data(), out.data());
+        ops_matvec<TW>(st, c, ops.data(), tin.data(), out.data());
     }
 
     void norms(const std::vector<int>& who, const std::vector<const cz*>& v, std::vector<double>& out) {
-        KrSetT<T> k{};
+        KrSetT<TV, TW> k{};
         k.nr = (int)who.size();
         for (int i = 0; i < k.nr; ++i) k.w[i] = const_cast<cz*>(v[i]);
         kr.norm2(k, s);
@@ -211,15 +215,18 @@ struct Batch {
     // dst_q = from_q + V_q[0:k_q]^T y_q   (from_q copied first when distinct)
     void combine(const std::vector<int>& who, const std::vector<std::vector<cd>>& ys, const std::vector<cz*>& dst,
                  const std::vector<const cz*>& from) {
-        VecOpsT<T> c{};
-        c.nr = (int)who.size();
+        VecOpsT<TW> cp{};
+        cp.nr = (int)who.size();
         bool any = false;
-        for (int i = 0; i < c.nr; ++i) {
-            c.dst[i] = dst[i];
-            c.a[i] = from[i];
+        for (int i = 0; i < cp.nr; ++i) {
+            cp.dst[i] = dst[i];
+            cp.a[i] = from[i];
             if (from[i] != dst[i]) any = true;
         }
-        if (any) k_copy(c, n, s);
+        if (any) k_copy(cp, n, s);
+        VecOpsT<TW, TV> c{};
+        c.nr = cp.nr;
+        for (int i = 0; i < c.nr; ++i) c.dst[i] = dst[i];
         spin_sync(s);                        // the previous copy out of ybuf has finished
         for (int i = 0; i < c.nr; ++i) {
             c.a[i] = R[who[i]].V.p;
@@ -230,7 +237,7 @@ struct Batch {
         k_combine(c, ydev.p, restart + 1, n, s);
     }
 
-    std::vector<cd> back_substitute(const Rhs<T>& q, int k) const {
+    std::vector<cd> back_substitute(const RhsT& q, int k) const {
         std::vector<cd> y(k);
         const int ld = restart;
         for (int i = k - 1; i >= 0; --i) {
@@ -247,7 +254,7 @@ struct Batch {
         std::vector<cz*> out;
         for (int q : who) out.push_back(R[q].u.p);
         apply(who, in, out);
-        VecOpsT<T> v{};
+        VecOpsT<TW> v{};
         v.nr = (int)who.size();
         for (int i = 0; i < v.nr; ++i) {
             v.dst[i] = R[who[i]].r.p;
@@ -257,7 +264,7 @@ struct Batch {
         k_sub(v, n, s);
     }
 
-    static int max_its_of(const Rhs<T>& q) {
+    static int max_its_of(const RhsT& q) {
         return q.p.max_its > 0 ? q.p.max_its : std::numeric_limits<int>::max();
     }
 
@@ -267,7 +274,7 @@ struct Batch {
         for (int i = 0; i < cap; ++i)
             if (R[i].state == PENDING) P.push_back(i);
         if (P.empty()) return;
-        VecOpsT<T> v{};
+        VecOpsT<TW> v{};
         v.nr = (int)P.size();
         for (int i = 0; i < v.nr; ++i) v.dst[i] = R[P[i]].x.p;
         k_zero(v, n, s);
@@ -281,7 +288,7 @@ struct Batch {
         std::vector<double> b0;
         norms(P, bs, b0);
         for (size_t i = 0; i < P.size(); ++i) {
-            Rhs<T>& q = R[P[i]];
+            RhsT& q = R[P[i]];
             q.bnorm = q.p.bnorm > 0 ? q.p.bnorm : b0[i];
             q.last = std::numeric_limits<double>::infinity();
             q.state = START;
@@ -295,10 +302,10 @@ struct Batch {
 
     // Gram-Schmidt for the slots `who` (all on the fused pass or all not), the values fetched to the host
     void project(const std::vector<int>& who, bool fuse) {
-        KrSetT<T> k{};
+        KrSetT<TV, TW> k{};
         k.nr = (int)who.size();
         for (int i = 0; i < k.nr; ++i) {
-            Rhs<T>& q = R[who[i]];
+            RhsT& q = R[who[i]];
             k.V[i] = q.V.p;
             k.m[i] = q.j + 1;
             k.w[i] = q.w.p;
@@ -324,9 +331,9 @@ struct Batch {
             for (int q : S) rs.push_back(R[q].r.p);
             std::vector<double> beta;
             norms(S, rs, beta);
-            VecOpsT<T> v{};
+            VecOpsT<TV, TW> v{};
             for (size_t i = 0; i < S.size(); ++i) {
-                Rhs<T>& q = R[S[i]];
+                RhsT& q = R[S[i]];
                 const double b = beta[i];
                 q.resid = b / q.bnorm;
                 if (b <= q.p.tol * q.bnorm) { q.state = DONE; q.conv = true; continue; }
@@ -363,10 +370,19 @@ struct Batch {
         {
             std::vector<const cz*> in;
             std::vector<cz*> out;
-            for (int q : A) {
-                in.push_back(row(R[q], R[q].j));
-                out.push_back(R[q].w.p);
+            if constexpr (std::is_same<TV, TW>::value) {
+                for (int q : A) in.push_back(row(R[q], R[q].j));
+            } else {                          // a single-precision basis row, widened into u for the product
+                VecOpsT<TW, TV> cv{};
+                for (int q : A) {
+                    cv.dst[cv.nr] = R[q].u.p;
+                    cv.a[cv.nr] = row(R[q], R[q].j);
+                    cv.f[cv.nr++] = 1.0;
+                    in.push_back(R[q].u.p);
+                }
+                k_scale(cv, n, s);
             }
+            for (int q : A) out.push_back(R[q].w.p);
             apply(A, in, out);
         }
         // CGS with the DGKS re-pass where |w| fell below 0.7 of its value before the projection. Up to 128 basis rows
@@ -381,7 +397,7 @@ struct Batch {
             if (G.empty()) continue;
             project(G, fuse);
             for (size_t i = 0; i < G.size(); ++i) {
-                Rhs<T>& q = R[G[i]];
+                RhsT& q = R[G[i]];
                 q.h.assign(q.j + 1, 0);
                 for (int r = 0; r <= q.j; ++r) q.h[r] = cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
                 const double w_norm = std::sqrt(val((int)i, kr.slot_w()));
@@ -400,7 +416,7 @@ struct Batch {
         if (!D.empty()) {                     // DGKS on the host's test: the second pass
             project(D, false);
             for (size_t i = 0; i < D.size(); ++i) {
-                Rhs<T>& q = R[D[i]];
+                RhsT& q = R[D[i]];
                 for (int r = 0; r <= q.j; ++r) q.h[r] += cd(val((int)i, 2 * r), val((int)i, 2 * r + 1));
                 q.hn = std::sqrt(val((int)i, kr.slot_w_after()));
             }
@@ -408,7 +424,7 @@ struct Batch {
         // ---- Givens rotations and the decisions
         std::vector<int> C, E, Nx;        // true-residual checks, cycle ends, next basis vector
         for (int qi : A) {
-            Rhs<T>& q = R[qi];
+            RhsT& q = R[qi];
             const int j = q.j, ld = restart;
             std::vector<cd> col(j + 2);
             for (int i = 0; i <= j; ++i) col[i] = q.h[i];
@@ -462,9 +478,9 @@ struct Batch {
             for (int qi : C) rs.push_back(R[qi].r.p);
             std::vector<double> rt;
             norms(C, rs, rt);
-            VecOpsT<T> cp{};
+            VecOpsT<TW> cp{};
             for (size_t i = 0; i < C.size(); ++i) {
-                Rhs<T>& q = R[C[i]];
+                RhsT& q = R[C[i]];
                 q.resid = rt[i] / q.bnorm;
                 if (rt[i] <= q.p.tol * q.bnorm) {
                     cp.dst[cp.nr] = q.x.p;
@@ -487,16 +503,16 @@ struct Batch {
         }
         // ---- the next basis vector, V[j + 1] = w / hn; a full cycle ends
         if (!Nx.empty()) {
-            VecOpsT<T> v{};
+            VecOpsT<TV, TW> v{};
             for (int qi : Nx) {
-                Rhs<T>& q = R[qi];
+                RhsT& q = R[qi];
                 v.dst[v.nr] = row(q, q.j + 1);
                 v.a[v.nr] = q.w.p;
                 v.f[v.nr++] = 1.0 / q.hn;
             }
             k_scale(v, n, s);
             for (int qi : Nx) {
-                Rhs<T>& q = R[qi];
+                RhsT& q = R[qi];
                 if (q.j + 1 >= restart) E.push_back(qi);
                 else q.j++;
             }
@@ -507,7 +523,7 @@ struct Batch {
             std::vector<cz*> dst;
             std::vector<const cz*> from;
             for (int qi : E) {
-                Rhs<T>& q = R[qi];
+                RhsT& q = R[qi];
                 int k = q.k;
                 while (k > 0 && q.H[(size_t)(k - 1) * restart + (k - 1)] == cd(0)) --k;
                 ys.push_back(k ? back_substitute(q, k) : std::vector<cd>());
@@ -532,12 +548,16 @@ struct Batch {
 
 int default_restart(int n) { return (int)std::min(200.0, std::max(100.0, std::floor(4e8 / (16.0 * n)))); }
 
-void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x, const gt_gmres_params& p,
-                 gt_gmres_result* res) {
+namespace {
+
+// TV: the basis storage (double, or float for the single-precision basis); vectors, products and reductions double
+template <class TV>
+void gmres_batch_t(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
+                   const gt_gmres_params& p, gt_gmres_result* res) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
     const int n = st->nb + st->nk;
     if (!st->ctx->gmres) st->ctx->gmres.reset(new GmresWork);
-    Batch<double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), *st->ctx->gmres);
+    Batch<TV, double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), *st->ctx->gmres);
     for (int i = 0; i < nr; ++i) {
         if (ops[i]->st != st) fail(GT_ERR_ARG, "a batch's operators must share one stack");
         bt.submit(i, ops[i], b[i], p);
@@ -554,6 +574,18 @@ void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd
         res[i].products = bt.R[i].products;
         res[i].residual = bt.R[i].resid;
     }
+}
+
+}  // namespace
+
+void gmres_batch(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x, const gt_gmres_params& p,
+                 gt_gmres_result* res) {
+    gmres_batch_t<double>(st, nr, ops, b, x, p, res);
+}
+
+void gmres_batch_fp32_basis(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
+                            const gt_gmres_params& p, gt_gmres_result* res) {
+    gmres_batch_t<float>(st, nr, ops, b, x, p, res);
 }
 
 void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
@@ -634,7 +666,7 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
         ip.tol = inner_tol;
         ip.bnorm = 0;
         ip.restart = restart;
-        Batch<float> bt(st, na, restart, gw);
+        Batch<float, float> bt(st, na, restart, gw);
         for (int i = 0; i < na; ++i) bt.submit(i, aop[i], r32[act[i]], ip);
         bt.run();
         // d = M^-1 y in FP64 (y widened), x += d, u = K x, r = b - u
@@ -710,7 +742,7 @@ struct gt_session_s {
     gt_stack* st = nullptr;
     int cap = 0, restart = 0;
     gt::GmresWork gw;
-    std::unique_ptr<gt::Batch<double>> bt;
+    std::unique_ptr<gt::Batch<double, double>> bt;
     std::vector<gt::zd*> xout;             // the caller's solution vector of each slot
 };
 
@@ -736,7 +768,7 @@ gt_session* session_create(gt_stack* st, int cap, int restart) {
     ss->st = st;
     ss->cap = cap;
     ss->restart = restart > 0 ? restart : default_restart(n);
-    ss->bt.reset(new Batch<double>(st, cap, ss->restart, ss->gw));
+    ss->bt.reset(new Batch<double, double>(st, cap, ss->restart, ss->gw));
     ss->xout.assign(cap, nullptr);
     st->work.ensure(st, cap);
     return ss.release();
@@ -747,7 +779,7 @@ int64_t session_bytes(const gt_session* ss) { return ss->gw.device_bytes(); }
 int session_submit(gt_session* ss, gt_op* op, const zd* b, zd* x, const gt_gmres_params& p) {
     if (op->st != ss->st) fail(GT_ERR_ARG, "the operator belongs to another stack");
     if (p.restart > 0 && p.restart != ss->restart) fail(GT_ERR_ARG, "the session's restart differs");
-    Batch<double>& bt = *ss->bt;
+    Batch<double, double>& bt = *ss->bt;
     for (int i = 0; i < ss->cap; ++i) {
         if (bt.R[i].state != FREE) continue;
         bt.submit(i, op, b, p);
@@ -760,7 +792,7 @@ int session_submit(gt_session* ss, gt_op* op, const zd* b, zd* x, const gt_gmres
 // Up to max_steps Arnoldi steps of the running slots (the newly submitted set up first); the solves that stopped
 // get x = M^-1 y and wait, READY, for their poll. Returns the slots still running.
 int session_step(gt_session* ss, int max_steps, int* ready) {
-    Batch<double>& bt = *ss->bt;
+    Batch<double, double>& bt = *ss->bt;
     bt.init();
     for (int k = 0; k < max_steps; ++k)
         if (!bt.step()) break;
@@ -793,7 +825,7 @@ int session_step(gt_session* ss, int max_steps, int* ready) {
 
 bool session_poll(gt_session* ss, int slot, gt_gmres_result* res) {
     if (slot < 0 || slot >= ss->cap) fail(GT_ERR_ARG, "slot out of range");
-    Rhs<double>& q = ss->bt->R[slot];
+    Rhs<double, double>& q = ss->bt->R[slot];
     if (q.state != READY) return false;
     res->iterations = q.its;
     res->converged = q.conv ? 1 : 0;

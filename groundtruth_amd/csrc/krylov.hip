@@ -36,13 +36,13 @@ __device__ __forceinline__ double block_sum(double val, double* lds) {
 
 // Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk: G basis rows per sweep over w, every row's sum
 // reduced within each wave by shuffles and across the four waves in LDS (two barriers a sweep).
-template <class T>
-__global__ __launch_bounds__(NTH) void kr_dots(KrSetT<T> k, int n, int chunk, double* part, int P, int nblk,
+template <class TV, class TW>
+__global__ __launch_bounds__(NTH) void kr_dots(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
                                                int wslot) {
     __shared__ double red[NTH / 64][2 * G + 1];
     const int p = blockIdx.y;
-    const cpx<T>* __restrict__ V = k.V[p];
-    const cpx<T>* __restrict__ w = k.w[p];
+    const cpx<TV>* __restrict__ V = k.V[p];
+    const cpx<TW>* __restrict__ w = k.w[p];
     const int m = k.m[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -93,15 +93,15 @@ __global__ __launch_bounds__(NTH) void kr_dots(KrSetT<T> k, int n, int chunk, do
 // (element e = lane & 7, row group g = lane >> 3): a lane holds rows g, g + 8, .. (at most RPL) of its element's basis
 // column, so each basis value is loaded once for both; the row groups' parts of V h meet by butterfly sums, and the
 // products are summed over the block's elements per row.
-template <class T, int RPL>
-__global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<T> k, int n, int chunk, const double* red, double* part,
+template <class TV, class TW, int RPL>
+__global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<TV, TW> k, int n, int chunk, const double* red, double* part,
                                                       int P, int nblk, int slot_after, int slot_h2) {
     __shared__ double hs_raw[2 * 8 * RPL];
     __shared__ double wsum[NTH / 64][2 * 8 * RPL + 1];
     zd* hs = reinterpret_cast<zd*>(hs_raw);
     const int p = blockIdx.y;
-    const cpx<T>* __restrict__ V = k.V[p];
-    cpx<T>* w = k.w[p];
+    const cpx<TV>* __restrict__ V = k.V[p];
+    cpx<TW>* w = k.w[p];
     const int m = k.m[p];
     const double* h = red + (size_t)p * P;
     for (int r = threadIdx.x; r < 8 * RPL; r += NTH) hs[r] = r < m ? zd(h[2 * r], h[2 * r + 1]) : zd();
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<T> k, int n, int ch
             sim += __shfl_xor(sim, off);
         }
         // the stored value (rounded to T) is what the products and the norm see
-        const cpx<T> ws = stz<T>(wi0.re - sre, wi0.im - sim);
+        const cpx<TW> ws = stz<TW>(wi0.re - sre, wi0.im - sim);
         const zd wi = ldz(ws);
         if (in && g == 0) {
             w[i] = ws;
@@ -178,8 +178,8 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<T> k, int n, int ch
 
 // w -= V h, |w|^2 after. With test_w >= 0 (the re-pass), only where sqrt(red[test_after]) < 0.7 sqrt(red[test_w]):
 // block 0 records the decision at red[flag], and a right-hand side without the re-pass leaves its vector alone.
-template <class T>
-__global__ __launch_bounds__(NTH) void kr_update(KrSetT<T> k, int n, int chunk, const double* red, double* part,
+template <class TV, class TW>
+__global__ __launch_bounds__(NTH) void kr_update(KrSetT<TV, TW> k, int n, int chunk, const double* red, double* part,
                                                  int P, int nblk, int slot, double* red_out, int test_w,
                                                  int test_after, int flag) {
     __shared__ double lds[NTH / 64];
@@ -192,8 +192,8 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<T> k, int n, int chunk, 
         if (blockIdx.x == 0 && threadIdx.x == 0) red_out[(size_t)p * P + flag] = take ? 1.0 : 0.0;
         if (!take) return;
     }
-    const cpx<T>* __restrict__ V = k.V[p];
-    cpx<T>* w = k.w[p];
+    const cpx<TV>* __restrict__ V = k.V[p];
+    cpx<TW>* w = k.w[p];
     const int m = k.m[p];
     const double* h = red + (size_t)p * P;
     for (int r = threadIdx.x; r < m; r += NTH) hs[r] = zd(h[2 * r], h[2 * r + 1]);
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<T> k, int n, int chunk, 
             acc.re -= v.re * hr.re - v.im * hr.im;
             acc.im -= v.re * hr.im + v.im * hr.re;
         }
-        const cpx<T> ws = stz<T>(acc.re, acc.im);
+        const cpx<TW> ws = stz<TW>(acc.re, acc.im);
         w[i] = ws;
         const zd a = ldz(ws);
         nn += a.re * a.re + a.im * a.im;
@@ -217,12 +217,12 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<T> k, int n, int chunk, 
     if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot] = s;
 }
 
-template <class T>
-__global__ __launch_bounds__(NTH) void kr_norm(KrSetT<T> k, int n, int chunk, double* part, int P, int nblk,
+template <class TV, class TW>
+__global__ __launch_bounds__(NTH) void kr_norm(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
                                                int slot) {
     __shared__ double lds[NTH / 64];
     const int p = blockIdx.y;
-    const cpx<T>* __restrict__ v = k.w[p];
+    const cpx<TW>* __restrict__ v = k.w[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double nn = 0;
     for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(NTH) void kr_reduce(const double* __restrict__ part
     if (threadIdx.x == 0) red[(size_t)p * P + c] = s;
 }
 
-template <class T>
-__global__ __launch_bounds__(NTH) void kr_scale(VecOpsT<T> v, int n) {
+template <class T, class TA>
+__global__ __launch_bounds__(NTH) void kr_scale(VecOpsT<T, TA> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
     if (i < n) {
         const zd a = ldz(v.a[p][i]);
@@ -273,8 +273,8 @@ __global__ __launch_bounds__(NTH) void kr_zero(VecOpsT<T> v, int n) {
     if (i < n) v.dst[p][i] = cpx<T>();
 }
 
-template <class T>
-__global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T> v, const zd* __restrict__ y, int rows, int n) {
+template <class T, class TA>
+__global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T, TA> v, const zd* __restrict__ y, int rows, int n) {
     __shared__ double ys_raw[1024];
     zd* ys = reinterpret_cast<zd*>(ys_raw);
     const int p = blockIdx.y, k = v.k[p];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T> v, const zd* __rest
     __syncthreads();
     const int i = blockIdx.x * NTH + threadIdx.x;
     if (i >= n) return;
-    const cpx<T>* V = v.a[p];
+    const cpx<TA>* V = v.a[p];
     zd acc = ldz(v.dst[p][i]);
     for (int r = 0; r < k; ++r) acc = acc + ldz(V[(size_t)r * n + i]) * ys[r];
     v.dst[p][i] = stz<T>(acc.re, acc.im);
@@ -315,12 +315,12 @@ void Krylov::init(int n_, int rows) {
     red.alloc((size_t)KMAX * P);
 }
 
-template <class T>
-void Krylov::dots(const KrSetT<T>& k, hipStream_t s) {
+template <class TV, class TW>
+void Krylov::dots(const KrSetT<TV, TW>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
     int mmax = 0;
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
-    hipLaunchKernelGGL(kr_dots<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_w());
+    hipLaunchKernelGGL((kr_dots<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_w());
     GT_LAUNCH("kr_dots");
     hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
     GT_LAUNCH("kr_reduce");
@@ -328,10 +328,10 @@ void Krylov::dots(const KrSetT<T>& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
-template <class T>
-void Krylov::project_update(const KrSetT<T>& k, hipStream_t s, bool second) {
+template <class TV, class TW>
+void Krylov::project_update(const KrSetT<TV, TW>& k, hipStream_t s, bool second) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_update<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
+    hipLaunchKernelGGL((kr_update<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
                        (const double*)(red.p + (second ? slot_h2() : 0)), part.p, P, nblk, slot_w_after(), red.p, -1,
                        -1, 0);
     GT_LAUNCH("kr_update");
@@ -339,21 +339,21 @@ void Krylov::project_update(const KrSetT<T>& k, hipStream_t s, bool second) {
     GT_LAUNCH("kr_reduce");
 }
 
-template <class T>
-void Krylov::update_dots(const KrSetT<T>& k, hipStream_t s) {
+template <class TV, class TW>
+void Krylov::update_dots(const KrSetT<TV, TW>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
     int mmax = 0;
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
     if (!fused(mmax)) fail(GT_ERR_ARG, "the fused Gram-Schmidt pass serves at most 128 basis rows");
     const double* rd = red.p;
     if (mmax <= 32)
-        hipLaunchKernelGGL((kr_update_dots<T, 4>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
+        hipLaunchKernelGGL((kr_update_dots<TV, TW, 4>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
                            slot_w_after(), slot_h2());
     else if (mmax <= 64)
-        hipLaunchKernelGGL((kr_update_dots<T, 8>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
+        hipLaunchKernelGGL((kr_update_dots<TV, TW, 8>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
                            slot_w_after(), slot_h2());
     else
-        hipLaunchKernelGGL((kr_update_dots<T, 16>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+        hipLaunchKernelGGL((kr_update_dots<TV, TW, 16>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
                            nblk, slot_w_after(), slot_h2());
     GT_LAUNCH("kr_update_dots");
     hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_h2(), red.p);
@@ -362,10 +362,10 @@ void Krylov::update_dots(const KrSetT<T>& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
-template <class T>
-void Krylov::repass(const KrSetT<T>& k, hipStream_t s) {
+template <class TV, class TW>
+void Krylov::repass(const KrSetT<TV, TW>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_update<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
+    hipLaunchKernelGGL((kr_update<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
                        (const double*)(red.p + slot_h2()), part.p, P, nblk, slot_w_after2(), red.p, slot_w(),
                        slot_w_after(), slot_repass());
     GT_LAUNCH("kr_update");
@@ -373,18 +373,18 @@ void Krylov::repass(const KrSetT<T>& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
-template <class T>
-void Krylov::norm2(const KrSetT<T>& k, hipStream_t s) {
+template <class TV, class TW>
+void Krylov::norm2(const KrSetT<TV, TW>& k, hipStream_t s) {
     const int chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(kr_norm<T>, dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
+    hipLaunchKernelGGL((kr_norm<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
     GT_LAUNCH("kr_norm");
     hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_norm(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
-template <class T>
-void k_scale(const VecOpsT<T>& v, int n, hipStream_t s) {
-    hipLaunchKernelGGL(kr_scale<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
+template <class T, class TA>
+void k_scale(const VecOpsT<T, TA>& v, int n, hipStream_t s) {
+    hipLaunchKernelGGL((kr_scale<T, TA>), dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n);
     GT_LAUNCH("kr_scale");
 }
 
@@ -406,11 +406,11 @@ void k_zero(const VecOpsT<T>& v, int n, hipStream_t s) {
     GT_LAUNCH("kr_zero");
 }
 
-template <class T>
-void k_combine(const VecOpsT<T>& v, const zd* y, int rows, int n, hipStream_t s) {
+template <class T, class TA>
+void k_combine(const VecOpsT<T, TA>& v, const zd* y, int rows, int n, hipStream_t s) {
     for (int p = 0; p < v.nr; ++p)
         if (v.k[p] > 512) fail(GT_ERR_ARG, "combination of more than 512 basis vectors");
-    hipLaunchKernelGGL(kr_combine<T>, dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n);
+    hipLaunchKernelGGL((kr_combine<T, TA>), dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n);
     GT_LAUNCH("kr_combine");
 }
 
@@ -434,19 +434,31 @@ void k_add_float(int nr, zd* const* dst, const cpx<float>* const* a, int n, hipS
     GT_LAUNCH("kr_add_float");
 }
 
-#define GT_KRYLOV(T)                                                                   \
-    template void Krylov::dots<T>(const KrSetT<T>&, hipStream_t);                    \
-    template void Krylov::project_update<T>(const KrSetT<T>&, hipStream_t, bool);    \
-    template void Krylov::update_dots<T>(const KrSetT<T>&, hipStream_t);             \
-    template void Krylov::repass<T>(const KrSetT<T>&, hipStream_t);                  \
-    template void Krylov::norm2<T>(const KrSetT<T>&, hipStream_t);                   \
-    template void k_scale<T>(const VecOpsT<T>&, int, hipStream_t);                   \
-    template void k_sub<T>(const VecOpsT<T>&, int, hipStream_t);                     \
-    template void k_copy<T>(const VecOpsT<T>&, int, hipStream_t);                    \
-    template void k_zero<T>(const VecOpsT<T>&, int, hipStream_t);                    \
-    template void k_combine<T>(const VecOpsT<T>&, const zd*, int, int, hipStream_t);
-GT_KRYLOV(double)
-GT_KRYLOV(float)
+#define GT_KRYLOV(TV, TW)                                                                         \
+    template void Krylov::dots<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);                      \
+    template void Krylov::project_update<TV, TW>(const KrSetT<TV, TW>&, hipStream_t, bool);      \
+    template void Krylov::update_dots<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);               \
+    template void Krylov::repass<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);                    \
+    template void Krylov::norm2<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);
+GT_KRYLOV(double, double)
+GT_KRYLOV(float, float)
+GT_KRYLOV(float, double)
 #undef GT_KRYLOV
+#define GT_VECOPS(T)                                                                              \
+    template void k_sub<T>(const VecOpsT<T>&, int, hipStream_t);                                 \
+    template void k_copy<T>(const VecOpsT<T>&, int, hipStream_t);                                \
+    template void k_zero<T>(const VecOpsT<T>&, int, hipStream_t);
+GT_VECOPS(double)
+GT_VECOPS(float)
+#undef GT_VECOPS
+// scale within one precision, and between them (a float basis row from a double vector and back)
+template void k_scale<double, double>(const VecOpsT<double, double>&, int, hipStream_t);
+template void k_scale<float, float>(const VecOpsT<float, float>&, int, hipStream_t);
+template void k_scale<float, double>(const VecOpsT<float, double>&, int, hipStream_t);
+template void k_scale<double, float>(const VecOpsT<double, float>&, int, hipStream_t);
+// combinations: a vector of either precision from a basis of its own, and a double vector from a float basis
+template void k_combine<double, double>(const VecOpsT<double, double>&, const zd*, int, int, hipStream_t);
+template void k_combine<float, float>(const VecOpsT<float, float>&, const zd*, int, int, hipStream_t);
+template void k_combine<double, float>(const VecOpsT<double, float>&, const zd*, int, int, hipStream_t);
 
 }  // namespace gt

@@ -9,14 +9,15 @@
 
 namespace gt {
 
-// The vectors of a batch: basis rows V_p[0:m_p] (row stride n) and the vector w_p each pass works on. T is the
-// storage type (double, or float for the mixed-precision inner solve); every product and sum is in double.
-template <class T>
+// The vectors of a batch: basis rows V_p[0:m_p] (row stride n) and the vector w_p each pass works on. TV and TW are
+// their storage types (double; float for the mixed-precision solves: the whole inner solve, or the basis alone);
+// every product and sum is in double.
+template <class TV, class TW = TV>
 struct KrSetT {
     int nr = 0;
-    const cpx<T>* V[KMAX];
+    const cpx<TV>* V[KMAX];
     int m[KMAX];
-    cpx<T>* w[KMAX];
+    cpx<TW>* w[KMAX];
 };
 using KrSet = KrSetT<double>;
 
@@ -36,39 +37,40 @@ struct Krylov {
     int slot_repass() const { return 2 * rows_cap + 4; }   // 1 where the re-pass was taken, else 0
     int slot_h2() const { return 2 * rows_cap + 8; }      // the re-pass coefficients V^H w' (2 rows_cap values)
     // h_p = V_p[0:m_p]^H w_p and |w_p|^2
-    template <class T>
-    void dots(const KrSetT<T>& k, hipStream_t s);
+    template <class TV, class TW>
+    void dots(const KrSetT<TV, TW>& k, hipStream_t s);
     // w_p -= V_p[0:m_p]^T h_p (h_p as reduced by dots, or the re-pass coefficients when `second`), |w_p|^2 after
-    template <class T>
-    void project_update(const KrSetT<T>& k, hipStream_t s, bool second = false);
+    template <class TV, class TW>
+    void project_update(const KrSetT<TV, TW>& k, hipStream_t s, bool second = false);
     // whether update_dots serves m rows
     static bool fused(int m) { return m <= 128; }
     // w_p -= V_p[0:m_p]^T h_p, |w_p|^2 after, and the re-pass coefficients V_p^H w_p of the updated vector: one
     // read of the basis for the update and the second classical Gram-Schmidt pass's products
-    template <class T>
-    void update_dots(const KrSetT<T>& k, hipStream_t s);
+    template <class TV, class TW>
+    void update_dots(const KrSetT<TV, TW>& k, hipStream_t s);
     // after update_dots: the second pass w_p -= V_p^T h2_p where |w'_p| < 0.7 |w_p| (the DGKS test, decided on the
     // device and recorded in slot_repass), |w_p|^2 after it in slot_w_after2
-    template <class T>
-    void repass(const KrSetT<T>& k, hipStream_t s);
+    template <class TV, class TW>
+    void repass(const KrSetT<TV, TW>& k, hipStream_t s);
     // |w_p|^2
-    template <class T>
-    void norm2(const KrSetT<T>& k, hipStream_t s);
+    template <class TV, class TW>
+    void norm2(const KrSetT<TV, TW>& k, hipStream_t s);
 };
 
-template <class T>
+// Per right-hand side: the destination (storage T) and the operands (storage TA; the same but in conversions)
+template <class T, class TA = T>
 struct VecOpsT {
     int nr = 0;
     cpx<T>* dst[KMAX];
-    const cpx<T>* a[KMAX];
-    const cpx<T>* b[KMAX];
+    const cpx<TA>* a[KMAX];
+    const cpx<TA>* b[KMAX];
     double f[KMAX];
     int k[KMAX];
 };
 using VecOps = VecOpsT<double>;
 
-template <class T>
-void k_scale(const VecOpsT<T>& v, int n, hipStream_t s);            // dst = f * a
+template <class T, class TA>
+void k_scale(const VecOpsT<T, TA>& v, int n, hipStream_t s);        // dst = f * a
 template <class T>
 void k_sub(const VecOpsT<T>& v, int n, hipStream_t s);              // dst = a - b
 template <class T>
@@ -76,8 +78,8 @@ void k_copy(const VecOpsT<T>& v, int n, hipStream_t s);             // dst = a
 template <class T>
 void k_zero(const VecOpsT<T>& v, int n, hipStream_t s);             // dst = 0
 // dst_p += sum_{r < k_p} a_p[r n : (r+1) n] * y[p rows + r]  (y on the device, `rows` values per right-hand side)
-template <class T>
-void k_combine(const VecOpsT<T>& v, const zd* y, int rows, int n, hipStream_t s);
+template <class T, class TA>
+void k_combine(const VecOpsT<T, TA>& v, const zd* y, int rows, int n, hipStream_t s);
 // precision changes of whole vectors: dst_p = (float) a_p, and dst_p += (double) a_p
 void k_to_float(int nr, cpx<float>* const* dst, const zd* const* a, int n, hipStream_t s);
 void k_add_float(int nr, zd* const* dst, const cpx<float>* const* a, int n, hipStream_t s);

@@ -538,8 +538,8 @@ class PlaneStack:
         device-resident vectors (torch complex128 tensors, their work finished), at most 16 per launch group. Each
         follows the reference's GMRES; the batch shares every launch. precision "mixed": FP64 iterative refinement
         to tol around FP32 inner solves to inner_tol (gt_gmres_batch_mixed). Returns one GmresResult per solve."""
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision is 'fp64' or 'mixed'")
+        if precision not in ("fp64", "mixed", "fp32-basis"):
+            raise ValueError("precision is 'fp64', 'mixed' or 'fp32-basis'")
         out = []
         for g0 in range(0, len(ops), BATCH):
             grp = slice(g0, g0 + BATCH)
@@ -553,6 +553,9 @@ class PlaneStack:
             res = (_lib.GmresResult * nr)()
             if precision == "fp64":
                 _lib.check(_lib.lib().gt_gmres_batch(self.handle, nr, H, B, X, C.byref(p), res), "gt_gmres_batch")
+            elif precision == "fp32-basis":
+                _lib.check(_lib.lib().gt_gmres_batch_fp32_basis(self.handle, nr, H, B, X, C.byref(p), res),
+                           "gt_gmres_batch_fp32_basis")
             else:
                 _lib.check(_lib.lib().gt_gmres_batch_mixed(self.handle, nr, H, B, X, C.byref(p), float(inner_tol),
                                                            int(max_refine), res), "gt_gmres_batch_mixed")

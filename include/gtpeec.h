@@ -249,6 +249,13 @@ int gt_gmres_batch(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* c
 int gt_gmres_batch_mixed(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
                          const gt_gmres_params* params, double inner_tol, int32_t max_refine,
                          gt_gmres_result* results);
+/* The same batch with the Krylov basis stored in FP32 and everything else FP64: the vectors w, x, r, the system
+ * products and the preconditioner, the Gram-Schmidt dot products against the widened basis, the Givens
+ * least-squares and the true-residual checks and restarts (each restart is a refinement step from the FP64
+ * residual). The basis, which every Arnoldi step reads two or three times, moves half the bytes; the iteration
+ * stays the reference's GMRES (peec.py:3093-3224) on a basis rounded to FP32. (No counterpart in the reference.) */
+int gt_gmres_batch_fp32_basis(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* const* b_dev,
+                              void* const* x_dev, const gt_gmres_params* params, gt_gmres_result* results);
 
 /* ---------------------------------------------------------------- serving sessions
  * Continuous batching of the solves of one stack (the device form of simulate.py:3964-4013 `run_parallel`, whose

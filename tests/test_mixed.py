@@ -1,7 +1,12 @@
-"""Mixed precision (the north star's first extension): FP64 iterative refinement around FP32 inner GMRES solves whose
-vectors, basis and operator products (the L_p FFT convolution, the incidence rows) run in FP32, with the Arnoldi
-dot products, the Givens least-squares, the Schur solve and every reduction in FP64. Bar: 1e-8 relative error on
-the strip currents and Z_loop against the FP64 oracle; a solve's bits the same alone and batched."""
+"""Mixed precision (the north star's first extension), two forms:
+  "mixed": FP64 iterative refinement around FP32 inner GMRES solves whose vectors, basis and operator products (the
+  L_p FFT convolution, the incidence rows) run in FP32, with the Arnoldi dot products, the Givens least-squares, the
+  Schur solve and every reduction in FP64;
+  "fp32-basis": one FP64 GMRES whose Krylov basis alone is stored in FP32.
+Bar: 1e-8 relative error on the strip currents and Z_loop against the FP64 oracle; the FP64 residual within the
+tolerance; a solve's bits the same alone and batched. The single GMRES stops at its FP64 true residual, as the plain
+method does, and a plain FP64 solve to 1e-10 is itself only some 1e-8 from the 1e-13 solution on these boards, so the
+FP32-basis form is asked for 1e-11 where the refinement form (whose last correction overshoots) is asked for 1e-10."""
 import numpy as np
 import pytest
 
@@ -19,8 +24,13 @@ def _currents(s):
     return np.concatenate([s.I_x.ravel(), s.I_y.ravel(), s.I_branch])
 
 
+PRECISIONS = ["mixed", "fp32-basis"]
+TOL_FOR_1E8 = {"mixed": 1e-10, "fp32-basis": 1e-11}
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("name", ["two_layer_20", "four_layer_24"])
-def test_mixed_precision_is_within_1e8_of_the_oracle(name):
+def test_mixed_precision_is_within_1e8_of_the_oracle(name, precision):
     b = (boards.scaled("two_layer", 20, n_vias=5, n_freqs=1) if name == "two_layer_20"
          else boards.scaled("four_layer", 24, n_nets=2, n_freqs=1, n_stitch=3))
     r = b.routes[0]
@@ -28,35 +38,51 @@ def test_mixed_precision_is_within_1e8_of_the_oracle(name):
     ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
     ox = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver)
     fs = [1e6, 1e8, 1e9]
-    many = ex.solve_many(fs, tol=1e-10, precision="mixed")
+    many = ex.solve_many(fs, tol=TOL_FOR_1E8[precision], precision=precision)
     for f, s in zip(fs, many):
         ref = ox.solve(f, tol=1e-13)
         assert rel(_currents(s), _currents(ref)) < 1e-8, f
         assert abs(s.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-8, f
 
 
-def test_mixed_precision_alone_and_batched_gives_the_same_bits():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mixed_precision_alone_and_batched_gives_the_same_bits(precision):
     b = boards.scaled("two_layer", 20, n_vias=5, n_freqs=1)
     r = b.routes[0]
     st = peec.PlaneStack(b.layers, b.branches)
     ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
     fs = [3e6, 2e8, 7e8]
-    many = ex.solve_many(fs, tol=1e-10, precision="mixed")
+    many = ex.solve_many(fs, tol=1e-10, precision=precision)
     for f, s in zip(fs, many):
-        one = ex.solve_many([f], tol=1e-10, precision="mixed")[0]
+        one = ex.solve_many([f], tol=1e-10, precision=precision)[0]
         assert np.array_equal(s.x, one.x) and s.iterations == one.iterations
 
 
-def test_mixed_precision_residual_meets_the_tolerance_in_fp64():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mixed_precision_residual_meets_the_tolerance_in_fp64(precision):
     b = boards.scaled("two_layer", 37, n_vias=7, n_freqs=1)
     r = b.routes[0]
     st = peec.PlaneStack(b.layers, b.branches)
     ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
     ref = O.PlaneStack(b.layers, b.branches)
     for f in (1e6, 5e8):
-        sol = ex.solve_many([f], tol=1e-10, precision="mixed")[0]
+        sol = ex.solve_many([f], tol=1e-10, precision=precision)[0]
         rhs_top, J = ex.rhs(f)
         rhs = np.concatenate([rhs_top, J[st.keep]])
         w, R, zt = ref._impedances(O.cache_value(f))
         Kx = O.HostMath(ref).matvec(O.Operator(w, R, zt, None, None), sol.x)
         assert np.linalg.norm(Kx - rhs) <= 2e-10 * np.linalg.norm(rhs)
+
+
+def test_fp32_basis_takes_about_the_fp64_iterations():
+    """The single-precision basis costs a bounded number of extra iterations over FP64's (the refinement form restarts
+    its inner solves and takes about twice as many)."""
+    b = boards.scaled("two_layer", 37, n_vias=7, n_freqs=1)
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    fs = [1e6, 1e8, 1e9]
+    d = ex.solve_many(fs, tol=1e-8)
+    f32 = ex.solve_many(fs, tol=1e-8, precision="fp32-basis")
+    for a, c in zip(f32, d):
+        assert a.iterations <= 1.5 * c.iterations + 4
