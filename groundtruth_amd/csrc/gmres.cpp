@@ -84,8 +84,8 @@ struct GmresWork {
     static int64_t plan_bytes(size_t n_, int rows_, int slots_) {
         const int64_t z = sizeof(zd);
         const int64_t P = 4 * (int64_t)rows_ + 8;
-        const int64_t nblk = std::min<int64_t>(2048, std::max<int64_t>(1, ((int64_t)n_ + 1023) / 1024));
-        return (int64_t)slots_ * ((int64_t)rows_ + 6) * (int64_t)n_ * z + (int64_t)KMAX * rows_ * z +
+        const int64_t nblk = Krylov::part_blocks((int)n_);
+        return (int64_t)slots_ * ((int64_t)rows_ * (int64_t)basis_ld(n_) + 6 * (int64_t)n_) * z + (int64_t)KMAX * rows_ * z +
                (int64_t)KMAX * nblk * P * 8 + (int64_t)KMAX * P * 8;
     }
     int64_t device_bytes() const {
@@ -107,7 +107,7 @@ struct GmresWork {
             ybuf.alloc((size_t)KMAX * rows);
         }
         for (; slots < nr; ++slots) {
-            V[slots].alloc((size_t)rows * n);
+            V[slots].alloc((size_t)rows * basis_ld(n));
             vec[slots].alloc(6 * n);
         }
         if (kr_n != n_ || kr_rows != rows_) {
@@ -132,6 +132,7 @@ struct Batch {
     gt_stack* st;
     hipStream_t s;
     int n, restart, cap;
+    size_t ld;                             // basis row stride
     std::vector<RhsT> R;
     GmresWork& gw;
     Krylov& kr;
@@ -141,7 +142,8 @@ struct Batch {
 
     // cap slots over the workspace gw, every one FREE
     Batch(gt_stack* st_, int cap_, int restart_, GmresWork& gw_)
-        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), cap(cap_), R(cap_), gw(gw_),
+        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), cap(cap_), ld(basis_ld(n)), R(cap_),
+          gw(gw_),
           kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
         gw.ensure((size_t)n, restart + 1, cap);
         for (int i = 0; i < cap; ++i) {
@@ -179,7 +181,7 @@ struct Batch {
         return false;
     }
 
-    cv* row(RhsT& q, int j) { return q.V.p + (size_t)j * n; }
+    cv* row(RhsT& q, int j) { return q.V.p + (size_t)j * ld; }
 
     void fetch(int cnt) {
         GT_HIP(hipMemcpyAsync(red.p, kr.red.p, sizeof(double) * kr.P * cnt, hipMemcpyDeviceToHost, s));
@@ -234,7 +236,7 @@ struct Batch {
             for (size_t r = 0; r < ys[i].size(); ++r) ybuf[(size_t)i * (restart + 1) + r] = ys[i][r];
         }
         GT_HIP(hipMemcpyAsync(ydev.p, ybuf.p, sizeof(zd) * c.nr * (restart + 1), hipMemcpyHostToDevice, s));
-        k_combine(c, ydev.p, restart + 1, n, s);
+        k_combine(c, ydev.p, restart + 1, n, (int)ld, s);
     }
 
     std::vector<cd> back_substitute(const RhsT& q, int k) const {
@@ -304,6 +306,7 @@ struct Batch {
     void project(const std::vector<int>& who, bool fuse) {
         KrSetT<TV, TW> k{};
         k.nr = (int)who.size();
+        k.ld = (int)ld;
         for (int i = 0; i < k.nr; ++i) {
             RhsT& q = R[who[i]];
             k.V[i] = q.V.p;

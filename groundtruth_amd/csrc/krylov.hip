@@ -34,126 +34,171 @@ __device__ __forceinline__ double block_sum(double val, double* lds) {
     return r;
 }
 
-// Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk: G basis rows per sweep over w, every row's sum
-// reduced within each wave by shuffles and across the four waves in LDS (two barriers a sweep).
+// Basis values a lane loads at once: one complex double (16 bytes), or two consecutive complex floats (one 16-byte
+// load). The basis rows have an even stride (a multiple of 64 elements), so a pair starting at an even element is
+// aligned; the pair past the end of an odd-length row reads the row's padding, which is replaced by zero.
+template <class TV>
+constexpr int lane_elems() { return sizeof(TV) == 4 ? 2 : 1; }
+
+template <class TV>
+__device__ __forceinline__ void ld_basis(const cpx<TV>* p, bool ok, bool ok2, zd (&v)[lane_elems<TV>()]);
+template <>
+__device__ __forceinline__ void ld_basis<double>(const zd* p, bool ok, bool, zd (&v)[1]) {
+    v[0] = ok ? *p : zd();
+}
+template <>
+__device__ __forceinline__ void ld_basis<float>(const cpx<float>* p, bool ok, bool ok2, zd (&v)[2]) {
+    float4 f = ok ? *reinterpret_cast<const float4*>(p) : float4{0.f, 0.f, 0.f, 0.f};
+    v[0] = zd(f.x, f.y);
+    v[1] = ok2 ? zd(f.z, f.w) : zd();
+}
+
+// Partial sums of h = V[0:m]^H w (and |w|^2) over one block's chunk of elements and one group of DG basis rows
+// (grid: chunks x row groups, right-hand sides): each lane loads its elements of the group's rows for DU element
+// steps at once, the sums meet by shuffles in each wave and across the four waves in LDS, in a fixed order.
 template <class TV, class TW>
 __global__ __launch_bounds__(NTH) void kr_dots(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
                                                int wslot) {
-    __shared__ double red[NTH / 64][2 * G + 1];
+    constexpr int E = lane_elems<TV>();
+    constexpr int DG = 4;
+    constexpr int DU = E == 1 ? 4 : 1;
+    __shared__ double red[NTH / 64][2 * DG + 1];
     const int p = blockIdx.y;
-    const cpx<TV>* __restrict__ V = k.V[p];
-    const cpx<TW>* __restrict__ w = k.w[p];
+    const int grp = blockIdx.x / nblk, blk = blockIdx.x - grp * nblk;
     const int m = k.m[p];
-    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    const int g0 = grp * DG;
+    if (g0 >= m) return;                        // the whole block: this right-hand side has fewer rows
+    const int gm = min(DG, m - g0);
+    const cpx<TV>* __restrict__ V = k.V[p] + (size_t)g0 * k.ld;
+    const cpx<TW>* __restrict__ w = k.w[p];
+    const int i0 = blk * chunk, i1 = min(n, i0 + chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double* out = part + ((size_t)p * nblk + blockIdx.x) * P;
-    for (int g0 = 0; g0 < m; g0 += G) {
-        const int gm = min(G, m - g0);
-        double ar[G], ai[G];
+    double ar[DG], ai[DG];
 #pragma unroll
-        for (int r = 0; r < G; ++r) ar[r] = ai[r] = 0;
-        double ww = 0;
-        for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
-            const zd wi = ldz(w[i]);
-            ww += wi.re * wi.re + wi.im * wi.im;
+    for (int r = 0; r < DG; ++r) ar[r] = ai[r] = 0;
+    double ww = 0;
+    for (int ib = i0 + E * threadIdx.x; ib < i1; ib += E * NTH * DU) {
+        zd wv[DU][E];
+        zd vv[DU][DG][E];
 #pragma unroll
-            for (int r = 0; r < G; ++r) {
-                if (r < gm) {
-                    const zd v = ldz(V[(size_t)(g0 + r) * n + i]);
-                    ar[r] += v.re * wi.re + v.im * wi.im;
-                    ai[r] += v.re * wi.im - v.im * wi.re;
+        for (int u = 0; u < DU; ++u) {
+            const int i = ib + u * E * NTH;
+            const bool in = i < i1, in2 = i + 1 < i1;
+#pragma unroll
+            for (int c = 0; c < E; ++c) wv[u][c] = (c == 0 ? in : in2) ? ldz(w[i + c]) : zd();
+#pragma unroll
+            for (int r = 0; r < DG; ++r) ld_basis<TV>(V + (size_t)r * k.ld + i, in && r < gm, in2, vv[u][r]);
+        }
+#pragma unroll
+        for (int u = 0; u < DU; ++u)
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                ww += wv[u][c].re * wv[u][c].re + wv[u][c].im * wv[u][c].im;
+#pragma unroll
+                for (int r = 0; r < DG; ++r) {
+                    ar[r] += vv[u][r][c].re * wv[u][c].re + vv[u][r][c].im * wv[u][c].im;
+                    ai[r] += vv[u][r][c].re * wv[u][c].im - vv[u][r][c].im * wv[u][c].re;
                 }
             }
-        }
-#pragma unroll
-        for (int r = 0; r < G; ++r) {
-            if (r < gm) {
-                ar[r] = wave_sum(ar[r]);
-                ai[r] = wave_sum(ai[r]);
-            }
-        }
-        ww = wave_sum(ww);
-        if (lane == 0) {
-#pragma unroll
-            for (int r = 0; r < G; ++r) {
-                red[wave][2 * r] = ar[r];
-                red[wave][2 * r + 1] = ai[r];
-            }
-            red[wave][2 * G] = ww;
-        }
-        __syncthreads();
-        const int t = threadIdx.x;
-        if (t < 2 * gm) out[2 * g0 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-        if (g0 == 0 && t == 2 * G) out[wslot] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-        __syncthreads();
     }
+#pragma unroll
+    for (int r = 0; r < DG; ++r) {
+        ar[r] = wave_sum(ar[r]);
+        ai[r] = wave_sum(ai[r]);
+    }
+    ww = wave_sum(ww);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < DG; ++r) {
+            red[wave][2 * r] = ar[r];
+            red[wave][2 * r + 1] = ai[r];
+        }
+        red[wave][2 * DG] = ww;
+    }
+    __syncthreads();
+    double* out = part + ((size_t)p * nblk + blk) * P;
+    const int t = threadIdx.x;
+    if (t < 2 * gm) out[2 * g0 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    if (grp == 0 && t == 2 * DG) out[wslot] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
 }
 
 // The update w' = w - V h fused with the products V^H w' (the second Gram-Schmidt pass's coefficients). Lane =
-// (element e = lane & 7, row group g = lane >> 3): a lane holds rows g, g + 8, .. (at most RPL) of its element's basis
-// column, so each basis value is loaded once for both; the row groups' parts of V h meet by butterfly sums, and the
-// products are summed over the block's elements per row.
-template <class TV, class TW, int RPL>
+// (element slot e = lane % EL, row group g = lane / EL): a lane holds rows g, g + NG, .. (at most RPL) of its E
+// elements' basis columns, so each basis value is loaded once for both; the row groups' parts of V h meet by
+// butterfly sums, and the products are summed over the block's elements per row. Launched once per class of m
+// (which fixes EL and RPL), its blocks of right-hand sides outside the class return at once: a right-hand side's
+// arithmetic depends on its own m only.
+template <class TV, class TW, int EL, int RPL>
 __global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<TV, TW> k, int n, int chunk, const double* red, double* part,
-                                                      int P, int nblk, int slot_after, int slot_h2) {
-    __shared__ double hs_raw[2 * 8 * RPL];
-    __shared__ double wsum[NTH / 64][2 * 8 * RPL + 1];
-    zd* hs = reinterpret_cast<zd*>(hs_raw);
+                                                      int P, int nblk, int slot_after, int slot_h2, int m_lo) {
+    constexpr int E = lane_elems<TV>();
+    constexpr int NG = 64 / EL;
+    constexpr int R = NG * RPL;
+    __shared__ double hs_raw[2 * R];
+    __shared__ double wsum[NTH / 64][2 * R + 1];
     const int p = blockIdx.y;
+    const int m = k.m[p];
+    if (m <= m_lo || m > R) return;
+    zd* hs = reinterpret_cast<zd*>(hs_raw);
     const cpx<TV>* __restrict__ V = k.V[p];
     cpx<TW>* w = k.w[p];
-    const int m = k.m[p];
     const double* h = red + (size_t)p * P;
-    for (int r = threadIdx.x; r < 8 * RPL; r += NTH) hs[r] = r < m ? zd(h[2 * r], h[2 * r + 1]) : zd();
+    for (int r = threadIdx.x; r < R; r += NTH) hs[r] = r < m ? zd(h[2 * r], h[2 * r + 1]) : zd();
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, e = lane & 7, g = lane >> 3;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, e = lane % EL, g = lane / EL;
     zd hr[RPL];
 #pragma unroll
-    for (int t = 0; t < RPL; ++t) hr[t] = hs[g + 8 * t];
+    for (int t = 0; t < RPL; ++t) hr[t] = hs[g + NG * t];
     double dre[RPL], dim[RPL];
 #pragma unroll
     for (int t = 0; t < RPL; ++t) dre[t] = dim[t] = 0.0;
     double nn = 0.0;
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-    for (int ib = i0; ib < i1; ib += 32) {            // 4 waves x 8 elements per step
-        const int i = ib + 8 * wave + e;
-        const bool in = i < i1;
-        zd vv[RPL];
+    for (int ib = i0; ib < i1; ib += 4 * EL * E) {
+        const int i = ib + E * (EL * wave + e);
+        const bool in = i < i1, in2 = i + 1 < i1;
+        zd vv[RPL][E];
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
-            const int r = g + 8 * t;
-            vv[t] = in && r < m ? ldz(V[(size_t)r * n + i]) : zd();
+            const int r = g + NG * t;
+            ld_basis<TV>(V + (size_t)r * k.ld + i, in && r < m, in2, vv[t]);
         }
-        const zd wi0 = in ? ldz(w[i]) : zd();
-        double sre = 0.0, sim = 0.0;
+        zd wi[E];
 #pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-            sre += vv[t].re * hr[t].re - vv[t].im * hr[t].im;
-            sim += vv[t].re * hr[t].im + vv[t].im * hr[t].re;
+        for (int c = 0; c < E; ++c) {
+            const zd w0 = (c == 0 ? in : in2) ? ldz(w[i + c]) : zd();
+            double sre = 0.0, sim = 0.0;
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                sre += vv[t][c].re * hr[t].re - vv[t][c].im * hr[t].im;
+                sim += vv[t][c].re * hr[t].im + vv[t][c].im * hr[t].re;
+            }
+#pragma unroll
+            for (int off = EL; off < 64; off <<= 1) {
+                sre += __shfl_xor(sre, off);
+                sim += __shfl_xor(sim, off);
+            }
+            // the stored value (rounded to TW) is what the products and the norm see
+            const cpx<TW> ws = stz<TW>(w0.re - sre, w0.im - sim);
+            wi[c] = ldz(ws);
+            if ((c == 0 ? in : in2) && g == 0) {
+                w[i + c] = ws;
+                nn += wi[c].re * wi[c].re + wi[c].im * wi[c].im;
+            }
         }
 #pragma unroll
-        for (int off = 8; off < 64; off <<= 1) {
-            sre += __shfl_xor(sre, off);
-            sim += __shfl_xor(sim, off);
-        }
-        // the stored value (rounded to T) is what the products and the norm see
-        const cpx<TW> ws = stz<TW>(wi0.re - sre, wi0.im - sim);
-        const zd wi = ldz(ws);
-        if (in && g == 0) {
-            w[i] = ws;
-            nn += wi.re * wi.re + wi.im * wi.im;
-        }
+        for (int t = 0; t < RPL; ++t)
 #pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-            dre[t] += vv[t].re * wi.re + vv[t].im * wi.im;
-            dim[t] += vv[t].re * wi.im - vv[t].im * wi.re;
-        }
+            for (int c = 0; c < E; ++c) {
+                dre[t] += vv[t][c].re * wi[c].re + vv[t][c].im * wi[c].im;
+                dim[t] += vv[t][c].re * wi[c].im - vv[t][c].im * wi[c].re;
+            }
     }
-    // the eight element lanes of each row group, then the four waves in order
+    // the EL element lanes of each row group, then the four waves in order
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
 #pragma unroll
-        for (int off = 1; off < 8; off <<= 1) {
+        for (int off = 1; off < EL; off <<= 1) {
             dre[t] += __shfl_xor(dre[t], off);
             dim[t] += __shfl_xor(dim[t], off);
         }
@@ -162,26 +207,31 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<TV, TW> k, int n, i
     if (e == 0) {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
-            wsum[wave][2 * (g + 8 * t)] = dre[t];
-            wsum[wave][2 * (g + 8 * t) + 1] = dim[t];
+            wsum[wave][2 * (g + NG * t)] = dre[t];
+            wsum[wave][2 * (g + NG * t) + 1] = dim[t];
         }
     }
-    if (lane == 0) wsum[wave][2 * 8 * RPL] = nn;
+    if (lane == 0) wsum[wave][2 * R] = nn;
     __syncthreads();
     double* out = part + ((size_t)p * nblk + blockIdx.x) * P;
-    for (int c = threadIdx.x; c <= 2 * 8 * RPL; c += NTH) {
+    for (int c = threadIdx.x; c <= 2 * R; c += NTH) {
         const double v = wsum[0][c] + wsum[1][c] + wsum[2][c] + wsum[3][c];
-        if (c == 2 * 8 * RPL) out[slot_after] = v;
+        if (c == 2 * R) out[slot_after] = v;
         else if (c < 2 * m) out[slot_h2 + c] = v;
     }
 }
 
 // w -= V h, |w|^2 after. With test_w >= 0 (the re-pass), only where sqrt(red[test_after]) < 0.7 sqrt(red[test_w]):
-// block 0 records the decision at red[flag], and a right-hand side without the re-pass leaves its vector alone.
+// block 0 records the decision at red[flag], and a right-hand side without the re-pass leaves its vector alone. A
+// lane takes E elements for UU element steps; the basis rows come UR at a time with their loads issued together,
+// and each element's sum runs over the rows in order.
 template <class TV, class TW>
 __global__ __launch_bounds__(NTH) void kr_update(KrSetT<TV, TW> k, int n, int chunk, const double* red, double* part,
                                                  int P, int nblk, int slot, double* red_out, int test_w,
                                                  int test_after, int flag) {
+    constexpr int E = lane_elems<TV>();
+    constexpr int UU = E == 1 ? 4 : 1;
+    constexpr int UR = E == 1 ? 4 : 8;
     __shared__ double lds[NTH / 64];
     __shared__ double hs_raw[1024];
     zd* hs = reinterpret_cast<zd*>(hs_raw);
@@ -200,18 +250,48 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<TV, TW> k, int n, int ch
     __syncthreads();
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double nn = 0;
-    for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
-        zd acc = ldz(w[i]);
-        for (int r = 0; r < m; ++r) {
-            const zd v = ldz(V[(size_t)r * n + i]);
-            const zd hr = hs[r];
-            acc.re -= v.re * hr.re - v.im * hr.im;
-            acc.im -= v.re * hr.im + v.im * hr.re;
+    for (int ib = i0 + E * threadIdx.x; ib < i1; ib += E * NTH * UU) {
+        zd acc[UU][E];
+#pragma unroll
+        for (int u = 0; u < UU; ++u)
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                const int i = ib + u * E * NTH + c;
+                acc[u][c] = i < i1 ? ldz(w[i]) : zd();
+            }
+        for (int r0 = 0; r0 < m; r0 += UR) {
+            zd vv[UU][UR][E];
+#pragma unroll
+            for (int u = 0; u < UU; ++u) {
+                const int i = ib + u * E * NTH;
+#pragma unroll
+                for (int r = 0; r < UR; ++r)
+                    ld_basis<TV>(V + (size_t)(r0 + r) * k.ld + i, i < i1 && r0 + r < m, i + 1 < i1, vv[u][r]);
+            }
+#pragma unroll
+            for (int r = 0; r < UR; ++r) {
+                const zd hr = r0 + r < m ? hs[r0 + r] : zd();
+#pragma unroll
+                for (int u = 0; u < UU; ++u)
+#pragma unroll
+                    for (int c = 0; c < E; ++c) {
+                        acc[u][c].re -= vv[u][r][c].re * hr.re - vv[u][r][c].im * hr.im;
+                        acc[u][c].im -= vv[u][r][c].re * hr.im + vv[u][r][c].im * hr.re;
+                    }
+            }
         }
-        const cpx<TW> ws = stz<TW>(acc.re, acc.im);
-        w[i] = ws;
-        const zd a = ldz(ws);
-        nn += a.re * a.re + a.im * a.im;
+#pragma unroll
+        for (int u = 0; u < UU; ++u)
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                const int i = ib + u * E * NTH + c;
+                if (i < i1) {
+                    const cpx<TW> ws = stz<TW>(acc[u][c].re, acc[u][c].im);
+                    w[i] = ws;
+                    const zd a = ldz(ws);
+                    nn += a.re * a.re + a.im * a.im;
+                }
+            }
     }
     const double s = block_sum(nn, lds);
     if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot] = s;
@@ -274,7 +354,7 @@ __global__ __launch_bounds__(NTH) void kr_zero(VecOpsT<T> v, int n) {
 }
 
 template <class T, class TA>
-__global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T, TA> v, const zd* __restrict__ y, int rows, int n) {
+__global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T, TA> v, const zd* __restrict__ y, int rows, int n, int ld) {
     __shared__ double ys_raw[1024];
     zd* ys = reinterpret_cast<zd*>(ys_raw);
     const int p = blockIdx.y, k = v.k[p];
@@ -284,7 +364,7 @@ __global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T, TA> v, const zd* __
     if (i >= n) return;
     const cpx<TA>* V = v.a[p];
     zd acc = ldz(v.dst[p][i]);
-    for (int r = 0; r < k; ++r) acc = acc + ldz(V[(size_t)r * n + i]) * ys[r];
+    for (int r = 0; r < k; ++r) acc = acc + ldz(V[(size_t)r * ld + i]) * ys[r];
     v.dst[p][i] = stz<T>(acc.re, acc.im);
 }
 
@@ -305,32 +385,49 @@ __global__ __launch_bounds__(NTH) void kr_add_float(Conv c, int n) {
 
 }  // namespace
 
+void Krylov::layout(int n, int& nblk, int& chunk, int& nblk_d, int& chunk_d) {
+    // chunks of whole pairs of elements: about 1024 elements per block for the passes that write w, 2048 for the
+    // multi-dot (whose blocks also split the rows); at most 2048 blocks of each
+    nblk = (int)std::min<long>(2048, std::max<long>(1, ((long)n + 1023) / 1024));
+    chunk = ((n + nblk - 1) / nblk + 1) & ~1;
+    nblk = std::max(1, (n + chunk - 1) / chunk);
+    nblk_d = (int)std::min<long>(2048, std::max<long>(1, ((long)n + 2047) / 2048));
+    chunk_d = ((n + nblk_d - 1) / nblk_d + 1) & ~1;
+    nblk_d = std::max(1, (n + chunk_d - 1) / chunk_d);
+}
+
+int Krylov::part_blocks(int n) {
+    int a, b, c, d;
+    layout(n, a, b, c, d);
+    return std::max(a, c);
+}
+
 void Krylov::init(int n_, int rows) {
     if (rows > 512) fail(GT_ERR_ARG, "GMRES restart above 511");
     n = n_;
     rows_cap = rows;
     P = 4 * rows + 8;
-    nblk = (int)std::min<long>(2048, std::max<long>(1, (n + 1023) / 1024));
-    part.alloc((size_t)KMAX * nblk * P);
+    layout(n, nblk, chunk, nblk_d, chunk_d);
+    part.alloc((size_t)KMAX * part_blocks(n) * P);
     red.alloc((size_t)KMAX * P);
 }
 
 template <class TV, class TW>
 void Krylov::dots(const KrSetT<TV, TW>& k, hipStream_t s) {
-    const int chunk = (n + nblk - 1) / nblk;
     int mmax = 0;
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
-    hipLaunchKernelGGL((kr_dots<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_w());
+    const int groups = (mmax + 3) / 4;
+    hipLaunchKernelGGL((kr_dots<TV, TW>), dim3(nblk_d * groups, k.nr), dim3(NTH), 0, s, k, n, chunk_d, part.p, P,
+                       nblk_d, slot_w());
     GT_LAUNCH("kr_dots");
-    hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk_d, P, 0, red.p);
     GT_LAUNCH("kr_reduce");
-    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w(), red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk_d, P, slot_w(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
 template <class TV, class TW>
 void Krylov::project_update(const KrSetT<TV, TW>& k, hipStream_t s, bool second) {
-    const int chunk = (n + nblk - 1) / nblk;
     hipLaunchKernelGGL((kr_update<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
                        (const double*)(red.p + (second ? slot_h2() : 0)), part.p, P, nblk, slot_w_after(), red.p, -1,
                        -1, 0);
@@ -339,22 +436,37 @@ void Krylov::project_update(const KrSetT<TV, TW>& k, hipStream_t s, bool second)
     GT_LAUNCH("kr_reduce");
 }
 
+// The fused pass's classes of m, each with its lane layout: a complex double basis takes 16 element lanes by 4 row
+// groups (up to 64 rows) and 8 by 8 above; a complex float basis 8 element lanes (pairs) by 8 row groups.
 template <class TV, class TW>
 void Krylov::update_dots(const KrSetT<TV, TW>& k, hipStream_t s) {
-    const int chunk = (n + nblk - 1) / nblk;
     int mmax = 0;
-    for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
+    bool cls[4] = {false, false, false, false};
+    for (int p = 0; p < k.nr; ++p) {
+        mmax = std::max(mmax, k.m[p]);
+        cls[k.m[p] <= 32 ? 0 : k.m[p] <= 64 ? 1 : 2] = true;
+    }
     if (!fused(mmax)) fail(GT_ERR_ARG, "the fused Gram-Schmidt pass serves at most 128 basis rows");
     const double* rd = red.p;
-    if (mmax <= 32)
-        hipLaunchKernelGGL((kr_update_dots<TV, TW, 4>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
-                           slot_w_after(), slot_h2());
-    else if (mmax <= 64)
-        hipLaunchKernelGGL((kr_update_dots<TV, TW, 8>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P, nblk,
-                           slot_w_after(), slot_h2());
-    else
-        hipLaunchKernelGGL((kr_update_dots<TV, TW, 16>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
-                           nblk, slot_w_after(), slot_h2());
+    const dim3 grid(nblk, k.nr);
+    if (sizeof(TV) == 8) {
+        if (cls[0])
+            hipLaunchKernelGGL((kr_update_dots<TV, TW, 16, 8>), grid, dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+                               nblk, slot_w_after(), slot_h2(), 0);
+        if (cls[1])
+            hipLaunchKernelGGL((kr_update_dots<TV, TW, 16, 16>), grid, dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+                               nblk, slot_w_after(), slot_h2(), 32);
+    } else {
+        if (cls[0])
+            hipLaunchKernelGGL((kr_update_dots<TV, TW, 8, 4>), grid, dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+                               nblk, slot_w_after(), slot_h2(), 0);
+        if (cls[1])
+            hipLaunchKernelGGL((kr_update_dots<TV, TW, 8, 8>), grid, dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+                               nblk, slot_w_after(), slot_h2(), 32);
+    }
+    if (cls[2])
+        hipLaunchKernelGGL((kr_update_dots<TV, TW, 8, 16>), grid, dim3(NTH), 0, s, k, n, chunk, rd, part.p, P,
+                           nblk, slot_w_after(), slot_h2(), 64);
     GT_LAUNCH("kr_update_dots");
     hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_h2(), red.p);
     GT_LAUNCH("kr_reduce");
@@ -364,7 +476,6 @@ void Krylov::update_dots(const KrSetT<TV, TW>& k, hipStream_t s) {
 
 template <class TV, class TW>
 void Krylov::repass(const KrSetT<TV, TW>& k, hipStream_t s) {
-    const int chunk = (n + nblk - 1) / nblk;
     hipLaunchKernelGGL((kr_update<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk,
                        (const double*)(red.p + slot_h2()), part.p, P, nblk, slot_w_after2(), red.p, slot_w(),
                        slot_w_after(), slot_repass());
@@ -375,7 +486,6 @@ void Krylov::repass(const KrSetT<TV, TW>& k, hipStream_t s) {
 
 template <class TV, class TW>
 void Krylov::norm2(const KrSetT<TV, TW>& k, hipStream_t s) {
-    const int chunk = (n + nblk - 1) / nblk;
     hipLaunchKernelGGL((kr_norm<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
     GT_LAUNCH("kr_norm");
     hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_norm(), red.p);
@@ -407,10 +517,10 @@ void k_zero(const VecOpsT<T>& v, int n, hipStream_t s) {
 }
 
 template <class T, class TA>
-void k_combine(const VecOpsT<T, TA>& v, const zd* y, int rows, int n, hipStream_t s) {
+void k_combine(const VecOpsT<T, TA>& v, const zd* y, int rows, int n, int ld, hipStream_t s) {
     for (int p = 0; p < v.nr; ++p)
         if (v.k[p] > 512) fail(GT_ERR_ARG, "combination of more than 512 basis vectors");
-    hipLaunchKernelGGL((kr_combine<T, TA>), dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n);
+    hipLaunchKernelGGL((kr_combine<T, TA>), dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, y, rows, n, ld);
     GT_LAUNCH("kr_combine");
 }
 
@@ -457,8 +567,8 @@ template void k_scale<float, float>(const VecOpsT<float, float>&, int, hipStream
 template void k_scale<float, double>(const VecOpsT<float, double>&, int, hipStream_t);
 template void k_scale<double, float>(const VecOpsT<double, float>&, int, hipStream_t);
 // combinations: a vector of either precision from a basis of its own, and a double vector from a float basis
-template void k_combine<double, double>(const VecOpsT<double, double>&, const zd*, int, int, hipStream_t);
-template void k_combine<float, float>(const VecOpsT<float, float>&, const zd*, int, int, hipStream_t);
-template void k_combine<double, float>(const VecOpsT<double, float>&, const zd*, int, int, hipStream_t);
+template void k_combine<double, double>(const VecOpsT<double, double>&, const zd*, int, int, int, hipStream_t);
+template void k_combine<float, float>(const VecOpsT<float, float>&, const zd*, int, int, int, hipStream_t);
+template void k_combine<double, float>(const VecOpsT<double, float>&, const zd*, int, int, int, hipStream_t);
 
 }  // namespace gt

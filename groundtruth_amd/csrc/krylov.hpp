@@ -15,6 +15,7 @@ namespace gt {
 template <class TV, class TW = TV>
 struct KrSetT {
     int nr = 0;
+    int ld = 0;               // row stride of the bases (elements; a multiple of 64)
     const cpx<TV>* V[KMAX];
     int m[KMAX];
     cpx<TW>* w[KMAX];
@@ -23,13 +24,17 @@ using KrSet = KrSetT<double>;
 
 struct Krylov {
     int n = 0;
-    int nblk = 0;             // partial-sum blocks of the reductions
+    int nblk = 0, chunk = 0;     // partial-sum blocks of the passes over w and their elements (even)
+    int nblk_d = 0, chunk_d = 0; // the same for the multi-dot
     int rows_cap = 0;
     int P = 0;                // values per (right-hand side, block): 2 rows_cap (h) + 8 (norms) + 2 rows_cap (h2)
     DevBuf<double> part;      // KMAX x nblk x P partial sums
     DevBuf<double> red;       // KMAX x P reduced values
 
     void init(int n, int rows);
+    // the blocks and chunks of the passes over vectors of n, and the partial-sum blocks the buffers hold
+    static void layout(int n, int& nblk, int& chunk, int& nblk_d, int& chunk_d);
+    static int part_blocks(int n);
     int slot_w() const { return 2 * rows_cap; }          // |w|^2 before the projection
     int slot_w_after() const { return 2 * rows_cap + 1; } // |w|^2 after it
     int slot_norm() const { return 2 * rows_cap + 2; }    // norm2's value
@@ -77,9 +82,12 @@ template <class T>
 void k_copy(const VecOpsT<T>& v, int n, hipStream_t s);             // dst = a
 template <class T>
 void k_zero(const VecOpsT<T>& v, int n, hipStream_t s);             // dst = 0
-// dst_p += sum_{r < k_p} a_p[r n : (r+1) n] * y[p rows + r]  (y on the device, `rows` values per right-hand side)
+// dst_p += sum_{r < k_p} a_p[r ld : r ld + n] * y[p rows + r]  (y on the device, `rows` values per right-hand side;
+// ld the basis row stride)
 template <class T, class TA>
-void k_combine(const VecOpsT<T, TA>& v, const zd* y, int rows, int n, hipStream_t s);
+void k_combine(const VecOpsT<T, TA>& v, const zd* y, int rows, int n, int ld, hipStream_t s);
+// the row stride of a basis over vectors of n: whole 64-element lines, so that rows start aligned
+inline size_t basis_ld(size_t n) { return (n + 63) / 64 * 64; }
 // precision changes of whole vectors: dst_p = (float) a_p, and dst_p += (double) a_p
 void k_to_float(int nr, cpx<float>* const* dst, const zd* const* a, int n, hipStream_t s);
 void k_add_float(int nr, zd* const* dst, const cpx<float>* const* a, int n, hipStream_t s);
