@@ -100,6 +100,8 @@ SIGNATURES = {
     "gt_session_poll": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(GmresResult), i32p]),
     "gt_session_device_bytes": (C.c_int64, [C.c_void_p]),
     "gt_session_plan_bytes": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32]),
+    "gt_session_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gt_session_open_bytes": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
 }
 
 _LIB = None

@@ -13,8 +13,8 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
                        const gt_gmres_params& p, double inner_tol, int max_refine, gt_gmres_result* res);
 void gmres_batch_fp32_basis(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
                             const gt_gmres_params& p, gt_gmres_result* res);
-int64_t session_plan_bytes(const gt_stack* st, int cap, int restart);
-gt_session* session_create(gt_stack* st, int cap, int restart);
+int64_t session_plan_bytes(const gt_stack* st, int cap, int restart, int basis, int max_extra);
+gt_session* session_create(gt_stack* st, int cap, int restart, int basis, int max_extra);
 int64_t session_bytes(const gt_session* ss);
 int session_submit(gt_session* ss, gt_op* op, const zd* b, zd* x, const gt_gmres_params& p);
 int session_step(gt_session* ss, int max_steps, int* ready);
@@ -179,7 +179,6 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
     }
     a.S = S;
     a.mode = mode;
-    a.nb = nb;
     a.t_stride = t_elems;
     a.a_keep = a_keep.p;
     a.b_keep = b_keep.p;
@@ -559,6 +558,8 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
         GT_HIP(hipSetDevice(base->ctx->device));
         std::unique_ptr<gt_stack_s> st(new gt_stack_s(base->core));
         st->derived = true;
+        st->lineage.push_back(base->id);
+        st->lineage.insert(st->lineage.end(), base->lineage.begin(), base->lineage.end());
         st->ctx = base->ctx;
         st->S = base->S;
         st->nx = base->nx;
@@ -814,15 +815,20 @@ int gt_gmres_batch_mixed(gt_stack* st, int32_t nr, gt_op* const* ops, const void
     });
 }
 
-int gt_session_create(gt_stack* st, int32_t capacity, int32_t restart, gt_session** out) {
+int gt_session_open(gt_stack* st, int32_t capacity, int32_t restart, int32_t basis, int32_t max_extra,
+                    gt_session** out) {
     return guard([&] {
         need(st && out, "NULL argument");
         std::lock_guard<std::mutex> lk(st->ctx->mu);
         *out = nullptr;
         need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
         GT_HIP(hipSetDevice(st->ctx->device));
-        *out = session_create(st, capacity, restart);
+        *out = session_create(st, capacity, restart, basis, max_extra);
     });
+}
+
+int gt_session_create(gt_stack* st, int32_t capacity, int32_t restart, gt_session** out) {
+    return gt_session_open(st, capacity, restart, GT_BASIS_FP64, 0, out);
 }
 
 void gt_session_destroy(gt_session* ss) {
@@ -867,9 +873,19 @@ int gt_session_poll(gt_session* ss, int32_t slot, gt_gmres_result* res, int32_t*
 
 int64_t gt_session_device_bytes(const gt_session* ss) { return ss ? session_bytes(ss) : 0; }
 
+int64_t gt_session_open_bytes(const gt_stack* st, int32_t capacity, int32_t restart, int32_t basis,
+                              int32_t max_extra) {
+    if (!st || capacity < 1 || capacity > KMAX || restart < 0 || max_extra < 0) return -1;
+    if (basis != GT_BASIS_FP64 && basis != GT_BASIS_FP32) return -1;
+    try {
+        return session_plan_bytes(st, capacity, restart, basis, max_extra);
+    } catch (...) {
+        return -1;
+    }
+}
+
 int64_t gt_session_plan_bytes(const gt_stack* st, int32_t capacity, int32_t restart) {
-    if (!st || capacity < 1 || capacity > KMAX || restart < 0) return -1;
-    return session_plan_bytes(st, capacity, restart);
+    return gt_session_open_bytes(st, capacity, restart, GT_BASIS_FP64, 0);
 }
 
 }  // extern "C"

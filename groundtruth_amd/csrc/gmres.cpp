@@ -60,6 +60,7 @@ struct GmresWork {
     size_t n = 0;                          // vector length the slots hold
     int rows = 0;                          // basis rows the slots hold
     int slots = 0;                         // right-hand-side slots allocated at that size
+    int vbytes = 0;                        // bytes per stored basis value the slots hold (16 or 8)
     DevBuf<zd> V[KMAX], vec[KMAX];         // per slot: the basis, and six vectors of n
     Krylov kr;
     int kr_rows = 0;
@@ -80,12 +81,17 @@ struct GmresWork {
         for (; ref_slots < nr; ++ref_slots) ref[ref_slots].alloc(4 * ref_n);
     }
 
+    // complex doubles of one slot's basis: rows of the padded stride, vbytes per stored value (16 or 8)
+    static size_t basis_count(size_t n_, int rows_, int vbytes_) {
+        return (size_t)rows_ * basis_ld(n_) * (size_t)vbytes_ / sizeof(zd);
+    }
     // device bytes of a workspace of `slots` slots of `rows` basis rows over vectors of n
-    static int64_t plan_bytes(size_t n_, int rows_, int slots_) {
+    static int64_t plan_bytes(size_t n_, int rows_, int slots_, int vbytes_ = sizeof(zd)) {
         const int64_t z = sizeof(zd);
         const int64_t P = 4 * (int64_t)rows_ + 8;
         const int64_t nblk = Krylov::part_blocks((int)n_);
-        return (int64_t)slots_ * ((int64_t)rows_ * (int64_t)basis_ld(n_) + 6 * (int64_t)n_) * z + (int64_t)KMAX * rows_ * z +
+        return (int64_t)slots_ * ((int64_t)basis_count(n_, rows_, vbytes_) + 6 * (int64_t)n_) * z +
+               (int64_t)KMAX * rows_ * z +
                (int64_t)KMAX * nblk * P * 8 + (int64_t)KMAX * P * 8;
     }
     int64_t device_bytes() const {
@@ -94,8 +100,8 @@ struct GmresWork {
         return b;
     }
 
-    void ensure(size_t n_, int rows_, int nr) {
-        if (n_ > n || rows_ > rows) {
+    void ensure(size_t n_, int rows_, int nr, int vbytes_ = sizeof(zd)) {
+        if (n_ > n || rows_ > rows || vbytes_ > vbytes) {
             for (int i = 0; i < slots; ++i) {
                 V[i].release();
                 vec[i].release();
@@ -103,11 +109,12 @@ struct GmresWork {
             slots = 0;
             n = std::max(n, n_);
             rows = std::max(rows, rows_);
+            vbytes = std::max(vbytes, vbytes_);
             ydev.alloc((size_t)KMAX * rows);
             ybuf.alloc((size_t)KMAX * rows);
         }
         for (; slots < nr; ++slots) {
-            V[slots].alloc((size_t)rows * basis_ld(n));
+            V[slots].alloc(basis_count(n, rows, vbytes));
             vec[slots].alloc(6 * n);
         }
         if (kr_n != n_ || kr_rows != rows_) {
@@ -140,12 +147,13 @@ struct Batch {
     HostBuf<cd>& ybuf;
     HostBuf<double>& red;
 
-    // cap slots over the workspace gw, every one FREE
-    Batch(gt_stack* st_, int cap_, int restart_, GmresWork& gw_)
-        : st(st_), s(st_->ctx->stream), n(st_->nb + st_->nk), restart(restart_), cap(cap_), ld(basis_ld(n)), R(cap_),
-          gw(gw_),
+    // cap slots over the workspace gw, every one FREE, for vectors of up to n_max (0: the host stack's): the slots may
+    // hold solves of what-if stacks derived from st, each over its own length
+    Batch(gt_stack* st_, int cap_, int restart_, GmresWork& gw_, int n_max = 0)
+        : st(st_), s(st_->ctx->stream), n(std::max(n_max, st_->nb + st_->nk)), restart(restart_), cap(cap_),
+          ld(basis_ld(n)), R(cap_), gw(gw_),
           kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
-        gw.ensure((size_t)n, restart + 1, cap);
+        gw.ensure((size_t)n, restart + 1, cap, (int)sizeof(cv));
         for (int i = 0; i < cap; ++i) {
             RhsT& q = R[i];
             q.V.p = reinterpret_cast<cv*>(gw.V[i].p);
@@ -182,6 +190,8 @@ struct Batch {
     }
 
     cv* row(RhsT& q, int j) { return q.V.p + (size_t)j * ld; }
+    // a slot's vector length: its operator's stack's unknowns
+    int len(int slot) const { return R[slot].op->st->nb + R[slot].op->st->nk; }
 
     void fetch(int cnt) {
         GT_HIP(hipMemcpyAsync(red.p, kr.red.p, sizeof(double) * kr.P * cnt, hipMemcpyDeviceToHost, s));
@@ -207,7 +217,10 @@ struct Batch {
     void norms(const std::vector<int>& who, const std::vector<const cz*>& v, std::vector<double>& out) {
         KrSetT<TV, TW> k{};
         k.nr = (int)who.size();
-        for (int i = 0; i < k.nr; ++i) k.w[i] = const_cast<cz*>(v[i]);
+        for (int i = 0; i < k.nr; ++i) {
+            k.w[i] = const_cast<cz*>(v[i]);
+            k.n[i] = len(who[i]);
+        }
         kr.norm2(k, s);
         fetch(k.nr);
         out.resize(who.size());
@@ -223,12 +236,16 @@ struct Batch {
         for (int i = 0; i < cp.nr; ++i) {
             cp.dst[i] = dst[i];
             cp.a[i] = from[i];
+            cp.n[i] = len(who[i]);
             if (from[i] != dst[i]) any = true;
         }
         if (any) k_copy(cp, n, s);
         VecOpsT<TW, TV> c{};
         c.nr = cp.nr;
-        for (int i = 0; i < c.nr; ++i) c.dst[i] = dst[i];
+        for (int i = 0; i < c.nr; ++i) {
+            c.dst[i] = dst[i];
+            c.n[i] = len(who[i]);
+        }
         spin_sync(s);                        // the previous copy out of ybuf has finished
         for (int i = 0; i < c.nr; ++i) {
             c.a[i] = R[who[i]].V.p;
@@ -262,6 +279,7 @@ struct Batch {
             v.dst[i] = R[who[i]].r.p;
             v.a[i] = R[who[i]].b;
             v.b[i] = R[who[i]].u.p;
+            v.n[i] = len(who[i]);
         }
         k_sub(v, n, s);
     }
@@ -278,7 +296,10 @@ struct Batch {
         if (P.empty()) return;
         VecOpsT<TW> v{};
         v.nr = (int)P.size();
-        for (int i = 0; i < v.nr; ++i) v.dst[i] = R[P[i]].x.p;
+        for (int i = 0; i < v.nr; ++i) {
+            v.dst[i] = R[P[i]].x.p;
+            v.n[i] = len(P[i]);
+        }
         k_zero(v, n, s);
         for (int i = 0; i < v.nr; ++i) {
             v.dst[i] = R[P[i]].r.p;
@@ -312,6 +333,7 @@ struct Batch {
             k.V[i] = q.V.p;
             k.m[i] = q.j + 1;
             k.w[i] = q.w.p;
+            k.n[i] = len(who[i]);
         }
         kr.dots(k, s);
         if (fuse) {
@@ -361,6 +383,7 @@ struct Batch {
                 v.dst[v.nr] = row(q, 0);
                 v.a[v.nr] = q.r.p;
                 v.f[v.nr] = 1.0 / b;
+                v.n[v.nr] = len(S[i]);
                 v.nr++;
             }
             if (v.nr) k_scale(v, n, s);
@@ -380,6 +403,7 @@ struct Batch {
                 for (int q : A) {
                     cv.dst[cv.nr] = R[q].u.p;
                     cv.a[cv.nr] = row(R[q], R[q].j);
+                    cv.n[cv.nr] = len(q);
                     cv.f[cv.nr++] = 1.0;
                     in.push_back(R[q].u.p);
                 }
@@ -487,6 +511,7 @@ struct Batch {
                 q.resid = rt[i] / q.bnorm;
                 if (rt[i] <= q.p.tol * q.bnorm) {
                     cp.dst[cp.nr] = q.x.p;
+                    cp.n[cp.nr] = len(C[i]);
                     cp.a[cp.nr++] = q.xt.p;
                     q.state = DONE;
                     q.conv = true;
@@ -495,6 +520,7 @@ struct Batch {
                 q.misses++;
                 if (q.misses >= (q.p.check_misses > 0 ? q.p.check_misses : 4)) {     // restart from the check: x = xt, r = rt
                     cp.dst[cp.nr] = q.x.p;
+                    cp.n[cp.nr] = len(C[i]);
                     cp.a[cp.nr++] = q.xt.p;
                     q.state = START;
                     continue;
@@ -511,6 +537,7 @@ struct Batch {
                 RhsT& q = R[qi];
                 v.dst[v.nr] = row(q, q.j + 1);
                 v.a[v.nr] = q.w.p;
+                v.n[v.nr] = len(qi);
                 v.f[v.nr++] = 1.0 / q.hn;
             }
             k_scale(v, n, s);
@@ -557,14 +584,13 @@ namespace {
 template <class TV>
 void gmres_batch_t(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
                    const gt_gmres_params& p, gt_gmres_result* res) {
-    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
+    check_batch(st, nr, ops);
     const int n = st->nb + st->nk;
+    int n_max = n;
+    for (int i = 0; i < nr; ++i) n_max = std::max(n_max, ops[i]->st->nb + ops[i]->st->nk);
     if (!st->ctx->gmres) st->ctx->gmres.reset(new GmresWork);
-    Batch<TV, double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), *st->ctx->gmres);
-    for (int i = 0; i < nr; ++i) {
-        if (ops[i]->st != st) fail(GT_ERR_ARG, "a batch's operators must share one stack");
-        bt.submit(i, ops[i], b[i], p);
-    }
+    Batch<TV, double> bt(st, nr, p.restart > 0 ? p.restart : default_restart(n), *st->ctx->gmres, n_max);
+    for (int i = 0; i < nr; ++i) bt.submit(i, ops[i], b[i], p);
     bt.run();
     // x = M^-1 y (zero when y is)
     std::vector<const zd*> ys;
@@ -741,102 +767,126 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
 // A serving session: up to 16 slots over one stack and a GMRES workspace of its own, so sessions of several stacks
 // interleave on one context. A solve enters a free slot at any time and joins the running batch at its next step;
 // every slot runs the same state machine as in a batch, so its result is bit-identical to the same solve alone.
+// The slots may hold solves of the stack itself and of what-if stacks derived from it with up to max_extra added
+// branches: one batch for a board and its what-ifs. The basis is stored in FP64 or FP32.
 struct gt_session_s {
     gt_stack* st = nullptr;
-    int cap = 0, restart = 0;
+    int cap = 0, restart = 0, n_max = 0;
     gt::GmresWork gw;
     std::unique_ptr<gt::Batch<double, double>> bt;
+    std::unique_ptr<gt::Batch<float, double>> bt32;
     std::vector<gt::zd*> xout;             // the caller's solution vector of each slot
 };
 
 namespace gt {
 
-int64_t session_plan_bytes(const gt_stack* st, int cap, int restart) {
-    const int n = st->nb + st->nk;
-    const int rows = (restart > 0 ? restart : default_restart(n)) + 1;
-    int64_t b = GmresWork::plan_bytes((size_t)n, rows, cap);
-    if (cap > st->work.cap) {              // the stack's product scratch grows to the session's batch
-        const int64_t per = 2 * (int64_t)std::max<size_t>(1, st->t_elems) + 2 * (int64_t)std::max(1, st->nk) +
-                            std::max(1, st->nbr);
-        b += per * (cap - st->work.cap) * (int64_t)sizeof(zd);
-    }
-    return b;
+namespace {
+
+template <class F>
+auto on_batch(gt_session* ss, F&& f) {
+    if (ss->bt32) return f(*ss->bt32);
+    return f(*ss->bt);
 }
 
-gt_session* session_create(gt_stack* st, int cap, int restart) {
+int basis_bytes(int basis) {
+    if (basis != GT_BASIS_FP64 && basis != GT_BASIS_FP32) fail(GT_ERR_ARG, "basis is GT_BASIS_FP64 or GT_BASIS_FP32");
+    return basis == GT_BASIS_FP32 ? (int)sizeof(zf) : (int)sizeof(zd);
+}
+
+}  // namespace
+
+int64_t session_plan_bytes(const gt_stack* st, int cap, int restart, int basis, int max_extra) {
+    if (max_extra < 0) fail(GT_ERR_ARG, "negative max_extra");
+    const int n = st->nb + max_extra + st->nk;
+    const int rows = (restart > 0 ? restart : default_restart(n)) + 1;
+    int64_t b = GmresWork::plan_bytes((size_t)n, rows, cap, basis_bytes(basis));
+    // the stack's product scratch grows to the session's batch and Woodbury columns
+    return b + st->work.bytes_after(st, cap, st->k + max_extra) - st->work.bytes();
+}
+
+gt_session* session_create(gt_stack* st, int cap, int restart, int basis, int max_extra) {
     if (cap < 1 || cap > KMAX) fail(GT_ERR_ARG, "1 to 16 slots per session");
     if (restart < 0) fail(GT_ERR_ARG, "negative restart");
-    const int n = st->nb + st->nk;
+    if (max_extra < 0) fail(GT_ERR_ARG, "negative max_extra");
+    const int vb = basis_bytes(basis);
     std::unique_ptr<gt_session_s> ss(new gt_session_s);
     ss->st = st;
     ss->cap = cap;
-    ss->restart = restart > 0 ? restart : default_restart(n);
-    ss->bt.reset(new Batch<double, double>(st, cap, ss->restart, ss->gw));
+    ss->n_max = st->nb + max_extra + st->nk;
+    ss->restart = restart > 0 ? restart : default_restart(ss->n_max);
+    if (vb == (int)sizeof(zf)) ss->bt32.reset(new Batch<float, double>(st, cap, ss->restart, ss->gw, ss->n_max));
+    else ss->bt.reset(new Batch<double, double>(st, cap, ss->restart, ss->gw, ss->n_max));
     ss->xout.assign(cap, nullptr);
-    st->work.ensure(st, cap);
+    st->work.ensure(st, cap, st->k + max_extra);
     return ss.release();
 }
 
 int64_t session_bytes(const gt_session* ss) { return ss->gw.device_bytes(); }
 
 int session_submit(gt_session* ss, gt_op* op, const zd* b, zd* x, const gt_gmres_params& p) {
-    if (op->st != ss->st) fail(GT_ERR_ARG, "the operator belongs to another stack");
+    check_batch(ss->st, 1, &op);
+    if (op->st->nb + op->st->nk > ss->n_max)
+        fail(GT_ERR_ARG, "the what-if adds more branches than the session holds");
     if (p.restart > 0 && p.restart != ss->restart) fail(GT_ERR_ARG, "the session's restart differs");
-    Batch<double, double>& bt = *ss->bt;
-    for (int i = 0; i < ss->cap; ++i) {
-        if (bt.R[i].state != FREE) continue;
-        bt.submit(i, op, b, p);
-        ss->xout[i] = x;
-        return i;
-    }
-    return -1;
+    return on_batch(ss, [&](auto& bt) {
+        for (int i = 0; i < ss->cap; ++i) {
+            if (bt.R[i].state != FREE) continue;
+            bt.submit(i, op, b, p);
+            ss->xout[i] = x;
+            return i;
+        }
+        return -1;
+    });
 }
 
 // Up to max_steps Arnoldi steps of the running slots (the newly submitted set up first); the solves that stopped
 // get x = M^-1 y and wait, READY, for their poll. Returns the slots still running.
 int session_step(gt_session* ss, int max_steps, int* ready) {
-    Batch<double, double>& bt = *ss->bt;
-    bt.init();
-    for (int k = 0; k < max_steps; ++k)
-        if (!bt.step()) break;
-    std::vector<int> fin;
-    for (int i = 0; i < ss->cap; ++i)
-        if (bt.R[i].state == DONE) fin.push_back(i);
-    if (!fin.empty()) {
-        const int c = (int)fin.size();
-        std::vector<gt_op*> ops(c);
-        std::vector<const zd*> ys(c);
-        std::vector<zd*> xs(c);
-        for (int i = 0; i < c; ++i) {
-            ops[i] = bt.R[fin[i]].op;
-            ys[i] = bt.R[fin[i]].x.p;
-            xs[i] = ss->xout[fin[i]];
+    return on_batch(ss, [&](auto& bt) {
+        bt.init();
+        for (int k = 0; k < max_steps; ++k)
+            if (!bt.step()) break;
+        std::vector<int> fin;
+        for (int i = 0; i < ss->cap; ++i)
+            if (bt.R[i].state == DONE) fin.push_back(i);
+        if (!fin.empty()) {
+            const int c = (int)fin.size();
+            std::vector<gt_op*> ops(c);
+            std::vector<const zd*> ys(c);
+            std::vector<zd*> xs(c);
+            for (int i = 0; i < c; ++i) {
+                ops[i] = bt.R[fin[i]].op;
+                ys[i] = bt.R[fin[i]].x.p;
+                xs[i] = ss->xout[fin[i]];
+            }
+            ops_precond<double>(ss->st, c, ops.data(), ys.data(), xs.data());
+            spin_sync(bt.s);
+            for (int i : fin) bt.R[i].state = READY;
         }
-        ops_precond<double>(ss->st, c, ops.data(), ys.data(), xs.data());
-        spin_sync(bt.s);
-        for (int i : fin) bt.R[i].state = READY;
-    }
-    int run = 0, rd = 0;
-    for (int i = 0; i < ss->cap; ++i) {
-        const State q = bt.R[i].state;
-        if (q == READY) rd++;
-        else if (q != FREE) run++;
-    }
-    if (ready) *ready = rd;
-    return run;
+        int run = 0, rd = 0;
+        for (int i = 0; i < ss->cap; ++i) {
+            const State q = bt.R[i].state;
+            if (q == READY) rd++;
+            else if (q != FREE) run++;
+        }
+        if (ready) *ready = rd;
+        return run;
+    });
 }
 
 bool session_poll(gt_session* ss, int slot, gt_gmres_result* res) {
     if (slot < 0 || slot >= ss->cap) fail(GT_ERR_ARG, "slot out of range");
-    Rhs<double, double>& q = ss->bt->R[slot];
-    if (q.state != READY) return false;
-    res->iterations = q.its;
-    res->converged = q.conv ? 1 : 0;
-    res->products = q.products;
-    res->residual = q.resid;
-    q.state = FREE;
-    ss->xout[slot] = nullptr;
-    return true;
+    return on_batch(ss, [&](auto& bt) {
+        auto& q = bt.R[slot];
+        if (q.state != READY) return false;
+        res->iterations = q.its;
+        res->converged = q.conv ? 1 : 0;
+        res->products = q.products;
+        res->residual = q.resid;
+        q.state = FREE;
+        ss->xout[slot] = nullptr;
+        return true;
+    });
 }
 
 void session_destroy(gt_session* ss) { delete ss; }

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(NTH) void kr_dots(KrSetT<TV, TW> k, int n, int chun
     const int gm = min(DG, m - g0);
     const cpx<TV>* __restrict__ V = k.V[p] + (size_t)g0 * k.ld;
     const cpx<TW>* __restrict__ w = k.w[p];
+    if (k.n[p] > 0) n = k.n[p];
     const int i0 = blk * chunk, i1 = min(n, i0 + chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double ar[DG], ai[DG];
@@ -143,6 +144,7 @@ __global__ __launch_bounds__(NTH) void kr_update_dots(KrSetT<TV, TW> k, int n, i
     const cpx<TV>* __restrict__ V = k.V[p];
     cpx<TW>* w = k.w[p];
     const double* h = red + (size_t)p * P;
+    if (k.n[p] > 0) n = k.n[p];
     for (int r = threadIdx.x; r < R; r += NTH) hs[r] = r < m ? zd(h[2 * r], h[2 * r + 1]) : zd();
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, e = lane % EL, g = lane / EL;
@@ -246,6 +248,7 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<TV, TW> k, int n, int ch
     cpx<TW>* w = k.w[p];
     const int m = k.m[p];
     const double* h = red + (size_t)p * P;
+    if (k.n[p] > 0) n = k.n[p];
     for (int r = threadIdx.x; r < m; r += NTH) hs[r] = zd(h[2 * r], h[2 * r + 1]);
     __syncthreads();
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
@@ -303,6 +306,7 @@ __global__ __launch_bounds__(NTH) void kr_norm(KrSetT<TV, TW> k, int n, int chun
     __shared__ double lds[NTH / 64];
     const int p = blockIdx.y;
     const cpx<TW>* __restrict__ v = k.w[p];
+    if (k.n[p] > 0) n = k.n[p];
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double nn = 0;
     for (int i = i0 + threadIdx.x; i < i1; i += NTH) {
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(NTH) void kr_reduce(const double* __restrict__ part
 template <class T, class TA>
 __global__ __launch_bounds__(NTH) void kr_scale(VecOpsT<T, TA> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) {
+    if (i < (v.n[p] > 0 ? v.n[p] : n)) {
         const zd a = ldz(v.a[p][i]);
         v.dst[p][i] = stz<T>(v.f[p] * a.re, v.f[p] * a.im);
     }
@@ -335,7 +339,7 @@ __global__ __launch_bounds__(NTH) void kr_scale(VecOpsT<T, TA> v, int n) {
 template <class T>
 __global__ __launch_bounds__(NTH) void kr_sub(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) {
+    if (i < (v.n[p] > 0 ? v.n[p] : n)) {
         const zd a = ldz(v.a[p][i]), b = ldz(v.b[p][i]);
         v.dst[p][i] = stz<T>(a.re - b.re, a.im - b.im);
     }
@@ -344,13 +348,13 @@ __global__ __launch_bounds__(NTH) void kr_sub(VecOpsT<T> v, int n) {
 template <class T>
 __global__ __launch_bounds__(NTH) void kr_copy(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) v.dst[p][i] = v.a[p][i];
+    if (i < (v.n[p] > 0 ? v.n[p] : n)) v.dst[p][i] = v.a[p][i];
 }
 
 template <class T>
 __global__ __launch_bounds__(NTH) void kr_zero(VecOpsT<T> v, int n) {
     const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
-    if (i < n) v.dst[p][i] = cpx<T>();
+    if (i < (v.n[p] > 0 ? v.n[p] : n)) v.dst[p][i] = cpx<T>();
 }
 
 template <class T, class TA>
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(NTH) void kr_combine(VecOpsT<T, TA> v, const zd* __
     for (int r = threadIdx.x; r < k; r += NTH) ys[r] = y[(size_t)p * rows + r];
     __syncthreads();
     const int i = blockIdx.x * NTH + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (v.n[p] > 0 ? v.n[p] : n)) return;
     const cpx<TA>* V = v.a[p];
     zd acc = ldz(v.dst[p][i]);
     for (int r = 0; r < k; ++r) acc = acc + ldz(V[(size_t)r * ld + i]) * ys[r];
@@ -385,29 +389,13 @@ __global__ __launch_bounds__(NTH) void kr_add_float(Conv c, int n) {
 
 }  // namespace
 
-void Krylov::layout(int n, int& nblk, int& chunk, int& nblk_d, int& chunk_d) {
-    // chunks of whole pairs of elements: about 1024 elements per block for the passes that write w, 2048 for the
-    // multi-dot (whose blocks also split the rows); at most 2048 blocks of each
-    nblk = (int)std::min<long>(2048, std::max<long>(1, ((long)n + 1023) / 1024));
-    chunk = ((n + nblk - 1) / nblk + 1) & ~1;
-    nblk = std::max(1, (n + chunk - 1) / chunk);
-    nblk_d = (int)std::min<long>(2048, std::max<long>(1, ((long)n + 2047) / 2048));
-    chunk_d = ((n + nblk_d - 1) / nblk_d + 1) & ~1;
-    nblk_d = std::max(1, (n + chunk_d - 1) / chunk_d);
-}
-
-int Krylov::part_blocks(int n) {
-    int a, b, c, d;
-    layout(n, a, b, c, d);
-    return std::max(a, c);
-}
-
 void Krylov::init(int n_, int rows) {
     if (rows > 512) fail(GT_ERR_ARG, "GMRES restart above 511");
     n = n_;
     rows_cap = rows;
     P = 4 * rows + 8;
-    layout(n, nblk, chunk, nblk_d, chunk_d);
+    nblk = part_blocks(n);
+    nblk_d = std::max(1, (n + chunk_d - 1) / chunk_d);
     part.alloc((size_t)KMAX * part_blocks(n) * P);
     red.alloc((size_t)KMAX * P);
 }

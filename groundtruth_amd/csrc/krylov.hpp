@@ -19,22 +19,23 @@ struct KrSetT {
     const cpx<TV>* V[KMAX];
     int m[KMAX];
     cpx<TW>* w[KMAX];
+    int n[KMAX];              // each right-hand side's vector length (0: the pass's n)
 };
 using KrSet = KrSetT<double>;
 
 struct Krylov {
     int n = 0;
-    int nblk = 0, chunk = 0;     // partial-sum blocks of the passes over w and their elements (even)
-    int nblk_d = 0, chunk_d = 0; // the same for the multi-dot
+    int nblk = 0, nblk_d = 0;    // partial-sum blocks of the passes over w, and of the multi-dot, at the longest n
+    static constexpr int chunk = 1024, chunk_d = 2048;    // their elements per block (whatever n: a right-hand
+                                                          // side's sums do not depend on the batch's lengths)
     int rows_cap = 0;
     int P = 0;                // values per (right-hand side, block): 2 rows_cap (h) + 8 (norms) + 2 rows_cap (h2)
     DevBuf<double> part;      // KMAX x nblk x P partial sums
     DevBuf<double> red;       // KMAX x P reduced values
 
     void init(int n, int rows);
-    // the blocks and chunks of the passes over vectors of n, and the partial-sum blocks the buffers hold
-    static void layout(int n, int& nblk, int& chunk, int& nblk_d, int& chunk_d);
-    static int part_blocks(int n);
+    // the partial-sum blocks the buffers hold for vectors of up to n
+    static int part_blocks(int n) { return std::max(1, (n + chunk - 1) / chunk); }
     int slot_w() const { return 2 * rows_cap; }          // |w|^2 before the projection
     int slot_w_after() const { return 2 * rows_cap + 1; } // |w|^2 after it
     int slot_norm() const { return 2 * rows_cap + 2; }    // norm2's value
@@ -71,6 +72,7 @@ struct VecOpsT {
     const cpx<TA>* b[KMAX];
     double f[KMAX];
     int k[KMAX];
+    int n[KMAX];              // each right-hand side's vector length (0: the call's n, which bounds them all)
 };
 using VecOps = VecOpsT<double>;
 

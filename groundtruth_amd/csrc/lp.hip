@@ -264,8 +264,8 @@ __global__ __launch_bounds__(64 * RW) void lp_rows_inv_w(LpArgs<T> a, int nb0, i
             vv.re -= wf * val.im;
             vv.im += wf * val.re;
             const int ka = a.a_keep[b], kk = a.b_keep[b];
-            if (ka >= 0) vv = vv - x[a.nb + ka];
-            if (kk >= 0) vv = vv + x[a.nb + kk];
+            if (ka >= 0) vv = vv - x[a.rhs.nb[p] + ka];
+            if (kk >= 0) vv = vv + x[a.rhs.nb[p] + kk];
             y[b] = vv;
         }
     }
@@ -392,37 +392,40 @@ __global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
             v.re -= w * val.im;
             v.im += w * val.re;
             const int ka = a.a_keep[b], kk = a.b_keep[b];
-            if (ka >= 0) v = v - x[a.nb + ka];
-            if (kk >= 0) v = v + x[a.nb + kk];
+            if (ka >= 0) v = v - x[a.rhs.nb[p] + ka];
+            if (kk >= 0) v = v + x[a.rhs.nb[p] + kk];
             y[b] = v;
         }
     }
 }
 
-// Branch rows (z I - Ar V) and node rows (Ar^T Ib) of the system product, one right-hand side per blockIdx.y.
+// Branch rows (z I - Ar V) and node rows (Ar^T Ib) of the system product, one right-hand side per blockIdx.y with
+// its own stack's branches.
 template <class T>
 __global__ __launch_bounds__(256) void tail_bottom(TailArgs<T> a) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     const int p = blockIdx.y;
     const cpx<T>* x = a.rhs.x[p];
     cpx<T>* y = a.rhs.y[p];
-    if (idx < a.nbr) {
+    const int nbr = a.nbr[p], nb = a.rhs.nb[p];
+    if (idx < nbr) {
         const int b = a.ns + idx;
         cpx<T> v = a.rhs.z_tail[p][idx] * x[b];
-        const int ka = a.a_keep[b], kk = a.b_keep[b];
-        if (ka >= 0) v = v - x[a.nb + ka];
-        if (kk >= 0) v = v + x[a.nb + kk];
+        const int ka = a.a_keep[p][b], kk = a.b_keep[p][b];
+        if (ka >= 0) v = v - x[nb + ka];
+        if (kk >= 0) v = v + x[nb + kk];
         y[b] = v;
-    } else if (idx < a.nbr + a.nk) {
-        const int n = idx - a.nbr;
+    } else if (idx < nbr + a.nk) {
+        const int n = idx - nbr;
+        const int* adj = a.adj[p];
         cpx<T> acc;
-        for (int e = a.adj_ptr[n]; e < a.adj_ptr[n + 1]; ++e) {
-            const int code = a.adj[e];
+        for (int e = a.adj_ptr[p][n]; e < a.adj_ptr[p][n + 1]; ++e) {
+            const int code = adj[e];
             const cpx<T> v = x[code >> 1];
             if (code & 1) acc = acc - v;
             else acc = acc + v;
         }
-        y[a.nb + n] = acc;
+        y[nb + n] = acc;
     }
 }
 
@@ -450,7 +453,9 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
 
 template <class T>
 void launch_tail_bottom(const TailArgs<T>& a, hipStream_t st) {
-    const int n = a.nbr + a.nk;
+    int nbr = 0;
+    for (int p = 0; p < a.rhs.nr; ++p) nbr = std::max(nbr, a.nbr[p]);
+    const int n = nbr + a.nk;
     if (n == 0) return;
     hipLaunchKernelGGL((tail_bottom<T>), dim3(blocks_for(n, 256), a.rhs.nr), dim3(256), 0, st, a);
     GT_LAUNCH("tail_bottom");

@@ -41,6 +41,7 @@ struct RhsSet {
     T w[KMAX];                        // angular frequency
     const cpx<T>* r_sheet[KMAX];      // S strip impedances
     const cpx<T>* z_tail[KMAX];       // branch impedances
+    int nb[KMAX];                     // offset of the node potentials in x and y (the stack's branch count)
 };
 
 template <class T>
@@ -49,21 +50,24 @@ struct LpArgs {
     int nblk_rows[2];         // workgroups of each orientation in the row passes
     int nblk_cols[2];         // workgroups of each orientation in the column pass (= its P1)
     int S;
-    int mode;                 // 0: y = L_p x (strip vectors); 1: the system product's strip rows
-    int nb;                   // offset of the node potentials in x (mode 1)
+    int mode;                 // 0: y = L_p x (strip vectors); 1: the system product's strip rows (node potentials
+                              // at rhs.nb[p] in x)
     size_t t_stride;          // elements of T1 / T2 per right-hand side
     const int* a_keep;
     const int* b_keep;
     RhsSet<T> rhs;
 };
 
+// The branch and node rows of the product. The right-hand sides may belong to different stacks of one core (a
+// board and its what-ifs): the strips are common, each has its own branches, incidence and node adjacency.
 template <class T>
 struct TailArgs {
-    int ns, nb, nbr, nk;
-    const int* a_keep;
-    const int* b_keep;
-    const int* adj_ptr;       // nk + 1
-    const int* adj;           // (branch << 1) | (1 if the node is the branch's b end)
+    int ns, nk;
+    int nbr[KMAX];
+    const int* a_keep[KMAX];
+    const int* b_keep[KMAX];
+    const int* adj_ptr[KMAX]; // nk + 1
+    const int* adj[KMAX];     // (branch << 1) | (1 if the node is the branch's b end)
     RhsSet<T> rhs;
 };
 

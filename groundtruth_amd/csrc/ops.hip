@@ -12,16 +12,22 @@ namespace gt {
 
 namespace {
 
+constexpr int WB_SMALL = 256;   // most Woodbury columns (branches) of one stack
+
 __device__ __forceinline__ zd ldz(const zd& v) { return v; }
 __device__ __forceinline__ zd ldz(const cpx<float>& v) { return zd(v.re, v.im); }
 template <class T>
 __device__ __forceinline__ cpx<T> stz(const zd& v) { return cpx<T>((T)v.re, (T)v.im); }
 
 // T: the storage of the input and output vectors (double, or float for the mixed-precision inner solve); the
-// arithmetic and the Schur solve are in double
+// arithmetic and the Schur solve are in double. The right-hand sides may belong to different stacks of one core (a
+// board and what-ifs derived from it): the strips, the kept nodes and the Schur factor are common; each has its own
+// branches (so its own node offset nb), incidence, node adjacency and Woodbury columns, the first k_sh of which are
+// the batch's host stack's (copied into every stack derived from it).
 template <class T>
 struct PreArgs {
-    int ns, nb, nk, k, nr;
+    int ns, nk, nr, k_sh, kt;
+    int nb[KMAX], k[KMAX];
     const cpx<T>* r[KMAX];    // n: [r1 ; r2]
     cpx<T>* out[KMAX];        // n: [x1 ; y]
     const zd* dinv_sheet[KMAX];
@@ -30,12 +36,13 @@ struct PreArgs {
     zd s_scale[KMAX];
     zd* s;                    // nk x nr interleaved: Schur right-hand sides
     const zd* z;              // nk x nr interleaved: B^-1 s
-    zd* t;                    // nr x k: Q (U^T z)
-    const double* W;          // k columns of nk
-    const int* a_keep;
-    const int* b_keep;
-    const int* adj_ptr;
-    const int* adj;
+    zd* t;                    // nr x kt: Q (U^T z)
+    const double* W;          // the shared columns: k_sh of nk
+    const double* Wx[KMAX];   // each right-hand side's own columns k_sh .. k - 1
+    const int* a_keep[KMAX];
+    const int* b_keep[KMAX];
+    const int* adj_ptr[KMAX];
+    const int* adj[KMAX];
     const int* node_sheet;
     const unsigned char* strip_sheet;
 };
@@ -46,14 +53,15 @@ template <class T>
 __global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs<T> a) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= a.nk) return;
-    const int e0 = a.adj_ptr[n], e1 = a.adj_ptr[n + 1];
     const int sh = a.node_sheet[n];
     for (int p = 0; p < a.nr; ++p) {
         const cpx<T>* r = a.r[p];
-        zd acc = ldz(r[a.nb + n]);
+        const int e0 = a.adj_ptr[p][n], e1 = a.adj_ptr[p][n + 1];
+        const int* adj = a.adj[p];
+        zd acc = ldz(r[a.nb[p] + n]);
         const zd ds = a.dinv_sheet[p][sh];
         for (int e = e0; e < e1; ++e) {
-            const int code = a.adj[e], b = code >> 1;
+            const int code = adj[e], b = code >> 1;
             const zd v = (b < a.ns ? ds : a.dinv_tail[p][b - a.ns]) * ldz(r[b]);
             if (code & 1) acc = acc + v;
             else acc = acc - v;
@@ -62,15 +70,16 @@ __global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs<T> a) {
     }
 }
 
-// t_p = Q_p (U^T z_p): one workgroup per right-hand side (k, the branch count, is small)
+// t_p = Q_p (U_p^T z_p): one workgroup per right-hand side (k, the branch count, is small)
 template <class T>
 __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs<T> a) {
-    __shared__ double ut_raw[512];
+    __shared__ double ut_raw[2 * WB_SMALL];
     zd* ut = reinterpret_cast<zd*>(ut_raw);
     const int p = blockIdx.x;
-    for (int m = threadIdx.x; m < a.k; m += 256) {
+    const int k = a.k[p];
+    for (int m = threadIdx.x; m < k; m += 256) {
         const int b = a.ns + m;
-        const int ka = a.a_keep[b], kb = a.b_keep[b];
+        const int ka = a.a_keep[p][b], kb = a.b_keep[p][b];
         zd v;
         if (ka >= 0) v = v + a.z[(size_t)ka * a.nr + p];
         if (kb >= 0) v = v - a.z[(size_t)kb * a.nr + p];
@@ -78,16 +87,17 @@ __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs<T> a) {
     }
     __syncthreads();
     const zd* q = a.q[p];
-    for (int i = threadIdx.x; i < a.k; i += 256) {
+    for (int i = threadIdx.x; i < k; i += 256) {
         zd acc;
-        for (int m = 0; m < a.k; ++m) acc = acc + q[i * a.k + m] * ut[m];
-        a.t[p * a.k + i] = acc;
+        for (int m = 0; m < k; ++m) acc = acc + q[i * k + m] * ut[m];
+        a.t[p * a.kt + i] = acc;
     }
 }
 
-// y_p = s_scale_p (z_p - W t_p), written to the node part of each output. A thread takes one node for the whole
-// batch: each W value is read once and applied to every right-hand side (t of the batch staged in LDS, in chunks
-// of WB branches). The sums over m run in the same order for every batch size.
+// y_p = s_scale_p (z_p - W_p t_p), written to the node part of each output. A thread takes one node for the whole
+// batch: each shared W value is read once and applied to every right-hand side (t of the batch staged in LDS, in
+// chunks of WB branches); a right-hand side's own columns follow. The sums over m run in ascending m for every
+// batch.
 constexpr int WB = 64;
 
 template <class T>
@@ -102,12 +112,12 @@ __global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs<T> a) {
         vre[p] = z.re;
         vim[p] = z.im;
     }
-    for (int m0 = 0; m0 < a.k; m0 += WB) {
-        const int mc = min(WB, a.k - m0);
+    for (int m0 = 0; m0 < a.k_sh; m0 += WB) {
+        const int mc = min(WB, a.k_sh - m0);
         __syncthreads();
         for (int e = threadIdx.x; e < mc * KMAX; e += 256) {      // right-hand sides past the batch: zeros
             const int p = e / mc, m = e - p * mc;
-            ts[m * KMAX + p] = p < a.nr ? a.t[p * a.k + m0 + m] : zd();
+            ts[m * KMAX + p] = p < a.nr ? a.t[p * a.kt + m0 + m] : zd();
         }
         __syncthreads();
         if (n < a.nk) {
@@ -124,20 +134,30 @@ __global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs<T> a) {
     }
     if (n >= a.nk) return;
 #pragma unroll
-    for (int p = 0; p < KMAX; ++p)
-        if (p < a.nr) a.out[p][a.nb + n] = stz<T>(a.s_scale[p] * zd(vre[p], vim[p]));
+    for (int p = 0; p < KMAX; ++p) {
+        if (p < a.nr) {
+            const double* Wx = a.Wx[p];
+            const zd* tp = a.t + p * a.kt;
+            for (int m = a.k_sh; m < a.k[p]; ++m) {
+                const double w = Wx[(size_t)(m - a.k_sh) * a.nk + n];
+                vre[p] -= w * tp[m].re;
+                vim[p] -= w * tp[m].im;
+            }
+            a.out[p][a.nb[p] + n] = stz<T>(a.s_scale[p] * zd(vre[p], vim[p]));
+        }
+    }
 }
 
 // x1 = D^-1 (r1 + Ar y)
 template <class T>
 __global__ __launch_bounds__(256) void pre_branches(PreArgs<T> a) {
     const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= a.nb) return;
-    const int ka = a.a_keep[b], kb = a.b_keep[b];
     const int sh = b < a.ns ? a.strip_sheet[b] : -1;
     for (int p = 0; p < a.nr; ++p) {
+        if (b >= a.nb[p]) continue;
+        const int ka = a.a_keep[p][b], kb = a.b_keep[p][b];
         zd v = ldz(a.r[p][b]);
-        const cpx<T>* y = a.out[p] + a.nb;
+        const cpx<T>* y = a.out[p] + a.nb[p];
         if (ka >= 0) v = v + ldz(y[ka]);
         if (kb >= 0) v = v - ldz(y[kb]);
         const zd d = sh >= 0 ? a.dinv_sheet[p][sh] : a.dinv_tail[p][b - a.ns];
@@ -147,14 +167,45 @@ __global__ __launch_bounds__(256) void pre_branches(PreArgs<T> a) {
 
 }  // namespace
 
-void Work::ensure(const gt_stack* st, int nr) {
-    if (nr <= cap) return;
-    T1.alloc(std::max<size_t>(1, st->t_elems) * nr);
-    T2.alloc(std::max<size_t>(1, st->t_elems) * nr);
-    s.alloc((size_t)std::max(1, st->nk) * nr);
-    z.alloc((size_t)std::max(1, st->nk) * nr);
-    t.alloc((size_t)std::max(1, st->nbr) * nr);
-    cap = nr;
+void Work::ensure(const gt_stack* st, int nr, int kt) {
+    kt = std::max(kt, st->nbr);
+    if (nr > cap) {
+        T1.alloc(std::max<size_t>(1, st->t_elems) * nr);
+        T2.alloc(std::max<size_t>(1, st->t_elems) * nr);
+        s.alloc((size_t)std::max(1, st->nk) * nr);
+        z.alloc((size_t)std::max(1, st->nk) * nr);
+    }
+    if (nr > cap || kt > t_cols) {
+        t_cols = std::max(kt, t_cols);
+        t.alloc((size_t)std::max(1, t_cols) * std::max(nr, cap));
+    }
+    cap = std::max(cap, nr);
+}
+
+int64_t Work::bytes_after(const gt_stack* st, int nr, int kt) const {
+    kt = std::max(kt, st->nbr);
+    const int64_t zb = sizeof(zd);
+    int64_t b = nr > cap ? (int64_t)(2 * std::max<size_t>(1, st->t_elems) + 2 * (size_t)std::max(1, st->nk)) * nr * zb
+                         : (int64_t)(T1.bytes() + T2.bytes() + s.bytes() + z.bytes());
+    if (nr > cap || kt > t_cols) b += (int64_t)std::max(1, std::max(kt, t_cols)) * std::max(nr, cap) * zb;
+    else b += (int64_t)t.bytes();
+    return b;
+}
+
+// The right-hand sides of one batch on the host stack st: each operator's stack is st or one derived from it (a
+// what-if sharing st's core, with st's Woodbury columns first)
+void check_batch(const gt_stack* st, int nr, gt_op* const* ops) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
+    for (int p = 0; p < nr; ++p) {
+        const gt_stack* o = ops[p]->st;
+        if (o == st) continue;
+        bool ok = o->core == st->core && o->k >= st->k && o->k_root == st->k_root;
+        if (ok) {
+            ok = false;
+            for (uint64_t id : o->lineage) ok = ok || id == st->id;
+        }
+        if (!ok) fail(GT_ERR_ARG, "a batch's operators must belong to one stack or to what-ifs derived from it");
+    }
 }
 
 namespace {
@@ -193,7 +244,6 @@ LpArgs<T> lp_args_t(const gt_stack* st) {
         }
         a.S = d.S;
         a.mode = 1;
-        a.nb = d.nb;
         a.t_stride = d.t_stride;
         a.a_keep = d.a_keep;
         a.b_keep = d.b_keep;
@@ -205,7 +255,7 @@ LpArgs<T> lp_args_t(const gt_stack* st) {
 
 template <class T>
 void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y) {
-    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
+    check_batch(st, nr, ops);
     hipStream_t s = st->ctx->stream;
     st->work.ensure(st, nr);
     RhsSet<T> rs{};
@@ -213,6 +263,7 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
     for (int p = 0; p < nr; ++p) {
         rs.x[p] = x[p];
         rs.y[p] = y[p];
+        rs.nb[p] = ops[p]->st->nb;
         rs.w[p] = (T)ops[p]->w;
         if constexpr (std::is_same<T, double>::value) {
             rs.r_sheet[p] = ops[p]->r_sheet.p;
@@ -243,7 +294,18 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
             launch_lp(la, st->shape, s);
         }
     }
-    TailArgs<T> ta{st->ns, st->nb, st->nbr, st->nk, st->a_keep.p, st->b_keep.p, st->adj_ptr.p, st->adj.p, rs};
+    TailArgs<T> ta{};
+    ta.ns = st->ns;
+    ta.nk = st->nk;
+    for (int p = 0; p < nr; ++p) {
+        const gt_stack* o = ops[p]->st;
+        ta.nbr[p] = o->nbr;
+        ta.a_keep[p] = o->a_keep.p;
+        ta.b_keep[p] = o->b_keep.p;
+        ta.adj_ptr[p] = o->adj_ptr.p;
+        ta.adj[p] = o->adj.p;
+    }
+    ta.rhs = rs;
     launch_tail_bottom(ta, s);
 }
 
@@ -315,6 +377,7 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
         }
     }
     st->k = k;
+    st->k_root = base ? base->k_root : k;
 }
 
 void woodbury_gram_device(const gt_stack* st, double* G_host) {
@@ -331,47 +394,55 @@ void woodbury_gram_device(const gt_stack* st, double* G_host) {
 
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z) {
-    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
+    check_batch(st, nr, ops);
     hipStream_t s = st->ctx->stream;
-    st->work.ensure(st, nr);
     PreArgs<T> a{};
     a.ns = st->ns;
-    a.nb = st->nb;
     a.nk = st->nk;
     a.nr = nr;
-    a.k = ops[0]->k;
+    a.k_sh = st->k_root;        // the lineage's first columns: the same split alone and in a board's batch
+    a.kt = 0;
+    int nbmax = 0;
     for (int p = 0; p < nr; ++p) {
-        if (ops[p]->st != st || ops[p]->k != a.k) fail(GT_ERR_ARG, "a batch's operators must share one stack");
+        const gt_stack* o = ops[p]->st;
+        if (ops[p]->k != o->k) fail(GT_ERR_ARG, "the operator's Woodbury size differs from its stack's");
+        if (o->k > WB_SMALL) fail(GT_ERR_ARG, "more than 256 branches on one stack");
+        a.kt = std::max(a.kt, o->k);
+        nbmax = std::max(nbmax, o->nb);
+        a.nb[p] = o->nb;
+        a.k[p] = o->k;
         a.r[p] = r[p];
         a.out[p] = z[p];
         a.dinv_sheet[p] = ops[p]->dinv_sheet.p;
         a.dinv_tail[p] = ops[p]->dinv_tail.p;
         a.q[p] = ops[p]->q.p;
         a.s_scale[p] = ops[p]->s_scale;
+        a.Wx[p] = o->W.p ? o->W.p + (size_t)a.k_sh * st->nk : nullptr;
+        a.a_keep[p] = o->a_keep.p;
+        a.b_keep[p] = o->b_keep.p;
+        a.adj_ptr[p] = o->adj_ptr.p;
+        a.adj[p] = o->adj.p;
     }
+    st->work.ensure(st, nr, a.kt);
     a.s = st->work.s.p;
     a.z = st->work.z.p;
     a.t = st->work.t.p;
     a.W = st->W.p;
-    a.a_keep = st->a_keep.p;
-    a.b_keep = st->b_keep.p;
-    a.adj_ptr = st->adj_ptr.p;
-    a.adj = st->adj.p;
     a.node_sheet = st->node_sheet.p;
     a.strip_sheet = st->strip_sheet.p;
     if (st->nk > 0) {
         hipLaunchKernelGGL(pre_schur_rhs<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_schur_rhs");
         schur_apply(st, st->work.s.p, st->work.z.p, nr);
-        if (a.k > 0) {
+        if (a.kt > 0) {
             hipLaunchKernelGGL(pre_woodbury_small<T>, dim3(nr), dim3(256), 0, s, a);
             GT_LAUNCH("pre_woodbury_small");
         }
         hipLaunchKernelGGL(pre_woodbury_apply<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_woodbury_apply");
     }
-    if (st->nb > 0) {
-        hipLaunchKernelGGL(pre_branches<T>, dim3(blocks_for(st->nb, 256)), dim3(256), 0, s, a);
+    if (nbmax > 0) {
+        hipLaunchKernelGGL(pre_branches<T>, dim3(blocks_for(nbmax, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_branches");
     }
 }

@@ -2,6 +2,8 @@
 // operator, and the device work each owns.
 #pragma once
 
+#include <atomic>
+
 #include <algorithm>
 #include <map>
 #include <memory>
@@ -51,8 +53,12 @@ struct Work {
     int cap = 0;              // right-hand sides the buffers hold
     DevBuf<zd> T1, T2;        // L_p intermediates, t_elems per right-hand side
     DevBuf<zd> s, z;          // Schur right-hand sides and solutions, interleaved (n_keep x batch)
-    DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (k each)
-    void ensure(const gt_stack_s* st, int nr);
+    DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (t_cols each)
+    int t_cols = 0;
+    void ensure(const gt_stack_s* st, int nr, int kt = 0);
+    int64_t bytes() const { return (int64_t)(T1.bytes() + T2.bytes() + s.bytes() + z.bytes() + t.bytes()); }
+    // the bytes it would hold after ensure(st, nr, kt)
+    int64_t bytes_after(const gt_stack_s* st, int nr, int kt) const;
 };
 
 }  // namespace gt
@@ -96,17 +102,24 @@ struct gt_stack_s {
     gt::Work work;
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
     int k = 0;
+    int k_root = 0;                           // the columns of the first stack of its lineage (copied first into W)
     bool derived = false;                     // a what-if stack: its core belongs to the stack it came from
+    uint64_t id = 0;                          // unique in the process
+    std::vector<uint64_t> lineage;            // the ids of the stacks it was derived from (parent first)
 
     explicit gt_stack_s(std::shared_ptr<gt::StackCore> c = std::make_shared<gt::StackCore>())
         : core(std::move(c)), o(core->o), tw(core->tw), twf(core->twf), node_sheet(core->node_sheet),
           strip_sheet(core->strip_sheet), h_strip_sheet(core->h_strip_sheet), shape(core->shape),
-          t_elems(core->t_elems), schur(core->schur) {}
+          t_elems(core->t_elems), schur(core->schur), id(next_id()) {}
     gt_stack_s(const gt_stack_s&) = delete;
     gt_stack_s& operator=(const gt_stack_s&) = delete;
 
     int64_t device_bytes() const;
     gt::LpArgs<double> lp_args(int mode) const;
+    static uint64_t next_id() {
+        static std::atomic<uint64_t> counter{0};
+        return ++counter;
+    }
 };
 
 struct gt_op_s {
@@ -119,8 +132,10 @@ struct gt_op_s {
 };
 
 namespace gt {
-// device work of a batch of operators on one stack (ops.hip): y_p = K_p x_p, z_p = M_p^-1 r_p for p < nr <= 16
-// (T: the storage of the vectors, double or float; the preconditioner's arithmetic and Schur solve are double)
+// device work of a batch of operators on one host stack (ops.hip): y_p = K_p x_p, z_p = M_p^-1 r_p for p < nr <= 16
+// (T: the storage of the vectors, double or float; the preconditioner's arithmetic and Schur solve are double).
+// Each operator belongs to st or to a what-if stack derived from it (check_batch).
+void check_batch(const gt_stack* st, int nr, gt_op* const* ops);
 template <class T>
 void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y);
 template <class T>

@@ -4,11 +4,14 @@ device-memory footprint, and every request's latency is recorded.
 The reference solves a board's (model, frequency) pairs as independent tasks: `run_parallel` (simulate.py:3964-4013)
 runs them on a thread pool, and `Admission` (simulate.py:3930-3961) lets a task start only while the memory it
 plans to use fits the budget. Here the tasks are GMRES solves on the device, and they share launches instead of
-threads: a `Session` (gt_session) runs up to 16 solves of one stack as one batch, a new solve joins it between two
-Arnoldi steps, and a finished one leaves without waiting for the rest. `Server` keeps one session per stack in use
-(a board, or a what-if stack derived from it, peec.py:1871-1909), turns them round-robin, admits the sessions by
-their exact device bytes (gt_session_plan_bytes, the footprint gpu.py:593-605 `device_bytes` estimates), and
-reports p50/p99 latency and solves per second.
+threads: a `Session` (gt_session) runs up to 16 solves as one batch, a new solve joins it between two Arnoldi
+steps, and a finished one leaves without waiting for the rest. A board's session also takes the solves of its
+what-if stacks (peec.py:1871-1909 `with_branches`: the board plus a few candidate vias), each over its own unknowns,
+so what-ifs of different sizes share launches with each other and with the board's own solves. `Server` keeps one
+session per board (and one for a what-if that cannot join its board's, e.g. a via joining two islands), turns them
+round-robin, admits sessions and what-if stacks by their exact device bytes (gt_session_open_bytes, the footprint
+gpu.py:593-605 `device_bytes` estimates), and reports p50/p99 latency and solves per second. The Krylov bases are
+FP64, or FP32 with everything else FP64 (precision "fp32-basis").
 
 A served solve is the same solve as `RouteExcitation.solve(f)`: the same right-hand side, norm and GMRES state
 machine, so its solution is bit-identical alone and in any mix (the library's batches keep each right-hand side's
@@ -30,31 +33,46 @@ from .peec import BATCH, GMRES_CHECK_MISSES, GMRES_EXTRA_CYCLES, PlaneStack, Rou
 from .problem import SolverError
 
 
-def session_plan_bytes(stack: PlaneStack, capacity: int = BATCH, restart: Optional[int] = None) -> int:
+_BASIS = {"fp64": 0, "fp32-basis": 1}
+
+
+def _basis(precision: str) -> int:
+    if precision not in _BASIS:
+        raise ValueError("precision is 'fp64' or 'fp32-basis'")
+    return _BASIS[precision]
+
+
+def session_plan_bytes(stack: PlaneStack, capacity: int = BATCH, restart: Optional[int] = None,
+                       precision: str = "fp64", max_extra: int = 0) -> int:
     """Device bytes a session of `capacity` slots on `stack` adds: its GMRES workspace and the growth of the stack's
-    product scratch to that batch."""
-    b = int(_lib.lib().gt_session_plan_bytes(stack.handle, int(capacity), int(restart or 0)))
+    product scratch to that batch (for what-ifs of up to max_extra added branches)."""
+    b = int(_lib.lib().gt_session_open_bytes(stack.handle, int(capacity), int(restart or 0), _basis(precision),
+                                             int(max_extra)))
     if b < 0:
-        raise ValueError("1 to 16 slots and a non-negative restart")
+        raise ValueError("1 to 16 slots and a non-negative restart and max_extra")
     return b
 
 
 class Session:
-    """Continuous batching of the solves of one stack (gt_session): `submit` puts a solve into a free slot, `step`
-    advances the running ones, `poll` collects a finished one."""
+    """Continuous batching of solves (gt_session) on `stack` and on what-if stacks derived from it with up to
+    max_extra added branches: `submit` puts a solve into a free slot, `step` advances the running ones, `poll`
+    collects a finished one. precision "fp32-basis" stores the Krylov bases in FP32."""
 
-    def __init__(self, stack: PlaneStack, capacity: int = BATCH, restart: Optional[int] = None):
+    def __init__(self, stack: PlaneStack, capacity: int = BATCH, restart: Optional[int] = None,
+                 precision: str = "fp64", max_extra: int = 0):
         self.stack = stack
         self.capacity = int(capacity)
+        self.max_extra = int(max_extra)
         h = C.c_void_p()
-        _lib.check(_lib.lib().gt_session_create(stack.handle, self.capacity, int(restart or 0), C.byref(h)),
-                   "gt_session_create")
+        _lib.check(_lib.lib().gt_session_open(stack.handle, self.capacity, int(restart or 0), _basis(precision),
+                                              self.max_extra, C.byref(h)), "gt_session_open")
         self._h = h
 
     def submit(self, op, b_dev, x_dev, tol: float, bnorm: float = 0.0, max_cycles: int = 8,
                extra_cycles: int = GMRES_EXTRA_CYCLES, max_its: int = 0) -> int:
-        """The slot the solve x = M^-1 y, (K M^-1) y = b took, or -1 when every slot is taken. b_dev, x_dev: torch
-        complex128 device vectors of nb + n_keep values, b's work finished; b stays unchanged until the poll."""
+        """The slot the solve x = M^-1 y, (K M^-1) y = b took, or -1 when every slot is taken. op: an operator of the
+        session's stack or of a what-if stack derived from it. b_dev, x_dev: torch complex128 device vectors of the
+        op's stack's nb + n_keep values, b's work finished; b stays unchanged until the poll."""
         p = _lib.GmresParams(float(tol), float(bnorm), 0, int(max_cycles), int(extra_cycles), int(max_its),
                              GMRES_CHECK_MISSES)
         slot = C.c_int32()
@@ -126,6 +144,18 @@ class Admission:
         self.used -= need
         self.held -= 1
 
+    def try_grow(self, need: float) -> bool:
+        """More bytes for a claim already held (a what-if stack running in its board's session): granted while
+        they fit, and always when that claim is the only one."""
+        if self.held > 1 and self.used + need > self.budget:
+            return False
+        self.used += need
+        self.used_peak = max(self.used_peak, self.used)
+        return True
+
+    def shrink(self, need: float):
+        self.used -= need
+
 
 @dataclass
 class Request:
@@ -145,6 +175,7 @@ class Request:
     done_at: float = 0.0
     solutions: list = field(default_factory=list)
     error: Optional[str] = None
+    _hold: float = 0.0               # device bytes held for its what-if stack while it runs in its board's session
 
     @property
     def latency(self) -> float:
@@ -152,12 +183,14 @@ class Request:
 
 
 class _Solve:
-    """One (request, frequency) solve in flight: GMRES on b = rhs - K x0 from the warm start x0 (None: cold)."""
-    __slots__ = ("req", "k", "f", "ex", "op", "b", "x", "bnorm", "x0", "slot")
+    """One (request, frequency) solve in flight on `stack` (the board or a what-if stack): GMRES on b = rhs - K x0
+    from the warm start x0 (None: cold)."""
+    __slots__ = ("req", "k", "f", "ex", "op", "b", "x", "bnorm", "x0", "slot", "stack")
 
-    def __init__(self, req, k, f, ex, op, b, x, bnorm, x0):
+    def __init__(self, req, k, f, ex, op, b, x, bnorm, x0, stack):
         self.req, self.k, self.f, self.ex, self.op, self.b, self.x, self.bnorm = req, k, f, ex, op, b, x, bnorm
         self.x0 = x0
+        self.stack = stack
         self.slot = -1
 
 
@@ -179,20 +212,27 @@ class Server:
     device); `submit` queues a request; `turn` admits what fits, steps every session once and collects the finished
     solves; `drain` turns until everything submitted has completed.
 
-    Per turn, each session with work gets `steps_per_turn` Arnoldi steps; a new solve enters its stack's session as
-    soon as a slot is free, so solves of different requests and frequencies share launches while they overlap. A
-    what-if request gets the stack derived from its board with the added branches (its own session: a batch serves
-    one stack). Sessions are admitted by their planned device bytes against `budget` (default: 90 % of the device
-    memory free when the server starts); idle sessions are closed, oldest first, when a new one needs their room."""
+    Per turn, each session with work gets `steps_per_turn` Arnoldi steps; a new solve enters its board's session as
+    soon as a slot is free, so solves of different requests, frequencies and what-ifs share launches while they
+    overlap. A what-if request gets the stack derived from its board with the added branches; with up to
+    `max_extra` of them it joins the board's session, else (or when the added vias join islands, so the stack has a
+    factor of its own) it gets a session of its own. Sessions and what-if stacks are admitted by their planned
+    device bytes against `budget` (default: 90 % of the device memory free when the server starts); idle sessions
+    are closed, oldest first, when a new one needs their room. precision: the sessions' Krylov bases, "fp64" or
+    "fp32-basis"."""
 
     def __init__(self, device: int = 0, budget: Optional[float] = None, capacity: int = BATCH,
-                 steps_per_turn: int = 1, restart: Optional[int] = None):
+                 steps_per_turn: int = 1, restart: Optional[int] = None, precision: str = "fp64",
+                 max_extra: int = 8):
         import torch
+        _basis(precision)
         self.device = device
         self.dev = f"cuda:{device}"
         self.capacity = int(capacity)
         self.steps_per_turn = int(steps_per_turn)
         self.restart = restart
+        self.precision = precision
+        self.max_extra = int(max_extra)
         if budget is None:
             free, _ = torch.cuda.mem_get_info(device)
             budget = 0.9 * free
@@ -271,25 +311,43 @@ class Server:
         ex = self.excitation(req.board, req.route)
         return ex if stack is self.boards[req.board] else ex.on(stack)
 
+    def _host(self, req: Request, stack: PlaneStack) -> PlaneStack:
+        """The stack whose session runs the request's solves: its board's, for the board itself and for a what-if
+        derived from it with up to max_extra added branches; else the what-if stack's own."""
+        board = self.boards[req.board]
+        if stack is board:
+            return board
+        if getattr(stack, "_base", None) is board and stack.nbr - board.nbr <= self.max_extra:
+            return board
+        return stack
+
+    def _hold(self, need: float) -> bool:
+        """Admission of `need` bytes, closing idle sessions (oldest first) to make room."""
+        while not self.admission.try_hold(need):
+            idle = [k for k, s in self._served.items() if not s.slots and not s.waiting]
+            if not idle:
+                return False
+            self._close(idle[0])
+        return True
+
     def _admit_served(self, stack: PlaneStack, n_solves: int) -> Optional[_Served]:
         """The stack's session, made when its bytes are admitted (None: not now). A board's session has every
-        slot; a what-if stack's, made for one request, as many as that request's solves."""
+        slot and room for what-ifs; a what-if stack's own, made for one request, as many slots as that request's
+        solves."""
         sv = self._served.get(id(stack))
         if sv is not None:
             return sv
         whatif = stack is not None and all(stack is not b for b in self.boards.values())
         cap = min(self.capacity, max(1, n_solves)) if whatif else self.capacity
-        need = session_plan_bytes(stack, cap, self.restart)
+        extra = 0 if whatif else self.max_extra
+        need = session_plan_bytes(stack, cap, self.restart, self.precision, extra)
         if whatif:
             need += stack.device_bytes()           # the what-if stack's own incidence and Woodbury columns
-        while not self.admission.try_hold(need):
-            idle = [k for k, s in self._served.items() if not s.slots and not s.waiting]
-            if not idle:
-                return None
-            self._close(idle[0])
+        if not self._hold(need):
+            return None
         sv = _Served(stack, need)
         sv.whatif = whatif
-        sv.session = Session(stack, cap, self.restart)
+        sv.session = Session(stack, cap, self.restart, self.precision, extra)
         self._served[id(stack)] = sv
         return sv
 
@@ -299,12 +357,12 @@ class Server:
             sv.session.close()
         self.admission.release(sv.need)
 
-    def _split(self, req: Request, sv: _Served):
-        """The request's solves, queued on its stack's session: the right-hand sides of RouteExcitation.solve, and
-        for a what-if the warm start of PlaneStack.solve from the board's solution at f (peec.py:2568-2576)."""
+    def _split(self, req: Request, sv: _Served, stack: PlaneStack):
+        """The request's solves on `stack`, queued on the session sv: the right-hand sides of RouteExcitation.solve,
+        and for a what-if the warm start of PlaneStack.solve from the board's solution at f (peec.py:2568-2576)."""
         import torch
-        stack = sv.stack
         board = self.boards[req.board]
+        whatif = stack is not board
         ex = self._excitation(req, stack)
         req.solutions = [None] * len(req.fs)
         for k, f in enumerate(req.fs):
@@ -313,7 +371,7 @@ class Server:
             op = stack.operator(f)
             b = torch.from_numpy(rhs).to(self.dev)
             x0 = None
-            if sv.whatif and req.warm:
+            if whatif and req.warm:
                 xb = self._board_x.get((req.board, id(req.route), PlaneStack.freq_key(f)))
                 g = stack.map_state_dev(xb, board)
                 if g is not None:
@@ -321,7 +379,7 @@ class Server:
                     x0, b = stack.warm_start_dev(op, b, [g])
                     self.warm_starts += x0 is not None
             x = torch.empty_like(b)
-            sv.waiting.append(_Solve(req, k, f, ex, op, b, x, float(np.linalg.norm(rhs)), x0))
+            sv.waiting.append(_Solve(req, k, f, ex, op, b, x, float(np.linalg.norm(rhs)), x0, stack))
         self._open[id(req)] = len(req.fs)
 
     def turn(self) -> list:
@@ -334,7 +392,15 @@ class Server:
             req = self._pending.popleft()
             try:
                 stack = self._stack_for(req)
-                sv = self._admit_served(stack, len(req.fs))
+                host = self._host(req, stack)
+                sv = self._admit_served(host, len(req.fs))
+                if sv is not None and host is not stack:
+                    # a what-if in its board's session: its own incidence and Woodbury columns, held while it runs
+                    hold = stack.device_bytes()
+                    if self.admission.try_grow(hold):
+                        req._hold = hold
+                    else:
+                        sv = None
             except (SolverError, ValueError, NotImplementedError) as e:
                 req.error = str(e)
                 req.done_at = time.perf_counter()
@@ -343,7 +409,7 @@ class Server:
             if sv is None:
                 still.append(req)
                 continue
-            self._split(req, sv)
+            self._split(req, sv, stack)
             split = True
         self._pending = still
         if split:
@@ -382,12 +448,12 @@ class Server:
             if not res.converged:
                 req.error = f"GMRES did not converge at {s.f:.4g} Hz ({res.iterations} iterations)"
             else:
-                st = sv.stack
+                st = s.stack
                 xd = s.x
                 if s.x0 is not None:
                     with torch.cuda.stream(st.torch_stream()):
                         xd = s.x + s.x0
-                if not sv.whatif:
+                if st is self.boards[req.board]:
                     self._board_x[(req.board, id(req.route), PlaneStack.freq_key(s.f))] = xd
                 with torch.cuda.stream(st.torch_stream()):
                     xv = xd.cpu().numpy()
@@ -399,6 +465,9 @@ class Server:
             self._open[id(req)] -= 1
             if self._open[id(req)] == 0:
                 del self._open[id(req)]
+                if req._hold:
+                    self.admission.shrink(req._hold)
+                    req._hold = 0.0
                 req.done_at = time.perf_counter()
                 self.completed.append(req)
                 done.append(req)

@@ -270,6 +270,16 @@ int gt_gmres_batch_fp32_basis(gt_stack* stack, int32_t nr, gt_op* const* ops, co
  * by the step that finishes the solve. */
 typedef struct gt_session_s gt_session;
 int gt_session_create(gt_stack* stack, int32_t capacity, int32_t restart, gt_session** out);
+/* The general form. basis: GT_BASIS_FP64, or GT_BASIS_FP32 for the Krylov basis stored in FP32 (as
+ * gt_gmres_batch_fp32_basis). max_extra: the session's slots also take solves of what-if stacks derived from `stack`
+ * (gt_stack_derive, directly or in a chain) with up to max_extra added branches, each over its own unknowns, in the
+ * same launches as the stack's own solves: the board's what-ifs of different sizes batch together. gt_session_create
+ * is gt_session_open(stack, capacity, restart, GT_BASIS_FP64, 0). (No counterpart in the reference, whose what-ifs
+ * run as separate thread-pool jobs, simulate.py:3964-4013.) */
+#define GT_BASIS_FP64 0
+#define GT_BASIS_FP32 1
+int gt_session_open(gt_stack* stack, int32_t capacity, int32_t restart, int32_t basis, int32_t max_extra,
+                    gt_session** out);
 void gt_session_destroy(gt_session* session);
 int gt_session_submit(gt_session* session, gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_params* params,
                       int32_t* slot);
@@ -282,6 +292,9 @@ int gt_session_poll(gt_session* session, int32_t slot, gt_gmres_result* result, 
  * product scratch, before it is made. */
 int64_t gt_session_device_bytes(const gt_session* session);
 int64_t gt_session_plan_bytes(const gt_stack* stack, int32_t capacity, int32_t restart);
+/* the same for gt_session_open's arguments (-1 for invalid ones) */
+int64_t gt_session_open_bytes(const gt_stack* stack, int32_t capacity, int32_t restart, int32_t basis,
+                              int32_t max_extra);
 
 #ifdef __cplusplus
 }

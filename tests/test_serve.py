@@ -14,6 +14,16 @@ from groundtruth_amd.serve import Admission, Request, Server, Session, session_p
 from test_whatif import _extra_vias
 
 
+def test_admission_grows_a_claim_while_it_fits_and_always_for_the_only_claim():
+    a = Admission(100)
+    assert a.try_hold(90)
+    assert a.try_grow(50) and a.used == 140        # the only claim: granted past the budget
+    a.shrink(50)
+    assert a.try_hold(5)
+    assert not a.try_grow(10) and a.try_grow(5) and a.used == 100
+    assert a.held == 2
+
+
 def test_admission_grants_while_the_budget_holds_and_always_when_empty():
     a = Admission(100)
     assert a.try_hold(150)                  # nothing held: granted however large
@@ -32,7 +42,8 @@ def test_admission_grants_while_the_budget_holds_and_always_when_empty():
 def test_session_entry_points_are_exported():
     L = _lib.lib()
     for name in ("gt_session_create", "gt_session_destroy", "gt_session_submit", "gt_session_step",
-                 "gt_session_poll", "gt_session_device_bytes", "gt_session_plan_bytes"):
+                 "gt_session_poll", "gt_session_device_bytes", "gt_session_plan_bytes", "gt_session_open",
+                 "gt_session_open_bytes"):
         assert hasattr(L, name)
 
 
@@ -94,12 +105,87 @@ def test_session_plan_bytes_are_the_bytes_it_holds():
     b = _board()
     st = peec.PlaneStack(b.layers, b.branches)
     st.handle
-    for cap in (1, 5, 16):
-        plan = session_plan_bytes(st, cap)
+    for cap, prec, mx in ((1, "fp64", 0), (5, "fp32-basis", 3), (16, "fp64", 8), (16, "fp32-basis", 12)):
+        plan = session_plan_bytes(st, cap, precision=prec, max_extra=mx)
         before = st.device_bytes()
-        ses = Session(st, capacity=cap)
-        assert plan == ses.device_bytes() + (st.device_bytes() - before)
+        ses = Session(st, capacity=cap, precision=prec, max_extra=mx)
+        assert plan == ses.device_bytes() + (st.device_bytes() - before), (cap, prec, mx)
         ses.close()
+
+
+def _run(ses, jobs, tol):
+    """Submit every (stack, excitation, f, b, x, |b|) job at once, step the session until all are done."""
+    slots = {}
+    for i, (s_, _, f, bd, xd, bn) in enumerate(jobs):
+        slot = ses.submit(s_.operator(f), bd, xd, tol, bnorm=bn)
+        assert slot >= 0
+        slots[slot] = i
+    out, steps = {}, 0
+    while slots:
+        ses.step(1)
+        steps += 1
+        for slot in list(slots):
+            res = ses.poll(slot)
+            if res is not None:
+                out[slots.pop(slot)] = res
+        assert steps < 5000
+    return out
+
+
+@pytest.mark.gpu
+def test_a_board_session_batches_whatifs_of_different_sizes_bit_identical_to_each_alone():
+    """The board and three what-ifs (1, 2 and 3 added vias: four vector lengths) at two frequencies, all in one
+    session's batch: each solve gives the bits of the same solve alone on its own stack."""
+    import torch
+    b = _board()
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    stacks = [st] + [st.with_branches(_extra_vias(b, k, 300 + k)) for k in (1, 2, 3)]
+    assert all(s_._base is st for s_ in stacks[1:])
+    assert len({s_.nb for s_ in stacks}) == 4
+    fs = [float(b.freqs[1]), float(b.freqs[3])]
+    jobs = []
+    for s_ in stacks:
+        e = ex if s_ is st else ex.on(s_)
+        for f in fs:
+            jobs.append((s_, e, f) + _rhs(s_, e, f))
+    torch.cuda.synchronize()
+    ses = Session(st, capacity=8, max_extra=4)
+    out = _run(ses, jobs, 1e-10)
+    for i, (s_, e, f, bd, xd, bn) in enumerate(jobs):
+        one = e.solve(f, tol=1e-10)
+        assert out[i].converged and out[i].iterations == one.iterations, i
+        assert np.array_equal(xd.cpu().numpy(), one.x), i
+    # a what-if with more added branches than the session holds is refused
+    big = st.with_branches(_extra_vias(b, 5, 400))
+    bd, xd, bn = _rhs(big, ex.on(big), fs[0])
+    torch.cuda.synchronize()
+    with pytest.raises(ValueError):
+        ses.submit(big.operator(fs[0]), bd, xd, 1e-10)
+    ses.close()
+
+
+@pytest.mark.gpu
+def test_an_fp32_basis_session_gives_the_fp32_basis_batch_bits():
+    import torch
+    b = _board()
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    d = st.with_branches(_extra_vias(b, 2, 500))
+    fs = [float(f) for f in b.freqs[:3]]
+    jobs = [(st, ex, f) + _rhs(st, ex, f) for f in fs] + [(d, ex.on(d), f) + _rhs(d, ex.on(d), f) for f in fs]
+    jobs = [(s_, e, f, bd, xd, 0.0) for (s_, e, f, bd, xd, _) in jobs]       # |b| on the device, as solve_many
+    torch.cuda.synchronize()
+    ses = Session(st, capacity=8, precision="fp32-basis", max_extra=2)
+    out = _run(ses, jobs, 1e-10)
+    ref = ex.solve_many(fs, tol=1e-10, precision="fp32-basis") + ex.on(d).solve_many(fs, tol=1e-10,
+                                                                                   precision="fp32-basis")
+    for i, (one, job) in enumerate(zip(ref, jobs)):
+        assert out[i].converged and out[i].iterations == one.iterations, i
+        assert np.array_equal(job[4].cpu().numpy(), one.x), i
+    ses.close()
 
 
 @pytest.mark.gpu
