@@ -16,7 +16,9 @@ WRITE_SIZE runs committed under profiles/ (launches of 8 right-hand sides), scal
 right-hand sides. Beside it, a single-right-hand-side product timed over 200 launches, and the oracle's CPU time on a
 bounded sample of the same workload. Then the served what-if stream (serve_stream): p50/p99 latency and solves/s of
 the continuous-batching server, beside the same stream solved one solve at a time; and the what-if latency at 512^2
-cells per layer (serve_whatif_512), the north star's p99 < 200 ms case.
+cells per layer (serve_whatif_512), the north star's p99 < 200 ms case; and the north star's throughput comparison
+(serve_backlog): a backlog of what-ifs on the workload board served warm, batched and with FP32 Krylov bases, against
+the plain FP64 GPU solver one solve at a time.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 python bench.py --lp-only [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N right-hand sides,
@@ -243,6 +245,85 @@ def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
     return out
 
 
+BACKLOG_REQUESTS = 40          # the throughput case: what-if requests on the workload board, submitted at once
+
+
+def serve_backlog(n_req: int = BACKLOG_REQUESTS, seed: int = 7) -> dict:
+    """The north star's throughput comparison on the workload board (two_layer, 2 x 256^2 cells): a backlog of
+    what-if requests (1-2 added stitching vias at random copper cells, 1-2 of the board's frequencies each, GMRES to
+    the workload's 1e-10) served by the continuous-batching server -- the board resident and primed, the what-ifs
+    warm-started from its solutions and batched together in its session -- with the Krylov bases in FP32 and in
+    FP64, against the same solves by the plain FP64 GPU solver one at a time from zero
+    (RouteExcitation.on(what-if).solve(f)). Each run is timed from the first submission to the last result; a sample
+    of the served solutions is compared with FP64 solves of the same systems at 1e-13."""
+    import torch
+
+    from groundtruth_amd import boards, peec
+    from groundtruth_amd.serve import Request, Server
+    b = boards.two_layer()
+    st = peec.PlaneStack(b.layers, b.branches, device=0)
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n_req):
+        extra = []
+        for e in range(int(rng.integers(1, 3))):
+            i, j = (int(v) for v in rng.integers(1, st.nx - 1, 2))
+            extra += boards._stitch(b.layers, i, j, f"backlog{k}_{e}")
+        fs = [float(f) for f in rng.choice(b.freqs, size=int(rng.integers(1, 3)), replace=False)]
+        specs.append((extra, fs))
+    n_solves = sum(len(fs) for _, fs in specs)
+    out = {"requests": n_req, "solves": n_solves, "tol": b.tol, "unknowns": st.nb + st.n_keep}
+    served_x = {}
+    for precision in ("fp32-basis", "fp64"):
+        srv = Server(capacity=16, precision=precision, restart=b.restart)
+        srv.add_board("b", st, routes=b.routes)
+        srv.prime("b", b.freqs, tol=b.tol)
+        srv.submit(Request("b", b.routes[0], [b.freqs[0]], extra=boards._stitch(b.layers, 3, 3, "warm-up"),
+                           tol=b.tol))
+        srv.drain()
+        srv.reset_stats()
+        reqs = [Request("b", b.routes[0], fs, extra=extra, tol=b.tol) for extra, fs in specs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q in reqs:
+            srv.submit(q)
+        srv.drain()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sstats = srv.stats()
+        its = [s.iterations for q in reqs if q.error is None for s in q.solutions]
+        out[precision] = {"solves_per_s": n_solves / dt, "span_s": dt, "failed": sstats["failed"],
+                          "iterations_mean": float(np.mean(its)) if its else None,
+                          "warm_starts": sstats["warm_starts"], "p50_ms": sstats["p50_ms"],
+                          "p99_ms": sstats["p99_ms"]}
+        if precision == "fp32-basis":
+            served_x = {(k, f): s.x for k, q in enumerate(reqs[:3]) for f, s in zip(q.fs, q.solutions)}
+        srv.close()
+    # the plain FP64 GPU solver, one solve at a time from zero
+    ex = peec.RouteExcitation(st, b.routes[0].trace, b.routes[0].driver, b.routes[0].receiver)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for extra, fs in specs:
+        e = ex.on(st.with_branches(extra))
+        for f in fs:
+            its.append(e.solve(f, tol=b.tol, restart=b.restart).iterations)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["plain_fp64_one_at_a_time"] = {"solves_per_s": n_solves / dt, "span_s": dt,
+                                       "iterations_mean": float(np.mean(its))}
+    for precision in ("fp32-basis", "fp64"):
+        out[precision]["speedup_vs_plain"] = out[precision]["solves_per_s"] / (n_solves / dt)
+    errs = []
+    for (k, f), x in served_x.items():
+        ref = ex.on(st.with_branches(specs[k][0])).solve(f, tol=1e-13, restart=b.restart)
+        errs.append(float(np.linalg.norm(x - ref.x) / np.linalg.norm(ref.x)))
+    out["fp32-basis"]["rel_error_sample"] = errs
+    out["error_reference"] = "FP64 solves of the same systems at tol 1e-13"
+    out["target_speedup"] = 10.0
+    return out
+
+
 WHATIF_RATE = 4.0              # the 512^2 what-if stream: Poisson arrivals per second
 WHATIF_SECONDS = 12.0
 
@@ -339,7 +420,8 @@ def main():
         lp_only(args)
         return
     if args.serve_only:
-        print(json.dumps({"served": serve_stream(args.serve_rate, args.serve_seconds),
+        print(json.dumps({"served_backlog": serve_backlog(),
+                          "served": serve_stream(args.serve_rate, args.serve_seconds),
                           "served_one_at_a_time": serve_stream(args.serve_rate, args.serve_seconds, batched=False)}))
         return
     import torch
@@ -504,6 +586,7 @@ def main():
                                     "stream solved in arrival order one solve at a time; what-ifs start from the "
                                     "board solution at f (the boards primed before the stream, untimed)")
         out["whatif_512"] = serve_whatif_512()
+        out["served_backlog"] = serve_backlog()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])
     print(json.dumps(out))

@@ -481,29 +481,37 @@ class PlaneStack:
         import torch
         n = self.nb + self.n_keep
         Gs = [g for g in guess if g is not None and g.numel() == n]
+        if not Gs:
+            return None, rhs
         with torch.cuda.stream(self.torch_stream()):
-            Gs = [g for g in Gs if bool(torch.any(g != 0))]
-            if not Gs:
-                return None, rhs
             KG = torch.empty((len(Gs), n), dtype=torch.complex128, device=rhs.device)
             for i, g in enumerate(Gs):
                 _lib.check(_lib.lib().gt_matvec(op.handle, C.c_void_p(g.data_ptr()), C.c_void_p(KG[i].data_ptr())),
                            "gt_matvec")
             m = len(Gs)
-            # the normal equations (KG^H KG) c = KG^H rhs by dot products over the n unknowns
+            # one read back: the normal equations (KG^H KG) c = KG^H rhs, |rhs|^2 and each guess's |g|^2
             dots = [torch.vdot(KG[i], KG[j]) for i in range(m) for j in range(m)]
             dots += [torch.vdot(KG[i], rhs) for i in range(m)]
+            dots += [torch.vdot(rhs, rhs)] + [torch.vdot(g, g) for g in Gs]
             v = torch.stack(dots).cpu().numpy()
-            A, y = v[: m * m].reshape(m, m), v[m * m:]
-            c, *_ = np.linalg.lstsq(A, y, rcond=None)
+        A, y = v[: m * m].reshape(m, m), v[m * m: m * m + m]
+        bb, gg = float(v[m * m + m].real), v[m * m + m + 1:].real
+        keep = [i for i in range(m) if gg[i] != 0]          # all-zero guesses take no part
+        if not keep:
+            return None, rhs
+        A, y = A[np.ix_(keep, keep)], y[keep]
+        c, *_ = np.linalg.lstsq(A, y, rcond=None)
+        # |rhs - KG c|^2 = |rhs|^2 - 2 Re(c^H KG^H rhs) + c^H (KG^H KG) c
+        r2 = bb - 2 * float(np.real(np.vdot(c, y))) + float(np.real(np.vdot(c, A @ c)))
+        if not math.sqrt(max(r2, 0.0)) < 0.9 * math.sqrt(bb):
+            return None, rhs
+        with torch.cuda.stream(self.torch_stream()):
             r = rhs.clone()
             x0 = torch.zeros_like(rhs)
-            for i in range(m):
-                r.sub_(KG[i], alpha=complex(c[i]))
-                x0.add_(Gs[i], alpha=complex(c[i]))
-            if float(torch.linalg.vector_norm(r)) < 0.9 * float(torch.linalg.vector_norm(rhs)):
-                return x0, r
-        return None, rhs
+            for ci, i in zip(c, keep):
+                r.sub_(KG[i], alpha=complex(ci))
+                x0.add_(Gs[i], alpha=complex(ci))
+        return x0, r
 
     def map_state_dev(self, x, stack: "PlaneStack"):
         """map_state on a device vector (torch)."""
@@ -732,6 +740,32 @@ class PlaneStack:
         return m
 
 
+class _LazySolution(Solution):
+    """A Solution whose grids (I_x, I_y, V) and moment are unpacked from the solver vector on first read."""
+
+    def __init__(self, ex: "RouteExcitation", f: float, Ib: np.ndarray, Vk: np.ndarray, its: int, x: np.ndarray):
+        st = ex.stack
+        self.f = f
+        self.I_branch = Ib[st.ns:].copy()
+        self.Z_loop = ex.z_loop(f, Ib, Vk)
+        self.iterations = its
+        self.x = x
+        self.residual = float("nan")
+        self._src = (ex, Ib, Vk)
+        self._full = None
+
+    def _unpacked(self) -> Solution:
+        if self._full is None:
+            ex, Ib, Vk = self._src
+            self._full = ex.unpack(self.f, Ib, Vk, self.iterations, self.x)
+        return self._full
+
+    I_x = property(lambda self: self._unpacked().I_x)
+    I_y = property(lambda self: self._unpacked().I_y)
+    V = property(lambda self: self._unpacked().V)
+    m = property(lambda self: self._unpacked().m)
+
+
 def _dist_to_segment(P: np.ndarray, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     ab = b - a
     L2 = float(ab @ ab)
@@ -838,16 +872,31 @@ class RouteExcitation:
                                      restart=restart, precision=precision)
         return [self.unpack(f, *s) for f, s in zip(fs, sols)]
 
-    def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
+    def z_loop(self, f: float, Ib: np.ndarray, V: np.ndarray) -> complex:
+        """Z_loop of a solution (peec.py:2787-2800): the trace's resistance and the injection nodes' potentials, plus
+        j w (L_trace + psi . Ib)."""
         st = self.stack
         w = 2 * math.pi * f
+        v_inj = sum(amps * V[n] for n, amps in self.injections)
+        return complex(trace_resistance(self.trace, f, st.sigma, self.trace_t) + v_inj
+                       + 1j * w * (self.L_trace + np.dot(self.psi, Ib)))
+
+    def unpack_lazy(self, f: float, x: np.ndarray, its: int) -> "Solution":
+        """The Solution of the solver vector x with Z_loop and the branch currents made now and the strip-current
+        and potential grids and the moment made when first read."""
+        st = self.stack
+        Ib = x[:st.nb]
+        V = np.zeros(st.n_nodes, complex)
+        V[st.keep] = x[st.nb:]
+        return _LazySolution(self, f, Ib, V, its, x)
+
+    def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
+        st = self.stack
         I_x = np.zeros((st.N, st.nx - 1, st.ny), complex)
         I_y = np.zeros((st.N, st.nx, st.ny - 1), complex)
         I_x[tuple(st.xa.T)] = Ib[:st.nxb]
         I_y[tuple(st.ya.T)] = Ib[st.nxb: st.ns]
         Vg = np.zeros((st.N, st.nx, st.ny), complex)
         Vg[np.nonzero(st.node)] = V[:st.n_grid]
-        v_inj = sum(amps * V[n] for n, amps in self.injections)
-        Z = complex(trace_resistance(self.trace, f, st.sigma, self.trace_t) + v_inj
-                    + 1j * w * (self.L_trace + np.dot(self.psi, Ib)))
+        Z = self.z_loop(f, Ib, V)
         return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, st.moment(self.trace, Ib), its, x)

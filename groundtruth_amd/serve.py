@@ -457,9 +457,8 @@ class Server:
                     self._board_x[(req.board, id(req.route), PlaneStack.freq_key(s.f))] = xd
                 with torch.cuda.stream(st.torch_stream()):
                     xv = xd.cpu().numpy()
-                V = np.zeros(st.n_nodes, complex)
-                V[st.keep] = xv[st.nb:]
-                sol = s.ex.unpack(s.f, xv[:st.nb], V, int(res.iterations), xv)
+                # Z_loop now, the grids when first read
+                sol = s.ex.unpack_lazy(s.f, xv, int(res.iterations))
                 sol.residual = float(res.residual)
                 req.solutions[s.k] = sol
             self._open[id(req)] -= 1

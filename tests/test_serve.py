@@ -39,6 +39,23 @@ def test_admission_grants_while_the_budget_holds_and_always_when_empty():
     assert not b.try_hold(1)
 
 
+def test_a_served_solution_unpacks_its_grids_lazily_to_the_eager_values():
+    b = boards.scaled("two_layer", 16, n_vias=3, n_freqs=1)
+    st = peec.PlaneStack(b.layers, b.branches, setup="host")
+    r = b.routes[0]
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(st.nb + st.n_keep) + 1j * rng.standard_normal(st.nb + st.n_keep)
+    V = np.zeros(st.n_nodes, complex)
+    V[st.keep] = x[st.nb:]
+    eager = ex.unpack(1e8, x[:st.nb], V, 7, x)
+    lazy = ex.unpack_lazy(1e8, x, 7)
+    assert isinstance(lazy, peec.Solution) and lazy.Z_loop == eager.Z_loop and lazy.iterations == 7
+    for name in ("I_x", "I_y", "V", "m", "I_branch"):
+        assert np.array_equal(getattr(lazy, name), getattr(eager, name)), name
+    assert np.array_equal(lazy.state_for(st), eager.state_for(st))
+
+
 def test_session_entry_points_are_exported():
     L = _lib.lib()
     for name in ("gt_session_create", "gt_session_destroy", "gt_session_submit", "gt_session_step",
