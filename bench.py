@@ -248,7 +248,7 @@ def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
 BACKLOG_REQUESTS = 40          # the throughput case: what-if requests on the workload board, submitted at once
 
 
-def serve_backlog(n_req: int = BACKLOG_REQUESTS, seed: int = 7) -> dict:
+def serve_backlog(n_req: int = BACKLOG_REQUESTS, seed: int = 7, prepare_thread: bool = True) -> dict:
     """The north star's throughput comparison on the workload board (two_layer, 2 x 256^2 cells): a backlog of
     what-if requests (1-2 added stitching vias at random copper cells, 1-2 of the board's frequencies each, GMRES to
     the workload's 1e-10) served by the continuous-batching server -- the board resident and primed, the what-ifs
@@ -275,7 +275,7 @@ def serve_backlog(n_req: int = BACKLOG_REQUESTS, seed: int = 7) -> dict:
     out = {"requests": n_req, "solves": n_solves, "tol": b.tol, "unknowns": st.nb + st.n_keep}
     served_x = {}
     for precision in ("fp32-basis", "fp64"):
-        srv = Server(capacity=16, precision=precision, restart=b.restart)
+        srv = Server(capacity=16, precision=precision, restart=b.restart, prepare_thread=prepare_thread)
         srv.add_board("b", st, routes=b.routes)
         srv.prime("b", b.freqs, tol=b.tol)
         srv.submit(Request("b", b.routes[0], [b.freqs[0]], extra=boards._stitch(b.layers, 3, 3, "warm-up"),

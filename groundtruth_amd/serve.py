@@ -23,6 +23,7 @@ import ctypes as C
 import math
 import time
 from collections import OrderedDict, deque
+from concurrent.futures import FIRST_COMPLETED, Future, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -223,7 +224,7 @@ class Server:
 
     def __init__(self, device: int = 0, budget: Optional[float] = None, capacity: int = BATCH,
                  steps_per_turn: int = 1, restart: Optional[int] = None, precision: str = "fp64",
-                 max_extra: int = 8):
+                 max_extra: int = 8, prepare_thread: bool = True):
         import torch
         _basis(precision)
         self.device = device
@@ -247,6 +248,11 @@ class Server:
         self.warm_starts = 0
         self.t0: Optional[float] = None
         self._board_x: dict = {}                    # (board, route id, f key) -> the board's latest x (device)
+        # a request's set-up (its what-if stack, right-hand sides, operators, warm starts) runs on a worker thread
+        # while the sessions step: the library's calls release the interpreter, and its own lock orders them
+        self._worker = ThreadPoolExecutor(max_workers=1, thread_name_prefix="gt-serve-prep") if prepare_thread else None
+        self._prepping: deque = deque()             # (request, future of its prepared solves)
+        self._prepared: deque = deque()             # (request, stack, solves, warm starts) waiting for admission
 
     # ---------------------------------------------------------------- boards and requests
     def add_board(self, name: str, stack: PlaneStack, routes=()):
@@ -290,7 +296,7 @@ class Server:
         self._pending.append(req)
 
     def busy(self) -> bool:
-        return bool(self._pending or self._open)
+        return bool(self._pending or self._prepping or self._prepared or self._open)
 
     # ---------------------------------------------------------------- the loop
     def _stack_for(self, req: Request) -> PlaneStack:
@@ -357,14 +363,17 @@ class Server:
             sv.session.close()
         self.admission.release(sv.need)
 
-    def _split(self, req: Request, sv: _Served, stack: PlaneStack):
-        """The request's solves on `stack`, queued on the session sv: the right-hand sides of RouteExcitation.solve,
-        and for a what-if the warm start of PlaneStack.solve from the board's solution at f (peec.py:2568-2576)."""
+    def _prepare(self, req: Request):
+        """The request's solves (run on the worker thread): its stack (a what-if derived from the board), the
+        right-hand sides of RouteExcitation.solve, the operators, and for a what-if the warm start of
+        PlaneStack.solve from the board's solution at f (peec.py:2568-2576). (stack, solves, warm starts taken)."""
         import torch
         board = self.boards[req.board]
+        stack = self._stack_for(req)
+        stack.handle
         whatif = stack is not board
         ex = self._excitation(req, stack)
-        req.solutions = [None] * len(req.fs)
+        solves, warm = [], 0
         for k, f in enumerate(req.fs):
             rhs_top, J = ex.rhs(f)
             rhs = np.concatenate([np.asarray(rhs_top, complex), np.asarray(J, float)[stack.keep].astype(complex)])
@@ -377,21 +386,23 @@ class Server:
                 if g is not None:
                     torch.cuda.synchronize(self.device)
                     x0, b = stack.warm_start_dev(op, b, [g])
-                    self.warm_starts += x0 is not None
+                    warm += x0 is not None
             x = torch.empty_like(b)
-            sv.waiting.append(_Solve(req, k, f, ex, op, b, x, float(np.linalg.norm(rhs)), x0, stack))
-        self._open[id(req)] = len(req.fs)
+            solves.append(_Solve(req, k, f, ex, op, b, x, float(np.linalg.norm(rhs)), x0, stack))
+        torch.cuda.synchronize(self.device)         # the right-hand sides are on the device
+        return stack, solves, warm
 
-    def turn(self) -> list:
-        """Admit, step every session with work once, collect. The requests completed in this turn."""
-        import torch
-        # requests whose stack's session is (or can be) admitted
+    def _fail(self, req: Request, e: Exception):
+        req.error = str(e)
+        req.done_at = time.perf_counter()
+        self.completed.append(req)
+
+    def _admit_prepared(self):
+        """Queue the prepared requests whose session (and what-if stack bytes) the admission grants."""
         still = deque()
-        split = False
-        while self._pending:
-            req = self._pending.popleft()
+        while self._prepared:
+            req, stack, solves, warm = item = self._prepared.popleft()
             try:
-                stack = self._stack_for(req)
                 host = self._host(req, stack)
                 sv = self._admit_served(host, len(req.fs))
                 if sv is not None and host is not stack:
@@ -402,19 +413,43 @@ class Server:
                     else:
                         sv = None
             except (SolverError, ValueError, NotImplementedError) as e:
-                req.error = str(e)
-                req.done_at = time.perf_counter()
-                self.completed.append(req)
+                self._fail(req, e)
                 continue
             if sv is None:
-                still.append(req)
+                still.append(item)
                 continue
-            self._split(req, sv, stack)
-            split = True
-        self._pending = still
-        if split:
-            torch.cuda.synchronize(self.device)        # the new right-hand sides are on the device
+            req.solutions = [None] * len(req.fs)
+            sv.waiting.extend(solves)
+            self.warm_starts += warm
+            self._open[id(req)] = len(req.fs)
+        self._prepared = still
+
+    def turn(self) -> list:
+        """Hand new requests to the set-up worker, admit the prepared ones, step every session with work once,
+        collect. The requests completed in this turn."""
+        while self._pending:
+            req = self._pending.popleft()
+            if self._worker is not None:
+                self._prepping.append((req, self._worker.submit(self._prepare, req)))
+            else:
+                fut = Future()
+                try:
+                    fut.set_result(self._prepare(req))
+                except Exception as e:          # noqa: BLE001 - handed on as the future's exception
+                    fut.set_exception(e)
+                self._prepping.append((req, fut))
+        while self._prepping and self._prepping[0][1].done():
+            req, fut = self._prepping.popleft()
+            e = fut.exception()
+            if e is not None:
+                if not isinstance(e, (SolverError, ValueError, NotImplementedError)):
+                    raise e
+                self._fail(req, e)
+                continue
+            self._prepared.append((req,) + fut.result())
+        self._admit_prepared()
         finished = []
+        stepped = False
         for key in list(self._served):
             sv = self._served[key]
             if not sv.waiting and not sv.slots:
@@ -427,12 +462,15 @@ class Server:
                 s.slot = slot
                 sv.slots[slot] = sv.waiting.popleft()
             _, ready = sv.session.step(self.steps_per_turn)
+            stepped = True
             if ready:
                 finished += self._collect(sv)
             sv.last_used = time.perf_counter()
             self._served.move_to_end(key)
             if sv.whatif and not sv.slots and not sv.waiting:
                 self._close(key)                       # a what-if stack serves its one request
+        if not stepped and self._prepping:
+            wait([self._prepping[0][1]], timeout=0.05, return_when=FIRST_COMPLETED)
         return finished
 
     def _collect(self, sv: _Served) -> list:
@@ -480,6 +518,11 @@ class Server:
                 raise TimeoutError("requests still open after the time limit")
 
     def close(self):
+        for _, fut in self._prepping:
+            fut.cancel()
+        if self._worker is not None:
+            self._worker.shutdown(wait=True)
+        self._prepping.clear()
         for key in list(self._served):
             self._close(key)
 
