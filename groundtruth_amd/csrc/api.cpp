@@ -291,6 +291,15 @@ int gt_ctx_graphs(gt_ctx* ctx, int enable) {
     });
 }
 
+int gt_ctx_orthogonalization(gt_ctx* ctx, int mode) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        need(mode == GT_GS_CGS_DGKS || mode == GT_GS_DELAYED_CGS2, "mode is GT_GS_CGS_DGKS or GT_GS_DELAYED_CGS2");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->gs_delayed = mode == GT_GS_DELAYED_CGS2;
+    });
+}
+
 int gt_ctx_lp_timing(gt_ctx* ctx, int enable) {
     return guard([&] {
         need(ctx != nullptr, "ctx is NULL");

@@ -50,6 +50,11 @@ struct Rhs {
     int cycle = 0, its = 0, j = 0, k = 0, misses = 0, products = 0;
     std::vector<cd> H, sn, g, h;
     std::vector<double> cs;
+    // delayed CGS2: the unrotated Hessenberg columns, the last column's first-pass coefficients and the scale of its
+    // subdiagonal, and this step's t, p, u^H v and alpha
+    std::vector<cd> Hraw, sprev, dt, dp;
+    cd duv = 0;
+    double alpha_prev = 1, dalpha = 1;
 };
 
 }  // namespace
@@ -142,6 +147,7 @@ struct Batch {
     size_t ld;                             // basis row stride
     std::vector<RhsT> R;
     GmresWork& gw;
+    bool delayed;                          // delayed CGS2 (else CGS with the reference's DGKS re-pass)
     Krylov& kr;
     DevBuf<zd>& ydev;
     HostBuf<cd>& ybuf;
@@ -151,7 +157,7 @@ struct Batch {
     // hold solves of what-if stacks derived from st, each over its own length
     Batch(gt_stack* st_, int cap_, int restart_, GmresWork& gw_, int n_max = 0)
         : st(st_), s(st_->ctx->stream), n(std::max(n_max, st_->nb + st_->nk)), restart(restart_), cap(cap_),
-          ld(basis_ld(n)), R(cap_), gw(gw_),
+          ld(basis_ld(n)), R(cap_), gw(gw_), delayed(st_->ctx->gs_delayed),
           kr(gw_.kr), ydev(gw_.ydev), ybuf(gw_.ybuf), red(gw_.red) {
         gw.ensure((size_t)n, restart + 1, cap, (int)sizeof(cv));
         for (int i = 0; i < cap; ++i) {
@@ -345,73 +351,188 @@ struct Batch {
         fetch(k.nr);
     }
 
-    // One Arnoldi step of every slot inside a cycle, after the cycle starts; false when no slot is left in one.
-    bool step() {
-        // ---- cycle starts: |r|, the stopping tests, V[0] = r / |r|
+    // ---- cycle starts: |r|, the stopping tests, V[0] = r / |r|
+    void start_cycles() {
         std::vector<int> S;
         for (int i = 0; i < cap; ++i)
             if (R[i].state == START) S.push_back(i);
-        if (!S.empty()) {
-            std::vector<const cz*> rs;
-            for (int q : S) rs.push_back(R[q].r.p);
-            std::vector<double> beta;
-            norms(S, rs, beta);
-            VecOpsT<TV, TW> v{};
-            for (size_t i = 0; i < S.size(); ++i) {
-                RhsT& q = R[S[i]];
-                const double b = beta[i];
-                q.resid = b / q.bnorm;
-                if (b <= q.p.tol * q.bnorm) { q.state = DONE; q.conv = true; continue; }
-                if (q.its >= max_its_of(q) || q.cycle >= q.p.max_cycles + q.p.extra_cycles ||
-                    (q.cycle >= q.p.max_cycles && b > PROGRESS * q.last)) {
-                    q.state = DONE;
-                    q.conv = false;
-                    continue;
-                }
-                q.last = b;
-                std::fill(q.H.begin(), q.H.end(), cd(0));
-                std::fill(q.cs.begin(), q.cs.end(), 0.0);
-                std::fill(q.sn.begin(), q.sn.end(), cd(0));
-                std::fill(q.g.begin(), q.g.end(), cd(0));
-                q.g[0] = b;
-                q.j = 0;
-                q.k = 0;
-                q.aim = q.p.tol * q.bnorm;
-                q.misses = 0;
-                q.cycle++;
-                q.state = ARNOLDI;
-                v.dst[v.nr] = row(q, 0);
-                v.a[v.nr] = q.r.p;
-                v.f[v.nr] = 1.0 / b;
-                v.n[v.nr] = len(S[i]);
-                v.nr++;
+        if (S.empty()) return;
+        std::vector<const cz*> rs;
+        for (int q : S) rs.push_back(R[q].r.p);
+        std::vector<double> beta;
+        norms(S, rs, beta);
+        VecOpsT<TV, TW> v{};
+        for (size_t i = 0; i < S.size(); ++i) {
+            RhsT& q = R[S[i]];
+            const double b = beta[i];
+            q.resid = b / q.bnorm;
+            if (b <= q.p.tol * q.bnorm) { q.state = DONE; q.conv = true; continue; }
+            if (q.its >= max_its_of(q) || q.cycle >= q.p.max_cycles + q.p.extra_cycles ||
+                (q.cycle >= q.p.max_cycles && b > PROGRESS * q.last)) {
+                q.state = DONE;
+                q.conv = false;
+                continue;
             }
-            if (v.nr) k_scale(v, n, s);
+            q.last = b;
+            std::fill(q.H.begin(), q.H.end(), cd(0));
+            std::fill(q.cs.begin(), q.cs.end(), 0.0);
+            std::fill(q.sn.begin(), q.sn.end(), cd(0));
+            std::fill(q.g.begin(), q.g.end(), cd(0));
+            if (delayed) q.Hraw.assign((size_t)(restart + 1) * restart, cd(0));
+            q.g[0] = b;
+            q.j = 0;
+            q.k = 0;
+            q.aim = q.p.tol * q.bnorm;
+            q.misses = 0;
+            q.cycle++;
+            q.state = ARNOLDI;
+            v.dst[v.nr] = row(q, 0);
+            v.a[v.nr] = q.r.p;
+            v.f[v.nr] = 1.0 / b;
+            v.n[v.nr] = len(S[i]);
+            v.nr++;
         }
+        if (v.nr) k_scale(v, n, s);
+    }
+
+    // w = K M^-1 V[j] for the slots listed (a single-precision basis row widened into u for the product)
+    void product(const std::vector<int>& A) {
+        std::vector<const cz*> in;
+        std::vector<cz*> out;
+        if constexpr (std::is_same<TV, TW>::value) {
+            for (int q : A) in.push_back(row(R[q], R[q].j));
+        } else {
+            VecOpsT<TW, TV> cv{};
+            for (int q : A) {
+                cv.dst[cv.nr] = R[q].u.p;
+                cv.a[cv.nr] = row(R[q], R[q].j);
+                cv.n[cv.nr] = len(q);
+                cv.f[cv.nr++] = 1.0;
+                in.push_back(R[q].u.p);
+            }
+            k_scale(cv, n, s);
+        }
+        for (int q : A) out.push_back(R[q].w.p);
+        apply(A, in, out);
+    }
+
+    // Column jc of the Hessenberg matrix (q.h, jc + 1 values, and q.hn below them): the previous rotations, a new
+    // one, the residual estimate, and the decision: cycle end (E), true-residual check (C) or go on (Nx).
+    void rotate(int qi, int jc, std::vector<int>& C, std::vector<int>& E, std::vector<int>& Nx) {
+        RhsT& q = R[qi];
+        const int ld = restart;
+        std::vector<cd> col(jc + 2);
+        for (int i = 0; i <= jc; ++i) col[i] = q.h[i];
+        col[jc + 1] = q.hn;
+        for (int i = 0; i < jc; ++i) {
+            const cd a = col[i], c = col[i + 1];
+            col[i] = q.cs[i] * a + q.sn[i] * c;
+            col[i + 1] = -std::conj(q.sn[i]) * a + q.cs[i] * c;
+        }
+        const cd a = col[jc];
+        const double rho = std::hypot(std::abs(a), q.hn);
+        if (rho == 0.0) {
+            q.cs[jc] = 1.0;
+            q.sn[jc] = 0.0;
+        } else if (std::abs(a) == 0.0) {
+            q.cs[jc] = 0.0;
+            q.sn[jc] = 1.0;
+            col[jc] = q.hn;
+        } else {
+            const cd phase = a / std::abs(a);
+            q.cs[jc] = std::abs(a) / rho;
+            q.sn[jc] = phase * q.hn / rho;
+            col[jc] = phase * rho;
+        }
+        col[jc + 1] = 0.0;
+        for (int i = 0; i <= jc + 1; ++i) q.H[(size_t)i * ld + jc] = col[i];
+        q.g[jc + 1] = -std::conj(q.sn[jc]) * q.g[jc];
+        q.g[jc] = q.cs[jc] * q.g[jc];
+        q.its++;
+        q.k = jc + 1;
+        double hnorm = 0;
+        for (int i = 0; i <= jc; ++i) hnorm += std::norm(q.h[i]);
+        hnorm = std::sqrt(hnorm);
+        if (q.hn <= 1e-14 * hnorm || q.its >= max_its_of(q)) E.push_back(qi);
+        else if (std::abs(q.g[jc + 1]) <= q.aim) C.push_back(qi);
+        else Nx.push_back(qi);
+    }
+
+    // ---- true-residual checks: xt = x + V y, r = b - K M^-1 xt; those that go on join Nx
+    void checks(const std::vector<int>& C, std::vector<int>& Nx) {
+        if (C.empty()) return;
+        std::vector<std::vector<cd>> ys;
+        std::vector<cz*> dst;
+        std::vector<const cz*> from;
+        for (int qi : C) {
+            ys.push_back(back_substitute(R[qi], R[qi].k));
+            dst.push_back(R[qi].xt.p);
+            from.push_back(R[qi].x.p);
+        }
+        combine(C, ys, dst, from);
+        residuals(C, dst);
+        std::vector<const cz*> rs;
+        for (int qi : C) rs.push_back(R[qi].r.p);
+        std::vector<double> rt;
+        norms(C, rs, rt);
+        VecOpsT<TW> cp{};
+        for (size_t i = 0; i < C.size(); ++i) {
+            RhsT& q = R[C[i]];
+            q.resid = rt[i] / q.bnorm;
+            if (rt[i] <= q.p.tol * q.bnorm) {
+                cp.dst[cp.nr] = q.x.p;
+                cp.n[cp.nr] = len(C[i]);
+                cp.a[cp.nr++] = q.xt.p;
+                q.state = DONE;
+                q.conv = true;
+                continue;
+            }
+            q.misses++;
+            if (q.misses >= (q.p.check_misses > 0 ? q.p.check_misses : 4)) {     // restart from the check: x = xt, r = rt
+                cp.dst[cp.nr] = q.x.p;
+                cp.n[cp.nr] = len(C[i]);
+                cp.a[cp.nr++] = q.xt.p;
+                q.state = START;
+                continue;
+            }
+            q.aim = 0.5 * std::abs(q.g[q.k]);
+            Nx.push_back(C[i]);
+        }
+        if (cp.nr) k_copy(cp, n, s);
+    }
+
+    // ---- cycle ends: x += V y (least squares by back substitution), r = b - K M^-1 x
+    void cycle_ends(const std::vector<int>& E) {
+        if (E.empty()) return;
+        std::vector<std::vector<cd>> ys;
+        std::vector<cz*> dst;
+        std::vector<const cz*> from;
+        for (int qi : E) {
+            RhsT& q = R[qi];
+            int k = q.k;
+            while (k > 0 && q.H[(size_t)(k - 1) * restart + (k - 1)] == cd(0)) --k;
+            ys.push_back(k ? back_substitute(q, k) : std::vector<cd>());
+            dst.push_back(q.x.p);
+            from.push_back(q.x.p);
+            q.state = START;
+        }
+        combine(E, ys, dst, from);
+        residuals(E, dst);
+    }
+
+    // One Arnoldi step of every slot inside a cycle, after the cycle starts; false when no slot is left in one.
+    bool step() {
+        start_cycles();
         std::vector<int> A;
         for (int i = 0; i < cap; ++i)
             if (R[i].state == ARNOLDI) A.push_back(i);
         if (A.empty()) return false;
-        // ---- one Arnoldi step each: w = K M^-1 V[j], one CGS pass, |w| before and after
-        {
-            std::vector<const cz*> in;
-            std::vector<cz*> out;
-            if constexpr (std::is_same<TV, TW>::value) {
-                for (int q : A) in.push_back(row(R[q], R[q].j));
-            } else {                          // a single-precision basis row, widened into u for the product
-                VecOpsT<TW, TV> cv{};
-                for (int q : A) {
-                    cv.dst[cv.nr] = R[q].u.p;
-                    cv.a[cv.nr] = row(R[q], R[q].j);
-                    cv.n[cv.nr] = len(q);
-                    cv.f[cv.nr++] = 1.0;
-                    in.push_back(R[q].u.p);
-                }
-                k_scale(cv, n, s);
-            }
-            for (int q : A) out.push_back(R[q].w.p);
-            apply(A, in, out);
+        if (delayed) {
+            step_delayed(A);
+            return true;
         }
+        // ---- one Arnoldi step each: w = K M^-1 V[j], one CGS pass, |w| before and after
+        product(A);
         // CGS with the DGKS re-pass where |w| fell below 0.7 of its value before the projection. Up to 128 basis rows
         // the update pass also forms the re-pass's coefficients V^H w' and the device takes the test and the re-pass
         // (its decision read back); longer bases test on the host. The path depends on the slot's own j only, so a
@@ -448,88 +569,9 @@ struct Batch {
                 q.hn = std::sqrt(val((int)i, kr.slot_w_after()));
             }
         }
-        // ---- Givens rotations and the decisions
         std::vector<int> C, E, Nx;        // true-residual checks, cycle ends, next basis vector
-        for (int qi : A) {
-            RhsT& q = R[qi];
-            const int j = q.j, ld = restart;
-            std::vector<cd> col(j + 2);
-            for (int i = 0; i <= j; ++i) col[i] = q.h[i];
-            col[j + 1] = q.hn;
-            for (int i = 0; i < j; ++i) {
-                const cd a = col[i], c = col[i + 1];
-                col[i] = q.cs[i] * a + q.sn[i] * c;
-                col[i + 1] = -std::conj(q.sn[i]) * a + q.cs[i] * c;
-            }
-            const cd a = col[j];
-            const double rho = std::hypot(std::abs(a), q.hn);
-            if (rho == 0.0) {
-                q.cs[j] = 1.0;
-                q.sn[j] = 0.0;
-            } else if (std::abs(a) == 0.0) {
-                q.cs[j] = 0.0;
-                q.sn[j] = 1.0;
-                col[j] = q.hn;
-            } else {
-                const cd phase = a / std::abs(a);
-                q.cs[j] = std::abs(a) / rho;
-                q.sn[j] = phase * q.hn / rho;
-                col[j] = phase * rho;
-            }
-            col[j + 1] = 0.0;
-            for (int i = 0; i <= j + 1; ++i) q.H[(size_t)i * ld + j] = col[i];
-            q.g[j + 1] = -std::conj(q.sn[j]) * q.g[j];
-            q.g[j] = q.cs[j] * q.g[j];
-            q.its++;
-            q.k = j + 1;
-            double hnorm = 0;
-            for (int i = 0; i <= j; ++i) hnorm += std::norm(q.h[i]);
-            hnorm = std::sqrt(hnorm);
-            if (q.hn <= 1e-14 * hnorm || q.its >= max_its_of(q)) E.push_back(qi);
-            else if (std::abs(q.g[j + 1]) <= q.aim) C.push_back(qi);
-            else Nx.push_back(qi);
-        }
-        // ---- true-residual checks: xt = x + V y, r = b - K M^-1 xt
-        if (!C.empty()) {
-            std::vector<std::vector<cd>> ys;
-            std::vector<cz*> dst;
-            std::vector<const cz*> from;
-            for (int qi : C) {
-                ys.push_back(back_substitute(R[qi], R[qi].k));
-                dst.push_back(R[qi].xt.p);
-                from.push_back(R[qi].x.p);
-            }
-            combine(C, ys, dst, from);
-            residuals(C, dst);
-            std::vector<const cz*> rs;
-            for (int qi : C) rs.push_back(R[qi].r.p);
-            std::vector<double> rt;
-            norms(C, rs, rt);
-            VecOpsT<TW> cp{};
-            for (size_t i = 0; i < C.size(); ++i) {
-                RhsT& q = R[C[i]];
-                q.resid = rt[i] / q.bnorm;
-                if (rt[i] <= q.p.tol * q.bnorm) {
-                    cp.dst[cp.nr] = q.x.p;
-                    cp.n[cp.nr] = len(C[i]);
-                    cp.a[cp.nr++] = q.xt.p;
-                    q.state = DONE;
-                    q.conv = true;
-                    continue;
-                }
-                q.misses++;
-                if (q.misses >= (q.p.check_misses > 0 ? q.p.check_misses : 4)) {     // restart from the check: x = xt, r = rt
-                    cp.dst[cp.nr] = q.x.p;
-                    cp.n[cp.nr] = len(C[i]);
-                    cp.a[cp.nr++] = q.xt.p;
-                    q.state = START;
-                    continue;
-                }
-                q.aim = 0.5 * std::abs(q.g[q.j + 1]);
-                Nx.push_back(C[i]);
-            }
-            if (cp.nr) k_copy(cp, n, s);
-        }
+        for (int qi : A) rotate(qi, R[qi].j, C, E, Nx);
+        checks(C, Nx);
         // ---- the next basis vector, V[j + 1] = w / hn; a full cycle ends
         if (!Nx.empty()) {
             VecOpsT<TV, TW> v{};
@@ -547,24 +589,112 @@ struct Batch {
                 else q.j++;
             }
         }
-        // ---- cycle ends: x += V y (least squares by back substitution), r = b - K M^-1 x
-        if (!E.empty()) {
-            std::vector<std::vector<cd>> ys;
-            std::vector<cz*> dst;
-            std::vector<const cz*> from;
-            for (int qi : E) {
-                RhsT& q = R[qi];
-                int k = q.k;
-                while (k > 0 && q.H[(size_t)(k - 1) * restart + (k - 1)] == cd(0)) --k;
-                ys.push_back(k ? back_substitute(q, k) : std::vector<cd>());
-                dst.push_back(q.x.p);
-                from.push_back(q.x.p);
-                q.state = START;
-            }
-            combine(E, ys, dst, from);
-            residuals(E, dst);
-        }
+        cycle_ends(E);
         return true;
+    }
+
+    // The delayed CGS2 step (classical Gram-Schmidt with its second pass one step late, as low-synchronisation
+    // GMRES does it): row j holds u_j, the once-projected and normalised vector, rows 0..j-1 are final. v = A u_j;
+    // pass A: t = Q^H u_j, p = Q^H v, u^H u, u^H v; alpha = |u_j - Q t| = sqrt(u^H u - |t|^2); column j - 1 of the
+    // Hessenberg matrix is then final: s_{j-1} + beta t above, beta alpha below (beta = |y_j|, y_j the vector u_j was
+    // normalised from); pass B: q_j = (u_j - Q t) / alpha over row j and y_{j+1} = v - Q p - q_j (q_j^H v), whose norm
+    // scales row j + 1 on the device; s_j = ([p ; q_j^H v] - H[0:j+1, 0:j] t) / alpha. The basis is read twice per
+    // step (three times with the re-pass). Every slot keeps its own row in the reductions (the set is indexed by
+    // slot), so |y|^2 stays on the device from one step to the next.
+    void step_delayed(const std::vector<int>& A) {
+        std::vector<int> Am;
+        for (int qi : A)
+            if (R[qi].j < restart) Am.push_back(qi);
+        if (!Am.empty()) product(Am);
+        KrSetT<TV, TW> k{};
+        k.nr = cap;
+        k.ld = (int)ld;
+        for (int i = 0; i < cap; ++i) k.m[i] = -1;
+        for (int qi : A) {
+            RhsT& q = R[qi];
+            k.V[qi] = q.V.p;
+            k.m[qi] = q.j;
+            k.w[qi] = q.j < restart ? q.w.p : nullptr;
+            k.n[qi] = len(qi);
+        }
+        kr.dots2(k, s);
+        fetch(cap);
+        std::vector<int> C, E, Nx;
+        for (int qi : A) {
+            RhsT& q = R[qi];
+            const int j = q.j;
+            q.dt.resize(j);
+            q.dp.resize(j);
+            double tt = 0;
+            for (int r = 0; r < j; ++r) {
+                q.dt[r] = cd(val(qi, kr.slot_h2() + 2 * r), val(qi, kr.slot_h2() + 2 * r + 1));
+                q.dp[r] = cd(val(qi, 2 * r), val(qi, 2 * r + 1));
+                tt += std::norm(q.dt[r]);
+            }
+            const double uu = val(qi, kr.slot_uu());
+            q.duv = cd(val(qi, kr.slot_uu() + 1), val(qi, kr.slot_uu() + 2));
+            q.dalpha = std::sqrt(std::max(uu - tt, 0.0));
+            if (j == 0) {
+                q.g[0] *= q.dalpha;                 // r = |r| u_0 = |r| alpha q_0
+                Nx.push_back(qi);
+                continue;
+            }
+            const double beta = std::sqrt(val(qi, kr.slot_w_after())) / q.alpha_prev;
+            q.h.assign(j, 0);
+            for (int r = 0; r < j; ++r) q.h[r] = q.sprev[r] + beta * q.dt[r];
+            q.hn = beta * q.dalpha;
+            for (int r = 0; r < j; ++r) q.Hraw[(size_t)r * restart + (j - 1)] = q.h[r];
+            q.Hraw[(size_t)j * restart + (j - 1)] = q.hn;
+            rotate(qi, j - 1, C, E, Nx);
+        }
+        checks(C, Nx);
+        // pass B for the slots going on with rows left; a full basis ends its cycle
+        std::vector<int> B;
+        for (int qi : Nx) {
+            if (R[qi].j < restart && R[qi].dalpha > 0) B.push_back(qi);
+            else E.push_back(qi);
+        }
+        if (!B.empty()) {
+            KrSetT<TV, TW> kb{};
+            kb.nr = cap;
+            kb.ld = (int)ld;
+            std::vector<double> ainv(cap, 0.0);
+            std::vector<zd> qv(cap);
+            VecOpsT<TV, TW> v{};
+            v.nr = cap;
+            for (int i = 0; i < cap; ++i) kb.m[i] = -1;
+            for (int qi : B) {
+                RhsT& q = R[qi];
+                const int j = q.j;
+                cd tp = 0;
+                for (int r = 0; r < j; ++r) tp += std::conj(q.dt[r]) * q.dp[r];
+                const cd qvc = (q.duv - tp) / q.dalpha;
+                ainv[qi] = 1.0 / q.dalpha;
+                qv[qi] = zd(qvc.real(), qvc.imag());
+                // s_j = ([p ; q^H v] - H[0:j+1, 0:j] t) / alpha, with the unrotated columns
+                q.sprev.assign(j + 1, cd(0));
+                for (int r = 0; r < j; ++r) q.sprev[r] = q.dp[r];
+                q.sprev[j] = qvc;
+                for (int c = 0; c < j; ++c) {
+                    const cd tc = q.dt[c];
+                    if (tc == cd(0)) continue;
+                    for (int r = 0; r <= c + 1 && r <= j; ++r) q.sprev[r] -= q.Hraw[(size_t)r * restart + c] * tc;
+                }
+                for (int r = 0; r <= j; ++r) q.sprev[r] /= q.dalpha;
+                q.alpha_prev = q.dalpha;
+                kb.V[qi] = q.V.p;
+                kb.m[qi] = j;
+                kb.w[qi] = q.w.p;
+                kb.n[qi] = len(qi);
+                v.dst[qi] = row(q, j + 1);
+                v.a[qi] = q.w.p;
+                v.n[qi] = len(qi);
+            }
+            kr.update2(kb, ainv.data(), qv.data(), s);
+            kr.scale_by_norm(v, s);
+            for (int qi : B) R[qi].j++;
+        }
+        cycle_ends(E);
     }
 
     void run() {

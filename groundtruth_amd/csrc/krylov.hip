@@ -8,7 +8,6 @@ namespace gt {
 namespace {
 
 constexpr int NTH = 256;
-constexpr int G = 16;             // basis rows per sweep of the multi-dot
 
 // a stored value as double
 __device__ __forceinline__ zd ldz(const zd& v) { return v; }
@@ -300,6 +299,177 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<TV, TW> k, int n, int ch
     if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot] = s;
 }
 
+
+// ---- delayed classical Gram-Schmidt with re-orthogonalisation (DCGS2): per Arnoldi step one multi-dot pass and one
+// update pass over the basis. Row m of a basis holds u, the once-projected and normalised vector the product was
+// taken of (v = A u); rows 0..m-1 are final.
+
+// Pass A: t = Q^H u and p = Q^H v over the rows [0, m) in groups of DG2 rows per workgroup, and u^H u, u^H v (group 0).
+// v may be absent (a basis's last row: t and u^H u only).
+template <class TV, class TW>
+__global__ __launch_bounds__(NTH) void kr_dots2(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
+                                                int slot_t, int slot_uu) {
+    constexpr int E = lane_elems<TV>();
+    constexpr int DG2 = 8;
+    __shared__ double red[NTH / 64][4 * DG2 + 3];
+    const int p = blockIdx.y;
+    const int grp = blockIdx.x / nblk, blk = blockIdx.x - grp * nblk;
+    const int m = k.m[p];
+    const int g0 = grp * DG2;
+    if (m < 0 || (g0 > 0 && g0 >= m)) return;     // m < 0: a slot not in this step
+    const int gm = max(0, min(DG2, m - g0));
+    const cpx<TV>* __restrict__ V = k.V[p] + (size_t)g0 * k.ld;
+    const cpx<TV>* __restrict__ u = k.V[p] + (size_t)m * k.ld;
+    const cpx<TW>* __restrict__ v = k.w[p];
+    const bool hv = v != nullptr;
+    if (k.n[p] > 0) n = k.n[p];
+    const int i0 = blk * chunk, i1 = min(n, i0 + chunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double tr[DG2], ti[DG2], pr[DG2], pi[DG2];
+#pragma unroll
+    for (int r = 0; r < DG2; ++r) tr[r] = ti[r] = pr[r] = pi[r] = 0;
+    double uu = 0, uvr = 0, uvi = 0;
+    for (int i = i0 + E * threadIdx.x; i < i1; i += E * NTH) {
+        const bool in2 = i + 1 < i1;
+        zd uv_[E], vv_[E];
+        ld_basis<TV>(u + i, true, in2, uv_);
+#pragma unroll
+        for (int c = 0; c < E; ++c) vv_[c] = hv && (c == 0 || in2) ? ldz(v[i + c]) : zd();
+        zd q[DG2][E];
+#pragma unroll
+        for (int r = 0; r < DG2; ++r) ld_basis<TV>(V + (size_t)r * k.ld + i, r < gm, in2, q[r]);
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            uu += uv_[c].re * uv_[c].re + uv_[c].im * uv_[c].im;
+            uvr += uv_[c].re * vv_[c].re + uv_[c].im * vv_[c].im;
+            uvi += uv_[c].re * vv_[c].im - uv_[c].im * vv_[c].re;
+#pragma unroll
+            for (int r = 0; r < DG2; ++r) {
+                tr[r] += q[r][c].re * uv_[c].re + q[r][c].im * uv_[c].im;
+                ti[r] += q[r][c].re * uv_[c].im - q[r][c].im * uv_[c].re;
+                pr[r] += q[r][c].re * vv_[c].re + q[r][c].im * vv_[c].im;
+                pi[r] += q[r][c].re * vv_[c].im - q[r][c].im * vv_[c].re;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < DG2; ++r) {
+        tr[r] = wave_sum(tr[r]);
+        ti[r] = wave_sum(ti[r]);
+        pr[r] = wave_sum(pr[r]);
+        pi[r] = wave_sum(pi[r]);
+    }
+    uu = wave_sum(uu);
+    uvr = wave_sum(uvr);
+    uvi = wave_sum(uvi);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < DG2; ++r) {
+            red[wave][2 * r] = tr[r];
+            red[wave][2 * r + 1] = ti[r];
+            red[wave][2 * DG2 + 2 * r] = pr[r];
+            red[wave][2 * DG2 + 2 * r + 1] = pi[r];
+        }
+        red[wave][4 * DG2] = uu;
+        red[wave][4 * DG2 + 1] = uvr;
+        red[wave][4 * DG2 + 2] = uvi;
+    }
+    __syncthreads();
+    double* out = part + ((size_t)p * nblk + blk) * P;
+    const int t = threadIdx.x;
+    const auto sum = [&](int c) { return red[0][c] + red[1][c] + red[2][c] + red[3][c]; };
+    if (t < 2 * gm) out[slot_t + 2 * g0 + t] = sum(t);
+    else if (t >= 64 && t < 64 + 2 * gm) out[2 * g0 + t - 64] = sum(2 * DG2 + t - 64);
+    else if (grp == 0 && t >= 128 && t < 131) out[slot_uu + t - 128] = sum(4 * DG2 + t - 128);
+}
+
+// Pass B: q = (u - Q t) / alpha written over u (row m), and y = v - Q p - q (q^H v) written over v, |y|^2 after; one
+// read of the rows [0, m). t and p as reduced by pass A; 1 / alpha and q^H v per right-hand side.
+struct DcgsCoef {
+    double ainv[KMAX];
+    zd qv[KMAX];
+};
+
+template <class TV, class TW>
+__global__ __launch_bounds__(NTH) void kr_update2(KrSetT<TV, TW> k, int n, int chunk, const double* red, double* part,
+                                                  int P, int nblk, int slot_t, int slot_y, DcgsCoef cf) {
+    constexpr int E = lane_elems<TV>();
+    constexpr int UR = E == 1 ? 4 : 8;
+    __shared__ double lds[NTH / 64];
+    __shared__ double tp_raw[2048];
+    zd* ts = reinterpret_cast<zd*>(tp_raw);
+    zd* ps = ts + 512;
+    const int p = blockIdx.y;
+    const int m = k.m[p];
+    if (m < 0) return;                              // a slot not in this step
+    cpx<TV>* u = const_cast<cpx<TV>*>(k.V[p]) + (size_t)m * k.ld;
+    const cpx<TV>* __restrict__ V = k.V[p];
+    cpx<TW>* w = k.w[p];
+    const double* rp = red + (size_t)p * P;
+    for (int r = threadIdx.x; r < m; r += NTH) {
+        ts[r] = zd(rp[slot_t + 2 * r], rp[slot_t + 2 * r + 1]);
+        ps[r] = zd(rp[2 * r], rp[2 * r + 1]);
+    }
+    __syncthreads();
+    if (k.n[p] > 0) n = k.n[p];
+    const double ainv = cf.ainv[p];
+    const zd qv = cf.qv[p];
+    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    double nn = 0;
+    for (int i = i0 + E * threadIdx.x; i < i1; i += E * NTH) {
+        const bool in2 = i + 1 < i1;
+        zd au[E], av[E];
+        ld_basis<TV>(u + i, true, in2, au);
+#pragma unroll
+        for (int c = 0; c < E; ++c) av[c] = c == 0 || in2 ? ldz(w[i + c]) : zd();
+        for (int r0 = 0; r0 < m; r0 += UR) {
+            zd vv[UR][E];
+#pragma unroll
+            for (int r = 0; r < UR; ++r) ld_basis<TV>(V + (size_t)(r0 + r) * k.ld + i, r0 + r < m, in2, vv[r]);
+#pragma unroll
+            for (int r = 0; r < UR; ++r) {
+                const zd tr = r0 + r < m ? ts[r0 + r] : zd(), pr = r0 + r < m ? ps[r0 + r] : zd();
+#pragma unroll
+                for (int c = 0; c < E; ++c) {
+                    au[c].re -= vv[r][c].re * tr.re - vv[r][c].im * tr.im;
+                    au[c].im -= vv[r][c].re * tr.im + vv[r][c].im * tr.re;
+                    av[c].re -= vv[r][c].re * pr.re - vv[r][c].im * pr.im;
+                    av[c].im -= vv[r][c].re * pr.im + vv[r][c].im * pr.re;
+                }
+            }
+        }
+        zd qs[E];
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            const cpx<TV> qst = stz<TV>(au[c].re * ainv, au[c].im * ainv);
+            qs[c] = ldz(qst);
+            if (c == 0 || in2) u[i + c] = qst;
+        }
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            if (!(c == 0 || in2)) continue;
+            const cpx<TW> ys = stz<TW>(av[c].re - (qs[c].re * qv.re - qs[c].im * qv.im),
+                                       av[c].im - (qs[c].re * qv.im + qs[c].im * qv.re));
+            w[i + c] = ys;
+            const zd y = ldz(ys);
+            nn += y.re * y.re + y.im * y.im;
+        }
+    }
+    const double sb = block_sum(nn, lds);
+    if (threadIdx.x == 0) part[((size_t)p * nblk + blockIdx.x) * P + slot_y] = sb;
+}
+
+// dst_p = a_p / sqrt(red[p][slot]) (the next basis row, scaled by the norm pass B left on the device)
+template <class T, class TA>
+__global__ __launch_bounds__(NTH) void kr_scale_red(VecOpsT<T, TA> v, int n, const double* red, int P, int slot) {
+    const int i = blockIdx.x * NTH + threadIdx.x, p = blockIdx.y;
+    if (v.dst[p] && i < (v.n[p] > 0 ? v.n[p] : n)) {
+        const double f = 1.0 / sqrt(red[(size_t)p * P + slot]);
+        const zd a = ldz(v.a[p][i]);
+        v.dst[p][i] = stz<T>(f * a.re, f * a.im);
+    }
+}
+
 template <class TV, class TW>
 __global__ __launch_bounds__(NTH) void kr_norm(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
                                                int slot) {
@@ -472,6 +642,47 @@ void Krylov::repass(const KrSetT<TV, TW>& k, hipStream_t s) {
     GT_LAUNCH("kr_reduce");
 }
 
+
+template <class TV, class TW>
+void Krylov::dots2(const KrSetT<TV, TW>& k, hipStream_t s) {
+    int mmax = 0;
+    for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
+    if (mmax + 1 > rows_cap) fail(GT_ERR_ARG, "a basis longer than the workspace's");
+    const int groups = std::max(1, (mmax + 7) / 8);
+    hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nblk * groups, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk,
+                       slot_h2(), slot_uu());
+    GT_LAUNCH("kr_dots2");
+    if (mmax > 0) {
+        hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
+        GT_LAUNCH("kr_reduce");
+    }
+    // u^H u, u^H v and t, adjacent
+    hipLaunchKernelGGL(kr_reduce, dim3(3 + 2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_uu(), red.p);
+    GT_LAUNCH("kr_reduce");
+}
+
+template <class TV, class TW>
+void Krylov::update2(const KrSetT<TV, TW>& k, const double* ainv, const zd* qv, hipStream_t s) {
+    DcgsCoef cf{};
+    for (int p = 0; p < k.nr; ++p) {
+        cf.ainv[p] = ainv[p];
+        cf.qv[p] = qv[p];
+        if (k.m[p] + 1 > rows_cap) fail(GT_ERR_ARG, "a basis longer than the workspace's");
+    }
+    hipLaunchKernelGGL((kr_update2<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, (const double*)red.p, part.p,
+                       P, nblk, slot_h2(), slot_w_after(), cf);
+    GT_LAUNCH("kr_update2");
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after(), red.p);
+    GT_LAUNCH("kr_reduce");
+}
+
+template <class T, class TA>
+void Krylov::scale_by_norm(const VecOpsT<T, TA>& v, hipStream_t s) {
+    hipLaunchKernelGGL((kr_scale_red<T, TA>), dim3(blocks_for(n, NTH), v.nr), dim3(NTH), 0, s, v, n, (const double*)red.p,
+                       P, slot_w_after());
+    GT_LAUNCH("kr_scale_red");
+}
+
 template <class TV, class TW>
 void Krylov::norm2(const KrSetT<TV, TW>& k, hipStream_t s) {
     hipLaunchKernelGGL((kr_norm<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk, slot_norm());
@@ -537,7 +748,10 @@ void k_add_float(int nr, zd* const* dst, const cpx<float>* const* a, int n, hipS
     template void Krylov::project_update<TV, TW>(const KrSetT<TV, TW>&, hipStream_t, bool);      \
     template void Krylov::update_dots<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);               \
     template void Krylov::repass<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);                    \
-    template void Krylov::norm2<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);
+    template void Krylov::norm2<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);                      \
+    template void Krylov::dots2<TV, TW>(const KrSetT<TV, TW>&, hipStream_t);                      \
+    template void Krylov::update2<TV, TW>(const KrSetT<TV, TW>&, const double*, const zd*, hipStream_t); \
+    template void Krylov::scale_by_norm<TV, TW>(const VecOpsT<TV, TW>&, hipStream_t);
 GT_KRYLOV(double, double)
 GT_KRYLOV(float, float)
 GT_KRYLOV(float, double)

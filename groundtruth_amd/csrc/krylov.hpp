@@ -23,6 +23,9 @@ struct KrSetT {
 };
 using KrSet = KrSetT<double>;
 
+template <class T, class TA>
+struct VecOpsT;
+
 struct Krylov {
     int n = 0;
     int nblk = 0, nblk_d = 0;    // partial-sum blocks of the passes over w, and of the multi-dot, at the longest n
@@ -41,6 +44,7 @@ struct Krylov {
     int slot_norm() const { return 2 * rows_cap + 2; }    // norm2's value
     int slot_w_after2() const { return 2 * rows_cap + 3; } // |w|^2 after the re-pass
     int slot_repass() const { return 2 * rows_cap + 4; }   // 1 where the re-pass was taken, else 0
+    int slot_uu() const { return 2 * rows_cap + 5; }      // DCGS2: u^H u, then u^H v (2 values), then t at slot_h2
     int slot_h2() const { return 2 * rows_cap + 8; }      // the re-pass coefficients V^H w' (2 rows_cap values)
     // h_p = V_p[0:m_p]^H w_p and |w_p|^2
     template <class TV, class TW>
@@ -61,6 +65,16 @@ struct Krylov {
     // |w_p|^2
     template <class TV, class TW>
     void norm2(const KrSetT<TV, TW>& k, hipStream_t s);
+    // Delayed CGS2 (row m_p of V_p holds u_p, w_p holds v_p = A u_p or is null): t = V[0:m]^H u at slot_h2,
+    // p = V[0:m]^H v at 0, u^H u and u^H v at slot_uu
+    template <class TV, class TW>
+    void dots2(const KrSetT<TV, TW>& k, hipStream_t s);
+    // then q = (u - V t) ainv over row m, w = v - V p - q qv, |w|^2 at slot_w_after
+    template <class TV, class TW>
+    void update2(const KrSetT<TV, TW>& k, const double* ainv, const zd* qv, hipStream_t s);
+    // dst_p = a_p / sqrt(red[p][slot_w_after])
+    template <class T, class TA>
+    void scale_by_norm(const VecOpsT<T, TA>& v, hipStream_t s);
 };
 
 // Per right-hand side: the destination (storage T) and the operands (storage TA; the same but in conversions)

@@ -63,6 +63,15 @@ int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min);
  * replay takes as long as the launches it replaces (the levels' time is their own latency on the device, not the
  * host's submissions). (No counterpart in the reference.) */
 int gt_ctx_graphs(gt_ctx* ctx, int enable);
+/* The GMRES orthogonalisation of the solves on this context. GT_GS_CGS_DGKS: classical Gram-Schmidt with the DGKS
+ * re-pass where |w| falls below 0.7 of itself, as peec.py:3093-3224 `_gmres` (three reads of the basis per
+ * Arnoldi step when the re-pass is taken). GT_GS_DELAYED_CGS2 (the default): the same second pass, always taken and
+ * one step late, fused with the next step's projection (two reads per step; the low-synchronisation GMRES of
+ * Swirydowicz et al., Numer. Linear Algebra Appl. 28 (2021) e2343). Both give the same Krylov space; iteration
+ * counts and solutions agree to rounding. (No counterpart in the reference beyond its CGS + DGKS.) */
+#define GT_GS_CGS_DGKS 0
+#define GT_GS_DELAYED_CGS2 1
+int gt_ctx_orthogonalization(gt_ctx* ctx, int mode);
 int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms);
 
 /* ---------------------------------------------------------------- stack
