@@ -2,9 +2,10 @@
 // own operator (frequency) and right-hand side (peec.py:3093-3224 `_gmres`, called as peec.py:2581 does:
 // (K M^-1) y = b, x = M^-1 y).
 //
-// Every right-hand side follows the reference's control flow exactly (CGS with a DGKS re-pass below 0.7, Givens
-// rotations, the true-residual check at the aim with its miss budget, restarts, extra cycles while each falls to
-// 0.7), as its own state machine on the host. The machines advance in lockstep: each phase (cycle starts, Arnoldi
+// Every right-hand side follows the reference's control flow (Givens rotations, the true-residual check at the aim
+// with its miss budget, restarts, extra cycles while each falls to 0.7), as its own state machine on the host. Its
+// orthogonalisation is the reference's CGS with a DGKS re-pass below 0.7, or (the context's default) delayed CGS2:
+// the same second pass, always taken and one step late, fused with the next step's projection. The machines advance in lockstep: each phase (cycle starts, Arnoldi
 // steps, re-passes, true-residual checks, cycle ends) is one batched launch sequence over the right-hand sides
 // that are in it, so the kernel spectra, the incidence and the Schur panels are read once per batch. A right-hand
 // side's arithmetic does not depend on what else is in its batch: its result is bit-identical alone and in any mix.
