@@ -71,6 +71,7 @@ SIGNATURES = {
     "gt_stack_derive": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, C.POINTER(C.c_void_p)]),
     "gt_ctx_lp_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_schur_device_fronts": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gt_ctx_schur_leaf": (C.c_int, [C.c_void_p, C.c_int32]),
     "gt_ctx_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_orthogonalization": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_lp_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

@@ -283,6 +283,15 @@ int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min) {
     });
 }
 
+int gt_ctx_schur_leaf(gt_ctx* ctx, int32_t nodes) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        need(nodes >= 4 && nodes <= 512, "a leaf of 4 to 512 nodes");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->schur_leaf = nodes;
+    });
+}
+
 int gt_ctx_graphs(gt_ctx* ctx, int enable) {
     return guard([&] {
         need(ctx != nullptr, "ctx is NULL");
@@ -540,6 +549,7 @@ int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
         laplacian(nk, nb, st->h_a_keep.data(), st->h_b_keep.data(), ea, eb, diag);
         auto S = std::make_unique<Schur>();
         S->h.dev_min = st->ctx->schur_dev_min;
+        S->h.leaf = st->ctx->schur_leaf;
         S->h.build(nk, ea, eb, diag, node_xyz);
         S->upload(st->ctx->stream);
         S->factor_device(st->ctx->stream);

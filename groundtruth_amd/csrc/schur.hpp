@@ -70,6 +70,8 @@ struct SchurHost {
     // over what their assembly needs: the original entries (front row, front column, value; duplicates summed) and
     // the update matrices of their host children (nr x nr, column-major).
     int dev_min = INT32_MAX;
+    // nodes of a nested-dissection leaf box at most (one dense supernode at the bottom of the tree)
+    int leaf = 32;
     std::vector<char> on_dev;
     std::vector<int> dev_order;                 // device fronts in postorder
     std::vector<int64_t> trip_off;              // per device front (index into dev_order): its original entries

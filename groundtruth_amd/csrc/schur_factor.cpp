@@ -17,10 +17,10 @@ namespace gt {
 
 namespace {
 
-constexpr int ND_LEAF = 32;     // nodes in a leaf box: one dense supernode
 
 struct NDTree {
     const int32_t* xyz;
+    int leaf = 32;                          // nodes in a leaf box at most: one dense supernode
     std::vector<int> order;                 // new -> old
     std::vector<int> c0, nc;
     std::vector<std::vector<int>> kids;
@@ -39,7 +39,7 @@ struct NDTree {
         auto I = [&](int v) { return xyz[3 * v + 1]; };
         auto J = [&](int v) { return xyz[3 * v + 2]; };
         auto K = [&](int v) { return xyz[3 * v + 0]; };
-        if ((int)idx.size() <= ND_LEAF || (ilo >= ihi && jlo >= jhi)) {
+        if ((int)idx.size() <= leaf || (ilo >= ihi && jlo >= jhi)) {
             std::sort(idx.begin(), idx.end(), [&](int a, int b) {
                 if (I(a) != I(b)) return I(a) < I(b);
                 if (J(a) != J(b)) return J(a) < J(b);
@@ -88,6 +88,7 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     }
     NDTree t;
     t.xyz = xyz;
+    t.leaf = leaf;
     t.order.reserve(n);
     std::vector<int> all(n);
     std::iota(all.begin(), all.end(), 0);

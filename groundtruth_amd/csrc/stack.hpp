@@ -27,7 +27,8 @@ struct gt_ctx_s {
     std::mutex mu;
     int schur_dev_min = 256;                  // fronts of at least this many rows are factorised on the device
     bool graphs = false;                      // Schur solves captured in hipGraphs and replayed (gt_ctx_graphs)
-    bool gs_delayed = true;                   // GMRES orthogonalisation: delayed CGS2, else CGS + DGKS re-pass
+    bool gs_delayed = true;
+    int schur_leaf = 32;                      // nested-dissection leaf size of the factorisations (gt_ctx_schur_leaf)                   // GMRES orthogonalisation: delayed CGS2, else CGS + DGKS re-pass
     std::unique_ptr<gt::GmresWork> gmres;     // GMRES device memory kept between calls (gmres.cpp)
     // L_p launch timing (gt_ctx_lp_timing): an event pair on the stream around every L_p launch while enabled
     bool lp_timing = false;

@@ -133,7 +133,7 @@ class PlaneStack:
 
     def __init__(self, layers: list, branches: Optional[list] = None, capacitance: Optional[list] = None,
                  sigma: float = SIGMA_CU, precond: str = "diag", screen: bool = False, device: int = 0,
-                 setup: str = "device", device_fronts: int = 256):
+                 setup: str = "device", device_fronts: int = 256, schur_leaf: int = 32):
         if not layers:
             raise SolverError("no plane layers")
         if precond not in ("diag", "sheet"):
@@ -162,6 +162,7 @@ class PlaneStack:
             raise ValueError("setup is 'device' or 'host'")
         self.setup_mode = setup             # where the kernel spectra and the strips' psi are made
         self.device_fronts = int(device_fronts)     # Schur fronts of at least this many rows factorised on the device
+        self.schur_leaf = int(schur_leaf)           # nodes of a nested-dissection leaf box at most
         self._topology(list(branches or []))
         if screen:
             self._check_no_split()
@@ -282,7 +283,8 @@ class PlaneStack:
         b = np.array([self.node_id(b.b) for b in extra], np.int64)
         if np.any(self.full_label[a] != self.full_label[b]):
             return PlaneStack(self.layers, self.branches + extra, sigma=self.sigma, precond=self.precond_mode,
-                              device=self.device, setup=self.setup_mode, device_fronts=self.device_fronts)
+                              device=self.device, setup=self.setup_mode, device_fronts=self.device_fronts,
+                              schur_leaf=self.schur_leaf)
         base = self if self._base is None else self._base
         new = copy.copy(self)
         new.branches = self.branches + extra
@@ -370,6 +372,7 @@ class PlaneStack:
         L = _lib.lib()
         _lib.check(L.gt_ctx_schur_device_fronts(self.ctx.handle, min(self.device_fronts, 2 ** 31 - 1)),
                    "gt_ctx_schur_device_fronts")
+        _lib.check(L.gt_ctx_schur_leaf(self.ctx.handle, self.schur_leaf), "gt_ctx_schur_leaf")
         keep_idx = np.nonzero(self.keep)[0]
         xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
                                              self.node_j[keep_idx]], axis=1).astype(np.int32))

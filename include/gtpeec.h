@@ -58,6 +58,10 @@ int gt_ctx_lp_timing(gt_ctx* ctx, int enable);
  * Default 256; INT32_MAX keeps the whole factorisation on the host. (No counterpart in the reference, which
  * factorises with SuperLU on the host, peec.py:2379-2424.) */
 int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min);
+/* The nested-dissection leaf size of the factorisations made on this context: boxes of at most `nodes` nodes
+ * (4-512; default 32) are one dense supernode each. Larger leaves give a shallower tree (fewer solve levels) and
+ * denser leaf panels. (No counterpart in the reference, whose SuperLU orders by COLAMD.) */
+int gt_ctx_schur_leaf(gt_ctx* ctx, int32_t nodes);
 /* Schur solves on this context captured in hipGraphs: the level launches of each batch size are recorded once and
  * replayed with one submission; the same kernels and arguments, so the same bits. Off by default: on the MI355X a
  * replay takes as long as the launches it replaces (the levels' time is their own latency on the device, not the

@@ -57,3 +57,18 @@ def test_full_two_layer_factor_on_the_device_matches_the_host_factor():
     rng = np.random.default_rng(12)
     s = rng.standard_normal(dev.n_keep) + 1j * rng.standard_normal(dev.n_keep)
     assert _rel(dev.schur_solve(s), host.schur_solve(s)) < 1e-11
+
+
+@pytest.mark.gpu
+def test_another_dissection_leaf_size_gives_the_same_solution():
+    """gt_ctx_schur_leaf: leaves of 64 or 128 nodes (a shallower tree, denser leaf panels) factor the same matrix."""
+    b = boards.scaled("two_layer", 40, n_vias=6, n_freqs=2)
+    r = b.routes[0]
+    sols = []
+    for leaf in (32, 64, 128):
+        st = peec.PlaneStack(b.layers, b.branches, schur_leaf=leaf)
+        ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+        sols.append(ex.solve_many([float(f) for f in b.freqs], tol=1e-12))
+    for other in sols[1:]:
+        for a, c in zip(other, sols[0]):
+            assert np.linalg.norm(a.x - c.x) / np.linalg.norm(c.x) < 1e-9
