@@ -405,6 +405,16 @@ int gt_stack_create_gen(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_des
     });
 }
 
+int gt_node_field(gt_stack* st, const void* x_dev, double d, void* kx_dev, void* ky_dev) {
+    return guard([&] {
+        need(st && x_dev && kx_dev && ky_dev, "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        GT_HIP(hipSetDevice(st->ctx->device));
+        node_field_device(st, (const zd*)x_dev, d, (zd*)kx_dev, (zd*)ky_dev);
+        GT_HIP(hipStreamSynchronize(st->ctx->stream));
+    });
+}
+
 int gt_strip_coupling(gt_stack* st, double d, double mu, int32_t nseg, const gt_segment* segs, const double* sheet_z,
                       const double* sheet_t, void* psi_dev) {
     return guard([&] {

@@ -143,6 +143,8 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z);
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides
+// (Kx, Ky) at every node of every sheet, S x nx x ny complex each, from the strip currents of x (post.hip)
+void node_field_device(gt_stack* st, const zd* x, double d, zd* kx, zd* ky);
 // psi of every strip (ns values at psi, device memory) from the trace's segments (psi.hip)
 void strip_coupling_device(gt_stack* st, double d, double mu, int nseg, const gt_segment* segs, const double* sheet_z,
                            const double* sheet_t, double* psi);

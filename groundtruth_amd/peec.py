@@ -466,6 +466,19 @@ class PlaneStack:
         V[self.keep] = x[nb:]
         return x[:nb], V, its, x
 
+    def node_field_dev(self, x_dev, d_mm: float):
+        """(Kx, Ky) at every grid node of every layer, torch complex128 tensors (N, nx, ny) on the device, from a solution
+        vector on the device (gt_node_field): Solution.node_field (copper=False) without copying x to the host."""
+        import torch
+        self.handle
+        shape = (self.n_sheets, self.nx, self.ny)
+        kx = torch.empty(shape, dtype=torch.complex128, device=x_dev.device)
+        ky = torch.empty(shape, dtype=torch.complex128, device=x_dev.device)
+        torch.cuda.synchronize(self.device)
+        _lib.check(_lib.lib().gt_node_field(self.handle, C.c_void_p(x_dev.data_ptr()), float(d_mm) * 1e-3,
+                                            C.c_void_p(kx.data_ptr()), C.c_void_p(ky.data_ptr())), "gt_node_field")
+        return kx, ky
+
     def torch_stream(self):
         """The library's stream as a torch stream: torch work in its context is ordered with the library's."""
         import torch

@@ -125,3 +125,73 @@ class Solution:
     def L_loop(self) -> float:
         """Loop inductance (H): Im(Z_loop) / w."""
         return self.Z_loop.imag / (2 * math.pi * self.f)
+
+    def at(self, node) -> complex:
+        """A grid node's potential, node = (layer, i, j) (peec.py:1118-1125; this build's stacks have no lid
+        nodes)."""
+        return complex(self.V[tuple(node)])
+
+    def node_field(self, k: int, d_mm: float, copper: bool = False) -> tuple:
+        """Complex (Kx, Ky) at the nodes of layer k, in A/m per A of signal (peec.py:1151-1199): at each node, the
+        mean of the strips that touch it in each direction and carry current, over the pitch. copper=True asks for
+        the current per width of the copper itself, which needs the strips' copper widths and the nodes' copper
+        areas from the reference's rasterisation; this build's strips are full cells, so it is refused."""
+        if copper:
+            raise NotImplementedError("copper-width fields need the rasterised copper widths and areas")
+        d = d_mm * 1e-3
+        Ix, Iy = self.I_x[k], self.I_y[k]
+        nx, ny = self.V.shape[1:]
+        kx = np.zeros((nx, ny), complex)
+        ky = np.zeros((nx, ny), complex)
+        cx = np.zeros((nx, ny))
+        cy = np.zeros((nx, ny))
+        # a strip (i, j) along x touches the nodes (i, j) and (i + 1, j); along y, (i, j) and (i, j + 1)
+        kx[:-1, :] += Ix
+        cx[:-1, :] += Ix != 0
+        kx[1:, :] += Ix
+        cx[1:, :] += Ix != 0
+        ky[:, :-1] += Iy
+        cy[:, :-1] += Iy != 0
+        ky[:, 1:] += Iy
+        cy[:, 1:] += Iy != 0
+        return kx / np.maximum(cx, 1) / d, ky / np.maximum(cy, 1) / d
+
+    def sheet_current(self, k: int, d_mm: float, copper: bool = False) -> np.ndarray:
+        """|K| at the nodes of layer k, A/m per A of signal (peec.py:1201-1204)."""
+        kx, ky = self.node_field(k, d_mm, copper)
+        return np.sqrt(np.abs(kx) ** 2 + np.abs(ky) ** 2)
+
+    def dominant_phase(self) -> float:
+        """The phase phi at which the strip currents are largest (peec.py:1206-1223): half the angle of the sum of
+        I^2 over every strip (the principal axis of the complex currents), turned by pi when Re(I e^-j phi) then runs
+        against Re(I); 0 when the sum is 0."""
+        cached = getattr(self, "_phase", None)
+        if cached is not None:
+            return cached
+        sq = complex(np.sum(self.I_x ** 2) + np.sum(self.I_y ** 2))
+        phi = 0.5 * math.atan2(sq.imag, sq.real) if abs(sq) > 0 else 0.0
+        rot = np.exp(-1j * phi)
+        corr = float(np.sum((self.I_x * rot).real * self.I_x.real) + np.sum((self.I_y * rot).real * self.I_y.real))
+        if corr < 0:
+            phi += math.pi
+        self._phase = phi
+        return phi
+
+    def sheet_vectors(self, k: int, d_mm: float, copper: bool = False) -> tuple:
+        """Real (Kx, Ky) at the nodes of layer k at the dominant phase (peec.py:1225-1230)."""
+        kx, ky = self.node_field(k, d_mm, copper)
+        rot = np.exp(-1j * self.dominant_phase())
+        return (kx * rot).real, (ky * rot).real
+
+    def layer_share(self, masks: Optional[list] = None) -> np.ndarray:
+        """Each layer's fraction of the summed strip-current magnitude (peec.py:1232-1245); masks: per layer a pair
+        of boolean arrays (x strips, y strips) of the strips to count."""
+        per = []
+        for k in range(len(self.I_x)):
+            ix, iy = np.abs(self.I_x[k]), np.abs(self.I_y[k])
+            if masks is not None:
+                ix, iy = ix * masks[k][0], iy * masks[k][1]
+            per.append(ix.sum() + iy.sum())
+        per = np.array(per)
+        tot = per.sum()
+        return per / tot if tot > 0 else per

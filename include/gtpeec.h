@@ -129,6 +129,13 @@ int gt_lp_matvec(gt_stack* stack, const void* I_dev, void* out_dev);
  * for the batch. */
 int gt_lp_matvec_batch(gt_stack* stack, int32_t nr, const void* const* I_dev, void* const* out_dev);
 
+/* The complex sheet-current density (Kx, Ky), in A/m per A of signal, at every grid node of every sheet from the
+ * strip currents of a solution vector on the device (x_dev: nb + n_keep complex values; kx_dev, ky_dev: S x nx x ny
+ * complex each, row-major, written): at a node, the mean of the touching strips along each axis that carry current,
+ * over the pitch d (m). Replaces peec.py:1151-1199 `Solution.node_field` (copper=False) for a served map, without
+ * copying the solution to the host; the values equal the host's to rounding. */
+int gt_node_field(gt_stack* stack, const void* x_dev, double d, void* kx_dev, void* ky_dev);
+
 /* psi of every strip (peec.py:2609-2656 `strip_coupling`): the integral of the trace's vector potential along each
  * strip per amp of loop current, the strips' right-hand side (rhs_top = -j w psi). One trace segment from a to b (m),
  * `width` wide (m), carrying `current` amps per amp of loop current. psi_dev: ns = nxb + nyb doubles (device).
