@@ -90,10 +90,10 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.log0 = ilog2(P[0]);
     o.log1 = ilog2(P[1]);
     const int S = st->S;
-    if ((size_t)S * o.P0 > (size_t)LpPts<double>::cols)
-        fail(GT_ERR_UNSUPPORTED, "sheets x column FFT size above this build's LDS budget (S * P0 <= 4096)");
-    if (o.P1 + 1 > LpPts<double>::rows)
-        fail(GT_ERR_UNSUPPORTED, "row FFT size above this build's LDS budget (P1 <= 2048)");
+    if ((size_t)S * o.P0 > (size_t)LpPts<double>::big)
+        fail(GT_ERR_UNSUPPORTED, "sheets x column FFT size above this build's LDS budget (S * P0 <= 8192)");
+    if (o.P1 > LpPts<double>::big / 2)
+        fail(GT_ERR_UNSUPPORTED, "row FFT size above this build's LDS budget (P1 <= 4096)");
     o.TR = std::max(1, std::min(8, LpPts<double>::rows / (o.P1 + 1)));
     // rows of the dense (sheet, i) grid as CSR over the sorted strips
     std::vector<int> rowptr((size_t)S * n0 + 1, 0), rowcol(cnt);

@@ -439,15 +439,43 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int nr = a.rhs.nr;
     const int bucket = nr == 1 ? 0 : nr <= 4 ? 1 : 2;
     const int rw = sh.rw[bucket], cw = sh.cw[bucket];
-    if (!launch_rows_w(a, nr, false, st, rw))
-        hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+    // the LDS passes of long transforms (up to 8192 points of a workgroup: 128 KB of FP64) ask for more than the
+    // default 64 KB of dynamic LDS, once per kernel
+    constexpr int BIG = LpPts<T>::big;
+    const bool big_rows = sh.lds_rows > (size_t)LpPts<T>::rows * sizeof(cpx<T>);
+    const bool big_cols = sh.lds_cols > (size_t)LpPts<T>::cols * sizeof(cpx<T>);
+    static bool lds_set = false;
+    if (!lds_set && (big_rows || big_cols)) {
+        const int bytes = (int)(BIG * sizeof(cpx<T>));
+        GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_fwd<T, NTH, BIG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   bytes));
+        GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_inv<T, NTH, BIG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   bytes));
+        GT_HIP(hipFuncSetAttribute((const void*)&lp_cols_mac<T, NTH, BIG, LP_MAX_SHEETS>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        lds_set = true;
+    }
+    if (!launch_rows_w(a, nr, false, st, rw)) {
+        if (big_rows)
+            hipLaunchKernelGGL((lp_rows_fwd<T, NTH, BIG>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+        else
+            hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+    }
     GT_LAUNCH("lp_rows_fwd");
-    if (!launch_cols_w(a, ncol, nr, st, cw))
-        hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols,
-                           st, a);
+    if (!launch_cols_w(a, ncol, nr, st, cw)) {
+        if (big_cols)
+            hipLaunchKernelGGL((lp_cols_mac<T, NTH, BIG, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols, st, a);
+        else
+            hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH),
+                               sh.lds_cols, st, a);
+    }
     GT_LAUNCH("lp_cols_mac");
-    if (!launch_rows_w(a, nr, true, st, rw))
-        hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+    if (!launch_rows_w(a, nr, true, st, rw)) {
+        if (big_rows)
+            hipLaunchKernelGGL((lp_rows_inv<T, NTH, BIG>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+        else
+            hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+    }
     GT_LAUNCH("lp_rows_inv");
 }
 

@@ -12,10 +12,12 @@ constexpr int KMAX = 16;              // right-hand sides one batched launch tak
 // column): 64 KiB of complex values.
 template <class T>
 struct LpPts;
+// points of the LDS passes' workgroup buffers: rows / cols for the usual sizes, big for the long transforms (a row
+// of 4096 points, or S * P0 up to 8192 in the column pass: 128 KB of FP64 LDS, launched only where needed)
 template <>
-struct LpPts<double> { static constexpr int rows = 4096, cols = 4096; };
+struct LpPts<double> { static constexpr int rows = 4096, cols = 4096, big = 8192; };
 template <>
-struct LpPts<float> { static constexpr int rows = 8192, cols = 8192; };
+struct LpPts<float> { static constexpr int rows = 8192, cols = 8192, big = 8192; };
 
 template <class T>
 struct OrientArgs {
