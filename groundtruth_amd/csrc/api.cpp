@@ -90,8 +90,9 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.log0 = ilog2(P[0]);
     o.log1 = ilog2(P[1]);
     const int S = st->S;
-    if ((size_t)S * o.P0 > (size_t)LpPts<double>::big)
-        fail(GT_ERR_UNSUPPORTED, "sheets x column FFT size above this build's LDS budget (S * P0 <= 8192)");
+    if (o.P0 > LpPts<double>::big / 2)
+        fail(GT_ERR_UNSUPPORTED, "column FFT size above this build's LDS budget (P0 <= 4096)");
+    if ((size_t)S * o.P0 > (size_t)LpPts<double>::big) st->shape.split = true;   // the column pass through HBM
     if (o.P1 > LpPts<double>::big / 2)
         fail(GT_ERR_UNSUPPORTED, "row FFT size above this build's LDS budget (P1 <= 4096)");
     o.TR = std::max(1, std::min(8, LpPts<double>::rows / (o.P1 + 1)));
@@ -172,6 +173,7 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
         d.K = h.K.p;
         d.T1 = work.T1.p + h.t_off;
         d.T2 = work.T2.p + h.t_off;
+        d.T3 = s_elems ? work.T3.p + h.s_off : nullptr;
         d.tw0 = h.present ? tw.at(h.P0).p : nullptr;
         d.tw1 = h.present ? tw.at(h.P1).p : nullptr;
         a.nblk_rows[oi] = h.present ? S * ((h.n0 + h.TR - 1) / h.TR) : 0;
@@ -180,6 +182,7 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
     a.S = S;
     a.mode = mode;
     a.t_stride = t_elems;
+    a.s_stride = s_elems;
     a.a_keep = a_keep.p;
     a.b_keep = b_keep.p;
     return a;
@@ -370,6 +373,13 @@ void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd,
         }
         setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx, kd ? &kg : nullptr);
         setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky, kd ? &kg : nullptr);
+        if (st->shape.split) {                  // both orientations' whole column spectra, each its own region
+            for (OrientHost& o : st->o) {
+                if (!o.present) continue;
+                o.s_off = st->s_elems;
+                st->s_elems += (size_t)st->S * o.P1 * o.P0;
+            }
+        }
         st->strip_sheet.upload(st->h_strip_sheet.empty() ? std::vector<unsigned char>{0} : st->h_strip_sheet, s);
         const int nb = st->nb, nk = st->nk;
         need(nb == 0 || (d->a_keep && d->b_keep), "incidence missing");

@@ -105,6 +105,72 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     }
 }
 
+
+// The column pass in three launches through HBM, for stacks whose S sheets of one column exceed the LDS budget
+// (S * P0 > 8192 points): (1) each sheet's column j, zero-padded, transformed along i, its whole spectrum to T3;
+// (2) at every pixel the kernel multiply-add over the source sheets (the same sums, in the same order, as the
+// one-workgroup pass); (3) each sheet's inverse transform, rows 0:n0 to T2. Item of (1) and (3): (sheet, column).
+template <class T, int NTH, int MAXPTS, bool INV>
+__global__ __launch_bounds__(NTH) void lp_cols_split_fft(LpArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    int item = blockIdx.x, oi = 0;
+    const int first = a.S * a.nblk_cols[0];
+    if (item >= first) { item -= first; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    const int s = item / o.P1, j = item - s * o.P1;
+    const int P0 = o.P0, n0 = o.n0;
+    cpx<T>* T3 = o.T3 + blockIdx.y * a.s_stride + ((size_t)s * o.P1 + j) * P0;
+    if (!INV) {
+        const cpx<T>* T1 = o.T1 + blockIdx.y * a.t_stride + ((size_t)s * o.P1 + j) * n0;
+        for (int i = threadIdx.x; i < P0; i += NTH) buf[i] = i < n0 ? T1[i] : cpx<T>();
+    } else {
+        for (int i = threadIdx.x; i < P0; i += NTH) buf[i] = T3[i];
+    }
+    __syncthreads();
+    fft_lds<T, INV, NTH, MAXPTS>(buf, P0, o.log0, 1, P0, o.tw0);
+    if (!INV) {
+        for (int i = threadIdx.x; i < P0; i += NTH) T3[i] = buf[i];
+    } else {
+        cpx<T>* T2 = o.T2 + blockIdx.y * a.t_stride + ((size_t)s * o.P1 + j) * n0;
+        for (int i = threadIdx.x; i < n0; i += NTH) T2[i] = buf[i];
+    }
+}
+
+template <class T, int MAXS>
+__global__ __launch_bounds__(256) void lp_cols_split_mix(LpArgs<T> a, long pix0) {
+    long idx = blockIdx.x * 256L + threadIdx.x;
+    int oi = 0;
+    if (idx >= pix0) { idx -= pix0; oi = 1; }
+    const OrientArgs<T> o = pick(a, oi);
+    if (a.nblk_cols[oi] == 0 || idx >= (long)o.P0 * o.P1) return;
+    const int S = a.S, P0 = o.P0;
+    const int j = (int)(idx / P0), i = (int)(idx - (long)j * P0);
+    cpx<T>* T3 = o.T3 + blockIdx.y * a.s_stride + (size_t)j * P0 + i;
+    const size_t sheet = (size_t)o.P1 * P0;
+    const int qj = min(j, o.P1 - j), qi = min(i, P0 - i), h0 = P0 / 2 + 1;
+    const T* Kp = o.K + ((size_t)qj * h0 + qi) * S * S;
+    cpx<T> v[MAXS];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+        if (s < S) v[s] = T3[s * sheet];
+#pragma unroll
+    for (int t = 0; t < MAXS; ++t) {
+        if (t < S) {
+            cpx<T> acc;
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) {
+                if (s < S) {
+                    const T k = Kp[t * S + s];
+                    acc.re += k * v[s].re;
+                    acc.im += k * v[s].im;
+                }
+            }
+            T3[t * sheet] = acc;
+        }
+    }
+}
+
 // Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w mod 8), each XCD with its own L2. Row tiles i0 and
 // i0 + RW share the 128-byte lines of the transposed intermediates ([t][j][i]: 8 complex doubles of consecutive i),
 // so the tile order is remapped for consecutive tiles to run on one XCD: tile = (w mod 8) (G / 8) + w / 8 over a
@@ -462,7 +528,20 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
             hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
     }
     GT_LAUNCH("lp_rows_fwd");
-    if (!launch_cols_w(a, ncol, nr, st, cw)) {
+    if (sh.split) {
+        const int items = a.S * ncol;
+        const int P0 = std::max(a.nblk_cols[0] ? a.o[0].P0 : 0, a.nblk_cols[1] ? a.o[1].P0 : 0);
+        const size_t lds = (size_t)P0 * sizeof(cpx<T>);
+        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH, LpPts<T>::cols, false>), dim3(items, nr), dim3(NTH), lds, st, a);
+        GT_LAUNCH("lp_cols_split_fft");
+        const long pix0 = a.nblk_cols[0] ? (long)a.o[0].P0 * a.o[0].P1 : 0;
+        const long pix = pix0 + (a.nblk_cols[1] ? (long)a.o[1].P0 * a.o[1].P1 : 0);
+        hipLaunchKernelGGL((lp_cols_split_mix<T, LP_MAX_SHEETS>), dim3((unsigned)((pix + 255) / 256), nr), dim3(256), 0,
+                           st, a, pix0);
+        GT_LAUNCH("lp_cols_split_mix");
+        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH, LpPts<T>::cols, true>), dim3(items, nr), dim3(NTH), lds, st, a);
+        GT_LAUNCH("lp_cols_split_fft");
+    } else if (!launch_cols_w(a, ncol, nr, st, cw)) {
         if (big_cols)
             hipLaunchKernelGGL((lp_cols_mac<T, NTH, BIG, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols, st, a);
         else

@@ -30,6 +30,7 @@ struct OrientArgs {
     const T* K;               // kernel quadrant [qj][qi][t][s], scaled by 1 / (P0 P1)
     cpx<T>* T1;               // [s][j][i] after the row FFT (S * P1 * n0), right-hand side 0
     cpx<T>* T2;               // [t][j][i] after the column pass
+    cpx<T>* T3;               // [s][j][k] whole column spectra (the split column pass), right-hand side 0
     const cpx<T>* tw0;        // twiddles of P0
     const cpx<T>* tw1;        // twiddles of P1
 };
@@ -55,6 +56,7 @@ struct LpArgs {
     int mode;                 // 0: y = L_p x (strip vectors); 1: the system product's strip rows (node potentials
                               // at rhs.nb[p] in x)
     size_t t_stride;          // elements of T1 / T2 per right-hand side
+    size_t s_stride;          // elements of T3 per right-hand side
     const int* a_keep;
     const int* b_keep;
     RhsSet<T> rhs;
@@ -78,6 +80,9 @@ struct LpShape {
     // launch shapes of the register passes per batch-size class (1; 2-4; 5-16 right-hand sides): rows per row-pass
     // workgroup and waves per column-pass workgroup. They change no arithmetic; an autotuner may set them.
     int rw[3] = {4, 4, 4}, cw[3] = {1, 1, 1};
+    // the column pass in three launches through HBM (column transforms, the sheets' multiply-add per pixel,
+    // inverse transforms), for stacks whose S sheets of a column exceed the LDS budget
+    bool split = false;
 };
 
 template <class T>

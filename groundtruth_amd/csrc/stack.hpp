@@ -45,6 +45,7 @@ struct OrientHost {
     bool present = false;
     int n0 = 0, n1 = 0, P0 = 0, P1 = 0, log0 = 0, log1 = 0, TR = 1, lo = 0, cnt = 0;
     size_t t_off = 0;          // offset of this orientation's L_p intermediates in T1 / T2
+    size_t s_off = 0;          // offset of its whole column spectra in T3 (the split column pass)
     DevBuf<int> rowptr, rowcol;
     DevBuf<double> K;
     DevBuf<float> Kf;
@@ -54,11 +55,14 @@ struct OrientHost {
 struct Work {
     int cap = 0;              // right-hand sides the buffers hold
     DevBuf<zd> T1, T2;        // L_p intermediates, t_elems per right-hand side
+    DevBuf<zd> T3;            // whole column spectra of the split column pass, s_elems per right-hand side
     DevBuf<zd> s, z;          // Schur right-hand sides and solutions, interleaved (n_keep x batch)
     DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (t_cols each)
     int t_cols = 0;
     void ensure(const gt_stack_s* st, int nr, int kt = 0);
-    int64_t bytes() const { return (int64_t)(T1.bytes() + T2.bytes() + s.bytes() + z.bytes() + t.bytes()); }
+    int64_t bytes() const {
+        return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes());
+    }
     // the bytes it would hold after ensure(st, nr, kt)
     int64_t bytes_after(const gt_stack_s* st, int nr, int kt) const;
 };
@@ -79,6 +83,7 @@ struct StackCore {
     std::vector<unsigned char> h_strip_sheet;
     LpShape shape;
     size_t t_elems = 0;                       // complex elements of the L_p intermediates (both orientations)
+    size_t s_elems = 0;                       // complex elements of the split column pass's spectra (0: not split)
     std::unique_ptr<Schur> schur;
 };
 
@@ -95,6 +100,7 @@ struct gt_stack_s {
     std::vector<unsigned char>& h_strip_sheet;
     gt::LpShape& shape;
     size_t& t_elems;
+    size_t& s_elems;
     std::unique_ptr<gt::Schur>& schur;
 
     gt_ctx* ctx = nullptr;
@@ -112,7 +118,7 @@ struct gt_stack_s {
     explicit gt_stack_s(std::shared_ptr<gt::StackCore> c = std::make_shared<gt::StackCore>())
         : core(std::move(c)), o(core->o), tw(core->tw), twf(core->twf), node_sheet(core->node_sheet),
           strip_sheet(core->strip_sheet), h_strip_sheet(core->h_strip_sheet), shape(core->shape),
-          t_elems(core->t_elems), schur(core->schur), id(next_id()) {}
+          t_elems(core->t_elems), s_elems(core->s_elems), schur(core->schur), id(next_id()) {}
     gt_stack_s(const gt_stack_s&) = delete;
     gt_stack_s& operator=(const gt_stack_s&) = delete;
 

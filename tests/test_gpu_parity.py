@@ -233,10 +233,11 @@ def _strip_board(nx, ny, sheets, seed):
     return layers, branches
 
 
-@pytest.mark.parametrize("nx,ny,sheets", [(1100, 6, 2), (6, 1100, 2), (1100, 5, 1)])
+@pytest.mark.parametrize("nx,ny,sheets", [(1100, 6, 2), (6, 1100, 2), (1100, 5, 1), (1100, 6, 3), (6, 1100, 5)])
 def test_Lp_matvec_with_4096_point_transforms_matches_the_oracle(nx, ny, sheets):
     """Transforms of 4096 points: the LDS passes' long-transform kernels (a 4096-point row per workgroup, S x P0 = 8192
-    column points: 128 KB of LDS) against the oracle's scipy FFTs; and a whole solve on such a stack."""
+    column points: 128 KB of LDS), and beyond 8192 column points the column pass through HBM (transforms, the
+    sheets' multiply-add per pixel, inverse transforms), against the oracle's scipy FFTs, with the system product."""
     layers, branches = _strip_board(nx, ny, sheets, seed=nx + ny + sheets)
     st, ref = peec.PlaneStack(layers, branches, setup="host"), O.PlaneStack(layers, branches)
     assert 4096 in tuple(st.Px) + tuple(st.Py)
