@@ -73,6 +73,9 @@ SIGNATURES = {
     "gt_ctx_lp_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_schur_device_fronts": (C.c_int, [C.c_void_p, C.c_int32]),
     "gt_ctx_schur_leaf": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gt_ctx_woodbury": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "gt_ctx_schur_ordering": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gt_stack_woodbury": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "gt_ctx_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_orthogonalization": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_lp_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

@@ -192,7 +192,7 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
 // stack needs resident)
 int64_t gt_stack_s::device_bytes() const {
     int64_t b = a_keep.bytes() + b_keep.bytes() + adj_ptr.bytes() + adj.bytes();
-    b += work.T1.bytes() + work.T2.bytes() + work.s.bytes() + work.z.bytes() + work.t.bytes() + W.bytes();
+    b += work.bytes() + W.bytes() + G.bytes();
     if (derived) return b;                 // the core is its base's
     for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + h.K.bytes() + h.Kf.bytes();
     for (const auto& t : tw) b += t.second.bytes();
@@ -292,6 +292,36 @@ int gt_ctx_schur_leaf(gt_ctx* ctx, int32_t nodes) {
         need(nodes >= 4 && nodes <= 512, "a leaf of 4 to 512 nodes");
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->schur_leaf = nodes;
+    });
+}
+
+int gt_ctx_schur_ordering(gt_ctx* ctx, int32_t ordering) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        need(ordering == GT_ND_SHEETS_APART || ordering == GT_ND_JOINT, "unknown ordering");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->schur_sheets_apart = ordering == GT_ND_SHEETS_APART;
+    });
+}
+
+int gt_ctx_woodbury(gt_ctx* ctx, int32_t form, int64_t stored_max_bytes) {
+    return guard([&] {
+        need(ctx != nullptr, "ctx is NULL");
+        need(form == GT_WOODBURY_AUTO || form == GT_WOODBURY_STORED || form == GT_WOODBURY_IMPLICIT,
+             "unknown Woodbury form");
+        need(stored_max_bytes >= 0, "negative byte limit");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->woodbury = form;
+        ctx->wb_stored_max = stored_max_bytes;
+    });
+}
+
+int gt_stack_woodbury(const gt_stack* st, int32_t* form, int32_t* k) {
+    return guard([&] {
+        need(st && form && k, "NULL argument");
+        need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
+        *form = st->wb_implicit ? GT_WOODBURY_IMPLICIT : GT_WOODBURY_STORED;
+        *k = st->k;
     });
 }
 
@@ -495,6 +525,7 @@ int gt_stack_lp_autotune(gt_stack* st, int32_t reps, int32_t* rw_out, int32_t* c
         hipStream_t s = st->ctx->stream;
         const int nrs[3] = {1, 4, 8};
         if (st->ns > 0) {
+            const int cap0 = st->work.cap;          // the products' workspace goes back to this afterwards
             st->work.ensure(st, 8);
             DevBuf<zd> X((size_t)st->ns * 8), Y((size_t)st->ns * 8);
             GT_HIP(hipMemsetAsync(X.p, 0, X.bytes(), s));
@@ -536,6 +567,7 @@ int gt_stack_lp_autotune(gt_stack* st, int32_t reps, int32_t* rw_out, int32_t* c
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             GT_HIP(hipStreamSynchronize(s));
+            st->work.shrink(st, cap0);
         }
         for (int b = 0; b < 3; ++b) {
             rw_out[b] = st->shape.rw[b];
@@ -570,12 +602,20 @@ int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
         auto S = std::make_unique<Schur>();
         S->h.dev_min = st->ctx->schur_dev_min;
         S->h.leaf = st->ctx->schur_leaf;
+        S->h.sheets_apart = st->ctx->schur_sheets_apart;
         S->h.build(nk, ea, eb, diag, node_xyz);
         S->upload(st->ctx->stream);
         S->factor_device(st->ctx->stream);
         st->schur = std::move(S);
-        // Woodbury columns W = B^-1 U (U: the branches' incidence) by device solves, and G = U^T W
+        // Woodbury columns W = B^-1 U (U: the branches' incidence) by device solves, and G = U^T W; W stays resident
+        // unless it is too large (then only G is kept and each product solves for B^-1 U t)
         const int k = st->nbr;
+        const int form = st->ctx->woodbury;
+        st->wb_implicit = form == GT_WOODBURY_IMPLICIT ||
+                          (form == GT_WOODBURY_AUTO && (k > WB_STORED_MAX_K ||
+                                                        (int64_t)k * nk * (int64_t)sizeof(double) > st->ctx->wb_stored_max));
+        need(k <= (st->wb_implicit ? WB_IMPLICIT_MAX_K : WB_STORED_MAX_K),
+             st->wb_implicit ? "more than 2048 branches on one stack" : "more than 256 branches with W resident");
         std::vector<int> ka(k), kb(k);
         for (int m = 0; m < k; ++m) {
             ka[m] = st->h_a_keep[st->ns + m];
@@ -620,7 +660,9 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
             st->h_b_keep.push_back(b_keep_extra[e]);
         }
         upload_incidence(st.get());
-        // W' = [W  B^-1 U']: the base's columns copied on the device, the added ones by device solves
+        // W' = [W  B^-1 U']: the base's columns copied on the device, the added ones by device solves (or, in the
+        // implicit form, G' from the base's G and the added columns' solves)
+        st->wb_implicit = base->wb_implicit;
         hipStream_t s = st->ctx->stream;
         woodbury_extend(st.get(), base, n_extra, a_keep_extra, b_keep_extra);
         if (G_out) woodbury_gram_device(st.get(), G_out);
@@ -702,7 +744,7 @@ int gt_op_create(gt_stack* st, const gt_op_desc* d, gt_op** out) {
         need(d->r_sheet && d->dinv_sheet, "per-sheet impedances missing");
         need(st->nbr == 0 || (d->z_tail && d->dinv_tail), "branch impedances missing");
         need(d->k == 0 || (d->k == st->k && d->q != nullptr), "Woodbury rank must be the stack's branch count");
-        need(d->k <= 256, "Woodbury rank above 256");
+        need(d->k <= (st->wb_implicit ? WB_IMPLICIT_MAX_K : WB_STORED_MAX_K), "Woodbury rank above the stack's limit");
         std::unique_ptr<gt_op_s> op(new gt_op_s);
         op->st = st;
         op->w = d->w;

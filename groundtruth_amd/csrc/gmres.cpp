@@ -931,8 +931,11 @@ int64_t session_plan_bytes(const gt_stack* st, int cap, int restart, int basis, 
     const int n = st->nb + max_extra + st->nk;
     const int rows = (restart > 0 ? restart : default_restart(n)) + 1;
     int64_t b = GmresWork::plan_bytes((size_t)n, rows, cap, basis_bytes(basis));
-    // the stack's product scratch grows to the session's batch and Woodbury columns
-    return b + st->work.bytes_after(st, cap, st->k + max_extra) - st->work.bytes();
+    // the stack's product scratch grows to the session's batch and Woodbury columns, the Schur solve's buffers to
+    // the batch
+    b += st->work.bytes_after(st, cap, st->k + max_extra) - st->work.bytes();
+    if (st->schur) b += st->schur->bytes_after(cap) - st->schur->bytes_after(0);
+    return b;
 }
 
 gt_session* session_create(gt_stack* st, int cap, int restart, int basis, int max_extra) {
@@ -949,6 +952,7 @@ gt_session* session_create(gt_stack* st, int cap, int restart, int basis, int ma
     else ss->bt.reset(new Batch<double, double>(st, cap, ss->restart, ss->gw, ss->n_max));
     ss->xout.assign(cap, nullptr);
     st->work.ensure(st, cap, st->k + max_extra);
+    if (st->schur) st->schur->ensure(cap);
     return ss.release();
 }
 

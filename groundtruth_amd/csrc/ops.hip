@@ -3,7 +3,8 @@
 // y = S^-1 (r2 - Ar^T D^-1 r1), x1 = D^-1 (r1 + Ar y), with S^-1 = (1/d_s) (B + U C U^T)^-1 applied as B^-1 (the
 // device triangular solves, all right-hand sides in one pass over the panels) plus the Woodbury correction
 // z - W Q (U^T z) (W = B^-1 U resident, Q each frequency's k x k matrix). Loops over the batch sit inside the
-// kernels, so the incidence and W are read once per batch.
+// kernels, so the incidence and W are read once per batch. Where W would be too large (many branches on many nodes)
+// the implicit form applies B^-1 (U t) by a second batched Schur solve instead.
 #include <type_traits>
 
 #include "stack.hpp"
@@ -11,8 +12,6 @@
 namespace gt {
 
 namespace {
-
-constexpr int WB_SMALL = 256;   // most Woodbury columns (branches) of one stack
 
 __device__ __forceinline__ zd ldz(const zd& v) { return v; }
 __device__ __forceinline__ zd ldz(const cpx<float>& v) { return zd(v.re, v.im); }
@@ -37,6 +36,7 @@ struct PreArgs {
     zd* s;                    // nk x nr interleaved: Schur right-hand sides
     const zd* z;              // nk x nr interleaved: B^-1 s
     zd* t;                    // nr x kt: Q (U^T z)
+    const zd* z2;             // implicit form: B^-1 (U t), interleaved like z
     const double* W;          // the shared columns: k_sh of nk
     const double* Wx[KMAX];   // each right-hand side's own columns k_sh .. k - 1
     const int* a_keep[KMAX];
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs<T> a) {
 // t_p = Q_p (U_p^T z_p): one workgroup per right-hand side (k, the branch count, is small)
 template <class T>
 __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs<T> a) {
-    __shared__ double ut_raw[2 * WB_SMALL];
+    __shared__ double ut_raw[2 * WB_STORED_MAX_K];
     zd* ut = reinterpret_cast<zd*>(ut_raw);
     const int p = blockIdx.x;
     const int k = a.k[p];
@@ -91,6 +91,74 @@ __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs<T> a) {
         zd acc;
         for (int m = 0; m < k; ++m) acc = acc + q[i * k + m] * ut[m];
         a.t[p * a.kt + i] = acc;
+    }
+}
+
+// The implicit form's t_p = Q_p (U_p^T z_p) for up to 2048 branches: a workgroup takes 32 rows of one right-hand
+// side, eight lanes a row (m strided by eight, then the eight partial sums added by a fixed butterfly)
+template <class T>
+__global__ __launch_bounds__(256) void pre_woodbury_q(PreArgs<T> a) {
+    __shared__ double ut_raw[2 * WB_IMPLICIT_MAX_K];
+    zd* ut = reinterpret_cast<zd*>(ut_raw);
+    const int p = blockIdx.x;
+    const int k = a.k[p], i0 = blockIdx.y * 32;
+    if (i0 >= k) return;
+    for (int m = threadIdx.x; m < k; m += 256) {
+        const int b = a.ns + m;
+        const int ka = a.a_keep[p][b], kb = a.b_keep[p][b];
+        zd v;
+        if (ka >= 0) v = v + a.z[(size_t)ka * a.nr + p];
+        if (kb >= 0) v = v - a.z[(size_t)kb * a.nr + p];
+        ut[m] = v;
+    }
+    __syncthreads();
+    const int i = i0 + (threadIdx.x >> 3), l = threadIdx.x & 7;
+    double re = 0, im = 0;
+    if (i < k) {
+        const zd* q = a.q[p] + (size_t)i * k;
+        for (int m = l; m < k; m += 8) {
+            const zd qm = q[m], um = ut[m];
+            re += qm.re * um.re - qm.im * um.im;
+            im += qm.re * um.im + qm.im * um.re;
+        }
+    }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off, 8);
+        im += __shfl_xor(im, off, 8);
+    }
+    if (i < k && l == 0) a.t[p * a.kt + i] = zd(re, im);
+}
+
+// The implicit form's s_p = U_p t_p at each kept node: its Woodbury branches (adjacency entries past the strips) in
+// adjacency order, + at a branch's a end, - at its b end
+template <class T>
+__global__ __launch_bounds__(256) void pre_woodbury_scatter(PreArgs<T> a) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= a.nk) return;
+    for (int p = 0; p < a.nr; ++p) {
+        const int e0 = a.adj_ptr[p][n], e1 = a.adj_ptr[p][n + 1];
+        const int* adj = a.adj[p];
+        const zd* tp = a.t + p * a.kt;
+        zd acc;
+        for (int e = e0; e < e1; ++e) {
+            const int code = adj[e], m = (code >> 1) - a.ns;
+            if (m < 0 || m >= a.k[p]) continue;
+            if (code & 1) acc = acc - tp[m];
+            else acc = acc + tp[m];
+        }
+        a.s[(size_t)n * a.nr + p] = acc;
+    }
+}
+
+// The implicit form's y_p = s_scale_p (z_p - B^-1 U_p t_p)
+template <class T>
+__global__ __launch_bounds__(256) void pre_woodbury_diff(PreArgs<T> a) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= a.nk) return;
+    for (int p = 0; p < a.nr; ++p) {
+        const size_t i = (size_t)n * a.nr + p;
+        a.out[p][a.nb[p] + n] = stz<T>(a.s_scale[p] * (a.z[i] - a.z2[i]));
     }
 }
 
@@ -181,6 +249,22 @@ void Work::ensure(const gt_stack* st, int nr, int kt) {
         t.alloc((size_t)std::max(1, t_cols) * std::max(nr, cap));
     }
     cap = std::max(cap, nr);
+    if (st->wb_implicit && z2.n < (size_t)std::max(1, st->nk) * cap) z2.alloc((size_t)std::max(1, st->nk) * cap);
+}
+
+void Work::shrink(const gt_stack* st, int nr) {
+    if (nr >= cap) return;
+    const int kt = t_cols;
+    T1.release();
+    T2.release();
+    T3.release();
+    s.release();
+    z.release();
+    z2.release();
+    t.release();
+    cap = 0;
+    t_cols = 0;
+    if (nr > 0) ensure(st, nr, kt);
 }
 
 int64_t Work::bytes_after(const gt_stack* st, int nr, int kt) const {
@@ -191,6 +275,7 @@ int64_t Work::bytes_after(const gt_stack* st, int nr, int kt) const {
                          : (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes());
     if (nr > cap || kt > t_cols) b += (int64_t)std::max(1, std::max(kt, t_cols)) * std::max(nr, cap) * zb;
     else b += (int64_t)t.bytes();
+    if (st->wb_implicit) b += std::max<int64_t>((int64_t)z2.bytes(), (int64_t)std::max(1, st->nk) * std::max(nr, cap) * zb);
     return b;
 }
 
@@ -353,17 +438,62 @@ __global__ __launch_bounds__(256) void wb_gram(const double* __restrict__ W, int
     G[i] = v;
 }
 
+// The implicit form's G[r][c0 + c] = (U^T Z)[r][c] over the k branches r, from nr solved columns Z (interleaved)
+__global__ __launch_bounds__(256) void wb_gram_cols(const zd* __restrict__ z, int nr, int k, int c0,
+                                                    const int* __restrict__ ka, const int* __restrict__ kb,
+                                                    double* __restrict__ G) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= k * nr) return;
+    const int r = i / nr, c = i % nr;
+    double v = 0;
+    if (ka[r] >= 0) v += z[(size_t)ka[r] * nr + c].re;
+    if (kb[r] >= 0) v -= z[(size_t)kb[r] * nr + c].re;
+    G[(size_t)r * k + c0 + c] = v;
+}
+
+// A derived stack's G (k x k): the base's k0 x k0 block copied
+__global__ __launch_bounds__(256) void wb_gram_base(const double* __restrict__ G0, int k0, double* __restrict__ G,
+                                                    int k) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= k0 * k0) return;
+    G[(size_t)(i / k0) * k + i % k0] = G0[i];
+}
+
+// ... and its added rows c >= k0 over the base's columns r < k0 by symmetry: G[c][r] = G[r][c]
+__global__ __launch_bounds__(256) void wb_gram_mirror(double* __restrict__ G, int k, int k0) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= (k - k0) * k0) return;
+    const int c = k0 + i / k0, r = i % k0;
+    G[(size_t)c * k + r] = G[(size_t)r * k + c];
+}
+
 }  // namespace
 
 void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra) {
     hipStream_t s = st->ctx->stream;
     const int nk = st->nk, k0 = base ? base->k : 0, k = k0 + n_extra;
-    st->W.alloc(std::max<size_t>(1, (size_t)k * nk));
-    if (k0 > 0 && nk > 0)
-        GT_HIP(hipMemcpyAsync(st->W.p, base->W.p, sizeof(double) * (size_t)k0 * nk, hipMemcpyDeviceToDevice, s));
+    const bool imp = st->wb_implicit;
+    if (imp) {
+        if (k > WB_IMPLICIT_MAX_K) fail(GT_ERR_ARG, "more than 2048 branches on one stack");
+        if (base && !base->wb_implicit) fail(GT_ERR_ARG, "a derived stack takes its base's Woodbury form");
+        st->G.alloc(std::max<size_t>(1, (size_t)k * k));
+        GT_HIP(hipMemsetAsync(st->G.p, 0, st->G.bytes(), s));
+        if (k0 > 0) {
+            hipLaunchKernelGGL(wb_gram_base, dim3(blocks_for((size_t)k0 * k0, 256)), dim3(256), 0, s, base->G.p, k0,
+                               st->G.p, k);
+            GT_LAUNCH("wb_gram_base");
+        }
+    } else {
+        st->W.alloc(std::max<size_t>(1, (size_t)k * nk));
+        if (k0 > 0 && nk > 0)
+            GT_HIP(hipMemcpyAsync(st->W.p, base->W.p, sizeof(double) * (size_t)k0 * nk, hipMemcpyDeviceToDevice, s));
+    }
     if (n_extra > 0 && nk > 0) {
+        // the columns' solves in batches of up to 16, on scratch of their own (the product workspace and the
+        // Schur solve's buffers go back to what they held)
         const int chunk = std::min(n_extra, KMAX);
-        st->work.ensure(st, chunk);
+        const int schur_cap = st->schur->cap;
+        DevBuf<zd> sb((size_t)nk * chunk), zb((size_t)nk * chunk);
         for (int m0 = 0; m0 < n_extra; m0 += KMAX) {
             const int nr = std::min(KMAX, n_extra - m0);
             UnitCols u{};
@@ -372,12 +502,25 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
                 u.kb[c] = b_extra[m0 + c];
             }
             const size_t cnt = (size_t)nk * nr;
-            hipLaunchKernelGGL(wb_units, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, st->work.s.p, nk, nr, u);
+            hipLaunchKernelGGL(wb_units, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, sb.p, nk, nr, u);
             GT_LAUNCH("wb_units");
-            schur_apply(st, st->work.s.p, st->work.z.p, nr);
-            hipLaunchKernelGGL(wb_take, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, st->work.z.p, nk, nr,
-                               st->W.p + (size_t)(k0 + m0) * nk);
-            GT_LAUNCH("wb_take");
+            schur_apply(st, sb.p, zb.p, nr);
+            if (imp) {
+                hipLaunchKernelGGL(wb_gram_cols, dim3(blocks_for((size_t)k * nr, 256)), dim3(256), 0, s, zb.p, nr,
+                                   k, k0 + m0, st->a_keep.p + st->ns, st->b_keep.p + st->ns, st->G.p);
+                GT_LAUNCH("wb_gram_cols");
+            } else {
+                hipLaunchKernelGGL(wb_take, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, zb.p, nk, nr,
+                                   st->W.p + (size_t)(k0 + m0) * nk);
+                GT_LAUNCH("wb_take");
+            }
+        }
+        GT_HIP(hipStreamSynchronize(s));
+        st->schur->shrink(schur_cap);
+        if (imp && k0 > 0) {
+            hipLaunchKernelGGL(wb_gram_mirror, dim3(blocks_for((size_t)n_extra * k0, 256)), dim3(256), 0, s, st->G.p,
+                               k, k0);
+            GT_LAUNCH("wb_gram_mirror");
         }
     }
     st->k = k;
@@ -388,6 +531,11 @@ void woodbury_gram_device(const gt_stack* st, double* G_host) {
     const int k = st->k;
     if (k == 0) return;
     hipStream_t s = st->ctx->stream;
+    if (st->wb_implicit) {
+        GT_HIP(hipMemcpyAsync(G_host, st->G.p, sizeof(double) * (size_t)k * k, hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        return;
+    }
     DevBuf<double> G((size_t)k * k);
     hipLaunchKernelGGL(wb_gram, dim3(blocks_for((size_t)k * k, 256)), dim3(256), 0, s, st->W.p, st->nk, k,
                        st->a_keep.p + st->ns, st->b_keep.p + st->ns, G.p);
@@ -404,13 +552,14 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
     a.ns = st->ns;
     a.nk = st->nk;
     a.nr = nr;
-    a.k_sh = st->k_root;        // the lineage's first columns: the same split alone and in a board's batch
+    const bool imp = st->wb_implicit;    // every stack of a batch has its lineage's form
+    a.k_sh = imp ? 0 : st->k_root;       // the lineage's first columns: the same split alone and in a board's batch
     a.kt = 0;
     int nbmax = 0;
     for (int p = 0; p < nr; ++p) {
         const gt_stack* o = ops[p]->st;
         if (ops[p]->k != o->k) fail(GT_ERR_ARG, "the operator's Woodbury size differs from its stack's");
-        if (o->k > WB_SMALL) fail(GT_ERR_ARG, "more than 256 branches on one stack");
+        if (!imp && o->k > WB_STORED_MAX_K) fail(GT_ERR_ARG, "more than 256 branches on one stack");
         a.kt = std::max(a.kt, o->k);
         nbmax = std::max(nbmax, o->nb);
         a.nb[p] = o->nb;
@@ -421,7 +570,7 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
         a.dinv_tail[p] = ops[p]->dinv_tail.p;
         a.q[p] = ops[p]->q.p;
         a.s_scale[p] = ops[p]->s_scale;
-        a.Wx[p] = o->W.p ? o->W.p + (size_t)a.k_sh * st->nk : nullptr;
+        a.Wx[p] = !imp && o->W.p ? o->W.p + (size_t)a.k_sh * st->nk : nullptr;
         a.a_keep[p] = o->a_keep.p;
         a.b_keep[p] = o->b_keep.p;
         a.adj_ptr[p] = o->adj_ptr.p;
@@ -438,12 +587,23 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
         hipLaunchKernelGGL(pre_schur_rhs<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_schur_rhs");
         schur_apply(st, st->work.s.p, st->work.z.p, nr);
-        if (a.kt > 0) {
-            hipLaunchKernelGGL(pre_woodbury_small<T>, dim3(nr), dim3(256), 0, s, a);
-            GT_LAUNCH("pre_woodbury_small");
+        if (imp && a.kt > 0) {
+            a.z2 = st->work.z2.p;
+            hipLaunchKernelGGL(pre_woodbury_q<T>, dim3(nr, (a.kt + 31) / 32), dim3(256), 0, s, a);
+            GT_LAUNCH("pre_woodbury_q");
+            hipLaunchKernelGGL(pre_woodbury_scatter<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+            GT_LAUNCH("pre_woodbury_scatter");
+            schur_apply(st, st->work.s.p, st->work.z2.p, nr);
+            hipLaunchKernelGGL(pre_woodbury_diff<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+            GT_LAUNCH("pre_woodbury_diff");
+        } else {
+            if (a.kt > 0) {
+                hipLaunchKernelGGL(pre_woodbury_small<T>, dim3(nr), dim3(256), 0, s, a);
+                GT_LAUNCH("pre_woodbury_small");
+            }
+            hipLaunchKernelGGL(pre_woodbury_apply<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+            GT_LAUNCH("pre_woodbury_apply");
         }
-        hipLaunchKernelGGL(pre_woodbury_apply<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
-        GT_LAUNCH("pre_woodbury_apply");
     }
     if (nbmax > 0) {
         hipLaunchKernelGGL(pre_branches<T>, dim3(blocks_for(nbmax, 256)), dim3(256), 0, s, a);

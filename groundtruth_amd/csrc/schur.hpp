@@ -1,6 +1,7 @@
 // The Schur complement factor: B = Ar^T Ar over the kept nodes (the graph Laplacian of strips and branches, one
 // reference node per connected piece removed, so B is symmetric positive definite), ordered by geometric nested
-// dissection and factorised B = L L^T by a multifrontal supernodal Cholesky on the host; the supernodes are the
+// dissection (each sheet's grid on its own under a top separator of the nodes that branches join across sheets,
+// or all sheets cut together) and factorised B = L L^T by a multifrontal supernodal Cholesky on the host; the supernodes are the
 // nested-dissection tree's nodes (leaf boxes and separators), so the assembly tree has the dissection's depth.
 // The device keeps the panels and solves L L^T x = b level by level (schur_solve.hip).
 #pragma once
@@ -72,6 +73,11 @@ struct SchurHost {
     int dev_min = INT32_MAX;
     // nodes of a nested-dissection leaf box at most (one dense supernode at the bottom of the tree)
     int leaf = 32;
+    // Sheets apart: the sheets are coupled only through branches, so the nodes those join across sheets (at most
+    // SHEETS_APART_MAX of them) form the top separator and each sheet's grid is dissected on its own below it; the
+    // separators span one sheet, not all. Else (or past the limit) the dissection cuts every sheet at once.
+    bool sheets_apart = true;
+    static constexpr int SHEETS_APART_MAX = 4096;
     std::vector<char> on_dev;
     std::vector<int> dev_order;                 // device fronts in postorder
     std::vector<int64_t> trip_off;              // per device front (index into dev_order): its original entries
@@ -118,10 +124,17 @@ struct Schur {
     DevBuf<SnItem> fwd, bwd, ffin, bfin;
     DevBuf<zd> part;                    // partial sums of split tiles
     DevBuf<double> panel;
-    DevBuf<zd> ubuf;                    // update vectors of one forward solve (up to 16 right-hand sides)
+    DevBuf<zd> ubuf;                    // update vectors of one forward solve (cap right-hand sides)
     DevBuf<zd> zp, xp;                  // forward and backward solutions in the new numbering
+    int cap = 0;                        // right-hand sides ubuf / zp / xp hold (indexed with the batch's stride)
 
     void upload(hipStream_t s);
+    // ubuf / zp / xp for nr right-hand sides: grown (dropping the captured graphs, whose pointers move) when nr > cap
+    void ensure(int nr);
+    // back to nr right-hand sides when they hold more (set-up work that took a larger batch)
+    void shrink(int nr);
+    // the bytes ubuf / zp / xp would hold after ensure(nr)
+    int64_t bytes_after(int nr) const;
     // the panels of the device fronts (schur_dense.hip), after upload: assembly, blocked partial Cholesky and solve
     // panels, straight into `panel`
     void factor_device(hipStream_t s);

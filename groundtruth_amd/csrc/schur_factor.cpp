@@ -90,9 +90,50 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     t.xyz = xyz;
     t.leaf = leaf;
     t.order.reserve(n);
-    std::vector<int> all(n);
-    std::iota(all.begin(), all.end(), 0);
-    std::vector<int> roots = t.rec(all, ilo, ihi, jlo, jhi);
+    int nsheet = 0;
+    for (int v = 0; v < n; ++v) nsheet = std::max(nsheet, xyz[3 * v] + 1);
+    std::vector<char> cross(n, 0);
+    int n_cross = 0;
+    if (sheets_apart && nsheet > 1) {
+        for (size_t e = 0; e < ea.size(); ++e)
+            if (xyz[3 * ea[e]] != xyz[3 * eb[e]]) cross[ea[e]] = cross[eb[e]] = 1;
+        for (int v = 0; v < n; ++v) n_cross += cross[v];
+    }
+    std::vector<int> roots;
+    if (sheets_apart && nsheet > 1 && n_cross <= SHEETS_APART_MAX) {
+        std::vector<std::vector<int>> per(nsheet);
+        std::vector<int> top;
+        for (int v = 0; v < n; ++v) (cross[v] ? top : per[xyz[3 * v]]).push_back(v);
+        std::vector<int> kids;
+        for (int k = 0; k < nsheet; ++k) {
+            if (per[k].empty()) continue;
+            int a0 = 1 << 30, a1 = -1, b0 = 1 << 30, b1 = -1;
+            for (int v : per[k]) {
+                a0 = std::min(a0, xyz[3 * v + 1]);
+                a1 = std::max(a1, xyz[3 * v + 1]);
+                b0 = std::min(b0, xyz[3 * v + 2]);
+                b1 = std::max(b1, xyz[3 * v + 2]);
+            }
+            std::vector<int> r = t.rec(per[k], a0, a1, b0, b1);
+            kids.insert(kids.end(), r.begin(), r.end());
+        }
+        if (top.empty()) {
+            roots = kids;
+        } else {
+            std::sort(top.begin(), top.end(), [&](int a, int b) {
+                for (int c : {1, 2, 0})
+                    if (xyz[3 * a + c] != xyz[3 * b + c]) return xyz[3 * a + c] < xyz[3 * b + c];
+                return false;
+            });
+            const int first = (int)t.order.size();
+            t.order.insert(t.order.end(), top.begin(), top.end());
+            roots = {t.make(first, (int)top.size(), std::move(kids))};
+        }
+    } else {
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        roots = t.rec(all, ilo, ihi, jlo, jhi);
+    }
     perm = t.order;
     if ((int)perm.size() != n) fail(GT_ERR_ARG, "nested dissection lost nodes");
     std::vector<int> iperm(n);

@@ -339,10 +339,33 @@ void Schur::upload(hipStream_t st) {
     part.alloc((size_t)std::max<int64_t>(1, h.slots) * SCHUR_TILE * KMAX);
     ffin.upload(h.ffin.empty() ? std::vector<SnItem>(1) : h.ffin, st);
     bfin.upload(h.bfin.empty() ? std::vector<SnItem>(1) : h.bfin, st);
-    ubuf.alloc((size_t)(h.upd_total > 0 ? h.upd_total : 1) * KMAX);
-    zp.alloc((size_t)(h.n > 0 ? h.n : 1) * KMAX);
-    xp.alloc((size_t)(h.n > 0 ? h.n : 1) * KMAX);
+    cap = 0;
+    ensure(1);
     GT_HIP(hipStreamSynchronize(st));
+}
+
+void Schur::ensure(int nr) {
+    if (nr <= cap) return;
+    for (SchurGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+    ubuf.release();
+    zp.release();
+    xp.release();
+    ubuf.alloc((size_t)std::max<int64_t>(1, h.upd_total) * nr);
+    zp.alloc((size_t)std::max(1, h.n) * nr);
+    xp.alloc((size_t)std::max(1, h.n) * nr);
+    cap = nr;
+}
+
+void Schur::shrink(int nr) {
+    if (nr >= cap) return;
+    cap = 0;
+    ensure(std::max(1, nr));
+}
+
+int64_t Schur::bytes_after(int nr) const {
+    const int c = std::max(cap, nr);
+    return (int64_t)sizeof(zd) * c * (std::max<int64_t>(1, h.upd_total) + 2 * (int64_t)std::max(1, h.n));
 }
 
 int64_t Schur::device_bytes() const {
@@ -369,6 +392,7 @@ void launch_levels(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) {
 // sequence holds kernels only (no memset nodes). At most SCHUR_GRAPHS are kept, the oldest dropped first.
 void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t st, bool graph) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
+    S.ensure(nr);
     if (!graph) {
         launch_levels(S, b, x, nr, st);
         return;

@@ -27,8 +27,13 @@ struct gt_ctx_s {
     std::mutex mu;
     int schur_dev_min = 256;                  // fronts of at least this many rows are factorised on the device
     bool graphs = false;                      // Schur solves captured in hipGraphs and replayed (gt_ctx_graphs)
-    bool gs_delayed = true;
-    int schur_leaf = 32;                      // nested-dissection leaf size of the factorisations (gt_ctx_schur_leaf)                   // GMRES orthogonalisation: delayed CGS2, else CGS + DGKS re-pass
+    bool gs_delayed = true;                   // GMRES orthogonalisation: delayed CGS2, else CGS + DGKS re-pass
+    int schur_leaf = 32;                      // nested-dissection leaf size of the factorisations (gt_ctx_schur_leaf)
+    bool schur_sheets_apart = true;           // each sheet dissected on its own (gt_ctx_schur_ordering)
+    // Woodbury form of the stacks factored from here on (gt_ctx_woodbury): GT_WOODBURY_AUTO keeps W = B^-1 U resident
+    // while it has at most 256 columns and wb_stored_max bytes, else applies B^-1 U t by a second Schur solve
+    int woodbury = 0;
+    int64_t wb_stored_max = int64_t(16) << 30;
     std::unique_ptr<gt::GmresWork> gmres;     // GMRES device memory kept between calls (gmres.cpp)
     // L_p launch timing (gt_ctx_lp_timing): an event pair on the stream around every L_p launch while enabled
     bool lp_timing = false;
@@ -58,10 +63,13 @@ struct Work {
     DevBuf<zd> T3;            // whole column spectra of the split column pass, s_elems per right-hand side
     DevBuf<zd> s, z;          // Schur right-hand sides and solutions, interleaved (n_keep x batch)
     DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (t_cols each)
+    DevBuf<zd> z2;            // implicit Woodbury form: B^-1 U t, interleaved like z
     int t_cols = 0;
     void ensure(const gt_stack_s* st, int nr, int kt = 0);
+    // back to nr right-hand sides when the buffers hold more (set-up work that took a larger batch)
+    void shrink(const gt_stack_s* st, int nr);
     int64_t bytes() const {
-        return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes());
+        return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes() + z2.bytes());
     }
     // the bytes it would hold after ensure(st, nr, kt)
     int64_t bytes_after(const gt_stack_s* st, int nr, int kt) const;
@@ -109,6 +117,8 @@ struct gt_stack_s {
     std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
     gt::Work work;
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
+    bool wb_implicit = false;                 // W not stored: B^-1 (U t) by a second Schur solve per product
+    gt::DevBuf<double> G;                     // implicit form: U^T B^-1 U (k x k, row-major)
     int k = 0;
     int k_root = 0;                           // the columns of the first stack of its lineage (copied first into W)
     bool derived = false;                     // a what-if stack: its core belongs to the stack it came from
@@ -162,8 +172,12 @@ struct KernelGen {
 };
 // The kernel quadrant of orientation oi (its FFT sizes and twiddles set) made on the device, FP64 and FP32 copies
 void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg);
-// Woodbury columns on the device: the base's W copied (none without a base), then B^-1 U' for the added branches
+// Woodbury columns on the device: the base's W copied (none without a base), then B^-1 U' for the added branches.
+// In the implicit form (st->wb_implicit) only G = U^T B^-1 U is kept: the base's block copied, the added branches'
+// columns from the same solves, their rows by symmetry.
 void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra);
-// G = U^T W (k x k row-major) of the stack's branches, from its device W, into host memory
+// G = U^T W (k x k row-major) of the stack's branches, from its device W (or its device G), into host memory
 void woodbury_gram_device(const gt_stack* st, double* G_host);
+constexpr int WB_STORED_MAX_K = 256;     // branches of one stack with W resident (one workgroup's LDS for U^T z)
+constexpr int WB_IMPLICIT_MAX_K = 2048;  // branches of one stack in the implicit form
 }  // namespace gt

@@ -127,13 +127,18 @@ class Operator:
             pass
 
 
+_WOODBURY_FORMS = {"auto": 0, "stored": 1, "implicit": 2}
+_ND_ORDERINGS = {"joint": 0, "sheets": 1}
+
+
 class PlaneStack:
     """Plane layers and lumped branches resident on one device (peec.py:1555). Unknown order:
     [x strips | y strips | branches]."""
 
     def __init__(self, layers: list, branches: Optional[list] = None, capacitance: Optional[list] = None,
                  sigma: float = SIGMA_CU, precond: str = "diag", screen: bool = False, device: int = 0,
-                 setup: str = "device", device_fronts: int = 256, schur_leaf: int = 32):
+                 setup: str = "device", device_fronts: int = 256, schur_leaf: int = 32, woodbury: str = "auto",
+                 schur_ordering: str = "sheets"):
         if not layers:
             raise SolverError("no plane layers")
         if precond not in ("diag", "sheet"):
@@ -163,6 +168,12 @@ class PlaneStack:
         self.setup_mode = setup             # where the kernel spectra and the strips' psi are made
         self.device_fronts = int(device_fronts)     # Schur fronts of at least this many rows factorised on the device
         self.schur_leaf = int(schur_leaf)           # nodes of a nested-dissection leaf box at most
+        if woodbury not in _WOODBURY_FORMS:
+            raise ValueError("woodbury is 'auto', 'stored' or 'implicit'")
+        self.woodbury = woodbury        # the branch correction's form (gt_ctx_woodbury); woodbury_form: the one taken
+        if schur_ordering not in _ND_ORDERINGS:
+            raise ValueError("schur_ordering is 'sheets' (each sheet dissected apart) or 'joint'")
+        self.schur_ordering = schur_ordering
         self._topology(list(branches or []))
         if screen:
             self._check_no_split()
@@ -284,7 +295,8 @@ class PlaneStack:
         if np.any(self.full_label[a] != self.full_label[b]):
             return PlaneStack(self.layers, self.branches + extra, sigma=self.sigma, precond=self.precond_mode,
                               device=self.device, setup=self.setup_mode, device_fronts=self.device_fronts,
-                              schur_leaf=self.schur_leaf)
+                              schur_leaf=self.schur_leaf, woodbury=self.woodbury,
+                              schur_ordering=self.schur_ordering)
         base = self if self._base is None else self._base
         new = copy.copy(self)
         new.branches = self.branches + extra
@@ -373,6 +385,9 @@ class PlaneStack:
         _lib.check(L.gt_ctx_schur_device_fronts(self.ctx.handle, min(self.device_fronts, 2 ** 31 - 1)),
                    "gt_ctx_schur_device_fronts")
         _lib.check(L.gt_ctx_schur_leaf(self.ctx.handle, self.schur_leaf), "gt_ctx_schur_leaf")
+        _lib.check(L.gt_ctx_woodbury(self.ctx.handle, _WOODBURY_FORMS[self.woodbury], 16 << 30), "gt_ctx_woodbury")
+        _lib.check(L.gt_ctx_schur_ordering(self.ctx.handle, _ND_ORDERINGS[self.schur_ordering]),
+                   "gt_ctx_schur_ordering")
         keep_idx = np.nonzero(self.keep)[0]
         xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
                                              self.node_j[keep_idx]], axis=1).astype(np.int32))
@@ -413,6 +428,14 @@ class PlaneStack:
     def device_bytes(self) -> int:
         """Device memory this stack holds (its resident spectra, layout, factor and scratch)."""
         return int(_lib.lib().gt_stack_device_bytes(self.handle))
+
+    @property
+    def woodbury_form(self) -> str:
+        """The branch correction's form this stack took: 'stored' (W = B^-1 U resident) or 'implicit' (only
+        U^T B^-1 U kept; each preconditioner product makes a second Schur solve)."""
+        form, k = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().gt_stack_woodbury(self.handle, C.byref(form), C.byref(k)), "gt_stack_woodbury")
+        return {1: "stored", 2: "implicit"}[form.value]
 
     def schur_stats(self) -> dict:
         nnz, nsup, lev, mc = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()

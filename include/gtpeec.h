@@ -62,6 +62,24 @@ int gt_ctx_schur_device_fronts(gt_ctx* ctx, int32_t m_min);
  * (4-512; default 32) are one dense supernode each. Larger leaves give a shallower tree (fewer solve levels) and
  * denser leaf panels. (No counterpart in the reference, whose SuperLU orders by COLAMD.) */
 int gt_ctx_schur_leaf(gt_ctx* ctx, int32_t nodes);
+/* The nested-dissection ordering of the factorisations made on this context. GT_ND_SHEETS_APART (the default): the
+ * kept nodes that branches join across sheets (at most 4096) form the top separator and each sheet's grid is
+ * dissected on its own below it, so every other separator spans one sheet; past 4096 such nodes, or with
+ * GT_ND_JOINT, every separator cuts all sheets at once. (No counterpart in the reference, whose SuperLU orders by
+ * COLAMD.) */
+#define GT_ND_JOINT 0
+#define GT_ND_SHEETS_APART 1
+int gt_ctx_schur_ordering(gt_ctx* ctx, int32_t ordering);
+/* The Woodbury form of the stacks factorised on this context from now on: how the preconditioner's branch
+ * correction is applied (the reference factorises a board's branches into S and keeps a what-if's S^-1 U dense on
+ * the host, peec.py:2364-2377). GT_WOODBURY_STORED keeps W = B^-1 U resident (n_keep x k doubles, at most 256 branches); GT_WOODBURY_IMPLICIT keeps
+ * only G = U^T B^-1 U (k x k, at most 2048 branches) and applies B^-1 U t by a second Schur solve in each
+ * preconditioner product. GT_WOODBURY_AUTO (the default) stores W while k <= 256 and W takes at most
+ * stored_max_bytes (default 16 GiB). A derived stack takes its base's form. (No counterpart in the reference.) */
+#define GT_WOODBURY_AUTO 0
+#define GT_WOODBURY_STORED 1
+#define GT_WOODBURY_IMPLICIT 2
+int gt_ctx_woodbury(gt_ctx* ctx, int32_t form, int64_t stored_max_bytes);
 /* Schur solves on this context captured in hipGraphs: the level launches of each batch size are recorded once and
  * replayed with one submission; the same kernels and arguments, so the same bits. Off by default: on the MI355X a
  * replay takes as long as the launches it replaces (the levels' time is their own latency on the device, not the
@@ -169,6 +187,9 @@ int gt_stack_set_lp_plan(gt_stack* stack, const int32_t* rw, const int32_t* cw);
  * G_out: k x k (k = nbr) real, row-major: U^T B^-1 U, from which the caller forms each frequency's
  *        Q = C (I + G C)^-1 (peec.py:2357-2377's Woodbury identity). May be NULL when nbr == 0. */
 int gt_schur_factor(gt_stack* stack, const int32_t* node_xyz, double* G_out);
+/* The Woodbury form the factorised stack took (GT_WOODBURY_STORED or GT_WOODBURY_IMPLICIT) and its rank k (the
+ * branches). (No counterpart in the reference.) */
+int gt_stack_woodbury(const gt_stack* stack, int32_t* form, int32_t* k);
 /* A what-if stack (peec.py:1871-1909 `with_branches`): `base` plus n_extra branches (candidate vias) whose kept end
  * nodes are a_keep_extra / b_keep_extra (-1: a reference node), appended after the base's branches. It shares the
  * base's spectra, strip layout and Schur factor (held while any stack uses them; both order their work on the

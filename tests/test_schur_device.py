@@ -72,3 +72,24 @@ def test_another_dissection_leaf_size_gives_the_same_solution():
     for other in sols[1:]:
         for a, c in zip(other, sols[0]):
             assert np.linalg.norm(a.x - c.x) / np.linalg.norm(c.x) < 1e-9
+
+
+@pytest.mark.gpu
+def test_both_dissection_orderings_give_the_oracle_solution():
+    """gt_ctx_schur_ordering: each sheet dissected apart under a top separator of the via nodes (the default), or
+    every separator cut through all sheets. The same matrix either way; the sheet-apart separators span one sheet,
+    so the largest supernode is narrower and L has fewer entries."""
+    b = boards.scaled("four_layer", 40, n_nets=2, n_freqs=1, n_stitch=5)
+    r = b.routes[0]
+    f = float(b.freqs[0])
+    ref = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver).solve(f, tol=1e-12)
+    stats = {}
+    for ordering in ("sheets", "joint"):
+        st = peec.PlaneStack(b.layers, b.branches, schur_ordering=ordering)
+        sol = peec.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(f, tol=1e-12)
+        cur = lambda s: np.concatenate([s.I_x.ravel(), s.I_y.ravel(), s.I_branch])
+        assert np.linalg.norm(cur(sol) - cur(ref)) / np.linalg.norm(cur(ref)) < 1e-9, ordering
+        assert abs(sol.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9, ordering
+        stats[ordering] = st.schur_stats()
+    assert stats["sheets"]["max_cols"] < stats["joint"]["max_cols"]
+    assert stats["sheets"]["nnz_L"] < stats["joint"]["nnz_L"]

@@ -1,0 +1,112 @@
+"""The Woodbury form of the preconditioner's branch correction (gt_ctx_woodbury).
+
+The Schur complement is S = d_s (B + U C U^T): B the Laplacian factored once, U the branches' incidence, C each
+frequency's diagonal. Its inverse needs B^-1 U. The stored form keeps W = B^-1 U resident (n_keep x k doubles).
+The implicit form keeps only G = U^T B^-1 U (k x k) and applies B^-1 (U t) by a second Schur solve in each
+product. A board with many branches on many nodes (six layers of 2048^2 cells, 999 via branches: W would be
+200 GB) takes the implicit form. The reference forms S^-1 U densely on the host for a what-if's added branches
+(peec.py:2364-2377) and factorises a board's own branches into S.
+
+GPU:
+- both forms' products and solves match the oracle, on a board and on a what-if derived from it;
+- the implicit G equals the stored G (the same solves);
+- a board with more than 256 branches takes the implicit form and solves to the oracle;
+- a batch of implicit what-ifs gives each solve's bits alone.
+"""
+import numpy as np
+import pytest
+
+from groundtruth_amd import boards, peec
+from oracle import peec as O
+from test_whatif import _extra_vias
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def _cur(s):
+    return np.concatenate([s.I_x.ravel(), s.I_y.ravel(), s.I_branch])
+
+
+def _board():
+    return boards.scaled("two_layer", 20, n_vias=5, n_freqs=1)
+
+
+@pytest.mark.gpu
+def test_both_forms_give_the_oracle_products_and_the_same_gram_matrix():
+    b = _board()
+    extra = _extra_vias(b, 3, 7)
+    st_s = peec.PlaneStack(b.layers, b.branches, woodbury="stored")
+    st_i = peec.PlaneStack(b.layers, b.branches, woodbury="implicit")
+    d_s, d_i = st_s.with_branches(extra), st_i.with_branches(extra)
+    assert st_s.woodbury_form == "stored" and st_i.woodbury_form == "implicit"
+    assert d_s.woodbury_form == "stored" and d_i.woodbury_form == "implicit"
+    # the root's G: the same solves and the same differences
+    assert np.array_equal(st_i.G, st_s.G)
+    # a what-if's added rows come from symmetry in the implicit form
+    assert rel(d_i.G, d_s.G) < 1e-13
+    ref = O.PlaneStack(b.layers, b.branches).with_branches(extra)
+    hm = O.HostMath(ref)
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(d_i.nb + d_i.n_keep) + 1j * rng.standard_normal(d_i.nb + d_i.n_keep)
+    for f in (1e6, 3e8):
+        rop = ref.operator(f)
+        pi = d_i.precond(d_i.operator(f), x)
+        assert rel(pi, hm.precond(rop, x)) < 1e-10
+        assert rel(pi, d_s.precond(d_s.operator(f), x)) < 1e-12
+        assert np.array_equal(d_i.matvec(d_i.operator(f), x), d_s.matvec(d_s.operator(f), x))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f", [1e6, 1e9])
+def test_implicit_board_and_whatif_solutions_match_the_oracle(f):
+    b = _board()
+    extra = _extra_vias(b, 4, 13)
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches, woodbury="implicit")
+    d = st.with_branches(extra)
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    for stack, branches, e in ((st, b.branches, ex), (d, b.branches + extra, ex.on(d))):
+        sol = e.solve(f, tol=1e-12)
+        ref = O.RouteExcitation(O.PlaneStack(b.layers, branches), r.trace, r.driver, r.receiver).solve(f, tol=1e-12)
+        assert rel(_cur(sol), _cur(ref)) < 1e-9
+        assert abs(sol.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9
+        assert abs(sol.iterations - ref.iterations) <= 2
+
+
+@pytest.mark.gpu
+def test_a_board_with_more_than_256_branches_takes_the_implicit_form():
+    b = boards.scaled("two_layer", 40, n_vias=300, n_freqs=1)
+    assert len(b.branches) > 256
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    sol = peec.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(2e8, tol=1e-12)
+    assert st.woodbury_form == "implicit"
+    with pytest.raises(Exception):
+        peec.PlaneStack(b.layers, b.branches, woodbury="stored").handle
+    ref = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver).solve(2e8, tol=1e-12)
+    assert rel(_cur(sol), _cur(ref)) < 1e-9
+    assert abs(sol.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9
+
+
+@pytest.mark.gpu
+def test_implicit_whatifs_in_one_batch_give_the_bits_of_each_alone():
+    import torch
+    from groundtruth_amd.serve import Session
+    from test_serve import _rhs, _run
+    b = _board()
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches, woodbury="implicit")
+    ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    stacks = [st] + [st.with_branches(_extra_vias(b, k, 500 + k)) for k in (1, 3)]
+    f = 3e7
+    jobs = [(s_, e, f) + _rhs(s_, e, f) for s_, e in ((s_, ex if s_ is st else ex.on(s_)) for s_ in stacks)]
+    torch.cuda.synchronize()
+    ses = Session(st, capacity=4, max_extra=3)
+    out = _run(ses, jobs, 1e-10)
+    for i, (s_, e, f_, bd, xd, bn) in enumerate(jobs):
+        one = e.solve(f_, tol=1e-10)
+        assert out[i].converged and out[i].iterations == one.iterations, i
+        assert np.array_equal(xd.cpu().numpy(), one.x), i
+    ses.close()
