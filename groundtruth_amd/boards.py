@@ -232,7 +232,8 @@ def large(n: int = 2048, pitch_mm: float = 0.05, seed: int = 1005, n_vias: int =
         branches += _stitch(layers, i, j, f"via{v}")
     pts = _manhattan(rng, n, n, 8, n // 16, (n // 32, n // 8))
     route = _route(pts, 0.2e-3, zs[0], d, 0.15e-3, 0)
-    return Board("large", layers, branches, [route], [1e9], tol=1e-8, restart=50, seed=seed)
+    # GMRES(100): the reference's default restart at this size (min(200, max(100, 4e8 / 16 n)), peec.py:3087-3090)
+    return Board("large", layers, branches, [route], [1e9], tol=1e-8, restart=100, seed=seed)
 
 
 WORKLOADS = {"single_plane": single_plane, "two_layer": two_layer, "four_layer": four_layer, "large": large}

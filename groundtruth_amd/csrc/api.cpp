@@ -192,7 +192,7 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
 // stack needs resident)
 int64_t gt_stack_s::device_bytes() const {
     int64_t b = a_keep.bytes() + b_keep.bytes() + adj_ptr.bytes() + adj.bytes();
-    b += work.bytes() + W.bytes() + G.bytes();
+    b += work.bytes() + W.bytes() + G.bytes() + Yr.bytes();
     if (derived) return b;                 // the core is its base's
     for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + h.K.bytes() + h.Kf.bytes();
     for (const auto& t : tw) b += t.second.bytes();
@@ -320,7 +320,7 @@ int gt_stack_woodbury(const gt_stack* st, int32_t* form, int32_t* k) {
     return guard([&] {
         need(st && form && k, "NULL argument");
         need(st->schur != nullptr, "the stack has no Schur factor (gt_schur_factor first)");
-        *form = st->wb_implicit ? GT_WOODBURY_IMPLICIT : GT_WOODBURY_STORED;
+        *form = st->wb_root ? GT_WOODBURY_ROOT : st->wb_implicit ? GT_WOODBURY_IMPLICIT : GT_WOODBURY_STORED;
         *k = st->k;
     });
 }

@@ -137,37 +137,62 @@ __global__ __launch_bounds__(NTH) void lp_cols_split_fft(LpArgs<T> a) {
     }
 }
 
-template <class T, int MAXS>
-__global__ __launch_bounds__(256) void lp_cols_split_mix(LpArgs<T> a, long pix0) {
+// The split pass's multiply-add over source sheets: one thread per kernel-quadrant pixel (qi, qj) takes the up to
+// four pixels (i, j) in {qi, P0 - qi} x {qj, P1 - qj} that read its S x S kernel values, so the kernel is read from
+// HBM once per product (the values stay in registers over the four)
+template <class T, int S>
+__global__ __launch_bounds__(256) void lp_cols_split_mix(LpArgs<T> a, long q0) {
     long idx = blockIdx.x * 256L + threadIdx.x;
     int oi = 0;
-    if (idx >= pix0) { idx -= pix0; oi = 1; }
+    if (idx >= q0) { idx -= q0; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
-    if (a.nblk_cols[oi] == 0 || idx >= (long)o.P0 * o.P1) return;
-    const int S = a.S, P0 = o.P0;
-    const int j = (int)(idx / P0), i = (int)(idx - (long)j * P0);
-    cpx<T>* T3 = o.T3 + blockIdx.y * a.s_stride + (size_t)j * P0 + i;
-    const size_t sheet = (size_t)o.P1 * P0;
-    const int qj = min(j, o.P1 - j), qi = min(i, P0 - i), h0 = P0 / 2 + 1;
+    const int P0 = o.P0, P1 = o.P1, h0 = P0 / 2 + 1, h1 = P1 / 2 + 1;
+    if (a.nblk_cols[oi] == 0 || idx >= (long)h0 * h1) return;
+    const int qj = (int)(idx / h0), qi = (int)(idx - (long)qj * h0);
     const T* Kp = o.K + ((size_t)qj * h0 + qi) * S * S;
-    cpx<T> v[MAXS];
+    T k[S * S];
 #pragma unroll
-    for (int s = 0; s < MAXS; ++s)
-        if (s < S) v[s] = T3[s * sheet];
+    for (int e = 0; e < S * S; ++e) k[e] = Kp[e];
+    cpx<T>* base = o.T3 + blockIdx.y * a.s_stride;
+    const size_t sheet = (size_t)P1 * P0;
+    const int ni = (qi == 0 || 2 * qi == P0) ? 1 : 2, nj = (qj == 0 || 2 * qj == P1) ? 1 : 2;
+    for (int b = 0; b < nj; ++b)
+        for (int c = 0; c < ni; ++c) {
+            cpx<T>* T3 = base + (size_t)(b ? P1 - qj : qj) * P0 + (c ? P0 - qi : qi);
+            cpx<T> v[S];
 #pragma unroll
-    for (int t = 0; t < MAXS; ++t) {
-        if (t < S) {
-            cpx<T> acc;
+            for (int s = 0; s < S; ++s) v[s] = T3[s * sheet];
 #pragma unroll
-            for (int s = 0; s < MAXS; ++s) {
-                if (s < S) {
-                    const T k = Kp[t * S + s];
-                    acc.re += k * v[s].re;
-                    acc.im += k * v[s].im;
+            for (int t = 0; t < S; ++t) {
+                cpx<T> acc;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    acc.re += k[t * S + s] * v[s].re;
+                    acc.im += k[t * S + s] * v[s].im;
                 }
+                T3[t * sheet] = acc;
             }
-            T3[t * sheet] = acc;
         }
+}
+
+template <class T, int S>
+void launch_split_mix(const LpArgs<T>& a, int nr, hipStream_t st) {
+    auto quad = [&](int oi) { return a.nblk_cols[oi] ? (long)(a.o[oi].P0 / 2 + 1) * (a.o[oi].P1 / 2 + 1) : 0L; };
+    const long q0 = quad(0), q = q0 + quad(1);
+    hipLaunchKernelGGL((lp_cols_split_mix<T, S>), dim3((unsigned)((q + 255) / 256), nr), dim3(256), 0, st, a, q0);
+}
+
+template <class T>
+void split_mix(const LpArgs<T>& a, int nr, hipStream_t st) {
+    switch (a.S) {
+        case 1: launch_split_mix<T, 1>(a, nr, st); break;
+        case 2: launch_split_mix<T, 2>(a, nr, st); break;
+        case 3: launch_split_mix<T, 3>(a, nr, st); break;
+        case 4: launch_split_mix<T, 4>(a, nr, st); break;
+        case 5: launch_split_mix<T, 5>(a, nr, st); break;
+        case 6: launch_split_mix<T, 6>(a, nr, st); break;
+        case 7: launch_split_mix<T, 7>(a, nr, st); break;
+        default: launch_split_mix<T, LP_MAX_SHEETS>(a, nr, st); break;
     }
 }
 
@@ -534,10 +559,7 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
         const size_t lds = (size_t)P0 * sizeof(cpx<T>);
         hipLaunchKernelGGL((lp_cols_split_fft<T, NTH, LpPts<T>::cols, false>), dim3(items, nr), dim3(NTH), lds, st, a);
         GT_LAUNCH("lp_cols_split_fft");
-        const long pix0 = a.nblk_cols[0] ? (long)a.o[0].P0 * a.o[0].P1 : 0;
-        const long pix = pix0 + (a.nblk_cols[1] ? (long)a.o[1].P0 * a.o[1].P1 : 0);
-        hipLaunchKernelGGL((lp_cols_split_mix<T, LP_MAX_SHEETS>), dim3((unsigned)((pix + 255) / 256), nr), dim3(256), 0,
-                           st, a, pix0);
+        split_mix(a, nr, st);
         GT_LAUNCH("lp_cols_split_mix");
         hipLaunchKernelGGL((lp_cols_split_fft<T, NTH, LpPts<T>::cols, true>), dim3(items, nr), dim3(NTH), lds, st, a);
         GT_LAUNCH("lp_cols_split_fft");

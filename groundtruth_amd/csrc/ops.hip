@@ -37,6 +37,14 @@ struct PreArgs {
     const zd* z;              // nk x nr interleaved: B^-1 s
     zd* t;                    // nr x kt: Q (U^T z)
     const zd* z2;             // implicit form: B^-1 (U t), interleaved like z
+    // root form (right-hand sides with root[p]): between the Schur solve's passes, u = Y_r^T w_r, t = Q u,
+    // w_r -= Y_r t on the forward result zp's last-supernode rows
+    int root[KMAX];
+    int qmode;                // which right-hand sides pre_woodbury_q takes: 0 all, 1 the root form's, 2 the others
+    const double* Yr;
+    int yr_c0, yr_n, yr_k;
+    zd* zp;
+    zd* u;
     const double* W;          // the shared columns: k_sh of nk
     const double* Wx[KMAX];   // each right-hand side's own columns k_sh .. k - 1
     const int* a_keep[KMAX];
@@ -102,13 +110,17 @@ __global__ __launch_bounds__(256) void pre_woodbury_q(PreArgs<T> a) {
     zd* ut = reinterpret_cast<zd*>(ut_raw);
     const int p = blockIdx.x;
     const int k = a.k[p], i0 = blockIdx.y * 32;
-    if (i0 >= k) return;
+    if (i0 >= k || (a.qmode == 1 && !a.root[p]) || (a.qmode == 2 && a.root[p])) return;
     for (int m = threadIdx.x; m < k; m += 256) {
-        const int b = a.ns + m;
-        const int ka = a.a_keep[p][b], kb = a.b_keep[p][b];
         zd v;
-        if (ka >= 0) v = v + a.z[(size_t)ka * a.nr + p];
-        if (kb >= 0) v = v - a.z[(size_t)kb * a.nr + p];
+        if (a.root[p]) {
+            v = a.u[p * a.kt + m];
+        } else {
+            const int b = a.ns + m;
+            const int ka = a.a_keep[p][b], kb = a.b_keep[p][b];
+            if (ka >= 0) v = v + a.z[(size_t)ka * a.nr + p];
+            if (kb >= 0) v = v - a.z[(size_t)kb * a.nr + p];
+        }
         ut[m] = v;
     }
     __syncthreads();
@@ -137,7 +149,7 @@ __global__ __launch_bounds__(256) void pre_woodbury_scatter(PreArgs<T> a) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= a.nk) return;
     for (int p = 0; p < a.nr; ++p) {
-        const int e0 = a.adj_ptr[p][n], e1 = a.adj_ptr[p][n + 1];
+        const int e0 = a.adj_ptr[p][n], e1 = a.root[p] ? e0 : a.adj_ptr[p][n + 1];
         const int* adj = a.adj[p];
         const zd* tp = a.t + p * a.kt;
         zd acc;
@@ -151,14 +163,65 @@ __global__ __launch_bounds__(256) void pre_woodbury_scatter(PreArgs<T> a) {
     }
 }
 
-// The implicit form's y_p = s_scale_p (z_p - B^-1 U_p t_p)
+// The implicit form's y_p = s_scale_p (z_p - B^-1 U_p t_p); the root form's (and every one's without a second solve,
+// z2 null) y_p = s_scale_p z_p
 template <class T>
 __global__ __launch_bounds__(256) void pre_woodbury_diff(PreArgs<T> a) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= a.nk) return;
     for (int p = 0; p < a.nr; ++p) {
         const size_t i = (size_t)n * a.nr + p;
-        a.out[p][a.nb[p] + n] = stz<T>(a.s_scale[p] * (a.z[i] - a.z2[i]));
+        const zd v = a.z2 && !a.root[p] ? a.z[i] - a.z2[i] : a.z[i];
+        a.out[p][a.nb[p] + n] = stz<T>(a.s_scale[p] * v);
+    }
+}
+
+// The root form's u_p = Y_r^T w_p,r: one workgroup per column m, the yr_n-long sums strided over the threads and
+// added by a fixed tree
+template <class T>
+__global__ __launch_bounds__(256) void pre_root_u(PreArgs<T> a) {
+    __shared__ double red[2 * 256];
+    const int m = blockIdx.x;
+    const double* y = a.Yr + (size_t)m * a.yr_n;
+    for (int p = 0; p < a.nr; ++p) {
+        if (!a.root[p]) continue;
+        double re = 0, im = 0;
+        for (int r = threadIdx.x; r < a.yr_n; r += 256) {
+            const zd w = a.zp[(size_t)(a.yr_c0 + r) * a.nr + p];
+            re += y[r] * w.re;
+            im += y[r] * w.im;
+        }
+        red[threadIdx.x] = re;
+        red[256 + threadIdx.x] = im;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) {
+                red[threadIdx.x] += red[threadIdx.x + h];
+                red[256 + threadIdx.x] += red[256 + threadIdx.x + h];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.u[p * a.kt + m] = zd(red[0], red[256]);
+        __syncthreads();
+    }
+}
+
+// The root form's w_p,r -= Y_r t_p: a thread per row r of the last supernode, the sum over m in ascending order
+template <class T>
+__global__ __launch_bounds__(256) void pre_root_sub(PreArgs<T> a) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= a.yr_n) return;
+    for (int p = 0; p < a.nr; ++p) {
+        if (!a.root[p]) continue;
+        const zd* tp = a.t + p * a.kt;
+        double re = 0, im = 0;
+        for (int m = 0; m < a.yr_k; ++m) {
+            const double y = a.Yr[(size_t)m * a.yr_n + r];
+            re += y * tp[m].re;
+            im += y * tp[m].im;
+        }
+        zd& w = a.zp[(size_t)(a.yr_c0 + r) * a.nr + p];
+        w = w - zd(re, im);
     }
 }
 
@@ -247,6 +310,7 @@ void Work::ensure(const gt_stack* st, int nr, int kt) {
     if (nr > cap || kt > t_cols) {
         t_cols = std::max(kt, t_cols);
         t.alloc((size_t)std::max(1, t_cols) * std::max(nr, cap));
+        u.alloc((size_t)std::max(1, t_cols) * std::max(nr, cap));
     }
     cap = std::max(cap, nr);
     if (st->wb_implicit && z2.n < (size_t)std::max(1, st->nk) * cap) z2.alloc((size_t)std::max(1, st->nk) * cap);
@@ -262,6 +326,7 @@ void Work::shrink(const gt_stack* st, int nr) {
     z.release();
     z2.release();
     t.release();
+    u.release();
     cap = 0;
     t_cols = 0;
     if (nr > 0) ensure(st, nr, kt);
@@ -273,8 +338,8 @@ int64_t Work::bytes_after(const gt_stack* st, int nr, int kt) const {
     int64_t b = nr > cap ? (int64_t)(2 * std::max<size_t>(1, st->t_elems) + st->s_elems + 2 * (size_t)std::max(1, st->nk)) *
                                nr * zb
                          : (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes());
-    if (nr > cap || kt > t_cols) b += (int64_t)std::max(1, std::max(kt, t_cols)) * std::max(nr, cap) * zb;
-    else b += (int64_t)t.bytes();
+    if (nr > cap || kt > t_cols) b += 2 * (int64_t)std::max(1, std::max(kt, t_cols)) * std::max(nr, cap) * zb;
+    else b += (int64_t)(t.bytes() + u.bytes());
     if (st->wb_implicit) b += std::max<int64_t>((int64_t)z2.bytes(), (int64_t)std::max(1, st->nk) * std::max(nr, cap) * zb);
     return b;
 }
@@ -467,6 +532,40 @@ __global__ __launch_bounds__(256) void wb_gram_mirror(double* __restrict__ G, in
     G[(size_t)c * k + r] = G[(size_t)r * k + c];
 }
 
+// The root form's Y_r columns c0 .. c0 + nr - 1: Re of the forward results' last-supernode rows
+__global__ __launch_bounds__(256) void wb_take_root(const zd* __restrict__ zp, int nr, int yc0, int yn, int c0,
+                                                    double* __restrict__ Yr) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= yn * nr) return;
+    const int r = i / nr, c = i % nr;
+    Yr[(size_t)(c0 + c) * yn + r] = zp[(size_t)(yc0 + r) * nr + c].re;
+}
+
+// The root form's G = Y_r^T Y_r (k x k row-major; each entry's sum over rows in ascending order, so G is symmetric)
+__global__ __launch_bounds__(256) void wb_gram_root(const double* __restrict__ Yr, int yn, int k, double* __restrict__ G) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= k * k) return;
+    const int r = i / k, c = i % k;
+    const double* a = Yr + (size_t)min(r, c) * yn;
+    const double* b = Yr + (size_t)max(r, c) * yn;
+    double v = 0;
+    for (int q = 0; q < yn; ++q) v += a[q] * b[q];
+    G[i] = v;
+}
+
+// Every end of the branches in the factor's last supernode (whose rows are then Y's only nonzero ones)
+bool ends_in_last_supernode(const gt_stack* st, int n, const int* ka, const int* kb) {
+    const SchurHost& H = st->schur->h;
+    if (H.c0.empty()) return false;
+    const int c0 = H.c0.back();
+    if (c0 + H.nc.back() != H.n) return false;
+    std::vector<char> last(H.n, 0);
+    for (int c = c0; c < H.n; ++c) last[H.perm[c]] = 1;
+    for (int m = 0; m < n; ++m)
+        if ((ka[m] >= 0 && !last[ka[m]]) || (kb[m] >= 0 && !last[kb[m]])) return false;
+    return true;
+}
+
 }  // namespace
 
 void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int* a_extra, const int* b_extra) {
@@ -488,9 +587,16 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
         if (k0 > 0 && nk > 0)
             GT_HIP(hipMemcpyAsync(st->W.p, base->W.p, sizeof(double) * (size_t)k0 * nk, hipMemcpyDeviceToDevice, s));
     }
+    st->wb_root = imp && !base && n_extra > 0 && nk > 0 && ends_in_last_supernode(st, n_extra, a_extra, b_extra);
+    if (st->wb_root) {
+        const SchurHost& H = st->schur->h;
+        st->yr_c0 = H.c0.back();
+        st->yr_n = H.n - st->yr_c0;
+        st->Yr.alloc((size_t)st->yr_n * k);
+    }
     if (n_extra > 0 && nk > 0) {
         // the columns' solves in batches of up to 16, on scratch of their own (the product workspace and the
-        // Schur solve's buffers go back to what they held)
+        // Schur solve's buffers go back to what they held); the root form takes only the forward passes
         const int chunk = std::min(n_extra, KMAX);
         const int schur_cap = st->schur->cap;
         DevBuf<zd> sb((size_t)nk * chunk), zb((size_t)nk * chunk);
@@ -504,6 +610,13 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
             const size_t cnt = (size_t)nk * nr;
             hipLaunchKernelGGL(wb_units, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, sb.p, nk, nr, u);
             GT_LAUNCH("wb_units");
+            if (st->wb_root) {
+                schur_forward_device(*st->schur, sb.p, nr, s);
+                hipLaunchKernelGGL(wb_take_root, dim3(blocks_for((size_t)st->yr_n * nr, 256)), dim3(256), 0, s,
+                                   st->schur->zp.p, nr, st->yr_c0, st->yr_n, m0, st->Yr.p);
+                GT_LAUNCH("wb_take_root");
+                continue;
+            }
             schur_apply(st, sb.p, zb.p, nr);
             if (imp) {
                 hipLaunchKernelGGL(wb_gram_cols, dim3(blocks_for((size_t)k * nr, 256)), dim3(256), 0, s, zb.p, nr,
@@ -514,6 +627,11 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
                                    st->W.p + (size_t)(k0 + m0) * nk);
                 GT_LAUNCH("wb_take");
             }
+        }
+        if (st->wb_root) {
+            hipLaunchKernelGGL(wb_gram_root, dim3(blocks_for((size_t)k * k, 256)), dim3(256), 0, s, st->Yr.p,
+                               st->yr_n, k, st->G.p);
+            GT_LAUNCH("wb_gram_root");
         }
         GT_HIP(hipStreamSynchronize(s));
         st->schur->shrink(schur_cap);
@@ -586,17 +704,48 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
     if (st->nk > 0) {
         hipLaunchKernelGGL(pre_schur_rhs<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
         GT_LAUNCH("pre_schur_rhs");
-        schur_apply(st, st->work.s.p, st->work.z.p, nr);
         if (imp && a.kt > 0) {
-            a.z2 = st->work.z2.p;
-            hipLaunchKernelGGL(pre_woodbury_q<T>, dim3(nr, (a.kt + 31) / 32), dim3(256), 0, s, a);
-            GT_LAUNCH("pre_woodbury_q");
-            hipLaunchKernelGGL(pre_woodbury_scatter<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
-            GT_LAUNCH("pre_woodbury_scatter");
-            schur_apply(st, st->work.s.p, st->work.z2.p, nr);
+            // the host stack's own right-hand sides in the root form, the others (what-ifs) with a second solve;
+            // each right-hand side's arithmetic is the same in any batch
+            bool any_root = false, any_second = false;
+            for (int p = 0; p < nr; ++p) {
+                a.root[p] = ops[p]->st == st && st->wb_root && a.k[p] > 0;
+                any_root = any_root || a.root[p];
+                any_second = any_second || (!a.root[p] && a.k[p] > 0);
+            }
+            if (any_root) {
+                Schur& S = *st->schur;
+                a.Yr = st->Yr.p;
+                a.yr_c0 = st->yr_c0;
+                a.yr_n = st->yr_n;
+                a.yr_k = st->k;
+                a.u = st->work.u.p;
+                schur_forward_device(S, st->work.s.p, nr, s);
+                a.zp = S.zp.p;
+                hipLaunchKernelGGL(pre_root_u<T>, dim3(st->k), dim3(256), 0, s, a);
+                GT_LAUNCH("pre_root_u");
+                a.qmode = 1;
+                hipLaunchKernelGGL(pre_woodbury_q<T>, dim3(nr, (a.kt + 31) / 32), dim3(256), 0, s, a);
+                GT_LAUNCH("pre_woodbury_q");
+                hipLaunchKernelGGL(pre_root_sub<T>, dim3(blocks_for(st->yr_n, 256)), dim3(256), 0, s, a);
+                GT_LAUNCH("pre_root_sub");
+                schur_backward_device(S, st->work.z.p, nr, s);
+            } else {
+                schur_apply(st, st->work.s.p, st->work.z.p, nr);
+            }
+            if (any_second) {
+                a.z2 = st->work.z2.p;
+                a.qmode = 2;
+                hipLaunchKernelGGL(pre_woodbury_q<T>, dim3(nr, (a.kt + 31) / 32), dim3(256), 0, s, a);
+                GT_LAUNCH("pre_woodbury_q");
+                hipLaunchKernelGGL(pre_woodbury_scatter<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+                GT_LAUNCH("pre_woodbury_scatter");
+                schur_apply(st, st->work.s.p, st->work.z2.p, nr);
+            }
             hipLaunchKernelGGL(pre_woodbury_diff<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
             GT_LAUNCH("pre_woodbury_diff");
         } else {
+            schur_apply(st, st->work.s.p, st->work.z.p, nr);
             if (a.kt > 0) {
                 hipLaunchKernelGGL(pre_woodbury_small<T>, dim3(nr), dim3(256), 0, s, a);
                 GT_LAUNCH("pre_woodbury_small");

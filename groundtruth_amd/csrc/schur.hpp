@@ -146,5 +146,10 @@ struct Schur {
 // x = B^-1 b on the device for nr <= 16 right-hand sides, interleaved: entry (node, p) at node * nr + p, in the old
 // numbering (b and x distinct)
 void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t s, bool graph);
+// The two passes of that solve apart (the same kernels, so the same bits): the forward pass leaves M^-1 P b in zp
+// (new numbering, interleaved; M the factor, B = P^T M M^T P), the backward pass writes x = P^T M^-T zp. Between them
+// a caller may change zp (the Woodbury correction on the last supernode's rows).
+void schur_forward_device(Schur& S, const zd* b, int nr, hipStream_t s);
+void schur_backward_device(Schur& S, zd* x, int nr, hipStream_t s);
 
 }  // namespace gt

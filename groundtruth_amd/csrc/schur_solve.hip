@@ -12,6 +12,8 @@
 // split over several chunks writes partial sums, which a finalize launch after the level adds in chunk order. Every
 // sum runs in a fixed order that does not depend on the batch: a right-hand side gets the same
 // bits alone or in any batch.
+#include <type_traits>
+
 #include "lp.hpp"
 #include "schur.hpp"
 
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(NTH) void schur_fin(SolveArgs a, int item0, int nr,
 }
 
 template <int KB>
-void run_levels(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
+void run_fwd(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
     constexpr size_t row_bytes = (KB + 1) * sizeof(zd);
     const std::vector<int>& fp = S.h.fwd_ptr;
     a.item = S.fwd.p;
@@ -308,6 +310,11 @@ void run_levels(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
             GT_LAUNCH("schur_fin");
         }
     }
+}
+
+template <int KB>
+void run_bwd(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
+    constexpr size_t row_bytes = (KB + 1) * sizeof(zd);
     const std::vector<int>& bp = S.h.bwd_ptr;
     a.item = S.bwd.p;
     for (size_t l = 0; l + 1 < bp.size(); ++l) {
@@ -375,15 +382,24 @@ int64_t Schur::device_bytes() const {
 
 namespace {
 
-void launch_levels(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) {
+// the passes for nr right-hand sides: forward (pass 1), backward (pass 2) or both (3), with the kernels' batch width
+// the next power of two
+void launch_passes(Schur& S, const zd* b, zd* x, int nr, hipStream_t st, int passes) {
     SolveArgs a{nullptr, S.perm.p, S.rows.p, reinterpret_cast<const int2*>(S.ctab.p), S.cidx.p, S.panel.p, S.ubuf.p,
                 S.zp.p, S.xp.p, b, x, S.part.p};
-    if (nr == 1) run_levels<1>(S, a, nr, st);
-    else if (nr == 2) run_levels<2>(S, a, nr, st);
-    else if (nr <= 4) run_levels<4>(S, a, nr, st);
-    else if (nr <= 8) run_levels<8>(S, a, nr, st);
-    else run_levels<16>(S, a, nr, st);
+    auto go = [&](auto kb) {
+        constexpr int KB = decltype(kb)::value;
+        if (passes & 1) run_fwd<KB>(S, a, nr, st);
+        if (passes & 2) run_bwd<KB>(S, a, nr, st);
+    };
+    if (nr == 1) go(std::integral_constant<int, 1>());
+    else if (nr == 2) go(std::integral_constant<int, 2>());
+    else if (nr <= 4) go(std::integral_constant<int, 4>());
+    else if (nr <= 8) go(std::integral_constant<int, 8>());
+    else go(std::integral_constant<int, 16>());
 }
+
+void launch_levels(Schur& S, const zd* b, zd* x, int nr, hipStream_t st) { launch_passes(S, b, x, nr, st, 3); }
 
 }  // namespace
 
@@ -422,6 +438,18 @@ void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t st, bo
     }
     S.graphs.push_back({nr, b, x, exec});
     GT_HIP(hipGraphLaunch(exec, st));
+}
+
+void schur_forward_device(Schur& S, const zd* b, int nr, hipStream_t st) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
+    S.ensure(nr);
+    launch_passes(S, b, nullptr, nr, st, 1);
+}
+
+void schur_backward_device(Schur& S, zd* x, int nr, hipStream_t st) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
+    if (nr > S.cap) fail(GT_ERR_ARG, "Schur backward pass without its forward pass");
+    launch_passes(S, nullptr, x, nr, st, 2);
 }
 
 Schur::~Schur() {

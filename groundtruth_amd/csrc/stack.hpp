@@ -64,12 +64,14 @@ struct Work {
     DevBuf<zd> s, z;          // Schur right-hand sides and solutions, interleaved (n_keep x batch)
     DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (t_cols each)
     DevBuf<zd> z2;            // implicit Woodbury form: B^-1 U t, interleaved like z
+    DevBuf<zd> u;             // root form: Y_r^T w per right-hand side (t_cols each)
     int t_cols = 0;
     void ensure(const gt_stack_s* st, int nr, int kt = 0);
     // back to nr right-hand sides when the buffers hold more (set-up work that took a larger batch)
     void shrink(const gt_stack_s* st, int nr);
     int64_t bytes() const {
-        return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes() + z2.bytes());
+        return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes() + z2.bytes() +
+                         u.bytes());
     }
     // the bytes it would hold after ensure(st, nr, kt)
     int64_t bytes_after(const gt_stack_s* st, int nr, int kt) const;
@@ -119,6 +121,12 @@ struct gt_stack_s {
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
     bool wb_implicit = false;                 // W not stored: B^-1 (U t) by a second Schur solve per product
     gt::DevBuf<double> G;                     // implicit form: U^T B^-1 U (k x k, row-major)
+    // Root form of the implicit one, when every branch end lies in the factor's last supernode: with Y = M^-1 P U
+    // (nonzero on that supernode's yr_n rows only) the correction is w_r -= Y_r Q Y_r^T w_r between the solve's
+    // passes, and G = Y_r^T Y_r
+    bool wb_root = false;
+    gt::DevBuf<double> Yr;                    // k columns of yr_n (column-major)
+    int yr_c0 = 0, yr_n = 0;
     int k = 0;
     int k_root = 0;                           // the columns of the first stack of its lineage (copied first into W)
     bool derived = false;                     // a what-if stack: its core belongs to the stack it came from

@@ -431,11 +431,12 @@ class PlaneStack:
 
     @property
     def woodbury_form(self) -> str:
-        """The branch correction's form this stack took: 'stored' (W = B^-1 U resident) or 'implicit' (only
-        U^T B^-1 U kept; each preconditioner product makes a second Schur solve)."""
+        """The branch correction's form this stack took: 'stored' (W = B^-1 U resident), 'implicit' (only
+        U^T B^-1 U kept; each preconditioner product makes a second Schur solve) or 'root' (every branch end in the
+        factor's last supernode: the correction runs on that supernode's rows between the solve's passes)."""
         form, k = C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().gt_stack_woodbury(self.handle, C.byref(form), C.byref(k)), "gt_stack_woodbury")
-        return {1: "stored", 2: "implicit"}[form.value]
+        return {1: "stored", 2: "implicit", 3: "root"}[form.value]
 
     def schur_stats(self) -> dict:
         nnz, nsup, lev, mc = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()

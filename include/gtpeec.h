@@ -80,6 +80,12 @@ int gt_ctx_schur_ordering(gt_ctx* ctx, int32_t ordering);
 #define GT_WOODBURY_STORED 1
 #define GT_WOODBURY_IMPLICIT 2
 int gt_ctx_woodbury(gt_ctx* ctx, int32_t form, int64_t stored_max_bytes);
+/* A stack in the implicit form whose branch ends all lie in the factor's last supernode (the sheet-apart ordering
+ * puts every node a branch joins across sheets there) takes its root form, reported by gt_stack_woodbury as
+ * GT_WOODBURY_ROOT: with Y = M^-1 P U (M the factor, B = P^T M M^T P), nonzero on that supernode's rows only, the
+ * correction is w -= Y Q Y^T w between the Schur solve's forward and backward passes (one solve per product, no
+ * second one) and G = Y^T Y. A what-if derived from it takes the second solve for its own right-hand sides. */
+#define GT_WOODBURY_ROOT 3
 /* Schur solves on this context captured in hipGraphs: the level launches of each batch size are recorded once and
  * replayed with one submission; the same kernels and arguments, so the same bits. Off by default: on the MI355X a
  * replay takes as long as the launches it replaces (the levels' time is their own latency on the device, not the
@@ -187,8 +193,8 @@ int gt_stack_set_lp_plan(gt_stack* stack, const int32_t* rw, const int32_t* cw);
  * G_out: k x k (k = nbr) real, row-major: U^T B^-1 U, from which the caller forms each frequency's
  *        Q = C (I + G C)^-1 (peec.py:2357-2377's Woodbury identity). May be NULL when nbr == 0. */
 int gt_schur_factor(gt_stack* stack, const int32_t* node_xyz, double* G_out);
-/* The Woodbury form the factorised stack took (GT_WOODBURY_STORED or GT_WOODBURY_IMPLICIT) and its rank k (the
- * branches). (No counterpart in the reference.) */
+/* The Woodbury form the factorised stack took (GT_WOODBURY_STORED, GT_WOODBURY_IMPLICIT or GT_WOODBURY_ROOT) and its
+ * rank k (the branches). (No counterpart in the reference.) */
 int gt_stack_woodbury(const gt_stack* stack, int32_t* form, int32_t* k);
 /* A what-if stack (peec.py:1871-1909 `with_branches`): `base` plus n_extra branches (candidate vias) whose kept end
  * nodes are a_keep_extra / b_keep_extra (-1: a reference node), appended after the base's branches. It shares the

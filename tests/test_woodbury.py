@@ -3,15 +3,17 @@
 The Schur complement is S = d_s (B + U C U^T): B the Laplacian factored once, U the branches' incidence, C each
 frequency's diagonal. Its inverse needs B^-1 U. The stored form keeps W = B^-1 U resident (n_keep x k doubles).
 The implicit form keeps only G = U^T B^-1 U (k x k) and applies B^-1 (U t) by a second Schur solve in each
-product. A board with many branches on many nodes (six layers of 2048^2 cells, 999 via branches: W would be
-200 GB) takes the implicit form. The reference forms S^-1 U densely on the host for a what-if's added branches
-(peec.py:2364-2377) and factorises a board's own branches into S.
+product; its root form, when every branch end lies in the factor's last supernode (the sheet-apart ordering puts
+the via nodes there), applies the correction on that supernode's rows between the one solve's passes. A board with
+many branches on many nodes (six layers of 2048^2 cells, 999 via branches: W would be 200 GB) takes the root form.
+The reference forms S^-1 U densely on the host for a what-if's added branches (peec.py:2364-2377) and factorises a
+board's own branches into S.
 
 GPU:
-- both forms' products and solves match the oracle, on a board and on a what-if derived from it;
-- the implicit G equals the stored G (the same solves);
-- a board with more than 256 branches takes the implicit form and solves to the oracle;
-- a batch of implicit what-ifs gives each solve's bits alone.
+- every form's products and solves match the oracle, on a board and on a what-if derived from it;
+- the implicit G (second solves) equals the stored G bit for bit, the root form's Y^T Y to rounding;
+- a board with more than 256 branches takes the root form and solves to the oracle;
+- a batch of a root-form board and its what-ifs (second solves) gives each solve's bits alone.
 """
 import numpy as np
 import pytest
@@ -34,28 +36,38 @@ def _board():
 
 
 @pytest.mark.gpu
-def test_both_forms_give_the_oracle_products_and_the_same_gram_matrix():
+@pytest.mark.parametrize("ordering,form", [("sheets", "root"), ("joint", "implicit")])
+def test_every_form_gives_the_oracle_products_and_the_stored_gram_matrix(ordering, form):
     b = _board()
     extra = _extra_vias(b, 3, 7)
-    st_s = peec.PlaneStack(b.layers, b.branches, woodbury="stored")
-    st_i = peec.PlaneStack(b.layers, b.branches, woodbury="implicit")
+    st_s = peec.PlaneStack(b.layers, b.branches, woodbury="stored", schur_ordering=ordering)
+    st_i = peec.PlaneStack(b.layers, b.branches, woodbury="implicit", schur_ordering=ordering)
     d_s, d_i = st_s.with_branches(extra), st_i.with_branches(extra)
-    assert st_s.woodbury_form == "stored" and st_i.woodbury_form == "implicit"
+    assert st_s.woodbury_form == "stored" and st_i.woodbury_form == form
     assert d_s.woodbury_form == "stored" and d_i.woodbury_form == "implicit"
-    # the root's G: the same solves and the same differences
-    assert np.array_equal(st_i.G, st_s.G)
+    if form == "implicit":
+        assert np.array_equal(st_i.G, st_s.G)       # the same solves and the same differences
+    else:
+        assert np.array_equal(st_i.G, st_i.G.T) and rel(st_i.G, st_s.G) < 1e-13
     # a what-if's added rows come from symmetry in the implicit form
     assert rel(d_i.G, d_s.G) < 1e-13
     ref = O.PlaneStack(b.layers, b.branches).with_branches(extra)
     hm = O.HostMath(ref)
     rng = np.random.default_rng(12)
     x = rng.standard_normal(d_i.nb + d_i.n_keep) + 1j * rng.standard_normal(d_i.nb + d_i.n_keep)
+    ref0 = O.PlaneStack(b.layers, b.branches)
+    hm0 = O.HostMath(ref0)
+    x0 = x[: st_i.nb + st_i.n_keep].copy()
+    x0[st_i.nb:] = x[d_i.nb:]
     for f in (1e6, 3e8):
         rop = ref.operator(f)
         pi = d_i.precond(d_i.operator(f), x)
         assert rel(pi, hm.precond(rop, x)) < 1e-10
         assert rel(pi, d_s.precond(d_s.operator(f), x)) < 1e-12
         assert np.array_equal(d_i.matvec(d_i.operator(f), x), d_s.matvec(d_s.operator(f), x))
+        p0 = st_i.precond(st_i.operator(f), x0)       # the board itself (the root form with "sheets")
+        assert rel(p0, hm0.precond(ref0.operator(f), x0)) < 1e-10
+        assert rel(p0, st_s.precond(st_s.operator(f), x0)) < 1e-12
 
 
 @pytest.mark.gpu
@@ -76,13 +88,13 @@ def test_implicit_board_and_whatif_solutions_match_the_oracle(f):
 
 
 @pytest.mark.gpu
-def test_a_board_with_more_than_256_branches_takes_the_implicit_form():
+def test_a_board_with_more_than_256_branches_takes_the_root_form():
     b = boards.scaled("two_layer", 40, n_vias=300, n_freqs=1)
     assert len(b.branches) > 256
     r = b.routes[0]
     st = peec.PlaneStack(b.layers, b.branches)
     sol = peec.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(2e8, tol=1e-12)
-    assert st.woodbury_form == "implicit"
+    assert st.woodbury_form == "root"
     with pytest.raises(Exception):
         peec.PlaneStack(b.layers, b.branches, woodbury="stored").handle
     ref = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver).solve(2e8, tol=1e-12)
@@ -91,7 +103,7 @@ def test_a_board_with_more_than_256_branches_takes_the_implicit_form():
 
 
 @pytest.mark.gpu
-def test_implicit_whatifs_in_one_batch_give_the_bits_of_each_alone():
+def test_a_root_form_board_and_its_whatifs_in_one_batch_give_the_bits_of_each_alone():
     import torch
     from groundtruth_amd.serve import Session
     from test_serve import _rhs, _run
@@ -100,6 +112,7 @@ def test_implicit_whatifs_in_one_batch_give_the_bits_of_each_alone():
     st = peec.PlaneStack(b.layers, b.branches, woodbury="implicit")
     ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
     stacks = [st] + [st.with_branches(_extra_vias(b, k, 500 + k)) for k in (1, 3)]
+    assert [s_.woodbury_form for s_ in stacks] == ["root", "implicit", "implicit"]
     f = 3e7
     jobs = [(s_, e, f) + _rhs(s_, e, f) for s_, e in ((s_, ex if s_ is st else ex.on(s_)) for s_ in stacks)]
     torch.cuda.synchronize()
