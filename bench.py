@@ -120,6 +120,44 @@ def lp_only(args):
                       "bytes_per_launch": args.lp_batch * lp_algorithmic_bytes(st), "copy_bytes": n}))
 
 
+def large_board() -> dict:
+    """The six-layer 2048^2 board (full inter-layer mutual blocks, two planes split into Voronoi regions, 200
+    stitching vias, 1 A at 1 GHz): set-up, one plain FP64 solve and one mixed-precision solve to 1e-8 with their true
+    residuals, and the L_p product's achieved HBM GB/s against peak."""
+    from groundtruth_amd import boards, peec
+    board = boards.large()
+    route = board.routes[0]
+    f = float(board.freqs[0])
+    t0 = time.perf_counter()
+    st = peec.PlaneStack(board.layers, board.branches, device=0)
+    ex = peec.RouteExcitation(st, route.trace, route.driver, route.receiver)
+    st.handle
+    setup_s = time.perf_counter() - t0
+    rhs_top, J = ex.rhs(f)
+    rhs = np.concatenate([rhs_top, np.asarray(J, float)[st.keep].astype(complex)])
+    op = st.operator(f)
+    out = {"workload": "six layers of 2048x2048 cells at 0.05 mm, two planes split into Voronoi regions, 200 "
+                       "stitching vias (999 branches), 1 A at 1 GHz, GMRES(100) to 1e-8",
+           "unknowns": int(st.nb + st.n_keep), "setup_s": setup_s, "woodbury": st.woodbury_form,
+           "schur": st.schur_stats(), "device_gb": st.device_bytes() / 1e9}
+    sols = {}
+    for prec in ("fp64", "mixed"):
+        t1 = time.perf_counter()
+        sol = ex.solve_many([f], tol=board.tol, restart=board.restart, precision=prec)[0]
+        dt = time.perf_counter() - t1
+        res = float(np.linalg.norm(rhs - st.matvec(op, sol.x)) / np.linalg.norm(rhs))
+        sols[prec] = sol
+        out[prec] = {"seconds": dt, "iterations": sol.iterations, "true_residual": res,
+                     "Z_loop": [sol.Z_loop.real, sol.Z_loop.imag]}
+    out["mixed"]["x_vs_fp64"] = float(np.linalg.norm(sols["mixed"].x - sols["fp64"].x) / np.linalg.norm(sols["fp64"].x))
+    out["mixed"]["Z_vs_fp64"] = abs(sols["mixed"].Z_loop - sols["fp64"].Z_loop) / abs(sols["fp64"].Z_loop)
+    t_lp = time_lp(st, reps=10, nr=1)
+    B = lp_algorithmic_bytes(st)
+    out["lp"] = {"us": t_lp * 1e6, "algorithmic_bytes": B, "achieved_gbs": B / t_lp / 1e9, "peak_gbs": HBM_PEAK_GBS,
+                 "frac": B / t_lp / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 SERVE_TOL = 1e-8                # the served stream's tolerance (plain FP64 GMRES)
 SERVE_RATE = 4.0                # served stream: Poisson arrivals per second
 SERVE_SECONDS = 15.0            # served stream: the window the arrivals fall in
@@ -410,6 +448,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=SERVE_SECONDS, help="served stream: arrival window")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--large", action="store_true", help="the six-layer 2048^2 board alone (a few minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported in this build "
@@ -418,6 +457,9 @@ def main():
         sys.exit(2)
     if args.lp_only:
         lp_only(args)
+        return
+    if args.large:
+        print(json.dumps({"large": large_board()}))
         return
     if args.serve_only:
         print(json.dumps({"served_backlog": serve_backlog(),

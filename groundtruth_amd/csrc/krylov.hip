@@ -649,15 +649,16 @@ void Krylov::dots2(const KrSetT<TV, TW>& k, hipStream_t s) {
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
     if (mmax + 1 > rows_cap) fail(GT_ERR_ARG, "a basis longer than the workspace's");
     const int groups = std::max(1, (mmax + 7) / 8);
-    hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nblk * groups, k.nr), dim3(NTH), 0, s, k, n, chunk, part.p, P, nblk,
+    const int c2 = chunk2<TV>(), nb2 = blocks2<TV>();
+    hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nb2 * groups, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
                        slot_h2(), slot_uu());
     GT_LAUNCH("kr_dots2");
     if (mmax > 0) {
-        hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, 0, red.p);
+        hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nb2, P, 0, red.p);
         GT_LAUNCH("kr_reduce");
     }
     // u^H u, u^H v and t, adjacent
-    hipLaunchKernelGGL(kr_reduce, dim3(3 + 2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_uu(), red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(3 + 2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nb2, P, slot_uu(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
@@ -669,10 +670,11 @@ void Krylov::update2(const KrSetT<TV, TW>& k, const double* ainv, const zd* qv, 
         cf.qv[p] = qv[p];
         if (k.m[p] + 1 > rows_cap) fail(GT_ERR_ARG, "a basis longer than the workspace's");
     }
-    hipLaunchKernelGGL((kr_update2<TV, TW>), dim3(nblk, k.nr), dim3(NTH), 0, s, k, n, chunk, (const double*)red.p, part.p,
-                       P, nblk, slot_h2(), slot_w_after(), cf);
+    const int c2 = chunk2<TV>(), nb2 = blocks2<TV>();
+    hipLaunchKernelGGL((kr_update2<TV, TW>), dim3(nb2, k.nr), dim3(NTH), 0, s, k, n, c2, (const double*)red.p, part.p,
+                       P, nb2, slot_h2(), slot_w_after(), cf);
     GT_LAUNCH("kr_update2");
-    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nblk, P, slot_w_after(), red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nb2, P, slot_w_after(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 
