@@ -26,11 +26,18 @@ __device__ __forceinline__ const OrientArgs<T>& pick(const LpArgs<T>& a, int oi)
     return oi ? a.o[1] : a.o[0];
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w mod 8), each XCD with its own L2. Row tiles i0 and
+// i0 + RW share the 128-byte lines of the transposed intermediates ([t][j][i]: 8 complex doubles of consecutive i),
+// so the tile order is remapped for consecutive tiles to run on one XCD: tile = (w mod 8) (G / 8) + w / 8 over a
+// grid of G = a multiple of 8 workgroups (those past the last tile exit).
+__device__ __forceinline__ int xcd_tile(int bx, int G) { return (bx & 7) * (G >> 3) + (bx >> 3); }
+
 template <class T, int NTH, int MAXPTS>
 __global__ __launch_bounds__(NTH) void lp_rows_fwd(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
-    int blk = blockIdx.x, oi = 0;
+    int blk = xcd_tile(blockIdx.x, gridDim.x), oi = 0;      // consecutive tiles on one XCD (grid: a multiple of 8)
+    if (blk >= a.nblk_rows[0] + a.nblk_rows[1]) return;
     if (blk >= a.nblk_rows[0]) { blk -= a.nblk_rows[0]; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
     const int ntiles = (o.n0 + o.TR - 1) / o.TR;
@@ -196,11 +203,6 @@ void split_mix(const LpArgs<T>& a, int nr, hipStream_t st) {
     }
 }
 
-// Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w mod 8), each XCD with its own L2. Row tiles i0 and
-// i0 + RW share the 128-byte lines of the transposed intermediates ([t][j][i]: 8 complex doubles of consecutive i),
-// so the tile order is remapped for consecutive tiles to run on one XCD: tile = (w mod 8) (G / 8) + w / 8 over a
-// grid of G = a multiple of 8 workgroups (those past the last tile exit).
-__device__ __forceinline__ int xcd_tile(int bx, int G) { return (bx & 7) * (G >> 3) + (bx >> 3); }
 
 // Pass B with the transforms in registers (column length P0 = 64 E, S <= 4 sheets): one wave per column j. Lane l
 // holds rows l + 64 m of every sheet, loaded straight from T1 (whole contiguous columns); the forward FFT, the
@@ -451,7 +453,8 @@ template <class T, int NTH, int MAXPTS>
 __global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
-    int blk = blockIdx.x, oi = 0;
+    int blk = xcd_tile(blockIdx.x, gridDim.x), oi = 0;      // consecutive tiles on one XCD (grid: a multiple of 8)
+    if (blk >= a.nblk_rows[0] + a.nblk_rows[1]) return;
     if (blk >= a.nblk_rows[0]) { blk -= a.nblk_rows[0]; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
     const int ntiles = (o.n0 + o.TR - 1) / o.TR;
@@ -527,6 +530,10 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int nrow = a.nblk_rows[0] + a.nblk_rows[1];
     const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
     constexpr int NTH = 256;
+    // the 128 KB LDS passes (one workgroup per CU) and the split pass's 64 KB column transforms take wider
+    // workgroups, so a CU holds 16 waves of them; a butterfly's arithmetic does not depend on its thread
+    constexpr int NTH_BIG = 1024, NTH_SPLIT = 512;
+    const int nrow8 = (nrow + 7) / 8 * 8;       // the generic row passes' grid: xcd_tile maps a multiple of 8
     const int nr = a.rhs.nr;
     const int bucket = nr == 1 ? 0 : nr <= 4 ? 1 : 2;
     const int rw = sh.rw[bucket], cw = sh.cw[bucket];
@@ -538,34 +545,37 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     static bool lds_set = false;
     if (!lds_set && (big_rows || big_cols)) {
         const int bytes = (int)(BIG * sizeof(cpx<T>));
-        GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_fwd<T, NTH, BIG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   bytes));
-        GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_inv<T, NTH, BIG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   bytes));
-        GT_HIP(hipFuncSetAttribute((const void*)&lp_cols_mac<T, NTH, BIG, LP_MAX_SHEETS>,
+        GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_fwd<T, NTH_BIG, BIG>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_inv<T, NTH_BIG, BIG>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        GT_HIP(hipFuncSetAttribute((const void*)&lp_cols_mac<T, NTH_BIG, BIG, LP_MAX_SHEETS>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         lds_set = true;
     }
     if (!launch_rows_w(a, nr, false, st, rw)) {
         if (big_rows)
-            hipLaunchKernelGGL((lp_rows_fwd<T, NTH, BIG>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+            hipLaunchKernelGGL((lp_rows_fwd<T, NTH_BIG, BIG>), dim3(nrow8, nr), dim3(NTH_BIG), sh.lds_rows, st, a);
         else
-            hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+            hipLaunchKernelGGL((lp_rows_fwd<T, NTH, LpPts<T>::rows>), dim3(nrow8, nr), dim3(NTH), sh.lds_rows, st, a);
     }
     GT_LAUNCH("lp_rows_fwd");
     if (sh.split) {
         const int items = a.S * ncol;
         const int P0 = std::max(a.nblk_cols[0] ? a.o[0].P0 : 0, a.nblk_cols[1] ? a.o[1].P0 : 0);
         const size_t lds = (size_t)P0 * sizeof(cpx<T>);
-        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH, LpPts<T>::cols, false>), dim3(items, nr), dim3(NTH), lds, st, a);
+        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH_SPLIT, LpPts<T>::cols, false>), dim3(items, nr), dim3(NTH_SPLIT),
+                           lds, st, a);
         GT_LAUNCH("lp_cols_split_fft");
         split_mix(a, nr, st);
         GT_LAUNCH("lp_cols_split_mix");
-        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH, LpPts<T>::cols, true>), dim3(items, nr), dim3(NTH), lds, st, a);
+        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH_SPLIT, LpPts<T>::cols, true>), dim3(items, nr), dim3(NTH_SPLIT),
+                           lds, st, a);
         GT_LAUNCH("lp_cols_split_fft");
     } else if (!launch_cols_w(a, ncol, nr, st, cw)) {
         if (big_cols)
-            hipLaunchKernelGGL((lp_cols_mac<T, NTH, BIG, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH), sh.lds_cols, st, a);
+            hipLaunchKernelGGL((lp_cols_mac<T, NTH_BIG, BIG, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH_BIG), sh.lds_cols,
+                               st, a);
         else
             hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH),
                                sh.lds_cols, st, a);
@@ -573,9 +583,9 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     GT_LAUNCH("lp_cols_mac");
     if (!launch_rows_w(a, nr, true, st, rw)) {
         if (big_rows)
-            hipLaunchKernelGGL((lp_rows_inv<T, NTH, BIG>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+            hipLaunchKernelGGL((lp_rows_inv<T, NTH_BIG, BIG>), dim3(nrow8, nr), dim3(NTH_BIG), sh.lds_rows, st, a);
         else
-            hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow, nr), dim3(NTH), sh.lds_rows, st, a);
+            hipLaunchKernelGGL((lp_rows_inv<T, NTH, LpPts<T>::rows>), dim3(nrow8, nr), dim3(NTH), sh.lds_rows, st, a);
     }
     GT_LAUNCH("lp_rows_inv");
 }
