@@ -13,6 +13,9 @@
 //   C  rows_inv : per (sheet, tile of rows): read T2, inverse FFT along j, gather at the strips and finish the
 //                 product's strip rows: R I + jw (L_p I) - Ar V.
 // The incidence rows (branch tail and Ar^T I) run in `tail_bottom`.
+#include <mutex>
+#include <set>
+
 #include "fft_lds.hpp"
 #include "fft_wave.hpp"
 #include "lp.hpp"
@@ -542,8 +545,15 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     constexpr int BIG = LpPts<T>::big;
     const bool big_rows = sh.lds_rows > (size_t)LpPts<T>::rows * sizeof(cpx<T>);
     const bool big_cols = sh.lds_cols > (size_t)LpPts<T>::cols * sizeof(cpx<T>);
-    static bool lds_set = false;
-    if (!lds_set && (big_rows || big_cols)) {
+    // once per device (a function's attributes are the current device's), under a lock: contexts on several
+    // devices may launch from several threads
+    static std::mutex lds_mu;
+    static std::set<int> lds_set;
+    int dev = 0;
+    if (big_rows || big_cols) GT_HIP(hipGetDevice(&dev));
+    std::unique_lock<std::mutex> lds_lk(lds_mu, std::defer_lock);
+    if (big_rows || big_cols) lds_lk.lock();
+    if ((big_rows || big_cols) && !lds_set.count(dev)) {
         const int bytes = (int)(BIG * sizeof(cpx<T>));
         GT_HIP(hipFuncSetAttribute((const void*)&lp_rows_fwd<T, NTH_BIG, BIG>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
@@ -551,8 +561,9 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         GT_HIP(hipFuncSetAttribute((const void*)&lp_cols_mac<T, NTH_BIG, BIG, LP_MAX_SHEETS>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        lds_set = true;
+        lds_set.insert(dev);
     }
+    if (lds_lk.owns_lock()) lds_lk.unlock();
     if (!launch_rows_w(a, nr, false, st, rw)) {
         if (big_rows)
             hipLaunchKernelGGL((lp_rows_fwd<T, NTH_BIG, BIG>), dim3(nrow8, nr), dim3(NTH_BIG), sh.lds_rows, st, a);
