@@ -16,7 +16,7 @@ WRITE_SIZE runs committed under profiles/ (launches of 8 right-hand sides), scal
 right-hand sides. Beside it, a single-right-hand-side product timed over 200 launches, and the oracle's CPU time on a
 bounded sample of the same workload. Then the served what-if stream (serve_stream): p50/p99 latency and solves/s of
 the continuous-batching server, beside the same stream solved one solve at a time; and the what-if latency at 512^2
-cells per layer (serve_whatif_512), the north star's p99 < 200 ms case; and the north star's throughput comparison
+cells per layer (serve_whatif_512), the p99 < 200 ms latency case; and the throughput comparison
 (serve_backlog): a backlog of what-ifs on the workload board served warm, batched and with FP32 Krylov bases, against
 the plain FP64 GPU solver one solve at a time.
 
@@ -289,7 +289,7 @@ BACKLOG_REQUESTS = 40          # the throughput case: what-if requests on the wo
 
 
 def serve_backlog(n_req: int = BACKLOG_REQUESTS, seed: int = 7, prepare_thread: bool = True) -> dict:
-    """The north star's throughput comparison on the workload board (two_layer, 2 x 256^2 cells): a backlog of
+    """The throughput comparison against the plain solver on the workload board (two_layer, 2 x 256^2 cells): a backlog of
     what-if requests (1-2 added stitching vias at random copper cells, 1-2 of the board's frequencies each, GMRES to
     the workload's 1e-10) served by the continuous-batching server -- the board resident and primed, the what-ifs
     warm-started from its solutions and batched together in its session -- with the Krylov bases in FP32 and in
@@ -369,7 +369,7 @@ WHATIF_SECONDS = 12.0
 
 
 def serve_whatif_512(rate: float = WHATIF_RATE, seconds: float = WHATIF_SECONDS, seed: int = 3) -> dict:
-    """The north star's latency case: what-if requests on a resident two-layer board of 512 x 512 cells per layer
+    """The latency case (p99 under 200 ms): what-if requests on a resident two-layer board of 512 x 512 cells per layer
     (1.56 M unknowns; 0.125 mm pitch, the 256^2 board's size), each one added stitching via at a random copper cell
     and one of the board's frequencies, Poisson arrivals, FP64 GMRES to 1e-8 from the board's solution (primed before
     the stream, untimed)."""
@@ -453,8 +453,8 @@ def main():
     ap.add_argument("--large", action="store_true", help="the six-layer 2048^2 board alone (a few minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported in this build "
-              "(independent boards and frequencies shard across GPUs without collectives; see DESIGN.md)",
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent boards and frequencies "
+              "would shard across GPUs without collectives, one process per GPU)",
               file=sys.stderr)
         sys.exit(2)
     if args.lp_only:

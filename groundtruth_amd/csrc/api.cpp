@@ -91,10 +91,10 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.log1 = ilog2(P[1]);
     const int S = st->S;
     if (o.P0 > LpPts<double>::big / 2)
-        fail(GT_ERR_UNSUPPORTED, "column FFT size above this build's LDS budget (P0 <= 4096)");
+        fail(GT_ERR_UNSUPPORTED, "column FFT size above this library's LDS budget (P0 <= 4096)");
     if ((size_t)S * o.P0 > (size_t)LpPts<double>::big) st->shape.split = true;   // the column pass through HBM
     if (o.P1 > LpPts<double>::big / 2)
-        fail(GT_ERR_UNSUPPORTED, "row FFT size above this build's LDS budget (P1 <= 4096)");
+        fail(GT_ERR_UNSUPPORTED, "row FFT size above this library's LDS budget (P1 <= 4096)");
     o.TR = std::max(1, std::min(8, LpPts<double>::rows / (o.P1 + 1)));
     // rows of the dense (sheet, i) grid as CSR over the sorted strips
     std::vector<int> rowptr((size_t)S * n0 + 1, 0), rowcol(cnt);

@@ -12,7 +12,7 @@ Drop-in for `src/groundtruth/solver/peec.py`:
 Errors are the reference's: SolverError for models and non-convergence, MemoryError for allocation failures,
 DeviceError for a failed device (the reference then redoes the solve on its CPU path, peec.py:2549-2558).
 
-Scope of this build: one current sheet per layer (screen=False, the Simulate default), the "diag" preconditioner,
+Scope of this package: one current sheet per layer (screen=False, the Simulate default), the "diag" preconditioner,
 full-width strips of one copper thickness, lumped branches (vias). Plane-pair capacitance, face sheets, the
 "sheet" preconditioner and layers of different thickness raise NotImplementedError.
 """
@@ -144,15 +144,15 @@ class PlaneStack:
         if precond not in ("diag", "sheet"):
             raise ValueError("precond must be 'diag' or 'sheet'")
         if precond == "sheet":
-            raise NotImplementedError("the 'sheet' preconditioner is not part of this build")
+            raise NotImplementedError("the 'sheet' preconditioner is not part of this library")
         if capacitance:
-            raise NotImplementedError("plane-pair capacitance is not part of this build")
+            raise NotImplementedError("plane-pair capacitance is not part of this library")
         g0 = layers[0].grid
         for l in layers:
             if (l.grid.nx, l.grid.ny, l.grid.d) != (g0.nx, g0.ny, g0.d):
                 raise SolverError("all layers must share one grid")
         if len({float(l.thickness) for l in layers}) != 1:
-            raise NotImplementedError("layers of different copper thickness are not part of this build")
+            raise NotImplementedError("layers of different copper thickness are not part of this library")
         self.precond_mode = precond
         self.layers = list(layers)
         self.N = self.n_sheets = len(layers)
@@ -230,7 +230,7 @@ class PlaneStack:
 
     def _check_no_split(self):
         """screen=True splits a layer with copper above and below it at >= 16 of its nodes into two face sheets
-        (peec.py:1716-1765); this build has one sheet per layer, so such a stack is refused."""
+        (peec.py:1716-1765); this library has one sheet per layer, so such a stack is refused."""
         if self.N < 3:
             return
         for k in range(self.N):
@@ -242,7 +242,7 @@ class PlaneStack:
                 elif self.zs[m] < self.zs[k]:
                     below |= self.node[m]
             if np.count_nonzero(self.node[k] & above & below) >= SCREEN_MIN_NODES:
-                raise NotImplementedError("face sheets (screen=True with an inner plane) are not part of this build")
+                raise NotImplementedError("face sheets (screen=True with an inner plane) are not part of this library")
 
     def node_id(self, node) -> int:
         """Node number of (layer, i, j), -1 off copper (peec.py:1864-1869)."""

@@ -127,7 +127,7 @@ class Solution:
         return self.Z_loop.imag / (2 * math.pi * self.f)
 
     def at(self, node) -> complex:
-        """A grid node's potential, node = (layer, i, j) (peec.py:1118-1125; this build's stacks have no lid
+        """A grid node's potential, node = (layer, i, j) (peec.py:1118-1125; this library's stacks have no lid
         nodes)."""
         return complex(self.V[tuple(node)])
 
@@ -135,7 +135,7 @@ class Solution:
         """Complex (Kx, Ky) at the nodes of layer k, in A/m per A of signal (peec.py:1151-1199): at each node, the
         mean of the strips that touch it in each direction and carry current, over the pitch. copper=True asks for
         the current per width of the copper itself, which needs the strips' copper widths and the nodes' copper
-        areas from the reference's rasterisation; this build's strips are full cells, so it is refused."""
+        areas from the reference's rasterisation; this library's strips are full cells, so it is refused."""
         if copper:
             raise NotImplementedError("copper-width fields need the rasterised copper widths and areas")
         d = d_mm * 1e-3

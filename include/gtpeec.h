@@ -30,7 +30,7 @@ extern "C" {
 #define GT_ERR_DEVICE 3       /* a HIP call failed (no device, launch failure)                                    */
 #define GT_ERR_NOCONV 4       /* GMRES did not converge: SolverError (peec.py:2599-2600)                          */
 #define GT_ERR_NUMERIC 5      /* the Schur factorisation met a zero or non-finite pivot: SolverError             */
-#define GT_ERR_UNSUPPORTED 6  /* a shape this build has no kernel for (FFT size, LDS budget)                      */
+#define GT_ERR_UNSUPPORTED 6  /* a shape this library has no kernel for (FFT size, LDS budget)                      */
 
 typedef struct gt_ctx_s gt_ctx;
 typedef struct gt_stack_s gt_stack;
@@ -48,7 +48,7 @@ void gt_ctx_destroy(gt_ctx* ctx);
 int gt_ctx_sync(gt_ctx* ctx);
 /* The context's hipStream_t (for callers that order their own device work with it). */
 void* gt_ctx_stream(gt_ctx* ctx);
-/* Measurement (no counterpart in the reference: the per-kernel HBM figures the north star asks for). While enabled,
+/* Measurement (no counterpart in the reference: per-kernel HBM figures against the device's peak). While enabled,
  * every L_p launch of the system products on this context is bracketed by a HIP event pair on its stream;
  * enabling (or disabling) clears the record. gt_ctx_lp_time waits for the stream and gives the launches recorded,
  * the right-hand sides they carried and their summed duration in ms. */

@@ -16,7 +16,7 @@ peec    restates the solve path of `src/groundtruth/solver/peec.py`: topology (`
 
 Scope: one current sheet per layer (`screen=False`, the Simulate default), the "diag" preconditioner, full-width
 strips, lumped branches (vias). Face sheets, the "sheet" preconditioner, narrow-strip corrections and plane-pair
-capacitance are not restated (no config of this build uses them).
+capacitance are not restated (no workload here uses them).
 
 How each function is pinned is listed in `tests/test_oracle.py` (golden vectors written from the reference's
 `kernel.py`, closed forms, adaptive quadrature, dense direct solves, brute-force convolutions).

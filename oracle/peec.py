@@ -595,7 +595,7 @@ def gmres(op: Callable, b, tol: float, max_cycles: int = 8, restart: Optional[in
 # --------------------------------------------------------------------------- the route (peec.py:2703-2817)
 @dataclass
 class Solution:
-    """One solve per amp of loop current (peec.py:1092-1110, the fields this build fills)."""
+    """One solve per amp of loop current (peec.py:1092-1110, the fields the device solver fills)."""
     f: float
     I_x: np.ndarray
     I_y: np.ndarray

@@ -5,7 +5,7 @@ Tolerances: the L_p product, the system product and the Schur solve are one FP64
 host's kernel values (setup="host"), the oracle's own, so they compare the device's FFTs and rows alone; the
 device-made kernels are pinned in test_kernel_gen.py. The whole solves use the device-made kernels. The preconditioner uses a different exact
 factorisation (Cholesky of B with the Woodbury update, against SuperLU of S): <= 1e-10. Whole solves at tol 1e-12:
-strip currents and Z_loop within 1e-9 relative (the north star's bar), iterations within 2 of the oracle's.
+strip currents and Z_loop within 1e-9 relative (the parity bar), iterations within 2 of the oracle's.
 """
 import numpy as np
 import pytest

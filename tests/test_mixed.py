@@ -1,4 +1,4 @@
-"""Mixed precision (the north star's first extension), two forms:
+"""Mixed precision, two forms:
   "mixed": FP64 iterative refinement around FP32 inner GMRES solves whose vectors, basis and operator products (the
   L_p FFT convolution, the incidence rows) run in FP32, with the Arnoldi dot products, the Givens least-squares, the
   Schur solve and every reduction in FP64;

@@ -3,7 +3,7 @@
 
 CPU: node_field against a loop over every node that averages the touching strips carrying current; the dominant phase
 against a brute-force search of the angle that maximises the strips' in-phase energy (and its sign rule); the layer
-shares sum to one; copper-width fields are refused (this build's strips are full cells).
+shares sum to one; copper-width fields are refused (this library's strips are full cells).
 GPU: the device's (Kx, Ky) at every node of every layer (gt_node_field) equal the host's to rounding (numpy divides a
 complex by a real through the reciprocal, the device divides) on solved boards, with the same zero pattern.
 """
