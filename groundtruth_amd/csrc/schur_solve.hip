@@ -353,6 +353,7 @@ void Schur::upload(hipStream_t st) {
 
 void Schur::ensure(int nr) {
     if (nr <= cap) return;
+    if (!graphs.empty()) GT_HIP(hipDeviceSynchronize());    // no captured solve still running on the old buffers
     for (SchurGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
     ubuf.release();
