@@ -152,6 +152,9 @@ def test_nested_dissection_keeps_fill_and_depth_small():
     n = st.n_keep
     assert nnz < 40 * n * math.log2(n)
     assert levels <= 3 * math.log2(n)
+    # each sheet dissected apart below a top separator of the 8 vias' 16 nodes: no separator spans both sheets'
+    # grid lines (a joint cut would take 2 x 64 nodes)
+    assert maxc <= 64
 
 
 def test_woodbury_form_equals_the_schur_inverse():
