@@ -123,3 +123,21 @@ def test_a_root_form_board_and_its_whatifs_in_one_batch_give_the_bits_of_each_al
         assert out[i].converged and out[i].iterations == one.iterations, i
         assert np.array_equal(xd.cpu().numpy(), one.x), i
     ses.close()
+
+
+@pytest.mark.gpu
+def test_the_six_layer_board_family_matches_the_oracle_in_both_implicit_forms():
+    """The six-layer board of two Voronoi-split planes and 200 stitching vias (975 branches here), at 32 x 32 cells
+    per layer: the root form (sheet-apart dissection, the default) and the second-solve form (joint dissection) give
+    the oracle's currents and Z_loop."""
+    b = boards.scaled("large", 32)
+    assert len(b.layers) == 6 and len(b.branches) > 256
+    r = b.routes[0]
+    f = float(b.freqs[0])
+    ref = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver).solve(f, tol=1e-12)
+    for ordering, form in (("sheets", "root"), ("joint", "implicit")):
+        st = peec.PlaneStack(b.layers, b.branches, schur_ordering=ordering)
+        sol = peec.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(f, tol=1e-12)
+        assert st.woodbury_form == form, ordering
+        assert rel(_cur(sol), _cur(ref)) < 1e-9, ordering
+        assert abs(sol.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9, ordering
