@@ -18,11 +18,13 @@ bounded sample of the same workload. Then the served what-if stream (serve_strea
 the continuous-batching server, beside the same stream solved one solve at a time; and the what-if latency at 512^2
 cells per layer (serve_whatif_512), the p99 < 200 ms latency case; and the throughput comparison
 (serve_backlog): a backlog of what-ifs on the workload board served warm, batched and with FP32 Krylov bases, against
-the plain FP64 GPU solver one solve at a time.
+the plain FP64 GPU solver one solve at a time. Last, the six-layer 2048^2 board's L_p product at one right-hand side
+(large_lp: its achieved GB/s against the peak, beside its measured traffic).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-python bench.py --lp-only [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N right-hand sides,
-                                                         then a 64 MiB device copy as the scale)
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-serve] [--no-large-lp]
+python bench.py --lp-only [--workload W] [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N
+                                                         right-hand sides, then a 64 MiB device copy as the scale)
+python bench.py --large      (the six-layer 2048^2 board's FP64, mixed and FP32-basis solves)
 """
 from __future__ import annotations
 
@@ -32,6 +34,7 @@ import os
 import platform
 import sys
 import time
+from typing import Optional
 
 import numpy as np
 
@@ -82,11 +85,11 @@ def time_lp(st, reps: int = 200, nr: int = 1):
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
-def measured_traffic(workload: str):
+def measured_traffic(workload: str, name: Optional[str] = None):
     """(HBM bytes per L_p launch, right-hand sides of that launch) from the committed FETCH_SIZE / WRITE_SIZE runs
     of this workload (lines `traffic_bytes_per_launch: N` and `traffic_rhs_per_launch: R` in
-    profiles/lp_traffic_<workload>.md), or None."""
-    path = os.path.join(ROOT, "profiles", f"lp_traffic_{workload}.md")
+    profiles/lp_traffic_<workload>.md, or profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name or f"lp_traffic_{workload}.md")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -118,6 +121,28 @@ def lp_only(args):
     torch.cuda.synchronize()
     print(json.dumps({"workload": args.workload, "rhs_per_launch": args.lp_batch, "lp_us": t_lp * 1e6,
                       "bytes_per_launch": args.lp_batch * lp_algorithmic_bytes(st), "copy_bytes": n}))
+
+
+def large_lp() -> dict:
+    """The six-layer 2048^2 board's L_p product (that configuration's FFT matvec) at one right-hand side: the average
+    of 10 launches timed by HIP events on the library's stream, against its algorithmic bytes and the HBM peak, with
+    the HBM bytes its FETCH_SIZE / WRITE_SIZE runs measured."""
+    import torch
+
+    from groundtruth_amd import boards, peec
+    b = boards.large()
+    st = peec.PlaneStack(b.layers, b.branches, device=0)
+    st.handle
+    t = time_lp(st, reps=10, nr=1)
+    B = lp_algorithmic_bytes(st)
+    out = {"workload": "six layers of 2048x2048 cells (bench.py --large's board), FFT 4096 x 4096 per orientation",
+           "us_per_launch": t * 1e6, "bytes_per_launch": B, "achieved": B / t / 1e9, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": B / t / 1e9 / HBM_PEAK_GBS}
+    mt = measured_traffic("large", "lp_traffic_six_layer_2048.md")
+    out["traffic"] = mt[0] if mt else None
+    del st
+    torch.cuda.empty_cache()
+    return out
 
 
 def large_board() -> dict:
@@ -451,6 +476,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--large", action="store_true", help="the six-layer 2048^2 board alone (a few minutes)")
+    ap.add_argument("--no-large-lp", action="store_true", help="skip the six-layer board's L_p timing (about a minute)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent boards and frequencies "
@@ -631,6 +657,11 @@ def main():
                                     "board solution at f (the boards primed before the stream, untimed)")
         out["whatif_512"] = serve_whatif_512()
         out["served_backlog"] = serve_backlog()
+    if not args.no_large_lp:
+        try:
+            out["large_lp"] = large_lp()
+        except Exception as e:      # noqa: BLE001 - the headline line above stands on its own
+            out["large_lp"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])
     print(json.dumps(out))
