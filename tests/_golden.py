@@ -1,0 +1,59 @@
+"""Loads the reference-solve fixtures under tests/golden/ (written by tests/golden/make_solve_fixtures.py) and
+rebuilds their boards as this library's problem objects, which the oracle's classes also take."""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from groundtruth_amd.problem import Branch, Grid, PlaneLayer, TraceSegment
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+NAMES = ["single_plane_64", "two_layer_48", "four_layer_24", "large_32", "mixed_copper_32"]
+
+
+@dataclass
+class Fixture:
+    name: str
+    data: dict
+    layers: list
+    branches: list
+    routes: list            # [(trace, driver, receiver)]
+
+    @property
+    def solves(self):
+        return list(enumerate(zip(self.data["solve_route"].tolist(), self.data["solve_f"].tolist())))
+
+
+def _z(R: float, L: float):
+    return lambda f, R=R, L=L: complex(R, 2 * math.pi * f * L)
+
+
+def load(name: str) -> Fixture:
+    with np.load(GOLDEN / f"solve_{name}.npz") as z:
+        a = {k: z[k] for k in z.files}
+    pitch = float(a["pitch_mm"])
+    layers = [PlaneLayer(f"L{k}", float(a["z"][k]), Grid.from_mask(m, pitch), float(a["t"][k]))
+              for k, m in enumerate(a["nodes"])]
+    branches = [Branch(tuple(int(v) for v in pa), tuple(int(v) for v in pb), _z(float(R), float(L)), f"br{n}",
+                       "via", float(rad))
+                for n, (pa, pb, R, L, rad) in enumerate(zip(a["br_a"], a["br_b"], a["br_R"], a["br_L"],
+                                                           a["br_radius"]))]
+    routes = []
+    for k in range(len(a["drv"])):
+        sel = np.nonzero(a["tr_route"] == k)[0]
+        trace = [TraceSegment(a["tr_a"][s].copy(), a["tr_b"][s].copy(), float(a["tr_w"][s]), float(a["tr_i"][s]))
+                 for s in sel]
+        routes.append((trace, tuple(int(v) for v in a["drv"][k]), tuple(int(v) for v in a["rcv"][k])))
+    return Fixture(name, a, layers, branches, routes)
+
+
+def l2(a, b) -> float:
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def currents(I_x, I_y, I_branch) -> np.ndarray:
+    return np.concatenate([np.ravel(I_x), np.ravel(I_y), np.ravel(I_branch)])
