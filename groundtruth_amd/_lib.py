@@ -27,7 +27,7 @@ class StackDesc(C.Structure):
 
 class OpDesc(C.Structure):
     _fields_ = [("w", C.c_double), ("r_sheet", f64p), ("z_tail", f64p), ("dinv_sheet", f64p), ("dinv_tail", f64p),
-                ("s_scale", C.c_double * 2), ("k", C.c_int32), ("q", f64p)]
+                ("s_scale", C.c_double * 2), ("k", C.c_int32), ("q", f64p), ("s_in", f64p), ("s_out", f64p)]
 
 
 class KernelDesc(C.Structure):
@@ -69,6 +69,7 @@ SIGNATURES = {
     "gt_lp_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_lp_matvec_batch": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "gt_schur_factor": (C.c_int, [C.c_void_p, i32p, f64p]),
+    "gt_schur_factor_nodes": (C.c_int, [C.c_void_p, i32p, i32p, i32p, f64p]),
     "gt_stack_derive": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, C.POINTER(C.c_void_p)]),
     "gt_ctx_lp_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_schur_device_fronts": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -1,4 +1,5 @@
 // The C ABI of libgtpeec (include/gtpeec.h): argument checks, uploads, and error codes for everything below it.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -591,22 +592,63 @@ int gt_stack_set_lp_plan(gt_stack* st, const int32_t* rw, const int32_t* cw) {
     });
 }
 
+namespace {
+
+// The real Laplacian B = Ar^T Ar of the stack (strips and branches at weight 1), factorised on the device
+void factor_laplacian(gt_stack* st, const int32_t* node_xyz) {
+    const int nk = st->nk, nb = st->nb;
+    std::vector<int> ea, eb;
+    std::vector<double> diag;
+    laplacian(nk, nb, st->h_a_keep.data(), st->h_b_keep.data(), ea, eb, diag);
+    auto S = std::make_unique<Schur>();
+    S->h.dev_min = st->ctx->schur_dev_min;
+    S->h.leaf = st->ctx->schur_leaf;
+    S->h.sheets_apart = st->ctx->schur_sheets_apart;
+    S->h.build(nk, ea, eb, diag, node_xyz);
+    S->upload(st->ctx->stream);
+    S->factor_device(st->ctx->stream);
+    st->schur = std::move(S);
+}
+
+}  // namespace
+
+int gt_schur_factor_nodes(gt_stack* st, const int32_t* node_xyz, int32_t* n_cols, int32_t* cols, double* G_out) {
+    return guard([&] {
+        need(st && node_xyz && n_cols, "NULL argument");
+        need(!st->derived, "a derived stack shares its base's factor");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        factor_laplacian(st, node_xyz);
+        // the distinct kept end nodes of the branches, ascending: one unit column each (wb_units with kb = -1)
+        std::vector<int> nodes;
+        for (int m = 0; m < st->nbr; ++m)
+            for (int e : {st->h_a_keep[st->ns + m], st->h_b_keep[st->ns + m]})
+                if (e >= 0) nodes.push_back(e);
+        std::sort(nodes.begin(), nodes.end());
+        nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        const int k = (int)nodes.size();
+        if (k > WB_IMPLICIT_MAX_K) fail(GT_ERR_UNSUPPORTED, "more than 2048 branch end nodes on one stack");
+        need(k == 0 || cols != nullptr, "NULL cols");
+        st->wb_implicit = true;
+        st->wb_nodes = true;
+        std::vector<int> none(k, -1);
+        woodbury_extend(st, nullptr, k, nodes.data(), none.data());
+        if (k > 0 && !st->wb_root)
+            fail(GT_ERR_UNSUPPORTED, "sheets of different admittance need every branch end in the factor's last "
+                                     "supernode (branches that join nodes of one sheet are not supported)");
+        *n_cols = k;
+        for (int c = 0; c < k; ++c) cols[c] = nodes[c];
+        if (G_out) woodbury_gram_device(st, G_out);
+        GT_HIP(hipStreamSynchronize(st->ctx->stream));
+    });
+}
+
 int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
     return guard([&] {
         need(st && node_xyz, "NULL argument");
+        need(!st->derived, "a derived stack shares its base's factor");
         std::lock_guard<std::mutex> lk(st->ctx->mu);
-        const int nk = st->nk, nb = st->nb;
-        std::vector<int> ea, eb;
-        std::vector<double> diag;
-        laplacian(nk, nb, st->h_a_keep.data(), st->h_b_keep.data(), ea, eb, diag);
-        auto S = std::make_unique<Schur>();
-        S->h.dev_min = st->ctx->schur_dev_min;
-        S->h.leaf = st->ctx->schur_leaf;
-        S->h.sheets_apart = st->ctx->schur_sheets_apart;
-        S->h.build(nk, ea, eb, diag, node_xyz);
-        S->upload(st->ctx->stream);
-        S->factor_device(st->ctx->stream);
-        st->schur = std::move(S);
+        const int nk = st->nk;
+        factor_laplacian(st, node_xyz);
         // Woodbury columns W = B^-1 U (U: the branches' incidence) by device solves, and G = U^T W; W stays resident
         // unless it is too large (then only G is kept and each product solves for B^-1 U t)
         const int k = st->nbr;
@@ -633,6 +675,8 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
         need(base && out && n_extra >= 0 && (n_extra == 0 || (a_keep_extra && b_keep_extra)), "bad argument");
         std::lock_guard<std::mutex> lk(base->ctx->mu);
         need(base->schur != nullptr, "the base stack has no Schur factor (gt_schur_factor first)");
+        if (base->wb_nodes)
+            fail(GT_ERR_UNSUPPORTED, "a stack of sheets of different admittance derives no what-ifs: make a stack");
         *out = nullptr;
         GT_HIP(hipSetDevice(base->ctx->device));
         std::unique_ptr<gt_stack_s> st(new gt_stack_s(base->core));
@@ -767,6 +811,13 @@ int gt_op_create(gt_stack* st, const gt_op_desc* d, gt_op** out) {
         upf(op->r_sheet_f, d->r_sheet, st->S);
         upf(op->z_tail_f, d->z_tail, st->nbr);
         op->s_scale = zd(d->s_scale[0], d->s_scale[1]);
+        need((d->s_in == nullptr) == (d->s_out == nullptr), "s_in and s_out go together");
+        need(!st->wb_nodes || d->s_in != nullptr, "a node-column stack's operator needs per-sheet scales");
+        if (d->s_in) {
+            up(op->s_in, d->s_in, st->S);
+            up(op->s_out, d->s_out, st->S);
+            op->per_sheet = true;
+        }
         op->k = d->k;
         GT_HIP(hipStreamSynchronize(s));
         *out = op.release();

@@ -33,6 +33,8 @@ struct PreArgs {
     const zd* dinv_tail[KMAX];
     const zd* q[KMAX];        // k x k row-major
     zd s_scale[KMAX];
+    const zd* s_in[KMAX];     // per-sheet scales (node-column stacks), or null: s_scale alone
+    const zd* s_out[KMAX];
     zd* s;                    // nk x nr interleaved: Schur right-hand sides
     const zd* z;              // nk x nr interleaved: B^-1 s
     zd* t;                    // nr x kt: Q (U^T z)
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs<T> a) {
             if (code & 1) acc = acc + v;
             else acc = acc - v;
         }
-        a.s[(size_t)n * a.nr + p] = acc;
+        a.s[(size_t)n * a.nr + p] = a.s_in[p] ? a.s_in[p][sh] * acc : acc;
     }
 }
 
@@ -172,7 +174,8 @@ __global__ __launch_bounds__(256) void pre_woodbury_diff(PreArgs<T> a) {
     for (int p = 0; p < a.nr; ++p) {
         const size_t i = (size_t)n * a.nr + p;
         const zd v = a.z2 && !a.root[p] ? a.z[i] - a.z2[i] : a.z[i];
-        a.out[p][a.nb[p] + n] = stz<T>(a.s_scale[p] * v);
+        const zd sc = a.s_out[p] ? a.s_out[p][a.node_sheet[n]] : a.s_scale[p];
+        a.out[p][a.nb[p] + n] = stz<T>(sc * v);
     }
 }
 
@@ -274,7 +277,8 @@ __global__ __launch_bounds__(256) void pre_woodbury_apply(PreArgs<T> a) {
                 vre[p] -= w * tp[m].re;
                 vim[p] -= w * tp[m].im;
             }
-            a.out[p][a.nb[p] + n] = stz<T>(a.s_scale[p] * zd(vre[p], vim[p]));
+            const zd sc = a.s_out[p] ? a.s_out[p][a.node_sheet[n]] : a.s_scale[p];
+            a.out[p][a.nb[p] + n] = stz<T>(sc * zd(vre[p], vim[p]));
         }
     }
 }
@@ -688,6 +692,8 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
         a.dinv_tail[p] = ops[p]->dinv_tail.p;
         a.q[p] = ops[p]->q.p;
         a.s_scale[p] = ops[p]->s_scale;
+        a.s_in[p] = ops[p]->per_sheet ? ops[p]->s_in.p : nullptr;
+        a.s_out[p] = ops[p]->per_sheet ? ops[p]->s_out.p : nullptr;
         a.Wx[p] = !imp && o->W.p ? o->W.p + (size_t)a.k_sh * st->nk : nullptr;
         a.a_keep[p] = o->a_keep.p;
         a.b_keep[p] = o->b_keep.p;

@@ -125,6 +125,9 @@ struct gt_stack_s {
     // (nonzero on that supernode's yr_n rows only) the correction is w_r -= Y_r Q Y_r^T w_r between the solve's
     // passes, and G = Y_r^T Y_r
     bool wb_root = false;
+    // Node columns (gt_schur_factor_nodes): the k Woodbury columns are unit columns at kept nodes, not branch
+    // incidence columns (sheets of different strip admittance); always in the root form
+    bool wb_nodes = false;
     gt::DevBuf<double> Yr;                    // k columns of yr_n (column-major)
     int yr_c0 = 0, yr_n = 0;
     int k = 0;
@@ -154,6 +157,8 @@ struct gt_op_s {
     gt::DevBuf<gt::zd> r_sheet, z_tail, dinv_sheet, dinv_tail, q;
     gt::DevBuf<gt::zf> r_sheet_f, z_tail_f;   // the product's impedances for single-precision products
     gt::zd s_scale;
+    gt::DevBuf<gt::zd> s_in, s_out;           // per-sheet scales of a node-column stack (empty: s_scale)
+    bool per_sheet = false;
     int k = 0;
 };
 

@@ -13,8 +13,8 @@ Errors are the reference's: SolverError for models and non-convergence, MemoryEr
 DeviceError for a failed device (the reference then redoes the solve on its CPU path, peec.py:2549-2558).
 
 Scope of this package: one current sheet per layer (screen=False, the Simulate default), the "diag" preconditioner,
-full-width strips of one copper thickness, lumped branches (vias). Plane-pair capacitance, face sheets, the
-"sheet" preconditioner and layers of different thickness raise NotImplementedError.
+full-width strips (one copper thickness per layer; layers may differ), lumped branches (vias). Plane-pair
+capacitance, face sheets and the "sheet" preconditioner raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -96,7 +96,11 @@ class Operator:
         self.dinv_tail = 1.0 / self.z_tail if len(self.z_tail) else np.zeros(0, complex)
         ds = self.dinv_sheet[0]
         k = stack.nbr
-        if k:
+        s_in = s_out = None
+        if stack.per_sheet:
+            Q, k, s_in = self._node_correction(stack)
+            s_out = s_in
+        elif k:
             # S = d_s (B + U C U^T): B holds the factored branches at weight 1 (C = d/d_s - 1); branches a what-if
             # stack added (with_branches) are not in B (C = d/d_s)
             in_B = np.arange(k) < stack.nbr_factored
@@ -115,9 +119,39 @@ class Operator:
         desc.s_scale[0], desc.s_scale[1] = sc.real, sc.imag
         desc.k = k
         desc.q = _lib.ptr(q)
+        if s_in is not None:
+            self._s = _lib.cplx(s_in)
+            desc.s_in = desc.s_out = _lib.ptr(self._s)
         h = C.c_void_p()
         _lib.check(_lib.lib().gt_op_create(stack.handle, C.byref(desc), C.byref(h)), "gt_op_create")
         self.handle = h
+
+    def _node_correction(self, stack: "PlaneStack"):
+        """Sheets of different strip admittance lambda_s (layers of different copper thickness): with Lambda the
+        diagonal of lambda over the kept nodes, S = Ar^T D^-1 Ar = Lambda^1/2 (B + E) Lambda^1/2, where B is the
+        factored Laplacian (strips and branches at weight 1) and E = sum_m (d_m u~ u~^T - u u^T), u~ = Lambda^-1/2 u,
+        lives on the branches' end nodes (the stack's unit Woodbury columns, stack.wb_cols). Returns
+        (Q = E_v (I + G E_v)^-1, its size, the per-sheet scale lambda^-1/2)."""
+        lam = self.dinv_sheet
+        scale = 1.0 / np.sqrt(lam)
+        cols = stack.wb_cols
+        k = len(cols)
+        if k == 0:
+            return np.zeros((0, 0), complex), 0, scale
+        pos = -np.ones(stack.n_keep, np.int64)
+        pos[cols] = np.arange(k)
+        keep_idx = np.nonzero(stack.keep)[0]
+        E = np.zeros((k, k), complex)
+        for m in range(stack.nbr):
+            ends = []
+            for kn, sign in ((stack.a_keep[stack.ns + m], 1.0), (stack.b_keep[stack.ns + m], -1.0)):
+                if kn >= 0:
+                    ends.append((pos[kn], sign, scale[stack.node_layer[keep_idx[kn]]]))
+            for pi, si, ci in ends:
+                for pj, sj, cj in ends:
+                    E[pi, pj] += self.dinv_tail[m] * si * sj * ci * cj - si * sj
+        Q = E @ np.linalg.inv(np.eye(k) + stack.G @ E)
+        return Q, k, scale
 
     def __del__(self):
         try:
@@ -151,8 +185,8 @@ class PlaneStack:
         for l in layers:
             if (l.grid.nx, l.grid.ny, l.grid.d) != (g0.nx, g0.ny, g0.d):
                 raise SolverError("all layers must share one grid")
-        if len({float(l.thickness) for l in layers}) != 1:
-            raise NotImplementedError("layers of different copper thickness are not part of this library")
+        # layers of different copper thickness: each sheet's strips have their own admittance (gt_schur_factor_nodes)
+        self.per_sheet = len({float(l.thickness) for l in layers}) != 1
         self.precond_mode = precond
         self.layers = list(layers)
         self.N = self.n_sheets = len(layers)
@@ -292,7 +326,7 @@ class PlaneStack:
             return self
         a = np.array([self.node_id(b.a) for b in extra], np.int64)
         b = np.array([self.node_id(b.b) for b in extra], np.int64)
-        if np.any(self.full_label[a] != self.full_label[b]):
+        if np.any(self.full_label[a] != self.full_label[b]) or self.per_sheet:
             return PlaneStack(self.layers, self.branches + extra, sigma=self.sigma, precond=self.precond_mode,
                               device=self.device, setup=self.setup_mode, device_fronts=self.device_fronts,
                               schur_leaf=self.schur_leaf, woodbury=self.woodbury,
@@ -391,13 +425,26 @@ class PlaneStack:
         keep_idx = np.nonzero(self.keep)[0]
         xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
                                              self.node_j[keep_idx]], axis=1).astype(np.int32))
-        Gm = np.zeros((max(self.nbr, 1), max(self.nbr, 1)))
-        code = L.gt_schur_factor(h, _lib.ptr(xyz.reshape(-1) if xyz.size else np.zeros(3, np.int32), _lib.i32p),
-                                 _lib.ptr(Gm))
-        if code != _lib.GT_OK:
-            L.gt_stack_destroy(h)
-            _lib.check(code, "gt_schur_factor")
-        self.G = Gm[: self.nbr, : self.nbr].copy()
+        xyz_p = _lib.ptr(xyz.reshape(-1) if xyz.size else np.zeros(3, np.int32), _lib.i32p)
+        if self.per_sheet:
+            kmax = max(2 * self.nbr, 1)
+            Gm = np.zeros((kmax, kmax))
+            cols = np.zeros(kmax, np.int32)
+            nc = C.c_int32()
+            code = L.gt_schur_factor_nodes(h, xyz_p, C.byref(nc), _lib.ptr(cols, _lib.i32p), _lib.ptr(Gm))
+            if code != _lib.GT_OK:
+                L.gt_stack_destroy(h)
+                _lib.check(code, "gt_schur_factor_nodes")
+            k = nc.value
+            self.wb_cols = cols[:k].astype(np.int64)
+            self.G = Gm.reshape(-1)[: k * k].reshape(k, k).copy()
+        else:
+            Gm = np.zeros((max(self.nbr, 1), max(self.nbr, 1)))
+            code = L.gt_schur_factor(h, xyz_p, _lib.ptr(Gm))
+            if code != _lib.GT_OK:
+                L.gt_stack_destroy(h)
+                _lib.check(code, "gt_schur_factor")
+            self.G = Gm[: self.nbr, : self.nbr].copy()
         self._handle = h
 
     def _derive(self):

@@ -193,6 +193,19 @@ int gt_stack_set_lp_plan(gt_stack* stack, const int32_t* rw, const int32_t* cw);
  * G_out: k x k (k = nbr) real, row-major: U^T B^-1 U, from which the caller forms each frequency's
  *        Q = C (I + G C)^-1 (peec.py:2357-2377's Woodbury identity). May be NULL when nbr == 0. */
 int gt_schur_factor(gt_stack* stack, const int32_t* node_xyz, double* G_out);
+/* The Schur factor of a stack whose sheets have different strip admittances (layers of different copper thickness;
+ * peec.py:2379-2424 factorises S = Ar^T D^-1 Ar for any D). With lambda_s each sheet's strip admittance and Lambda
+ * the diagonal of lambda over the kept nodes, S = Lambda^1/2 (B + E) Lambda^1/2 exactly: B = Ar^T Ar (the same real
+ * Laplacian as gt_schur_factor's, branches at weight 1) and E = sum_m (d_m u~_m u~_m^T - u_m u_m^T) with
+ * u~_m = Lambda^-1/2 u_m, nonzero only at the branches' kept end nodes. The Woodbury columns are therefore the
+ * unit columns P of those nodes, ascending: cols[0 .. *n_cols) (at most 2048, the caller's array holds nbr * 2), and
+ * G_out (n_cols x n_cols, row-major) = P^T B^-1 P. Each frequency's caller forms E_v = P^T E P and
+ * Q = E_v (I + G E_v)^-1, and passes s_in = s_out = lambda^-1/2 per sheet in gt_op_desc. Every end node must lie in
+ * the factor's last supernode (the sheet-apart ordering puts the nodes that branches join across sheets there), so
+ * the correction runs on that supernode's rows between the solve's passes (the root form); otherwise
+ * GT_ERR_UNSUPPORTED. Such a stack derives no what-if stacks (gt_stack_derive refuses): a what-if of it is a stack
+ * of its own. */
+int gt_schur_factor_nodes(gt_stack* stack, const int32_t* node_xyz, int32_t* n_cols, int32_t* cols, double* G_out);
 /* The Woodbury form the factorised stack took (GT_WOODBURY_STORED, GT_WOODBURY_IMPLICIT or GT_WOODBURY_ROOT) and its
  * rank k (the branches). (No counterpart in the reference.) */
 int gt_stack_woodbury(const gt_stack* stack, int32_t* form, int32_t* k);
@@ -233,8 +246,11 @@ typedef struct {
     const double* dinv_sheet; /* 2 S: preconditioner admittance of a strip on each sheet, 1/(R + jw L_self)      */
     const double* dinv_tail;  /* 2 nbr: 1 / z_tail                                                                */
     double s_scale[2];        /* 1 / d_s: scales B^-1 to S^-1                                                      */
-    int32_t k;                /* Woodbury rank (nbr, or 0)                                                        */
+    int32_t k;                /* Woodbury rank (the stack's column count, or 0)                                   */
     const double* q;          /* 2 k k: Q = C (I + G C)^-1, row-major                                             */
+    const double* s_in;       /* 2 S or NULL: per-sheet scales of a stack factorised by gt_schur_factor_nodes     */
+    const double* s_out;      /* (S^-1 = diag(s_out) (B + P E P^T)^-1 diag(s_in) by each kept node's sheet); NULL: */
+                              /* the single s_scale                                                               */
 } gt_op_desc;
 
 int gt_op_create(gt_stack* stack, const gt_op_desc* desc, gt_op** out);
