@@ -68,6 +68,8 @@ SIGNATURES = {
     "gt_stack_device_bytes": (C.c_int64, [C.c_void_p]),
     "gt_lp_matvec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_lp_matvec_batch": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "gt_lp_matvec_batch_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p),
+                                             C.POINTER(C.c_void_p)]),
     "gt_schur_factor": (C.c_int, [C.c_void_p, i32p, f64p]),
     "gt_schur_factor_nodes": (C.c_int, [C.c_void_p, i32p, i32p, i32p, f64p]),
     "gt_stack_derive": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, C.POINTER(C.c_void_p)]),

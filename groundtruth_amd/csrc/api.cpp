@@ -247,6 +247,7 @@ int gt_ctx_create(int device, gt_ctx** out) {
         GT_HIP(hipGetDeviceCount(&count));
         if (device < 0 || device >= count) fail(GT_ERR_DEVICE, "no HIP device " + std::to_string(device));
         GT_HIP(hipSetDevice(device));
+        noalloc_scratch_setup(device);
         hipDeviceProp_t prop;
         GT_HIP(hipGetDeviceProperties(&prop, device));
         auto* c = new gt_ctx_s;
@@ -509,6 +510,39 @@ int gt_lp_matvec_batch(gt_stack* st, int32_t nr, const void* const* I_dev, void*
         if (st->ns == 0) return;
         st->work.ensure(st, nr);
         LpArgs<double> a = st->lp_args(0);
+        a.rhs.nr = nr;
+        for (int i = 0; i < nr; ++i) {
+            a.rhs.x[i] = (const zd*)I_dev[i];
+            a.rhs.y[i] = (zd*)out_dev[i];
+        }
+        launch_lp(a, st->shape, st->ctx->stream);
+    });
+}
+
+int gt_lp_matvec_batch_noalloc(gt_stack* st, int32_t nr, const void* const* I_dev, void* const* out_dev) {
+    return guard([&] {
+        need(st && I_dev && out_dev, "NULL argument");
+        need(nr >= 1 && nr <= KMAX, "1 to 16 right-hand sides per batch");
+        for (int i = 0; i < nr; ++i) need(I_dev[i] && out_dev[i] && I_dev[i] != out_dev[i], "NULL or aliased vector");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        if (st->ns == 0) return;
+        if (st->s_elems)
+            fail(GT_ERR_ARG, "gt_lp_matvec_batch_noalloc: this stack's column pass splits through HBM (S * P0 > "
+                             "8192), which needs a third intermediate; use gt_lp_matvec_batch");
+        const size_t need_bytes = 2 * sizeof(zd) * (size_t)nr * std::max<size_t>(1, st->t_elems);
+        if (need_bytes > GT_LP_NOALLOC_SCRATCH_BYTES)
+            fail(GT_ERR_ARG, "gt_lp_matvec_batch_noalloc: " + std::to_string(nr) + " right-hand sides need " +
+                                 std::to_string(need_bytes) + " bytes of scratch, above its " +
+                                 std::to_string((size_t)GT_LP_NOALLOC_SCRATCH_BYTES) + "; use fewer per call");
+        zd* T1 = static_cast<zd*>(noalloc_scratch(st->ctx->device));
+        zd* T2 = T1 + (size_t)nr * std::max<size_t>(1, st->t_elems);
+        GT_HIP(hipSetDevice(st->ctx->device));
+        LpArgs<double> a = st->lp_args(0);
+        for (int oi = 0; oi < 2; ++oi) {
+            a.o[oi].T1 = T1 + st->o[oi].t_off;
+            a.o[oi].T2 = T2 + st->o[oi].t_off;
+            a.o[oi].T3 = nullptr;
+        }
         a.rhs.nr = nr;
         for (int i = 0; i < nr; ++i) {
             a.rhs.x[i] = (const zd*)I_dev[i];

@@ -172,6 +172,9 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z);
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides
+// The no-allocation scratch (noalloc.hip): looked up once per device at context creation; its address on a device
+void noalloc_scratch_setup(int device);
+void* noalloc_scratch(int device);
 // (Kx, Ky) at every node of every sheet, S x nx x ny complex each, from the strip currents of x (post.hip)
 void node_field_device(gt_stack* st, const zd* x, double d, zd* kx, zd* ky);
 // psi of every strip (ns values at psi, device memory) from the trace's segments (psi.hip)

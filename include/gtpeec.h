@@ -152,6 +152,16 @@ int gt_lp_matvec(gt_stack* stack, const void* I_dev, void* out_dev);
 /* The same for nr <= 16 vectors in one launch (peec.py:2115-2135 applied to each): the kernel spectra are read once
  * for the batch. */
 int gt_lp_matvec_batch(gt_stack* stack, int32_t nr, const void* const* I_dev, void* const* out_dev);
+/* gt_lp_matvec_batch that allocates nothing. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * same kernels and launch shapes, so each output equals gt_lp_matvec_batch's bit for bit; the intermediates live in
+ * a fixed device array of GT_LP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), not in the stack's
+ * workspace. A batch needs 2 * 16 B * nr * (the stack's per-product intermediate elements) of it, and a stack whose
+ * column pass splits through HBM (S * P0 > 8192) needs a third buffer: such calls return GT_ERR_ARG with the sizes.
+ * Calls that overlap on one GPU (from other contexts or threads, before an earlier call's kernels finish) share the
+ * array and race: order them on one stream. */
+#define GT_LP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gt_lp_matvec_batch_noalloc(gt_stack* stack, int32_t nr, const void* const* I_dev, void* const* out_dev);
 
 /* The complex sheet-current density (Kx, Ky), in A/m per A of signal, at every grid node of every sheet from the
  * strip currents of a solution vector on the device (x_dev: nb + n_keep complex values; kx_dev, ky_dev: S x nx x ny
