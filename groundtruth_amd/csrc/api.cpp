@@ -118,7 +118,7 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     if (kg) {
         kernel_quadrant_device(st, oi, *kg);
     } else {
-        const size_t kq = (size_t)(o.P1 / 2 + 1) * (o.P0 / 2 + 1) * S * S;
+        const size_t kq = (size_t)(o.P1 / 2 + 1) * (o.P0 / 2 + 1) * S * S;      // G = S * S: the identity map
         std::vector<double> kd(K, K + kq);
         std::vector<float> kf(kq);
         for (size_t e = 0; e < kq; ++e) kf[e] = (float)kd[e];
@@ -184,6 +184,8 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
     a.mode = mode;
     a.t_stride = t_elems;
     a.s_stride = s_elems;
+    a.G = kG;
+    a.gmap = kgmap_dev.p;
     a.a_keep = a_keep.p;
     a.b_keep = b_keep.p;
     return a;
@@ -396,13 +398,40 @@ void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd,
         st->nk = d->n_keep;
         hipStream_t s = ctx->stream;
         KernelGen kg;
+        const int S = st->S;
+        st->kgmap.assign((size_t)S * S, 0);
         if (kd) {
             need(kd->d > 0 && kd->dz && kd->near_field, "kernel description incomplete");
             kg.d = kd->d;
             kg.mu = kd->mu;
             kg.dz = kd->dz;
             kg.nearv = kd->near_field;
+            // one kernel per distinct face gap: sheet pairs with the same gap and near-field table share it
+            constexpr int NF = 25;
+            for (int a = 0; a < S; ++a)
+                for (int b = a; b < S; ++b) {
+                    const int e = a * S + b;
+                    int g = 0;
+                    for (; g < kg.G; ++g) {
+                        const int r = kg.rep_s[g] * S + kg.rep_t[g];
+                        if (kd->dz[r] == kd->dz[e] &&
+                            std::memcmp(kd->near_field + (size_t)r * NF, kd->near_field + (size_t)e * NF,
+                                        NF * sizeof(double)) == 0)
+                            break;
+                    }
+                    if (g == kg.G) {
+                        kg.rep_s[g] = a;
+                        kg.rep_t[g] = b;
+                        ++kg.G;
+                    }
+                    st->kgmap[(size_t)a * S + b] = st->kgmap[(size_t)b * S + a] = g;
+                }
+            st->kG = kg.G;
+        } else {
+            for (int e = 0; e < S * S; ++e) st->kgmap[e] = e;
+            st->kG = S * S;
         }
+        st->kgmap_dev.upload(st->kgmap, s);
         setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx, kd ? &kg : nullptr);
         setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky, kd ? &kg : nullptr);
         if (st->shape.split) {                  // both orientations' whole column spectra, each its own region
@@ -473,8 +502,14 @@ int gt_stack_kernel(const gt_stack* st, int32_t orient, double* out) {
         std::lock_guard<std::mutex> lk(st->ctx->mu);
         const OrientHost& o = st->o[orient];
         if (!o.present) return;
-        GT_HIP(hipMemcpyAsync(out, o.K.p, o.K.bytes(), hipMemcpyDeviceToHost, st->ctx->stream));
+        // the resident G kernels per pixel, expanded to the S x S sheet pairs
+        std::vector<double> k(o.K.n);
+        GT_HIP(hipMemcpyAsync(k.data(), o.K.p, o.K.bytes(), hipMemcpyDeviceToHost, st->ctx->stream));
         GT_HIP(hipStreamSynchronize(st->ctx->stream));
+        const int S = st->S, G = st->kG;
+        const size_t npix = o.K.n / G;
+        for (size_t q = 0; q < npix; ++q)
+            for (int e = 0; e < S * S; ++e) out[q * S * S + e] = k[q * G + st->kgmap[e]];
     });
 }
 

@@ -184,12 +184,12 @@ void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg) {
         f.gx[q] = 0.5 * (1.0 + z);
         f.gw[q] = 1.0 / ((1.0 - z * z) * dp * dp);   // 2 / ((1 - z^2) P_n'(z)^2), halved for [0, 1]
     }
-    for (int a = 0; a < S; ++a)
-        for (int b = a; b < S; ++b) {
-            f.ps[npair] = a;
-            f.pt[npair] = b;
-            ++npair;
-        }
+    // one kernel per distinct face gap (sheet pairs at equal gaps have equal kernels), from a representative pair
+    for (int g = 0; g < kg.G; ++g) {
+        f.ps[npair] = kg.rep_s[g];
+        f.pt[npair] = kg.rep_t[g];
+        ++npair;
+    }
     f.npair = npair;
     DevBuf<double> dz((size_t)S * S), nearv((size_t)S * S * KG_NEAR * KG_NEAR);
     dz.upload(kg.dz, (size_t)S * S, s);
@@ -221,8 +221,8 @@ void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg) {
     c1.K = n1;
     hipLaunchKernelGGL(kg_cos, dim3((h1 + CT - 1) / CT, (n0 + CT - 1) / CT, npair), dim3(256), 0, s, c1);
     GT_LAUNCH("kg_cos");
-    // F[qj][qi][t][s] = sum_a w_a G[pair][a][qj] cos(2 pi qi a / P0) / (P0 P1), both sheet orders
-    const size_t kq = (size_t)h1 * h0 * S * S;
+    // F[qj][qi][g] = sum_a w_a G[g][a][qj] cos(2 pi qi a / P0) / (P0 P1)
+    const size_t kq = (size_t)h1 * h0 * npair;
     o.K.alloc(kq);
     CosArgs c2{};
     c2.A = G.p;
@@ -232,12 +232,11 @@ void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg) {
     c2.tw = st->tw.at(P0).p;
     c2.P = P0;
     c2.C = o.K.p;
-    c2.sc_m = (long)h0 * S * S;
-    c2.sc_n = (long)S * S;
+    c2.sc_m = (long)h0 * npair;
+    c2.sc_n = npair;
     for (int p = 0; p < npair; ++p) {
-        const int a = f.ps[p], b = f.pt[p];
-        c2.c_off[p] = (long)b * S + a;
-        c2.c_off2[p] = a == b ? -1 : (long)a * S + b;
+        c2.c_off[p] = p;
+        c2.c_off2[p] = -1;
     }
     c2.scale = 1.0 / ((double)P0 * (double)P1);
     c2.M = h1;

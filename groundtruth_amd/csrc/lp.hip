@@ -86,7 +86,7 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     const int h0 = P0 / 2 + 1;
     for (int i = threadIdx.x; i < P0; i += NTH) {
         const int qi = min(i, P0 - i);
-        const T* Kp = o.K + ((size_t)qj * h0 + qi) * S * S;
+        const T* Kp = o.K + ((size_t)qj * h0 + qi) * a.G;
         cpx<T> v[MAXS];
 #pragma unroll
         for (int s = 0; s < MAXS; ++s)
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
 #pragma unroll
                 for (int s = 0; s < MAXS; ++s) {
                     if (s < S) {
-                        const T k = Kp[t * S + s];
+                        const T k = Kp[a.gmap[t * S + s]];
                         acc.re += k * v[s].re;
                         acc.im += k * v[s].im;
                     }
@@ -159,10 +159,10 @@ __global__ __launch_bounds__(256) void lp_cols_split_mix(LpArgs<T> a, long q0) {
     const int P0 = o.P0, P1 = o.P1, h0 = P0 / 2 + 1, h1 = P1 / 2 + 1;
     if (a.nblk_cols[oi] == 0 || idx >= (long)h0 * h1) return;
     const int qj = (int)(idx / h0), qi = (int)(idx - (long)qj * h0);
-    const T* Kp = o.K + ((size_t)qj * h0 + qi) * S * S;
+    const T* Kp = o.K + ((size_t)qj * h0 + qi) * a.G;
     T k[S * S];
 #pragma unroll
-    for (int e = 0; e < S * S; ++e) k[e] = Kp[e];
+    for (int e = 0; e < S * S; ++e) k[e] = Kp[a.gmap[e]];
     cpx<T>* base = o.T3 + blockIdx.y * a.s_stride;
     const size_t sheet = (size_t)P1 * P0;
     const int ni = (qi == 0 || 2 * qi == P0) ? 1 : 2, nj = (qj == 0 || 2 * qj == P1) ? 1 : 2;
@@ -248,13 +248,13 @@ __global__ __launch_bounds__(64 * CW) void lp_cols_mac_w(LpArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int i = lane + 64 * m;
-        const T* Kp = o.K + ((size_t)qj * h0 + min(i, N - i)) * S * S;
+        const T* Kp = o.K + ((size_t)qj * h0 + min(i, N - i)) * a.G;
         cpx<T> out[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const T k = Kp[t * S + s];
+                const T k = Kp[a.gmap[t * S + s]];
                 out[t].re += k * v[s][m].re;
                 out[t].im += k * v[s][m].im;
             }

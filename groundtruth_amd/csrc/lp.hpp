@@ -27,7 +27,7 @@ struct OrientArgs {
     int lo, cnt;              // the orientation's strips: x[lo, lo + cnt)
     const int* rowptr;        // S * n0 + 1: first strip of each (sheet, row), relative to lo
     const int* rowcol;        // cnt: (i << 16) | j of each strip
-    const T* K;               // kernel quadrant [qj][qi][t][s], scaled by 1 / (P0 P1)
+    const T* K;               // kernel quadrant [qj][qi][g], scaled by 1 / (P0 P1): one value per distinct face gap g
     cpx<T>* T1;               // [s][j][i] after the row FFT (S * P1 * n0), right-hand side 0
     cpx<T>* T2;               // [t][j][i] after the column pass
     cpx<T>* T3;               // [s][j][k] whole column spectra (the split column pass), right-hand side 0
@@ -57,6 +57,8 @@ struct LpArgs {
                               // at rhs.nb[p] in x)
     size_t t_stride;          // elements of T1 / T2 per right-hand side
     size_t s_stride;          // elements of T3 per right-hand side
+    int G;                    // distinct kernels (face gaps) per pixel; sheets t, s read K[.][gmap[t S + s]]
+    const int* gmap;          // S x S, device memory
     const int* a_keep;
     const int* b_keep;
     RhsSet<T> rhs;

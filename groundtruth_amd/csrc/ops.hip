@@ -403,6 +403,8 @@ LpArgs<T> lp_args_t(const gt_stack* st) {
         a.mode = 1;
         a.t_stride = d.t_stride;
         a.s_stride = d.s_stride;
+        a.G = d.G;
+        a.gmap = d.gmap;
         a.a_keep = d.a_keep;
         a.b_keep = d.b_keep;
         return a;

@@ -49,8 +49,8 @@ namespace gt {
 struct OrientHost {
     bool present = false;
     int n0 = 0, n1 = 0, P0 = 0, P1 = 0, log0 = 0, log1 = 0, TR = 1, lo = 0, cnt = 0;
-    size_t t_off = 0;          // offset of this orientation's L_p intermediates in T1 / T2
     size_t s_off = 0;          // offset of its whole column spectra in T3 (the split column pass)
+    size_t t_off = 0;          // offset of this orientation's L_p intermediates in T1 / T2
     DevBuf<int> rowptr, rowcol;
     DevBuf<double> K;
     DevBuf<float> Kf;
@@ -94,6 +94,9 @@ struct StackCore {
     LpShape shape;
     size_t t_elems = 0;                       // complex elements of the L_p intermediates (both orientations)
     size_t s_elems = 0;                       // complex elements of the split column pass's spectra (0: not split)
+    int kG = 0;                               // distinct kernels per pixel (face gaps) and the sheets' map to them
+    std::vector<int> kgmap;
+    DevBuf<int> kgmap_dev;
     std::unique_ptr<Schur> schur;
 };
 
@@ -111,6 +114,9 @@ struct gt_stack_s {
     gt::LpShape& shape;
     size_t& t_elems;
     size_t& s_elems;
+    int& kG;
+    std::vector<int>& kgmap;
+    gt::DevBuf<int>& kgmap_dev;
     std::unique_ptr<gt::Schur>& schur;
 
     gt_ctx* ctx = nullptr;
@@ -139,7 +145,7 @@ struct gt_stack_s {
     explicit gt_stack_s(std::shared_ptr<gt::StackCore> c = std::make_shared<gt::StackCore>())
         : core(std::move(c)), o(core->o), tw(core->tw), twf(core->twf), node_sheet(core->node_sheet),
           strip_sheet(core->strip_sheet), h_strip_sheet(core->h_strip_sheet), shape(core->shape),
-          t_elems(core->t_elems), s_elems(core->s_elems), schur(core->schur), id(next_id()) {}
+          t_elems(core->t_elems), s_elems(core->s_elems), kG(core->kG), kgmap(core->kgmap), kgmap_dev(core->kgmap_dev), schur(core->schur), id(next_id()) {}
     gt_stack_s(const gt_stack_s&) = delete;
     gt_stack_s& operator=(const gt_stack_s&) = delete;
 
@@ -185,6 +191,8 @@ struct KernelGen {
     double d = 0, mu = 0;       // pitch (m), mu0 / (4 pi)
     const double* dz = nullptr;     // S x S face gaps
     const double* nearv = nullptr;  // S x S x 5 x 5 near-field mutuals [s][t][axial][lateral]
+    int G = 0;                      // distinct kernels: one per face gap, made from the sheet pair (rep_s, rep_t)
+    int rep_s[LP_MAX_SHEETS * LP_MAX_SHEETS] = {}, rep_t[LP_MAX_SHEETS * LP_MAX_SHEETS] = {};
 };
 // The kernel quadrant of orientation oi (its FFT sizes and twiddles set) made on the device, FP64 and FP32 copies
 void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg);

@@ -233,18 +233,25 @@ def _strip_board(nx, ny, sheets, seed):
     return layers, branches
 
 
-@pytest.mark.parametrize("nx,ny,sheets", [(1100, 6, 2), (6, 1100, 2), (1100, 5, 1), (1100, 6, 3), (6, 1100, 5)])
-def test_Lp_matvec_with_4096_point_transforms_matches_the_oracle(nx, ny, sheets):
+@pytest.mark.parametrize("nx,ny,sheets,setup", [(1100, 6, 2, "host"), (6, 1100, 2, "host"), (1100, 5, 1, "host"),
+                                                  (1100, 6, 3, "host"), (6, 1100, 5, "host"), (1100, 6, 6, "host"),
+                                                  (1100, 6, 6, "device"), (1100, 7, 8, "device")])
+def test_Lp_matvec_with_4096_point_transforms_matches_the_oracle(nx, ny, sheets, setup):
     """Transforms of 4096 points: the LDS passes' long-transform kernels (a 4096-point row per workgroup, S x P0 = 8192
     column points: 128 KB of LDS), and beyond 8192 column points the column pass through HBM (transforms, the
-    sheets' multiply-add per pixel, inverse transforms), against the oracle's scipy FFTs, with the system product."""
+    sheets' multiply-add per pixel, inverse transforms), against the oracle's scipy FFTs, with the system product.
+    setup "device": the device-made kernels, one per distinct face gap (equally spaced sheets: S of S x S). The
+    host's far-field closed form (the reference's) loses digits to cancellation along a board 1100 cells long, so the
+    two kernels differ by ~3e-11 of the product there: the bar is 1e-10 with the device's kernels (a wrong sheet-pair
+    map would be off by 1e-2 or more) and 1e-13 with the host's, which pins the transforms themselves."""
     layers, branches = _strip_board(nx, ny, sheets, seed=nx + ny + sheets)
-    st, ref = peec.PlaneStack(layers, branches, setup="host"), O.PlaneStack(layers, branches)
+    st, ref = peec.PlaneStack(layers, branches, setup=setup), O.PlaneStack(layers, branches)
+    bar = 1e-13 if setup == "host" else 1e-10
     assert 4096 in tuple(st.Px) + tuple(st.Py)
     rng = np.random.default_rng(10)
     I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)
-    assert rel(st.Lp_matvec(I), ref.Lp_matvec(I)) < 1e-13
+    assert rel(st.Lp_matvec(I), ref.Lp_matvec(I)) < bar
     f = 2e8
     x = rng.standard_normal(st.nb + st.n_keep) + 1j * rng.standard_normal(st.nb + st.n_keep)
     hm = O.HostMath(ref)
-    assert rel(st.matvec(st.operator(f), x), hm.matvec(ref.operator(f), x)) < 1e-12
+    assert rel(st.matvec(st.operator(f), x), hm.matvec(ref.operator(f), x)) < 10 * bar
