@@ -1,30 +1,32 @@
 """Benchmark of the plane-current PEEC solve on one MI355X.
 
-Workload (default): the two-layer board, 256 x 256 cells per layer at 0.25 mm, a 0.5 mm slot in the lower plane,
-20 stitching vias, one Manhattan route, solved at 10 frequencies log-spaced 1 MHz - 1 GHz, plain FP64, GMRES(50)
-to a relative residual of 1e-10. One step = the 10 solves (one route at every frequency) with the board and the
-right-hand sides resident, run as one batch (each solve follows the reference's GMRES and is bit-identical to its
-solve alone). value = solves per second over the timed steps. The same batch is then timed in mixed precision
-(FP32 FFT / Krylov inner solves, FP64 Arnoldi, Schur and iterative refinement to 1e-8) and reported beside it
-with each solution's relative error (and the plain run's) against an FP64 solve of the same system at tol 1e-13.
+Headline workload (the timed steps): the two-layer board, 256 x 256 cells per layer at 0.25 mm, a 0.5 mm slot in
+the lower plane, 20 stitching vias, one Manhattan route, solved at 10 frequencies log-spaced 1 MHz - 1 GHz, plain
+FP64, GMRES(50) to a relative residual of 1e-10. One step = the 10 solves (one route at every frequency) with the
+board and the right-hand sides resident, run as one batch (each solve follows the reference's GMRES and is
+bit-identical to its solve alone). value = solves per second over the timed steps. The same batch is then timed in
+mixed precision (FP32 FFT / Krylov inner solves under FGMRES, FP64 Arnoldi, Schur and refinement to 1e-8) and with
+FP32 Krylov bases, each reported with its solutions' relative errors against an FP64 solve of the same system at tol
+1e-13.
 
 Also reported: the L_p product's (the FFT matvec's) roofline. Every L_p launch of the timed steps is bracketed by
-HIP events on the library's stream (gt_ctx_lp_timing); achieved = the right-hand sides per launch x the algorithmic
-bytes of one product (strip currents in and out, one read and one write of the padded spectra per transform, the
-kernel quadrant once) / the average launch. traffic = the HBM bytes of an L_p launch from the FETCH_SIZE /
-WRITE_SIZE runs committed under profiles/ (launches of 8 right-hand sides), scaled to the timed launches' average
-right-hand sides. Beside it, a single-right-hand-side product timed over 200 launches, and the oracle's CPU time on a
-bounded sample of the same workload. Then the served what-if stream (serve_stream): p50/p99 latency and solves/s of
-the continuous-batching server, beside the same stream solved one solve at a time; and the what-if latency at 512^2
-cells per layer (serve_whatif_512), the p99 < 200 ms latency case; and the throughput comparison
-(serve_backlog): a backlog of what-ifs on the workload board served warm, batched and with FP32 Krylov bases, against
-the plain FP64 GPU solver one solve at a time. Last, the six-layer 2048^2 board's L_p product at one right-hand side
-(large_lp: its achieved GB/s against the peak, beside its measured traffic).
+HIP events on the library's stream (gt_ctx_lp_timing); achieved = the algorithmic bytes of a launch (per right-hand
+side: strip currents in and out, one read and one write of the padded spectra per transform; once per launch: the
+kernel quadrants) / the average launch. traffic = the HBM bytes of an L_p launch from the FETCH_SIZE / WRITE_SIZE runs
+committed under profiles/, scaled to the timed launches' average right-hand sides. Beside it: a single-right-hand-side
+product over 200 launches; the served what-if stream (serve_stream: p50/p99 latency and solves/s of the continuous-
+batching server beside the same stream solved one solve at a time), the 512^2 what-if latency (serve_whatif_512) and
+the served backlog against the plain FP64 solver one solve at a time (serve_backlog); the oracle's CPU solve of one
+frequency (best of 3, with the host's CPU model, affinity and library versions); and the six-layer 2048^2 board
+(large: FP64, FP32-basis and mixed solves to 1e-8 with their solves/s and errors, its L_p product's GB/s against peak,
+and the same product on the host).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-serve] [--no-large-lp]
+Progress lines go to stderr; stdout carries the one JSON line.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-serve] [--no-large]
 python bench.py --lp-only [--workload W] [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N
                                                          right-hand sides, then a 64 MiB device copy as the scale)
-python bench.py --large      (the six-layer 2048^2 board's FP64, mixed and FP32-basis solves)
+python bench.py --large      (the six-layer 2048^2 board alone)
 """
 from __future__ import annotations
 
@@ -42,18 +44,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "PEEC solves/sec + matvec HBM GB/s; p50/p99 what-if latency vs plain FP64 at 1e-8"
-MIXED_TOL = 1e-8                # the mixed-precision run's FP64 refinement target (the workload's 1e-8)
 HBM_PEAK_GBS = 8000.0
 
 
-def lp_algorithmic_bytes(st) -> int:
-    """B = sum over orientations of [2c n_o + 4c S P0 P1 + r S^2 (P0/2+1)(P1/2+1)], c = 16, r = 8 (FP64)."""
+def lp_bytes(st) -> tuple:
+    """The algorithmic bytes of one L_p product, split as (per right-hand side, per launch): the strip currents in
+    and out once and one HBM read and write of the padded spectra per transform, 2c n_o + 4c S P0 P1, for every
+    right-hand side; the kernel quadrants, r S^2 (P0/2+1)(P1/2+1), once per launch whatever its right-hand sides
+    (c = 16, r = 8: FP64), summed over the orientations."""
     c, r, S = 16, 8, st.n_sheets
-    total = 0
+    vec = ker = 0
     for P, n_o in ((st.Px, st.nxb), (st.Py, st.nyb)):
         if n_o:
-            total += 2 * c * n_o + 4 * c * S * P[0] * P[1] + r * S * S * (P[0] // 2 + 1) * (P[1] // 2 + 1)
-    return total
+            vec += 2 * c * n_o + 4 * c * S * P[0] * P[1]
+            ker += r * S * S * (P[0] // 2 + 1) * (P[1] // 2 + 1)
+    return vec, ker
+
+
+def lp_algorithmic_bytes(st, nr: int = 1) -> int:
+    """B of one launch of nr right-hand sides (lp_bytes): nr times the vector bytes, the kernel bytes once."""
+    vec, ker = lp_bytes(st)
+    return nr * vec + ker
+
+
+def log(msg: str):
+    """A progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def time_lp(st, reps: int = 200, nr: int = 1):
@@ -120,40 +136,46 @@ def lp_only(args):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"workload": args.workload, "rhs_per_launch": args.lp_batch, "lp_us": t_lp * 1e6,
-                      "bytes_per_launch": args.lp_batch * lp_algorithmic_bytes(st), "copy_bytes": n}))
+                      "bytes_per_launch": lp_algorithmic_bytes(st, args.lp_batch), "copy_bytes": n}))
 
 
-def large_lp() -> dict:
-    """The six-layer 2048^2 board's L_p product (that configuration's FFT matvec) at one right-hand side: the average
-    of 10 launches timed by HIP events on the library's stream, against its algorithmic bytes and the HBM peak, with
-    the HBM bytes its FETCH_SIZE / WRITE_SIZE runs measured."""
+def cpu_lp_large(st, I: np.ndarray, y_dev: np.ndarray) -> dict:
+    """The six-layer board's L_p product on the host by the oracle's numpy / scipy path (the reference's FFT matvec,
+    peec.py:2054-2135: fft2 / ifft2 with workers=-1 and the numpy multiply-add over sheet pairs), with the device
+    stack's own kernel quadrants: timed once (one product takes tens of seconds on the host), and its result held to
+    the device product of the same vector (a full-size check)."""
+    from oracle import peec as O
+
+    from groundtruth_amd import peec
+    o = O.PlaneStack.__new__(O.PlaneStack)
+    o.nx, o.ny, o.n_sheets = st.nx, st.ny, st.n_sheets
+    o.xa, o.ya, o.nxb, o.nyb, o.ns = st.xa, st.ya, st.nxb, st.nyb, st.ns
+    o.Px, o.Py = tuple(st.Px), tuple(st.Py)
+    # the device keeps its quadrants [qj][qi][t][s] pre-scaled by 1 / (P0 P1); the oracle's are [qi][qj][t][s], unscaled
+    o._kq = {"x": np.ascontiguousarray(peec.kernel_quadrant(st, 0).transpose(1, 0, 2, 3)) * (o.Px[0] * o.Px[1]),
+             "y": np.ascontiguousarray(peec.kernel_quadrant(st, 1).transpose(1, 0, 2, 3)) * (o.Py[0] * o.Py[1])}
+    t0 = time.perf_counter()
+    y = o.Lp_matvec(I)
+    dt = time.perf_counter() - t0
+    err = float(np.max(np.abs(y - y_dev)) / np.max(np.abs(y)))
+    return {"seconds": dt, "cores": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": "one L_p product of the six-layer 2048^2 board (FFT 4096 x 4096, 6 sheets, both orientations), "
+                      "oracle's numpy / scipy path, timed once", "rel_diff_vs_device": err}
+
+
+def large(cpu: bool = True) -> dict:
+    """The six-layer 2048^2 board (full inter-layer mutual blocks, two planes split into Voronoi regions, 200
+    stitching vias, 1 A at 1 GHz): set-up, one plain FP64 solve to 1e-8 (its solves/s), one FP64 GMRES over an FP32
+    basis and one mixed-precision solve (FP32 inner GMRES under FP64 refinement), each with its true residual and its
+    distance from the FP64 solve, the L_p product's achieved HBM GB/s against peak (10 one-right-hand-side launches
+    timed by HIP events on the library's stream), and the same product on the host."""
     import torch
 
-    from groundtruth_amd import boards, peec
-    b = boards.large()
-    st = peec.PlaneStack(b.layers, b.branches, device=0)
-    st.handle
-    t = time_lp(st, reps=10, nr=1)
-    B = lp_algorithmic_bytes(st)
-    out = {"workload": "six layers of 2048x2048 cells (bench.py --large's board), FFT 4096 x 4096 per orientation",
-           "us_per_launch": t * 1e6, "bytes_per_launch": B, "achieved": B / t / 1e9, "peak": HBM_PEAK_GBS,
-           "unit": "GB/s", "frac": B / t / 1e9 / HBM_PEAK_GBS}
-    mt = measured_traffic("large", "lp_traffic_six_layer_2048.md")
-    out["traffic"] = mt[0] if mt else None
-    del st
-    torch.cuda.empty_cache()
-    return out
-
-
-def large_board() -> dict:
-    """The six-layer 2048^2 board (full inter-layer mutual blocks, two planes split into Voronoi regions, 200
-    stitching vias, 1 A at 1 GHz): set-up, one plain FP64 solve, one mixed-precision solve (FP32 inner GMRES under
-    FP64 refinement) and one FP64 GMRES over an FP32 basis, each to 1e-8 with its true residual, and the L_p
-    product's achieved HBM GB/s against peak."""
     from groundtruth_amd import boards, peec
     board = boards.large()
     route = board.routes[0]
     f = float(board.freqs[0])
+    log("six-layer 2048^2 board: set-up")
     t0 = time.perf_counter()
     st = peec.PlaneStack(board.layers, board.branches, device=0)
     ex = peec.RouteExcitation(st, route.trace, route.driver, route.receiver)
@@ -162,26 +184,41 @@ def large_board() -> dict:
     rhs_top, J = ex.rhs(f)
     rhs = np.concatenate([rhs_top, np.asarray(J, float)[st.keep].astype(complex)])
     op = st.operator(f)
-    out = {"workload": "six layers of 2048x2048 cells at 0.05 mm, two planes split into Voronoi regions, 200 "
+    out = {"workload": "large: six layers of 2048x2048 cells at 0.05 mm, two planes split into Voronoi regions, 200 "
                        "stitching vias (999 branches), 1 A at 1 GHz, GMRES(100) to 1e-8",
-           "unknowns": int(st.nb + st.n_keep), "setup_s": setup_s, "woodbury": st.woodbury_form,
-           "schur": st.schur_stats(), "device_gb": st.device_bytes() / 1e9}
+           "unknowns": int(st.nb + st.n_keep), "setup_s": round(setup_s, 2), "woodbury": st.woodbury_form,
+           "schur": st.schur_stats(), "device_gb": round(st.device_bytes() / 1e9, 2)}
     sols = {}
-    for prec in ("fp64", "mixed", "fp32-basis"):
+    for prec in ("fp64", "fp32-basis", "mixed"):
+        log(f"six-layer board: one {prec} solve")
+        torch.cuda.synchronize()
         t1 = time.perf_counter()
         sol = ex.solve_many([f], tol=board.tol, restart=board.restart, precision=prec)[0]
         dt = time.perf_counter() - t1
         res = float(np.linalg.norm(rhs - st.matvec(op, sol.x)) / np.linalg.norm(rhs))
         sols[prec] = sol
-        out[prec] = {"seconds": dt, "iterations": sol.iterations, "true_residual": res,
-                     "Z_loop": [sol.Z_loop.real, sol.Z_loop.imag]}
+        out[prec] = {"seconds": round(dt, 3), "solves_per_s": 1.0 / dt, "iterations": sol.iterations,
+                     "true_residual": res, "Z_loop": [sol.Z_loop.real, sol.Z_loop.imag]}
     for prec in ("mixed", "fp32-basis"):
         out[prec]["x_vs_fp64"] = float(np.linalg.norm(sols[prec].x - sols["fp64"].x) / np.linalg.norm(sols["fp64"].x))
         out[prec]["Z_vs_fp64"] = abs(sols[prec].Z_loop - sols["fp64"].Z_loop) / abs(sols["fp64"].Z_loop)
+        out[prec]["speedup_vs_fp64"] = out["fp64"]["seconds"] / out[prec]["seconds"]
+    del sols
     t_lp = time_lp(st, reps=10, nr=1)
-    B = lp_algorithmic_bytes(st)
-    out["lp"] = {"us": t_lp * 1e6, "algorithmic_bytes": B, "achieved_gbs": B / t_lp / 1e9, "peak_gbs": HBM_PEAK_GBS,
-                 "frac": B / t_lp / 1e9 / HBM_PEAK_GBS}
+    B = lp_algorithmic_bytes(st, 1)
+    out["lp"] = {"us_per_launch": t_lp * 1e6, "bytes_per_launch": B, "achieved": B / t_lp / 1e9,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": B / t_lp / 1e9 / HBM_PEAK_GBS}
+    mt = measured_traffic("large", "lp_traffic_six_layer_2048.md")
+    out["lp"]["traffic"] = mt[0] if mt else None
+    if cpu:
+        log("six-layer board: the L_p product on the host")
+        rng = np.random.default_rng(4)
+        I = rng.standard_normal(st.ns) + 1j * rng.standard_normal(st.ns)
+        y = st.Lp_matvec(I)
+        out["cpu_lp"] = cpu_lp_large(st, I, y)
+        out["cpu_lp"]["host_over_device"] = out["cpu_lp"]["seconds"] / t_lp
+    del st, ex, op
+    torch.cuda.empty_cache()
     return out
 
 
@@ -444,21 +481,48 @@ def serve_whatif_512(rate: float = WHATIF_RATE, seconds: float = WHATIF_SECONDS,
             "target_p99_ms": 200.0}
 
 
+def host_info() -> dict:
+    """The host the CPU numbers ran on: cores, the affinity the process got, the CPU model, numpy and scipy."""
+    import scipy
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "numpy": np.__version__, "scipy": scipy.__version__}
+
+
 def cpu_baseline(board, route, f: float) -> dict:
-    """The oracle (numpy / scipy restatement of the reference's path) solving one frequency of the workload on
-    this host: its factorisation and its GMRES, timed once after a warm-up of the kernels."""
+    """The oracle (the numpy / scipy restatement of the reference's path, with the reference's settings: scipy FFTs on
+    every core, the numpy multiply-add, a complex128 basis with zgemv Gram-Schmidt, SuperLU on one thread) solving one
+    frequency of the workload on this host: after a warm-up solve, the best of 3, each its own SuperLU factorisation
+    (timed apart) plus its GMRES. The GPU's value excludes the board's set-up (a resident board); the host's includes
+    the factorisation, which the reference redoes per frequency."""
     from oracle import peec as O
     st = O.PlaneStack(board.layers, board.branches)
     ex = O.RouteExcitation(st, route.trace, route.driver, route.receiver)
-    t0 = time.perf_counter()
-    st.operator(f)
-    t1 = time.perf_counter()
-    sol = ex.solve(f, tol=board.tol, restart=board.restart)
-    t2 = time.perf_counter()
-    return {"value": 1.0 / (t2 - t0), "unit": "solves/s", "cores": os.cpu_count(), "kind": "port",
-            "sample": f"1 of the 10 solves (f = {f:.4g} Hz, GMRES({board.restart}), {sol.iterations} iterations): factorisation "
-                      f"{t1 - t0:.2f} s + GMRES {t2 - t1:.2f} s; scipy FFT on all {os.cpu_count()} cores, "
-                      f"SuperLU 1 thread; {platform.processor() or platform.machine()}"}
+    ex.solve(f, tol=board.tol, restart=board.restart)          # warm-up (and the kernels, cached)
+    best, fac, its = None, None, 0
+    for _ in range(3):
+        st._ops.clear()
+        t0 = time.perf_counter()
+        st.operator(f)
+        t1 = time.perf_counter()
+        sol = ex.solve(f, tol=board.tol, restart=board.restart)
+        t2 = time.perf_counter()
+        if best is None or t2 - t0 < best:
+            best, fac, its = t2 - t0, t1 - t0, sol.iterations
+    info = host_info()
+    return {"value": 1.0 / best, "unit": "solves/s", "cores": info["affinity"], "kind": "port",
+            "sample": f"one of the workload's 10 solves (f = {f:.4g} Hz, GMRES({board.restart}) to {board.tol:g}, "
+                      f"{its} iterations), best of 3 after a warm-up: {best:.2f} s, of which the SuperLU "
+                      f"factorisation {fac:.2f} s; scipy FFTs on all {info['affinity']} cores, SuperLU on 1",
+            "host": info}
 
 
 def main():
@@ -476,7 +540,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--large", action="store_true", help="the six-layer 2048^2 board alone (a few minutes)")
-    ap.add_argument("--no-large-lp", action="store_true", help="skip the six-layer board's L_p timing (about a minute)")
+    ap.add_argument("--no-large", action="store_true", help="skip the six-layer board (about three minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent boards and frequencies "
@@ -487,7 +551,7 @@ def main():
         lp_only(args)
         return
     if args.large:
-        print(json.dumps({"large": large_board()}))
+        print(json.dumps({"large": large(cpu=not args.no_cpu_baseline)}))
         return
     if args.serve_only:
         print(json.dumps({"served_backlog": serve_backlog(),
@@ -499,6 +563,7 @@ def main():
     from groundtruth_amd import boards, peec
     board = boards.WORKLOADS[args.workload]()
     route = board.routes[0]
+    log(f"workload {args.workload}: set-up")
     t_setup = time.perf_counter()
     st = peec.PlaneStack(board.layers, board.branches, device=0)
     ex = peec.RouteExcitation(st, route.trace, route.driver, route.receiver)
@@ -527,6 +592,7 @@ def main():
 
     from groundtruth_amd import _lib
     L = _lib.lib()
+    log(f"{args.warmup} warm-up and {args.steps} timed steps (plain FP64)")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -546,12 +612,13 @@ def main():
     x_mix = [torch.zeros_like(r) for r in rhs_dev]
 
     def step_mixed():
-        res = st.gmres_batch_dev(ops, rhs_dev, x_mix, MIXED_TOL, restart=board.restart, precision="mixed")
+        res = st.gmres_batch_dev(ops, rhs_dev, x_mix, board.tol, restart=board.restart, precision="mixed")
         for op, r in zip(ops, res):
             if not r.converged:
-                raise RuntimeError(f"mixed-precision refinement did not reach {MIXED_TOL:g} at {op.f:.4g} Hz")
+                raise RuntimeError(f"mixed-precision GMRES did not reach {board.tol:g} at {op.f:.4g} Hz")
         return [int(r.iterations) for r in res], [float(r.residual) for r in res]
 
+    log("the same steps in mixed precision")
     for _ in range(args.warmup):
         step_mixed()
     torch.cuda.synchronize()
@@ -570,6 +637,7 @@ def main():
                 raise RuntimeError(f"FP32-basis GMRES did not converge at {op.f:.4g} Hz")
         return [int(r.iterations) for r in res], [float(r.residual) for r in res]
 
+    log("the same steps with FP32 Krylov bases")
     for _ in range(args.warmup):
         step_f32b()
     torch.cuda.synchronize()
@@ -595,7 +663,8 @@ def main():
     dt_host = time.perf_counter() - t1
     t_lp = time_lp(st)
     B = lp_algorithmic_bytes(st)
-    achieved = lp_rhs * B / (lp_us * 1e-6) / 1e9
+    B_launch = lp_algorithmic_bytes(st, 1) + (lp_rhs - 1) * lp_bytes(st)[0]    # the kernel term once per launch
+    achieved = B_launch / (lp_us * 1e-6) / 1e9
     traffic = measured_traffic(args.workload)
     out = {
         "metric": METRIC,
@@ -625,17 +694,20 @@ def main():
                      "kernel": "L_p product (lp_rows_fwd + lp_cols_mac + lp_rows_inv), as launched by the timed "
                                "batched solves",
                      "launches": n_lp.value, "rhs_per_launch": lp_rhs, "us_per_launch": lp_us,
-                     "bytes_per_launch": round(lp_rhs * B),
+                     "bytes_per_launch": round(B_launch),
+                     "bytes_formula": "rhs_per_launch x (2c n_o + 4c S P0 P1) + r S^2 (P0/2+1)(P1/2+1) summed over "
+                                      "the orientations, c = 16, r = 8: the kernel quadrants once per launch",
                      "single_rhs": {"us_per_launch": t_lp * 1e6, "achieved": B / t_lp / 1e9,
                                     "bytes_per_launch": B}},
     }
     out["mixed"] = {"value": solves / dt_mix, "unit": "solves/s", "ms_per_step": dt_mix / args.steps * 1e3,
-                    "speedup_vs_fp64": (solves / dt_mix) / (solves / dt), "tol": MIXED_TOL,
+                    "speedup_vs_fp64": (solves / dt_mix) / (solves / dt), "tol": board.tol,
                     "iterations": its_mix, "residual": res_mix,
                     "rel_error": err_mix, "rel_error_fp64_run": err_fp64,
                     "error_reference": "the plain FP64 solve of the same system at tol 1e-13",
-                    "precision": "FP32 L_p FFT convolution and Krylov vectors; FP64 Arnoldi dot products, Givens, "
-                                 "Schur preconditioner and iterative refinement"}
+                    "precision": "the reference's restarted GMRES with FP32 cycles (L_p FFT convolution, incidence "
+                                 "rows, Krylov vectors and basis) and every restart an FP64 refinement step (r = b - K x,"
+                                 " x += M^-1 y); FP64 Arnoldi dot products, Givens and Schur preconditioner"}
     out["fp32_basis"] = {"value": solves / dt_f32b, "unit": "solves/s", "ms_per_step": dt_f32b / args.steps * 1e3,
                          "speedup_vs_fp64": (solves / dt_f32b) / (solves / dt), "tol": board.tol,
                          "iterations": its_f32b, "residual": res_f32b, "rel_error": err_f32b,
@@ -644,6 +716,7 @@ def main():
                          "precision": "Krylov basis stored in FP32; operator products, Arnoldi sums, Givens, Schur "
                                       "preconditioner, iterate and true-residual checks in FP64"}
     if not args.no_serve:
+        log("served what-if stream, batched and one at a time; 512^2 what-ifs; the backlog")
         served = serve_stream(args.serve_rate, args.serve_seconds)
         plain = serve_stream(args.serve_rate, args.serve_seconds, batched=False)
         out["served"] = dict(served, rate_per_s=args.serve_rate, tol=SERVE_TOL,
@@ -657,13 +730,16 @@ def main():
                                     "board solution at f (the boards primed before the stream, untimed)")
         out["whatif_512"] = serve_whatif_512()
         out["served_backlog"] = serve_backlog()
-    if not args.no_large_lp:
-        try:
-            out["large_lp"] = large_lp()
-        except Exception as e:      # noqa: BLE001 - the headline line above stands on its own
-            out["large_lp"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[5])
+        log("the oracle's solve on the host (best of 3)")
+        out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[0])
+    del st, ex, ops, rhs_dev, x_dev, x_mix, x_f32b, x_ref
+    torch.cuda.empty_cache()
+    if not args.no_large:
+        try:
+            out["large"] = large(cpu=not args.no_cpu_baseline)
+        except Exception as e:      # noqa: BLE001 - the headline line above stands on its own
+            out["large"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

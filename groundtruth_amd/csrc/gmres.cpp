@@ -748,6 +748,8 @@ void gmres_batch_fp32_basis(gt_stack* st, int nr, gt_op* const* ops, const zd* c
     gmres_batch_t<float>(st, nr, ops, b, x, p, res);
 }
 
+constexpr double GMRES_PROGRESS_MIXED = 0.7;    // peec.py:99: extra cycles while each falls to this fraction
+
 void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const* b, zd* const* x,
                        const gt_gmres_params& p, double inner_tol, int max_refine, gt_gmres_result* res) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "1 to 16 right-hand sides per batch");
@@ -817,17 +819,27 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
         else act.push_back(i);
     }
     last = b0;
+    // Each round is one restart cycle of the reference's GMRES (peec.py:3093-3224) run in FP32 on the current FP64
+    // residual: (K M^-1) y = r, stopping early only where it meets the outer tolerance (relative to |r|, but never
+    // below inner_tol, what an FP32 cycle can resolve); then x += M^-1 y and r = b - K x in FP64. The rounds follow the
+    // reference's restart rule: max_cycles of them, then extra ones while each brings |r| to 0.7 of the round before.
+    std::vector<int> rounds(nr, 0);
     for (int round = 0; round < max_refine && !act.empty(); ++round) {
         const int na = (int)act.size();
         const std::vector<gt_op*> aop = subset(act, std::vector<gt_op*>(ops, ops + nr));
-        // the inner solve in FP32: (K M^-1) y = r to inner_tol, y in the inner batch's x
         k_to_float(na, subset(act, r32).data(), subset(act, r).data(), n, s);
-        gt_gmres_params ip = p;
-        ip.tol = inner_tol;
-        ip.bnorm = 0;
-        ip.restart = restart;
         Batch<float, float> bt(st, na, restart, gw);
-        for (int i = 0; i < na; ++i) bt.submit(i, aop[i], r32[act[i]], ip);
+        for (int i = 0; i < na; ++i) {
+            const int q = act[i];
+            gt_gmres_params ip = p;
+            ip.tol = std::min(0.5, std::max(inner_tol, p.tol * bnorm[q] / last[q]));
+            ip.bnorm = 0;
+            ip.restart = restart;
+            ip.max_cycles = 1;
+            ip.extra_cycles = 0;
+            ip.max_its = 0;
+            bt.submit(i, aop[i], r32[q], ip);
+        }
         bt.run();
         // d = M^-1 y in FP64 (y widened), x += d, u = K x, r = b - u
         {
@@ -873,10 +885,12 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
             its[q] += bt.R[i].its;
             prods[q] += bt.R[i].products + 2;
             resid[q] = rn[i] / bnorm[q];
+            ++rounds[q];
             if (rn[i] <= p.tol * bnorm[q]) {
                 conv[q] = 1;
-            } else if (rn[i] > 0.5 * last[q]) {
-                conv[q] = 0;                   // the refinement stalled
+            } else if (rounds[q] >= p.max_cycles &&
+                       (rounds[q] >= p.max_cycles + p.extra_cycles || rn[i] > GMRES_PROGRESS_MIXED * last[q])) {
+                conv[q] = 0;                   // out of cycles, or no longer falling to 0.7 per cycle
             } else {
                 next.push_back(q);
             }

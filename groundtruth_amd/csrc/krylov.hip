@@ -304,15 +304,16 @@ __global__ __launch_bounds__(NTH) void kr_update(KrSetT<TV, TW> k, int n, int ch
 // update pass over the basis. Row m of a basis holds u, the once-projected and normalised vector the product was
 // taken of (v = A u); rows 0..m-1 are final.
 
-// Pass A: t = Q^H u and p = Q^H v over the rows [0, m) in groups of DG2 rows, and u^H u, u^H v (with group 0).
+// Pass A for long FP64 vectors: t = Q^H u and p = Q^H v over the rows [0, m) in groups of DG2 rows, and u^H u,
+// u^H v (with group 0).
 // v may be absent (a basis's last row: t and u^H u only). A workgroup takes one chunk of elements and gpw
 // consecutive groups: its u and v elements are loaded once into registers and serve every group it takes (long
 // vectors take every group in one workgroup, so u and v are read once per pass; short ones one group per workgroup,
 // for enough workgroups). Each (row, chunk) sum is the same whatever gpw is: the same elements per lane in the same
 // order, then the same wave and workgroup trees.
 template <class TV, class TW>
-__global__ __launch_bounds__(NTH) void kr_dots2(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
-                                                int slot_t, int slot_uu, int gpw) {
+__global__ __launch_bounds__(NTH) void kr_dots2_long(KrSetT<TV, TW> k, int n, int chunk, double* part, int P,
+                                                     int nblk, int slot_t, int slot_uu, int gpw) {
     constexpr int E = lane_elems<TV>();
     constexpr int DG2 = 8;
     constexpr int ST = 4;                       // element steps per lane: chunk = ST * E * NTH
@@ -403,6 +404,86 @@ __global__ __launch_bounds__(NTH) void kr_dots2(KrSetT<TV, TW> k, int n, int chu
         else if (t >= 64 && t < 64 + 2 * gm) out[2 * g0 + t - 64] = sum(2 * DG2 + t - 64);
         else if (grp == 0 && t >= 128 && t < 131) out[slot_uu + t - 128] = sum(4 * DG2 + t - 128);
     }
+}
+
+// Pass A (short vectors, and FP32 bases): t = Q^H u and p = Q^H v over the rows [0, m) in groups of DG2 rows per
+// workgroup, and u^H u, u^H v (group 0).
+// v may be absent (a basis's last row: t and u^H u only).
+template <class TV, class TW>
+__global__ __launch_bounds__(NTH) void kr_dots2(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
+                                                int slot_t, int slot_uu) {
+    constexpr int E = lane_elems<TV>();
+    constexpr int DG2 = 8;
+    __shared__ double red[NTH / 64][4 * DG2 + 3];
+    const int p = blockIdx.y;
+    const int grp = blockIdx.x / nblk, blk = blockIdx.x - grp * nblk;
+    const int m = k.m[p];
+    const int g0 = grp * DG2;
+    if (m < 0 || (g0 > 0 && g0 >= m)) return;     // m < 0: a slot not in this step
+    const int gm = max(0, min(DG2, m - g0));
+    const cpx<TV>* __restrict__ V = k.V[p] + (size_t)g0 * k.ld;
+    const cpx<TV>* __restrict__ u = k.V[p] + (size_t)m * k.ld;
+    const cpx<TW>* __restrict__ v = k.w[p];
+    const bool hv = v != nullptr;
+    if (k.n[p] > 0) n = k.n[p];
+    const int i0 = blk * chunk, i1 = min(n, i0 + chunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double tr[DG2], ti[DG2], pr[DG2], pi[DG2];
+#pragma unroll
+    for (int r = 0; r < DG2; ++r) tr[r] = ti[r] = pr[r] = pi[r] = 0;
+    double uu = 0, uvr = 0, uvi = 0;
+    for (int i = i0 + E * threadIdx.x; i < i1; i += E * NTH) {
+        const bool in2 = i + 1 < i1;
+        zd uv_[E], vv_[E];
+        ld_basis<TV>(u + i, true, in2, uv_);
+#pragma unroll
+        for (int c = 0; c < E; ++c) vv_[c] = hv && (c == 0 || in2) ? ldz(v[i + c]) : zd();
+        zd q[DG2][E];
+#pragma unroll
+        for (int r = 0; r < DG2; ++r) ld_basis<TV>(V + (size_t)r * k.ld + i, r < gm, in2, q[r]);
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            uu += uv_[c].re * uv_[c].re + uv_[c].im * uv_[c].im;
+            uvr += uv_[c].re * vv_[c].re + uv_[c].im * vv_[c].im;
+            uvi += uv_[c].re * vv_[c].im - uv_[c].im * vv_[c].re;
+#pragma unroll
+            for (int r = 0; r < DG2; ++r) {
+                tr[r] += q[r][c].re * uv_[c].re + q[r][c].im * uv_[c].im;
+                ti[r] += q[r][c].re * uv_[c].im - q[r][c].im * uv_[c].re;
+                pr[r] += q[r][c].re * vv_[c].re + q[r][c].im * vv_[c].im;
+                pi[r] += q[r][c].re * vv_[c].im - q[r][c].im * vv_[c].re;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < DG2; ++r) {
+        tr[r] = wave_sum(tr[r]);
+        ti[r] = wave_sum(ti[r]);
+        pr[r] = wave_sum(pr[r]);
+        pi[r] = wave_sum(pi[r]);
+    }
+    uu = wave_sum(uu);
+    uvr = wave_sum(uvr);
+    uvi = wave_sum(uvi);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < DG2; ++r) {
+            red[wave][2 * r] = tr[r];
+            red[wave][2 * r + 1] = ti[r];
+            red[wave][2 * DG2 + 2 * r] = pr[r];
+            red[wave][2 * DG2 + 2 * r + 1] = pi[r];
+        }
+        red[wave][4 * DG2] = uu;
+        red[wave][4 * DG2 + 1] = uvr;
+        red[wave][4 * DG2 + 2] = uvi;
+    }
+    __syncthreads();
+    double* out = part + ((size_t)p * nblk + blk) * P;
+    const int t = threadIdx.x;
+    const auto sum = [&](int c) { return red[0][c] + red[1][c] + red[2][c] + red[3][c]; };
+    if (t < 2 * gm) out[slot_t + 2 * g0 + t] = sum(t);
+    else if (t >= 64 && t < 64 + 2 * gm) out[2 * g0 + t - 64] = sum(2 * DG2 + t - 64);
+    else if (grp == 0 && t >= 128 && t < 131) out[slot_uu + t - 128] = sum(4 * DG2 + t - 128);
 }
 
 // Pass B: q = (u - Q t) / alpha written over u (row m), and y = v - Q p - q (q^H v) written over v, |y|^2 after; one
@@ -674,10 +755,14 @@ void Krylov::dots2(const KrSetT<TV, TW>& k, hipStream_t s) {
     static_assert(chunk2<TV>() == 4 * lane_elems<TV>() * NTH, "kr_dots2 takes four element steps per lane");
     const int c2 = chunk2<TV>(), nb2 = blocks2<TV>();
     // enough chunks to fill the device: every group in one workgroup per chunk (u and v read once)
-    const int gpw = (int64_t)nb2 * k.nr >= 8192 ? groups : 1;
-    const int gsets = (groups + gpw - 1) / gpw;
-    hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nb2 * gsets, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
-                       slot_h2(), slot_uu(), gpw);
+    // (FP64 bases only: an FP32 pass's u and v pairs in registers would cost it occupancy). Both kernels give the
+    // same bits: the same elements per lane in the same order, the same trees.
+    if (sizeof(TV) == 8 && (int64_t)nb2 * k.nr >= 8192)
+        hipLaunchKernelGGL((kr_dots2_long<TV, TW>), dim3(nb2, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
+                           slot_h2(), slot_uu(), groups);
+    else
+        hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nb2 * groups, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
+                           slot_h2(), slot_uu());
     GT_LAUNCH("kr_dots2");
     if (mmax > 0) {
         hipLaunchKernelGGL(kr_reduce, dim3(2 * mmax, k.nr), dim3(NTH), 0, s, part.p, nb2, P, 0, red.p);

@@ -84,6 +84,9 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
     // pixel (i, j) of the full spectrum reads quadrant (min(i, P0 - i), min(j, P1 - j)): the spectrum is even
     const int qj = min(j, o.P1 - j);
     const int h0 = P0 / 2 + 1;
+    __shared__ int gm[MAXS * MAXS];
+    for (int e = threadIdx.x; e < S * S; e += NTH) gm[e] = a.gmap[e];
+    __syncthreads();
     for (int i = threadIdx.x; i < P0; i += NTH) {
         const int qi = min(i, P0 - i);
         const T* Kp = o.K + ((size_t)qj * h0 + qi) * a.G;
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
 #pragma unroll
                 for (int s = 0; s < MAXS; ++s) {
                     if (s < S) {
-                        const T k = Kp[a.gmap[t * S + s]];
+                        const T k = Kp[gm[t * S + s]];
                         acc.re += k * v[s].re;
                         acc.im += k * v[s].im;
                     }
@@ -245,6 +248,9 @@ __global__ __launch_bounds__(64 * CW) void lp_cols_mac_w(LpArgs<T> a) {
     // pixel (i, j) of the full spectrum reads quadrant (min(i, P0 - i), min(j, P1 - j)): the spectrum is even
     const int qj = min(j, P1 - j);
     constexpr int h0 = N / 2 + 1;
+    int gi[S * S];
+#pragma unroll
+    for (int e = 0; e < S * S; ++e) gi[e] = a.gmap[e];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int i = lane + 64 * m;
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(64 * CW) void lp_cols_mac_w(LpArgs<T> a) {
         for (int t = 0; t < S; ++t) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const T k = Kp[a.gmap[t * S + s]];
+                const T k = Kp[gi[t * S + s]];
                 out[t].re += k * v[s][m].re;
                 out[t].im += k * v[s][m].im;
             }

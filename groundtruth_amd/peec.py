@@ -37,8 +37,8 @@ GMRES_CHECK_MISSES = 4          # peec.py:100
 SCREEN_MIN_NODES = 16           # peec.py:112
 OPERATORS_KEPT = 32             # per-frequency operators kept per stack (small device arrays each)
 BATCH = 16                      # right-hand sides per batched device launch
-MIXED_INNER_TOL = 3e-4          # mixed precision: each FP32 inner solve's relative residual
-MIXED_MAX_REFINE = 10           # mixed precision: FP64 refinement rounds at most
+MIXED_INNER_TOL = 1e-6          # mixed precision: the smallest relative residual an FP32 cycle is asked for
+MIXED_MAX_REFINE = 64           # mixed precision: FP64 restarts at most (the reference's cycle rule governs)
 
 _KERNELS: dict = {}
 

@@ -311,14 +311,16 @@ int gt_gmres_dev(gt_op* op, const void* b_dev, void* x_dev, const gt_gmres_param
 int gt_gmres_batch(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
                    const gt_gmres_params* params, gt_gmres_result* results);
 
-/* The same batch in mixed precision: iterative refinement in FP64 (peec.py:2560-2607's residual form) around an
- * inner GMRES whose vectors, basis and operator products run in FP32 (the L_p FFT convolution and the incidence rows
- * in FP32; the Arnoldi dot products, the Givens least-squares, the Schur solve of the preconditioner and every
- * reduction in FP64). Each round: r = b - K x in FP64, the inner solve of (K M^-1) y = r to inner_tol relative to
- * |r|, x += M^-1 y in FP64; until |r| <= tol * bnorm (params->tol), a round that does not halve |r|, or max_refine
- * rounds. results[i].iterations sums the inner iterations; residual is the final FP64 relative residual. Not in the
- * reference, which stays FP64 (its FP32 product stalled GMRES, docs/ARCHITECTURE.md:299-301): here FP32 only
- * carries the inner solves. */
+/* The same batch in mixed precision: the reference's restarted GMRES (peec.py:3093-3224) whose restart cycles run in
+ * FP32 (vectors, basis and operator products: the L_p FFT convolution and the incidence rows) while every restart
+ * is an FP64 refinement step (peec.py:2560-2607's residual form); the Arnoldi dot products, the Givens
+ * least-squares, the Schur solve of the preconditioner and every reduction stay FP64. Each round: r = b - K x in
+ * FP64, one FP32 cycle of (K M^-1) y = r (params->restart iterations; it stops early only at the outer tolerance
+ * relative to |r|, or at inner_tol, the smallest relative residual an FP32 cycle is asked for), x += M^-1 y in FP64;
+ * until |r| <= tol * bnorm, after max_cycles rounds a round that does not bring |r| to 0.7 of the one before (the
+ * reference's extra cycles, up to extra_cycles more), or max_refine rounds. results[i].iterations sums the cycles'
+ * iterations; residual is the final FP64 relative residual. Not in the reference, which stays FP64 (its FP32 product
+ * stalled GMRES, docs/ARCHITECTURE.md:299-301): here FP32 only carries the cycles, and each restart is exact. */
 int gt_gmres_batch_mixed(gt_stack* stack, int32_t nr, gt_op* const* ops, const void* const* b_dev, void* const* x_dev,
                          const gt_gmres_params* params, double inner_tol, int32_t max_refine,
                          gt_gmres_result* results);
