@@ -71,6 +71,10 @@ SIGNATURES = {
     "gt_lp_matvec_batch_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p),
                                              C.POINTER(C.c_void_p)]),
     "gt_schur_factor": (C.c_int, [C.c_void_p, i32p, f64p]),
+    "gt_stack_derive_bytes": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64)]),
+    "gt_stack_create_like": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "gt_matvec_on": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_schur_factor_nodes": (C.c_int, [C.c_void_p, i32p, i32p, i32p, f64p]),
     "gt_stack_derive": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, f64p, C.POINTER(C.c_void_p)]),
     "gt_ctx_lp_timing": (C.c_int, [C.c_void_p, C.c_int]),

@@ -236,6 +236,46 @@ def large(n: int = 2048, pitch_mm: float = 0.05, seed: int = 1005, n_vias: int =
     return Board("large", layers, branches, [route], [1e9], tol=1e-8, restart=100, seed=seed)
 
 
+def moved_slot(board: Board, rng, protect=()) -> tuple:
+    """A topology-changing what-if of `board`: its seeded slot (the two-layer board's, in its lower plane) moved to a
+    seeded new place, or on a board without one a new slot of the same kind (0.5 mm wide, 20-60 mm scaled to the
+    board) cut into a seeded plane. The slot avoids the cells in `protect` ((layer, i, j): the routes' injection
+    nodes) and leaves a margin of two cells at the board's edge. (new layers, (layer, vertical, a0, c0, length,
+    width))."""
+    g0 = board.layers[0].grid
+    n0, n1, pitch = g0.nx, g0.ny, g0.d
+    masks = [np.array(l.grid.node, bool) for l in board.layers]
+    old = board.meta.get("slot")
+    k = 1 if old is not None else int(rng.integers(len(board.layers)))
+    if old is not None:
+        vertical, a0, c0, length, width = old
+        if vertical:
+            masks[k][c0:c0 + width, a0:a0 + length] = True
+        else:
+            masks[k][a0:a0 + length, c0:c0 + width] = True
+    if old is not None:                     # the same slot, moved
+        length, width = old[3], old[4]
+    else:                                   # 20-60 mm of the two-layer board's 64 mm side, 0.5 mm wide
+        width = max(1, int(round(0.5 / pitch)))
+        length = int(round(rng.uniform(20.0, 60.0) / 64.0 * min(n0, n1)))
+    length = max(2, min(length, min(n0, n1) - 6))
+    keep_out = {(i, j) for (l, i, j) in protect if l == k}
+    for _ in range(100):
+        vertical = bool(rng.integers(2))
+        a0 = int(rng.integers(2, min(n0, n1) - length - 2)) if min(n0, n1) - length - 4 > 0 else 2
+        c0 = int(rng.integers(2, (n0 if vertical else n1) - width - 2))
+        cells = ({(c0 + w, a0 + t) for w in range(width) for t in range(length)} if vertical
+                 else {(a0 + t, c0 + w) for w in range(width) for t in range(length)})
+        if not cells & keep_out:
+            break
+    m = masks[k].copy()
+    for (i, j) in cells:
+        m[i, j] = False
+    masks[k] = m
+    layers = [PlaneLayer(l.name, l.z, Grid.from_mask(mk, pitch), l.thickness) for l, mk in zip(board.layers, masks)]
+    return layers, (k, vertical, a0, c0, length, width)
+
+
 WORKLOADS = {"single_plane": single_plane, "two_layer": two_layer, "four_layer": four_layer, "large": large}
 
 

@@ -74,7 +74,7 @@ const DevBuf<zd>& twiddles(gt_stack* st, int P) {
 }
 
 void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int n0, int n1, const int32_t* P,
-                  const double* K, const KernelGen* kg) {
+                  const double* K, const KernelGen* kg, const gt_stack* donor = nullptr) {
     OrientHost& o = st->o[oi];
     o.present = cnt > 0;
     o.cnt = cnt;
@@ -82,7 +82,7 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.n0 = n0;
     o.n1 = n1;
     if (!o.present) return;
-    need(K != nullptr || kg != nullptr, "kernel spectrum missing");
+    need(K != nullptr || kg != nullptr || donor != nullptr, "kernel spectrum missing");
     need(pow2(P[0]) && pow2(P[1]), "FFT sizes must be powers of two");
     need(P[0] >= 2 * n0 - 1 && P[1] >= 2 * n1 - 1, "FFT sizes below 2n - 1 would wrap the convolution");
     need(n0 < 65536 && n1 < 65536, "grid above 65535 cells");
@@ -115,15 +115,22 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.rowcol.upload(rowcol, st->ctx->stream);
     twiddles(st, o.P0);
     twiddles(st, o.P1);
-    if (kg) {
+    if (donor) {
+        const OrientHost& od = donor->o[oi];
+        need(od.present && od.P0 == o.P0 && od.P1 == o.P1 && od.n0 == n0 && od.n1 == n1,
+             "the donor stack's kernel spectra are of another shape");
+        o.K = od.K;
+        o.Kf = od.Kf;
+        o.k_shared = true;
+    } else if (kg) {
         kernel_quadrant_device(st, oi, *kg);
     } else {
         const size_t kq = (size_t)(o.P1 / 2 + 1) * (o.P0 / 2 + 1) * S * S;      // G = S * S: the identity map
         std::vector<double> kd(K, K + kq);
         std::vector<float> kf(kq);
         for (size_t e = 0; e < kq; ++e) kf[e] = (float)kd[e];
-        o.K.upload(kd, st->ctx->stream);
-        o.Kf.upload(kf, st->ctx->stream);
+        o.K->upload(kd, st->ctx->stream);
+        o.Kf->upload(kf, st->ctx->stream);
     }
     o.t_off = st->t_elems;                      // each orientation its own region: they run in one launch
     st->t_elems += (size_t)S * o.P1 * o.n0;
@@ -171,7 +178,7 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
         d.cnt = h.cnt;
         d.rowptr = h.rowptr.p;
         d.rowcol = h.rowcol.p;
-        d.K = h.K.p;
+        d.K = h.K->p;
         d.T1 = work.T1.p + h.t_off;
         d.T2 = work.T2.p + h.t_off;
         d.T3 = s_elems ? work.T3.p + h.s_off : nullptr;
@@ -197,7 +204,7 @@ int64_t gt_stack_s::device_bytes() const {
     int64_t b = a_keep.bytes() + b_keep.bytes() + adj_ptr.bytes() + adj.bytes();
     b += work.bytes() + W.bytes() + G.bytes() + Yr.bytes();
     if (derived) return b;                 // the core is its base's
-    for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + h.K.bytes() + h.Kf.bytes();
+    for (const auto& h : o) b += h.rowptr.bytes() + h.rowcol.bytes() + (h.k_shared ? 0 : h.K->bytes() + h.Kf->bytes());
     for (const auto& t : tw) b += t.second.bytes();
     for (const auto& t : twf) b += t.second.bytes();
     b += node_sheet.bytes() + strip_sheet.bytes();
@@ -377,7 +384,8 @@ int gt_ctx_lp_time(gt_ctx* ctx, int64_t* launches, int64_t* rhs, double* ms) {
 
 namespace {
 
-void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd, gt_stack** out) {
+void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd, gt_stack** out,
+                  const gt_stack* donor = nullptr) {
         need(ctx && d && out, "NULL argument");
         std::lock_guard<std::mutex> lk(ctx->mu);
         *out = nullptr;
@@ -400,7 +408,12 @@ void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd,
         KernelGen kg;
         const int S = st->S;
         st->kgmap.assign((size_t)S * S, 0);
-        if (kd) {
+        if (donor) {
+            need(donor->ctx == ctx && donor->S == S && donor->nx == d->nx && donor->ny == d->ny,
+                 "the donor stack is of another context, sheet count or grid");
+            st->kgmap = donor->kgmap;
+            st->kG = donor->kG;
+        } else if (kd) {
             need(kd->d > 0 && kd->dz && kd->near_field, "kernel description incomplete");
             kg.d = kd->d;
             kg.mu = kd->mu;
@@ -432,8 +445,8 @@ void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd,
             st->kG = S * S;
         }
         st->kgmap_dev.upload(st->kgmap, s);
-        setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx, kd ? &kg : nullptr);
-        setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky, kd ? &kg : nullptr);
+        setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx, kd ? &kg : nullptr, donor);
+        setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky, kd ? &kg : nullptr, donor);
         if (st->shape.split) {                  // both orientations' whole column spectra, each its own region
             for (OrientHost& o : st->o) {
                 if (!o.present) continue;
@@ -476,6 +489,16 @@ int gt_stack_create_gen(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_des
     });
 }
 
+int gt_stack_create_like(gt_ctx* ctx, const gt_stack_desc* d, const gt_stack* donor, gt_stack** out) {
+    return guard([&] {
+        need(donor != nullptr, "NULL donor");
+        need(d && d->px[0] == donor->o[0].P0 && d->px[1] == donor->o[0].P1 && d->py[0] == donor->o[1].P0 &&
+                 d->py[1] == donor->o[1].P1,
+             "the donor's FFT sizes differ");
+        create_stack(ctx, d, nullptr, out, donor);
+    });
+}
+
 int gt_node_field(gt_stack* st, const void* x_dev, double d, void* kx_dev, void* ky_dev) {
     return guard([&] {
         need(st && x_dev && kx_dev && ky_dev, "NULL argument");
@@ -503,11 +526,11 @@ int gt_stack_kernel(const gt_stack* st, int32_t orient, double* out) {
         const OrientHost& o = st->o[orient];
         if (!o.present) return;
         // the resident G kernels per pixel, expanded to the S x S sheet pairs
-        std::vector<double> k(o.K.n);
-        GT_HIP(hipMemcpyAsync(k.data(), o.K.p, o.K.bytes(), hipMemcpyDeviceToHost, st->ctx->stream));
+        std::vector<double> k(o.K->n);
+        GT_HIP(hipMemcpyAsync(k.data(), o.K->p, o.K->bytes(), hipMemcpyDeviceToHost, st->ctx->stream));
         GT_HIP(hipStreamSynchronize(st->ctx->stream));
         const int S = st->S, G = st->kG;
-        const size_t npix = o.K.n / G;
+        const size_t npix = o.K->n / G;
         for (size_t q = 0; q < npix; ++q)
             for (int e = 0; e < S * S; ++e) out[q * S * S + e] = k[q * G + st->kgmap[e]];
     });
@@ -528,6 +551,7 @@ int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
         need(st && I_dev && out_dev, "NULL argument");
         std::lock_guard<std::mutex> lk(st->ctx->mu);
         if (st->ns == 0) return;
+        st->work.ensure(st, 1);                  // a what-if stack makes its scratch on its first own product
         LpArgs<double> a = st->lp_args(0);
         a.rhs.nr = 1;
         a.rhs.x[0] = (const zd*)I_dev;
@@ -779,9 +803,45 @@ int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra
         hipStream_t s = st->ctx->stream;
         woodbury_extend(st.get(), base, n_extra, a_keep_extra, b_keep_extra);
         if (G_out) woodbury_gram_device(st.get(), G_out);
-        st->work.ensure(st.get(), 1);
+        // no scratch of its own: its solves run in its base's batches (check_batch), and its own products make it
+        // when first called
         GT_HIP(hipStreamSynchronize(s));
         *out = st.release();
+    });
+}
+
+int gt_stack_derive_bytes(const gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra,
+                          const int32_t* b_keep_extra, int64_t* held, int64_t* peak) {
+    return guard([&] {
+        need(base && held && peak && n_extra >= 0 && (n_extra == 0 || (a_keep_extra && b_keep_extra)),
+             "bad argument");
+        std::lock_guard<std::mutex> lk(base->ctx->mu);
+        need(base->schur != nullptr, "the base stack has no Schur factor (gt_schur_factor first)");
+        const int64_t nb = base->nb + n_extra, nk = base->nk, k = base->k + n_extra;
+        int64_t ends = 0;
+        for (int b = 0; b < base->nb; ++b) ends += (base->h_a_keep[b] >= 0) + (base->h_b_keep[b] >= 0);
+        for (int e = 0; e < n_extra; ++e) ends += (a_keep_extra[e] >= 0) + (b_keep_extra[e] >= 0);
+        const int64_t i4 = sizeof(int), d8 = sizeof(double);
+        int64_t h = 2 * i4 * std::max<int64_t>(1, nb) + i4 * (nk + 1) + i4 * std::max<int64_t>(1, ends);
+        h += base->wb_implicit ? d8 * std::max<int64_t>(1, k * k) : d8 * std::max<int64_t>(1, k * nk);
+        int64_t t = 0;                           // set-up only: the added columns' solves, 16 at a time
+        if (n_extra > 0 && nk > 0) {
+            const int chunk = std::min<int>(n_extra, KMAX);
+            t = 2 * (int64_t)sizeof(zd) * nk * chunk;
+            t += std::max<int64_t>(0, base->schur->bytes_after(chunk) - base->schur->bytes_after(0));
+        }
+        *held = h;
+        *peak = h + t;
+    });
+}
+
+int gt_matvec_on(gt_stack* host, gt_op* op, const void* x_dev, void* y_dev) {
+    return guard([&] {
+        need(host && op && x_dev && y_dev && x_dev != y_dev, "NULL or aliased argument");
+        std::lock_guard<std::mutex> lk(host->ctx->mu);
+        const zd* x = (const zd*)x_dev;
+        zd* y = (zd*)y_dev;
+        ops_matvec(host, 1, &op, &x, &y);
     });
 }
 

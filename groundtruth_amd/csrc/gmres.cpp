@@ -967,6 +967,7 @@ gt_session* session_create(gt_stack* st, int cap, int restart, int basis, int ma
     ss->xout.assign(cap, nullptr);
     st->work.ensure(st, cap, st->k + max_extra);
     if (st->schur) st->schur->ensure(cap);
+    ++st->sessions;
     return ss.release();
 }
 
@@ -1038,7 +1039,16 @@ bool session_poll(gt_session* ss, int slot, gt_gmres_result* res) {
     });
 }
 
-void session_destroy(gt_session* ss) { delete ss; }
+// The last session of a stack gives back the scratch the sessions grew (the product workspace and the Schur solve's
+// buffers, to one right-hand side): a server that closes an idle session frees what it admitted for it.
+void session_destroy(gt_session* ss) {
+    gt_stack* st = ss->st;
+    delete ss;
+    if (--st->sessions == 0) {
+        st->work.reset(st);
+        if (st->schur) st->schur->shrink(1);
+    }
+}
 
 gt_stack* session_stack(gt_session* ss) { return ss->st; }
 

@@ -223,7 +223,7 @@ void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg) {
     GT_LAUNCH("kg_cos");
     // F[qj][qi][g] = sum_a w_a G[g][a][qj] cos(2 pi qi a / P0) / (P0 P1)
     const size_t kq = (size_t)h1 * h0 * npair;
-    o.K.alloc(kq);
+    o.K->alloc(kq);
     CosArgs c2{};
     c2.A = G.p;
     c2.a_pair = (long)n0 * h1;
@@ -231,7 +231,7 @@ void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg) {
     c2.sa_k = h1;
     c2.tw = st->tw.at(P0).p;
     c2.P = P0;
-    c2.C = o.K.p;
+    c2.C = o.K->p;
     c2.sc_m = (long)h0 * npair;
     c2.sc_n = npair;
     for (int p = 0; p < npair; ++p) {
@@ -244,8 +244,8 @@ void kernel_quadrant_device(gt_stack* st, int oi, const KernelGen& kg) {
     c2.K = n0;
     hipLaunchKernelGGL(kg_cos, dim3((h0 + CT - 1) / CT, (h1 + CT - 1) / CT, npair), dim3(256), 0, s, c2);
     GT_LAUNCH("kg_cos");
-    o.Kf.alloc(kq);
-    hipLaunchKernelGGL(kg_to_float, dim3(blocks_for(kq, 256)), dim3(256), 0, s, o.K.p, o.Kf.p, kq);
+    o.Kf->alloc(kq);
+    hipLaunchKernelGGL(kg_to_float, dim3(blocks_for(kq, 256)), dim3(256), 0, s, o.K->p, o.Kf->p, kq);
     GT_LAUNCH("kg_to_float");
     GT_HIP(hipStreamSynchronize(s));
 }

@@ -336,6 +336,20 @@ void Work::shrink(const gt_stack* st, int nr) {
     if (nr > 0) ensure(st, nr, kt);
 }
 
+void Work::reset(const gt_stack* st) {
+    T1.release();
+    T2.release();
+    T3.release();
+    s.release();
+    z.release();
+    z2.release();
+    t.release();
+    u.release();
+    cap = 0;
+    t_cols = 0;
+    ensure(st, 1);
+}
+
 int64_t Work::bytes_after(const gt_stack* st, int nr, int kt) const {
     kt = std::max(kt, st->nbr);
     const int64_t zb = sizeof(zd);
@@ -390,7 +404,7 @@ LpArgs<T> lp_args_t(const gt_stack* st) {
             o.cnt = od.cnt;
             o.rowptr = od.rowptr;
             o.rowcol = od.rowcol;
-            o.K = h.Kf.p;
+            o.K = h.Kf->p;
             o.T1 = reinterpret_cast<zf*>(st->work.T1.p) + h.t_off;
             o.T2 = reinterpret_cast<zf*>(st->work.T2.p) + h.t_off;
             o.T3 = st->s_elems ? reinterpret_cast<zf*>(st->work.T3.p) + h.s_off : nullptr;

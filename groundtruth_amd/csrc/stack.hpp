@@ -52,8 +52,10 @@ struct OrientHost {
     size_t s_off = 0;          // offset of its whole column spectra in T3 (the split column pass)
     size_t t_off = 0;          // offset of this orientation's L_p intermediates in T1 / T2
     DevBuf<int> rowptr, rowcol;
-    DevBuf<double> K;
-    DevBuf<float> Kf;
+    // the kernel spectra, shared by the stacks made like this one (gt_stack_create_like: same grid and sheet gaps)
+    std::shared_ptr<DevBuf<double>> K = std::make_shared<DevBuf<double>>();
+    std::shared_ptr<DevBuf<float>> Kf = std::make_shared<DevBuf<float>>();
+    bool k_shared = false;     // K and Kf came from another stack (not counted in this one's device bytes)
 };
 
 // Scratch of one batched product / preconditioner application (grown to the largest batch used).
@@ -69,6 +71,8 @@ struct Work {
     void ensure(const gt_stack_s* st, int nr, int kt = 0);
     // back to nr right-hand sides when the buffers hold more (set-up work that took a larger batch)
     void shrink(const gt_stack_s* st, int nr);
+    // back to what a new stack holds (one right-hand side, its own branches' Woodbury width)
+    void reset(const gt_stack_s* st);
     int64_t bytes() const {
         return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes() + z2.bytes() +
                          u.bytes());
@@ -124,6 +128,7 @@ struct gt_stack_s {
     gt::DevBuf<int> a_keep, b_keep, adj_ptr, adj;
     std::vector<int> h_a_keep, h_b_keep;     // host copies (Woodbury columns, the factor's pattern)
     gt::Work work;
+    int sessions = 0;                         // sessions open on this stack (its scratch is theirs while any is)
     gt::DevBuf<double> W;                     // Woodbury: B^-1 U, k columns of n_keep (column-major)
     bool wb_implicit = false;                 // W not stored: B^-1 (U t) by a second Schur solve per product
     gt::DevBuf<double> G;                     // implicit form: U^T B^-1 U (k x k, row-major)

@@ -348,6 +348,51 @@ class PlaneStack:
         new._ops = OrderedDict()
         return new
 
+    def with_layers(self, layers: list, branches: Optional[list] = None) -> "PlaneStack":
+        """A stack of another copper topology on this stack's grid (a moved slot, a changed plane): the same sheets,
+        heights and thicknesses with new masks, and this stack's branches (or `branches`) where both ends still have
+        copper. Its numbering and factor are its own (the reference builds a new model for such a what-if); the
+        inductance kernels, which depend only on the grid and the sheet gaps (peec.py:3288-3290's cache key), are
+        this stack's resident spectra, shared on the device."""
+        if len(layers) != self.N or any((l.grid.nx, l.grid.ny, l.grid.d) != (self.layers[0].grid.nx,
+                                                                         self.layers[0].grid.ny,
+                                                                         self.layers[0].grid.d) for l in layers):
+            raise ValueError("with_layers keeps the stack's grid and sheet count")
+        if any(a.z != b.z or a.thickness != b.thickness for a, b in zip(layers, self.layers)):
+            raise ValueError("with_layers keeps each layer's height and thickness (the kernels' gaps)")
+        new = PlaneStack(layers, self.branches if branches is None else branches, sigma=self.sigma,
+                         precond=self.precond_mode, device=self.device, setup=self.setup_mode,
+                         device_fronts=self.device_fronts, schur_leaf=self.schur_leaf, woodbury=self.woodbury,
+                         schur_ordering=self.schur_ordering)
+        if self.setup_mode == "device":
+            donor = self if self._base is None else self._base
+            new._kernel_donor = donor.__dict__.get("_kernel_donor") or donor
+        return new
+
+    def plan_bytes(self) -> tuple:
+        """(bytes held, bytes at the peak of its set-up) this stack's device copy will take, before it is made: a
+        what-if derived from a resident base, exactly (gt_stack_derive_bytes); any other stack, its kernel donor's
+        or its own resident bytes if it has one, else an estimate from its unknowns (0 when already resident)."""
+        if self._handle is not None:
+            return 0, 0
+        if self._base is not None and self._base._handle is not None:
+            nb0 = self._base.nb
+            ea = np.ascontiguousarray(self.a_keep[nb0:], np.int32)
+            eb = np.ascontiguousarray(self.b_keep[nb0:], np.int32)
+            n = len(ea)
+            held, peak = C.c_int64(), C.c_int64()
+            _lib.check(_lib.lib().gt_stack_derive_bytes(
+                self._base.handle, n, _lib.ptr(ea if n else np.zeros(1, np.int32), _lib.i32p),
+                _lib.ptr(eb if n else np.zeros(1, np.int32), _lib.i32p), C.byref(held), C.byref(peak)),
+                "gt_stack_derive_bytes")
+            return int(held.value), int(peak.value)
+        donor = self.__dict__.get("_kernel_donor")
+        if donor is not None and donor._handle is not None:
+            b = donor.device_bytes()            # the same grid: a like topology's layout, factor and scratch
+            return b, b
+        b = int(2000 * (self.nb + self.n_keep))
+        return b, b
+
     def map_state(self, x: Optional[np.ndarray], stack: "PlaneStack") -> Optional[np.ndarray]:
         """A solution vector of `stack` in this stack's unknowns (a warm start): the same stack, or this stack's
         base, whose added branches start at zero current (peec.py:1911-1923)."""
@@ -388,7 +433,12 @@ class PlaneStack:
         d.px[0], d.px[1] = self.Px
         d.py[0], d.py[1] = self.Py
         h = C.c_void_p()
-        if self.setup_mode == "device":
+        donor = self.__dict__.get("_kernel_donor")
+        if donor is not None and np.array_equal(donor._dz, self._dz) and donor.d == self.d:
+            _lib.check(L.gt_stack_create_like(self.ctx.handle, C.byref(d), donor.handle, C.byref(h)),
+                       "gt_stack_create_like")
+            del self._kernel_donor              # the spectra are shared on the device now
+        elif self.setup_mode == "device":
             kd = _lib.KernelDesc(G.key(self.d), G.MU0 / (4 * math.pi), _lib.ptr(self._dz), _lib.ptr(self._near))
             _lib.check(L.gt_stack_create_gen(self.ctx.handle, C.byref(d), C.byref(kd), C.byref(h)),
                        "gt_stack_create_gen")
@@ -463,6 +513,20 @@ class PlaneStack:
                                               _lib.ptr(Gm), C.byref(h)), "gt_stack_derive")
         self.G = Gm[: self.nbr, : self.nbr].copy()
         self._handle = h
+
+    def release(self):
+        """Give the device copy back (its operators first), keeping the host topology: a finished what-if's
+        solutions still unpack. The next device use makes it again."""
+        if self._handle is None:
+            return
+        L = _lib.lib()
+        for op in self._ops.values():
+            if op.handle:
+                L.gt_op_destroy(op.handle)
+                op.handle = None
+        self._ops.clear()
+        L.gt_stack_destroy(self._handle)
+        self._handle = None
 
     def __del__(self):
         try:
@@ -560,44 +624,43 @@ class PlaneStack:
                                                                    device=f"cuda:{self.device}")
         return ts
 
-    def warm_start_dev(self, op: Operator, rhs, guess: list):
+    def warm_start_dev(self, op: Operator, rhs, guess: list, host: Optional["PlaneStack"] = None):
         """(x0, r0) on device vectors (torch complex128, their work finished; the results ordered on the library's
         stream): x0 = G c with c minimising |K G c - rhs| over the guesses, r0 = rhs - K x0 the residual left to
-        GMRES, taken when it removes at least a tenth of |rhs|; else (None, rhs) (peec.py:2568-2576). The products
-        K g and the m x m normal equations stay on the device; c is solved on the host."""
+        GMRES, taken when it removes at least a tenth of |rhs|; else (None, rhs) (peec.py:2568-2576). As the
+        reference, c is the least-squares solution of K G c = rhs itself (a QR factorisation of the n x m KG on the
+        device, not its normal equations), and the acceptance is on the norm of the residual formed on the device.
+        host: the stack whose scratch runs the products K g (this stack's base: then a what-if stack needs none of
+        its own)."""
         import torch
         n = self.nb + self.n_keep
         Gs = [g for g in guess if g is not None and g.numel() == n]
         if not Gs:
             return None, rhs
+        L = _lib.lib()
         with torch.cuda.stream(self.torch_stream()):
-            KG = torch.empty((len(Gs), n), dtype=torch.complex128, device=rhs.device)
-            for i, g in enumerate(Gs):
-                _lib.check(_lib.lib().gt_matvec(op.handle, C.c_void_p(g.data_ptr()), C.c_void_p(KG[i].data_ptr())),
-                           "gt_matvec")
-            m = len(Gs)
-            # one read back: the normal equations (KG^H KG) c = KG^H rhs, |rhs|^2 and each guess's |g|^2
-            dots = [torch.vdot(KG[i], KG[j]) for i in range(m) for j in range(m)]
-            dots += [torch.vdot(KG[i], rhs) for i in range(m)]
-            dots += [torch.vdot(rhs, rhs)] + [torch.vdot(g, g) for g in Gs]
-            v = torch.stack(dots).cpu().numpy()
-        A, y = v[: m * m].reshape(m, m), v[m * m: m * m + m]
-        bb, gg = float(v[m * m + m].real), v[m * m + m + 1:].real
-        keep = [i for i in range(m) if gg[i] != 0]          # all-zero guesses take no part
-        if not keep:
-            return None, rhs
-        A, y = A[np.ix_(keep, keep)], y[keep]
-        c, *_ = np.linalg.lstsq(A, y, rcond=None)
-        # |rhs - KG c|^2 = |rhs|^2 - 2 Re(c^H KG^H rhs) + c^H (KG^H KG) c
-        r2 = bb - 2 * float(np.real(np.vdot(c, y))) + float(np.real(np.vdot(c, A @ c)))
-        if not math.sqrt(max(r2, 0.0)) < 0.9 * math.sqrt(bb):
-            return None, rhs
-        with torch.cuda.stream(self.torch_stream()):
-            r = rhs.clone()
+            keep = [g for g in Gs if bool(torch.any(g != 0))]     # all-zero guesses take no part
+            if not keep:
+                return None, rhs
+            KG = torch.empty((n, len(keep)), dtype=torch.complex128, device=rhs.device)
+            col = torch.empty(n, dtype=torch.complex128, device=rhs.device)
+            for i, g in enumerate(keep):
+                if host is not None:
+                    _lib.check(L.gt_matvec_on(host.handle, op.handle, C.c_void_p(g.data_ptr()),
+                                              C.c_void_p(col.data_ptr())), "gt_matvec_on")
+                else:
+                    _lib.check(L.gt_matvec(op.handle, C.c_void_p(g.data_ptr()), C.c_void_p(col.data_ptr())),
+                               "gt_matvec")
+                KG[:, i] = col
+            Q, R = torch.linalg.qr(KG)
+            c = torch.linalg.solve_triangular(R, (Q.conj().T @ rhs).unsqueeze(1), upper=True).squeeze(1)
+            r = rhs - KG @ c
+            ok = bool(torch.linalg.vector_norm(r) < 0.9 * torch.linalg.vector_norm(rhs))
+            if not ok:
+                return None, rhs
             x0 = torch.zeros_like(rhs)
-            for ci, i in zip(c, keep):
-                r.sub_(KG[i], alpha=complex(ci))
-                x0.add_(Gs[i], alpha=complex(ci))
+            for i, g in enumerate(keep):
+                x0.add_(g, alpha=complex(c[i].item()))
         return x0, r
 
     def map_state_dev(self, x, stack: "PlaneStack"):

@@ -157,6 +157,11 @@ class Admission:
     def shrink(self, need: float):
         self.used -= need
 
+    def true_up(self, planned: float, actual: float):
+        """A claim admitted by its plan, now of its exact bytes (a what-if stack once made)."""
+        self.used += actual - planned
+        self.used_peak = max(self.used_peak, self.used)
+
 
 @dataclass
 class Request:
@@ -167,6 +172,7 @@ class Request:
     route: object                    # .trace, .driver, .receiver (a boards.Route), as RouteExcitation takes them
     fs: list
     extra: list = field(default_factory=list)
+    layers: Optional[list] = None    # the board's layers with new copper (a moved slot): a topology of its own
     tol: float = 1e-8
     max_cycles: int = 8
     warm: bool = True                # a what-if starts from the board's solution at f when the server holds one
@@ -176,7 +182,11 @@ class Request:
     done_at: float = 0.0
     solutions: list = field(default_factory=list)
     error: Optional[str] = None
-    _hold: float = 0.0               # device bytes held for its what-if stack while it runs in its board's session
+    _hold: float = 0.0               # device bytes its admission holds: planned before its stack is made, then the
+                                     # what-if stack's own while it runs in its board's session
+    _stack: object = None            # its what-if stack, planned (not yet on the device) at admission
+    _grow: bool = False              # _hold is a growth of its board's session claim (else a claim of its own)
+    _attached: bool = False          # admitted into its board's session, its solves not yet queued there
 
     @property
     def latency(self) -> float:
@@ -203,6 +213,7 @@ class _Served:
         self.need = need
         self.session: Optional[Session] = None
         self.whatif = False
+        self.attached = 0                      # what-ifs admitted into this session, not yet queued in it
         self.waiting: deque = deque()
         self.slots: dict = {}
         self.last_used = 0.0
@@ -300,8 +311,54 @@ class Server:
 
     # ---------------------------------------------------------------- the loop
     def _stack_for(self, req: Request) -> PlaneStack:
+        """The request's stack, not yet on the device: its board; a what-if derived from it (added branches); or,
+        for new copper (a moved slot), a stack of its own on the board's grid sharing its kernel spectra."""
         base = self.boards[req.board]
+        if req.layers is not None:
+            return base.with_layers(req.layers, base.branches + list(req.extra))
         return base.with_branches(req.extra) if req.extra else base
+
+    def _plan(self, req: Request) -> tuple:
+        """The device bytes to admit a request by before anything of it is made, as (a claim of its own, a growth
+        of its board's session claim): nothing for the board itself; for a what-if derived from the board that joins
+        the board's session, the exact set-up peak of its stack (gt_stack_derive_bytes) as a growth; one with a
+        session of its own, that peak and the session as a claim; for a stack of its own (new copper, or vias
+        joining islands) the board's resident bytes and a session of the request's size, its planned peer."""
+        board = self.boards[req.board]
+        session = lambda: session_plan_bytes(board, min(self.capacity, max(1, len(req.fs))), self.restart,  # noqa: E731
+                                             self.precision, 0)
+        if req.layers is None and not req.extra:
+            req._stack = board
+            return 0.0, 0.0
+        if req.layers is None:
+            st = board.with_branches(req.extra)
+            req._stack = st
+            if st is board:
+                return 0.0, 0.0
+            if st._base is board:
+                _, peak = st.plan_bytes()
+                if self._host(req, st) is board:
+                    return 0.0, float(peak)
+                return float(peak) + session(), 0.0
+        else:
+            req._stack = None                   # new copper: the stack is built on the worker thread
+        return float(board.device_bytes() + session()), 0.0
+
+    def _admit_request(self, req: Request) -> bool:
+        """Admit a request by its plan (nothing of it is made before): a claim of its own, or a growth of its
+        board's session claim (the session opened first, and kept open while the request is attached)."""
+        claim, grow = self._plan(req)
+        if claim > 0:
+            if not self._hold(claim):
+                return False
+            req._hold, req._grow = claim, False
+        if grow > 0:
+            sv = self._admit_served(self.boards[req.board], len(req.fs))
+            if sv is None or not self.admission.try_grow(grow):
+                return False
+            sv.attached += 1
+            req._hold, req._grow, req._attached = grow, True, True
+        return True
 
     def excitation(self, board: str, route) -> RouteExcitation:
         """The route's excitation on the board, its strip coupling computed once per board and route."""
@@ -313,9 +370,16 @@ class Server:
         return hit[1]
 
     def _excitation(self, req: Request, stack: PlaneStack) -> RouteExcitation:
-        """The request's excitation on its stack (moved onto a what-if stack by RouteExcitation.on)."""
+        """The request's excitation on its stack: the board's; moved onto a what-if derived from it by
+        RouteExcitation.on (the strips' coupling reused); computed afresh on a stack of other copper."""
+        board = self.boards[req.board]
         ex = self.excitation(req.board, req.route)
-        return ex if stack is self.boards[req.board] else ex.on(stack)
+        if stack is board:
+            return ex
+        if getattr(stack, "_base", None) is board:
+            return ex.on(stack)
+        r = req.route
+        return RouteExcitation(stack, r.trace, r.driver, r.receiver)
 
     def _host(self, req: Request, stack: PlaneStack) -> PlaneStack:
         """The stack whose session runs the request's solves: its board's, for the board itself and for a what-if
@@ -330,16 +394,17 @@ class Server:
     def _hold(self, need: float) -> bool:
         """Admission of `need` bytes, closing idle sessions (oldest first) to make room."""
         while not self.admission.try_hold(need):
-            idle = [k for k, s in self._served.items() if not s.slots and not s.waiting]
+            idle = [k for k, s in self._served.items() if not s.slots and not s.waiting and not s.attached]
             if not idle:
                 return False
             self._close(idle[0])
         return True
 
-    def _admit_served(self, stack: PlaneStack, n_solves: int) -> Optional[_Served]:
+    def _admit_served(self, stack: PlaneStack, n_solves: int, claim: float = 0.0) -> Optional[_Served]:
         """The stack's session, made when its bytes are admitted (None: not now). A board's session has every
         slot and room for what-ifs; a what-if stack's own, made for one request, as many slots as that request's
-        solves."""
+        solves: its bytes are the request's claim (admitted as planned before the stack was made), trued up to the
+        session's and the stack's exact bytes."""
         sv = self._served.get(id(stack))
         if sv is not None:
             return sv
@@ -348,8 +413,9 @@ class Server:
         extra = 0 if whatif else self.max_extra
         need = session_plan_bytes(stack, cap, self.restart, self.precision, extra)
         if whatif:
-            need += stack.device_bytes()           # the what-if stack's own incidence and Woodbury columns
-        if not self._hold(need):
+            need += stack.device_bytes()           # the what-if stack's own layout, incidence, factor, columns
+            self.admission.true_up(claim, need)    # the request's claim becomes the session's
+        elif not self._hold(need):
             return None
         sv = _Served(stack, need)
         sv.whatif = whatif
@@ -361,15 +427,19 @@ class Server:
         sv = self._served.pop(key)
         if sv.session is not None:
             sv.session.close()
+        if sv.whatif:
+            sv.stack.release()                 # a what-if stack of its own: its device copy goes with its session
         self.admission.release(sv.need)
 
     def _prepare(self, req: Request):
-        """The request's solves (run on the worker thread): its stack (a what-if derived from the board), the
-        right-hand sides of RouteExcitation.solve, the operators, and for a what-if the warm start of
-        PlaneStack.solve from the board's solution at f (peec.py:2568-2576). (stack, solves, warm starts taken)."""
+        """The request's solves (run on the worker thread, after its admission): its stack (a what-if derived from
+        the board, or a topology of its own), the right-hand sides of RouteExcitation.solve, the operators, and for
+        a what-if derived from the board the warm start of PlaneStack.solve from the board's solution at f
+        (peec.py:2568-2576), its products in the board's scratch. (stack, solves, warm starts taken)."""
         import torch
         board = self.boards[req.board]
-        stack = self._stack_for(req)
+        stack = req._stack if req._stack is not None else self._stack_for(req)
+        req._stack = stack
         stack.handle
         whatif = stack is not board
         ex = self._excitation(req, stack)
@@ -380,17 +450,30 @@ class Server:
             op = stack.operator(f)
             b = torch.from_numpy(rhs).to(self.dev)
             x0 = None
-            if whatif and req.warm:
+            if whatif and req.warm and getattr(stack, "_base", None) is board:
                 xb = self._board_x.get((req.board, id(req.route), PlaneStack.freq_key(f)))
                 g = stack.map_state_dev(xb, board)
                 if g is not None:
                     torch.cuda.synchronize(self.device)
-                    x0, b = stack.warm_start_dev(op, b, [g])
+                    x0, b = stack.warm_start_dev(op, b, [g], host=board)
                     warm += x0 is not None
             x = torch.empty_like(b)
             solves.append(_Solve(req, k, f, ex, op, b, x, float(np.linalg.norm(rhs)), x0, stack))
         torch.cuda.synchronize(self.device)         # the right-hand sides are on the device
         return stack, solves, warm
+
+    def _release_request(self, req: Request):
+        if req._attached:                      # a failed what-if lets its board's session close again
+            sv = self._served.get(id(self.boards[req.board]))
+            if sv is not None:
+                sv.attached -= 1
+            req._attached = False
+        if req._hold:
+            if req._grow:
+                self.admission.shrink(req._hold)
+            else:
+                self.admission.release(req._hold)
+            req._hold = 0.0
 
     def _fail(self, req: Request, e: Exception):
         req.error = str(e)
@@ -404,15 +487,21 @@ class Server:
             req, stack, solves, warm = item = self._prepared.popleft()
             try:
                 host = self._host(req, stack)
-                sv = self._admit_served(host, len(req.fs))
-                if sv is not None and host is not stack:
-                    # a what-if in its board's session: its own incidence and Woodbury columns, held while it runs
-                    hold = stack.device_bytes()
-                    if self.admission.try_grow(hold):
-                        req._hold = hold
-                    else:
-                        sv = None
+                if host is stack and stack is not self.boards[req.board]:
+                    sv = self._admit_served(host, len(req.fs), req._hold)   # the claim moves to its session
+                    req._hold = 0.0
+                else:
+                    sv = self._admit_served(host, len(req.fs))
+                    if sv is not None and host is not stack and req._hold:
+                        # a what-if in its board's session: its growth trued up to the stack's own bytes (incidence
+                        # and Woodbury columns), held while it runs
+                        own = float(stack.device_bytes())
+                        self.admission.true_up(req._hold, own)
+                        req._hold = own
+                        sv.attached -= 1
+                        req._attached = False
             except (SolverError, ValueError, NotImplementedError) as e:
+                self._release_request(req)
                 self._fail(req, e)
                 continue
             if sv is None:
@@ -427,8 +516,20 @@ class Server:
     def turn(self) -> list:
         """Hand new requests to the set-up worker, admit the prepared ones, step every session with work once,
         collect. The requests completed in this turn."""
+        refused = deque()
         while self._pending:
             req = self._pending.popleft()
+            # admitted by its planned bytes before any of its device memory is made; refused ones wait holding none
+            try:
+                ok = self._admit_request(req)
+            except (SolverError, ValueError, NotImplementedError) as e:
+                self._release_request(req)
+                self._fail(req, e)
+                continue
+            if not ok:
+                self._release_request(req)
+                refused.append(req)
+                continue
             if self._worker is not None:
                 self._prepping.append((req, self._worker.submit(self._prepare, req)))
             else:
@@ -438,10 +539,12 @@ class Server:
                 except Exception as e:          # noqa: BLE001 - handed on as the future's exception
                     fut.set_exception(e)
                 self._prepping.append((req, fut))
+        self._pending = refused
         while self._prepping and self._prepping[0][1].done():
             req, fut = self._prepping.popleft()
             e = fut.exception()
             if e is not None:
+                self._release_request(req)
                 if not isinstance(e, (SolverError, ValueError, NotImplementedError)):
                     raise e
                 self._fail(req, e)
@@ -502,9 +605,10 @@ class Server:
             self._open[id(req)] -= 1
             if self._open[id(req)] == 0:
                 del self._open[id(req)]
-                if req._hold:
-                    self.admission.shrink(req._hold)
-                    req._hold = 0.0
+                self._release_request(req)
+                st = req._stack
+                if st is not None and st is not self.boards[req.board] and self._served.get(id(st)) is None:
+                    st.release()                # a what-if derived from its board: its incidence and columns
                 req.done_at = time.perf_counter()
                 self.completed.append(req)
                 done.append(req)

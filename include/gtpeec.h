@@ -228,6 +228,18 @@ int gt_stack_woodbury(const gt_stack* stack, int32_t* form, int32_t* k);
  * copper the base keeps apart changes the reference nodes; such a what-if needs a stack of its own. */
 int gt_stack_derive(gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra, const int32_t* b_keep_extra,
                     double* G_out, gt_stack** out);
+/* The device bytes gt_stack_derive with these arguments would hold (*held: the stack's incidence and Woodbury
+ * columns; it keeps no scratch of its own, its solves running in its base's batches) and need at its peak (*peak:
+ * with the added columns' set-up solves), without making it: what a caller admits before the stack exists
+ * (simulate.py:3930-3961 admits a task by its planned memory before it starts). (No counterpart in the reference.) */
+int gt_stack_derive_bytes(const gt_stack* base, int32_t n_extra, const int32_t* a_keep_extra,
+                          const int32_t* b_keep_extra, int64_t* held, int64_t* peak);
+/* A stack of another copper topology on the same grid as `donor` (the same sheets, pitch and face gaps, so the same
+ * inductance kernels): peec.py:3288-3290 reuses a board's kernels by their (nx, ny, d, dz) key for any model on that
+ * grid; here the new stack shares the donor's resident kernel spectra (held while either stack lives) and makes its
+ * own strip layout, incidence and (gt_schur_factor) factor. A topology-changing what-if (a moved slot). desc as for
+ * gt_stack_create, without kernels (kx, ky ignored); GT_ERR_ARG when the grid or FFT sizes differ. */
+int gt_stack_create_like(gt_ctx* ctx, const gt_stack_desc* desc, const gt_stack* donor, gt_stack** out);
 /* Statistics of the factor: entries of L, supernodes, tree levels, the largest supernode's columns. */
 int gt_schur_stats(const gt_stack* stack, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels, int32_t* max_cols);
 /* B^-1 b on the host with the same factor (a check of the factorisation that needs no device). b, x: n_keep
@@ -268,6 +280,9 @@ void gt_op_destroy(gt_op* op);
 
 /* The system product y = K x = [Z Ib - Ar V ; Ar^T Ib] (peec.py:3043-3062 _HostMath.matvec, gpu.py:551-564). */
 int gt_matvec(gt_op* op, const void* x_dev, void* y_dev);
+/* gt_matvec run in host's scratch: op belongs to host or to a what-if stack derived from it, which then needs no
+ * scratch of its own (a server's warm-start products; peec.py:2568-2576). (No counterpart in the reference.) */
+int gt_matvec_on(gt_stack* host, gt_op* op, const void* x_dev, void* y_dev);
 /* The preconditioner z = M^-1 r by block elimination (peec.py:3064-3083 _HostMath.precond, gpu.py:566-573),
  * with the Schur solve on the device. */
 int gt_precond(gt_op* op, const void* r_dev, void* z_dev);

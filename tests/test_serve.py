@@ -311,3 +311,121 @@ def test_whatif_requests_start_from_the_primed_board_solutions():
             Kx = d.matvec(d.operator(f), got.x)
             assert np.linalg.norm(rhs - Kx) / np.linalg.norm(rhs) < 2e-10
     srv.close()
+
+
+def _mix():
+    """Resident boards of 1, 2 and 4 layers (the served stream's kinds at sizes the oracle solves in seconds)."""
+    return {"one": boards.single_plane(n=16),
+            "two": boards.scaled("two_layer", 24, n_vias=4, n_freqs=2),
+            "four": boards.scaled("four_layer", 24, n_nets=2, n_freqs=2, n_stitch=3)}
+
+
+def _mix_requests(bds, tol):
+    """Per board: the board's own solve, a moved-slot what-if (a topology of its own), and on the multi-layer
+    boards an added-via what-if; seeded."""
+    rng = np.random.default_rng(11)
+    reqs = []
+    for name, b in bds.items():
+        r = b.routes[0]
+        fs = [float(f) for f in b.freqs]
+        reqs.append(Request(name, r, fs, tol=tol, warm=False, tag=(name, "board")))
+        layers, _ = boards.moved_slot(b, rng, protect=[r.driver, r.receiver])
+        reqs.append(Request(name, r, fs[:1], layers=layers, tol=tol, warm=False, tag=(name, "slot")))
+        if len(b.layers) > 1:
+            reqs.append(Request(name, r, fs, extra=_extra_vias(b, 2, 300 + len(reqs)), tol=tol, warm=False,
+                                tag=(name, "via")))
+    return reqs
+
+
+def _mix_alone(b, req):
+    """The request solved alone on a fresh stack of its own board (cold)."""
+    st = peec.PlaneStack(b.layers, b.branches)
+    r = req.route
+    if req.layers is not None:
+        st = st.with_layers(req.layers, b.branches + list(req.extra))
+        ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+    else:
+        ex = peec.RouteExcitation(st, r.trace, r.driver, r.receiver)
+        if req.extra:
+            ex = ex.on(st.with_branches(req.extra))
+    return [ex.solve(f, tol=req.tol) for f in req.fs]
+
+
+def _mix_oracle(b, req, f):
+    from oracle import peec as O
+    r = req.route
+    layers = req.layers if req.layers is not None else b.layers
+    st = O.PlaneStack(layers, b.branches + list(req.extra))
+    return O.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(f, tol=1e-13)
+
+
+@pytest.mark.gpu
+def test_served_mix_of_boards_vias_and_moved_slots_equals_alone_and_the_oracle():
+    """A stream over resident 1-, 2- and 4-layer boards with added-via what-ifs (in their board's session) and
+    moved-slot what-ifs (stacks of their own sharing the board's kernel spectra), submitted at once: each served
+    solution equals the same solve alone bit for bit, and the oracle's at 1e-8 (served at tol 1e-11)."""
+    bds = _mix()
+    srv = Server(capacity=8)
+    for name, b in bds.items():
+        srv.add_board(name, peec.PlaneStack(b.layers, b.branches), routes=b.routes[:1])
+    reqs = _mix_requests(bds, 1e-11)
+    for q in reqs:
+        srv.submit(q)
+    srv.drain(limit_s=300)
+    st = srv.stats()
+    assert st["requests"] == len(reqs) and st["failed"] == 0, [q.error for q in reqs if q.error]
+    for q in reqs:
+        b = bds[q.board]
+        for f, got, one in zip(q.fs, q.solutions, _mix_alone(b, q)):
+            assert np.array_equal(got.x, one.x) and got.iterations == one.iterations, q.tag
+            ref = _mix_oracle(b, q, f)
+            cur = np.concatenate([got.I_x.ravel(), got.I_y.ravel(), got.I_branch])
+            cref = np.concatenate([ref.I_x.ravel(), ref.I_y.ravel(), ref.I_branch])
+            assert np.linalg.norm(cur - cref) <= 1e-8 * np.linalg.norm(cref), q.tag
+            assert abs(got.Z_loop - ref.Z_loop) <= 1e-8 * abs(ref.Z_loop), q.tag
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_whatif_bytes_are_planned_exactly_and_admitted_before_they_are_made():
+    """A derived what-if's planned bytes are the bytes it then holds; with a budget for the board and one claim, a
+    queue of what-ifs (added vias and a moved slot) is served with no request's stack made before its admission, and
+    the bytes the library reports held never exceeding what the admission has granted."""
+    b = _board()
+    st = peec.PlaneStack(b.layers, b.branches)
+    st.handle
+    d = st.with_branches(_extra_vias(b, 2, 41))
+    held, peak = d.plan_bytes()
+    d.handle
+    assert d.device_bytes() == held and peak >= held
+    srv = Server(capacity=8)
+    srv.add_board("b", st, routes=b.routes[:1])
+    srv.admission.budget = srv.admission.used + session_plan_bytes(st, 8, None, "fp64", srv.max_extra) + 1
+    r = b.routes[0]
+    rng = np.random.default_rng(3)
+    reqs = [Request("b", r, [float(b.freqs[i % 2])], extra=_extra_vias(b, 1, 50 + i), tol=1e-10, warm=False)
+            for i in range(4)]
+    reqs.append(Request("b", r, [float(b.freqs[0])], tol=1e-10, warm=False,
+                        layers=boards.moved_slot(b, rng, protect=[r.driver, r.receiver])[0]))
+    for q in reqs:
+        srv.submit(q)
+    for _ in range(2000):
+        if not srv.busy():
+            break
+        srv.turn()
+        for q in srv._pending:              # refused: nothing of it on the device, nothing held
+            assert q._hold == 0 and (q._stack is None or q._stack is st or q._stack._handle is None)
+        parts = {"board": st.device_bytes()}
+        for sv in srv._served.values():
+            parts[f"session{id(sv) % 997}"] = sv.session.device_bytes() + (sv.stack.device_bytes() if sv.whatif else 0)
+        for i, q in enumerate(reqs):
+            if (q._stack is not None and q._stack is not st and q._stack._handle is not None
+                    and srv._served.get(id(q._stack)) is None):
+                parts[f"req{i}"] = (q._stack.device_bytes(), q._hold, q._grow)
+        live = sum(v[0] if isinstance(v, tuple) else v for v in parts.values())
+        assert live <= srv.admission.used + 1, (parts, srv.admission.used, srv.admission.held)
+    # a finished what-if's stack gives its device memory back (its solutions keep the host topology)
+    assert all(q._stack is None or q._stack is st or q._stack._handle is None for q in reqs)
+    assert not srv.busy()
+    assert all(q.error is None for q in reqs), [q.error for q in reqs]
+    srv.close()
