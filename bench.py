@@ -228,10 +228,10 @@ SERVE_SECONDS = 15.0            # served stream: the window the arrivals fall in
 
 
 def serve_boards():
-    """The served stream's resident boards (name, Board, stack): 1, 4 and 2 layers at 64^2, 128^2 and 256^2 cells."""
+    """The served stream's resident boards (name, Board, stack): 1, 4 and 2 layers at 128^2, 128^2 and 256^2 cells."""
     from groundtruth_amd import boards, peec
     out = []
-    for name, b in (("single_plane_64", boards.single_plane()),
+    for name, b in (("single_plane_128", boards.scaled("single_plane", 128)),
                     ("four_layer_128", boards.scaled("four_layer", 128, n_nets=4)),
                     ("two_layer_256", boards.two_layer())):
         out.append((name, b, peec.PlaneStack(b.layers, b.branches, device=0)))
@@ -240,8 +240,10 @@ def serve_boards():
 
 def serve_requests(bds, rate: float, seconds: float, seed: int = 1):
     """A seeded Poisson stream: arrival times at `rate` per second over `seconds`; each request a board drawn
-    uniformly, one of its nets, 1-4 of its frequencies, and for half of the requests on boards of two or more layers
-    a what-if edit of 1-2 added stitching vias at copper cells."""
+    uniformly, one of its nets and 1-4 of its frequencies; half of the requests are what-if edits of the resident
+    board: on boards of two or more layers half of those add 1-2 stitching vias at copper cells (a what-if stack on
+    the board's factor), the rest (and every edit of a one-layer board) move the board's slot (a topology of its
+    own on the board's kernel spectra: its own numbering, factor and session)."""
     from groundtruth_amd import boards
     from groundtruth_amd.serve import Request
     rng = np.random.default_rng(seed)
@@ -251,14 +253,18 @@ def serve_requests(bds, rate: float, seconds: float, seed: int = 1):
         if t >= seconds:
             return out
         name, b, st = bds[int(rng.integers(len(bds)))]
+        route = b.routes[int(rng.integers(len(b.routes)))]
         fs = [float(f) for f in rng.choice(b.freqs, size=int(rng.integers(1, min(4, len(b.freqs)) + 1)),
                                            replace=False)]
-        extra = []
-        if len(b.layers) > 1 and rng.random() < 0.5:
-            while len(extra) < int(rng.integers(1, 3)):
-                i, j = (int(v) for v in rng.integers(1, st.nx - 1, 2))
-                extra += boards._stitch(b.layers, i, j, f"whatif{len(out)}_{len(extra)}")
-        out.append((t, Request(name, b.routes[int(rng.integers(len(b.routes)))], fs, extra=extra, tol=SERVE_TOL)))
+        extra, layers = [], None
+        if rng.random() < 0.5:
+            if len(b.layers) > 1 and rng.random() < 0.5:
+                while len(extra) < int(rng.integers(1, 3)):
+                    i, j = (int(v) for v in rng.integers(1, st.nx - 1, 2))
+                    extra += boards._stitch(b.layers, i, j, f"whatif{len(out)}_{len(extra)}")
+            else:
+                layers, _ = boards.moved_slot(b, rng, protect=[route.driver, route.receiver])
+        out.append((t, Request(name, route, fs, extra=extra, layers=layers, tol=SERVE_TOL)))
 
 
 def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
@@ -282,13 +288,16 @@ def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
     for name, b, st in bds:
         srv.prime(name, b.freqs, tol=SERVE_TOL)
     prime_s = time.perf_counter() - t_pr
-    # one what-if per multi-layer board (untimed): the first use of the warm start's device routines
+    # one what-if of each kind per board (untimed): the first use of the warm start's and the moved slot's routines
     from groundtruth_amd import boards as B
     from groundtruth_amd.serve import Request
+    rng = np.random.default_rng(99)
     for name, b, st in bds:
+        r = b.routes[0]
         if len(b.layers) > 1:
-            srv.submit(Request(name, b.routes[0], [b.freqs[0]], extra=B._stitch(b.layers, 3, 3, "warm-up"),
-                               tol=SERVE_TOL))
+            srv.submit(Request(name, r, [b.freqs[0]], extra=B._stitch(b.layers, 3, 3, "warm-up"), tol=SERVE_TOL))
+        srv.submit(Request(name, r, [b.freqs[0]], tol=SERVE_TOL,
+                           layers=B.moved_slot(b, rng, protect=[r.driver, r.receiver])[0]))
     srv.drain()
     srv.reset_stats()
     torch.cuda.synchronize()
@@ -324,15 +333,22 @@ def serve_stream(rate: float, seconds: float, batched: bool = True) -> dict:
             if key not in exs:
                 exs[key] = srv.excitation(req.board, req.route)
             ex = exs[key]
-            d = st.with_branches(req.extra) if req.extra else st
-            if req.extra:
-                ex = ex.on(d)
             try:
+                if req.layers is not None:      # a topology of its own: a new stack on the board's spectra, cold
+                    d = st.with_layers(req.layers, st.branches + list(req.extra))
+                    ex = peec.RouteExcitation(d, req.route.trace, req.route.driver, req.route.receiver)
+                else:
+                    d = st.with_branches(req.extra) if req.extra else st
+                    if req.extra:
+                        ex = ex.on(d)
                 for f in req.fs:
-                    xb = srv.board_solution(req.board, req.route, f) if req.extra else None
+                    xb = (srv.board_solution(req.board, req.route, f)
+                          if req.extra and req.layers is None else None)
                     ex.solve(f, tol=SERVE_TOL, guess=[d.map_state(xb, st)] if xb is not None else None)
                     solves += 1
                 lat.append(time.perf_counter() - (t0 + t_arr))
+                if d is not st:
+                    d.release()
             except peec.SolverError:
                 failed += 1
         extra = {}
@@ -722,12 +738,14 @@ def main():
         out["served"] = dict(served, rate_per_s=args.serve_rate, tol=SERVE_TOL,
                              p99_vs_one_at_a_time=plain["p99_ms"] / served["p99_ms"],
                              one_at_a_time=plain,
-                             stream="Poisson arrivals; boards single_plane 64^2, four_layer 4x128^2 (4 nets), "
+                             stream="Poisson arrivals; boards single_plane 128^2, four_layer 4x128^2 (4 nets), "
                                     "two_layer 2x256^2 drawn uniformly; 1 net x 1-4 frequencies per request; half "
-                                    "the requests on multi-layer boards add 1-2 stitching vias (a what-if stack on "
-                                    "the resident factor); plain FP64 GMRES to 1e-8; one_at_a_time: the same "
-                                    "stream solved in arrival order one solve at a time; what-ifs start from the "
-                                    "board solution at f (the boards primed before the stream, untimed)")
+                                    "the requests are what-ifs: on multi-layer boards half of those add 1-2 "
+                                    "stitching vias (a what-if stack on the resident factor, warm from the board "
+                                    "solution at f), the rest move the board's slot (a new topology on the board's "
+                                    "kernel spectra, cold); plain FP64 GMRES to 1e-8; one_at_a_time: the same "
+                                    "stream solved in arrival order one solve at a time (the boards primed before "
+                                    "the stream, untimed)")
         out["whatif_512"] = serve_whatif_512()
         out["served_backlog"] = serve_backlog()
     if not args.no_cpu_baseline:
