@@ -139,22 +139,18 @@ class Solution:
         if copper:
             raise NotImplementedError("copper-width fields need the rasterised copper widths and areas")
         d = d_mm * 1e-3
-        Ix, Iy = self.I_x[k], self.I_y[k]
-        nx, ny = self.V.shape[1:]
-        kx = np.zeros((nx, ny), complex)
-        ky = np.zeros((nx, ny), complex)
-        cx = np.zeros((nx, ny))
-        cy = np.zeros((nx, ny))
-        # a strip (i, j) along x touches the nodes (i, j) and (i + 1, j); along y, (i, j) and (i, j + 1)
-        kx[:-1, :] += Ix
-        cx[:-1, :] += Ix != 0
-        kx[1:, :] += Ix
-        cx[1:, :] += Ix != 0
-        ky[:, :-1] += Iy
-        cy[:, :-1] += Iy != 0
-        ky[:, 1:] += Iy
-        cy[:, 1:] += Iy != 0
-        return kx / np.maximum(cx, 1) / d, ky / np.maximum(cy, 1) / d
+
+        def mean_of_touching(I: np.ndarray, axis: int) -> np.ndarray:
+            # strip s along `axis` joins nodes s and s + 1: pad one empty strip at each end, so node n sees the
+            # strips n - 1 and n
+            pad = [(0, 0), (0, 0)]
+            pad[axis] = (1, 1)
+            P = np.pad(I, pad)
+            lo, hi = np.take(P, range(P.shape[axis] - 1), axis), np.take(P, range(1, P.shape[axis]), axis)
+            n = (lo != 0).astype(float) + (hi != 0)
+            return (lo + hi) / np.maximum(n, 1.0) / d
+
+        return mean_of_touching(self.I_x[k], 0), mean_of_touching(self.I_y[k], 1)
 
     def sheet_current(self, k: int, d_mm: float, copper: bool = False) -> np.ndarray:
         """|K| at the nodes of layer k, A/m per A of signal (peec.py:1201-1204)."""
@@ -162,36 +158,32 @@ class Solution:
         return np.sqrt(np.abs(kx) ** 2 + np.abs(ky) ** 2)
 
     def dominant_phase(self) -> float:
-        """The phase phi at which the strip currents are largest (peec.py:1206-1223): half the angle of the sum of
-        I^2 over every strip (the principal axis of the complex currents), turned by pi when Re(I e^-j phi) then runs
-        against Re(I); 0 when the sum is 0."""
-        cached = getattr(self, "_phase", None)
-        if cached is not None:
-            return cached
-        sq = complex(np.sum(self.I_x ** 2) + np.sum(self.I_y ** 2))
-        phi = 0.5 * math.atan2(sq.imag, sq.real) if abs(sq) > 0 else 0.0
-        rot = np.exp(-1j * phi)
-        corr = float(np.sum((self.I_x * rot).real * self.I_x.real) + np.sum((self.I_y * rot).real * self.I_y.real))
-        if corr < 0:
-            phi += math.pi
-        self._phase = phi
-        return phi
+        """The phase phi at which the strip currents are largest (peec.py:1206-1223). Over all strips z = I, the
+        in-phase energy sum Re(z e^-j phi)^2 = (sum |z|^2 + Re(e^-2j phi sum z^2)) / 2 peaks at phi = arg(sum z^2) / 2
+        (modulo pi); of the two ends, the one along which Re(z e^-j phi) follows Re(z) (the 1 A excitation is real).
+        0 when sum z^2 is 0. Computed once per solution."""
+        if getattr(self, "_phase", None) is None:
+            z = np.concatenate([np.ravel(self.I_x), np.ravel(self.I_y)])
+            q = complex(np.sum(z * z))
+            phi = 0.5 * math.atan2(q.imag, q.real) if q != 0 else 0.0
+            if np.dot((z * complex(math.cos(phi), -math.sin(phi))).real, z.real) < 0:
+                phi += math.pi
+            self._phase = phi
+        return self._phase
 
     def sheet_vectors(self, k: int, d_mm: float, copper: bool = False) -> tuple:
         """Real (Kx, Ky) at the nodes of layer k at the dominant phase (peec.py:1225-1230)."""
         kx, ky = self.node_field(k, d_mm, copper)
-        rot = np.exp(-1j * self.dominant_phase())
-        return (kx * rot).real, (ky * rot).real
+        turn = complex(math.cos(self.dominant_phase()), -math.sin(self.dominant_phase()))
+        return (kx * turn).real, (ky * turn).real
 
     def layer_share(self, masks: Optional[list] = None) -> np.ndarray:
         """Each layer's fraction of the summed strip-current magnitude (peec.py:1232-1245); masks: per layer a pair
         of boolean arrays (x strips, y strips) of the strips to count."""
-        per = []
-        for k in range(len(self.I_x)):
-            ix, iy = np.abs(self.I_x[k]), np.abs(self.I_y[k])
-            if masks is not None:
-                ix, iy = ix * masks[k][0], iy * masks[k][1]
-            per.append(ix.sum() + iy.sum())
-        per = np.array(per)
+        ax, ay = np.abs(self.I_x), np.abs(self.I_y)
+        if masks is not None:
+            ax = ax * np.stack([m[0] for m in masks])
+            ay = ay * np.stack([m[1] for m in masks])
+        per = ax.reshape(len(ax), -1).sum(axis=1) + ay.reshape(len(ay), -1).sum(axis=1)
         tot = per.sum()
         return per / tot if tot > 0 else per

@@ -21,7 +21,9 @@ The boards are this repository's seeded generator (groundtruth_amd/boards.py) at
 the board as arrays (copper masks, layer heights and thicknesses, branch ends with their R and L, trace
 filaments, injection nodes), so the tests rebuild exactly what the reference solved without running the
 generator. Outputs per solve: I_x, I_y, V, I_branch, Z_loop, m, iterations; per route: psi (strips and branches),
-L_trace and R_trace at each frequency. Nothing at test time reads the reference.
+L_trace and R_trace at each frequency; for the first solve, the solution's post-processing (node fields, sheet
+currents, vectors at the dominant phase, the phase, layer shares with and without seeded masks). Nothing at test
+time reads the reference.
 """
 from __future__ import annotations
 
@@ -103,6 +105,27 @@ def reference_board(R, a: dict):
     return layers, branches, routes
 
 
+def post_masks(shape_x, shape_y, n_layers: int) -> list:
+    """Seeded per-layer (x strips, y strips) masks for layer_share: about half of each layer's strips."""
+    rng = np.random.default_rng(5)
+    return [(rng.random(shape_x) < 0.5, rng.random(shape_y) < 0.5) for _ in range(n_layers)]
+
+
+def post_outputs(out: dict, sol, d_mm: float):
+    """The reference's post-processing of `sol` (peec.py:1151-1245): every layer's node field and its vectors at
+    the dominant phase, the dominant phase, and the layer shares (all strips, and seeded masks)."""
+    n = sol.I_x.shape[0]
+    for k in range(n):
+        kx, ky = sol.node_field(k, d_mm)
+        out[f"nf_x_{k}"], out[f"nf_y_{k}"] = kx, ky
+        vx, vy = sol.sheet_vectors(k, d_mm)
+        out[f"sv_x_{k}"], out[f"sv_y_{k}"] = vx, vy
+        out[f"sc_{k}"] = sol.sheet_current(k, d_mm)
+    out["phase"] = np.float64(sol.dominant_phase())
+    out["share"] = sol.layer_share()
+    out["share_masked"] = sol.layer_share(post_masks(sol.I_x.shape[1:], sol.I_y.shape[1:], n))
+
+
 def solve_case(R, name: str, b, solves: list, note: str):
     """Solve each (route, f) of `solves` with the reference and write tests/golden/solve_<name>.npz."""
     a = board_arrays(b)
@@ -125,6 +148,8 @@ def solve_case(R, name: str, b, solves: list, note: str):
         out[f"m_{s}"] = np.asarray(sol.m)
         out[f"its_{s}"] = np.int64(sol.iterations)
         out[f"R_trace_{s}"] = np.float64(R._trace_resistance(trace, f, st.sigma, ex.trace_t))
+        if s == 0:
+            post_outputs(out, sol, float(a["pitch_mm"]))
         if r not in done_routes:
             done_routes.add(r)
             out[f"psi_{r}"] = np.asarray(ex.psi, float)

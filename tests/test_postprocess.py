@@ -4,8 +4,11 @@
 CPU: node_field against a loop over every node that averages the touching strips carrying current; the dominant phase
 against a brute-force search of the angle that maximises the strips' in-phase energy (and its sign rule); the layer
 shares sum to one; copper-width fields are refused (this library's strips are full cells).
+Golden: on the reference's own first solve of every fixture under tests/golden/, every one of these equals the
+reference's own post-processing of that solve to rounding.
 GPU: the device's (Kx, Ky) at every node of every layer (gt_node_field) equal the host's to rounding (numpy divides a
-complex by a real through the reciprocal, the device divides) on solved boards, with the same zero pattern.
+complex by a real through the reciprocal, the device divides) on solved boards, with the same zero pattern, and the
+reference's own node fields on its own solves.
 """
 import math
 
@@ -14,6 +17,8 @@ import pytest
 
 from groundtruth_amd import boards, peec
 from groundtruth_amd.problem import Solution
+
+import _golden
 
 
 def _random_solution(seed=3, N=2, nx=7, ny=5):
@@ -95,3 +100,56 @@ def test_device_node_field_equals_the_host(name):
         for dev, host in ((kx[k], hx), (ky[k], hy)):
             assert np.array_equal(dev == 0, host == 0), k
             assert np.max(np.abs(dev - host)) <= 2e-16 * np.max(np.abs(host)), k
+
+
+@pytest.mark.parametrize("name", _golden.NAMES)
+def test_post_processing_equals_the_reference_on_its_own_solution(name):
+    """The reference's own post-processing of its own first solve of each fixture (node fields and sheet currents
+    of every layer, vectors at the dominant phase, the phase, layer shares with and without seeded masks), against
+    this library's on the same currents."""
+    fx = _golden.load(name)
+    a = fx.data
+    s = Solution(float(a["solve_f"][0]), a["I_x_0"], a["I_y_0"], a["V_0"], a["I_branch_0"], complex(a["Z_0"]),
+                 a["m_0"], int(a["its_0"]))
+    d_mm = float(a["pitch_mm"])
+    assert abs(s.dominant_phase() - float(a["phase"])) <= 1e-15 * max(1.0, abs(float(a["phase"])))
+    for k in range(s.I_x.shape[0]):
+        kx, ky = s.node_field(k, d_mm)
+        for mine, ref in ((kx, a[f"nf_x_{k}"]), (ky, a[f"nf_y_{k}"])):
+            assert np.array_equal(mine == 0, ref == 0)
+            assert np.max(np.abs(mine - ref)) <= 1e-15 * np.max(np.abs(ref)), (name, k)
+        vx, vy = s.sheet_vectors(k, d_mm)
+        for mine, ref in ((vx, a[f"sv_x_{k}"]), (vy, a[f"sv_y_{k}"])):
+            assert np.max(np.abs(mine - ref)) <= 1e-14 * np.max(np.abs(ref)), (name, k)
+        sc = s.sheet_current(k, d_mm)
+        assert np.max(np.abs(sc - a[f"sc_{k}"])) <= 1e-15 * np.max(a[f"sc_{k}"]), (name, k)
+    assert np.max(np.abs(s.layer_share() - a["share"])) <= 1e-15
+    masks = _golden_masks(s)
+    assert np.max(np.abs(s.layer_share(masks) - a["share_masked"])) <= 1e-15
+
+
+def _golden_masks(s):
+    """The fixture generator's seeded layer_share masks (about half of each layer's strips)."""
+    rng = np.random.default_rng(5)
+    return [(rng.random(s.I_x.shape[1:]) < 0.5, rng.random(s.I_y.shape[1:]) < 0.5) for _ in range(s.I_x.shape[0])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["two_layer_48", "large_32"])
+def test_device_node_field_equals_the_reference_on_its_own_solution(name):
+    """gt_node_field on the reference's own solve, laid out in this library's unknown order, against the
+    reference's node fields."""
+    import torch
+    fx = _golden.load(name)
+    a = fx.data
+    st = peec.PlaneStack(fx.layers, fx.branches)
+    s = Solution(float(a["solve_f"][0]), a["I_x_0"], a["I_y_0"], a["V_0"], a["I_branch_0"], complex(a["Z_0"]),
+                 a["m_0"], int(a["its_0"]))
+    xd = torch.from_numpy(np.ascontiguousarray(s.state_for(st))).to("cuda:0")
+    d_mm = float(a["pitch_mm"])
+    kx, ky = st.node_field_dev(xd, d_mm)
+    kx, ky = kx.cpu().numpy(), ky.cpu().numpy()
+    for k in range(st.n_sheets):
+        for dev, ref in ((kx[k], a[f"nf_x_{k}"]), (ky[k], a[f"nf_y_{k}"])):
+            assert np.array_equal(dev == 0, ref == 0), k
+            assert np.max(np.abs(dev - ref)) <= 2e-16 * np.max(np.abs(ref)), k
