@@ -13,7 +13,12 @@ Each oracle function is checked against something that does not restate it:
   * gmres (peec.py:3093-3224) -> the reference's GMRES tests (tests/test_speed.py:18-46, 260-350) and numpy.linalg.solve;
   * the whole solve     -> a dense direct solve of the assembled system, and the reference's physics pins: image
                             theory (6 %), Wheeler's loop inductance (3 %), the loop area (1 %) (tests/test_solver.py:40-57);
-  * RouteExcitation's checks -> the reference's SolverError cases (tests/test_solver.py:85-88).
+  * RouteExcitation's checks -> the reference's SolverError cases (tests/test_solver.py:85-88);
+  * the Z_loop trace terms (peec.py:3227-3250), for the oracle and the product alike -> R_trace's DC limit
+                            L/(sigma w t) and its skin-depth limit L/(sigma w delta); L_trace of L- and U-shaped traces
+                            against Neumann's double integral by scipy's adaptive quadrature (perpendicular legs couple
+                            by exactly zero, antiparallel ones by minus the parallel mutual). The golden solve fixtures
+                            hold both to the reference's own values as well.
 """
 import math
 from pathlib import Path
@@ -305,3 +310,44 @@ def test_solves_are_bit_reproducible():
         r = b.routes[0]
         runs.append(O.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(5e7, tol=1e-10))
     assert np.array_equal(runs[0].x, runs[1].x) and runs[0].Z_loop == runs[1].Z_loop
+
+
+def _neumann_parallel(L1: float, L2: float, axial: float, sep: float) -> float:
+    """Mutual of two parallel coaxial-direction filaments (lengths L1, L2, start offset `axial`, distance `sep`),
+    mu0/4pi double integral of 1/r by adaptive quadrature."""
+    inner = lambda s: quad(lambda t: 1.0 / math.hypot(axial + t - s, sep), 0, L2, epsrel=1e-13, limit=200)[0]
+    return 1e-7 * quad(inner, 0, L1, epsrel=1e-12, limit=200)[0]
+
+
+def test_trace_resistance_reaches_its_dc_and_skin_depth_limits():
+    from groundtruth_amd import peec as P
+    sigma, t = 5.8e7, 35e-6
+    trace = [TraceSegment(np.array([0.0, 0.0, 1e-4]), np.array([4e-3, 0.0, 1e-4]), 0.2e-3, 1.0),
+             TraceSegment(np.array([4e-3, 0.0, 1e-4]), np.array([4e-3, 3e-3, 1e-4]), 0.15e-3, 1.0),
+             TraceSegment(np.array([4e-3, 3e-3, 1e-4]), np.array([4e-3, 3e-3, 5e-4]), 0.3e-3, 1.0)]   # vertical: none
+    dc = 4e-3 / (sigma * 0.2e-3 * t) + 3e-3 / (sigma * 0.15e-3 * t)
+    for fn in (O.trace_resistance, P.trace_resistance):
+        assert fn(trace, 0.0, sigma, t) == pytest.approx(dc, rel=1e-15)
+        f = 2e10                                    # t / delta = 77: the current in one skin depth
+        delta = 1.0 / math.sqrt(math.pi * f * 4e-7 * math.pi * sigma)
+        hf = 4e-3 / (sigma * 0.2e-3 * delta) + 3e-3 / (sigma * 0.15e-3 * delta)
+        assert fn(trace, f, sigma, t) == pytest.approx(hf, rel=1e-12)
+
+
+def test_trace_self_is_neumanns_integral_over_its_legs():
+    from groundtruth_amd import peec as P
+    t, w = 35e-6, 0.2e-3
+    rho = K.GMD_FLAT * (w + t)
+    z = np.array([0.0, 0.0, 1e-4])
+    ex, ey = np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])
+    La, Lb, sep = 10e-3, 6e-3, 2e-3
+    # an L: two perpendicular legs, so only the legs' own terms
+    ell = [TraceSegment(z, z + La * ex, w, 1.0), TraceSegment(z + La * ex, z + La * ex + sep * ey, w, 1.0)]
+    self_a, self_s = _neumann_parallel(La, La, 0.0, rho), _neumann_parallel(sep, sep, 0.0, rho)
+    # a U: out along x, across, back along -x over part of the way; the return leg runs antiparallel at `sep`
+    u = ell + [TraceSegment(z + La * ex + sep * ey, z + (La - Lb) * ex + sep * ey, w, 1.0)]
+    self_b = _neumann_parallel(Lb, Lb, 0.0, rho)
+    m_ab = -_neumann_parallel(La, Lb, La - Lb, sep)
+    for fn in (O.trace_self, P.trace_self):
+        assert fn(ell, t) == pytest.approx(self_a + self_s, rel=1e-10)
+        assert fn(u, t) == pytest.approx(self_a + self_s + self_b + 2 * m_ab, rel=1e-10)
