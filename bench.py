@@ -139,6 +139,49 @@ def lp_only(args):
                       "bytes_per_launch": lp_algorithmic_bytes(st, args.lp_batch), "copy_bytes": n}))
 
 
+def steps_only(args):
+    """Set-up, one warm-up step and --steps timed steps of the workload's batch in one precision (--steps-only
+    fp64 | fp32-basis | mixed), then three plain device copies of 64 MiB (the scale for the memory counters): the
+    kernels of the headline steps alone, for a per-kernel profile."""
+    import torch
+
+    from groundtruth_amd import boards, peec
+    board = boards.WORKLOADS[args.workload]()
+    route = board.routes[0]
+    st = peec.PlaneStack(board.layers, board.branches, device=0)
+    ex = peec.RouteExcitation(st, route.trace, route.driver, route.receiver)
+    ops = [st.operator(f) for f in board.freqs]
+    rhs = []
+    for f in board.freqs:
+        top, J = ex.rhs(f)
+        rhs.append(torch.from_numpy(np.concatenate([top, J[st.keep].astype(complex)])).to("cuda:0"))
+    x = [torch.zeros_like(r) for r in rhs]
+    prec = args.steps_only
+
+    def step():
+        res = st.gmres_batch_dev(ops, rhs, x, board.tol, restart=board.restart, precision=prec)
+        if not all(r.converged for r in res):
+            raise RuntimeError(f"{prec} GMRES did not converge")
+        return [int(r.iterations) for r in res]
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = 64 << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": args.workload, "precision": prec, "steps": args.steps,
+                      "ms_per_step": dt / args.steps * 1e3, "iterations": its, "copy_bytes": n}))
+
+
 def cpu_lp_large(st, I: np.ndarray, y_dev: np.ndarray) -> dict:
     """The six-layer board's L_p product on the host by the oracle's numpy / scipy path (the reference's FFT matvec,
     peec.py:2054-2135: fft2 / ifft2 with workers=-1 and the numpy multiply-add over sheet pairs), with the device
@@ -557,6 +600,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--large", action="store_true", help="the six-layer 2048^2 board alone (a few minutes)")
     ap.add_argument("--no-large", action="store_true", help="skip the six-layer board (about three minutes)")
+    ap.add_argument("--steps-only", choices=["fp64", "fp32-basis", "mixed"],
+                    help="only the workload's batched steps in this precision, then 64 MiB copies (for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (independent boards and frequencies "
@@ -565,6 +610,9 @@ def main():
         sys.exit(2)
     if args.lp_only:
         lp_only(args)
+        return
+    if args.steps_only:
+        steps_only(args)
         return
     if args.large:
         print(json.dumps({"large": large(cpu=not args.no_cpu_baseline)}))

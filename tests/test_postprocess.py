@@ -152,4 +152,5 @@ def test_device_node_field_equals_the_reference_on_its_own_solution(name):
     for k in range(st.n_sheets):
         for dev, ref in ((kx[k], a[f"nf_x_{k}"]), (ky[k], a[f"nf_y_{k}"])):
             assert np.array_equal(dev == 0, ref == 0), k
-            assert np.max(np.abs(dev - ref)) <= 2e-16 * np.max(np.abs(ref)), k
+            # the device divides where numpy multiplies by the reciprocal: within two units in the last place
+            assert np.max(np.abs(dev - ref)) <= 5e-16 * np.max(np.abs(ref)), k
