@@ -62,6 +62,9 @@ SIGNATURES = {
     "gt_stack_lp_autotune": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p]),
     "gt_stack_set_lp_plan": (C.c_int, [C.c_void_p, i32p, i32p]),
     "gt_node_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
+    "gt_loop_terms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, i32p, f64p, C.c_double,
+                                f64p, f64p]),
+    "gt_via_coupling": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.c_int32, C.POINTER(Segment), C.c_double, f64p]),
     "gt_strip_coupling": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.POINTER(Segment), f64p, f64p,
                                     C.c_void_p]),
     "gt_stack_destroy": (None, [C.c_void_p]),

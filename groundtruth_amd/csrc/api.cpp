@@ -509,6 +509,28 @@ int gt_node_field(gt_stack* st, const void* x_dev, double d, void* kx_dev, void*
     });
 }
 
+int gt_loop_terms(gt_stack* st, const void* x_dev, const void* psi_dev, const void* bm_dev, int32_t n_inj,
+                  const int32_t* inj_idx, const double* inj_amps, double d, const double* sheet_z, double* out) {
+    return guard([&] {
+        need(st && x_dev && psi_dev && sheet_z && out && n_inj >= 0 && (n_inj == 0 || (inj_idx && inj_amps)),
+             "NULL argument");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        GT_HIP(hipSetDevice(st->ctx->device));
+        loop_terms_device(st, (const zd*)x_dev, (const double*)psi_dev, (const double*)bm_dev, n_inj, inj_idx, inj_amps,
+                          d, sheet_z, out);
+    });
+}
+
+int gt_via_coupling(gt_ctx* ctx, int32_t nv, const double* vias, int32_t nseg, const gt_segment* segs, double mu,
+                    double* out) {
+    return guard([&] {
+        need(ctx && nv >= 0 && nseg >= 0 && (nv == 0 || (vias && out)) && (nseg == 0 || segs), "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        via_coupling_device(ctx, nv, vias, nseg, segs, mu, out);
+    });
+}
+
 int gt_strip_coupling(gt_stack* st, double d, double mu, int32_t nseg, const gt_segment* segs, const double* sheet_z,
                       const double* sheet_t, void* psi_dev) {
     return guard([&] {

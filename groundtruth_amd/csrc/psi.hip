@@ -85,7 +85,119 @@ __global__ __launch_bounds__(256) void psi_strips(PsiArgs a) {
     }
 }
 
+// The vias' coupling to the trace's vertical segments (peec.py:2658-2666 `branch_coupling`): per via, the sum over
+// segments of (current share) times the mutual of the segment and the via's axis, the segment's potential integrated
+// along the axis by a 16-point Gauss rule times the cosine between them (the via's radius the least distance).
+constexpr int VIA_GAUSS = 16;
+
+struct ViaArgs {
+    const Seg* seg;
+    const double* via;          // per via: a (3), b (3), radius
+    int nseg, nv;
+    double mu;
+    double gx[VIA_GAUSS], gw[VIA_GAUSS];     // nodes on [0, 1] and weights summing to 1
+    double* out;
+};
+
+__global__ __launch_bounds__(64) void psi_vias(ViaArgs a) {
+    const int v = blockIdx.x * 64 + threadIdx.x;
+    if (v >= a.nv) return;
+    const double* p = a.via + 7 * (size_t)v;
+    const double dx = p[3] - p[0], dy = p[4] - p[1], dz = p[5] - p[2];
+    const double L2 = sqrt(dx * dx + dy * dy + dz * dz);
+    double acc = 0.0;
+    if (L2 > 0) {
+        for (int q = 0; q < a.nseg; ++q) {
+            Seg g = a.seg[q];
+            g.rho_min = p[6];
+            double sum = 0.0;
+            for (int k = 0; k < VIA_GAUSS; ++k)
+                sum += a.gw[k] * seg_pot(g, p[0] + a.gx[k] * dx, p[1] + a.gx[k] * dy, p[2] + a.gx[k] * dz, a.mu);
+            const double cosang = (g.u[0] * dx + g.u[1] * dy + g.u[2] * dz) / L2;
+            acc += g.coef[0] * cosang * L2 * sum;
+        }
+    }
+    a.out[v] = acc;
+}
+
+// Gauss-Legendre nodes and weights of order n on [-1, 1]: Newton's iteration on P_n from the asymptotic roots.
+void gauss_legendre(int n, double* x, double* w) {
+    for (int i = 0; i < n; ++i) {
+        double t = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1, p1 = t;
+            for (int k = 2; k <= n; ++k) {
+                const double pk = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = pk;
+            }
+            dp = n * (t * p1 - p0) / (t * t - 1);
+            const double dt = p1 / dp;
+            t -= dt;
+            if (std::fabs(dt) < 1e-16) break;
+        }
+        double p0 = 1, p1 = t;
+        for (int k = 2; k <= n; ++k) {
+            const double pk = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = pk;
+        }
+        dp = n * (t * p1 - p0) / (t * t - 1);
+        x[i] = t;
+        w[i] = 2.0 / ((1 - t * t) * dp * dp);
+    }
+}
+
 }  // namespace
+
+void via_coupling_device(gt_ctx* ctx, int nv, const double* vias, int nseg, const gt_segment* segs, double mu,
+                         double* out) {
+    hipStream_t s = ctx->stream;
+    std::vector<Seg> hs;
+    for (int q = 0; q < nseg; ++q) {
+        const gt_segment& g = segs[q];
+        double ab[3];
+        for (int c = 0; c < 3; ++c) ab[c] = g.b[c] - g.a[c];
+        // vertical segments only: along z with no run in the plane
+        if (!(std::fabs(ab[2]) > 0) || !(std::hypot(ab[0], ab[1]) < 1e-12)) continue;
+        const double L = std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+        Seg v{};
+        for (int c = 0; c < 3; ++c) {
+            v.a[c] = g.a[c];
+            v.u[c] = ab[c] / L;
+        }
+        v.L = L;
+        v.coef[0] = g.current;
+        hs.push_back(v);
+    }
+    if (nv == 0) return;
+    if (hs.empty()) {
+        std::fill(out, out + nv, 0.0);
+        return;
+    }
+    for (int v = 0; v < nv; ++v)
+        if (!(vias[7 * v + 6] > 0)) fail(GT_ERR_ARG, "a via's radius must be positive");
+    DevBuf<unsigned char> buf(hs.size() * sizeof(Seg) + (size_t)nv * 8 * sizeof(double));
+    ViaArgs a{};
+    a.seg = (const Seg*)buf.p;
+    a.via = (const double*)(buf.p + hs.size() * sizeof(Seg));
+    a.out = (double*)(buf.p + hs.size() * sizeof(Seg) + (size_t)nv * 7 * sizeof(double));
+    a.nseg = (int)hs.size();
+    a.nv = nv;
+    a.mu = mu;
+    double x[VIA_GAUSS], w[VIA_GAUSS];
+    gauss_legendre(VIA_GAUSS, x, w);
+    for (int k = 0; k < VIA_GAUSS; ++k) {
+        a.gx[k] = 0.5 * (x[k] + 1.0);
+        a.gw[k] = 0.5 * w[k];
+    }
+    GT_HIP(hipMemcpyAsync((void*)a.seg, hs.data(), hs.size() * sizeof(Seg), hipMemcpyHostToDevice, s));
+    GT_HIP(hipMemcpyAsync((void*)a.via, vias, (size_t)nv * 7 * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(psi_vias, dim3((nv + 63) / 64), dim3(64), 0, s, a);
+    GT_LAUNCH("psi_vias");
+    GT_HIP(hipMemcpyAsync(out, a.out, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+}
 
 void strip_coupling_device(gt_stack* st, double d, double mu, int nseg, const gt_segment* segs, const double* sheet_z,
                            const double* sheet_t, double* psi) {

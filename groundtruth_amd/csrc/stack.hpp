@@ -188,6 +188,10 @@ void noalloc_scratch_setup(int device);
 void* noalloc_scratch(int device);
 // (Kx, Ky) at every node of every sheet, S x nx x ny complex each, from the strip currents of x (post.hip)
 void node_field_device(gt_stack* st, const zd* x, double d, zd* kx, zd* ky);
+void loop_terms_device(gt_stack* st, const zd* x, const double* psi, const double* bm, int n_inj, const int* inj_idx,
+                       const double* inj_amps, double d, const double* sheet_z, double* out);
+void via_coupling_device(gt_ctx* ctx, int nv, const double* vias, int nseg, const gt_segment* segs, double mu,
+                         double* out);
 // psi of every strip (ns values at psi, device memory) from the trace's segments (psi.hip)
 void strip_coupling_device(gt_stack* st, double d, double mu, int nseg, const gt_segment* segs, const double* sheet_z,
                            const double* sheet_t, double* psi);

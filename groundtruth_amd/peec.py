@@ -805,12 +805,12 @@ class PlaneStack:
         set-up is the host's."""
         if self.setup_mode == "host" or self.ns == 0:
             return self.strip_coupling_host(trace)
+        return self.strip_coupling_dev(trace).cpu().numpy()
+
+    def strip_coupling_dev(self, trace: list):
+        """strip_coupling as a device tensor (torch float64, ns values; its work finished)."""
         import torch
-        segs = (_lib.Segment * max(1, len(trace)))()
-        for k, s in enumerate(trace):
-            segs[k].a[:] = [float(v) for v in s.a]
-            segs[k].b[:] = [float(v) for v in s.b]
-            segs[k].width, segs[k].current = float(s.width), float(s.i)
+        segs = _segments(trace)
         psi = torch.zeros(self.ns, dtype=torch.float64, device=f"cuda:{self.device}")
         torch.cuda.synchronize(self.device)
         z = np.ascontiguousarray(self.sheet_z, float)
@@ -818,7 +818,8 @@ class PlaneStack:
         _lib.check(_lib.lib().gt_strip_coupling(self.handle, float(self.d), G.MU0 / (4 * math.pi), len(trace), segs,
                                                 _lib.ptr(z), _lib.ptr(t), C.c_void_p(psi.data_ptr())),
                    "gt_strip_coupling")
-        return psi.cpu().numpy()
+        self.ctx.sync()
+        return psi
 
     def strip_coupling_host(self, trace: list) -> np.ndarray:
         """strip_coupling in numpy."""
@@ -861,7 +862,26 @@ class PlaneStack:
         return psi
 
     def branch_coupling(self, trace: list, start: int = 0) -> np.ndarray:
-        """psi of via branches from the vertical trace segments (peec.py:2658-2666)."""
+        """psi of via branches from the vertical trace segments (peec.py:2658-2666): each via's mutual with every
+        vertical segment, a 16-point Gauss rule along the via. On the device (gt_via_coupling) unless the stack's
+        set-up is the host's; branches that are not vias have none."""
+        if self.setup_mode == "host":
+            return self.branch_coupling_host(trace, start)
+        out = np.zeros(self.nbr - start)
+        vias = [(n, b) for n, b in enumerate(self.branches[start:]) if b.kind == "via"]
+        if not vias or not trace:
+            return out
+        ends = np.array([[*self.pos(b.a), *self.pos(b.b), float(b.radius)] for _, b in vias], float)
+        val = np.zeros(len(vias))
+        segs = _segments(trace)
+        self.handle
+        _lib.check(_lib.lib().gt_via_coupling(self.ctx.handle, len(vias), _lib.ptr(ends), len(trace), segs,
+                                              G.MU0 / (4 * math.pi), _lib.ptr(val)), "gt_via_coupling")
+        out[[n for n, _ in vias]] = val
+        return out
+
+    def branch_coupling_host(self, trace: list, start: int = 0) -> np.ndarray:
+        """branch_coupling in numpy."""
         vert = [s for s in trace if abs((s.b - s.a)[2]) > 0 and np.linalg.norm((s.b - s.a)[:2]) < 1e-12]
         out = np.zeros(self.nbr - start)
         for n, b in enumerate(self.branches[start:]):
@@ -888,6 +908,87 @@ class PlaneStack:
         for b, cur in zip(self.branches, Ib[self.ns:]):
             m += 0.5 * np.cross(self.pos(b.a), self.pos(b.b)) * cur
         return m
+
+
+    def loop_terms_dev(self, x_dev, psi_dev, injections: list):
+        """The loop's sums over a device solution (gt_loop_terms): (psi . Ib, the strips' and branches' dipole moment
+        (3 complex values, the trace's own part left out), sum a V over the injections), with one small read-back."""
+        self.handle
+        if getattr(self, "_bm_dev", None) is None:
+            import torch
+            bm = np.array([0.5 * np.cross(self.pos(b.a), self.pos(b.b)) for b in self.branches], float).reshape(-1, 3)
+            with torch.cuda.stream(self.torch_stream()):
+                self._bm_dev = torch.from_numpy(np.ascontiguousarray(bm)).to(f"cuda:{self.device}")
+        if getattr(self, "_keep_pos", None) is None:
+            self._keep_pos = np.where(self.keep, np.cumsum(self.keep) - 1, -1)
+        kp = self._keep_pos
+        terms = [(self.nb + int(kp[n]), float(a)) for n, a in injections if kp[n] >= 0]   # V = 0 off the kept nodes
+        idx = np.array([t[0] for t in terms] or [0], np.int32)
+        amps = np.array([t[1] for t in terms] or [0.0], float)
+        out = np.zeros(10)
+        z = np.ascontiguousarray(self.sheet_z, float)
+        bm = C.c_void_p(self._bm_dev.data_ptr()) if self.nbr else None
+        _lib.check(_lib.lib().gt_loop_terms(self.handle, C.c_void_p(x_dev.data_ptr()), C.c_void_p(psi_dev.data_ptr()),
+                                            bm, len(terms), _lib.ptr(idx, _lib.i32p), _lib.ptr(amps),
+                                            float(self.d), _lib.ptr(z), _lib.ptr(out)), "gt_loop_terms")
+        c = out.view(np.complex128)
+        return complex(c[0]), c[1:4].copy(), complex(c[4])
+
+
+def _segments(trace: list):
+    """The trace as the library's segment records."""
+    segs = (_lib.Segment * max(1, len(trace)))()
+    for k, s in enumerate(trace):
+        segs[k].a[:] = [float(v) for v in s.a]
+        segs[k].b[:] = [float(v) for v in s.b]
+        segs[k].width, segs[k].current = float(s.width), float(s.i)
+    return segs
+
+
+class _DeviceSolution(Solution):
+    """A Solution whose solver vector stays on the device: Z_loop and the moment come from the loop's sums on the
+    device (gt_loop_terms) and the branch currents from a slice; the vector itself is copied to pinned host memory
+    behind the solve on the library's stream, and the strip-current and potential grids are unpacked from it when
+    first read."""
+
+    def __init__(self, ex: "RouteExcitation", f: float, x_dev, its: int):
+        import torch
+        st = ex.stack
+        self.f = f
+        self.iterations = its
+        self.residual = float("nan")
+        P, m_grid, v_inj = st.loop_terms_dev(x_dev, ex.psi_dev, ex.injections)
+        self.Z_loop = ex.z_loop_terms(f, P, v_inj)
+        self.m = ex.trace_moment() + m_grid
+        with torch.cuda.stream(st.torch_stream()):
+            self.I_branch = x_dev[st.ns:st.nb].cpu().numpy()
+            host = torch.empty(x_dev.shape, dtype=x_dev.dtype, pin_memory=True)
+            host.copy_(x_dev, non_blocking=True)
+            self._done = torch.cuda.Event()
+            self._done.record()
+        self._host = host
+        self._ex = ex
+        self._full = None
+
+    @property
+    def x(self) -> np.ndarray:
+        if self._done is not None:
+            self._done.synchronize()
+            self._done = None
+        return self._host.numpy()
+
+    def _unpacked(self) -> Solution:
+        if self._full is None:
+            st = self._ex.stack
+            x = self.x
+            V = np.zeros(st.n_nodes, complex)
+            V[st.keep] = x[st.nb:]
+            self._full = self._ex.unpack(self.f, x[:st.nb], V, self.iterations, x, moment=False)
+        return self._full
+
+    I_x = property(lambda self: self._unpacked().I_x)
+    I_y = property(lambda self: self._unpacked().I_y)
+    V = property(lambda self: self._unpacked().V)
 
 
 class _LazySolution(Solution):
@@ -981,6 +1082,10 @@ class RouteExcitation:
                 raise SolverError("receiver return node is not connected to the driver's return node")
             raise SolverError("the injected currents do not balance on every connected piece of copper")
         self.injections = ids
+        self._strip_psi_dev = None
+        if strip_psi is None and stack.setup_mode == "device" and stack.ns:
+            self._strip_psi_dev = stack.strip_coupling_dev(trace)
+            strip_psi = self._strip_psi_dev.cpu().numpy()
         self.strip_psi = stack.strip_coupling(trace) if strip_psi is None else strip_psi
         bp = branch_psi if branch_psi is not None else np.zeros(0)
         if len(bp) < stack.nbr:
@@ -993,8 +1098,10 @@ class RouteExcitation:
         """The same loop on a what-if stack (with_branches): the strip coupling is reused, the added branches'
         coupling computed (peec.py:2771-2774)."""
         drv, rcv = self.inject
-        return RouteExcitation(stack, self.trace, drv, rcv, self.trace_t, self.strip_psi, self.branch_psi,
-                               self.extra)
+        ex = RouteExcitation(stack, self.trace, drv, rcv, self.trace_t, self.strip_psi, self.branch_psi, self.extra)
+        if len(ex.strip_psi) == len(self.strip_psi):
+            ex._strip_psi_dev = self._strip_psi_dev       # the same strips: their psi stays shared on the device
+        return ex
 
     def rhs(self, f: float):
         """(rhs_top = -jw psi, J) at f (peec.py:2789-2795)."""
@@ -1025,11 +1132,40 @@ class RouteExcitation:
     def z_loop(self, f: float, Ib: np.ndarray, V: np.ndarray) -> complex:
         """Z_loop of a solution (peec.py:2787-2800): the trace's resistance and the injection nodes' potentials, plus
         j w (L_trace + psi . Ib)."""
-        st = self.stack
-        w = 2 * math.pi * f
         v_inj = sum(amps * V[n] for n, amps in self.injections)
-        return complex(trace_resistance(self.trace, f, st.sigma, self.trace_t) + v_inj
-                       + 1j * w * (self.L_trace + np.dot(self.psi, Ib)))
+        return self.z_loop_terms(f, complex(np.dot(self.psi, Ib)), v_inj)
+
+    def z_loop_terms(self, f: float, psi_dot_Ib: complex, v_inj: complex) -> complex:
+        """Z_loop from its two solution sums: R_trace + sum a V + j w (L_trace + psi . Ib)."""
+        w = 2 * math.pi * f
+        return complex(trace_resistance(self.trace, f, self.stack.sigma, self.trace_t) + v_inj
+                       + 1j * w * (self.L_trace + psi_dot_Ib))
+
+    def trace_moment(self) -> np.ndarray:
+        """The trace's own part of the loop's dipole moment, 1/2 sum i a x b over its segments (peec.py:2668-2700)."""
+        m = np.zeros(3, complex)
+        for s in self.trace:
+            m += 0.5 * s.i * np.cross(s.a, s.b)
+        return m
+
+    @property
+    def psi_dev(self):
+        """psi on the device (torch float64, nb values), made once per excitation: the strips' part as the device
+        computed it (shared by the what-ifs of one board), the branches' uploaded."""
+        if getattr(self, "_psi_dev", None) is None:
+            import torch
+            dev = f"cuda:{self.stack.device}"
+            with torch.cuda.stream(self.stack.torch_stream()):     # ordered before the library's use of it
+                if self._strip_psi_dev is None:
+                    self._strip_psi_dev = torch.from_numpy(np.ascontiguousarray(self.strip_psi, float)).to(dev)
+                br = torch.from_numpy(np.ascontiguousarray(self.branch_psi, float)).to(dev)
+                self._psi_dev = torch.cat([self._strip_psi_dev, br])
+        return self._psi_dev
+
+    def unpack_dev(self, f: float, x_dev, its: int) -> "Solution":
+        """The Solution of a solver vector on the device (torch complex128, its work ordered on the library's stream):
+        Z_loop, the moment and the branch currents made now on the device, the rest when first read."""
+        return _DeviceSolution(self, f, x_dev, its)
 
     def unpack_lazy(self, f: float, x: np.ndarray, its: int) -> "Solution":
         """The Solution of the solver vector x with Z_loop and the branch currents made now and the strip-current
@@ -1040,7 +1176,7 @@ class RouteExcitation:
         V[st.keep] = x[st.nb:]
         return _LazySolution(self, f, Ib, V, its, x)
 
-    def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None) -> Solution:
+    def unpack(self, f: float, Ib: np.ndarray, V: np.ndarray, its: int, x=None, moment: bool = True) -> Solution:
         st = self.stack
         I_x = np.zeros((st.N, st.nx - 1, st.ny), complex)
         I_y = np.zeros((st.N, st.nx, st.ny - 1), complex)
@@ -1049,4 +1185,5 @@ class RouteExcitation:
         Vg = np.zeros((st.N, st.nx, st.ny), complex)
         Vg[np.nonzero(st.node)] = V[:st.n_grid]
         Z = self.z_loop(f, Ib, V)
-        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, st.moment(self.trace, Ib), its, x)
+        m = st.moment(self.trace, Ib) if moment else None
+        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, m, its, x)

@@ -596,10 +596,9 @@ class Server:
                         xd = s.x + s.x0
                 if st is self.boards[req.board]:
                     self._board_x[(req.board, id(req.route), PlaneStack.freq_key(s.f))] = xd
-                with torch.cuda.stream(st.torch_stream()):
-                    xv = xd.cpu().numpy()
-                # Z_loop now, the grids when first read
-                sol = s.ex.unpack_lazy(s.f, xv, int(res.iterations))
+                # Z_loop, the moment and the branch currents now from the device; the vector copied to the host
+                # behind the running work, the grids unpacked when first read
+                sol = s.ex.unpack_dev(s.f, xd, int(res.iterations))
                 sol.residual = float(res.residual)
                 req.solutions[s.k] = sol
             self._open[id(req)] -= 1

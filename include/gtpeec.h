@@ -182,6 +182,23 @@ typedef struct {
 int gt_strip_coupling(gt_stack* stack, double d, double mu, int32_t nseg, const gt_segment* segs,
                       const double* sheet_z, const double* sheet_t, void* psi_dev);
 
+/* The loop's sums over a solution on the device, for Z_loop and the moment without copying the solution to the host
+ * (peec.py:2787-2811 `RouteExcitation.solve`'s Z = R_trace + sum a_k V[n_k] + j w (L_trace + psi . Ib), and
+ * peec.py:2668-2700 `PlaneStack.moment`'s m = 1/2 sum r x I dl over the strips and branches). x_dev: nb + n_keep
+ * complex values; psi_dev: nb doubles (the strips' then the branches' coupling); bm_dev: per branch 1/2 r_a x r_b
+ * (3 doubles, m), or NULL to leave the branches out of m; inj_idx / inj_amps (host): the injection terms, each an index
+ * into x_dev (a kept node's potential, nb <= idx < nb + n_keep) and its amps; d: the pitch (m); sheet_z (host): each
+ * sheet's height (m). out (host, 10 doubles, written): psi . Ib, the strips' and branches' m (3 complex values, the
+ * trace's own part not included) and sum a V, each as (re, im). */
+int gt_loop_terms(gt_stack* stack, const void* x_dev, const void* psi_dev, const void* bm_dev, int32_t n_inj,
+                  const int32_t* inj_idx, const double* inj_amps, double d, const double* sheet_z, double* out);
+
+/* psi of via branches (peec.py:2658-2666 `branch_coupling`): each via's mutual with the trace's vertical segments
+ * (those with no run in the plane), times the segments' current shares. vias (host): per via its ends a, b (m) and
+ * its radius (m), 7 doubles; mu: mu0 / (4 pi); out (host, nv doubles, written). */
+int gt_via_coupling(gt_ctx* ctx, int32_t nv, const double* vias, int32_t nseg, const gt_segment* segs, double mu,
+                    double* out);
+
 /* The L_p product's launch shapes, tuned (no counterpart in the reference): for each batch-size class (1, 2-4, 5-16
  * right-hand sides) the rows per row-pass workgroup (1, 2, 4) and waves per column-pass workgroup (1, 2, 4) that
  * ran `reps` products of 1, 4 and 8 right-hand sides fastest on this stack. The shapes change no arithmetic, so a

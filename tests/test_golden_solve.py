@@ -80,3 +80,43 @@ def test_device_solve_matches_the_reference(fixtures, name, s):
     assert G.l2(sol.V, a[f"V_{s}"]) <= 1e-8
     assert G.l2(sol.m, a[f"m_{s}"]) <= 1e-8
     assert abs(sol.iterations - int(a[f"its_{s}"])) <= 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,s", CASES)
+def test_device_loop_terms_match_the_reference(fixtures, name, s):
+    """Z_loop, the moment and the branch currents formed on the device (gt_loop_terms) from a solution that stays on
+    the device, and the strips' and vias' psi made on the device (gt_strip_coupling, gt_via_coupling), against the
+    reference's own values; and the device sums against the host's on the same solution vector."""
+    import torch
+    from groundtruth_amd import peec
+    fx = fixtures[name]
+    a = fx.data
+    r, f = fx.solves[s][1]
+    trace, drv, rcv = fx.routes[r]
+    st = peec.PlaneStack(fx.layers, fx.branches)
+    ex = peec.RouteExcitation(st, trace, drv, rcv)
+    assert G.l2(ex.psi, a[f"psi_{r}"]) <= 1e-13
+    if st.nbr:
+        assert G.l2(ex.branch_psi, st.branch_coupling_host(trace)) <= 1e-13
+    rhs_top, J = ex.rhs(f)
+    b = torch.from_numpy(np.concatenate([rhs_top, J[st.keep].astype(complex)])).to("cuda:0")
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+    its, ok, _ = st.gmres_dev(st.operator(f), b, x, float(a["tol"]))
+    assert ok
+    sol = ex.unpack_dev(f, x, its)
+    assert abs(sol.Z_loop - a[f"Z_{s}"]) / abs(a[f"Z_{s}"]) <= 1e-9
+    assert G.l2(sol.m, a[f"m_{s}"]) <= 1e-8
+    if st.nbr:
+        assert G.l2(sol.I_branch, a[f"I_branch_{s}"]) <= 1e-8
+    ref_I = G.currents(a[f"I_x_{s}"], a[f"I_y_{s}"], a[f"I_branch_{s}"])
+    assert G.l2(G.currents(sol.I_x, sol.I_y, sol.I_branch), ref_I) <= 1e-9
+    assert G.l2(sol.V, a[f"V_{s}"]) <= 1e-8
+    xh = sol.x
+    Ib = xh[:st.nb]
+    V = np.zeros(st.n_nodes, complex)
+    V[st.keep] = xh[st.nb:]
+    Zh = ex.z_loop(f, Ib, V)
+    assert abs(sol.Z_loop - Zh) <= 1e-12 * abs(Zh)
+    assert G.l2(sol.m, st.moment(trace, Ib)) <= 1e-12

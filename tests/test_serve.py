@@ -275,7 +275,9 @@ def test_served_requests_equal_each_request_solved_alone(budget):
         ref = _alone(b, q)
         for got, one in zip(q.solutions, ref):
             assert np.array_equal(got.x, one.x) and got.iterations == one.iterations
-            assert got.Z_loop == one.Z_loop
+            # served: Z_loop summed on the device; alone: on the host, from the same bits of x
+            assert abs(got.Z_loop - one.Z_loop) <= 1e-13 * abs(one.Z_loop)
+            assert np.allclose(got.m, one.m, rtol=1e-12, atol=0)
     srv.close()
 
 
