@@ -115,6 +115,7 @@ void setup_orient(gt_stack* st, int oi, const int32_t* sij, int cnt, int lo, int
     o.rowcol.upload(rowcol, st->ctx->stream);
     twiddles(st, o.P0);
     twiddles(st, o.P1);
+    if (st->shape.split) twiddles(st, o.P0 / 4);         // the residue-class column pass
     if (donor) {
         const OrientHost& od = donor->o[oi];
         need(od.present && od.P0 == o.P0 && od.P1 == o.P1 && od.n0 == n0 && od.n1 == n1,
@@ -181,18 +182,24 @@ LpArgs<double> gt_stack_s::lp_args(int mode) const {
         d.K = h.K->p;
         d.T1 = work.T1.p + h.t_off;
         d.T2 = work.T2.p + h.t_off;
-        d.T3 = s_elems ? work.T3.p + h.s_off : nullptr;
         d.tw0 = h.present ? tw.at(h.P0).p : nullptr;
         d.tw1 = h.present ? tw.at(h.P1).p : nullptr;
+        d.twq = h.present && shape.split ? tw.at(h.P0 / 4).p : nullptr;
         a.nblk_rows[oi] = h.present ? S * ((h.n0 + h.TR - 1) / h.TR) : 0;
         a.nblk_cols[oi] = h.present ? h.P1 : 0;
     }
     a.S = S;
     a.mode = mode;
     a.t_stride = t_elems;
-    a.s_stride = s_elems;
     a.G = kG;
     a.gmap = kgmap_dev.p;
+    a.toeplitz = kG == S && !kgmap.empty();
+    for (int t = 0; t < S; ++t)
+        for (int u = 0; u < S; ++u) {
+            const int g = kgmap.empty() ? 0 : kgmap[(size_t)t * S + u];
+            a.gpad[t * LP_MAX_SHEETS + u] = g;
+            if (g != (t > u ? t - u : u - t)) a.toeplitz = 0;
+        }
     a.a_keep = a_keep.p;
     a.b_keep = b_keep.p;
     return a;
@@ -447,13 +454,6 @@ void create_stack(gt_ctx* ctx, const gt_stack_desc* d, const gt_kernel_desc* kd,
         st->kgmap_dev.upload(st->kgmap, s);
         setup_orient(st.get(), 0, d->x_sij, d->nxb, 0, d->nx - 1, d->ny, d->px, d->kx, kd ? &kg : nullptr, donor);
         setup_orient(st.get(), 1, d->y_sij, d->nyb, d->nxb, d->nx, d->ny - 1, d->py, d->ky, kd ? &kg : nullptr, donor);
-        if (st->shape.split) {                  // both orientations' whole column spectra, each its own region
-            for (OrientHost& o : st->o) {
-                if (!o.present) continue;
-                o.s_off = st->s_elems;
-                st->s_elems += (size_t)st->S * o.P1 * o.P0;
-            }
-        }
         st->strip_sheet.upload(st->h_strip_sheet.empty() ? std::vector<unsigned char>{0} : st->h_strip_sheet, s);
         const int nb = st->nb, nk = st->nk;
         need(nb == 0 || (d->a_keep && d->b_keep), "incidence missing");
@@ -607,9 +607,6 @@ int gt_lp_matvec_batch_noalloc(gt_stack* st, int32_t nr, const void* const* I_de
         for (int i = 0; i < nr; ++i) need(I_dev[i] && out_dev[i] && I_dev[i] != out_dev[i], "NULL or aliased vector");
         std::lock_guard<std::mutex> lk(st->ctx->mu);
         if (st->ns == 0) return;
-        if (st->s_elems)
-            fail(GT_ERR_ARG, "gt_lp_matvec_batch_noalloc: this stack's column pass splits through HBM (S * P0 > "
-                             "8192), which needs a third intermediate; use gt_lp_matvec_batch");
         const size_t need_bytes = 2 * sizeof(zd) * (size_t)nr * std::max<size_t>(1, st->t_elems);
         if (need_bytes > GT_LP_NOALLOC_SCRATCH_BYTES)
             fail(GT_ERR_ARG, "gt_lp_matvec_batch_noalloc: " + std::to_string(nr) + " right-hand sides need " +
@@ -622,7 +619,6 @@ int gt_lp_matvec_batch_noalloc(gt_stack* st, int32_t nr, const void* const* I_de
         for (int oi = 0; oi < 2; ++oi) {
             a.o[oi].T1 = T1 + st->o[oi].t_off;
             a.o[oi].T2 = T2 + st->o[oi].t_off;
-            a.o[oi].T3 = nullptr;
         }
         a.rhs.nr = nr;
         for (int i = 0; i < nr; ++i) {

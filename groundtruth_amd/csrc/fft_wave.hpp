@@ -16,8 +16,6 @@
 
 namespace gt {
 
-__device__ __forceinline__ int xpad(int p) { return p + (p >> 3); }
-
 template <int E>
 struct WavePlan;                          // radices of the passes for N = 64 E
 template <>

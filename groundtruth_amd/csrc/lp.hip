@@ -29,6 +29,8 @@ __device__ __forceinline__ const OrientArgs<T>& pick(const LpArgs<T>& a, int oi)
     return oi ? a.o[1] : a.o[0];
 }
 
+__device__ __forceinline__ int ilog2_dev(int v) { return 31 - __clz(v); }
+
 // Workgroups go to the 8 XCDs round-robin (workgroup w to XCD w mod 8), each XCD with its own L2. Row tiles i0 and
 // i0 + RW share the 128-byte lines of the transposed intermediates ([t][j][i]: 8 complex doubles of consecutive i),
 // so the tile order is remapped for consecutive tiles to run on one XCD: tile = (w mod 8) (G / 8) + w / 8 over a
@@ -119,96 +121,185 @@ __global__ __launch_bounds__(NTH) void lp_cols_mac(LpArgs<T> a) {
 }
 
 
-// The column pass in three launches through HBM, for stacks whose S sheets of one column exceed the LDS budget
-// (S * P0 > 8192 points): (1) each sheet's column j, zero-padded, transformed along i, its whole spectrum to T3;
-// (2) at every pixel the kernel multiply-add over the source sheets (the same sums, in the same order, as the
-// one-workgroup pass); (3) each sheet's inverse transform, rows 0:n0 to T2. Item of (1) and (3): (sheet, column).
-template <class T, int NTH, int MAXPTS, bool INV>
-__global__ __launch_bounds__(NTH) void lp_cols_split_fft(LpArgs<T> a) {
+// Pass B for columns whose S sheets exceed the LDS budget (S * P0 > 8192 points), in one launch, by a split of the
+// spectrum into its four residue classes of bins k = 4 k' + r [the decimation-in-frequency split of Cooley and Tukey
+// 1965]. A column's input u_s[i] is zero for i >= n0, and n0 <= P0 / 2 = 2 M (M = P0 / 4), so with i = m + M q:
+//   U_s[4 k' + r] = FFT_M(z_{s,r})[k'],   z_{s,r}[m] = w^(m r) (u_s[m] + (-i)^r u_s[m + M]),   w = e^(-2 pi i / P0),
+// and the inverse transform's rows i = m + M q (q = 0, 1) of output sheet t are
+//   y_t[m + M q] = sum_r i^(q r) conj(w^(m r)) IFFT_M(Y_{t,r})[m],   Y_{t,r}[k'] = sum_s K_ts(4 k' + r) U_s[4 k' + r].
+// One workgroup per column takes the residues in turn: the S transforms of one residue class (S * M points, 96 KB of
+// FP64 LDS for six sheets of 4096-point columns), the kernel multiply-add per bin, the S inverse transforms, and the
+// rows accumulated in registers (thread m holds rows m and m + M of every sheet). Nothing goes through HBM between
+// the forward transform and the inverse one; the column's input is read once per residue class (from the L2 or the
+// last-level cache after the first). The sums are the same DFT as the one-workgroup pass, grouped differently.
+constexpr int RES4_FG = 3;       // sheets per group of transforms in the residue-class pass
+constexpr int RES4_KQ = 8;       // distinct kernels per pixel the residue-class pass stages in LDS (more: read direct)
+
+template <class T, int NTH, int SMAX, int MPT, bool TOEP>
+__global__ __launch_bounds__(NTH) void lp_cols_res4(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
-    int item = blockIdx.x, oi = 0;
-    const int first = a.S * a.nblk_cols[0];
-    if (item >= first) { item -= first; oi = 1; }
+    constexpr int M = NTH * MPT;                              // thread x holds rows x + NTH c (c < MPT)
+    constexpr int MS = M + M / 8;                             // a sheet's transform in LDS, padded one in eight
+    const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
+    int k = xcd_tile(blockIdx.x, gridDim.x), oi = 0;        // the columns j and P1 - j (one kernel row) on one XCD
+    if (k >= ncol) return;
+    if (k >= a.nblk_cols[0]) { k -= a.nblk_cols[0]; oi = 1; }
     const OrientArgs<T> o = pick(a, oi);
-    const int s = item / o.P1, j = item - s * o.P1;
-    const int P0 = o.P0, n0 = o.n0;
-    cpx<T>* T3 = o.T3 + blockIdx.y * a.s_stride + ((size_t)s * o.P1 + j) * P0;
-    if (!INV) {
-        const cpx<T>* T1 = o.T1 + blockIdx.y * a.t_stride + ((size_t)s * o.P1 + j) * n0;
-        for (int i = threadIdx.x; i < P0; i += NTH) buf[i] = i < n0 ? T1[i] : cpx<T>();
-    } else {
-        for (int i = threadIdx.x; i < P0; i += NTH) buf[i] = T3[i];
-    }
-    __syncthreads();
-    fft_lds<T, INV, NTH, MAXPTS>(buf, P0, o.log0, 1, P0, o.tw0);
-    if (!INV) {
-        for (int i = threadIdx.x; i < P0; i += NTH) T3[i] = buf[i];
-    } else {
-        cpx<T>* T2 = o.T2 + blockIdx.y * a.t_stride + ((size_t)s * o.P1 + j) * n0;
-        for (int i = threadIdx.x; i < n0; i += NTH) T2[i] = buf[i];
-    }
-}
-
-// The split pass's multiply-add over source sheets: one thread per kernel-quadrant pixel (qi, qj) takes the up to
-// four pixels (i, j) in {qi, P0 - qi} x {qj, P1 - qj} that read its S x S kernel values, so the kernel is read from
-// HBM once per product (the values stay in registers over the four)
-template <class T, int S>
-__global__ __launch_bounds__(256) void lp_cols_split_mix(LpArgs<T> a, long q0) {
-    long idx = blockIdx.x * 256L + threadIdx.x;
-    int oi = 0;
-    if (idx >= q0) { idx -= q0; oi = 1; }
-    const OrientArgs<T> o = pick(a, oi);
-    const int P0 = o.P0, P1 = o.P1, h0 = P0 / 2 + 1, h1 = P1 / 2 + 1;
-    if (a.nblk_cols[oi] == 0 || idx >= (long)h0 * h1) return;
-    const int qj = (int)(idx / h0), qi = (int)(idx - (long)qj * h0);
-    const T* Kp = o.K + ((size_t)qj * h0 + qi) * a.G;
-    T k[S * S];
+    const int S = a.S, P0 = o.P0, P1 = o.P1, n0 = o.n0;
+    const int j = k < 2 ? (k ? P1 / 2 : 0) : ((k & 1) ? P1 - (k >> 1) : (k >> 1));
+    const size_t sheet = (size_t)P1 * n0;
+    const cpx<T>* T1 = o.T1 + blockIdx.y * a.t_stride + (size_t)j * n0;
+    cpx<T>* T2 = o.T2 + blockIdx.y * a.t_stride + (size_t)j * n0;
+    // the M-point transforms' twiddles, pass by pass (conflict-free reads), in LDS after the sheets
+    const int logM = ilog2_dev(M);
+    cpx<T>* twm = buf + S * MS;
+    fft_twiddle_table<T>(twm, M, logM, o.twq, threadIdx.x, NTH);
+    // a bin's kernel values, one column of RES4_KQ per thread, after the twiddles (not for eight FP64 sheets of
+    // 4096-point columns: their LDS is full)
+    constexpr bool KQ = !(sizeof(T) == 8 && SMAX == 8 && MPT == 2);
+    T* kq = reinterpret_cast<T*>(twm + fft_twiddle_count(logM));
+    const int qj = min(j, P1 - j), h0 = P0 / 2 + 1;
+    const T* Kcol = o.K + (size_t)qj * h0 * a.G;
+    cpx<T> acc[MPT][SMAX][2];
+    for (int r = 0; r < 4; ++r) {
+        // z_{s,r}[m] = w^(m r) (u_s[m] + (-i)^r u_s[m + M]) for every sheet: a row's loads all in flight at once
+#pragma unroll 1
+        for (int c = 0; c < MPT; ++c) {
+            const int m = threadIdx.x + NTH * c;
+            const cpx<T> wr = o.tw0[m * r];                   // w^(m r), m r < 3 M < P0
+            cpx<T> u0[SMAX], u1[SMAX];
 #pragma unroll
-    for (int e = 0; e < S * S; ++e) k[e] = Kp[a.gmap[e]];
-    cpx<T>* base = o.T3 + blockIdx.y * a.s_stride;
-    const size_t sheet = (size_t)P1 * P0;
-    const int ni = (qi == 0 || 2 * qi == P0) ? 1 : 2, nj = (qj == 0 || 2 * qj == P1) ? 1 : 2;
-    for (int b = 0; b < nj; ++b)
-        for (int c = 0; c < ni; ++c) {
-            cpx<T>* T3 = base + (size_t)(b ? P1 - qj : qj) * P0 + (c ? P0 - qi : qi);
-            cpx<T> v[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) v[s] = T3[s * sheet];
-#pragma unroll
-            for (int t = 0; t < S; ++t) {
-                cpx<T> acc;
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    acc.re += k[t * S + s] * v[s].re;
-                    acc.im += k[t * S + s] * v[s].im;
+            for (int s = 0; s < SMAX; ++s) {
+                if (s < S) {
+                    u0[s] = m < n0 ? T1[s * sheet + m] : cpx<T>();
+                    u1[s] = m + M < n0 ? T1[s * sheet + m + M] : cpx<T>();
                 }
-                T3[t * sheet] = acc;
+            }
+#pragma unroll
+            for (int s = 0; s < SMAX; ++s) {
+                if (s < S) {
+                    const cpx<T> p = u0[s], q = u1[s];
+                    const cpx<T> z = r == 0 ? p + q : r == 1 ? cpx<T>(p.re + q.im, p.im - q.re)
+                                   : r == 2 ? p - q : cpx<T>(p.re - q.im, p.im + q.re);
+                    buf[s * MS + xpad(m)] = z * wr;
+                }
             }
         }
-}
-
-template <class T, int S>
-void launch_split_mix(const LpArgs<T>& a, int nr, hipStream_t st) {
-    auto quad = [&](int oi) { return a.nblk_cols[oi] ? (long)(a.o[oi].P0 / 2 + 1) * (a.o[oi].P1 / 2 + 1) : 0L; };
-    const long q0 = quad(0), q = q0 + quad(1);
-    hipLaunchKernelGGL((lp_cols_split_mix<T, S>), dim3((unsigned)((q + 255) / 256), nr), dim3(256), 0, st, a, q0);
-}
-
-template <class T>
-void split_mix(const LpArgs<T>& a, int nr, hipStream_t st) {
-    switch (a.S) {
-        case 1: launch_split_mix<T, 1>(a, nr, st); break;
-        case 2: launch_split_mix<T, 2>(a, nr, st); break;
-        case 3: launch_split_mix<T, 3>(a, nr, st); break;
-        case 4: launch_split_mix<T, 4>(a, nr, st); break;
-        case 5: launch_split_mix<T, 5>(a, nr, st); break;
-        case 6: launch_split_mix<T, 6>(a, nr, st); break;
-        case 7: launch_split_mix<T, 7>(a, nr, st); break;
-        default: launch_split_mix<T, LP_MAX_SHEETS>(a, nr, st); break;
+        __syncthreads();
+        // the sheets' transforms in groups of RES4_FG: fewer butterflies, so fewer registers, per thread and pass
+        for (int h = 0; h < S; h += RES4_FG)
+            fft_lds<T, false, NTH, RES4_FG * M, true, true>(buf + h * MS, M, logM, min(RES4_FG, S - h), MS, twm);
+        // bins k = 4 k' + r: the kernel multiply-add over source sheets. Sheets equally spaced (TOEP: the kernel of
+        // sheets t, s is kernel |t - s|): every bin's kernel values loaded at once into registers. Otherwise a bin's G
+        // kernel values (one per distinct face gap) are loaded together into the thread's column of kq, then read per
+        // sheet pair through the map.
+        T kl[MPT][SMAX];
+        if constexpr (TOEP) {
+#pragma unroll
+            for (int c = 0; c < MPT; ++c) {
+                const int kk = 4 * (threadIdx.x + NTH * c) + r;
+                const T* Kp = Kcol + (size_t)min(kk, P0 - kk) * a.G;
+#pragma unroll
+                for (int g = 0; g < SMAX; ++g)
+                    if (g < S) kl[c][g] = Kp[g];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < MPT; ++c) {
+            const int m = threadIdx.x + NTH * c;
+            const int kk = 4 * m + r;
+            const T* Kp = Kcol + (size_t)min(kk, P0 - kk) * a.G;
+            cpx<T> v[SMAX];
+#pragma unroll
+            for (int s = 0; s < SMAX; ++s)
+                if (s < S) v[s] = buf[s * MS + xpad(m)];
+            if constexpr (TOEP) {
+#pragma unroll
+                for (int t = 0; t < SMAX; ++t) {
+                    if (t < S) {
+                        cpx<T> y;
+#pragma unroll
+                        for (int s = 0; s < SMAX; ++s) {
+                            if (s < S) {
+                                const T kv = kl[c][t > s ? t - s : s - t];
+                                y.re += kv * v[s].re;
+                                y.im += kv * v[s].im;
+                            }
+                        }
+                        buf[t * MS + xpad(m)] = y;
+                    }
+                }
+            } else if (KQ && a.G <= RES4_KQ) {
+                T kl[RES4_KQ];
+#pragma unroll
+                for (int g = 0; g < RES4_KQ; ++g)
+                    if (g < a.G) kl[g] = Kp[g];
+#pragma unroll
+                for (int g = 0; g < RES4_KQ; ++g)
+                    if (g < a.G) kq[g * NTH + threadIdx.x] = kl[g];
+#pragma unroll 1
+                for (int t = 0; t < S; ++t) {
+                    cpx<T> y;
+#pragma unroll
+                    for (int s = 0; s < SMAX; ++s) {
+                        if (s < S) {
+                            const T kv = kq[a.gpad[t * LP_MAX_SHEETS + s] * NTH + threadIdx.x];
+                            y.re += kv * v[s].re;
+                            y.im += kv * v[s].im;
+                        }
+                    }
+                    buf[t * MS + xpad(m)] = y;
+                }
+            } else {
+#pragma unroll 1
+                for (int t = 0; t < S; ++t) {
+                    cpx<T> y;
+#pragma unroll
+                    for (int s = 0; s < SMAX; ++s) {
+                        if (s < S) {
+                            const T kv = Kp[a.gpad[t * LP_MAX_SHEETS + s]];
+                            y.re += kv * v[s].re;
+                            y.im += kv * v[s].im;
+                        }
+                    }
+                    buf[t * MS + xpad(m)] = y;
+                }
+            }
+        }
+        __syncthreads();
+        for (int h = 0; h < S; h += RES4_FG)
+            fft_lds<T, true, NTH, RES4_FG * M, true, true>(buf + h * MS, M, logM, min(RES4_FG, S - h), MS, twm);
+        // rows m and m + M: conj(w^(m r)) Z, and i^r times it
+#pragma unroll
+        for (int c = 0; c < MPT; ++c) {
+            const int m = threadIdx.x + NTH * c;
+            const cpx<T> wr = o.tw0[m * r];
+#pragma unroll
+            for (int t = 0; t < SMAX; ++t) {
+                if (t < S) {
+                    const cpx<T> z = buf[t * MS + xpad(m)];
+                    const cpx<T> A(z.re * wr.re + z.im * wr.im, z.im * wr.re - z.re * wr.im);
+                    acc[c][t][0] = acc[c][t][0] + A;
+                    acc[c][t][1] = r == 0 ? acc[c][t][1] + A : r == 1 ? acc[c][t][1] + cpx<T>(-A.im, A.re)
+                                 : r == 2 ? acc[c][t][1] - A : acc[c][t][1] + cpx<T>(A.im, -A.re);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < MPT; ++c) {
+        const int m = threadIdx.x + NTH * c;
+#pragma unroll
+        for (int t = 0; t < SMAX; ++t) {
+            if (t < S) {
+                if (m < n0) T2[t * sheet + m] = acc[c][t][0];
+                if (m + M < n0) T2[t * sheet + m + M] = acc[c][t][1];
+            }
+        }
     }
 }
-
 
 // Pass B with the transforms in registers (column length P0 = 64 E, S <= 4 sheets): one wave per column j. Lane l
 // holds rows l + 64 m of every sheet, loaded straight from T1 (whole contiguous columns); the forward FFT, the
@@ -534,6 +625,76 @@ __global__ __launch_bounds__(256) void tail_bottom(TailArgs<T> a) {
 
 }  // namespace
 
+// the residue-class column pass's dynamic LDS at most: the CU's 160 KB (it has no static LDS)
+constexpr size_t lds_res4_max() { return 160 * 1024; }
+
+static int ilog2_host(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+static int dev_of() {
+    int d = 0;
+    GT_HIP(hipGetDevice(&d));
+    return d;
+}
+
+// kernels whose LDS above the default 64 KB was asked for, per device (a function's attributes are the current
+// device's); contexts on several devices may launch from several threads
+static std::mutex lds_mu;
+static std::set<int> lds_set;
+static std::set<std::pair<int, const void*>> res4_set;
+
+
+// The residue-class column pass over the columns of length P0 (2048 or 4096) that `a` names.
+template <class T>
+void cols_res4(const LpArgs<T>& a, int P0, int nr, hipStream_t st) {
+    const int ncol = a.nblk_cols[0] + a.nblk_cols[1];
+    if (ncol == 0) return;
+    if (P0 != 2048 && P0 != 4096) fail(GT_ERR_UNSUPPORTED, "residue-class column pass: P0 is 2048 or 4096");
+    if (a.S > LP_MAX_SHEETS) fail(GT_ERR_UNSUPPORTED, "residue-class column pass: more sheets than it holds");
+    const int grid = (ncol + 7) / 8 * 8;              // xcd_tile maps a multiple of 8
+    // columns of 4096 points: 512 threads of two rows each (FP64 accumulators take 96 registers), of 2048 points one
+    // row each; the twiddles in LDS beside up to six FP64 sheets
+    constexpr bool dbl = sizeof(T) == 8;
+    constexpr int NT4 = 512, MP4 = 2;
+    const bool wide = a.S > 6;
+    const int M = P0 / 4, MS = M + M / 8;
+    const bool kq = !(dbl && wide && P0 == 4096);
+    const size_t lds = ((size_t)a.S * MS + fft_twiddle_count(ilog2_host(M))) * sizeof(cpx<T>) +
+                       (kq ? (size_t)RES4_KQ * 512 * sizeof(T) : 0);
+    const bool toep = a.toeplitz != 0;
+    const void* fn;
+#define GT_RES4(NT, SM, MP, TP) (const void*)&lp_cols_res4<T, NT, SM, MP, TP>
+    if (P0 == 4096)
+        fn = wide ? (toep ? GT_RES4(NT4, 8, MP4, true) : GT_RES4(NT4, 8, MP4, false))
+                  : (toep ? GT_RES4(NT4, 6, MP4, true) : GT_RES4(NT4, 6, MP4, false));
+    else
+        fn = wide ? (toep ? GT_RES4(512, 8, 1, true) : GT_RES4(512, 8, 1, false))
+                  : (toep ? GT_RES4(512, 6, 1, true) : GT_RES4(512, 6, 1, false));
+#undef GT_RES4
+    {
+        std::lock_guard<std::mutex> lk(lds_mu);
+        const int d = dev_of();
+        if (!res4_set.count({d, fn})) {
+            GT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_res4_max()));
+            res4_set.insert({d, fn});
+        }
+    }
+    if (lds > lds_res4_max()) fail(GT_ERR_UNSUPPORTED, "residue-class column pass: LDS above its budget");
+#define GT_RES4_GO(NT, SM, MP, TP) hipLaunchKernelGGL((lp_cols_res4<T, NT, SM, MP, TP>), dim3(grid, nr), dim3(NT), lds, st, a)
+    if (P0 == 4096) {
+        if (!wide) { if (toep) GT_RES4_GO(NT4, 6, MP4, true); else GT_RES4_GO(NT4, 6, MP4, false); }
+        else { if (toep) GT_RES4_GO(NT4, 8, MP4, true); else GT_RES4_GO(NT4, 8, MP4, false); }
+    } else {
+        if (!wide) { if (toep) GT_RES4_GO(512, 6, 1, true); else GT_RES4_GO(512, 6, 1, false); }
+        else { if (toep) GT_RES4_GO(512, 8, 1, true); else GT_RES4_GO(512, 8, 1, false); }
+    }
+#undef GT_RES4_GO
+    GT_LAUNCH("lp_cols_res4");
+}
+
 template <class T>
 void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const int nrow = a.nblk_rows[0] + a.nblk_rows[1];
@@ -541,7 +702,7 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     constexpr int NTH = 256;
     // the 128 KB LDS passes (one workgroup per CU) and the split pass's 64 KB column transforms take wider
     // workgroups, so a CU holds 16 waves of them; a butterfly's arithmetic does not depend on its thread
-    constexpr int NTH_BIG = 1024, NTH_SPLIT = 512;
+    constexpr int NTH_BIG = 1024;
     const int nrow8 = (nrow + 7) / 8 * 8;       // the generic row passes' grid: xcd_tile maps a multiple of 8
     const int nr = a.rhs.nr;
     const int bucket = nr == 1 ? 0 : nr <= 4 ? 1 : 2;
@@ -553,8 +714,6 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     const bool big_cols = sh.lds_cols > (size_t)LpPts<T>::cols * sizeof(cpx<T>);
     // once per device (a function's attributes are the current device's), under a lock: contexts on several
     // devices may launch from several threads
-    static std::mutex lds_mu;
-    static std::set<int> lds_set;
     int dev = 0;
     if (big_rows || big_cols) GT_HIP(hipGetDevice(&dev));
     std::unique_lock<std::mutex> lds_lk(lds_mu, std::defer_lock);
@@ -578,17 +737,30 @@ void launch_lp(const LpArgs<T>& a, const LpShape& sh, hipStream_t st) {
     }
     GT_LAUNCH("lp_rows_fwd");
     if (sh.split) {
-        const int items = a.S * ncol;
-        const int P0 = std::max(a.nblk_cols[0] ? a.o[0].P0 : 0, a.nblk_cols[1] ? a.o[1].P0 : 0);
-        const size_t lds = (size_t)P0 * sizeof(cpx<T>);
-        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH_SPLIT, LpPts<T>::cols, false>), dim3(items, nr), dim3(NTH_SPLIT),
-                           lds, st, a);
-        GT_LAUNCH("lp_cols_split_fft");
-        split_mix(a, nr, st);
-        GT_LAUNCH("lp_cols_split_mix");
-        hipLaunchKernelGGL((lp_cols_split_fft<T, NTH_SPLIT, LpPts<T>::cols, true>), dim3(items, nr), dim3(NTH_SPLIT),
-                           lds, st, a);
-        GT_LAUNCH("lp_cols_split_fft");
+        const int P0a = a.nblk_cols[0] ? a.o[0].P0 : 0, P0b = a.nblk_cols[1] ? a.o[1].P0 : 0;
+        if (P0a == P0b || P0a == 0 || P0b == 0) {
+            cols_res4(a, std::max(P0a, P0b), nr, st);
+        } else {
+            // the orientations' column lengths differ (nx = 2^k + 1): each its own launch, the shorter one (S x P0
+            // <= 8192 points) by the one-workgroup pass
+            for (int oi = 0; oi < 2; ++oi) {
+                LpArgs<T> b = a;
+                b.nblk_cols[1 - oi] = 0;
+                const int P0 = a.o[oi].P0;
+                if (P0 >= 2048) {
+                    cols_res4(b, P0, nr, st);
+                } else {
+                    const size_t lds = (size_t)a.S * P0 * sizeof(cpx<T>);
+                    if (lds > (size_t)LpPts<T>::cols * sizeof(cpx<T>))
+                        hipLaunchKernelGGL((lp_cols_mac<T, NTH_BIG, BIG, LP_MAX_SHEETS>), dim3(b.nblk_cols[oi], nr),
+                                           dim3(NTH_BIG), lds, st, b);
+                    else
+                        hipLaunchKernelGGL((lp_cols_mac<T, NTH, LpPts<T>::cols, LP_MAX_SHEETS>), dim3(b.nblk_cols[oi], nr),
+                                           dim3(NTH), lds, st, b);
+                    GT_LAUNCH("lp_cols_mac");
+                }
+            }
+        }
     } else if (!launch_cols_w(a, ncol, nr, st, cw)) {
         if (big_cols)
             hipLaunchKernelGGL((lp_cols_mac<T, NTH_BIG, BIG, LP_MAX_SHEETS>), dim3(ncol, nr), dim3(NTH_BIG), sh.lds_cols,

@@ -30,9 +30,9 @@ struct OrientArgs {
     const T* K;               // kernel quadrant [qj][qi][g], scaled by 1 / (P0 P1): one value per distinct face gap g
     cpx<T>* T1;               // [s][j][i] after the row FFT (S * P1 * n0), right-hand side 0
     cpx<T>* T2;               // [t][j][i] after the column pass
-    cpx<T>* T3;               // [s][j][k] whole column spectra (the split column pass), right-hand side 0
     const cpx<T>* tw0;        // twiddles of P0
     const cpx<T>* tw1;        // twiddles of P1
+    const cpx<T>* twq;        // twiddles of P0 / 4 (the residue-class column pass)
 };
 
 // The right-hand sides of one batched product: input and output vectors, and each one's operator data.
@@ -56,9 +56,10 @@ struct LpArgs {
     int mode;                 // 0: y = L_p x (strip vectors); 1: the system product's strip rows (node potentials
                               // at rhs.nb[p] in x)
     size_t t_stride;          // elements of T1 / T2 per right-hand side
-    size_t s_stride;          // elements of T3 per right-hand side
     int G;                    // distinct kernels (face gaps) per pixel; sheets t, s read K[.][gmap[t S + s]]
     const int* gmap;          // S x S, device memory
+    int gpad[LP_MAX_SHEETS * LP_MAX_SHEETS];   // the same map at row stride LP_MAX_SHEETS, in the arguments
+    int toeplitz;             // the map is g = |t - s| with G = S (equally spaced sheets)
     const int* a_keep;
     const int* b_keep;
     RhsSet<T> rhs;
@@ -82,8 +83,8 @@ struct LpShape {
     // launch shapes of the register passes per batch-size class (1; 2-4; 5-16 right-hand sides): rows per row-pass
     // workgroup and waves per column-pass workgroup. They change no arithmetic; an autotuner may set them.
     int rw[3] = {4, 4, 4}, cw[3] = {1, 1, 1};
-    // the column pass in three launches through HBM (column transforms, the sheets' multiply-add per pixel,
-    // inverse transforms), for stacks whose S sheets of a column exceed the LDS budget
+    // the column pass by residue classes of bins mod 4 (lp_cols_res4), for stacks whose S sheets of a column
+    // exceed one workgroup's LDS
     bool split = false;
 };
 

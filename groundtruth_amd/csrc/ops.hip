@@ -307,7 +307,6 @@ void Work::ensure(const gt_stack* st, int nr, int kt) {
     if (nr > cap) {
         T1.alloc(std::max<size_t>(1, st->t_elems) * nr);
         T2.alloc(std::max<size_t>(1, st->t_elems) * nr);
-        if (st->s_elems) T3.alloc(st->s_elems * nr);
         s.alloc((size_t)std::max(1, st->nk) * nr);
         z.alloc((size_t)std::max(1, st->nk) * nr);
     }
@@ -325,7 +324,6 @@ void Work::shrink(const gt_stack* st, int nr) {
     const int kt = t_cols;
     T1.release();
     T2.release();
-    T3.release();
     s.release();
     z.release();
     z2.release();
@@ -339,7 +337,6 @@ void Work::shrink(const gt_stack* st, int nr) {
 void Work::reset(const gt_stack* st) {
     T1.release();
     T2.release();
-    T3.release();
     s.release();
     z.release();
     z2.release();
@@ -353,9 +350,9 @@ void Work::reset(const gt_stack* st) {
 int64_t Work::bytes_after(const gt_stack* st, int nr, int kt) const {
     kt = std::max(kt, st->nbr);
     const int64_t zb = sizeof(zd);
-    int64_t b = nr > cap ? (int64_t)(2 * std::max<size_t>(1, st->t_elems) + st->s_elems + 2 * (size_t)std::max(1, st->nk)) *
+    int64_t b = nr > cap ? (int64_t)(2 * std::max<size_t>(1, st->t_elems) + 2 * (size_t)std::max(1, st->nk)) *
                                nr * zb
-                         : (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes());
+                         : (int64_t)(T1.bytes() + T2.bytes() + s.bytes() + z.bytes());
     if (nr > cap || kt > t_cols) b += 2 * (int64_t)std::max(1, std::max(kt, t_cols)) * std::max(nr, cap) * zb;
     else b += (int64_t)(t.bytes() + u.bytes());
     if (st->wb_implicit) b += std::max<int64_t>((int64_t)z2.bytes(), (int64_t)std::max(1, st->nk) * std::max(nr, cap) * zb);
@@ -407,18 +404,19 @@ LpArgs<T> lp_args_t(const gt_stack* st) {
             o.K = h.Kf->p;
             o.T1 = reinterpret_cast<zf*>(st->work.T1.p) + h.t_off;
             o.T2 = reinterpret_cast<zf*>(st->work.T2.p) + h.t_off;
-            o.T3 = st->s_elems ? reinterpret_cast<zf*>(st->work.T3.p) + h.s_off : nullptr;
             o.tw0 = h.present ? st->twf.at(h.P0).p : nullptr;
             o.tw1 = h.present ? st->twf.at(h.P1).p : nullptr;
+            o.twq = h.present && st->shape.split ? st->twf.at(h.P0 / 4).p : nullptr;
             a.nblk_rows[oi] = d.nblk_rows[oi];
             a.nblk_cols[oi] = d.nblk_cols[oi];
         }
         a.S = d.S;
         a.mode = 1;
         a.t_stride = d.t_stride;
-        a.s_stride = d.s_stride;
         a.G = d.G;
         a.gmap = d.gmap;
+        std::copy(std::begin(d.gpad), std::end(d.gpad), std::begin(a.gpad));
+        a.toeplitz = d.toeplitz;
         a.a_keep = d.a_keep;
         a.b_keep = d.b_keep;
         return a;

@@ -49,7 +49,6 @@ namespace gt {
 struct OrientHost {
     bool present = false;
     int n0 = 0, n1 = 0, P0 = 0, P1 = 0, log0 = 0, log1 = 0, TR = 1, lo = 0, cnt = 0;
-    size_t s_off = 0;          // offset of its whole column spectra in T3 (the split column pass)
     size_t t_off = 0;          // offset of this orientation's L_p intermediates in T1 / T2
     DevBuf<int> rowptr, rowcol;
     // the kernel spectra, shared by the stacks made like this one (gt_stack_create_like: same grid and sheet gaps)
@@ -62,7 +61,6 @@ struct OrientHost {
 struct Work {
     int cap = 0;              // right-hand sides the buffers hold
     DevBuf<zd> T1, T2;        // L_p intermediates, t_elems per right-hand side
-    DevBuf<zd> T3;            // whole column spectra of the split column pass, s_elems per right-hand side
     DevBuf<zd> s, z;          // Schur right-hand sides and solutions, interleaved (n_keep x batch)
     DevBuf<zd> t;             // Woodbury: Q U^T z per right-hand side (t_cols each)
     DevBuf<zd> z2;            // implicit Woodbury form: B^-1 U t, interleaved like z
@@ -74,7 +72,7 @@ struct Work {
     // back to what a new stack holds (one right-hand side, its own branches' Woodbury width)
     void reset(const gt_stack_s* st);
     int64_t bytes() const {
-        return (int64_t)(T1.bytes() + T2.bytes() + T3.bytes() + s.bytes() + z.bytes() + t.bytes() + z2.bytes() +
+        return (int64_t)(T1.bytes() + T2.bytes() + s.bytes() + z.bytes() + t.bytes() + z2.bytes() +
                          u.bytes());
     }
     // the bytes it would hold after ensure(st, nr, kt)
@@ -97,7 +95,6 @@ struct StackCore {
     std::vector<unsigned char> h_strip_sheet;
     LpShape shape;
     size_t t_elems = 0;                       // complex elements of the L_p intermediates (both orientations)
-    size_t s_elems = 0;                       // complex elements of the split column pass's spectra (0: not split)
     int kG = 0;                               // distinct kernels per pixel (face gaps) and the sheets' map to them
     std::vector<int> kgmap;
     DevBuf<int> kgmap_dev;
@@ -117,7 +114,6 @@ struct gt_stack_s {
     std::vector<unsigned char>& h_strip_sheet;
     gt::LpShape& shape;
     size_t& t_elems;
-    size_t& s_elems;
     int& kG;
     std::vector<int>& kgmap;
     gt::DevBuf<int>& kgmap_dev;
@@ -150,7 +146,7 @@ struct gt_stack_s {
     explicit gt_stack_s(std::shared_ptr<gt::StackCore> c = std::make_shared<gt::StackCore>())
         : core(std::move(c)), o(core->o), tw(core->tw), twf(core->twf), node_sheet(core->node_sheet),
           strip_sheet(core->strip_sheet), h_strip_sheet(core->h_strip_sheet), shape(core->shape),
-          t_elems(core->t_elems), s_elems(core->s_elems), kG(core->kG), kgmap(core->kgmap), kgmap_dev(core->kgmap_dev), schur(core->schur), id(next_id()) {}
+          t_elems(core->t_elems), kG(core->kG), kgmap(core->kgmap), kgmap_dev(core->kgmap_dev), schur(core->schur), id(next_id()) {}
     gt_stack_s(const gt_stack_s&) = delete;
     gt_stack_s& operator=(const gt_stack_s&) = delete;
 

@@ -156,8 +156,8 @@ int gt_lp_matvec_batch(gt_stack* stack, int32_t nr, const void* const* I_dev, vo
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
  * same kernels and launch shapes, so each output equals gt_lp_matvec_batch's bit for bit; the intermediates live in
  * a fixed device array of GT_LP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), not in the stack's
- * workspace. A batch needs 2 * 16 B * nr * (the stack's per-product intermediate elements) of it, and a stack whose
- * column pass splits through HBM (S * P0 > 8192) needs a third buffer: such calls return GT_ERR_ARG with the sizes.
+ * workspace. A batch needs 2 * 16 B * nr * (the stack's per-product intermediate elements) of it: a batch above that
+ * returns GT_ERR_ARG with the sizes.
  * Calls that overlap on one GPU (from other contexts or threads, before an earlier call's kernels finish) share the
  * array and race: order them on one stream. */
 #define GT_LP_NOALLOC_SCRATCH_BYTES (64u << 20)

@@ -87,3 +87,20 @@ def test_both_forms_take_about_the_fp64_iterations(precision):
     f32 = ex.solve_many(fs, tol=1e-8, precision=precision)
     for a, c in zip(f32, d):
         assert a.iterations <= 1.5 * c.iterations + 4
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mixed_precision_with_4096_point_columns_of_six_sheets(precision):
+    """Six sheets of 4096-point columns (S x P0 above one workgroup's LDS): the FP32 products take the residue-class
+    column pass (four classes of bins mod 4, each S x 1024 points in LDS); the solve against the FP64 oracle."""
+    from test_gpu_parity import _strip_board
+    layers, branches = _strip_board(1100, 6, 6, seed=77)
+    st, ref = peec.PlaneStack(layers, branches), O.PlaneStack(layers, branches)
+    assert 4096 in tuple(st.Px) and st.n_sheets == 6
+    rng = np.random.default_rng(11)
+    rhs_top = rng.standard_normal(st.nb) + 1j * rng.standard_normal(st.nb)
+    J = np.zeros(st.n_nodes)
+    f = 3e8
+    (Ib, V, its, x), = st.solve_many([f], [rhs_top], [J], tol=1e-11, precision=precision)
+    Ib0, V0, its0, x0 = ref.solve(f, rhs_top, J, tol=1e-13)
+    assert rel(Ib, Ib0) < 1e-8

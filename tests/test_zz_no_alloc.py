@@ -51,3 +51,19 @@ def test_noalloc_product_refuses_a_batch_above_its_scratch():
     code, msg, _ = _batch(st, _lib.lib().gt_lp_matvec_batch_noalloc, _vecs(st, 16, 13))
     assert code == _lib.GT_ERR_ARG
     assert b"scratch" in msg
+
+
+def test_noalloc_product_on_a_stack_whose_columns_exceed_one_workgroup():
+    """Three sheets of 4096-point columns (the residue-class column pass): the same bits without allocation."""
+    from test_gpu_parity import _strip_board
+    L = _lib.lib()
+    layers, branches = _strip_board(1100, 6, 3, seed=21)
+    st = peec.PlaneStack(layers, branches)
+    assert 4096 in tuple(st.Px)
+    vecs = _vecs(st, 2, 14)
+    code, msg, got = _batch(st, L.gt_lp_matvec_batch_noalloc, vecs)
+    assert code == _lib.GT_OK, msg
+    code, msg, ref = _batch(st, L.gt_lp_matvec_batch, vecs)
+    assert code == _lib.GT_OK, msg
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
