@@ -5,7 +5,7 @@ the lower plane, 20 stitching vias, one Manhattan route, solved at 10 frequencie
 FP64, GMRES(50) to a relative residual of 1e-10. One step = the 10 solves (one route at every frequency) with the
 board and the right-hand sides resident, run as one batch (each solve follows the reference's GMRES and is
 bit-identical to its solve alone). value = solves per second over the timed steps. The same batch is then timed in
-mixed precision (FP32 FFT / Krylov inner solves under FGMRES, FP64 Arnoldi, Schur and refinement to 1e-8) and with
+mixed precision (the reference's restart cycles in FP32, every restart an FP64 refinement step) and with
 FP32 Krylov bases, each reported with its solutions' relative errors against an FP64 solve of the same system at tol
 1e-13.
 
@@ -19,14 +19,16 @@ batching server beside the same stream solved one solve at a time), the 512^2 wh
 the served backlog against the plain FP64 solver one solve at a time (serve_backlog); the oracle's CPU solve of one
 frequency (best of 3, with the host's CPU model, affinity and library versions); and the six-layer 2048^2 board
 (large: FP64, FP32-basis and mixed solves to 1e-8 with their solves/s and errors, its L_p product's GB/s against peak,
-and the same product on the host).
+and the same product on the host); and the four-layer 512^2 board's 256 right-hand sides (batched: 64 nets x 4
+frequencies in blocks of 16, mixed to 1e-8, FP64 on two of the blocks beside it, with a sample's errors).
 
 Progress lines go to stderr; stdout carries the one JSON line.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-serve] [--no-large]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-serve] [--no-large] [--no-batched]
 python bench.py --lp-only [--workload W] [--lp-reps R] [--lp-batch N]   (the counter runs: L_p launches of N
                                                          right-hand sides, then a 64 MiB device copy as the scale)
 python bench.py --large      (the six-layer 2048^2 board alone)
+python bench.py --batched    (the four-layer 512^2 board's 256 right-hand sides alone)
 """
 from __future__ import annotations
 
@@ -485,6 +487,76 @@ def serve_backlog(n_req: int = BACKLOG_REQUESTS, seed: int = 7, prepare_thread: 
     return out
 
 
+def batched(fp64_blocks: Optional[list] = None) -> dict:
+    """The batched multi-right-hand-side case at its own size: the four-layer board of 512 x 512 cells per layer at
+    0.2 mm (seeded voids, 64 nets of 5-40 segments with 0-4 layer changes), each net at the board's 4 frequencies in
+    10-500 MHz: 256 uneven right-hand sides, GMRES(50) to 1e-8 in mixed precision (FP32 cycles, FP64 refinement), in
+    blocks of 16 of one frequency (similar iteration counts), each block's launches shared by its 16 solves. The plain
+    FP64 solve of the blocks `fp64_blocks` (all when None) beside it, and the error of a sample against FP64 solves
+    at 1e-13."""
+    import torch
+
+    from groundtruth_amd import boards, peec
+    board = boards.four_layer()
+    log("four-layer 512^2 board, 64 nets x 4 frequencies: set-up")
+    t0 = time.perf_counter()
+    st = peec.PlaneStack(board.layers, board.branches, device=0)
+    st.handle
+    exs = [peec.RouteExcitation(st, r.trace, r.driver, r.receiver) for r in board.routes]
+    setup_s = time.perf_counter() - t0
+    jobs = [(ex, float(f)) for f in board.freqs for ex in exs]       # by frequency: a block's solves alike
+    ops = [st.operator(f) for _, f in jobs]
+    rhs = []
+    for ex, f in jobs:
+        top, J = ex.rhs(f)
+        rhs.append(torch.from_numpy(np.concatenate([top, np.asarray(J, float)[st.keep].astype(complex)])).to("cuda:0"))
+    torch.cuda.synchronize()
+    B = 16
+    blocks = [list(range(k, min(k + B, len(jobs)))) for k in range(0, len(jobs), B)]
+    fp64_blocks = list(range(len(blocks))) if fp64_blocks is None else fp64_blocks
+    out = {"workload": "four_layer: 4 x 512x512 cells at 0.2 mm, 64 nets x 4 frequencies (10-500 MHz) = 256 "
+                       "right-hand sides, GMRES(50) to 1e-8",
+           "unknowns": int(st.nb + st.n_keep), "branches": int(st.nbr), "setup_s": round(setup_s, 2),
+           "right_hand_sides": len(jobs), "block": B}
+    xs, tb = {}, {}
+    for prec, which in (("mixed", range(len(blocks))), ("fp64", fp64_blocks)):
+        x = {}
+        its, res_max, conv, tb[prec] = [], 0.0, 0, {}
+        log(f"four-layer board: {sum(len(blocks[k]) for k in which)} right-hand sides, {prec}")
+        for k in which:
+            blk = blocks[k]
+            for i in blk:
+                x[i] = torch.zeros_like(rhs[i])
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            res = st.gmres_batch_dev([ops[i] for i in blk], [rhs[i] for i in blk], [x[i] for i in blk], board.tol,
+                                     restart=board.restart, precision=prec)
+            torch.cuda.synchronize()
+            tb[prec][k] = time.perf_counter() - t1
+            its += [int(r.iterations) for r in res]
+            conv += sum(bool(r.converged) for r in res)
+            res_max = max(res_max, max(float(r.residual) for r in res))
+        n = sum(len(blocks[k]) for k in which)
+        dt = sum(tb[prec].values())
+        out[prec] = {"right_hand_sides": n, "seconds": round(dt, 3), "solves_per_s": n / dt, "converged": conv,
+                     "iterations": {"min": min(its), "median": float(np.median(its)), "max": max(its)},
+                     "residual_max": res_max}
+        xs[prec] = x
+    same = sorted(tb["fp64"])
+    out["mixed"]["speedup_vs_fp64"] = sum(tb["fp64"][k] for k in same) / sum(tb["mixed"][k] for k in same)
+    out["mixed"]["speedup_blocks"] = same
+    sample = [i for k in same[:2] for i in (blocks[k][0], blocks[k][-1])]
+    xr = [torch.zeros_like(rhs[i]) for i in sample]
+    st.gmres_batch_dev([ops[i] for i in sample], [rhs[i] for i in sample], xr, 1e-13, restart=board.restart)
+    torch.cuda.synchronize()
+    for prec, x in xs.items():
+        out[prec]["rel_error_sample"] = [float((x[i] - r).norm() / r.norm()) for i, r in zip(sample, xr)]
+    out["error_reference"] = f"FP64 solves of right-hand sides {sample} at tol 1e-13"
+    del rhs, xs, xr, st, exs, ops
+    torch.cuda.empty_cache()
+    return out
+
+
 WHATIF_RATE = 4.0              # the 512^2 what-if stream: Poisson arrivals per second
 WHATIF_SECONDS = 12.0
 
@@ -599,6 +671,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--large", action="store_true", help="the six-layer 2048^2 board alone (a few minutes)")
+    ap.add_argument("--batched", action="store_true", help="the four-layer 512^2 board's 256 right-hand sides alone")
+    ap.add_argument("--no-batched", action="store_true", help="skip the four-layer 256-right-hand-side case")
     ap.add_argument("--no-large", action="store_true", help="skip the six-layer board (about three minutes)")
     ap.add_argument("--steps-only", choices=["fp64", "fp32-basis", "mixed"],
                     help="only the workload's batched steps in this precision, then 64 MiB copies (for profiles)")
@@ -616,6 +690,9 @@ def main():
         return
     if args.large:
         print(json.dumps({"large": large(cpu=not args.no_cpu_baseline)}))
+        return
+    if args.batched:
+        print(json.dumps({"batched": batched()}))
         return
     if args.serve_only:
         print(json.dumps({"served_backlog": serve_backlog(),
@@ -801,6 +878,11 @@ def main():
         out["cpu_baseline"] = cpu_baseline(board, route, board.freqs[0])
     del st, ex, ops, rhs_dev, x_dev, x_mix, x_f32b, x_ref
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        try:
+            out["batched"] = batched(fp64_blocks=[0, 9])     # FP64 beside mixed on two blocks (f0, f2)
+        except Exception as e:      # noqa: BLE001 - the headline line above stands on its own
+            out["batched"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_large:
         try:
             out["large"] = large(cpu=not args.no_cpu_baseline)
