@@ -30,12 +30,13 @@ from scipy.sparse.csgraph import connected_components
 
 from . import _lib
 from . import greens as G
-from .problem import SIGMA_CU, Solution, SolverError
+from .problem import SIGMA_CU, DeviceTooSmall, Solution, SolverError
 
 GMRES_EXTRA_CYCLES = 24         # peec.py:98
 GMRES_CHECK_MISSES = 4          # peec.py:100
 SCREEN_MIN_NODES = 16           # peec.py:112
 OPERATORS_KEPT = 32             # per-frequency operators kept per stack (small device arrays each)
+DEVICE_MEMORY_SHARE = 0.8       # a model uses at most this share of the device's free memory (gpu.py:580)
 BATCH = 16                      # right-hand sides per batched device launch
 MIXED_INNER_TOL = 1e-6          # mixed precision: the smallest relative residual an FP32 cycle is asked for
 MIXED_MAX_REFINE = 64           # mixed precision: FP64 restarts at most (the reference's cycle rule governs)
@@ -404,11 +405,45 @@ class PlaneStack:
             return np.concatenate([x[:stack.nb], np.zeros(self.nb - stack.nb, complex), x[stack.nb:]])
         return None
 
+    def device_bytes_estimate(self) -> int:
+        """About what this stack and its solves put on the device (bytes), before any of it is made: the kernel
+        quadrants (FP64 and FP32, one per distinct face gap at most S of them for equally spaced sheets, S^2 at
+        worst), the strip layout and incidence, the L_p intermediates (2 complex per padded row point), the Schur
+        factor's panels (nested dissection of a grid: about n (3.8 log2(n / S) - 12.5) entries, fitted to the
+        factors of 2 x 256^2, 4 x 512^2 and 6 x 2048^2 boards), the Schur solve's vectors, and the GMRES basis at its
+        longest with a few more vectors (the reference's count, gpu.py:593-605)."""
+        S = self.n_sheets
+        n = self.nb + self.n_keep
+        G_k = len({float(v) for v in np.ravel(getattr(self, "_dz", np.zeros((S, S))))}) or S * S
+        kern = sum((P[0] // 2 + 1) * (P[1] // 2 + 1) for P, m in ((self.Px, self.nxb), (self.Py, self.nyb)) if m)
+        kernels = kern * G_k * 12
+        rows = sum(S * P[1] * (nrow) for P, nrow, m in ((self.Px, self.nx - 1, self.nxb), (self.Py, self.nx, self.nyb))
+                   if m)
+        lp = 2 * rows * 16
+        layout = 4 * (self.ns + 2 * S * self.nx + 6 * self.nb + self.n_keep)
+        nk = max(self.n_keep, 2)
+        nnz = nk * max(8.0, 3.8 * math.log2(max(nk / S, 2.0)) - 12.5)
+        factor = int(nnz * 8 + 3 * nk * 16 * BATCH)
+        restart = min(200, max(100, int(4e8 / (16 * n))))
+        basis = (restart + 1) * n * 16
+        return int(kernels + lp + layout + factor + basis + 20 * n * 16)
+
+    def _check_fits(self):
+        """DeviceTooSmall when the stack would take more than DEVICE_MEMORY_SHARE of the device's free memory
+        (gpu.py:483-488), before anything is uploaded."""
+        import torch
+        free = int(torch.cuda.mem_get_info(self.device)[0])
+        need = self.device_bytes_estimate()
+        if need > DEVICE_MEMORY_SHARE * free:
+            raise DeviceTooSmall(f"the model needs ~{need / 1e9:.3g} GB on device {self.device}, {free / 1e9:.3g} GB "
+                                 "free")
+
     # ---------------------------------------------------------------- the device copy
     def _upload(self):
         if self._base is not None:
             self._derive()
             return
+        self._check_fits()
         h = self._create()
         self._tune(h)
         self._factor(h)

@@ -23,6 +23,11 @@ class SolverError(RuntimeError):
     converging): reported by the caller, not a bug (peec.py:124-127)."""
 
 
+class DeviceTooSmall(RuntimeError):
+    """The device's free memory cannot hold the model: raised before anything is uploaded (gpu.py:576-577, raised
+    at gpu.py:483-488 when the model needs more than 0.8 of the free memory)."""
+
+
 class DeviceError(RuntimeError):
     """The HIP library failed (missing library, out of device memory, a failed launch)."""
 

@@ -225,3 +225,30 @@ def test_product_refuses_what_this_build_does_not_solve():
 def SolverErrorType():
     from groundtruth_amd.problem import SolverError
     return SolverError
+
+
+def test_device_footprint_estimate_scales_with_the_board():
+    """The estimate that DeviceTooSmall is checked against grows with the board and counts the GMRES basis at its
+    longest; a what-if stack is planned exactly elsewhere (gt_stack_derive_bytes)."""
+    bs, bb = boards.scaled("two_layer", 32, n_vias=4, n_freqs=1), boards.scaled("two_layer", 128, n_vias=4, n_freqs=1)
+    small, big = peec.PlaneStack(bs.layers, bs.branches), peec.PlaneStack(bb.layers, bb.branches)
+    e_small, e_big = small.device_bytes_estimate(), big.device_bytes_estimate()
+    n = big.nb + big.n_keep
+    assert 0 < e_small < e_big
+    assert e_big >= (100 + 1) * n * 16           # the basis of GMRES(100) alone
+
+
+@pytest.mark.gpu
+def test_a_stack_too_large_for_the_free_memory_is_refused_before_upload(monkeypatch):
+    import torch
+    from groundtruth_amd.problem import DeviceTooSmall
+    b = boards.scaled("two_layer", 32, n_vias=4, n_freqs=1)
+    st = peec.PlaneStack(b.layers, b.branches)
+    monkeypatch.setattr(peec, "DEVICE_MEMORY_SHARE", 1e-9)
+    before = torch.cuda.mem_get_info(0)[0]
+    with pytest.raises(DeviceTooSmall):
+        st.handle
+    assert st._handle is None
+    assert torch.cuda.mem_get_info(0)[0] >= before - (64 << 20)
+    monkeypatch.setattr(peec, "DEVICE_MEMORY_SHARE", 0.8)
+    assert st.handle is not None                     # fits at the reference's share
