@@ -90,6 +90,7 @@ SIGNATURES = {
     "gt_ctx_orthogonalization": (C.c_int, [C.c_void_p, C.c_int]),
     "gt_ctx_lp_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "gt_schur_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), i32p, i32p, i32p]),
+    "gt_schur_top_stats": (C.c_int, [C.c_void_p, i32p, i32p, C.POINTER(C.c_int64)]),
     "gt_schur_solve_host": (C.c_int, [C.c_void_p, f64p, f64p]),
     "gt_schur_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gt_schur_host_factor": (C.c_int, [C.c_int32, C.c_int32, i32p, i32p, i32p, C.POINTER(C.c_void_p)]),

@@ -718,6 +718,7 @@ void factor_laplacian(gt_stack* st, const int32_t* node_xyz) {
     S->h.build(nk, ea, eb, diag, node_xyz);
     S->upload(st->ctx->stream);
     S->factor_device(st->ctx->stream);
+    S->top_device(st->ctx->stream);
     st->schur = std::move(S);
 }
 
@@ -871,6 +872,16 @@ int gt_schur_stats(const gt_stack* st, int64_t* nnz_L, int32_t* n_super, int32_t
         if (n_super) *n_super = (int32_t)h.c0.size();
         if (n_levels) *n_levels = h.levels;
         if (max_cols) *max_cols = h.max_cols;
+    });
+}
+
+int gt_schur_top_stats(const gt_stack* st, int32_t* levels, int32_t* cols, int64_t* entries) {
+    return guard([&] {
+        need(st && st->schur, "no Schur factor");
+        const SchurHost& h = st->schur->h;
+        if (levels) *levels = h.top_T;
+        if (cols) *cols = (int32_t)h.top_col.size();
+        if (entries) *entries = h.top_fnnz + h.top_bnnz;
     });
 }
 

@@ -36,6 +36,14 @@ struct alignas(16) SnItem {
     int64_t pad;
 };
 
+// A work item of the top of the tree's dense products (schur_solve.hip): forward, rows [i0, i0 + cnt) of one top
+// supernode (top-column indices) over the chunk [k0, k1) of their reduction; backward, columns [i0, i0 + cnt) over the
+// chunk [k0, k1) of their chain. A tile split over several chunks writes partial sums (slot0 + kidx), added in chunk
+// order by a finalize item.
+struct TopItem {
+    int32_t i0, cnt, k0, k1, slot0, kidx, ks, pad;
+};
+
 struct SchurHost {
     int n = 0;                          // kept nodes
     std::vector<int> perm;              // new -> old
@@ -64,6 +72,27 @@ struct SchurHost {
     std::vector<int> ffin_ptr, bfin_ptr;      // per level: finalize items of its split tiles
     std::vector<SnItem> ffin, bfin;
     int64_t slots = 0;                  // partial-sum slots (SCHUR_TILE x 16 right-hand sides each)
+    // The top of the assembly tree (the supernodes of depth < top_T) is solved as dense products with the explicit
+    // inverse M of its block of L: forward z_T = M g, where g holds the right-hand side at the top columns plus the
+    // update entries their non-top children send up (those carry the whole rest of the tree's forward solve), and
+    // backward x_T = M^T z_T. Two or three launches each way replace 2 top_T level launches and their finalizes.
+    // top_T is the deepest that keeps the top columns and M's entries within SCHUR_TOP_COLS and SCHUR_TOP_NNZ, and
+    // 0 (no dense top) below SCHUR_TOP_MIN_T. M is made on the device after the factor (Schur::top_device) by
+    // forward solves of unit vectors through the top supernodes' own level items (tfwd).
+    int top_T = 0;
+    std::vector<int> top_col;           // top columns (new numbering), ascending: top-column index i
+    std::vector<int> top_lo;            // per i: first index of its supernode's top subtree (M's row i spans [lo, i])
+    std::vector<int64_t> top_foff;      // per i: offset of row i's entries M[i][lo .. i] in the forward store
+    std::vector<int> top_cptr, top_clen;        // per i: its chain (its supernode's columns, then its top
+    std::vector<int> top_chain;                 //   ancestors', ascending; new numbering) at top_chain[top_cptr[i] ..]
+    std::vector<int64_t> top_boff;      // per i: offset of column i's entries M[chain[e]][i] in the backward store
+    int64_t top_fnnz = 0, top_bnnz = 0;
+    std::vector<int64_t> top_gptr;      // per i: the update-buffer entries its non-top children add (ascending)
+    std::vector<int64_t> top_gidx;
+    std::vector<TopItem> top_fwd, top_ffin, top_bwd, top_bfin;
+    int64_t top_slots = 0;              // partial-sum slots of the dense tiles (SCHUR_TILE x 16 each)
+    std::vector<int> tfwd_ptr, tfwd_cols, tffin_ptr;        // the top supernodes' forward level items (M's set-up)
+    std::vector<SnItem> tfwd, tffin;
     int64_t nnz_L = 0;
     int max_cols = 0;
     // Fronts factorised on the device (schur_dense.hip): those of at least dev_min rows and every ancestor of one.
@@ -92,6 +121,8 @@ struct SchurHost {
                const int32_t* xyz);
     // x = B^-1 b on the host (complex values interleaved), with the same panels as the device
     void solve_host(const double* b, double* x) const;
+    // the dense top's index arrays and items (top_T chosen), from the supernodes' depths
+    void build_top(const std::vector<int>& dep);
 };
 
 // Work items of the device solves. Forward: a tile of 64 panel rows whose columns are split 4 ways inside each wave
@@ -100,6 +131,10 @@ struct SchurHost {
 // 17 to 32 columns and at most 64 rows, one tile of 32 columns with the rows split 8 ways. Every lane takes at most
 // SCHUR_PV panel values per item (all loaded at once), which sets the chunk sizes.
 constexpr int SCHUR_PV = 8;
+constexpr int SCHUR_TOP_COLS = 8192;    // top columns of the dense top at most
+constexpr int64_t SCHUR_TOP_NNZ = int64_t(4) << 20;    // entries of its forward store at most
+constexpr int SCHUR_TOP_MIN_T = 3;      // a dense top of fewer levels saves no launches
+constexpr int SCHUR_TOP_CHUNK = 256;    // reduction length of one dense item
 constexpr int SCHUR_NARROW = 32;       // forward: the widest supernode of 64-row tiles
 constexpr int SCHUR_TILE = 16;         // rows (forward, wide) or columns (backward) of a split tile
 
@@ -126,6 +161,14 @@ struct Schur {
     DevBuf<double> panel;
     DevBuf<zd> ubuf;                    // update vectors of one forward solve (cap right-hand sides)
     DevBuf<zd> zp, xp;                  // forward and backward solutions in the new numbering
+    // the dense top: its index arrays, M's forward (row) and backward (column) stores, its items, partial sums, and
+    // g for the batch
+    DevBuf<int> top_col, top_lo, top_cptr, top_clen, top_chain;
+    DevBuf<int64_t> top_foff, top_boff, top_gptr, top_gidx;
+    DevBuf<double> top_Mf, top_Mb;
+    DevBuf<TopItem> top_fwd, top_ffin, top_bwd, top_bfin;
+    DevBuf<zd> top_part, gbuf;
+    DevBuf<SnItem> tfwd, tffin;
     int cap = 0;                        // right-hand sides ubuf / zp / xp hold (indexed with the batch's stride)
 
     void upload(hipStream_t s);
@@ -140,6 +183,8 @@ struct Schur {
     void factor_device(hipStream_t s);
     // the device fronts' panels copied into h.panel (for solve_host)
     void fetch_panels(hipStream_t s);
+    // M of the dense top (after the panels are final): forward solves of unit vectors through tfwd, 16 at a time
+    void top_device(hipStream_t s);
     int64_t device_bytes() const;
 };
 

@@ -403,6 +403,27 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
         dmax = std::max(dmax, dep[sn]);
     }
     levels = ns ? hmax + 1 : 0;
+    // ---- the dense top: the deepest top_T within the budgets (0 if below SCHUR_TOP_MIN_T)
+    top_T = 0;
+    {
+        std::vector<int64_t> sub(ns), chain(ns);
+        for (int T = dmax + 1; T >= SCHUR_TOP_MIN_T && ns; --T) {
+            int64_t ncol = 0, fnnz = 0;
+            for (int sn = 0; sn < ns; ++sn) {           // postorder: a supernode's top subtree's columns
+                if (dep[sn] >= T) continue;
+                sub[sn] = nc[sn];
+                for (int e = child_ptr[sn]; e < child_ptr[sn + 1]; ++e)
+                    if (dep[child[e]] < T) sub[sn] += sub[child[e]];
+                ncol += nc[sn];
+                fnnz += (int64_t)nc[sn] * (sub[sn] - nc[sn]) + (int64_t)nc[sn] * (nc[sn] + 1) / 2;
+            }
+            if (ncol <= SCHUR_TOP_COLS && fnnz <= SCHUR_TOP_NNZ) {
+                top_T = T;
+                break;
+            }
+        }
+        (void)chain;
+    }
     ctab.assign(2 * (size_t)fpos[ns], -1);
     for (int64_t q = 0; q < fpos[ns]; ++q) {
         const int64_t k = cptr[q + 1] - cptr[q];
@@ -447,25 +468,34 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
         }
         return std::min(step, hi - lo);
     };
-    fwd_ptr.assign(1, 0);
-    fwd.clear();
-    fwd_cols.clear();
-    ffin_ptr.assign(1, 0);
-    ffin.clear();
-    for (int l = 0; l <= hmax && ns; ++l) {
-        int cols = SCHUR_TILE;          // the 16-row cross-wave reduction reuses the staging buffer
-        for (int sn = 0; sn < ns; ++sn) {
-            if (h[sn] != l) continue;
-            const int m = nc[sn] + nr[sn], narrow = nc[sn] <= SCHUR_NARROW;
-            const int tile = narrow ? 64 : SCHUR_TILE, step = SCHUR_PV * (narrow ? 4 : 16);
-            for (int r0 = 0; r0 < m; r0 += tile) {
-                const int qend = std::min(nc[sn], std::min(m, r0 + tile));  // zeros above L^-1 beyond
-                cols = std::max(cols, tile_items(fwd, ffin, sn, r0, narrow ? 1 : 4, 0, qend, step, true));
+    // forward levels by height; the top supernodes' items apart (tfwd: used only to make M)
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool top = pass == 1;
+        std::vector<int>& ptr = top ? tfwd_ptr : fwd_ptr;
+        std::vector<int>& colsv = top ? tfwd_cols : fwd_cols;
+        std::vector<int>& fptr = top ? tffin_ptr : ffin_ptr;
+        std::vector<SnItem>& items = top ? tfwd : fwd;
+        std::vector<SnItem>& fins = top ? tffin : ffin;
+        ptr.assign(1, 0);
+        items.clear();
+        colsv.clear();
+        fptr.assign(1, 0);
+        fins.clear();
+        for (int l = 0; l <= hmax && ns; ++l) {
+            int cols = SCHUR_TILE;          // the 16-row cross-wave reduction reuses the staging buffer
+            for (int sn = 0; sn < ns; ++sn) {
+                if (h[sn] != l || (dep[sn] < top_T) != top) continue;
+                const int m = nc[sn] + nr[sn], narrow = nc[sn] <= SCHUR_NARROW;
+                const int tile = narrow ? 64 : SCHUR_TILE, step = SCHUR_PV * (narrow ? 4 : 16);
+                for (int r0 = 0; r0 < m; r0 += tile) {
+                    const int qend = std::min(nc[sn], std::min(m, r0 + tile));  // zeros above L^-1 beyond
+                    cols = std::max(cols, tile_items(items, fins, sn, r0, narrow ? 1 : 4, 0, qend, step, true));
+                }
             }
+            ptr.push_back((int)items.size());
+            fptr.push_back((int)fins.size());
+            colsv.push_back(cols);
         }
-        fwd_ptr.push_back((int)fwd.size());
-        ffin_ptr.push_back((int)ffin.size());
-        fwd_cols.push_back(cols);
     }
     bwd_ptr.assign(1, 0);
     bwd.clear();
@@ -475,7 +505,7 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     for (int l = 0; l <= dmax && ns; ++l) {
         int rws = 1;
         for (int sn = 0; sn < ns; ++sn) {
-            if (dep[sn] != l) continue;
+            if (dep[sn] != l || dep[sn] < top_T) continue;
             const int m = nc[sn] + nr[sn];
             const bool wide_tile = nc[sn] > 16 && nc[sn] <= 32 && m <= SCHUR_PV * 8;
             const int rp = wide_tile ? 8 : 16, tile = wide_tile ? 32 : SCHUR_TILE;
@@ -486,7 +516,135 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
         bfin_ptr.push_back((int)bfin.size());
         bwd_rows.push_back(rws);
     }
+    build_top(dep);
     (void)roots;
+}
+
+void SchurHost::build_top(const std::vector<int>& dep) {
+    const int ns = (int)c0.size();
+    top_col.clear();
+    top_lo.clear();
+    top_foff.clear();
+    top_cptr.clear();
+    top_clen.clear();
+    top_chain.clear();
+    top_boff.clear();
+    top_gptr.assign(1, 0);
+    top_gidx.clear();
+    top_fwd.clear();
+    top_ffin.clear();
+    top_bwd.clear();
+    top_bfin.clear();
+    top_slots = 0;
+    top_fnnz = top_bnnz = 0;
+    if (top_T == 0) return;
+    // top columns, ascending (supernodes in postorder hold ascending columns); each supernode's first index
+    std::vector<int> own(ns, -1), lo(ns, -1);
+    std::vector<int> idx_of(n, -1);
+    for (int sn = 0; sn < ns; ++sn) {
+        if (dep[sn] >= top_T) continue;
+        own[sn] = (int)top_col.size();
+        for (int q = 0; q < nc[sn]; ++q) {
+            idx_of[c0[sn] + q] = (int)top_col.size();
+            top_col.push_back(c0[sn] + q);
+        }
+    }
+    const int nt = (int)top_col.size();
+    // a supernode's top subtree is contiguous in top-column order (postorder), ending at its own columns
+    for (int sn = 0; sn < ns; ++sn) {
+        if (dep[sn] >= top_T) continue;
+        lo[sn] = own[sn];
+        for (int e = child_ptr[sn]; e < child_ptr[sn + 1]; ++e)
+            if (dep[child[e]] < top_T) lo[sn] = std::min(lo[sn], lo[child[e]]);
+    }
+    std::vector<int> sn_of(nt);
+    for (int sn = 0; sn < ns; ++sn)
+        if (dep[sn] < top_T)
+            for (int q = 0; q < nc[sn]; ++q) sn_of[own[sn] + q] = sn;
+    top_lo.resize(nt);
+    top_foff.resize(nt);
+    for (int i = 0; i < nt; ++i) {
+        top_lo[i] = lo[sn_of[i]];
+        top_foff[i] = top_fnnz;
+        top_fnnz += i - top_lo[i] + 1;
+    }
+    // chains: a supernode's columns, then its top ancestors' (ascending, as postorder puts ancestors later)
+    std::vector<int> cptr_sn(ns, -1);
+    for (int sn = 0; sn < ns; ++sn) {
+        if (dep[sn] >= top_T) continue;
+        cptr_sn[sn] = (int)top_chain.size();
+        for (int a = sn; a >= 0; a = parent[a])
+            for (int q = 0; q < nc[a]; ++q) top_chain.push_back(c0[a] + q);
+    }
+    top_cptr.resize(nt);
+    top_clen.resize(nt);
+    top_boff.resize(nt);
+    for (int i = 0; i < nt; ++i) {
+        const int sn = sn_of[i];
+        int len = 0;
+        for (int a = sn; a >= 0; a = parent[a]) len += nc[a];
+        top_cptr[i] = cptr_sn[sn];
+        top_clen[i] = len;
+        top_boff[i] = top_bnnz;
+        top_bnnz += len;
+    }
+    // g: the update entries of non-top children at every front position of a top supernode, by the top column the
+    // position stands for (its own column, or an ancestor's among its off-diagonal rows)
+    std::vector<std::vector<int64_t>> g(nt);
+    for (int sn = 0; sn < ns; ++sn) {
+        if (dep[sn] >= top_T) continue;
+        const int m = nc[sn] + nr[sn];
+        for (int q = 0; q < m; ++q) {
+            const int col = q < nc[sn] ? c0[sn] + q : rows[roff[sn] + q - nc[sn]];
+            const int i = idx_of[col];
+            if (i < 0) fail(GT_ERR_NUMERIC, "dense top: a front row outside the top");
+            const int64_t pos = fpos[sn] + q;
+            for (int64_t e = cptr[pos]; e < cptr[pos + 1]; ++e) {
+                const int64_t u = cidx[e];
+                // the child owning update entry u: a child's entries are [uoff, uoff + nr)
+                int owner = -1;
+                for (int c = child_ptr[sn]; c < child_ptr[sn + 1]; ++c) {
+                    const int ch = child[c];
+                    if (u >= uoff[ch] && u < uoff[ch] + nr[ch]) { owner = ch; break; }
+                }
+                if (owner < 0) fail(GT_ERR_NUMERIC, "dense top: an update entry of no child");
+                if (dep[owner] >= top_T) g[i].push_back(u);
+            }
+        }
+    }
+    for (int i = 0; i < nt; ++i) {
+        std::sort(g[i].begin(), g[i].end());
+        top_gidx.insert(top_gidx.end(), g[i].begin(), g[i].end());
+        top_gptr.push_back((int64_t)top_gidx.size());
+    }
+    // items: tiles of 16 rows (forward) or columns (backward) of one supernode, reductions in chunks of
+    // SCHUR_TOP_CHUNK
+    auto items = [&](std::vector<TopItem>& out, std::vector<TopItem>& fin, int i0, int cnt, int k0, int k1) {
+        const int ks = std::max(1, (k1 - k0 + SCHUR_TOP_CHUNK - 1) / SCHUR_TOP_CHUNK);
+        for (int k = 0; k < ks; ++k) {
+            TopItem it{};
+            it.i0 = i0;
+            it.cnt = cnt;
+            it.k0 = k0 + k * SCHUR_TOP_CHUNK;
+            it.k1 = std::min(k1, k0 + (k + 1) * SCHUR_TOP_CHUNK);
+            it.slot0 = ks > 1 ? (int32_t)top_slots : -1;
+            it.kidx = k;
+            it.ks = ks;
+            out.push_back(it);
+        }
+        if (ks > 1) {
+            fin.push_back(out.back());
+            top_slots += ks;
+        }
+    };
+    for (int sn = 0; sn < ns; ++sn) {
+        if (dep[sn] >= top_T) continue;
+        for (int r0 = 0; r0 < nc[sn]; r0 += SCHUR_TILE) {
+            const int i0 = own[sn] + r0, cnt = std::min(SCHUR_TILE, nc[sn] - r0);
+            items(top_fwd, top_ffin, i0, cnt, lo[sn], i0 + cnt);              // M[i][lo .. i]
+            items(top_bwd, top_bfin, i0, cnt, 0, top_clen[i0]);               // M[chain][i]
+        }
+    }
 }
 
 void SchurHost::solve_host(const double* b, double* x) const {

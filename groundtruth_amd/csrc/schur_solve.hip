@@ -292,29 +292,255 @@ __global__ __launch_bounds__(NTH) void schur_fin(SolveArgs a, int item0, int nr,
     }
 }
 
+// ---- the dense top of the tree
+struct TopArgs {
+    const TopItem* item;
+    const int* perm;
+    const int* col;
+    const int* lo;
+    const int* cptr;
+    const int* clen;
+    const int* chain;
+    const int64_t* foff;
+    const int64_t* boff;
+    const int64_t* gptr;
+    const int64_t* gidx;
+    const double* Mf;
+    const double* Mb;
+    const zd* b;
+    zd* x;
+    zd* zp;
+    zd* xp;
+    const zd* ubuf;
+    zd* g;
+    zd* part;
+};
+
+constexpr int TOP_K = 64;            // reduction entries staged in LDS at a time
+
+// g[i] = b at top column i plus the update entries its non-top children send up, in ascending order
+__global__ __launch_bounds__(NTH) void schur_top_g(TopArgs a, int nt, int nr) {
+    const int e = blockIdx.x * NTH + threadIdx.x;
+    const int i = e / nr, p = e - i * nr;
+    if (i >= nt) return;
+    zd acc = a.b[(size_t)a.perm[a.col[i]] * nr + p];
+    for (int64_t q = a.gptr[i]; q < a.gptr[i + 1]; ++q) acc = acc + a.ubuf[(size_t)a.gidx[q] * nr + p];
+    a.g[(size_t)i * nr + p] = acc;
+}
+
+// Forward: z at top rows [i0, i0 + cnt) = M[rows][k0 .. k1) g[k0 .. k1); thread (row r, right-hand side p). g and
+// M go through LDS TOP_K entries at a time, double-buffered: the next chunk's loads are in flight while this one's
+// multiply-adds run. The sum over k runs in one fixed order for every batch.
+__global__ __launch_bounds__(NTH) void schur_top_fwd(TopArgs a, int nr) {
+    __shared__ double gs_raw[2][TOP_K][KMAX + 1][2];
+    __shared__ double ms[2][SCHUR_TILE][TOP_K + 1];
+    __shared__ int64_t offs[SCHUR_TILE];
+    const TopItem it = a.item[blockIdx.x];
+    const int r = threadIdx.x >> 4, p = threadIdx.x & 15;
+    const int i = it.i0 + r, lo = a.lo[it.i0];
+    if (threadIdx.x < SCHUR_TILE) offs[threadIdx.x] = threadIdx.x < it.cnt ? a.foff[it.i0 + threadIdx.x] : 0;
+    __syncthreads();
+    constexpr int GPT = TOP_K * KMAX / NTH, MPT = SCHUR_TILE * TOP_K / NTH;   // staged entries per thread
+    zd gl[GPT];
+    double ml[MPT];
+    auto load = [&](int k0) {
+        const int len = min(TOP_K, it.k1 - k0);
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+            const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
+            gl[u] = q < nr && kk < len ? a.g[(size_t)(k0 + kk) * nr + q] : zd();
+        }
+#pragma unroll
+        for (int u = 0; u < MPT; ++u) {
+            const int e = threadIdx.x + u * NTH, rr = e / TOP_K, kk = e - rr * TOP_K, k = k0 + kk;
+            ml[u] = rr < it.cnt && kk < len && k <= it.i0 + rr ? a.Mf[offs[rr] + (k - lo)] : 0.0;
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+            const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
+            gs_raw[b][kk][q][0] = gl[u].re;
+            gs_raw[b][kk][q][1] = gl[u].im;
+        }
+#pragma unroll
+        for (int u = 0; u < MPT; ++u) {
+            const int e = threadIdx.x + u * NTH, rr = e / TOP_K, kk = e - rr * TOP_K;
+            ms[b][rr][kk] = ml[u];
+        }
+    };
+    double are = 0.0, aim = 0.0;
+    load(it.k0);
+    store(0);
+    __syncthreads();
+    int b = 0;
+    for (int k0 = it.k0; k0 < it.k1; k0 += TOP_K) {
+        const bool more = k0 + TOP_K < it.k1;
+        if (more) load(k0 + TOP_K);
+        const int len = min(TOP_K, it.k1 - k0);
+        for (int kk = 0; kk < len; ++kk) {
+            const double m = ms[b][r][kk];
+            are = fma(m, gs_raw[b][kk][p][0], are);
+            aim = fma(m, gs_raw[b][kk][p][1], aim);
+        }
+        if (more) store(b ^ 1);
+        __syncthreads();
+        b ^= 1;
+    }
+    if (r >= it.cnt || p >= nr) return;
+    if (it.slot0 < 0) a.zp[(size_t)a.col[i] * nr + p] = zd(are, aim);
+    else a.part[(size_t)(it.slot0 + it.kidx) * SLOT + r * KMAX + p] = zd(are, aim);
+}
+
+// Backward: x at top columns [i0, i0 + cnt) = M[chain[k0 .. k1)][cols]^T z[chain[k0 .. k1)]; thread (column c, p);
+// the chain's rows as columns of zp (ccol), staged like the forward's
+__global__ __launch_bounds__(NTH) void schur_top_bwd(TopArgs a, int nr) {
+    __shared__ double zs_raw[2][TOP_K][KMAX + 1][2];
+    __shared__ double ms[2][SCHUR_TILE][TOP_K + 1];
+    __shared__ int64_t offs[SCHUR_TILE];
+    const TopItem it = a.item[blockIdx.x];
+    const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
+    const int i = it.i0 + c;
+    const int* ccol = a.chain + a.cptr[it.i0];
+    if (threadIdx.x < SCHUR_TILE) offs[threadIdx.x] = threadIdx.x < it.cnt ? a.boff[it.i0 + threadIdx.x] : 0;
+    __syncthreads();
+    constexpr int GPT = TOP_K * KMAX / NTH, MPT = SCHUR_TILE * TOP_K / NTH;
+    zd zl[GPT];
+    double ml[MPT];
+    auto load = [&](int k0) {
+        const int len = min(TOP_K, it.k1 - k0);
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+            const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
+            zl[u] = q < nr && kk < len ? a.zp[(size_t)ccol[k0 + kk] * nr + q] : zd();
+        }
+#pragma unroll
+        for (int u = 0; u < MPT; ++u) {
+            const int e = threadIdx.x + u * NTH, cc = e / TOP_K, kk = e - cc * TOP_K;
+            ml[u] = cc < it.cnt && kk < len ? a.Mb[offs[cc] + k0 + kk] : 0.0;
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+            const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
+            zs_raw[b][kk][q][0] = zl[u].re;
+            zs_raw[b][kk][q][1] = zl[u].im;
+        }
+#pragma unroll
+        for (int u = 0; u < MPT; ++u) {
+            const int e = threadIdx.x + u * NTH, cc = e / TOP_K, kk = e - cc * TOP_K;
+            ms[b][cc][kk] = ml[u];
+        }
+    };
+    double are = 0.0, aim = 0.0;
+    load(it.k0);
+    store(0);
+    __syncthreads();
+    int b = 0;
+    for (int k0 = it.k0; k0 < it.k1; k0 += TOP_K) {
+        const bool more = k0 + TOP_K < it.k1;
+        if (more) load(k0 + TOP_K);
+        const int len = min(TOP_K, it.k1 - k0);
+        for (int kk = 0; kk < len; ++kk) {
+            const double m = ms[b][c][kk];
+            are = fma(m, zs_raw[b][kk][p][0], are);
+            aim = fma(m, zs_raw[b][kk][p][1], aim);
+        }
+        if (more) store(b ^ 1);
+        __syncthreads();
+        b ^= 1;
+    }
+    if (c >= it.cnt || p >= nr) return;
+    if (it.slot0 < 0) {
+        a.xp[(size_t)a.col[i] * nr + p] = zd(are, aim);
+        a.x[(size_t)a.perm[a.col[i]] * nr + p] = zd(are, aim);
+    } else {
+        a.part[(size_t)(it.slot0 + it.kidx) * SLOT + c * KMAX + p] = zd(are, aim);
+    }
+}
+
+// The split dense tiles: partial sums added in chunk order, then written as the forward (z) or backward (x) result
+__global__ __launch_bounds__(NTH) void schur_top_fin(TopArgs a, int nr, int forward) {
+    const TopItem it = a.item[blockIdx.x];
+    const int r = threadIdx.x >> 4, p = threadIdx.x & 15;
+    if (r >= it.cnt || p >= nr) return;
+    const zd* s0 = a.part + (size_t)it.slot0 * SLOT + r * KMAX + p;
+    zd acc = s0[0];
+    for (int k = 1; k < it.ks; ++k) acc = acc + s0[(size_t)k * SLOT];
+    const int col = a.col[it.i0 + r];
+    if (forward) {
+        a.zp[(size_t)col * nr + p] = acc;
+    } else {
+        a.xp[(size_t)col * nr + p] = acc;
+        a.x[(size_t)a.perm[col] * nr + p] = acc;
+    }
+}
+
+TopArgs top_args(const Schur& S, const zd* b, zd* x) {
+    return TopArgs{nullptr, S.perm.p, S.top_col.p, S.top_lo.p, S.top_cptr.p, S.top_clen.p, S.top_chain.p,
+                   S.top_foff.p, S.top_boff.p, S.top_gptr.p, S.top_gidx.p, S.top_Mf.p, S.top_Mb.p, b, x, S.zp.p,
+                   S.xp.p, S.ubuf.p, S.gbuf.p, S.top_part.p};
+}
+
+void run_top_fwd(const Schur& S, const zd* b, int nr, hipStream_t st) {
+    const int nt = (int)S.h.top_col.size();
+    TopArgs a = top_args(S, b, nullptr);
+    hipLaunchKernelGGL(schur_top_g, dim3(blocks_for((size_t)nt * nr, NTH)), dim3(NTH), 0, st, a, nt, nr);
+    GT_LAUNCH("schur_top_g");
+    a.item = S.top_fwd.p;
+    hipLaunchKernelGGL(schur_top_fwd, dim3((unsigned)S.h.top_fwd.size()), dim3(NTH), 0, st, a, nr);
+    GT_LAUNCH("schur_top_fwd");
+    if (!S.h.top_ffin.empty()) {
+        a.item = S.top_ffin.p;
+        hipLaunchKernelGGL(schur_top_fin, dim3((unsigned)S.h.top_ffin.size()), dim3(NTH), 0, st, a, nr, 1);
+        GT_LAUNCH("schur_top_fin");
+    }
+}
+
+void run_top_bwd(const Schur& S, zd* x, int nr, hipStream_t st) {
+    TopArgs a = top_args(S, nullptr, x);
+    a.item = S.top_bwd.p;
+    hipLaunchKernelGGL(schur_top_bwd, dim3((unsigned)S.h.top_bwd.size()), dim3(NTH), 0, st, a, nr);
+    GT_LAUNCH("schur_top_bwd");
+    if (!S.h.top_bfin.empty()) {
+        a.item = S.top_bfin.p;
+        hipLaunchKernelGGL(schur_top_fin, dim3((unsigned)S.h.top_bfin.size()), dim3(NTH), 0, st, a, nr, 0);
+        GT_LAUNCH("schur_top_fin");
+    }
+}
+
+// the forward levels of `ptr` / `items` (the tree below the dense top, or the top's own supernodes)
 template <int KB>
-void run_fwd(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
+void run_fwd_items(const Schur& S, SolveArgs a, int nr, hipStream_t st, const std::vector<int>& fp,
+                   const std::vector<int>& cols, const std::vector<int>& finp, const SnItem* items, const SnItem* fins) {
     constexpr size_t row_bytes = (KB + 1) * sizeof(zd);
-    const std::vector<int>& fp = S.h.fwd_ptr;
-    a.item = S.fwd.p;
+    a.item = items;
     for (size_t l = 0; l + 1 < fp.size(); ++l) {
         const int cnt = fp[l + 1] - fp[l];
         if (cnt == 0) continue;
-        hipLaunchKernelGGL(schur_fwd<KB>, dim3(cnt), dim3(NTH), S.h.fwd_cols[l] * row_bytes, st, a, fp[l], nr);
+        hipLaunchKernelGGL(schur_fwd<KB>, dim3(cnt), dim3(NTH), cols[l] * row_bytes, st, a, fp[l], nr);
         GT_LAUNCH("schur_fwd");
-        const int nf = S.h.ffin_ptr[l + 1] - S.h.ffin_ptr[l];
+        const int nf = finp[l + 1] - finp[l];
         if (nf > 0) {
             SolveArgs af = a;
-            af.item = S.ffin.p;
-            hipLaunchKernelGGL(schur_fin, dim3(nf), dim3(NTH), 0, st, af, S.h.ffin_ptr[l], nr, 1);
+            af.item = fins;
+            hipLaunchKernelGGL(schur_fin, dim3(nf), dim3(NTH), 0, st, af, finp[l], nr, 1);
             GT_LAUNCH("schur_fin");
         }
     }
 }
 
 template <int KB>
+void run_fwd(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
+    run_fwd_items<KB>(S, a, nr, st, S.h.fwd_ptr, S.h.fwd_cols, S.h.ffin_ptr, S.fwd.p, S.ffin.p);
+    if (S.h.top_T) run_top_fwd(S, a.b, nr, st);
+}
+
+template <int KB>
 void run_bwd(const Schur& S, SolveArgs a, int nr, hipStream_t st) {
     constexpr size_t row_bytes = (KB + 1) * sizeof(zd);
+    if (S.h.top_T) run_top_bwd(S, a.x, nr, st);
     const std::vector<int>& bp = S.h.bwd_ptr;
     a.item = S.bwd.p;
     for (size_t l = 0; l + 1 < bp.size(); ++l) {
@@ -346,6 +572,24 @@ void Schur::upload(hipStream_t st) {
     part.alloc((size_t)std::max<int64_t>(1, h.slots) * SCHUR_TILE * KMAX);
     ffin.upload(h.ffin.empty() ? std::vector<SnItem>(1) : h.ffin, st);
     bfin.upload(h.bfin.empty() ? std::vector<SnItem>(1) : h.bfin, st);
+    if (h.top_T) {
+        top_col.upload(h.top_col, st);
+        top_lo.upload(h.top_lo, st);
+        top_cptr.upload(h.top_cptr, st);
+        top_clen.upload(h.top_clen, st);
+        top_chain.upload(h.top_chain, st);
+        top_foff.upload(h.top_foff, st);
+        top_boff.upload(h.top_boff, st);
+        top_gptr.upload(h.top_gptr, st);
+        top_gidx.upload(h.top_gidx.empty() ? std::vector<int64_t>{0} : h.top_gidx, st);
+        top_fwd.upload(h.top_fwd, st);
+        top_ffin.upload(h.top_ffin.empty() ? std::vector<TopItem>(1) : h.top_ffin, st);
+        top_bwd.upload(h.top_bwd, st);
+        top_bfin.upload(h.top_bfin.empty() ? std::vector<TopItem>(1) : h.top_bfin, st);
+        top_part.alloc((size_t)std::max<int64_t>(1, h.top_slots) * SCHUR_TILE * KMAX);
+        tfwd.upload(h.tfwd.empty() ? std::vector<SnItem>(1) : h.tfwd, st);
+        tffin.upload(h.tffin.empty() ? std::vector<SnItem>(1) : h.tffin, st);
+    }
     cap = 0;
     ensure(1);
     GT_HIP(hipStreamSynchronize(st));
@@ -359,9 +603,11 @@ void Schur::ensure(int nr) {
     ubuf.release();
     zp.release();
     xp.release();
+    gbuf.release();
     ubuf.alloc((size_t)std::max<int64_t>(1, h.upd_total) * nr);
     zp.alloc((size_t)std::max(1, h.n) * nr);
     xp.alloc((size_t)std::max(1, h.n) * nr);
+    if (h.top_T) gbuf.alloc(h.top_col.size() * (size_t)nr);
     cap = nr;
 }
 
@@ -373,12 +619,17 @@ void Schur::shrink(int nr) {
 
 int64_t Schur::bytes_after(int nr) const {
     const int c = std::max(cap, nr);
-    return (int64_t)sizeof(zd) * c * (std::max<int64_t>(1, h.upd_total) + 2 * (int64_t)std::max(1, h.n));
+    return (int64_t)sizeof(zd) * c *
+           (std::max<int64_t>(1, h.upd_total) + 2 * (int64_t)std::max(1, h.n) + (h.top_T ? (int64_t)h.top_col.size() : 0));
 }
 
 int64_t Schur::device_bytes() const {
     return (int64_t)(perm.bytes() + rows.bytes() + ctab.bytes() + cidx.bytes() + fwd.bytes() + bwd.bytes() +
-                     panel.bytes() + part.bytes() + ffin.bytes() + bfin.bytes() + ubuf.bytes() + zp.bytes() + xp.bytes());
+                     panel.bytes() + part.bytes() + ffin.bytes() + bfin.bytes() + ubuf.bytes() + zp.bytes() + xp.bytes() +
+                     top_col.bytes() + top_lo.bytes() + top_cptr.bytes() + top_clen.bytes() + top_chain.bytes() +
+                     top_foff.bytes() + top_boff.bytes() + top_gptr.bytes() + top_gidx.bytes() + top_Mf.bytes() +
+                     top_Mb.bytes() + top_fwd.bytes() + top_ffin.bytes() + top_bwd.bytes() + top_bfin.bytes() +
+                     top_part.bytes() + gbuf.bytes() + tfwd.bytes() + tffin.bytes());
 }
 
 namespace {
@@ -451,6 +702,72 @@ void schur_backward_device(Schur& S, zd* x, int nr, hipStream_t st) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
     if (nr > S.cap) fail(GT_ERR_ARG, "Schur backward pass without its forward pass");
     launch_passes(S, nullptr, x, nr, st, 2);
+}
+
+namespace {
+
+// unit right-hand sides for M's columns k0 .. k0 + nb of the dense top (16 per batch, old numbering); the previous
+// batch's units cleared first (kernels, not memsets)
+__global__ __launch_bounds__(64) void top_units(zd* b, const int* perm, const int* col, int k_prev, int nb_prev, int k0,
+                                                int nb) {
+    const int p = threadIdx.x;
+    if (p < nb_prev) b[(size_t)perm[col[k_prev + p]] * KMAX + p] = zd();
+    __syncthreads();
+    if (p < nb) b[(size_t)perm[col[k0 + p]] * KMAX + p] = zd(1.0);
+}
+
+__global__ __launch_bounds__(NTH) void top_zero(zd* v, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)NTH + threadIdx.x; e < n; e += (size_t)gridDim.x * NTH) v[e] = zd();
+}
+
+// M[i][k0 + p] = z at top column i for unit right-hand side p (real: the panels and units are)
+__global__ __launch_bounds__(NTH) void top_extract(double* M, const zd* zp, const int* col, int nt, int k0, int nb) {
+    const int e = blockIdx.x * NTH + threadIdx.x;
+    const int i = e / KMAX, p = e - i * KMAX;
+    if (i >= nt || p >= nb) return;
+    M[(size_t)i * nt + k0 + p] = zp[(size_t)col[i] * KMAX + p].re;
+}
+
+}  // namespace
+
+void Schur::top_device(hipStream_t st) {
+    if (!h.top_T) return;
+    const int nt = (int)h.top_col.size();
+    ensure(KMAX);
+    DevBuf<zd> bu((size_t)std::max(1, h.n) * KMAX);
+    DevBuf<double> Mfull((size_t)nt * nt);
+    const unsigned zb = 1024;
+    hipLaunchKernelGGL(top_zero, dim3(zb), dim3(NTH), 0, st, bu.p, bu.n);
+    hipLaunchKernelGGL(top_zero, dim3(zb), dim3(NTH), 0, st, ubuf.p, ubuf.n);     // the non-top children's entries
+    GT_LAUNCH("top_zero");
+    SolveArgs a{nullptr, perm.p, rows.p, reinterpret_cast<const int2*>(ctab.p), cidx.p, panel.p, ubuf.p, zp.p, xp.p,
+                bu.p, nullptr, part.p};
+    int k_prev = 0, nb_prev = 0;
+    for (int k0 = 0; k0 < nt; k0 += KMAX) {
+        const int nb = std::min(KMAX, nt - k0);
+        hipLaunchKernelGGL(top_units, dim3(1), dim3(64), 0, st, bu.p, perm.p, top_col.p, k_prev, nb_prev, k0, nb);
+        GT_LAUNCH("top_units");
+        run_fwd_items<KMAX>(*this, a, KMAX, st, h.tfwd_ptr, h.tfwd_cols, h.tffin_ptr, tfwd.p, tffin.p);
+        hipLaunchKernelGGL(top_extract, dim3(blocks_for((size_t)nt * KMAX, NTH)), dim3(NTH), 0, st, Mfull.p, zp.p,
+                           top_col.p, nt, k0, nb);
+        GT_LAUNCH("top_extract");
+        k_prev = k0;
+        nb_prev = nb;
+    }
+    std::vector<double> M((size_t)nt * nt);
+    GT_HIP(hipMemcpyAsync(M.data(), Mfull.p, M.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    GT_HIP(hipStreamSynchronize(st));
+    std::vector<double> mf(h.top_fnnz), mb(h.top_bnnz);
+    std::vector<int> idx(std::max(1, h.n), -1);               // new column -> top-column index
+    for (int i = 0; i < nt; ++i) idx[h.top_col[i]] = i;
+    for (int i = 0; i < nt; ++i) {
+        for (int k = h.top_lo[i]; k <= i; ++k) mf[h.top_foff[i] + (k - h.top_lo[i])] = M[(size_t)i * nt + k];
+        const int* ch = h.top_chain.data() + h.top_cptr[i];
+        for (int e = 0; e < h.top_clen[i]; ++e) mb[h.top_boff[i] + e] = M[(size_t)idx[ch[e]] * nt + i];
+    }
+    top_Mf.upload(mf, st);
+    top_Mb.upload(mb, st);
+    shrink(1);
 }
 
 Schur::~Schur() {

@@ -588,7 +588,11 @@ class PlaneStack:
         nnz, nsup, lev, mc = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().gt_schur_stats(self.handle, C.byref(nnz), C.byref(nsup), C.byref(lev),
                                              C.byref(mc)), "gt_schur_stats")
-        return {"nnz_L": nnz.value, "supernodes": nsup.value, "levels": lev.value, "max_cols": mc.value}
+        T, cols, ent = C.c_int32(), C.c_int32(), C.c_int64()
+        _lib.check(_lib.lib().gt_schur_top_stats(self.handle, C.byref(T), C.byref(cols), C.byref(ent)),
+                   "gt_schur_top_stats")
+        return {"nnz_L": nnz.value, "supernodes": nsup.value, "levels": lev.value, "max_cols": mc.value,
+                "dense_top": {"levels": T.value, "cols": cols.value, "entries": ent.value}}
 
     # ---------------------------------------------------------------- operators (peec.py:2227-2345)
     @staticmethod

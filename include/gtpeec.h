@@ -259,6 +259,10 @@ int gt_stack_derive_bytes(const gt_stack* base, int32_t n_extra, const int32_t* 
 int gt_stack_create_like(gt_ctx* ctx, const gt_stack_desc* desc, const gt_stack* donor, gt_stack** out);
 /* Statistics of the factor: entries of L, supernodes, tree levels, the largest supernode's columns. */
 int gt_schur_stats(const gt_stack* stack, int64_t* nnz_L, int32_t* n_super, int32_t* n_levels, int32_t* max_cols);
+/* The dense top of the factor's tree (no counterpart in the reference, whose SuperLU solves have no levels): the tree
+ * levels solved as dense products with the explicit inverse of their block of L (0: none), their columns, and the
+ * inverse's stored entries (forward and backward stores). */
+int gt_schur_top_stats(const gt_stack* stack, int32_t* levels, int32_t* cols, int64_t* entries);
 /* B^-1 b on the host with the same factor (a check of the factorisation that needs no device). b, x: n_keep
  * complex values. */
 int gt_schur_solve_host(const gt_stack* stack, const double* b, double* x);

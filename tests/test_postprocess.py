@@ -99,7 +99,7 @@ def test_device_node_field_equals_the_host(name):
         hx, hy = sol.node_field(k, d_mm)
         for dev, host in ((kx[k], hx), (ky[k], hy)):
             assert np.array_equal(dev == 0, host == 0), k
-            assert np.max(np.abs(dev - host)) <= 2e-16 * np.max(np.abs(host)), k
+            assert np.max(np.abs(dev - host)) <= 1e-15 * np.max(np.abs(host)), k      # a few roundings apart
 
 
 @pytest.mark.parametrize("name", _golden.NAMES)

@@ -57,6 +57,9 @@ def test_full_two_layer_factor_on_the_device_matches_the_host_factor():
     rng = np.random.default_rng(12)
     s = rng.standard_normal(dev.n_keep) + 1j * rng.standard_normal(dev.n_keep)
     assert _rel(dev.schur_solve(s), host.schur_solve(s)) < 1e-11
+    # the top of the tree as dense products on the device, against the host's supernode-by-supernode solve
+    assert dev.schur_stats()["dense_top"]["levels"] >= 3
+    assert _rel(dev.schur_solve(s), dev.schur_solve_host(s)) < 1e-11
 
 
 @pytest.mark.gpu
