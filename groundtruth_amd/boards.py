@@ -23,7 +23,7 @@ from typing import Optional
 
 import numpy as np
 
-from .problem import Branch, Grid, PlaneLayer, TraceSegment, via_impedance
+from .problem import Branch, Grid, LidNode, PlaneLayer, TraceSegment, via_impedance
 
 CU_T = 35e-6
 DRILL = 0.3e-3
@@ -59,6 +59,26 @@ def _stitch(layers: list, i: int, j: int, name: str) -> list:
         length = abs(layers[a].z - layers[b].z)
         out.append(Branch((a, i, j), (b, i, j), via_impedance(length, DRILL), name=name, kind="via",
                           radius=DRILL / 2))
+    return out
+
+
+def decoupling_cap(layers: list, a: tuple, b: tuple, lid: int, C: float = 100e-9, esr: float = 5e-3,
+                   esl: float = 0.5e-9, pad: int = 2, name: str = "decap") -> list:
+    """A capacitor between two planes as the reference models a part (peec.py:1009-1038): a lid (LidNode) over cell
+    a = (layer, i, j) and one over b, each joined by `bond` branches to the copper nodes of a pad x pad block of
+    cells at its corner, and the part itself (ESR + jw ESL + 1/(jw C)) from lid to lid. Lids numbered lid, lid + 1."""
+    la, lb = LidNode(*a, lid), LidNode(*b, lid + 1)
+    out = []
+    for lnode, (k, i, j) in ((la, a), (lb, b)):
+        g = layers[k].grid.node
+        for di in range(pad):
+            for dj in range(pad):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < g.shape[0] and 0 <= jj < g.shape[1] and g[ii, jj]:
+                    out.append(Branch(lnode, (k, ii, jj), lambda f: complex(1e-4, 2 * math.pi * f * 5e-11),
+                                      name=f"{name}_bond", kind="bond"))
+    out.append(Branch(la, lb, lambda f, C=C, R=esr, L=esl: complex(R, 2 * math.pi * f * L - 1 / (2 * math.pi * f * C)),
+                      name=name, kind="part"))
     return out
 
 

@@ -30,7 +30,7 @@ from scipy.sparse.csgraph import connected_components
 
 from . import _lib
 from . import greens as G
-from .problem import SIGMA_CU, DeviceTooSmall, Solution, SolverError
+from .problem import SIGMA_CU, DeviceTooSmall, LidNode, Solution, SolverError
 
 GMRES_EXTRA_CYCLES = 24         # peec.py:98
 GMRES_CHECK_MISSES = 4          # peec.py:100
@@ -235,13 +235,25 @@ class PlaneStack:
         at = np.nonzero(node)
         self.nid = -np.ones(node.shape, np.int64)
         self.nid[at] = np.arange(len(at[0]))
-        self.n_grid = self.n_nodes = len(at[0])
-        self.node_layer, self.node_i, self.node_j = (a.astype(np.int32) for a in at)
+        self.n_grid = len(at[0])
+        # the parts' lids (LidNode) numbered after the grid's nodes, in the order of their numbers (peec.py:1613-1616)
+        lids = {}
+        for b in branches:
+            for e in (b.a, b.b):
+                if isinstance(e, LidNode):
+                    lids.setdefault(e.lid, e)
+        order = sorted(lids)
+        self.lid_nid = {k: self.n_grid + n for n, k in enumerate(order)}
+        self.n_nodes = self.n_grid + len(order)
+        lid_cells = np.array([lids[k][:3] for k in order], np.int64).reshape(-1, 3)
+        self.node_layer, self.node_i, self.node_j = (np.concatenate([a, c]).astype(np.int32)
+                                                     for a, c in zip(at, lid_cells.T))
+        self.node_is_lid = np.concatenate([np.zeros(self.n_grid, bool), np.ones(len(order), bool)])
         self.xa, self.ya = np.argwhere(self.xb), np.argwhere(self.yb)
         self.nxb, self.nyb = len(self.xa), len(self.ya)
         self.ns = self.nxb + self.nyb
         self.branches = [b for b in branches if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0
-                         and tuple(b.a) != tuple(b.b)]
+                         and self.node_id(b.a) != self.node_id(b.b)]
         self.nbr = len(self.branches)
         self.ncap = 0
         self.nb = self.ns + self.nbr
@@ -280,7 +292,10 @@ class PlaneStack:
                 raise NotImplementedError("face sheets (screen=True with an inner plane) are not part of this library")
 
     def node_id(self, node) -> int:
-        """Node number of (layer, i, j), -1 off copper (peec.py:1864-1869)."""
+        """Node number of (layer, i, j), -1 off copper; of a LidNode its own number after the grid's, -1 when no
+        branch of this stack names it (peec.py:1864-1869)."""
+        if isinstance(node, LidNode):
+            return self.lid_nid.get(node.lid, -1)
         return int(self.nid[tuple(node)])
 
     def face_gap(self, s: int, t: int) -> float:
@@ -322,7 +337,9 @@ class PlaneStack:
         spectra and Schur factor; the added branches enter the preconditioner as Woodbury columns of the base's
         factor (peec.py:1871-1909). A branch that joins two islands changes the reference nodes: then a stack of its
         own."""
-        extra = [b for b in extra if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0 and tuple(b.a) != tuple(b.b)]
+        # (a branch to a lid this stack does not have joins nothing here: new lids need a stack of their own)
+        extra = [b for b in extra if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0
+                 and self.node_id(b.a) != self.node_id(b.b)]
         if not extra:
             return self
         a = np.array([self.node_id(b.a) for b in extra], np.int64)
@@ -508,7 +525,10 @@ class PlaneStack:
         _lib.check(L.gt_ctx_schur_ordering(self.ctx.handle, _ND_ORDERINGS[self.schur_ordering]),
                    "gt_ctx_schur_ordering")
         keep_idx = np.nonzero(self.keep)[0]
-        xyz = np.ascontiguousarray(np.stack([self.node_layer[keep_idx], self.node_i[keep_idx],
+        # the nested dissection's sheet of a node: its layer; a lid's a sheet of its own (its bonds then join sheets,
+        # so it and the nodes under its pads sit in the top separator with the vias' ends)
+        sheet = np.where(self.node_is_lid, self.n_sheets, self.node_layer)
+        xyz = np.ascontiguousarray(np.stack([sheet[keep_idx], self.node_i[keep_idx],
                                              self.node_j[keep_idx]], axis=1).astype(np.int32))
         xyz_p = _lib.ptr(xyz.reshape(-1) if xyz.size else np.zeros(3, np.int32), _lib.i32p)
         if self.per_sheet:
@@ -1028,6 +1048,8 @@ class _DeviceSolution(Solution):
     I_x = property(lambda self: self._unpacked().I_x)
     I_y = property(lambda self: self._unpacked().I_y)
     V = property(lambda self: self._unpacked().V)
+    V_lid = property(lambda self: self._unpacked().V_lid)
+    lid_at = property(lambda self: self._unpacked().lid_at)
 
 
 class _LazySolution(Solution):
@@ -1054,6 +1076,8 @@ class _LazySolution(Solution):
     I_y = property(lambda self: self._unpacked().I_y)
     V = property(lambda self: self._unpacked().V)
     m = property(lambda self: self._unpacked().m)
+    V_lid = property(lambda self: self._unpacked().V_lid)
+    lid_at = property(lambda self: self._unpacked().lid_at)
 
 
 def _dist_to_segment(P: np.ndarray, a: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -1225,4 +1249,7 @@ class RouteExcitation:
         Vg[np.nonzero(st.node)] = V[:st.n_grid]
         Z = self.z_loop(f, Ib, V)
         m = st.moment(self.trace, Ib) if moment else None
-        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, m, its, x)
+        # the parts' lids after the grid's nodes (peec.py:2807-2816)
+        lid_at = {k: v - st.n_grid for k, v in st.lid_nid.items()} if st.lid_nid else None
+        V_lid = V[st.n_grid:].copy() if st.lid_nid else None
+        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, m, its, x, V_lid=V_lid, lid_at=lid_at)

@@ -1,8 +1,9 @@
 """Inputs and outputs of the plane-current PEEC solve, with the reference's names and meanings.
 
-Grid / PlaneLayer / Branch / TraceSegment / Solution / SolverError mirror `src/groundtruth/solver/peec.py`
-(Grid :197-223, PlaneLayer :987-993, Branch :996-1006, TraceSegment :1051-1058, via_impedance :1061-1072,
-Solution :1092-1110, SolverError :124-127) so a caller of the reference's solver passes the same objects.
+Grid / PlaneLayer / Branch / LidNode / TraceSegment / Solution / SolverError mirror `src/groundtruth/solver/peec.py`
+(Grid :197-223, PlaneLayer :987-993, Branch :996-1006, LidNode :1009-1038, TraceSegment :1051-1058,
+via_impedance :1061-1072, track_impedance :1075-1089, Solution :1092-1110, SolverError :124-127) so a caller of the
+reference's solver passes the same objects.
 Only the fields the solve reads are kept on Grid (copper masks and pitch); rasterisation is not part of this path.
 """
 from __future__ import annotations
@@ -100,6 +101,44 @@ def via_impedance(length_m: float, drill_m: float, sigma: float = SIGMA_CU, fill
     return lambda f, R=R, L=L: complex(R, 2 * math.pi * f * L)
 
 
+class LidNode(tuple):
+    """A circuit node that is no grid node: the metal of one side of a lumped part (a capacitor's terminal), which
+    `bond` branches join to the grid nodes under its pads (peec.py:1009-1038). As a tuple it is the (layer, i, j) of
+    the cell it sits over (its position for the loop moment); `lid` numbers it within its model. Equality and hashing
+    go by the number, so two lids over one cell stay two nodes, and a grid node never equals a lid."""
+
+    def __new__(cls, layer: int, i: int, j: int, lid: int):
+        t = super().__new__(cls, (int(layer), int(i), int(j)))
+        t.lid = int(lid)
+        return t
+
+    def __getnewargs__(self):
+        return (self[0], self[1], self[2], self.lid)
+
+    def __eq__(self, other):
+        return isinstance(other, LidNode) and other.lid == self.lid and tuple.__eq__(self, other)
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
+
+    def __hash__(self):
+        return hash(("lid", self.lid))
+
+
+def track_impedance(length_m: float, width_m: float, thickness_m: float = 35e-6, sigma: float = SIGMA_CU
+                    ) -> Callable[[float], complex]:
+    """A piece of track as a lumped branch (a bridge across another net's cells, peec.py:1075-1089): its resistance
+    with the copper's effective thickness at f for the skin effect, plus jw its partial self inductance as a flat
+    strip (the filament mutual at the strip's geometric mean distance 0.2235 (w + t) [Grover 1946])."""
+    from .greens import skin_thickness, strip_self_gmd
+    length_m = max(length_m, 1e-6)
+    L = strip_self_gmd(length_m, width_m, thickness_m)
+
+    def z(f: float) -> complex:
+        return complex(length_m / (sigma * width_m * skin_thickness(thickness_m, f, sigma)), 2 * math.pi * f * L)
+    return z
+
+
 @dataclass
 class Solution:
     """One solve per amp of loop current (peec.py:1092-1110): strip currents, node potentials, branch currents,
@@ -115,6 +154,8 @@ class Solution:
     iterations: int
     x: Optional[np.ndarray] = field(default=None, repr=False)
     residual: float = float("nan")
+    V_lid: Optional[np.ndarray] = None      # the potentials of the stack's lids, in their numbers' order
+    lid_at: Optional[dict] = None           # a lid's number -> its place in V_lid
 
     def state_for(self, stack) -> np.ndarray:
         """The solver's unknown vector [branch currents; kept node potentials] of this solution on `stack`, the
@@ -124,7 +165,9 @@ class Solution:
             return self.x
         strips = np.concatenate([self.I_x[tuple(stack.xa.T)], self.I_y[tuple(stack.ya.T)]])
         Ib = np.concatenate([strips, np.asarray(self.I_branch, complex)])
-        return np.concatenate([Ib, self.V[np.nonzero(stack.node)][stack.keep]])
+        n_lid = stack.n_nodes - stack.n_grid
+        lids = self.V_lid if self.V_lid is not None and len(self.V_lid) == n_lid else np.zeros(n_lid, complex)
+        return np.concatenate([Ib, np.concatenate([self.V[np.nonzero(stack.node)], lids])[stack.keep]])
 
     @property
     def L_loop(self) -> float:
@@ -132,8 +175,12 @@ class Solution:
         return self.Z_loop.imag / (2 * math.pi * self.f)
 
     def at(self, node) -> complex:
-        """A grid node's potential, node = (layer, i, j) (peec.py:1118-1125; this library's stacks have no lid
-        nodes)."""
+        """A node's potential (peec.py:1118-1125): a grid node's from V, a LidNode's from V_lid by its place in the
+        stack; KeyError for a lid the stack does not have."""
+        if isinstance(node, LidNode):
+            if self.V_lid is None or self.lid_at is None:
+                raise KeyError(f"no potential kept for lid {node.lid}")
+            return complex(self.V_lid[self.lid_at[node.lid]])
         return complex(self.V[tuple(node)])
 
     def node_field(self, k: int, d_mm: float, copper: bool = False) -> tuple:

@@ -138,12 +138,17 @@ class PlaneStack:
         self.nid = -np.ones(node.shape, int)
         at = np.nonzero(node)
         self.nid[at] = np.arange(len(at[0]))
-        self.n_grid = self.n_nodes = len(at[0])
+        self.n_grid = len(at[0])
+        # the parts' lids (nodes with a `lid` number: no grid node) after the grid's, by their numbers
+        # (peec.py:1613-1616)
+        lids = sorted({e.lid for b in branches for e in (b.a, b.b) if _is_lid(e)})
+        self.lid_nid = {k: self.n_grid + n for n, k in enumerate(lids)}
+        self.n_nodes = self.n_grid + len(lids)
         self.xa, self.ya = np.argwhere(self.xb), np.argwhere(self.yb)
         self.nxb, self.nyb = len(self.xa), len(self.ya)
         self.ns = self.nxb + self.nyb
         self.branches = [b for b in branches if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0
-                         and tuple(b.a) != tuple(b.b)]
+                         and self.node_id(b.a) != self.node_id(b.b)]
         self.nbr = len(self.branches)
         self.ncap = 0
         self.nb = self.ns + self.nbr
@@ -168,7 +173,10 @@ class PlaneStack:
         self.ArT = self.Ar.T.tocsr()
 
     def node_id(self, node) -> int:
-        """Node number of (layer, i, j), -1 off copper (peec.py:1864-1869)."""
+        """Node number of (layer, i, j), -1 off copper; a lid's its own after the grid's, -1 when no branch names it
+        (peec.py:1864-1869)."""
+        if _is_lid(node):
+            return self.lid_nid.get(node.lid, -1)
         return int(self.nid[tuple(node)])
 
     # ---------------------------------------------------------------- what-if stacks (peec.py:1871-1923)
@@ -176,7 +184,8 @@ class PlaneStack:
         """This stack plus a few branches (peec.py:1871-1909): a shallow copy with the branch list and incidence
         extended, the kernels shared; a branch joining two islands makes a stack of its own. Its operators factorise
         the extended S afresh (the reference reuses the base factor through Woodbury: the same S^-1)."""
-        extra = [b for b in extra if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0 and tuple(b.a) != tuple(b.b)]
+        extra = [b for b in extra if self.node_id(b.a) >= 0 and self.node_id(b.b) >= 0
+                 and self.node_id(b.a) != self.node_id(b.b)]
         if not extra:
             return self
         a = np.array([self.node_id(b.a) for b in extra])
@@ -605,10 +614,16 @@ class Solution:
     m: np.ndarray
     iterations: int
     x: Optional[np.ndarray] = field(default=None, repr=False)
+    V_lid: Optional[np.ndarray] = None      # the lids' potentials, in their numbers' order (peec.py:1110-1112)
 
     @property
     def L_loop(self) -> float:
         return self.Z_loop.imag / (2 * math.pi * self.f)
+
+
+def _is_lid(node) -> bool:
+    """A part's lid (peec.py:1009-1038): a (layer, i, j) tuple that carries its own `lid` number."""
+    return getattr(node, "lid", None) is not None
 
 
 def dist_to_segment(P: np.ndarray, a: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -715,4 +730,5 @@ class RouteExcitation:
         v_inj = sum(amps * V[n] for n, amps in self.injections)
         Z = complex(trace_resistance(self.trace, f, st.sigma, self.trace_t) + v_inj
                     + 1j * w * (self.L_trace + np.dot(self.psi, Ib)))
-        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, st.moment(self.trace, Ib), its, x)
+        V_lid = V[st.n_grid:].copy()
+        return Solution(f, I_x, I_y, Vg, Ib[st.ns:].copy(), Z, st.moment(self.trace, Ib), its, x, V_lid)

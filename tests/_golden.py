@@ -8,10 +8,11 @@ from pathlib import Path
 
 import numpy as np
 
-from groundtruth_amd.problem import Branch, Grid, PlaneLayer, TraceSegment
+from groundtruth_amd.problem import Branch, Grid, LidNode, PlaneLayer, TraceSegment
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-NAMES = ["single_plane_64", "two_layer_48", "four_layer_24", "large_32", "mixed_copper_32"]
+NAMES = ["single_plane_64", "two_layer_48", "four_layer_24", "large_32", "mixed_copper_32", "decap_lid_32"]
+KINDS = ["via", "part", "bond"]
 
 
 @dataclass
@@ -27,8 +28,16 @@ class Fixture:
         return list(enumerate(zip(self.data["solve_route"].tolist(), self.data["solve_f"].tolist())))
 
 
-def _z(R: float, L: float):
+def _z(R: float, L: float, C: float = 0.0):
+    """R + jwL (+ 1/(jwC) when C > 0)."""
+    if C > 0:
+        return lambda f, R=R, L=L, C=C: complex(R, 2 * math.pi * f * L - 1.0 / (2 * math.pi * f * C))
     return lambda f, R=R, L=L: complex(R, 2 * math.pi * f * L)
+
+
+def _end(p, lid: int):
+    """A branch end: a grid node (layer, i, j), or the lid numbered `lid` over that cell."""
+    return LidNode(*(int(v) for v in p), int(lid)) if lid >= 0 else tuple(int(v) for v in p)
 
 
 def load(name: str) -> Fixture:
@@ -37,10 +46,16 @@ def load(name: str) -> Fixture:
     pitch = float(a["pitch_mm"])
     layers = [PlaneLayer(f"L{k}", float(a["z"][k]), Grid.from_mask(m, pitch), float(a["t"][k]))
               for k, m in enumerate(a["nodes"])]
-    branches = [Branch(tuple(int(v) for v in pa), tuple(int(v) for v in pb), _z(float(R), float(L)), f"br{n}",
-                       "via", float(rad))
-                for n, (pa, pb, R, L, rad) in enumerate(zip(a["br_a"], a["br_b"], a["br_R"], a["br_L"],
-                                                           a["br_radius"]))]
+    nbr = len(a["br_R"])
+    lid_a = a.get("br_lid_a", -np.ones(nbr, np.int64))
+    lid_b = a.get("br_lid_b", -np.ones(nbr, np.int64))
+    caps = a.get("br_C", np.zeros(nbr))
+    kinds = a.get("br_kind", np.zeros(nbr, np.int64))
+    branches = [Branch(_end(pa, la), _end(pb, lb), _z(float(R), float(L), float(Cc)), f"br{n}", KINDS[int(kd)],
+                       float(rad))
+                for n, (pa, pb, la, lb, R, L, Cc, kd, rad) in enumerate(zip(a["br_a"], a["br_b"], lid_a, lid_b,
+                                                                          a["br_R"], a["br_L"], caps, kinds,
+                                                                          a["br_radius"]))]
     routes = []
     for k in range(len(a["drv"])):
         sel = np.nonzero(a["tr_route"] == k)[0]

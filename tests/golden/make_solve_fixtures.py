@@ -20,7 +20,9 @@ What is imported, and how:
 The boards are this repository's seeded generator (groundtruth_amd/boards.py) at reduced sizes; each fixture stores
 the board as arrays (copper masks, layer heights and thicknesses, branch ends with their R and L, trace
 filaments, injection nodes), so the tests rebuild exactly what the reference solved without running the
-generator. Outputs per solve: I_x, I_y, V, I_branch, Z_loop, m, iterations; per route: psi (strips and branches),
+generator. Branch ends that are a part's lid (LidNode) carry its number (br_lid_a / br_lid_b, -1 for a grid node),
+and a branch's impedance is R + jwL + 1/(jwC) (br_C = 0: no capacitance). Outputs per solve: I_x, I_y, V, the lids'
+potentials V_lid, I_branch, Z_loop, m, iterations; per route: psi (strips and branches),
 L_trace and R_trace at each frequency; for the first solve, the solution's post-processing (node fields, sheet
 currents, vectors at the dominant phase, the phase, layer shares with and without seeded masks). Nothing at test
 time reads the reference.
@@ -63,12 +65,22 @@ def board_arrays(b) -> dict:
            "nodes": np.stack([l.grid.node for l in b.layers]).astype(bool),
            "z": np.array([l.z for l in b.layers], float),
            "t": np.array([l.thickness for l in b.layers], float)}
-    f1 = 1e6
+    f1, f2 = 1e6, 1e9
     br = b.branches
-    out["br_a"] = np.array([x.a for x in br], np.int64).reshape(-1, 3)
-    out["br_b"] = np.array([x.b for x in br], np.int64).reshape(-1, 3)
-    out["br_R"] = np.array([x.z(0.0).real for x in br], float)
-    out["br_L"] = np.array([x.z(f1).imag / (2 * math.pi * f1) for x in br], float)
+    out["br_a"] = np.array([tuple(x.a)[:3] for x in br], np.int64).reshape(-1, 3)
+    out["br_b"] = np.array([tuple(x.b)[:3] for x in br], np.int64).reshape(-1, 3)
+    out["br_lid_a"] = np.array([getattr(x.a, "lid", -1) for x in br], np.int64)
+    out["br_lid_b"] = np.array([getattr(x.b, "lid", -1) for x in br], np.int64)
+    # z = R + j (w L - 1 / (w C)): L and 1/C from the reactance at two frequencies
+    w1, w2 = 2 * math.pi * f1, 2 * math.pi * f2
+    X1, X2 = np.array([x.z(f1).imag for x in br], float), np.array([x.z(f2).imag for x in br], float)
+    L = (X2 * w2 - X1 * w1) / (w2 ** 2 - w1 ** 2)
+    inv_c = w1 ** 2 * L - X1 * w1
+    inv_c = np.where(np.abs(inv_c) < 1e-6 * np.abs(L) * w2 ** 2, 0.0, inv_c)     # rounding: no capacitance
+    out["br_R"] = np.array([x.z(f1).real for x in br], float)
+    out["br_L"] = np.asarray(L, float)
+    out["br_C"] = np.where(inv_c > 0, 1.0 / np.where(inv_c > 0, inv_c, 1.0), 0.0)
+    out["br_kind"] = np.array([{"via": 0, "part": 1, "bond": 2}[x.kind] for x in br], np.int64)
     out["br_radius"] = np.array([x.radius for x in br], float)
     segs, owner = [], []
     for k, r in enumerate(b.routes):
@@ -84,6 +96,14 @@ def board_arrays(b) -> dict:
     return out
 
 
+def branch_z(R: float, L: float, C: float):
+    """z(f) = R + jwL (+ 1/(jwC) when C > 0)."""
+    R, L, C = float(R), float(L), float(C)
+    if C > 0:
+        return lambda f: complex(R, 2 * math.pi * f * L - 1.0 / (2 * math.pi * f * C))
+    return lambda f: complex(R, 2 * math.pi * f * L)
+
+
 def reference_board(R, a: dict):
     """Reference PlaneLayer / Branch / TraceSegment objects rebuilt from the arrays."""
     layers = []
@@ -91,11 +111,15 @@ def reference_board(R, a: dict):
         g = R.Grid(0.0, 0.0, float(a["pitch_mm"]), m.shape[0], m.shape[1], m.copy(), m[:-1, :] & m[1:, :],
                    m[:, :-1] & m[:, 1:])
         layers.append(R.PlaneLayer(f"L{k}", float(a["z"][k]), g, float(a["t"][k])))
-    branches = [R.Branch(tuple(int(v) for v in pa), tuple(int(v) for v in pb),
-                         (lambda f, Rr=float(Rr), Ll=float(Ll): complex(Rr, 2 * math.pi * f * Ll)), f"br{n}", "via",
-                         float(rad))
-                for n, (pa, pb, Rr, Ll, rad) in enumerate(zip(a["br_a"], a["br_b"], a["br_R"], a["br_L"],
-                                                              a["br_radius"]))]
+    kinds = ["via", "part", "bond"]
+
+    def end(p, lid):
+        return R.LidNode(*(int(v) for v in p), int(lid)) if lid >= 0 else tuple(int(v) for v in p)
+
+    branches = [R.Branch(end(pa, la), end(pb, lb), branch_z(Rr, Ll, Cc), f"br{n}", kinds[int(kd)], float(rad))
+                for n, (pa, pb, la, lb, Rr, Ll, Cc, kd, rad) in
+                enumerate(zip(a["br_a"], a["br_b"], a["br_lid_a"], a["br_lid_b"], a["br_R"], a["br_L"], a["br_C"],
+                              a["br_kind"], a["br_radius"]))]
     routes = []
     for k in range(len(a["drv"])):
         sel = np.nonzero(a["tr_route"] == k)[0]
@@ -143,6 +167,7 @@ def solve_case(R, name: str, b, solves: list, note: str):
         ex = R.RouteExcitation(st, trace, drv, rcv)
         sol = ex.solve(f)
         out[f"I_x_{s}"], out[f"I_y_{s}"], out[f"V_{s}"] = sol.I_x, sol.I_y, sol.V
+        out[f"V_lid_{s}"] = np.asarray(sol.V_lid if sol.V_lid is not None else np.zeros(0), complex)
         out[f"I_branch_{s}"] = sol.I_branch
         out[f"Z_{s}"] = np.complex128(sol.Z_loop)
         out[f"m_{s}"] = np.asarray(sol.m)
@@ -161,10 +186,14 @@ def solve_case(R, name: str, b, solves: list, note: str):
     print(f"wrote {path} ({path.stat().st_size} bytes)", flush=True)
 
 
-def main():
+def main(only=None):
+    """Every fixture, or those named in `only` (their solve_<name> names, and "track")."""
     sys.path.insert(0, str(REPO))
     from groundtruth_amd import boards
     R = load_reference()
+    if only:
+        real = solve_case
+        globals()["solve_case"] = lambda R_, name, *a: real(R_, name, *a) if name in only else None
     # the 64 x 64 single plane at 100 MHz (the CPU-runnable workload) at full size
     solve_case(R, "single_plane_64", boards.single_plane(), [(0, 1e8)], "single plane 64x64 at 0.5 mm, 100 MHz")
     # the two-layer workload's construction at 48 x 48 (slot, stitching vias, Manhattan route), 1 MHz and 1 GHz
@@ -182,7 +211,25 @@ def main():
     bm.layers[1].thickness = 18e-6
     solve_case(R, "mixed_copper_32", bm, [(0, 1e6), (0, 1e8), (0, 1e9)],
                "two planes 32x32, 35 um over 18 um copper, 6 vias, 1 MHz / 100 MHz / 1 GHz")
+    # a decoupling capacitor between the two planes as the reference models a part: two lids (LidNode) bonded to
+    # the copper under their pads, the capacitor between them
+    bl = boards.scaled("two_layer", 32, n_vias=6, n_freqs=1)
+    bl.branches += boards.decoupling_cap(bl.layers, (0, 9, 20), (1, 9, 23), lid=0)
+    solve_case(R, "decap_lid_32", bl, [(0, 1e7), (0, 5e8)],
+               "two planes 32x32, 6 vias, a 100 nF capacitor on lids bonded to both planes, 10 MHz and 500 MHz")
+    if not only or "track" in only:
+        track_vectors(R)
+
+
+def track_vectors(R):
+    """track_impedance (peec.py:1075-1089) of a few tracks at DC and through the skin-effect range."""
+    rows = [(l, w, t, f) for l in (0.2e-3, 1e-3, 5e-3) for w in (0.1e-3, 0.25e-3) for t in (18e-6, 35e-6)
+            for f in (0.0, 1e6, 1e8, 3e9)]
+    out = np.array([R.track_impedance(l, w, t)(f) for l, w, t, f in rows], complex)
+    path = OUT / "track_impedance.npz"
+    np.savez_compressed(path, ti_in=np.array(rows, float), ti_out=out)
+    print(f"wrote {path} ({path.stat().st_size} bytes)", flush=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -45,6 +45,24 @@ def test_oracle_reproduces_the_reference_solve(fixtures, name, s):
     assert abs(sol.Z_loop - a[f"Z_{s}"]) / abs(a[f"Z_{s}"]) <= 1e-12
     assert G.l2(sol.V, a[f"V_{s}"]) <= 1e-10
     assert G.l2(sol.m, a[f"m_{s}"]) <= 1e-9
+    if a.get(f"V_lid_{s}") is not None and len(a[f"V_lid_{s}"]):
+        assert G.l2(sol.V_lid, a[f"V_lid_{s}"]) <= 1e-10          # the parts' lids, numbered after the grid
+
+
+def test_track_impedance_matches_the_reference():
+    """track_impedance (peec.py:1075-1089) at DC and through the skin-effect range, against the reference's values
+    (tests/golden/track_impedance.npz); and its closed forms: R at DC is L / (sigma w t), L_self the strip's."""
+    from groundtruth_amd import greens
+    from groundtruth_amd.problem import SIGMA_CU, track_impedance
+    with np.load(G.GOLDEN / "track_impedance.npz") as z:
+        rows, ref = z["ti_in"], z["ti_out"]
+    got = np.array([track_impedance(l, w, t)(f) for l, w, t, f in rows])
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-14
+    for (l, w, t, f), zz in zip(rows, got):
+        if f == 0:
+            assert abs(zz.real - l / (SIGMA_CU * w * t)) <= 1e-15 * zz.real
+        else:
+            assert abs(zz.imag / (2 * np.pi * f) - greens.strip_self_gmd(l, w, t)) <= 1e-15 * zz.imag
 
 
 @pytest.mark.parametrize("name", G.NAMES)
@@ -80,6 +98,10 @@ def test_device_solve_matches_the_reference(fixtures, name, s):
     assert G.l2(sol.V, a[f"V_{s}"]) <= 1e-8
     assert G.l2(sol.m, a[f"m_{s}"]) <= 1e-8
     assert abs(sol.iterations - int(a[f"its_{s}"])) <= 2
+    if a.get(f"V_lid_{s}") is not None and len(a[f"V_lid_{s}"]):
+        assert G.l2(sol.V_lid, a[f"V_lid_{s}"]) <= 1e-8
+        lid = next(b.a for b in fx.branches if hasattr(b.a, "lid"))
+        assert sol.at(lid) == sol.V_lid[sol.lid_at[lid.lid]]
 
 
 @pytest.mark.gpu
