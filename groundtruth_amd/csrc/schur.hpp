@@ -90,7 +90,7 @@ struct SchurHost {
     std::vector<int64_t> top_gptr;      // per i: the update-buffer entries its non-top children add (ascending)
     std::vector<int64_t> top_gidx;
     std::vector<TopItem> top_fwd, top_ffin, top_bwd, top_bfin;
-    int64_t top_slots = 0;              // partial-sum slots of the dense tiles (SCHUR_TILE x 16 each)
+    int64_t top_slots = 0;              // partial-sum slots of the dense tiles (SCHUR_TOP_ROWS x 16 each)
     std::vector<int> tfwd_ptr, tfwd_cols, tffin_ptr;        // the top supernodes' forward level items (M's set-up)
     std::vector<SnItem> tfwd, tffin;
     int64_t nnz_L = 0;
@@ -134,7 +134,8 @@ constexpr int SCHUR_PV = 8;
 constexpr int SCHUR_TOP_COLS = 8192;    // top columns of the dense top at most
 constexpr int64_t SCHUR_TOP_NNZ = int64_t(4) << 20;    // entries of its forward store at most
 constexpr int SCHUR_TOP_MIN_T = 3;      // a dense top of fewer levels saves no launches
-constexpr int SCHUR_TOP_CHUNK = 256;    // reduction length of one dense item
+constexpr int SCHUR_TOP_CHUNK = 128;    // reduction length of one dense item
+constexpr int SCHUR_TOP_ROWS = 64;      // rows (forward) or columns (backward) of one dense item
 constexpr int SCHUR_NARROW = 32;       // forward: the widest supernode of 64-row tiles
 constexpr int SCHUR_TILE = 16;         // rows (forward, wide) or columns (backward) of a split tile
 

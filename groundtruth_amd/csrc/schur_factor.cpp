@@ -617,7 +617,7 @@ void SchurHost::build_top(const std::vector<int>& dep) {
         top_gidx.insert(top_gidx.end(), g[i].begin(), g[i].end());
         top_gptr.push_back((int64_t)top_gidx.size());
     }
-    // items: tiles of 16 rows (forward) or columns (backward) of one supernode, reductions in chunks of
+    // items: tiles of SCHUR_TOP_ROWS rows (forward) or columns (backward) of one supernode, reductions in chunks of
     // SCHUR_TOP_CHUNK
     auto items = [&](std::vector<TopItem>& out, std::vector<TopItem>& fin, int i0, int cnt, int k0, int k1) {
         const int ks = std::max(1, (k1 - k0 + SCHUR_TOP_CHUNK - 1) / SCHUR_TOP_CHUNK);
@@ -639,8 +639,8 @@ void SchurHost::build_top(const std::vector<int>& dep) {
     };
     for (int sn = 0; sn < ns; ++sn) {
         if (dep[sn] >= top_T) continue;
-        for (int r0 = 0; r0 < nc[sn]; r0 += SCHUR_TILE) {
-            const int i0 = own[sn] + r0, cnt = std::min(SCHUR_TILE, nc[sn] - r0);
+        for (int r0 = 0; r0 < nc[sn]; r0 += SCHUR_TOP_ROWS) {
+            const int i0 = own[sn] + r0, cnt = std::min(SCHUR_TOP_ROWS, nc[sn] - r0);
             items(top_fwd, top_ffin, i0, cnt, lo[sn], i0 + cnt);              // M[i][lo .. i]
             items(top_bwd, top_bfin, i0, cnt, 0, top_clen[i0]);               // M[chain][i]
         }

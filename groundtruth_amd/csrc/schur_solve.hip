@@ -316,7 +316,8 @@ struct TopArgs {
     zd* part;
 };
 
-constexpr int TOP_K = 64;            // reduction entries staged in LDS at a time
+constexpr int TOP_K = 32;            // reduction entries staged in LDS at a time
+constexpr int TOP_SLOT = SCHUR_TOP_ROWS * KMAX;      // entries of one partial-sum slot of a dense tile
 
 // g[i] = b at top column i plus the update entries its non-top children send up, in ascending order
 __global__ __launch_bounds__(NTH) void schur_top_g(TopArgs a, int nt, int nr) {
@@ -328,19 +329,21 @@ __global__ __launch_bounds__(NTH) void schur_top_g(TopArgs a, int nt, int nr) {
     a.g[(size_t)i * nr + p] = acc;
 }
 
-// Forward: z at top rows [i0, i0 + cnt) = M[rows][k0 .. k1) g[k0 .. k1); thread (row r, right-hand side p). g and
-// M go through LDS TOP_K entries at a time, double-buffered: the next chunk's loads are in flight while this one's
+// Forward: z at top rows [i0, i0 + cnt) (cnt <= SCHUR_TOP_ROWS) = M[rows][k0 .. k1) g[k0 .. k1). Thread (row
+// group rg, right-hand side p) holds rows 4 rg .. 4 rg + 3, so each staged g value feeds 8 multiply-adds. g and M go
+// through LDS TOP_K entries at a time, double-buffered: the next chunk's loads are in flight while this one's
 // multiply-adds run. The sum over k runs in one fixed order for every batch.
+constexpr int TOP_R = 4;                      // rows (columns) per thread
 __global__ __launch_bounds__(NTH) void schur_top_fwd(TopArgs a, int nr) {
-    __shared__ double gs_raw[2][TOP_K][KMAX + 1][2];
-    __shared__ double ms[2][SCHUR_TILE][TOP_K + 1];
-    __shared__ int64_t offs[SCHUR_TILE];
+    __shared__ double gs[2][TOP_K][KMAX][2];
+    __shared__ double ms[2][TOP_K][SCHUR_TOP_ROWS + 1];
+    __shared__ int64_t offs[SCHUR_TOP_ROWS];
     const TopItem it = a.item[blockIdx.x];
-    const int r = threadIdx.x >> 4, p = threadIdx.x & 15;
-    const int i = it.i0 + r, lo = a.lo[it.i0];
-    if (threadIdx.x < SCHUR_TILE) offs[threadIdx.x] = threadIdx.x < it.cnt ? a.foff[it.i0 + threadIdx.x] : 0;
+    const int rg = threadIdx.x >> 4, p = threadIdx.x & 15;
+    const int lo = a.lo[it.i0];
+    if (threadIdx.x < SCHUR_TOP_ROWS) offs[threadIdx.x] = threadIdx.x < it.cnt ? a.foff[it.i0 + threadIdx.x] : 0;
     __syncthreads();
-    constexpr int GPT = TOP_K * KMAX / NTH, MPT = SCHUR_TILE * TOP_K / NTH;   // staged entries per thread
+    constexpr int GPT = TOP_K * KMAX / NTH, MPT = SCHUR_TOP_ROWS * TOP_K / NTH;   // staged entries per thread
     zd gl[GPT];
     double ml[MPT];
     auto load = [&](int k0) {
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(NTH) void schur_top_fwd(TopArgs a, int nr) {
             gl[u] = q < nr && kk < len ? a.g[(size_t)(k0 + kk) * nr + q] : zd();
         }
 #pragma unroll
-        for (int u = 0; u < MPT; ++u) {
+        for (int u = 0; u < MPT; ++u) {         // row rr's run of k: consecutive threads, consecutive k
             const int e = threadIdx.x + u * NTH, rr = e / TOP_K, kk = e - rr * TOP_K, k = k0 + kk;
             ml[u] = rr < it.cnt && kk < len && k <= it.i0 + rr ? a.Mf[offs[rr] + (k - lo)] : 0.0;
         }
@@ -360,16 +363,16 @@ __global__ __launch_bounds__(NTH) void schur_top_fwd(TopArgs a, int nr) {
 #pragma unroll
         for (int u = 0; u < GPT; ++u) {
             const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
-            gs_raw[b][kk][q][0] = gl[u].re;
-            gs_raw[b][kk][q][1] = gl[u].im;
+            gs[b][kk][q][0] = gl[u].re;
+            gs[b][kk][q][1] = gl[u].im;
         }
 #pragma unroll
         for (int u = 0; u < MPT; ++u) {
             const int e = threadIdx.x + u * NTH, rr = e / TOP_K, kk = e - rr * TOP_K;
-            ms[b][rr][kk] = ml[u];
+            ms[b][kk][rr] = ml[u];
         }
     };
-    double are = 0.0, aim = 0.0;
+    double are[TOP_R] = {}, aim[TOP_R] = {};
     load(it.k0);
     store(0);
     __syncthreads();
@@ -379,32 +382,42 @@ __global__ __launch_bounds__(NTH) void schur_top_fwd(TopArgs a, int nr) {
         if (more) load(k0 + TOP_K);
         const int len = min(TOP_K, it.k1 - k0);
         for (int kk = 0; kk < len; ++kk) {
-            const double m = ms[b][r][kk];
-            are = fma(m, gs_raw[b][kk][p][0], are);
-            aim = fma(m, gs_raw[b][kk][p][1], aim);
+            const double gr = gs[b][kk][p][0], gi = gs[b][kk][p][1];
+#pragma unroll
+            for (int u = 0; u < TOP_R; ++u) {
+                const double m = ms[b][kk][TOP_R * rg + u];
+                are[u] = fma(m, gr, are[u]);
+                aim[u] = fma(m, gi, aim[u]);
+            }
         }
         if (more) store(b ^ 1);
         __syncthreads();
         b ^= 1;
     }
-    if (r >= it.cnt || p >= nr) return;
-    if (it.slot0 < 0) a.zp[(size_t)a.col[i] * nr + p] = zd(are, aim);
-    else a.part[(size_t)(it.slot0 + it.kidx) * SLOT + r * KMAX + p] = zd(are, aim);
+    if (p >= nr) return;
+#pragma unroll
+    for (int u = 0; u < TOP_R; ++u) {
+        const int r = TOP_R * rg + u;
+        if (r >= it.cnt) break;
+        if (it.slot0 < 0) a.zp[(size_t)a.col[it.i0 + r] * nr + p] = zd(are[u], aim[u]);
+        else a.part[(size_t)(it.slot0 + it.kidx) * TOP_SLOT + r * KMAX + p] = zd(are[u], aim[u]);
+    }
 }
 
-// Backward: x at top columns [i0, i0 + cnt) = M[chain[k0 .. k1)][cols]^T z[chain[k0 .. k1)]; thread (column c, p);
-// the chain's rows as columns of zp (ccol), staged like the forward's
+// Backward: x at top columns [i0, i0 + cnt) = M[chain[k0 .. k1)][cols]^T z[chain[k0 .. k1)], the columns of one
+// supernode (one chain); thread (column group, p) holds 4 columns. The chain's rows as columns of zp (ccol).
 __global__ __launch_bounds__(NTH) void schur_top_bwd(TopArgs a, int nr) {
-    __shared__ double zs_raw[2][TOP_K][KMAX + 1][2];
-    __shared__ double ms[2][SCHUR_TILE][TOP_K + 1];
-    __shared__ int64_t offs[SCHUR_TILE];
+    __shared__ double zs[2][TOP_K][KMAX][2];
+    __shared__ double ms[2][TOP_K][SCHUR_TOP_ROWS + 1];
+    __shared__ int64_t offs[SCHUR_TOP_ROWS];
+    __shared__ int ccol[SCHUR_TOP_CHUNK];       // the item's chain rows [k0, k1), as columns of zp
     const TopItem it = a.item[blockIdx.x];
-    const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
-    const int i = it.i0 + c;
-    const int* ccol = a.chain + a.cptr[it.i0];
-    if (threadIdx.x < SCHUR_TILE) offs[threadIdx.x] = threadIdx.x < it.cnt ? a.boff[it.i0 + threadIdx.x] : 0;
+    const int cg = threadIdx.x >> 4, p = threadIdx.x & 15;
+    const int* chain = a.chain + a.cptr[it.i0];
+    if (threadIdx.x < SCHUR_TOP_ROWS) offs[threadIdx.x] = threadIdx.x < it.cnt ? a.boff[it.i0 + threadIdx.x] : 0;
+    for (int c = threadIdx.x; c < it.k1 - it.k0; c += NTH) ccol[c] = chain[it.k0 + c];
     __syncthreads();
-    constexpr int GPT = TOP_K * KMAX / NTH, MPT = SCHUR_TILE * TOP_K / NTH;
+    constexpr int GPT = TOP_K * KMAX / NTH, MPT = SCHUR_TOP_ROWS * TOP_K / NTH;
     zd zl[GPT];
     double ml[MPT];
     auto load = [&](int k0) {
@@ -412,7 +425,7 @@ __global__ __launch_bounds__(NTH) void schur_top_bwd(TopArgs a, int nr) {
 #pragma unroll
         for (int u = 0; u < GPT; ++u) {
             const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
-            zl[u] = q < nr && kk < len ? a.zp[(size_t)ccol[k0 + kk] * nr + q] : zd();
+            zl[u] = q < nr && kk < len ? a.zp[(size_t)ccol[k0 - it.k0 + kk] * nr + q] : zd();
         }
 #pragma unroll
         for (int u = 0; u < MPT; ++u) {
@@ -424,16 +437,16 @@ __global__ __launch_bounds__(NTH) void schur_top_bwd(TopArgs a, int nr) {
 #pragma unroll
         for (int u = 0; u < GPT; ++u) {
             const int e = threadIdx.x + u * NTH, kk = e >> 4, q = e & 15;
-            zs_raw[b][kk][q][0] = zl[u].re;
-            zs_raw[b][kk][q][1] = zl[u].im;
+            zs[b][kk][q][0] = zl[u].re;
+            zs[b][kk][q][1] = zl[u].im;
         }
 #pragma unroll
         for (int u = 0; u < MPT; ++u) {
             const int e = threadIdx.x + u * NTH, cc = e / TOP_K, kk = e - cc * TOP_K;
-            ms[b][cc][kk] = ml[u];
+            ms[b][kk][cc] = ml[u];
         }
     };
-    double are = 0.0, aim = 0.0;
+    double are[TOP_R] = {}, aim[TOP_R] = {};
     load(it.k0);
     store(0);
     __syncthreads();
@@ -443,31 +456,49 @@ __global__ __launch_bounds__(NTH) void schur_top_bwd(TopArgs a, int nr) {
         if (more) load(k0 + TOP_K);
         const int len = min(TOP_K, it.k1 - k0);
         for (int kk = 0; kk < len; ++kk) {
-            const double m = ms[b][c][kk];
-            are = fma(m, zs_raw[b][kk][p][0], are);
-            aim = fma(m, zs_raw[b][kk][p][1], aim);
+            const double zr = zs[b][kk][p][0], zi = zs[b][kk][p][1];
+#pragma unroll
+            for (int u = 0; u < TOP_R; ++u) {
+                const double m = ms[b][kk][TOP_R * cg + u];
+                are[u] = fma(m, zr, are[u]);
+                aim[u] = fma(m, zi, aim[u]);
+            }
         }
         if (more) store(b ^ 1);
         __syncthreads();
         b ^= 1;
     }
-    if (c >= it.cnt || p >= nr) return;
-    if (it.slot0 < 0) {
-        a.xp[(size_t)a.col[i] * nr + p] = zd(are, aim);
-        a.x[(size_t)a.perm[a.col[i]] * nr + p] = zd(are, aim);
-    } else {
-        a.part[(size_t)(it.slot0 + it.kidx) * SLOT + c * KMAX + p] = zd(are, aim);
+    if (p >= nr) return;
+#pragma unroll
+    for (int u = 0; u < TOP_R; ++u) {
+        const int c = TOP_R * cg + u;
+        if (c >= it.cnt) break;
+        const int col = a.col[it.i0 + c];
+        if (it.slot0 < 0) {
+            a.xp[(size_t)col * nr + p] = zd(are[u], aim[u]);
+            a.x[(size_t)a.perm[col] * nr + p] = zd(are[u], aim[u]);
+        } else {
+            a.part[(size_t)(it.slot0 + it.kidx) * TOP_SLOT + c * KMAX + p] = zd(are[u], aim[u]);
+        }
     }
 }
 
-// The split dense tiles: partial sums added in chunk order, then written as the forward (z) or backward (x) result
-__global__ __launch_bounds__(NTH) void schur_top_fin(TopArgs a, int nr, int forward) {
+// The split dense tiles: partial sums added in chunk order, then written as the forward (z) or backward (x) result;
+// thread (row or column r, right-hand side p)
+__global__ __launch_bounds__(SCHUR_TOP_ROWS * KMAX) void schur_top_fin(TopArgs a, int nr, int forward) {
     const TopItem it = a.item[blockIdx.x];
     const int r = threadIdx.x >> 4, p = threadIdx.x & 15;
     if (r >= it.cnt || p >= nr) return;
-    const zd* s0 = a.part + (size_t)it.slot0 * SLOT + r * KMAX + p;
+    const zd* s0 = a.part + (size_t)it.slot0 * TOP_SLOT + r * KMAX + p;
     zd acc = s0[0];
-    for (int k = 1; k < it.ks; ++k) acc = acc + s0[(size_t)k * SLOT];
+    for (int k0 = 1; k0 < it.ks; k0 += 16) {    // sixteen loads in flight, added in chunk order
+        zd t[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) t[u] = k0 + u < it.ks ? s0[(size_t)(k0 + u) * TOP_SLOT] : zd();
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (k0 + u < it.ks) acc = acc + t[u];
+    }
     const int col = a.col[it.i0 + r];
     if (forward) {
         a.zp[(size_t)col * nr + p] = acc;
@@ -493,7 +524,8 @@ void run_top_fwd(const Schur& S, const zd* b, int nr, hipStream_t st) {
     GT_LAUNCH("schur_top_fwd");
     if (!S.h.top_ffin.empty()) {
         a.item = S.top_ffin.p;
-        hipLaunchKernelGGL(schur_top_fin, dim3((unsigned)S.h.top_ffin.size()), dim3(NTH), 0, st, a, nr, 1);
+        hipLaunchKernelGGL(schur_top_fin, dim3((unsigned)S.h.top_ffin.size()), dim3(SCHUR_TOP_ROWS * KMAX), 0, st, a,
+                           nr, 1);
         GT_LAUNCH("schur_top_fin");
     }
 }
@@ -505,7 +537,8 @@ void run_top_bwd(const Schur& S, zd* x, int nr, hipStream_t st) {
     GT_LAUNCH("schur_top_bwd");
     if (!S.h.top_bfin.empty()) {
         a.item = S.top_bfin.p;
-        hipLaunchKernelGGL(schur_top_fin, dim3((unsigned)S.h.top_bfin.size()), dim3(NTH), 0, st, a, nr, 0);
+        hipLaunchKernelGGL(schur_top_fin, dim3((unsigned)S.h.top_bfin.size()), dim3(SCHUR_TOP_ROWS * KMAX), 0, st, a,
+                           nr, 0);
         GT_LAUNCH("schur_top_fin");
     }
 }
@@ -586,7 +619,7 @@ void Schur::upload(hipStream_t st) {
         top_ffin.upload(h.top_ffin.empty() ? std::vector<TopItem>(1) : h.top_ffin, st);
         top_bwd.upload(h.top_bwd, st);
         top_bfin.upload(h.top_bfin.empty() ? std::vector<TopItem>(1) : h.top_bfin, st);
-        top_part.alloc((size_t)std::max<int64_t>(1, h.top_slots) * SCHUR_TILE * KMAX);
+        top_part.alloc((size_t)std::max<int64_t>(1, h.top_slots) * SCHUR_TOP_ROWS * KMAX);
         tfwd.upload(h.tfwd.empty() ? std::vector<SnItem>(1) : h.tfwd, st);
         tffin.upload(h.tffin.empty() ? std::vector<SnItem>(1) : h.tffin, st);
     }
