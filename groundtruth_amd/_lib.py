@@ -15,6 +15,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libgtpeec.so"
 GT_OK, GT_ERR_ARG, GT_ERR_OOM, GT_ERR_DEVICE, GT_ERR_NOCONV, GT_ERR_NUMERIC, GT_ERR_UNSUPPORTED = range(7)
 
 i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+i8p = C.POINTER(C.c_int8)
 f64p = C.POINTER(C.c_double)
 
 
@@ -65,6 +67,13 @@ SIGNATURES = {
     "gt_loop_terms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, i32p, f64p, C.c_double,
                                 f64p, f64p]),
     "gt_via_coupling": (C.c_int, [C.c_void_p, C.c_int32, f64p, C.c_int32, C.POINTER(Segment), C.c_double, f64p]),
+    "gt_raster_label": (C.c_int, [C.c_void_p, C.c_int32, i64p, f64p, i32p, C.c_int32, f64p, C.c_int32, f64p, i32p]),
+    "gt_raster_fine": (C.c_int, [C.c_void_p, C.c_int32, i64p, f64p, i32p, C.c_int32, C.c_int32, C.c_int32, f64p,
+                                 f64p, i32p, f64p, f64p, f64p, f64p, f64p, i32p]),
+    "gt_raster_state": (C.c_int, [C.c_void_p, C.c_int64, f64p, i32p, C.c_int32, f64p, C.c_int32, f64p, i8p]),
+    "gt_raster_cover": (C.c_int, [C.c_void_p, C.c_int64, f64p, C.c_double, C.c_double, C.c_int32, C.c_double,
+                                  C.c_double, C.c_int32, f64p]),
+    "gt_raster_points": (C.c_int, [C.c_void_p, C.c_int64, f64p, C.c_int64, f64p, i8p, i8p, f64p]),
     "gt_strip_coupling": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.POINTER(Segment), f64p, f64p,
                                     C.c_void_p]),
     "gt_stack_destroy": (None, [C.c_void_p]),

@@ -531,6 +531,60 @@ int gt_via_coupling(gt_ctx* ctx, int32_t nv, const double* vias, int32_t nseg, c
     });
 }
 
+int gt_raster_label(gt_ctx* ctx, int32_t n_pieces, const int64_t* edge_ptr, const double* edges,
+                    const int32_t* windows, int32_t nu, const double* xs, int32_t nv, const double* ys,
+                    int32_t* labels) {
+    return guard([&] {
+        need(ctx, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        raster_label(ctx, n_pieces, edge_ptr, edges, windows, nu, xs, nv, ys, labels);
+    });
+}
+
+int gt_raster_fine(gt_ctx* ctx, int32_t n_pieces, const int64_t* edge_ptr, const double* edges,
+                   const int32_t* windows, int32_t nx, int32_t ny, int32_t sub, const double* xf, const double* yf,
+                   int32_t* part, double* fill, double* cross_x, double* off_x, double* cross_y, double* off_y,
+                   int32_t* fine_labels) {
+    return guard([&] {
+        need(ctx, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        raster_fine(ctx, n_pieces, edge_ptr, edges, windows, nx, ny, sub, xf, yf, part, fill, cross_x, off_x, cross_y,
+                    off_y, fine_labels);
+    });
+}
+
+int gt_raster_state(gt_ctx* ctx, int64_t n_edges, const double* edges, const int32_t* window, int32_t nu,
+                    const double* xs, int32_t nv, const double* ys, int8_t* state) {
+    return guard([&] {
+        need(ctx, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        raster_state(ctx, n_edges, edges, window, nu, xs, nv, ys, state);
+    });
+}
+
+int gt_raster_cover(gt_ctx* ctx, int64_t n_edges, const double* edges, double u0, double du, int32_t n_lines,
+                    double v0, double d, int32_t n_cells, double* out) {
+    return guard([&] {
+        need(ctx, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        raster_cover(ctx, n_edges, edges, u0, du, n_lines, v0, d, n_cells, out);
+    });
+}
+
+int gt_raster_points(gt_ctx* ctx, int64_t n_edges, const double* edges, int64_t n_points, const double* points,
+                     int8_t* inside, int8_t* on_edge, double* nearest) {
+    return guard([&] {
+        need(ctx, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        raster_points(ctx, n_edges, edges, n_points, points, inside, on_edge, nearest);
+    });
+}
+
 int gt_strip_coupling(gt_stack* st, double d, double mu, int32_t nseg, const gt_segment* segs, const double* sheet_z,
                       const double* sheet_t, void* psi_dev) {
     return guard([&] {

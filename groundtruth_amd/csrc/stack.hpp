@@ -188,6 +188,18 @@ void loop_terms_device(gt_stack* st, const zd* x, const double* psi, const doubl
                        const double* inj_amps, double d, const double* sheet_z, double* out);
 void via_coupling_device(gt_ctx* ctx, int nv, const double* vias, int nseg, const gt_segment* segs, double mu,
                          double* out);
+// copper rasterisation (raster.hip)
+void raster_label(gt_ctx* ctx, int32_t ng, const int64_t* eptr, const double* edges, const int32_t* win, int32_t nu,
+                  const double* xs, int32_t nv, const double* ys, int32_t* lab);
+void raster_fine(gt_ctx* ctx, int32_t ng, const int64_t* eptr, const double* edges, const int32_t* win, int32_t nx,
+                 int32_t ny, int32_t s, const double* xf, const double* yf, int32_t* part, double* fill,
+                 double* cross_x, double* off_x, double* cross_y, double* off_y, int32_t* fine);
+void raster_state(gt_ctx* ctx, int64_t ne, const double* edges, const int32_t* win, int32_t nu, const double* xs,
+                  int32_t nv, const double* ys, int8_t* state);
+void raster_cover(gt_ctx* ctx, int64_t ne, const double* edges, double u0, double du, int32_t n_lines, double v0,
+                  double d, int32_t n_cells, double* out);
+void raster_points(gt_ctx* ctx, int64_t ne, const double* edges, int64_t np, const double* pts, int8_t* inside,
+                   int8_t* onb, double* nearest);
 // psi of every strip (ns values at psi, device memory) from the trace's segments (psi.hip)
 void strip_coupling_device(gt_stack* st, double d, double mu, int nseg, const gt_segment* segs, const double* sheet_z,
                            const double* sheet_t, double* psi);

@@ -166,6 +166,18 @@ _WOODBURY_FORMS = {"auto": 0, "stored": 1, "implicit": 2}
 _ND_ORDERINGS = {"joint": 0, "sheets": 1}
 
 
+def _narrow_strips(g) -> bool:
+    """A grid whose strips are not all a pitch wide for their inductance and resistance (peec.py:1732-1757:
+    a width of 0 or none counts as full)."""
+    for w, b in ((getattr(g, "wx", None), g.xb), (getattr(g, "wy", None), g.yb),
+                 (getattr(g, "rx", None), g.xb), (getattr(g, "ry", None), g.yb)):
+        if w is not None:
+            v = np.asarray(w)[np.asarray(b, bool)]
+            if np.any((v > 0) & (v != 1.0)):
+                return True
+    return False
+
+
 class PlaneStack:
     """Plane layers and lumped branches resident on one device (peec.py:1555). Unknown order:
     [x strips | y strips | branches]."""
@@ -186,6 +198,9 @@ class PlaneStack:
         for l in layers:
             if (l.grid.nx, l.grid.ny, l.grid.d) != (g0.nx, g0.ny, g0.d):
                 raise SolverError("all layers must share one grid")
+            if _narrow_strips(l.grid):
+                raise NotImplementedError("strips narrower than the pitch (a grid measured on a sub-grid, Grid.wx / wy "
+                                          "/ rx / ry) are not solved by this library: build the grid with sub=1")
         # layers of different copper thickness: each sheet's strips have their own admittance (gt_schur_factor_nodes)
         self.per_sheet = len({float(l.thickness) for l in layers}) != 1
         self.precond_mode = precond

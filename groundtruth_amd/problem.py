@@ -44,6 +44,102 @@ class Grid:
     node: np.ndarray            # (nx, ny) bool
     xb: np.ndarray              # (nx-1, ny) bool: x strip between (i, j) and (i+1, j)
     yb: np.ndarray              # (nx, ny-1) bool
+    # what the rasteriser measures (raster.build_grid; peec.py:197-223), None where a grid has no such field
+    label: Optional[np.ndarray] = None      # (nx, ny) index into nets, -1 no copper
+    nets: list = field(default_factory=list)
+    fill: Optional[np.ndarray] = None       # (nx, ny) copper share of each node's cell (the sub-grid's)
+    bridges: list = field(default_factory=list)     # [(i0, j0, i1, j1, length mm, width mm)] tracks across
+    wx: Optional[np.ndarray] = None         # (nx-1, ny) strip copper width, a fraction of d (None: full)
+    wy: Optional[np.ndarray] = None         # (nx, ny-1)
+    ox: Optional[np.ndarray] = None         # offset of that copper's centre across the strip, a fraction of d
+    oy: Optional[np.ndarray] = None
+    area: Optional[np.ndarray] = None       # (nx, ny) the node's own copper in its cell, a fraction of d^2
+    rx: Optional[np.ndarray] = None         # the strip's width for its resistance (a change of width in series)
+    ry: Optional[np.ndarray] = None
+    hx: Optional[np.ndarray] = None         # (nx, ny) mm from a node outside its copper to the copper (else 0)
+    hy: Optional[np.ndarray] = None
+
+    def xy(self, i, j):
+        """Board coordinates (mm) of node (i, j) (arrays too)."""
+        return self.x0 + np.asarray(i) * self.d, self.y0 + np.asarray(j) * self.d
+
+    def nearest_node(self, x: float, y: float, max_dist: Optional[float] = None,
+                     net: Optional[str] = None) -> Optional[tuple]:
+        """The copper node nearest (x, y) mm, of `net` only when given; None when there is none (within max_dist)
+        (peec.py:229-255)."""
+        mask = self.node
+        if net is not None and self.label is not None:
+            if net not in self.nets:
+                return None
+            mask = self.label == self.nets.index(net)
+        ci, cj = int(round((x - self.x0) / self.d)), int(round((y - self.y0) / self.d))
+        if max_dist is not None:
+            r = max(int(math.ceil(max_dist / self.d)) + 1, 2)
+            i0, j0 = max(ci - r, 0), max(cj - r, 0)
+            ii, jj = np.nonzero(mask[i0:ci + r + 1, j0:cj + r + 1])
+            ii, jj = ii + i0, jj + j0
+        else:
+            ii, jj = np.nonzero(mask)
+        if len(ii) == 0:
+            return None
+        px, py = self.xy(ii, jj)
+        d2 = (px - x) ** 2 + (py - y) ** 2
+        k = int(np.argmin(d2))
+        if max_dist is not None and d2[k] > max_dist ** 2:
+            return None
+        return int(ii[k]), int(jj[k])
+
+    def net_at(self, node: tuple) -> str:
+        """The net of a node's copper ("" for none or a grid without nets)."""
+        if self.label is None or self.label[node] < 0:
+            return ""
+        return self.nets[self.label[node]]
+
+    @property
+    def extent(self) -> tuple:
+        """(x0, x1, y0, y1) mm the cells cover, each node at its cell's centre."""
+        return (self.x0 - self.d / 2, self.x0 + (self.nx - 0.5) * self.d,
+                self.y0 - self.d / 2, self.y0 + (self.ny - 0.5) * self.d)
+
+    def copper_area(self) -> np.ndarray:
+        """Each node's copper as a fraction of its cell (peec.py:259-313): the measured area shared over the nodes of
+        a 3 x 3 neighbourhood of one net in proportion to their strips' widths along their run; the mean width of
+        its strips where nothing was measured; 1 at full strips, 0 off copper."""
+        out = self.node.astype(float)
+        if self.wx is None or self.wy is None:
+            return out
+        xw, yw = np.where(self.xb, self.wx, 0.0), np.where(self.yb, self.wy, 0.0)
+        sx, sy = np.zeros(out.shape), np.zeros(out.shape)
+        nxs, nys = np.zeros(out.shape), np.zeros(out.shape)
+        sx[:-1] += xw
+        sx[1:] += xw
+        nxs[:-1] += self.xb
+        nxs[1:] += self.xb
+        sy[:, :-1] += yw
+        sy[:, 1:] += yw
+        nys[:, :-1] += self.yb
+        nys[:, 1:] += self.yb
+        n = nxs + nys
+        mean_width = np.where(n > 0, (sx + sy) / np.maximum(n, 1), out)
+        if self.area is None:
+            return mean_width
+        xr, yr = nxs == 2, nys == 2
+        run_width = np.where(xr & ~yr, sx / np.maximum(nxs, 1), np.where(yr & ~xr, sy / np.maximum(nys, 1),
+                                                                         mean_width))
+        area = np.where(self.node, self.area, 0.0)
+        width = np.where(self.node, run_width, 0.0)
+        lab = np.where(self.node, self.label if self.label is not None else 0, -1)
+        sum_a, sum_w = np.zeros(out.shape), np.zeros(out.shape)
+        ni, nj = out.shape
+        for di in (-1, 0, 1):
+            for dj in (-1, 0, 1):
+                src = (slice(max(di, 0), ni + min(di, 0)), slice(max(dj, 0), nj + min(dj, 0)))
+                dst = (slice(max(-di, 0), ni + min(-di, 0)), slice(max(-dj, 0), nj + min(-dj, 0)))
+                same = (lab[src] == lab[dst]) & (lab[dst] >= 0)
+                sum_a[dst] += np.where(same, area[src], 0.0)
+                sum_w[dst] += np.where(same, width[src], 0.0)
+        shared = np.where(sum_w > 0, width * sum_a / np.maximum(sum_w, 1e-12), 0.0)
+        return np.where(self.node, np.where(shared > 0, shared, mean_width), 0.0)
 
     @classmethod
     def from_mask(cls, node: np.ndarray, d_mm: float, x0: float = 0.0, y0: float = 0.0) -> "Grid":

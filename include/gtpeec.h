@@ -199,6 +199,48 @@ int gt_loop_terms(gt_stack* stack, const void* x_dev, const void* psi_dev, const
 int gt_via_coupling(gt_ctx* ctx, int32_t nv, const double* vias, int32_t nseg, const gt_segment* segs, double mu,
                     double* out);
 
+/* ---------------------------------------------------------------- copper rasterisation
+ * Replaces the shapely point and line queries of peec.py:329-983 `build_grid` and its helpers over a board's area.
+ * Copper is given as pieces, each a run of ring edges (ax, ay, bx, by per edge, mm; edge_ptr[k] .. edge_ptr[k + 1]
+ * for piece k) whose even-odd region is the piece's copper. windows: per piece the lattice columns [p0, p1) and
+ * samples [q0, q1) that can hold it (4 int32). Point tests are exact: a point on an edge is on the boundary whatever
+ * the edge's slope, as shapely's predicates decide it. All arrays are host memory. */
+
+/* Lattice labels (peec.py:379-385 `contains_xy` of each net's copper at the nodes and strip midpoints): labels[p * nv
+ * + q] (in/out) takes k for the first piece k whose interior holds the point (xs[p], ys[q]), where it is still < 0.
+ * xs increasing. A lattice column crossing more than 4096 edges of one piece is refused. */
+int gt_raster_label(gt_ctx* ctx, int32_t n_pieces, const int64_t* edge_ptr, const double* edges,
+                    const int32_t* windows, int32_t nu, const double* xs, int32_t nv, const double* ys,
+                    int32_t* labels);
+
+/* The sub-grid (peec.py:774-849 `_coarse_from_fine`): the fine lattice xf (nx sub) x yf (ny sub) labelled by piece
+ * as above, each coarse cell's largest same-piece group of fine samples (4-neighbours inside the cell; ties to the
+ * group met first in row-major order) kept when it has at least `sub` samples: part (nx x ny, its piece or -1), fill
+ * (its share of the cell); and per cell border the share of its `sub` fine strips that join kept groups of one piece
+ * (cross_x (nx-1) x ny, cross_y nx x (ny-1)) and their mean offset from the border's middle (off_x, off_y), each a
+ * fraction of the pitch. fine_labels (nx sub x ny sub) may be NULL. sub: 2 .. 8. */
+int gt_raster_fine(gt_ctx* ctx, int32_t n_pieces, const int64_t* edge_ptr, const double* edges,
+                   const int32_t* windows, int32_t nx, int32_t ny, int32_t sub, const double* xf, const double* yf,
+                   int32_t* part, double* fill, double* cross_x, double* off_x, double* cross_y, double* off_y,
+                   int32_t* fine_labels);
+
+/* Lattice points against one region (peec.py:472 `intersects_xy` of a net's nodes): state[p * nv + q] = 1 inside,
+ * 2 on an edge, 0 outside, for the points of the window (4 int32: columns, then samples); 0 elsewhere. */
+int gt_raster_state(gt_ctx* ctx, int64_t n_edges, const double* edges, const int32_t* window, int32_t nu,
+                    const double* xs, int32_t nv, const double* ys, int8_t* state);
+
+/* Line covers (peec.py:533-583 `_line_cover`): on each line u = u0 + m du (m < n_lines) the length of each interval
+ * [v0 + (c - 1/2) d, v0 + (c + 1/2) d] (c < n_cells) inside the even-odd region of the edges (ua, va, ub, vb per
+ * edge); out: n_lines x n_cells (written). A line with an odd number of crossings covers nothing. */
+int gt_raster_cover(gt_ctx* ctx, int64_t n_edges, const double* edges, double u0, double du, int32_t n_lines,
+                    double v0, double d, int32_t n_cells, double* out);
+
+/* Points against one region's edges (peec.py:400 `covers`, :472-474 `intersects_xy` and `shortest_line`): inside
+ * (even-odd, the boundary aside), on_edge, and the nearest point of the edges (x, y per point; the first of equal
+ * distances; edges binned when there are many). Any output may be NULL. */
+int gt_raster_points(gt_ctx* ctx, int64_t n_edges, const double* edges, int64_t n_points, const double* points,
+                     int8_t* inside, int8_t* on_edge, double* nearest);
+
 /* The L_p product's launch shapes, tuned (no counterpart in the reference): for each batch-size class (1, 2-4, 5-16
  * right-hand sides) the rows per row-pass workgroup (1, 2, 4) and waves per column-pass workgroup (1, 2, 4) that
  * ran `reps` products of 1, 4 and 8 right-hand sides fastest on this stack. The shapes change no arithmetic, so a

@@ -1,0 +1,238 @@
+"""The rasteriser (groundtruth_amd/raster.py + csrc/raster.hip) against the oracle (oracle/raster.py).
+
+CPU: the geometry reader, the pieces of a net, and the host steps (`_adopt`, `_series_factor`, `_cell_areas`,
+`_strip_widths`, the track walk) against the oracle's loop restatements.
+GPU: `build_grid` on the device against the oracle's, case by case: labels, nodes, strips, pieces, fill, bridges
+bit for bit; widths, offsets, areas, series-factor widths and heat points to 1e-12 (absolute, fractions of the
+pitch or mm); the device's line covers and point queries against the oracle's; and a 60 x 60 mm board at 0.1 mm
+with a sub-grid of 4 (3.6 M fine samples a net) whose measured copper adds up to its polygons' area.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from groundtruth_amd import geometry as geo
+from groundtruth_amd import raster as RS
+from oracle import raster as R
+from tests import _raster_cases as C
+
+TOL = 1e-12
+
+
+# ---------------------------------------------------------------------------------------------- CPU
+def test_geometry_reader_matches_geojson():
+    g = C.webbed_plane()
+    mine = geo.polygons(g)
+    theirs = R.polygons(json.loads(json.dumps(g.__geo_interface__)))
+    assert len(mine) == len(theirs)
+    for p, q in zip(mine, theirs):
+        assert len(p) == len(q) and all(np.array_equal(a, b) for a, b in zip(p, q))
+    assert geo.polygons({"type": "LineString", "coordinates": [(0, 0), (1, 1)]}) == []
+    coll = {"type": "GeometryCollection", "geometries": [C.box(0, 0, 1, 1).__geo_interface__,
+                                                         {"type": "Point", "coordinates": (3, 3)}]}
+    assert len(geo.polygons(coll)) == 1
+    assert geo.is_geometry(C.box(0, 0, 1, 1)) and not geo.is_geometry({"A": C.box(0, 0, 1, 1)})
+
+
+def _oracle_inside(polys, pts):
+    a, b = R.locate(polys, np.asarray(pts)[:, 0], np.asarray(pts)[:, 1])
+    return a | b
+
+
+@pytest.mark.parametrize("geom", [C.touching_fills()["P"], C.two_nets_and_island()["A"],
+                                  geo.MultiPolygon([C.box(0, 0, 10, 10, holes=[[(2, 2), (8, 2), (8, 8), (2, 8)]]),
+                                                    C.box(4, 4, 6, 6), C.box(10.003, 0, 12, 3)])])
+def test_pieces_match_the_oracle(geom):
+    mine = geo.pieces(geo.polygons(geom), _oracle_inside)
+    theirs = R.pieces(geom)
+    assert [len(p) for p in mine] == [len(p) for p in theirs]
+    for p, q in zip(mine, theirs):
+        for a, b in zip(p, q):
+            assert np.array_equal(a[0], b[0])
+
+
+def test_host_steps_match_the_oracle():
+    rng = np.random.default_rng(7)
+    cov = np.where(rng.random((9, 13)) < 0.5, rng.random((9, 13)), 0.0)
+    has = rng.random((9, 13)) < 0.4
+    for steps, reach in ((((0, 1), (0, -1)), 3), (((1, 0), (-1, 0), (0, 1), (0, -1)), 2)):
+        assert np.allclose(RS._adopt(cov, has, steps, reach), R.adopt(cov, has, steps, reach), rtol=0, atol=TOL)
+    for axis in (0, 1):
+        h = rng.random((11, 17)) < 0.25
+        face = np.where(h, rng.uniform(0.05, 1.2, h.shape), 0.0)
+        samples = [np.where(h & (rng.random(h.shape) < 0.9), rng.uniform(0.05, 1.2, h.shape), 0.0) for _ in range(6)]
+        assert np.allclose(RS._series_factor(face, samples, h, axis), R.series_factor(face, samples, h, axis),
+                           rtol=0, atol=TOL)
+    polys = geo.polygons(C.webbed_plane())
+    E = geo.oriented_edges(polys)
+    pa, pb = R.ring_edges(polys)
+    assert np.array_equal(E, np.hstack([pa, pb]))
+    right = rng.random((40, 20))
+    assert np.allclose(RS._cell_areas(E, 0.13, -0.21, 0.5, 40, 20, right),
+                       R.cell_areas(pa, pb, 0.13, -0.21, 0.5, 40, 20, right), rtol=0, atol=TOL)
+    ex = np.where(rng.random((8, 9)) < 0.5, rng.uniform(0, 2, (8, 9)), 0.0)
+    args = (rng.random((8, 9)) < 0.7, np.where(rng.random((8, 9)) < 0.5, rng.random((8, 9)), 0.0),
+            rng.uniform(-0.5, 0.5, (8, 9)), np.where(rng.random((8, 9)) < 0.2, 0.3, 0.0), 0.5)
+    for a, b in zip(RS._strip_widths(*args, ex), R.strip_widths(*args, ex)):
+        assert np.array_equal(a, b)
+
+
+def test_track_walk_matches_the_oracle():
+    pts, geom = C.track_through_pour()
+    for d in (0.86, 0.5, 0.3):
+        g = R.build_grid(geom, (0, 0, 20, 10), d)           # sub-grid 1: labels only
+        k = g.nets.index("A")
+        outs = []
+        for walk in (RS._walk_tracks, R.rasterise_lines):
+            lab, seeded = g.label.copy(), np.zeros(g.label.shape, bool)
+            fx, fy = np.zeros((g.nx - 1, g.ny)), np.zeros((g.nx, g.ny - 1))
+            br = walk(k, [(pts, 0.4)], g.x0, g.y0, d, lab, seeded, fx, fy, lab, None)
+            outs.append((lab, seeded, fx, fy, br))
+        for a, b in zip(outs[0][:4], outs[1][:4]):
+            assert np.array_equal(a, b)
+        assert outs[0][4] == outs[1][4]
+
+
+# ---------------------------------------------------------------------------------------------- GPU
+def _cases():
+    pts, pour = C.track_through_pour()
+    return {
+        "webbed_sub1": (C.webbed_plane(), (0, 0, 20, 10), 0.5, {}),
+        "webbed_sub5": (C.webbed_plane(), (0, 0, 20, 10), 0.5, {"sub": 5}),
+        "slotted_sub5": (C.slotted_plane(), (0, 0, 20, 10), 0.5, {"sub": 5}),
+        "perforated_sub4": (C.perforated_plane(), (0, 0, 20, 10), 0.25, {"sub": 4}),
+        "track_sub6": (pour, (0, 0, 20, 10), 0.86, {"sub": 6, "seeds": {"A": [pts[0], pts[-1]]},
+                                                    "lines": {"A": [(pts, 0.4)]}}),
+        "track_sub2": (pour, (0, 0, 20, 10), 0.3, {"sub": 2, "seeds": {"A": [pts[0], pts[-1]]},
+                                                   "lines": {"A": [(pts, 0.4)]}}),
+        "track_sub1": (pour, (0, 0, 20, 10), 0.43, {"seeds": {"A": [pts[0], pts[-1]]}, "lines": {"A": [(pts, 0.4)]}}),
+        "sp27_sub7": (C.narrow_strip(12.4), (0, 0, 20, 24), 1.0, {"sub": 7}),
+        "necked_sub4": (C.necked_track(), (0, 0, 20, 10), 0.2, {"sub": 4}),
+        "angled_sub3": (C.angled_track(30.0)[1], (0, 0, 20, 12), 0.35, {"sub": 3,
+                                                                       "lines": {"T": [(C.angled_track(30.0)[0], 0.3)]}}),
+        "island_sub1": (C.two_nets_and_island(), (0, 0, 20, 8), 0.1, {}),
+        "touching_sub4": (C.touching_fills(), (0, 0, 15, 5), 0.5, {"sub": 4}),
+        "diagonal_sub1": (C.diagonal_lattice(), (0, 0, 17, 10), 0.5, {}),
+        "diagonal_sub4": (C.diagonal_lattice(), (0, 0, 17, 10), 0.5, {"sub": 4}),
+        "overlap_sub1": (C.overlapping_nets(), (0, 0, 16, 10), 0.25, {}),
+        "overlap_sub3": (C.overlapping_nets(), (0, 0, 16, 10), 0.25, {"sub": 3}),
+        "pads_sub1": ({"A": C.box(-0.25, -0.25, 20.25, 2.25), "B": C.box(24.75, -0.25, 26.25, 1.25)},
+                      (0.0, 0.0, 26.0, 2.0), 0.5, {}),
+    }
+
+
+CASES = _cases()
+EXACT = ("node", "xb", "yb", "label", "fill")
+NEAR = ("wx", "wy", "ox", "oy", "area", "rx", "ry", "hx", "hy")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_build_grid_matches_the_oracle(name):
+    geom, bbox, d, kw = CASES[name]
+    g = RS.build_grid(geom, bbox, d, **kw)
+    o = R.build_grid(geom, bbox, d, **kw)
+    assert (g.nx, g.ny, g.nets) == (o.nx, o.ny, o.nets)
+    for f in EXACT:
+        a, b = getattr(g, f), getattr(o, f)
+        assert (a is None) == (b is None), f
+        if a is not None:
+            assert np.array_equal(a, b), f
+    assert g.bridges == o.bridges
+    for f in NEAR:
+        a, b = getattr(g, f), getattr(o, f)
+        assert (a is None) == (b is None), f
+        if a is not None:
+            assert np.max(np.abs(a - b), initial=0.0) <= TOL, f
+    if g.area is not None:
+        assert np.allclose(g.copper_area(), R.copper_area(o), rtol=0, atol=TOL)
+
+
+@pytest.mark.gpu
+def test_device_covers_and_points_match_the_oracle():
+    polys = geo.polygons(C.perforated_plane()) + geo.polygons(C.diagonal_lattice()["D"])
+    E = geo.oriented_edges(polys)
+    dev = RS._Device(0)
+    for (u0, du, nl, v0, d, nc) in ((-0.3, 0.29, 75, -0.2, 0.37, 30), (0.0, 0.25, 81, 0.0, 0.25, 41),
+                                    (1.0 / 12, 1.0 / 6, 120, 0.5, 1.0, 11)):
+        got = dev.cover(E, u0, du, nl, v0, d, nc)
+        want = R.line_cover(E[:, 0], E[:, 1], E[:, 2], E[:, 3], u0, du, nl, v0, d, nc)
+        assert np.max(np.abs(got - want)) <= TOL
+    rng = np.random.default_rng(11)
+    pts = np.vstack([rng.uniform((-1, -1), (21, 11), (2000, 2)),
+                     np.column_stack([np.repeat(np.arange(0, 20.5, 0.5), 21), np.tile(np.arange(0, 10.5, 0.5), 41)])])
+    ins, onb = dev.locate(polys, pts)
+    a, b = R.locate(polys, pts[:, 0], pts[:, 1])
+    assert np.array_equal(ins, a) and np.array_equal(onb, b)
+    q = dev.nearest(polys, pts[:50])
+    AX, AY, BX, BY = R._edges(polys)
+    for n in range(50):
+        ux, uy = BX - AX, BY - AY
+        L = ux * ux + uy * uy
+        t = np.clip(((pts[n, 0] - AX) * ux + (pts[n, 1] - AY) * uy) / L, 0.0, 1.0)
+        qx, qy = AX + t * ux, AY + t * uy
+        e = int(np.argmin((qx - pts[n, 0]) ** 2 + (qy - pts[n, 1]) ** 2))
+        assert (q[n, 0], q[n, 1]) == (qx[e], qy[e])
+
+
+@pytest.mark.gpu
+def test_a_large_board_measures_its_copper():
+    """60 x 60 mm at 0.1 mm, sub-grid 4: a ground pour with 900 antipads and a second net's island; the nodes' areas
+    add up to the copper's area (each antipad's copper has a node of its net within reach), the fill to within the
+    sub-grid's sampling, and every strip's width is positive."""
+    holes = [geo.disc(2 + 1.9 * i, 2 + 1.9 * j, 0.35, 16)[::-1] for i in range(30) for j in range(30)]
+    gnd = geo.box(0.05, 0.05, 59.95, 59.95, holes=holes)
+    geom = {"GND": gnd, "PWR": geo.box(70, 70, 71, 71)}
+    g = RS.build_grid(geom, (0, 0, 60, 60), 0.1, sub=4)
+    assert g.nx == 601 and g.ny == 601
+    on = g.label == 0
+    assert float(np.sum(np.where(on, g.area, 0.0))) * 0.01 == pytest.approx(gnd.area, rel=1e-9)
+    assert float(np.sum(g.fill)) * 0.01 == pytest.approx(gnd.area, rel=0.01)
+    assert np.all(g.wx[g.xb] > 0) and np.all(g.wy[g.yb] > 0)
+    assert not np.any(g.label == 1)
+
+
+def test_the_solver_refuses_narrow_strips():
+    """A grid measured on a sub-grid has strips narrower than the pitch (SP-27), whose inductance and resistance
+    corrections (peec.py:1732-1843) this library does not make: the stack refuses it rather than solve it as full."""
+    from groundtruth_amd import peec
+    from groundtruth_amd.problem import PlaneLayer
+    g = R.build_grid(C.narrow_strip(12.4), (0, 0, 20, 24), 1.0, sub=7)
+    with pytest.raises(NotImplementedError):
+        peec.PlaneStack([PlaneLayer("N", 0.0, g, 35e-6)])
+
+
+@pytest.mark.gpu
+def test_a_rasterised_board_solves_like_the_oracle():
+    """Two planes drawn as polygons (a ground pour with antipads and a slotted plane), rasterised on the device at
+    0.25 mm, stitched by vias, driven by a route above: the device solve equals the oracle's on the oracle's own
+    rasterisation of the same polygons (I and Z_loop to 1e-9)."""
+    from groundtruth_amd import boards, peec
+    from groundtruth_amd.problem import PlaneLayer
+    from oracle import peec as O
+    d = 0.25
+    top = {"GND": geo.box(0.1, 0.1, 11.9, 7.9, holes=[geo.disc(2 + 2.5 * i, 4.0, 0.3, 16)[::-1] for i in range(4)])}
+    low = {"PWR": geo.MultiPolygon([geo.box(0.1, 0.1, 5.8, 7.9), geo.box(6.1, 0.1, 11.9, 7.9),
+                                    geo.box(5.7, 6.0, 6.2, 7.9)])}
+    bbox = (0.0, 0.0, 12.0, 8.0)
+    z_top, z_low = 0.0, -(0.2e-3 + boards.CU_T)
+    layers, o_layers = [], []
+    for name, geom, z in (("L1", top, z_top), ("L2", low, z_low)):
+        g = RS.build_grid(geom, bbox, d)
+        o = R.build_grid(geom, bbox, d)
+        assert np.array_equal(g.node, o.node) and np.array_equal(g.xb, o.xb) and np.array_equal(g.yb, o.yb)
+        layers.append(PlaneLayer(name, z, g, boards.CU_T))
+        o_layers.append(PlaneLayer(name, z, o, boards.CU_T))
+    vias = []
+    for (i, j) in ((4, 4), (20, 28), (40, 8), (44, 27)):
+        vias += boards._stitch(layers, i, j, f"via{i}_{j}")
+    route = boards._route([(4, 16), (24, 16), (24, 24), (44, 24)], 0.15e-3, 0.0, d * 1e-3, 0.15e-3, 0)
+    st = peec.PlaneStack(layers, vias)
+    sol = peec.RouteExcitation(st, route.trace, route.driver, route.receiver).solve(1e8, tol=1e-12)
+    ref = O.RouteExcitation(O.PlaneStack(o_layers, vias), route.trace, route.driver, route.receiver).solve(
+        1e8, tol=1e-12)
+    I = np.concatenate([sol.I_x.ravel(), sol.I_y.ravel()])
+    I0 = np.concatenate([ref.I_x.ravel(), ref.I_y.ravel()])
+    assert np.linalg.norm(I - I0) <= 1e-9 * np.linalg.norm(I0)
+    assert abs(sol.Z_loop - ref.Z_loop) <= 1e-9 * abs(ref.Z_loop)
