@@ -20,7 +20,9 @@ the served backlog against the plain FP64 solver one solve at a time (serve_back
 frequency (best of 3, with the host's CPU model, affinity and library versions); and the six-layer 2048^2 board
 (large: FP64, FP32-basis and mixed solves to 1e-8 with their solves/s and errors, its L_p product's GB/s against peak,
 and the same product on the host); and the four-layer 512^2 board's 256 right-hand sides (batched: 64 nets x 4
-frequencies in blocks of 16, mixed to 1e-8, FP64 on two of the blocks beside it, with a sample's errors).
+frequencies in blocks of 16, mixed to 1e-8, FP64 on two of the blocks beside it, with a sample's errors); and the
+copper rasteriser (raster: build_grid of a 200 x 200 mm pour with 9801 antipads at 0.1 mm, sub-grid 4, against the
+oracle's rate on a 10 mm cut).
 
 Progress lines go to stderr; stdout carries the one JSON line.
 
@@ -29,6 +31,7 @@ python bench.py --lp-only [--workload W] [--lp-reps R] [--lp-batch N]   (the cou
                                                          right-hand sides, then a 64 MiB device copy as the scale)
 python bench.py --large      (the six-layer 2048^2 board alone)
 python bench.py --batched    (the four-layer 512^2 board's 256 right-hand sides alone)
+python bench.py --raster     (the copper rasteriser's 200 x 200 mm board alone)
 """
 from __future__ import annotations
 
@@ -612,6 +615,47 @@ def serve_whatif_512(rate: float = WHATIF_RATE, seconds: float = WHATIF_SECONDS,
             "target_p99_ms": 200.0}
 
 
+RASTER_MM = 200.0               # the rasteriser's board: 200 x 200 mm, antipads every 2 mm
+RASTER_D = 0.1                 # pitch (mm)
+RASTER_SUB = 4                 # sub-grid factor
+RASTER_SAMPLE_MM = 10.0        # the oracle's sample of the same pattern
+
+
+def raster(reps: int = 3, cpu: bool = True) -> dict:
+    """The copper rasteriser (build_grid) on a 200 x 200 mm ground pour with 9801 antipads at 0.1 mm and a sub-grid
+    of 4 (2001^2 nodes, 8004^2 fine samples): seconds per grid (best of `reps`, from the polygons to every Grid field),
+    fine samples per second, and the oracle's rate on a 10 x 10 mm cut of the same pattern (its loops take minutes
+    at 20 mm)."""
+    from groundtruth_amd import boards
+    from groundtruth_amd import raster as RS
+    geom = boards.antipad_pour(RASTER_MM)
+    bbox = (0.0, 0.0, RASTER_MM, RASTER_MM)
+    RS.build_grid(boards.antipad_pour(20.0), (0.0, 0.0, 20.0, 20.0), RASTER_D, sub=RASTER_SUB)      # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = RS.build_grid(geom, bbox, RASTER_D, sub=RASTER_SUB)
+        times.append(time.perf_counter() - t0)
+    fine = (g.nx * RASTER_SUB) * (g.ny * RASTER_SUB)
+    out = {"workload": f"{RASTER_MM:.0f} x {RASTER_MM:.0f} mm ground pour, {len(geom['GND'].rings) - 1} antipads "
+                       f"(64-gons), d {RASTER_D} mm, sub-grid {RASTER_SUB}: {g.nx}^2 nodes, {g.nx * RASTER_SUB}^2 "
+                       "fine samples",
+           "seconds_per_grid": min(times), "fine_samples_per_s": fine / min(times), "nodes": int(g.node.sum()),
+           "strips": int(g.xb.sum() + g.yb.sum())}
+    if cpu:
+        from oracle import raster as OR
+        small = boards.antipad_pour(RASTER_SAMPLE_MM)
+        t0 = time.perf_counter()
+        o = OR.build_grid(small, (0.0, 0.0, RASTER_SAMPLE_MM, RASTER_SAMPLE_MM), RASTER_D, sub=RASTER_SUB)
+        dt = time.perf_counter() - t0
+        n_small = (o.nx * RASTER_SUB) * (o.ny * RASTER_SUB)
+        out["cpu_oracle"] = {"fine_samples_per_s": n_small / dt, "seconds": dt, "cores": 1, "kind": "port",
+                             "sample": f"the oracle's build_grid of a {RASTER_SAMPLE_MM:.0f} x {RASTER_SAMPLE_MM:.0f} "
+                                       f"mm cut of the same pattern ({o.nx}^2 nodes, {n_small} fine samples)"}
+        out["speedup_vs_cpu_oracle"] = out["fine_samples_per_s"] / out["cpu_oracle"]["fine_samples_per_s"]
+    return out
+
+
 def host_info() -> dict:
     """The host the CPU numbers ran on: cores, the affinity the process got, the CPU model, numpy and scipy."""
     import scipy
@@ -674,6 +718,8 @@ def main():
     ap.add_argument("--batched", action="store_true", help="the four-layer 512^2 board's 256 right-hand sides alone")
     ap.add_argument("--no-batched", action="store_true", help="skip the four-layer 256-right-hand-side case")
     ap.add_argument("--no-large", action="store_true", help="skip the six-layer board (about three minutes)")
+    ap.add_argument("--raster", action="store_true", help="the copper rasteriser's board alone")
+    ap.add_argument("--no-raster", action="store_true", help="skip the copper rasteriser's board")
     ap.add_argument("--steps-only", choices=["fp64", "fp32-basis", "mixed"],
                     help="only the workload's batched steps in this precision, then 64 MiB copies (for profiles)")
     args = ap.parse_args()
@@ -693,6 +739,9 @@ def main():
         return
     if args.batched:
         print(json.dumps({"batched": batched()}))
+        return
+    if args.raster:
+        print(json.dumps({"raster": raster(cpu=not args.no_cpu_baseline)}))
         return
     if args.serve_only:
         print(json.dumps({"served_backlog": serve_backlog(),
@@ -888,6 +937,12 @@ def main():
             out["large"] = large(cpu=not args.no_cpu_baseline)
         except Exception as e:      # noqa: BLE001 - the headline line above stands on its own
             out["large"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_raster:
+        try:
+            log("the copper rasteriser: 200 x 200 mm board")
+            out["raster"] = raster(cpu=not args.no_cpu_baseline)
+        except Exception as e:      # noqa: BLE001 - the headline line above stands on its own
+            out["raster"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

@@ -304,3 +304,15 @@ def scaled(name: str, n: int, **kw) -> Board:
     full = {"single_plane": (64, 0.5), "two_layer": (256, 0.25), "four_layer": (512, 0.2), "large": (2048, 0.05)}
     n0, p0 = full[name]
     return WORKLOADS[name](n=n, pitch_mm=p0 * n0 / n, **kw)
+
+
+def antipad_pour(size_mm: float = 200.0, pitch_mm: float = 2.0, radius_mm: float = 0.3, seed: int = 1006) -> dict:
+    """Copper for the rasteriser: a size x size mm ground pour 0.5 mm in from the board edge, with an antipad (a
+    16-segment-a-quadrant circle of radius_mm) every pitch_mm, jittered by up to 0.3 mm: {"GND": geometry}."""
+    from .geometry import box, disc
+    rng = np.random.default_rng(seed)
+    holes = []
+    for x in np.arange(2.0, size_mm - 1.5, pitch_mm):
+        for y in np.arange(2.0, size_mm - 1.5, pitch_mm):
+            holes.append(disc(x + rng.uniform(-0.3, 0.3), y + rng.uniform(-0.3, 0.3), radius_mm, 16)[::-1])
+    return {"GND": box(0.5, 0.5, size_mm - 0.5, size_mm - 0.5, holes=holes)}

@@ -109,24 +109,34 @@ def bounds(polys: list) -> tuple:
     return float(pts[:, 0].min()), float(pts[:, 1].min()), float(pts[:, 0].max()), float(pts[:, 1].max())
 
 
+def _ring_rows(rings: list) -> np.ndarray:
+    """(ax, ay, bx, by) rows of rings' edges, ring after ring, each ring closed back to its first point."""
+    if not rings:
+        return np.zeros((0, 4))
+    pts = np.concatenate(rings)
+    n = np.array([len(r) for r in rings])
+    start = np.repeat(np.cumsum(n) - n, n)
+    nxt = np.arange(len(pts)) + 1
+    ends = np.cumsum(n) - 1
+    nxt[ends] = start[ends]
+    return np.ascontiguousarray(np.hstack([pts, pts[nxt]]))
+
+
 def edges(polys: list) -> np.ndarray:
     """Every ring edge as (ax, ay, bx, by) rows, ring by ring, as stored."""
-    if not polys:
-        return np.zeros((0, 4))
-    rows = [np.hstack([r, np.roll(r, -1, axis=0)]) for p in polys for r in p]
-    return np.ascontiguousarray(np.concatenate(rows))
+    return _ring_rows([r for p in polys for r in p])
 
 
 def oriented_edges(polys: list) -> np.ndarray:
     """Ring edges with each exterior counter-clockwise and each hole clockwise (the copper on every edge's left;
     peec.py:514-530), as (ax, ay, bx, by) rows."""
-    rows = []
+    rings = []
     for p in polys:
         for h, r in enumerate(p):
             if (_signed_area(r) > 0) != (h == 0):
                 r = np.concatenate([r[:1], r[:0:-1]])
-            rows.append(np.hstack([r, np.roll(r, -1, axis=0)]))
-    return np.ascontiguousarray(np.concatenate(rows)) if rows else np.zeros((0, 4))
+            rings.append(r)
+    return _ring_rows(rings)
 
 
 def _segment_distances(P: np.ndarray, Q: np.ndarray) -> float:

@@ -105,35 +105,36 @@ def _pack(groups: list):
     return eptr, E
 
 
-def _shift(a: np.ndarray, di: int, dj: int, fill) -> np.ndarray:
-    """b[i, j] = a[i + di, j + dj] (fill off the array)."""
-    b = np.full_like(a, fill)
-    ni, nj = a.shape
-    b[max(-di, 0):ni - max(di, 0), max(-dj, 0):nj - max(dj, 0)] = a[max(di, 0):ni + min(di, 0),
-                                                                    max(dj, 0):nj + min(dj, 0)]
-    return b
-
-
 def _adopt(cov: np.ndarray, has: np.ndarray, steps, reach: int = 1) -> np.ndarray:
     """peec.py:628-656: `cov` where `has`, and each value where not (orphan copper) added to the nearest cell along
     `steps` that has, up to `reach` steps away (of equals, the one with more copper of its own, then the first step);
-    orphans with none in reach are dropped."""
+    orphans with none in reach are dropped. Worked over the orphan cells only; each cell receives the moved values in
+    step order, one per step at most."""
     own = np.where(has, cov, 0.0)
-    orphan = np.where(has, 0.0, cov)
     out = own.copy()
+    oi, oj = np.nonzero(~has & (cov > 0))
+    val = cov[oi, oj]
+    ni, nj = cov.shape
     for k in range(1, reach + 1):
-        if not np.any(orphan > 0):
+        if len(oi) == 0:
             break
-        best = np.full(cov.shape, -1.0)
-        pick = np.full(cov.shape, -1)
+        best = np.full(len(oi), -1.0)
+        pick = np.full(len(oi), -1)
         for s, (di, dj) in enumerate(steps):
-            nb_has, nb_cov = _shift(has, di * k, dj * k, False), _shift(own, di * k, dj * k, 0.0)
-            better = (orphan > 0) & nb_has & (nb_cov > best)
-            best = np.where(better, nb_cov, best)
+            ti, tj = oi + di * k, oj + dj * k
+            ok = (ti >= 0) & (ti < ni) & (tj >= 0) & (tj < nj)
+            nb = np.full(len(oi), -1.0)
+            sel = np.nonzero(ok)[0]
+            hs = has[ti[sel], tj[sel]]
+            nb[sel[hs]] = own[ti[sel[hs]], tj[sel[hs]]]
+            better = nb > best                  # a neighbour with copper of the net (own >= 0) beats none (-1)
+            best = np.where(better, nb, best)
             pick = np.where(better, s, pick)
         for s, (di, dj) in enumerate(steps):
-            out += _shift(np.where(pick == s, orphan, 0.0), -di * k, -dj * k, 0.0)
-        orphan = np.where(pick >= 0, 0.0, orphan)
+            m = pick == s
+            out[oi[m] + di * k, oj[m] + dj * k] += val[m]
+        keep = pick < 0
+        oi, oj, val = oi[keep], oj[keep], val[keep]
     return out
 
 
@@ -162,16 +163,19 @@ def _series_factor(face: np.ndarray, samples: list, has: np.ndarray, axis: int) 
     before = np.full(nk, far)
     before[key[first]] = gap[first]
     after = np.where(np.r_[size[1:] > 0, False], np.r_[before[1:], far], far)
-    alone = (size <= BAND_STRIPS) & (before > 2 * ADOPT_FACES) & (after > 2 * ADOPT_FACES)
-    inv, ok = np.zeros(nk), np.ones(nk, bool)
-    for s in rows:
-        t = np.bincount(key, s[h], nk)
-        ok &= t > 0
-        inv += 1.0 / np.maximum(t, 1e-300)
-    harmonic = len(rows) / np.maximum(inv, 1e-300)
-    factor = np.where(alone & ok & (total > 0), harmonic / np.maximum(total, 1e-300), 1.0)
+    alone = (size <= BAND_STRIPS) & (before > 2 * ADOPT_FACES) & (after > 2 * ADOPT_FACES) & (total > 0)
     out = np.ones(h.shape)
-    out[h] = factor[key]
+    lone = np.nonzero(alone[key])[0]            # the strips of lone bands: only they can take a factor
+    if len(lone):
+        kl = key[lone]
+        inv, ok = np.zeros(nk), np.ones(nk, bool)
+        for s in rows:
+            t = np.bincount(kl, s[ri[lone], ci[lone]], nk)
+            ok &= t > 0
+            inv += 1.0 / np.maximum(t, 1e-300)
+        harmonic = len(rows) / np.maximum(inv, 1e-300)
+        factor = np.where(alone & ok, harmonic / np.maximum(total, 1e-300), 1.0)
+        out[ri[lone], ci[lone]] = factor[kl]
     return out if axis == 1 else out.T
 
 
