@@ -61,6 +61,7 @@ SIGNATURES = {
     "gt_stack_create": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(C.c_void_p)]),
     "gt_stack_create_gen": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(KernelDesc), C.POINTER(C.c_void_p)]),
     "gt_stack_kernel": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
+    "gt_stack_set_narrow": (C.c_int, [C.c_void_p, f64p, f64p, C.c_int64, i32p, i32p, f64p]),
     "gt_stack_lp_autotune": (C.c_int, [C.c_void_p, C.c_int32, i32p, i32p]),
     "gt_stack_set_lp_plan": (C.c_int, [C.c_void_p, i32p, i32p]),
     "gt_node_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),

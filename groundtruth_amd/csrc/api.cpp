@@ -215,6 +215,8 @@ int64_t gt_stack_s::device_bytes() const {
     for (const auto& t : tw) b += t.second.bytes();
     for (const auto& t : twf) b += t.second.bytes();
     b += node_sheet.bytes() + strip_sheet.bytes();
+    b += core->nw_fac.bytes() + core->nw_dl.bytes() + core->nw_val.bytes() + core->nw_ptr.bytes() +
+         core->nw_col.bytes();
     if (schur) b += schur->device_bytes();
     return b;
 }
@@ -633,6 +635,7 @@ int gt_lp_matvec(gt_stack* st, const void* I_dev, void* out_dev) {
         a.rhs.x[0] = (const zd*)I_dev;
         a.rhs.y[0] = (zd*)out_dev;
         launch_lp(a, st->shape, st->ctx->stream);
+        narrow_apply<double>(st, 1, nullptr, a.rhs.x, a.rhs.y);
     });
 }
 
@@ -651,6 +654,7 @@ int gt_lp_matvec_batch(gt_stack* st, int32_t nr, const void* const* I_dev, void*
             a.rhs.y[i] = (zd*)out_dev[i];
         }
         launch_lp(a, st->shape, st->ctx->stream);
+        narrow_apply<double>(st, nr, nullptr, a.rhs.x, a.rhs.y);
     });
 }
 
@@ -680,6 +684,42 @@ int gt_lp_matvec_batch_noalloc(gt_stack* st, int32_t nr, const void* const* I_de
             a.rhs.y[i] = (zd*)out_dev[i];
         }
         launch_lp(a, st->shape, st->ctx->stream);
+        narrow_apply<double>(st, nr, nullptr, a.rhs.x, a.rhs.y);
+    });
+}
+
+int gt_stack_set_narrow(gt_stack* st, const double* r_factor, const double* dL, int64_t nnz, const int32_t* rowptr,
+                        const int32_t* col, const double* val) {
+    return guard([&] {
+        need(st, "NULL stack");
+        need(nnz >= 0 && (nnz == 0 || (rowptr && col && val)), "bad corrections");
+        std::lock_guard<std::mutex> lk(st->ctx->mu);
+        GT_HIP(hipSetDevice(st->ctx->device));
+        const int ns = st->ns;
+        if (ns == 0) return;
+        std::vector<double> fac(ns, 0.0), dl(ns, 0.0);
+        for (int s = 0; s < ns; ++s) {
+            if (r_factor) fac[s] = r_factor[s] - 1.0;
+            if (dL) dl[s] = dL[s];
+        }
+        std::vector<int> ptr(ns + 1, 0), cc(std::max<int64_t>(1, nnz), 0);
+        std::vector<double> vv(std::max<int64_t>(1, nnz), 0.0);
+        if (nnz) {
+            need(rowptr[0] == 0 && rowptr[ns] == nnz, "CSR row pointers do not span the entries");
+            for (int s = 0; s < ns; ++s) need(rowptr[s + 1] >= rowptr[s], "CSR row pointers decrease");
+            for (int64_t k = 0; k < nnz; ++k) need(col[k] >= 0 && col[k] < ns, "CSR column off the strips");
+            std::copy(rowptr, rowptr + ns + 1, ptr.begin());
+            std::copy(col, col + nnz, cc.begin());
+            std::copy(val, val + nnz, vv.begin());
+        }
+        gt::StackCore& c = *st->core;
+        hipStream_t s = st->ctx->stream;
+        c.nw_fac.upload(fac, s);
+        c.nw_dl.upload(dl, s);
+        c.nw_ptr.upload(ptr, s);
+        c.nw_col.upload(cc, s);
+        c.nw_val.upload(vv, s);
+        c.narrow = true;
     });
 }
 

@@ -425,6 +425,79 @@ LpArgs<T> lp_args_t(const gt_stack* st) {
 
 }  // namespace
 
+namespace {
+
+template <class T>
+struct NarrowArgs {
+    const double* fac;              // ns: 1 / r_width - 1
+    const double* dl;               // ns: the narrow self term (H)
+    const int* ptr;                 // side-by-side corrections (H), CSR over the strips
+    const int* col;
+    const double* val;
+    const unsigned char* sheet;     // ns
+    int ns;
+    const cpx<T>* x[KMAX];
+    cpx<T>* y[KMAX];
+    const cpx<T>* r_sheet[KMAX];    // NULL: the L_p product's corrections alone
+    T w[KMAX];
+};
+
+// y_s += z_sheet (1/r_width - 1) x_s + j w (dL_s x_s + sum_k M_sk x_k) (peec.py:2129-2132, :2159); one thread per strip,
+// blockIdx.y the right-hand side; the neighbours summed in CSR order
+template <class T>
+__global__ __launch_bounds__(256) void op_narrow(NarrowArgs<T> a) {
+    const int s = blockIdx.x * 256 + threadIdx.x, p = blockIdx.y;
+    if (s >= a.ns) return;
+    const cpx<T>* x = a.x[p];
+    const cpx<T> xs = x[s];
+    const T dl = (T)a.dl[s];
+    T lr = dl * xs.re, li = dl * xs.im;
+    for (int k = a.ptr[s]; k < a.ptr[s + 1]; ++k) {
+        const T m = (T)a.val[k];
+        const cpx<T> xk = x[a.col[k]];
+        lr += m * xk.re;
+        li += m * xk.im;
+    }
+    cpx<T> add;
+    if (a.r_sheet[p]) {
+        const cpx<T> z = a.r_sheet[p][a.sheet[s]];
+        const T f = (T)a.fac[s], w = a.w[p];
+        add = cpx<T>(f * (z.re * xs.re - z.im * xs.im) - w * li, f * (z.re * xs.im + z.im * xs.re) + w * lr);
+    } else {
+        add = cpx<T>(lr, li);
+    }
+    cpx<T>* y = a.y[p];
+    y[s] = y[s] + add;
+}
+
+}  // namespace
+
+template <class T>
+void narrow_apply(const gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y) {
+    if (!st->core->narrow || st->ns == 0) return;
+    const StackCore& c = *st->core;
+    NarrowArgs<T> a{};
+    a.fac = c.nw_fac.p;
+    a.dl = c.nw_dl.p;
+    a.ptr = c.nw_ptr.p;
+    a.col = c.nw_col.p;
+    a.val = c.nw_val.p;
+    a.sheet = c.strip_sheet.p;
+    a.ns = st->ns;
+    for (int p = 0; p < nr; ++p) {
+        a.x[p] = x[p];
+        a.y[p] = y[p];
+        if (ops) {
+            if constexpr (std::is_same<T, double>::value) a.r_sheet[p] = ops[p]->r_sheet.p;
+            else a.r_sheet[p] = ops[p]->r_sheet_f.p;
+            a.w[p] = (T)ops[p]->w;
+        }
+    }
+    hipLaunchKernelGGL((op_narrow<T>), dim3((unsigned)((st->ns + 255) / 256), nr), dim3(256), 0, st->ctx->stream,
+                       a);
+    GT_LAUNCH("op_narrow");
+}
+
 template <class T>
 void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y) {
     check_batch(st, nr, ops);
@@ -479,6 +552,7 @@ void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x,
     }
     ta.rhs = rs;
     launch_tail_bottom(ta, s);
+    narrow_apply<T>(st, nr, ops, x, y);
 }
 
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr) {
@@ -781,6 +855,8 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
 }
 
 template void ops_matvec<double>(gt_stack*, int, gt_op* const*, const zd* const*, zd* const*);
+template void narrow_apply<double>(const gt_stack*, int, gt_op* const*, const zd* const*, zd* const*);
+template void narrow_apply<float>(const gt_stack*, int, gt_op* const*, const zf* const*, zf* const*);
 template void ops_matvec<float>(gt_stack*, int, gt_op* const*, const zf* const*, zf* const*);
 template void ops_precond<double>(gt_stack*, int, gt_op* const*, const zd* const*, zd* const*);
 template void ops_precond<float>(gt_stack*, int, gt_op* const*, const zf* const*, zf* const*);

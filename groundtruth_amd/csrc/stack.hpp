@@ -99,6 +99,12 @@ struct StackCore {
     std::vector<int> kgmap;
     DevBuf<int> kgmap_dev;
     std::unique_ptr<Schur> schur;
+    // strips narrower than the pitch (gt_stack_set_narrow, peec.py:1732-1843): per strip its resistance factor less
+    // one (1 / r_width - 1) and its narrow self term (H), and the side-by-side neighbours' mutual corrections (H) as a
+    // CSR over the strips; absent while every strip is a pitch wide
+    bool narrow = false;
+    DevBuf<double> nw_fac, nw_dl, nw_val;
+    DevBuf<int> nw_ptr, nw_col;
 };
 
 }  // namespace gt
@@ -176,6 +182,10 @@ namespace gt {
 void check_batch(const gt_stack* st, int nr, gt_op* const* ops);
 template <class T>
 void ops_matvec(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y);
+// y_p += the narrow strips' corrections of x_p (ops.hip): with ops, the operator's (z_sheet (1 / r_width - 1) x +
+// j w (dL x + M_near x)); without, the L_p product's (dL x + M_near x). Nothing when the stack has none.
+template <class T>
+void narrow_apply(const gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* x, cpx<T>* const* y);
 template <class T>
 void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r, cpx<T>* const* z);
 void schur_apply(gt_stack* st, const zd* b, zd* x, int nr);   // B^-1 b, nr interleaved right-hand sides

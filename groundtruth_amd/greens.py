@@ -69,6 +69,47 @@ def _rule(p: float, q: float, graded_to, panels: int):
     return nodes.ravel(), wts.ravel()
 
 
+def strip_self_numeric(length: float, width):
+    """Self inductance of a flat strip (zero thickness) with its current even across it: 2 int_0^1 (1 - s)
+    M_fil(w s) ds, on PANELS panels graded toward s = 0 (kernel.py:187-206); `width` may be an array."""
+    s, ws = _rule(0.0, 1.0, 0.0, PANELS)
+    w = np.atleast_1d(np.asarray(width, float))
+    out = np.empty(w.size)
+    flat = w.ravel()
+    for a in range(0, len(flat), 2048):
+        u = flat[a:a + 2048, None] * s[None, :]
+        out[a:a + 2048] = 2.0 * (filament(length, 0.0, u) * ((1.0 - s) * ws)[None, :]).sum(axis=1)
+    return float(out[0]) if np.ndim(width) == 0 else out.reshape(w.shape)
+
+
+NEAR_AXIAL = 2     # peec.py:3325: side-by-side partial strips corrected up to this many strips along
+
+
+def narrow_self(d: float, phi, t):
+    """What a strip d long and phi d wide of copper t thick adds to the full square strip's self term (peec.py:3309-
+    3322): the flat-strip integral's change, and the thickness's change from the GMD formula [Grover 1946]."""
+    phi, t = np.asarray(phi, float), np.asarray(t, float)
+
+    def thick(w):
+        return filament(d, 0.0, GMD_FLAT * w) - filament(d, 0.0, GMD_FLAT * (w + t))
+    return strip_self_numeric(d, phi * d) - strip_self_numeric(d, d) - thick(phi * d) + thick(d)
+
+
+def side_mutual(length: float, wa, wb, gap, steps=0.0, n: int = 8) -> np.ndarray:
+    """Mutual of two flat coplanar parallel strips `length` long, wa and wb wide, centres gap across and `steps`
+    lengths along apart (fractions of length; arrays broadcast): the mean filament mutual of n filaments across each,
+    touching filaments a filament's GMD apart (peec.py:3328-3341)."""
+    wa, wb, gap, steps = (np.atleast_1d(v) * 1.0 for v in np.broadcast_arrays(*(np.asarray(v, float)
+                                                                                 for v in (wa, wb, gap, steps))))
+    wa, wb, gap = wa * length, wb * length, gap * length
+    u = (np.arange(n) + 0.5) / n - 0.5
+    ya = wa[:, None, None] * u[None, :, None]
+    yb = gap[:, None, None] + wb[:, None, None] * u[None, None, :]
+    rho = np.maximum(np.abs(yb - ya), GMD_FLAT * np.maximum(wa, wb)[:, None, None] / n)
+    axial = np.broadcast_to((steps * length)[:, None, None], rho.shape)
+    return filament(length, axial, rho).mean(axis=(1, 2))
+
+
 def tape(d: float, axial: float, lateral: float, dz: float) -> float:
     """Mutual of two parallel square tapes of side d (zero thickness), centres `axial` apart along them and
     `lateral` across, in planes dz apart: (1/d^2) int W(v) M_fil(d, axial, |(lateral + v, dz)|) dv with W the

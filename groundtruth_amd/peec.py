@@ -166,18 +166,6 @@ _WOODBURY_FORMS = {"auto": 0, "stored": 1, "implicit": 2}
 _ND_ORDERINGS = {"joint": 0, "sheets": 1}
 
 
-def _narrow_strips(g) -> bool:
-    """A grid whose strips are not all a pitch wide for their inductance and resistance (peec.py:1732-1757:
-    a width of 0 or none counts as full)."""
-    for w, b in ((getattr(g, "wx", None), g.xb), (getattr(g, "wy", None), g.yb),
-                 (getattr(g, "rx", None), g.xb), (getattr(g, "ry", None), g.yb)):
-        if w is not None:
-            v = np.asarray(w)[np.asarray(b, bool)]
-            if np.any((v > 0) & (v != 1.0)):
-                return True
-    return False
-
-
 class PlaneStack:
     """Plane layers and lumped branches resident on one device (peec.py:1555). Unknown order:
     [x strips | y strips | branches]."""
@@ -198,9 +186,7 @@ class PlaneStack:
         for l in layers:
             if (l.grid.nx, l.grid.ny, l.grid.d) != (g0.nx, g0.ny, g0.d):
                 raise SolverError("all layers must share one grid")
-            if _narrow_strips(l.grid):
-                raise NotImplementedError("strips narrower than the pitch (a grid measured on a sub-grid, Grid.wx / wy "
-                                          "/ rx / ry) are not solved by this library: build the grid with sub=1")
+
         # layers of different copper thickness: each sheet's strips have their own admittance (gt_schur_factor_nodes)
         self.per_sheet = len({float(l.thickness) for l in layers}) != 1
         self.precond_mode = precond
@@ -228,6 +214,7 @@ class PlaneStack:
         if screen:
             self._check_no_split()
         self._kernels()
+        self._widths()
         self._handle = None
         self._ops: OrderedDict = OrderedDict()
         self._base: Optional["PlaneStack"] = None
@@ -319,6 +306,103 @@ class PlaneStack:
             return 0.0
         gap = abs(self.sheet_z[s] - self.sheet_z[t]) - 0.5 * (self.sheet_t[s] + self.sheet_t[t])
         return max(gap, 1e-6)
+
+    # ---------------------------------------------------------------- narrow strips (peec.py:1732-1843)
+    def _widths(self):
+        """The strips' widths from the grids (Grid.wx / wy, rx / ry, ox / oy; full where a grid has none):
+        `strip_width` (the inductance's), `r_width` (the resistance's) and, where any strip is narrower than the
+        pitch, `dL` (per strip, H) and `M_near` (CSR over the strips, H): what a narrower strip adds to the full
+        strip's self term, and the change of the side-by-side neighbours' mutual up to NEAR_AXIAL strips along.
+        None of them while every strip is full."""
+        ns = self.ns
+
+        def per_strip(attr, default):
+            vals = []
+            for idx, name in ((self.xa, attr[0]), (self.ya, attr[1])):
+                arrs = [getattr(l.grid, name, None) for l in self.layers]
+                if all(a is None for a in arrs):
+                    vals.append(np.full(len(idx), default))
+                    continue
+                full = np.stack([np.asarray(a, float) if a is not None else
+                                 np.full(self.xb.shape[1:] if name.endswith("x") else self.yb.shape[1:], default)
+                                 for a in arrs])
+                vals.append(full[tuple(idx.T)] if len(idx) else np.zeros(0))
+            return np.concatenate(vals)
+        w = per_strip(("wx", "wy"), 1.0)
+        ok = w > 0
+        self.strip_width = np.where(ok, w, 1.0)
+        rw = per_strip(("rx", "ry"), 0.0)
+        self.r_width = np.where(ok & (rw > 0), rw, self.strip_width)
+        off = np.where(ok, per_strip(("ox", "oy"), 0.0), 0.0)
+        self.dL = self.M_near = None
+        self.narrow = bool(np.any(self.strip_width != 1.0) or np.any(self.r_width != 1.0))
+        narrow = self.strip_width < 1.0
+        if not self.narrow or not np.any(narrow):
+            return
+        d = self.d
+        t = self.sheet_t[np.concatenate([self.xa[:, 0], self.ya[:, 0]])]
+        keys, inv = np.unique(np.round(np.column_stack([self.strip_width[narrow], t[narrow]]), 12), axis=0,
+                              return_inverse=True)
+        dL = np.zeros(ns)
+        dL[narrow] = G.narrow_self(d, keys[:, 0], keys[:, 1])[np.ravel(inv)]
+        self.dL = dL
+        # side-by-side pairs: x strips in rows j and j + 1, y strips in columns i and i + 1, |k| <= NEAR_AXIAL along
+        a_all, b_all, k_all = [], [], []
+        for idx, base, across, shape in ((self.xa, 0, 2, (self.n_sheets, self.nx - 1, self.ny)),
+                                         (self.ya, self.nxb, 1, (self.n_sheets, self.nx, self.ny - 1))):
+            if len(idx) == 0:
+                continue
+            at = -np.ones(shape, np.int64)
+            at[tuple(idx.T)] = base + np.arange(len(idx))
+            along = 1 if across == 2 else 2
+            n_ax = shape[along]
+            for k in range(-G.NEAR_AXIAL, G.NEAR_AXIAL + 1):
+                if abs(k) >= n_ax:
+                    continue
+                sa = [slice(None)] * 3
+                sb = [slice(None)] * 3
+                sa[along] = slice(max(-k, 0), n_ax - max(k, 0))
+                sb[along] = slice(max(k, 0), n_ax - max(-k, 0))
+                sa[across] = slice(0, -1)
+                sb[across] = slice(1, None)
+                A, B = at[tuple(sa)], at[tuple(sb)]
+                both = (A >= 0) & (B >= 0)
+                A, B = A[both], B[both]
+                m = narrow[A] | narrow[B]
+                a_all.append(A[m])
+                b_all.append(B[m])
+                k_all.append(np.full(int(m.sum()), abs(k)))
+        a = np.concatenate(a_all) if a_all else np.zeros(0, np.int64)
+        b = np.concatenate(b_all) if b_all else np.zeros(0, np.int64)
+        if len(a) == 0:
+            return
+        kk = np.concatenate(k_all)
+        gap = 1.0 + off[b] - off[a]
+        keys, inv = np.unique(np.round(np.column_stack([self.strip_width[a], self.strip_width[b], gap, kk]), 9),
+                              axis=0, return_inverse=True)
+        ones = np.ones(len(keys))
+        dM = (G.side_mutual(d, keys[:, 0], keys[:, 1], keys[:, 2], keys[:, 3])
+              - G.side_mutual(d, ones, ones, ones, keys[:, 3]))[np.ravel(inv)]
+        M = sp.coo_matrix((dM, (a, b)), shape=(ns, ns))
+        self.M_near = (M + M.T).tocsr()
+
+    def _set_narrow(self, h):
+        """The narrow strips' terms on the device stack (gt_stack_set_narrow)."""
+        if not self.narrow:
+            return
+        rf = np.ascontiguousarray(1.0 / self.r_width)
+        dl = np.ascontiguousarray(self.dL if self.dL is not None else np.zeros(self.ns))
+        if self.M_near is not None:
+            M = self.M_near.tocsr()
+            M.sort_indices()
+            ptr = np.ascontiguousarray(M.indptr, np.int32)
+            col = np.ascontiguousarray(M.indices, np.int32)
+            val = np.ascontiguousarray(M.data, np.float64)
+        else:
+            ptr = np.zeros(self.ns + 1, np.int32)
+            col, val = np.zeros(1, np.int32), np.zeros(1)
+        _lib.check(_lib.lib().gt_stack_set_narrow(h, _lib.ptr(rf), _lib.ptr(dl), int(ptr[-1]), _lib.ptr(ptr, _lib.i32p),
+                                                 _lib.ptr(col, _lib.i32p), _lib.ptr(val)), "gt_stack_set_narrow")
 
     # ---------------------------------------------------------------- kernels (peec.py:1941-1988)
     def _kernels(self):
@@ -477,6 +561,7 @@ class PlaneStack:
             return
         self._check_fits()
         h = self._create()
+        self._set_narrow(h)
         self._tune(h)
         self._factor(h)
 

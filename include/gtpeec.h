@@ -241,6 +241,15 @@ int gt_raster_cover(gt_ctx* ctx, int64_t n_edges, const double* edges, double u0
 int gt_raster_points(gt_ctx* ctx, int64_t n_edges, const double* edges, int64_t n_points, const double* points,
                      int8_t* inside, int8_t* on_edge, double* nearest);
 
+/* Strips narrower than the pitch (peec.py:1732-1843 `PlaneStack._widths`, applied at :2129-2132 and :2159): per strip
+ * r_factor = 1 / r_width (its resistance and internal impedance are its sheet's / r_width; NULL: all 1) and dL, the
+ * partial self inductance its narrow copper adds to the full strip's (H; NULL: none), and M_near, the side-by-side
+ * neighbours' mutual corrections (H), a CSR over the strips (rowptr ns + 1, col, val; nnz 0: none). The operator
+ * then adds z_sheet (1 / r_width - 1) I + j w (dL I + M_near I) to the strip rows and the L_p product dL I + M_near I;
+ * the preconditioner keeps each sheet's full-strip admittance. Shared with the stack's what-if stacks. */
+int gt_stack_set_narrow(gt_stack* stack, const double* r_factor, const double* dL, int64_t nnz, const int32_t* rowptr,
+                        const int32_t* col, const double* val);
+
 /* The L_p product's launch shapes, tuned (no counterpart in the reference): for each batch-size class (1, 2-4, 5-16
  * right-hand sides) the rows per row-pass workgroup (1, 2, 4) and waves per column-pass workgroup (1, 2, 4) that
  * ran `reps` products of 1, 4 and 8 right-hand sides fastest on this stack. The shapes change no arithmetic, so a

@@ -14,9 +14,12 @@ peec    restates the solve path of `src/groundtruth/solver/peec.py`: topology (`
         (`_gmres`) and the route excitation (`RouteExcitation`, `strip_coupling`, `branch_coupling`, `moment`,
         `_trace_self`, `_trace_resistance`).
 
-Scope: one current sheet per layer (`screen=False`, the Simulate default), the "diag" preconditioner, full-width
-strips, lumped branches (vias). Face sheets, the "sheet" preconditioner, narrow-strip corrections and plane-pair
-capacitance are not restated (no workload here uses them).
+Scope: one current sheet per layer (`screen=False`, the Simulate default), the "diag" preconditioner, strips of
+any measured width (the narrow-strip corrections, `_widths`), lumped branches (vias). Face sheets, the "sheet"
+preconditioner and plane-pair capacitance are not restated (no workload here uses them).
+raster  restates the copper rasterisation of `src/groundtruth/solver/peec.py:329-983` (`build_grid` and its helpers)
+        without shapely: exact point tests, line covers, cell areas, pieces, the sub-grid, adoption, series factors,
+        heat points and the track walk.
 
 How each function is pinned is listed in `tests/test_oracle.py` (golden vectors written from the reference's
 `kernel.py`, closed forms, adaptive quadrature, dense direct solves, brute-force convolutions).

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE (see oracle/__init__.py): only tests, smoke() and bench.py's cpu_baseline use it.
 
 Scope: one sheet per layer (PlaneStack(..., screen=False), the Simulate default, peec.py:1561 / simulate.py:318),
-"diag" preconditioner, full-width strips, lumped branches; no plane-pair capacitance. The classes take the same duck
+"diag" preconditioner, strips of any measured width (`_widths`), lumped branches; no plane-pair capacitance. The classes take the same duck
 types the reference's do: layers with `.z`, `.thickness`, `.grid` (`.nx`, `.ny`, `.d` in mm, `.node`, `.xb`, `.yb`),
 branches with `.a`, `.b` ((layer, i, j) tuples), `.z(f)`, `.kind`, `.radius`, trace segments with `.a`, `.b` (m),
 `.width`, `.i`.
@@ -29,7 +29,7 @@ from scipy.sparse.csgraph import connected_components
 from scipy.sparse.linalg import splu
 
 from .kernel import (GMD_FLAT, filament_mutual, leggauss, segment_mutual, segment_vector_potential,
-                     skin_thickness, slab_impedance, strip_kernel, strip_self)
+                     skin_thickness, slab_impedance, strip_kernel, strip_self, strip_self_numeric)
 
 SIGMA_CU = 5.8e7                # peec.py:67
 VIA_PLATING = 25e-6             # peec.py:68
@@ -105,9 +105,36 @@ class Operator:
     solve_S: Callable
 
 
+NEAR_AXIAL = 2      # peec.py:3325
+
+
+def narrow_self(d: float, phi, t):
+    """What a strip d long and phi d wide adds to the full square strip's self inductance (peec.py:3309-3322)."""
+    phi, t = np.asarray(phi, float), np.asarray(t, float)
+
+    def thick(w):
+        return filament_mutual(d, 0.0, GMD_FLAT * w) - filament_mutual(d, 0.0, GMD_FLAT * (w + t))
+    out = strip_self_numeric(d, phi * d) - strip_self_numeric(d, d) - thick(phi * d) + thick(d)
+    return float(out) if np.ndim(out) == 0 else out
+
+
+def side_mutual(length: float, wa, wb, gap, steps=0.0, n: int = 8) -> np.ndarray:
+    """Two flat coplanar parallel strips' mutual: n filaments across each, the mean filament mutual
+    (peec.py:3328-3341)."""
+    wa, wb, gap, steps = (np.atleast_1d(v) for v in np.broadcast_arrays(*(np.asarray(v, float) for v in
+                                                                            (wa, wb, gap, steps))))
+    wa, wb, gap = wa * length, wb * length, gap * length
+    u = (np.arange(n) + 0.5) / n - 0.5
+    ya = wa[:, None, None] * u[None, :, None]
+    yb = gap[:, None, None] + wb[:, None, None] * u[None, None, :]
+    rho = np.maximum(np.abs(yb - ya), GMD_FLAT * np.maximum(wa, wb)[:, None, None] / n)
+    axial = np.broadcast_to((steps * length)[:, None, None], rho.shape)
+    return filament_mutual(length, axial, rho).mean(axis=(1, 2))
+
+
 class PlaneStack:
     """Plane layers plus lumped branches: topology, inductance kernels, per-frequency operators and the solve
-    (peec.py:1555-2700 restricted to one sheet per layer, diag preconditioner, full strips, no capacitance)."""
+    (peec.py:1555-2700 restricted to one sheet per layer, diag preconditioner, no capacitance)."""
 
     def __init__(self, layers: list, branches: Optional[list] = None, sigma: float = SIGMA_CU):
         if not layers:
@@ -127,7 +154,73 @@ class PlaneStack:
         self.sheet_t = np.array([l.thickness for l in layers], float)
         self._topology(list(branches or []))
         self._kernels()
+        self._widths()
         self._ops: dict = {}
+
+    # ---------------------------------------------------------------- narrow strips (peec.py:1732-1822)
+    def _widths(self):
+        """strip_width, r_width, dL and M_near from the grids' wx / wy, rx / ry, ox / oy (one sheet per layer: no
+        face twins, peec.py:1823-1843 does nothing)."""
+        def gather(attr, idx, shape, default):
+            if len(idx) == 0:
+                return np.zeros(0)
+            if all(getattr(l.grid, attr, None) is None for l in self.layers):
+                return np.full(len(idx), default)
+            w = np.stack([getattr(l.grid, attr) if getattr(l.grid, attr, None) is not None
+                          else np.full(shape, default) for l in self.layers])
+            return w[tuple(idx.T)]
+        sx, sy = (self.nx - 1, self.ny), (self.nx, self.ny - 1)
+        lx, ly = self.xa, self.ya
+        w = np.concatenate([gather("wx", lx, sx, 1.0), gather("wy", ly, sy, 1.0)])
+        ok = w > 0
+        self.strip_width = np.where(ok, w, 1.0)
+        rw = np.concatenate([gather("rx", lx, sx, 0.0), gather("ry", ly, sy, 0.0)])
+        self.r_width = np.where(ok & (rw > 0), rw, self.strip_width)
+        off = np.where(ok, np.concatenate([gather("ox", lx, sx, 0.0), gather("oy", ly, sy, 0.0)]), 0.0)
+        self.dL = self.M_near = None
+        if not np.any(self.strip_width != 1.0) and not np.any(self.r_width != 1.0):
+            return
+        narrow = self.strip_width < 1.0
+        if not np.any(narrow):
+            return
+        d = self.d
+        t = np.array([l.thickness for l in self.layers])[np.concatenate([lx[:, 0], ly[:, 0]])]
+        dL = np.zeros(self.ns)
+        keys, inv = np.unique(np.round(np.column_stack([self.strip_width[narrow], t[narrow]]), 12), axis=0,
+                              return_inverse=True)
+        dL[narrow] = narrow_self(d, keys[:, 0], keys[:, 1])[np.ravel(inv)]
+        self.dL = dL
+        rows, cols, steps = [], [], []
+        for idx, shape, lo, axis in ((self.xa, sx, 0, 2), (self.ya, sy, self.nxb, 1)):
+            if len(idx) == 0:
+                continue
+            at = -np.ones((self.n_sheets, *shape), int)
+            at[tuple(idx.T)] = lo + np.arange(len(idx))
+            n_ax = shape[0] if axis == 2 else shape[1]
+            for k in range(-NEAR_AXIAL, NEAR_AXIAL + 1):
+                if abs(k) >= n_ax:
+                    continue
+                ax_a = slice(max(-k, 0), n_ax - max(k, 0))
+                ax_b = slice(max(k, 0), n_ax - max(-k, 0))
+                a = at[:, ax_a, :-1] if axis == 2 else at[:, :-1, ax_a]
+                b = at[:, ax_b, 1:] if axis == 2 else at[:, 1:, ax_b]
+                both = (a >= 0) & (b >= 0)
+                a, b = a[both], b[both]
+                part = narrow[a] | narrow[b]
+                rows.append(a[part])
+                cols.append(b[part])
+                steps.append(np.full(int(part.sum()), abs(k)))
+        a, b = np.concatenate(rows), np.concatenate(cols)
+        if len(a) == 0:
+            return
+        k = np.concatenate(steps)
+        wa, wb = self.strip_width[a], self.strip_width[b]
+        gap = 1.0 + off[b] - off[a]
+        keys, inv = np.unique(np.round(np.column_stack([wa, wb, gap, k]), 9), axis=0, return_inverse=True)
+        dM = (side_mutual(d, keys[:, 0], keys[:, 1], keys[:, 2], keys[:, 3])
+              - side_mutual(d, np.ones(len(keys)), np.ones(len(keys)), np.ones(len(keys)), keys[:, 3]))
+        M = sp.coo_matrix((dM[np.ravel(inv)], (a, b)), shape=(self.ns, self.ns))
+        self.M_near = (M + M.T).tocsr()
 
     # ---------------------------------------------------------------- topology (peec.py:1596-1656, 1845-1862)
     def _topology(self, branches: list):
@@ -281,6 +374,10 @@ class PlaneStack:
                 X = np.zeros(shape[0] * shape[1] * shape[2], complex)
                 X[into] = I_s[lo:hi]
                 out[lo:hi] = self._conv(X.reshape(shape), orient).reshape(-1)[outof]
+        if self.dL is not None:             # strips narrower than the pitch (peec.py:2129-2132)
+            out += self.dL * I_s
+            if self.M_near is not None:
+                out += self.M_near @ I_s
         return out
 
     # ---------------------------------------------------------------- operators (peec.py:2144-2178, 2345-2511)
@@ -288,7 +385,7 @@ class PlaneStack:
         """(w, strip sheet impedance per strip, branch impedances) (peec.py:2150-2178, no capacitance)."""
         w = 2 * math.pi * f
         zs = np.array([slab_impedance(l.thickness, f, self.sigma) for l in self.layers])
-        R_strip = (np.concatenate([zs[self.xa[:, 0]], zs[self.ya[:, 0]]]) if self.ns
+        R_strip = (np.concatenate([zs[self.xa[:, 0]], zs[self.ya[:, 0]]]) / self.r_width if self.ns
                    else np.zeros(0, complex))
         z_br = np.array([complex(b.z(f)) for b in self.branches], complex)
         return w, R_strip, z_br
@@ -296,7 +393,8 @@ class PlaneStack:
     def admittance(self, w: float, R_strip: np.ndarray, z_tail: np.ndarray) -> np.ndarray:
         """diag(Z)^-1 of the diag preconditioner: strips with their own self inductance only
         (peec.py:2465-2478)."""
-        return np.concatenate([1.0 / (R_strip + 1j * w * self.L_self), 1.0 / z_tail])
+        L_own = self.L_self if self.dL is None else self.L_self + self.dL
+        return np.concatenate([1.0 / (R_strip + 1j * w * L_own), 1.0 / z_tail])
 
     def operator(self, f: float) -> Operator:
         """The operator at f with its own SuperLU factorisation of S = Ar^T D^-1 Ar (peec.py:2227-2445: a fresh

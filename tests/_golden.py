@@ -11,7 +11,8 @@ import numpy as np
 from groundtruth_amd.problem import Branch, Grid, LidNode, PlaneLayer, TraceSegment
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-NAMES = ["single_plane_64", "two_layer_48", "four_layer_24", "large_32", "mixed_copper_32", "decap_lid_32"]
+NAMES = ["single_plane_64", "two_layer_48", "four_layer_24", "large_32", "mixed_copper_32", "decap_lid_32",
+         "narrow_sub4"]
 KINDS = ["via", "part", "bond"]
 
 
@@ -44,8 +45,14 @@ def load(name: str) -> Fixture:
     with np.load(GOLDEN / f"solve_{name}.npz") as z:
         a = {k: z[k] for k in z.files}
     pitch = float(a["pitch_mm"])
-    layers = [PlaneLayer(f"L{k}", float(a["z"][k]), Grid.from_mask(m, pitch), float(a["t"][k]))
-              for k, m in enumerate(a["nodes"])]
+    layers = []
+    for k, m in enumerate(a["nodes"]):
+        if "wx" in a:            # a rasterised board: its strips and measured widths
+            g = Grid(0.0, 0.0, pitch, m.shape[0], m.shape[1], m.copy(), a["xb"][k].copy(), a["yb"][k].copy(),
+                     **{f: a[f][k].copy() for f in ("wx", "wy", "ox", "oy", "rx", "ry")})
+        else:
+            g = Grid.from_mask(m, pitch)
+        layers.append(PlaneLayer(f"L{k}", float(a["z"][k]), g, float(a["t"][k])))
     nbr = len(a["br_R"])
     lid_a = a.get("br_lid_a", -np.ones(nbr, np.int64))
     lid_b = a.get("br_lid_b", -np.ones(nbr, np.int64))
