@@ -276,3 +276,18 @@ def test_areas_sum_to_the_copper():
         polys = R.polygons(geom if not isinstance(geom, dict) else list(geom.values())[0])
         total = sum(abs(R._signed_area(p[0])) - sum(abs(R._signed_area(h)) for h in p[1:]) for p in polys)
         assert float(np.sum(np.where(g.node, g.area, 0.0))) * d * d == pytest.approx(total, rel=0.005)
+
+
+def test_sp27_two_touching_partial_strips_are_one_strip():
+    """tests/test_validation_audit.py SP-27: two side-by-side strips phi d wide, touching (centres phi d apart),
+    carrying half the current each, have the inductance of one 2 phi d strip: (self + dL + M) / 2 against the flat
+    strip's integral, within 1 % (the oracle's narrow_self and side_mutual, zero thickness)."""
+    from oracle import kernel as K
+    from oracle import peec as O
+    d = 1e-3
+    full = K.strip_kernel(3, 3, d)[2, 2]
+    assert full == pytest.approx(K.strip_self_numeric(d, d), rel=1e-6)
+    for phi in (0.1, 0.15, 0.3, 0.5):
+        self_ = full + O.narrow_self(d, phi, 0.0)
+        m = float(O.side_mutual(d, phi, phi, phi)[0])
+        assert (self_ + m) / 2 == pytest.approx(K.strip_self_numeric(d, 2 * phi * d), rel=0.01), phi
