@@ -199,6 +199,7 @@ def cpu_lp_large(st, I: np.ndarray, y_dev: np.ndarray) -> dict:
     o.nx, o.ny, o.n_sheets = st.nx, st.ny, st.n_sheets
     o.xa, o.ya, o.nxb, o.nyb, o.ns = st.xa, st.ya, st.nxb, st.nyb, st.ns
     o.Px, o.Py = tuple(st.Px), tuple(st.Py)
+    o.dL = o.M_near = None              # every strip of this board a pitch wide
     # the device keeps its quadrants [qj][qi][t][s] pre-scaled by 1 / (P0 P1); the oracle's are [qi][qj][t][s], unscaled
     o._kq = {"x": np.ascontiguousarray(peec.kernel_quadrant(st, 0).transpose(1, 0, 2, 3)) * (o.Px[0] * o.Px[1]),
              "y": np.ascontiguousarray(peec.kernel_quadrant(st, 1).transpose(1, 0, 2, 3)) * (o.Py[0] * o.Py[1])}
@@ -223,7 +224,10 @@ def large(cpu: bool = True) -> dict:
     board = boards.large()
     route = board.routes[0]
     f = float(board.freqs[0])
-    log("six-layer 2048^2 board: set-up")
+    import gc
+    gc.collect()                    # the earlier sections' stacks and vectors, before the footprint check
+    free, total = torch.cuda.mem_get_info(0)
+    log(f"six-layer 2048^2 board: set-up ({free / 1e9:.0f} of {total / 1e9:.0f} GB free)")
     t0 = time.perf_counter()
     st = peec.PlaneStack(board.layers, board.branches, device=0)
     ex = peec.RouteExcitation(st, route.trace, route.driver, route.receiver)

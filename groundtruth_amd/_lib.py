@@ -57,6 +57,7 @@ SIGNATURES = {
     "gt_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "gt_ctx_destroy": (None, [C.c_void_p]),
     "gt_ctx_sync": (C.c_int, [C.c_void_p]),
+    "gt_ctx_trim": (C.c_int, [C.c_void_p]),
     "gt_ctx_stream": (C.c_void_p, [C.c_void_p]),
     "gt_stack_create": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(C.c_void_p)]),
     "gt_stack_create_gen": (C.c_int, [C.c_void_p, C.POINTER(StackDesc), C.POINTER(KernelDesc), C.POINTER(C.c_void_p)]),

@@ -287,6 +287,16 @@ void gt_ctx_destroy(gt_ctx* ctx) {
     delete ctx;
 }
 
+int gt_ctx_trim(gt_ctx* ctx) {
+    return guard([&] {
+        need(ctx, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        GT_HIP(hipSetDevice(ctx->device));
+        GT_HIP(hipStreamSynchronize(ctx->stream));
+        ctx_trim_gmres(ctx);
+    });
+}
+
 int gt_ctx_sync(gt_ctx* ctx) {
     return guard([&] {
         need(ctx != nullptr, "ctx is NULL");

@@ -914,6 +914,11 @@ void gmres_batch_mixed(gt_stack* st, int nr, gt_op* const* ops, const zd* const*
 // every slot runs the same state machine as in a batch, so its result is bit-identical to the same solve alone.
 // The slots may hold solves of the stack itself and of what-if stacks derived from it with up to max_extra added
 // branches: one batch for a board and its what-ifs. The basis is stored in FP64 or FP32.
+namespace gt {
+// The context's kept GMRES workspace freed (the next solve makes it again); the caller holds the context's lock.
+void ctx_trim_gmres(gt_ctx* ctx) { ctx->gmres.reset(); }
+}  // namespace gt
+
 struct gt_session_s {
     gt_stack* st = nullptr;
     int cap = 0, restart = 0, n_max = 0;

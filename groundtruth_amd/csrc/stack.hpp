@@ -198,6 +198,8 @@ void loop_terms_device(gt_stack* st, const zd* x, const double* psi, const doubl
                        const double* inj_amps, double d, const double* sheet_z, double* out);
 void via_coupling_device(gt_ctx* ctx, int nv, const double* vias, int nseg, const gt_segment* segs, double mu,
                          double* out);
+// frees the context's kept GMRES workspace (gmres.cpp)
+void ctx_trim_gmres(gt_ctx* ctx);
 // copper rasterisation (raster.hip)
 void raster_label(gt_ctx* ctx, int32_t ng, const int64_t* eptr, const double* edges, const int32_t* win, int32_t nu,
                   const double* xs, int32_t nv, const double* ys, int32_t* lab);

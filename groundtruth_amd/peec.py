@@ -551,6 +551,12 @@ class PlaneStack:
         free = int(torch.cuda.mem_get_info(self.device)[0])
         need = self.device_bytes_estimate()
         if need > DEVICE_MEMORY_SHARE * free:
+            # memory this process keeps for reuse counts as free: torch's cached blocks and the context's GMRES
+            # workspace (made again by the next solve)
+            torch.cuda.empty_cache()
+            _lib.check(_lib.lib().gt_ctx_trim(_lib.Context.get(self.device).handle), "gt_ctx_trim")
+            free = int(torch.cuda.mem_get_info(self.device)[0])
+        if need > DEVICE_MEMORY_SHARE * free:
             raise DeviceTooSmall(f"the model needs ~{need / 1e9:.3g} GB on device {self.device}, {free / 1e9:.3g} GB "
                                  "free")
 

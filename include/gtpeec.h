@@ -46,6 +46,10 @@ int gt_abi_version(void);
 int gt_ctx_create(int device, gt_ctx** out);
 void gt_ctx_destroy(gt_ctx* ctx);
 int gt_ctx_sync(gt_ctx* ctx);
+/* Frees the device memory the context keeps between solves (its GMRES workspace, grown to the largest batch it has
+ * served; the next solve makes it again). No counterpart in the reference; PlaneStack calls it, with torch's cache
+ * emptied, before it refuses a model whose footprint exceeds 0.8 of the free memory (gpu.py:483-488). */
+int gt_ctx_trim(gt_ctx* ctx);
 /* The context's hipStream_t (for callers that order their own device work with it). */
 void* gt_ctx_stream(gt_ctx* ctx);
 /* Measurement (no counterpart in the reference: per-kernel HBM figures against the device's peak). While enabled,
