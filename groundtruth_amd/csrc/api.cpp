@@ -842,7 +842,7 @@ int gt_schur_factor_nodes(gt_stack* st, const int32_t* node_xyz, int32_t* n_cols
         std::sort(nodes.begin(), nodes.end());
         nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
         const int k = (int)nodes.size();
-        if (k > WB_IMPLICIT_MAX_K) fail(GT_ERR_UNSUPPORTED, "more than 2048 branch end nodes on one stack");
+        if (k > WB_NODES_MAX_K) fail(GT_ERR_UNSUPPORTED, "more than 2048 branch end nodes on one stack");
         need(k == 0 || cols != nullptr, "NULL cols");
         st->wb_implicit = true;
         st->wb_nodes = true;
@@ -873,7 +873,7 @@ int gt_schur_factor(gt_stack* st, const int32_t* node_xyz, double* G_out) {
                           (form == GT_WOODBURY_AUTO && (k > WB_STORED_MAX_K ||
                                                         (int64_t)k * nk * (int64_t)sizeof(double) > st->ctx->wb_stored_max));
         need(k <= (st->wb_implicit ? WB_IMPLICIT_MAX_K : WB_STORED_MAX_K),
-             st->wb_implicit ? "more than 2048 branches on one stack" : "more than 256 branches with W resident");
+             st->wb_implicit ? "more than 8192 branches on one stack" : "more than 256 branches with W resident");
         std::vector<int> ka(k), kb(k);
         for (int m = 0; m < k; ++m) {
             ka[m] = st->h_a_keep[st->ns + m];

@@ -104,11 +104,12 @@ __global__ __launch_bounds__(256) void pre_woodbury_small(PreArgs<T> a) {
     }
 }
 
-// The implicit form's t_p = Q_p (U_p^T z_p) for up to 2048 branches: a workgroup takes 32 rows of one right-hand
-// side, eight lanes a row (m strided by eight, then the eight partial sums added by a fixed butterfly)
-template <class T>
+// The implicit form's t_p = Q_p (U_p^T z_p) for up to KM branches (U^T z staged in LDS: 32 KB for 2048, 128 KB for
+// the 8192 of the largest stacks): a workgroup takes 32 rows of one right-hand side, eight lanes a row (m strided by
+// eight, then the eight partial sums added by a fixed butterfly)
+template <class T, int KM>
 __global__ __launch_bounds__(256) void pre_woodbury_q(PreArgs<T> a) {
-    __shared__ double ut_raw[2 * WB_IMPLICIT_MAX_K];
+    __shared__ double ut_raw[2 * KM];
     zd* ut = reinterpret_cast<zd*>(ut_raw);
     const int p = blockIdx.x;
     const int k = a.k[p], i0 = blockIdx.y * 32;
@@ -665,7 +666,7 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
     const int nk = st->nk, k0 = base ? base->k : 0, k = k0 + n_extra;
     const bool imp = st->wb_implicit;
     if (imp) {
-        if (k > WB_IMPLICIT_MAX_K) fail(GT_ERR_ARG, "more than 2048 branches on one stack");
+        if (k > WB_IMPLICIT_MAX_K) fail(GT_ERR_ARG, "more than 8192 branches on one stack");
         if (base && !base->wb_implicit) fail(GT_ERR_ARG, "a derived stack takes its base's Woodbury form");
         st->G.alloc(std::max<size_t>(1, (size_t)k * k));
         GT_HIP(hipMemsetAsync(st->G.p, 0, st->G.bytes(), s));
@@ -819,7 +820,12 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
                 hipLaunchKernelGGL(pre_root_u<T>, dim3(st->k), dim3(256), 0, s, a);
                 GT_LAUNCH("pre_root_u");
                 a.qmode = 1;
-                hipLaunchKernelGGL(pre_woodbury_q<T>, dim3(nr, (a.kt + 31) / 32), dim3(256), 0, s, a);
+                if (a.kt <= WB_IMPLICIT_SMALL_K)
+                    hipLaunchKernelGGL((pre_woodbury_q<T, WB_IMPLICIT_SMALL_K>), dim3(nr, (a.kt + 31) / 32), dim3(256),
+                                       0, s, a);
+                else
+                    hipLaunchKernelGGL((pre_woodbury_q<T, WB_IMPLICIT_MAX_K>), dim3(nr, (a.kt + 31) / 32), dim3(256),
+                                       0, s, a);
                 GT_LAUNCH("pre_woodbury_q");
                 hipLaunchKernelGGL(pre_root_sub<T>, dim3(blocks_for(st->yr_n, 256)), dim3(256), 0, s, a);
                 GT_LAUNCH("pre_root_sub");
@@ -830,7 +836,12 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
             if (any_second) {
                 a.z2 = st->work.z2.p;
                 a.qmode = 2;
-                hipLaunchKernelGGL(pre_woodbury_q<T>, dim3(nr, (a.kt + 31) / 32), dim3(256), 0, s, a);
+                if (a.kt <= WB_IMPLICIT_SMALL_K)
+                    hipLaunchKernelGGL((pre_woodbury_q<T, WB_IMPLICIT_SMALL_K>), dim3(nr, (a.kt + 31) / 32), dim3(256),
+                                       0, s, a);
+                else
+                    hipLaunchKernelGGL((pre_woodbury_q<T, WB_IMPLICIT_MAX_K>), dim3(nr, (a.kt + 31) / 32), dim3(256),
+                                       0, s, a);
                 GT_LAUNCH("pre_woodbury_q");
                 hipLaunchKernelGGL(pre_woodbury_scatter<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
                 GT_LAUNCH("pre_woodbury_scatter");

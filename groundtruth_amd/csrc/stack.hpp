@@ -232,5 +232,7 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
 // G = U^T W (k x k row-major) of the stack's branches, from its device W (or its device G), into host memory
 void woodbury_gram_device(const gt_stack* st, double* G_host);
 constexpr int WB_STORED_MAX_K = 256;     // branches of one stack with W resident (one workgroup's LDS for U^T z)
-constexpr int WB_IMPLICIT_MAX_K = 2048;  // branches of one stack in the implicit form
+constexpr int WB_IMPLICIT_MAX_K = 8192;  // branches of one stack in the implicit form
+constexpr int WB_IMPLICIT_SMALL_K = 2048;   // ... up to which its product stages U^T z in 32 KB of LDS (else 128 KB)
+constexpr int WB_NODES_MAX_K = 2048;     // branch end nodes of a stack of sheets of different admittance (root form)
 }  // namespace gt

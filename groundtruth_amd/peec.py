@@ -82,6 +82,14 @@ def kernel_quadrant(stack: "PlaneStack", orient: int) -> np.ndarray:
     return out
 
 
+def _woodbury_q(c: np.ndarray, G: np.ndarray) -> np.ndarray:
+    """diag(c) (I + G diag(c))^-1, the Woodbury capacitance of one frequency (k x k, LAPACK on the host: a second for
+    a few thousand branches)."""
+    k = len(c)
+    A = np.eye(k, dtype=complex) + G * np.asarray(c)[None, :]
+    return np.asarray(c)[:, None] * np.linalg.inv(A)
+
+
 class Operator:
     """One frequency's device operator (gt_op) and the host values it was made from (peec.py:1505-1516)."""
 
@@ -106,7 +114,7 @@ class Operator:
             # stack added (with_branches) are not in B (C = d/d_s)
             in_B = np.arange(k) < stack.nbr_factored
             c = self.dinv_tail / ds - in_B
-            Q = np.diag(c) @ np.linalg.inv(np.eye(k) + stack.G @ np.diag(c))
+            Q = _woodbury_q(c, stack.G)
         else:
             Q = np.zeros((0, 0), complex)
         self.Q = Q

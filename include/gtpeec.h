@@ -77,7 +77,7 @@ int gt_ctx_schur_ordering(gt_ctx* ctx, int32_t ordering);
 /* The Woodbury form of the stacks factorised on this context from now on: how the preconditioner's branch
  * correction is applied (the reference factorises a board's branches into S and keeps a what-if's S^-1 U dense on
  * the host, peec.py:2364-2377). GT_WOODBURY_STORED keeps W = B^-1 U resident (n_keep x k doubles, at most 256 branches); GT_WOODBURY_IMPLICIT keeps
- * only G = U^T B^-1 U (k x k, at most 2048 branches) and applies B^-1 U t by a second Schur solve in each
+ * only G = U^T B^-1 U (k x k, at most 8192 branches) and applies B^-1 U t by a second Schur solve in each
  * preconditioner product. GT_WOODBURY_AUTO (the default) stores W while k <= 256 and W takes at most
  * stored_max_bytes (default 16 GiB). A derived stack takes its base's form. (No counterpart in the reference.) */
 #define GT_WOODBURY_AUTO 0

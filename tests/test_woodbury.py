@@ -141,3 +141,18 @@ def test_the_six_layer_board_family_matches_the_oracle_in_both_implicit_forms():
         assert st.woodbury_form == form, ordering
         assert rel(_cur(sol), _cur(ref)) < 1e-9, ordering
         assert abs(sol.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9, ordering
+
+
+@pytest.mark.gpu
+def test_a_board_with_thousands_of_branches_matches_the_oracle():
+    """More branches than one workgroup's 32 KB of LDS holds U^T z for (3000 vias on two 64^2 planes): the
+    Woodbury product stages it in 128 KB, the per-frequency capacitance is a device solve; the solution equals the
+    oracle's (SuperLU of the whole S, any number of branches)."""
+    b = boards.scaled("two_layer", 64, n_vias=3000, n_freqs=1)
+    assert len(b.branches) == 3000
+    r = b.routes[0]
+    st = peec.PlaneStack(b.layers, b.branches)
+    sol = peec.RouteExcitation(st, r.trace, r.driver, r.receiver).solve(3e8, tol=1e-12)
+    ref = O.RouteExcitation(O.PlaneStack(b.layers, b.branches), r.trace, r.driver, r.receiver).solve(3e8, tol=1e-12)
+    assert rel(_cur(sol), _cur(ref)) < 1e-9
+    assert abs(sol.Z_loop - ref.Z_loop) / abs(ref.Z_loop) < 1e-9
