@@ -668,7 +668,7 @@ int64_t Schur::device_bytes() const {
 namespace {
 
 // the passes for nr right-hand sides: forward (pass 1), backward (pass 2) or both (3), with the kernels' batch width
-// the next power of two
+// 1, 2, 4, 8, 12 or 16 (the least that holds nr: the padded slots are multiply-adds of zeros)
 void launch_passes(Schur& S, const zd* b, zd* x, int nr, hipStream_t st, int passes) {
     SolveArgs a{nullptr, S.perm.p, S.rows.p, reinterpret_cast<const int2*>(S.ctab.p), S.cidx.p, S.panel.p, S.ubuf.p,
                 S.zp.p, S.xp.p, b, x, S.part.p};
@@ -681,6 +681,7 @@ void launch_passes(Schur& S, const zd* b, zd* x, int nr, hipStream_t st, int pas
     else if (nr == 2) go(std::integral_constant<int, 2>());
     else if (nr <= 4) go(std::integral_constant<int, 4>());
     else if (nr <= 8) go(std::integral_constant<int, 8>());
+    else if (nr <= 12) go(std::integral_constant<int, 12>());
     else go(std::integral_constant<int, 16>());
 }
 
