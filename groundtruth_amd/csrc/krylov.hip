@@ -406,6 +406,11 @@ __global__ __launch_bounds__(NTH) void kr_dots2_long(KrSetT<TV, TW> k, int n, in
     }
 }
 
+// basis rows per workgroup of the grouped multi-dot: 8 over an FP64 basis, 4 over an FP32 one (whose lanes hold two
+// elements each: half the accumulators keep its occupancy)
+template <class TV>
+constexpr int dots2_group() { return sizeof(TV) == 4 ? 4 : 8; }
+
 // Pass A (short vectors, and FP32 bases): t = Q^H u and p = Q^H v over the rows [0, m) in groups of DG2 rows per
 // workgroup, and u^H u, u^H v (group 0).
 // v may be absent (a basis's last row: t and u^H u only).
@@ -413,7 +418,7 @@ template <class TV, class TW>
 __global__ __launch_bounds__(NTH) void kr_dots2(KrSetT<TV, TW> k, int n, int chunk, double* part, int P, int nblk,
                                                 int slot_t, int slot_uu) {
     constexpr int E = lane_elems<TV>();
-    constexpr int DG2 = 8;
+    constexpr int DG2 = dots2_group<TV>();
     __shared__ double red[NTH / 64][4 * DG2 + 3];
     const int p = blockIdx.y;
     const int grp = blockIdx.x / nblk, blk = blockIdx.x - grp * nblk;
@@ -497,7 +502,7 @@ template <class TV, class TW>
 __global__ __launch_bounds__(NTH) void kr_update2(KrSetT<TV, TW> k, int n, int chunk, const double* red, double* part,
                                                   int P, int nblk, int slot_t, int slot_y, DcgsCoef cf) {
     constexpr int E = lane_elems<TV>();
-    constexpr int UR = E == 1 ? 4 : 8;
+    constexpr int UR = 4;
     __shared__ double lds[NTH / 64];
     __shared__ double tp_raw[2048];
     zd* ts = reinterpret_cast<zd*>(tp_raw);
@@ -752,6 +757,7 @@ void Krylov::dots2(const KrSetT<TV, TW>& k, hipStream_t s) {
     for (int p = 0; p < k.nr; ++p) mmax = std::max(mmax, k.m[p]);
     if (mmax + 1 > rows_cap) fail(GT_ERR_ARG, "a basis longer than the workspace's");
     const int groups = std::max(1, (mmax + 7) / 8);
+    const int gsz = dots2_group<TV>(), groups_g = std::max(1, (mmax + gsz - 1) / gsz);
     static_assert(chunk2<TV>() == 4 * lane_elems<TV>() * NTH, "kr_dots2 takes four element steps per lane");
     const int c2 = chunk2<TV>(), nb2 = blocks2<TV>();
     // enough chunks to fill the device: every group in one workgroup per chunk (u and v read once)
@@ -761,7 +767,7 @@ void Krylov::dots2(const KrSetT<TV, TW>& k, hipStream_t s) {
         hipLaunchKernelGGL((kr_dots2_long<TV, TW>), dim3(nb2, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
                            slot_h2(), slot_uu(), groups);
     else
-        hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nb2 * groups, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
+        hipLaunchKernelGGL((kr_dots2<TV, TW>), dim3(nb2 * groups_g, k.nr), dim3(NTH), 0, s, k, n, c2, part.p, P, nb2,
                            slot_h2(), slot_uu());
     GT_LAUNCH("kr_dots2");
     if (mmax > 0) {
@@ -781,11 +787,11 @@ void Krylov::update2(const KrSetT<TV, TW>& k, const double* ainv, const zd* qv, 
         cf.qv[p] = qv[p];
         if (k.m[p] + 1 > rows_cap) fail(GT_ERR_ARG, "a basis longer than the workspace's");
     }
-    const int c2 = chunk2<TV>(), nb2 = blocks2<TV>();
-    hipLaunchKernelGGL((kr_update2<TV, TW>), dim3(nb2, k.nr), dim3(NTH), 0, s, k, n, c2, (const double*)red.p, part.p,
-                       P, nb2, slot_h2(), slot_w_after(), cf);
+    const int nbu = blocks_u();
+    hipLaunchKernelGGL((kr_update2<TV, TW>), dim3(nbu, k.nr), dim3(NTH), 0, s, k, n, (int)chunk, (const double*)red.p,
+                       part.p, P, nbu, slot_h2(), slot_w_after(), cf);
     GT_LAUNCH("kr_update2");
-    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nb2, P, slot_w_after(), red.p);
+    hipLaunchKernelGGL(kr_reduce, dim3(1, k.nr), dim3(NTH), 0, s, part.p, nbu, P, slot_w_after(), red.p);
     GT_LAUNCH("kr_reduce");
 }
 

@@ -39,12 +39,14 @@ struct Krylov {
     void init(int n, int rows);
     // the partial-sum blocks the buffers hold for vectors of up to n
     static int part_blocks(int n) { return std::max(1, (n + chunk - 1) / chunk); }
-    // The delayed-CGS2 passes (dots2, update2): an FP32 basis takes twice the elements per block, so a block moves
-    // the bytes of an FP64 one (and its reductions weigh half as much); fixed per type, whatever the batch
+    // The delayed-CGS2 passes: the multi-dot (dots2) takes twice the elements per block over an FP32 basis, so a
+    // block moves the bytes of an FP64 one; the update (update2) takes `chunk` elements for either type, twice the
+    // workgroups of an FP32 multi-dot to fill the device. Fixed per type, whatever the batch.
     template <class TV>
     static constexpr int chunk2() { return sizeof(TV) == sizeof(float) ? 2 * chunk : chunk; }
     template <class TV>
     int blocks2() const { return std::max(1, (n + chunk2<TV>() - 1) / chunk2<TV>()); }
+    int blocks_u() const { return std::max(1, (n + chunk - 1) / chunk); }
     int slot_w() const { return 2 * rows_cap; }          // |w|^2 before the projection
     int slot_w_after() const { return 2 * rows_cap + 1; } // |w|^2 after it
     int slot_norm() const { return 2 * rows_cap + 2; }    // norm2's value
