@@ -37,8 +37,14 @@ __device__ __forceinline__ int ilog2_dev(int v) { return 31 - __clz(v); }
 // grid of G = a multiple of 8 workgroups (those past the last tile exit).
 __device__ __forceinline__ int xcd_tile(int bx, int G) { return (bx & 7) * (G >> 3) + (bx >> 3); }
 
+// Loads a thread of the generic row passes keeps in flight before their LDS stores: a 4096-point row (one per
+// workgroup) reads its column-strided T2 entries with every load issued at once, not one round trip per entry.
+template <int NTH, int MAXPTS>
+constexpr int row_depth() { return MAXPTS / NTH >= 8 ? 8 : MAXPTS / NTH >= 1 ? MAXPTS / NTH : 1; }
+constexpr int STRIP_DEPTH = 2;      // strips per thread in flight in the scatter and gather
+
 template <class T, int NTH, int MAXPTS>
-__global__ __launch_bounds__(NTH) void lp_rows_fwd(LpArgs<T> a) {
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH >= 1024 ? 8 : 1))) void lp_rows_fwd(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     int blk = xcd_tile(blockIdx.x, gridDim.x), oi = 0;      // consecutive tiles on one XCD (grid: a multiple of 8)
@@ -54,9 +60,20 @@ __global__ __launch_bounds__(NTH) void lp_rows_fwd(LpArgs<T> a) {
     __syncthreads();
     const int kb = o.rowptr[s * o.n0 + i0], ke = o.rowptr[s * o.n0 + i0 + nr];
     const cpx<T>* xin = a.rhs.x[p] + o.lo;
-    for (int k = kb + threadIdx.x; k < ke; k += NTH) {
-        const int ij = o.rowcol[k];
-        buf[((ij >> 16) - i0) * P1s + (ij & 0xffff)] = xin[k];
+    for (int k0 = kb + threadIdx.x; k0 < ke; k0 += NTH * STRIP_DEPTH) {
+        int ij[STRIP_DEPTH];
+        cpx<T> v[STRIP_DEPTH];
+#pragma unroll
+        for (int u = 0; u < STRIP_DEPTH; ++u) {
+            const int k = k0 + u * NTH;
+            if (k < ke) {
+                ij[u] = o.rowcol[k];
+                v[u] = xin[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < STRIP_DEPTH; ++u)
+            if (k0 + u * NTH < ke) buf[((ij[u] >> 16) - i0) * P1s + (ij[u] & 0xffff)] = v[u];
     }
     __syncthreads();
     fft_lds<T, false, NTH, MAXPTS>(buf, o.P1, o.log1, nr, P1s, o.tw1);
@@ -550,7 +567,7 @@ bool launch_cols_w(const LpArgs<T>& a, int ncol, int nr, hipStream_t st, int cw)
 }
 
 template <class T, int NTH, int MAXPTS>
-__global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH >= 1024 ? 8 : 1))) void lp_rows_inv(LpArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     int blk = xcd_tile(blockIdx.x, gridDim.x), oi = 0;      // consecutive tiles on one XCD (grid: a multiple of 8)
@@ -563,32 +580,79 @@ __global__ __launch_bounds__(NTH) void lp_rows_inv(LpArgs<T> a) {
     const int P1s = o.P1 + 1;
     const int p = blockIdx.y;
     const cpx<T>* src = o.T2 + p * a.t_stride + (size_t)t * o.P1 * o.n0 + i0;
-    for (int e = threadIdx.x; e < nr * o.P1; e += NTH) {
-        const int r = e % nr, j = e / nr;
-        buf[r * P1s + j] = src[(size_t)j * o.n0 + r];
+    {
+        constexpr int U = row_depth<NTH, MAXPTS>();
+        const int tot = nr * o.P1;
+        for (int e0 = threadIdx.x; e0 < tot; e0 += NTH * U) {
+            cpx<T> v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + u * NTH;
+                if (e < tot) {
+                    const int r = e % nr, j = e / nr;
+                    v[u] = src[(size_t)j * o.n0 + r];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + u * NTH;
+                if (e < tot) {
+                    const int r = e % nr, j = e / nr;
+                    buf[r * P1s + j] = v[u];
+                }
+            }
+        }
     }
     __syncthreads();
     fft_lds<T, true, NTH, MAXPTS>(buf, o.P1, o.log1, nr, P1s, o.tw1);
     const int kb = o.rowptr[t * o.n0 + i0], ke = o.rowptr[t * o.n0 + i0 + nr];
-    for (int k = kb + threadIdx.x; k < ke; k += NTH) {
-        const int ij = o.rowcol[k];
-        const cpx<T> val = buf[((ij >> 16) - i0) * P1s + (ij & 0xffff)];
-        const int b = o.lo + k;
-        cpx<T>* y = a.rhs.y[p];
-        if (a.mode == 0) {
-            y[b] = val;
-        } else {
-            const cpx<T>* x = a.rhs.x[p];
-            const cpx<T> xb = x[b];
-            const cpx<T> R = a.rhs.r_sheet[p][t];
-            const T w = a.rhs.w[p];
-            cpx<T> v = R * xb;
-            v.re -= w * val.im;
-            v.im += w * val.re;
-            const int ka = a.a_keep[b], kk = a.b_keep[b];
-            if (ka >= 0) v = v - x[a.rhs.nb[p] + ka];
-            if (kk >= 0) v = v + x[a.rhs.nb[p] + kk];
-            y[b] = v;
+    cpx<T>* y = a.rhs.y[p];
+    const cpx<T>* x = a.rhs.x[p];
+    constexpr int V = STRIP_DEPTH;
+    for (int k0 = kb + threadIdx.x; k0 < ke; k0 += NTH * V) {
+        // each strip's loads (its column, x, its end nodes, then their potentials) issued for all V strips at once
+        int ij[V], ka[V], kk[V];
+        cpx<T> xb[V], xa[V], xk[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int k = k0 + u * NTH;
+            if (k < ke) {
+                ij[u] = o.rowcol[k];
+                if (a.mode != 0) {
+                    const int b = o.lo + k;
+                    xb[u] = x[b];
+                    ka[u] = a.a_keep[b];
+                    kk[u] = a.b_keep[b];
+                }
+            }
+        }
+        if (a.mode != 0) {
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                if (k0 + u * NTH < ke) {
+                    if (ka[u] >= 0) xa[u] = x[a.rhs.nb[p] + ka[u]];
+                    if (kk[u] >= 0) xk[u] = x[a.rhs.nb[p] + kk[u]];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int k = k0 + u * NTH;
+            if (k >= ke) continue;
+            const cpx<T> val = buf[((ij[u] >> 16) - i0) * P1s + (ij[u] & 0xffff)];
+            const int b = o.lo + k;
+            if (a.mode == 0) {
+                y[b] = val;
+            } else {
+                const cpx<T> R = a.rhs.r_sheet[p][t];
+                const T w = a.rhs.w[p];
+                cpx<T> v = R * xb[u];
+                v.re -= w * val.im;
+                v.im += w * val.re;
+                if (ka[u] >= 0) v = v - xa[u];
+                if (kk[u] >= 0) v = v + xk[u];
+                y[b] = v;
+            }
         }
     }
 }
