@@ -7,6 +7,9 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <new>
+#include <utility>
 
 #include <vector>
 
@@ -44,6 +47,23 @@ struct TopItem {
     int32_t i0, cnt, k0, k1, slot0, kidx, ks, pad;
 };
 
+// An allocator whose default construction leaves its values unset, for buffers that their writers fill entirely
+// (the solve panels: gigabytes that a pass of zeros would only page in twice).
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept { ::new ((void*)p) U; }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+
 struct SchurHost {
     int n = 0;                          // kept nodes
     std::vector<int> perm;              // new -> old
@@ -53,8 +73,10 @@ struct SchurHost {
     std::vector<int> rows;              // off-diagonal row indices (new numbering), ascending per supernode
     std::vector<int64_t> poff;          // offset of s's panel in `panel`: (nc + nr) x nc, column-major
     // the solve panel Q = [L_CC^-1 ; -L_RC L_CC^-1] (column-major, m x nc): the forward solve of a supernode is
-    // [x_C ; u_R] = Q f_C + [0 ; f_R] and the backward x_C = Q^T [z_C ; x_R]: dense products, no serial chain
-    std::vector<double> panel;
+    // [x_C ; u_R] = Q f_C + [0 ; f_R] and the backward x_C = Q^T [z_C ; x_R]: dense products, no serial chain.
+    // Every entry is written: a host front's by its factorisation, a device front's by the device (its host copy
+    // holds them only after fetch_panels).
+    std::vector<double, UninitAlloc<double>> panel;
     std::vector<int> parent;            // assembly-tree parent (-1 at a root)
     std::vector<int> child_ptr, child;  // children per supernode (CSR, ascending)
     std::vector<int64_t> uoff;          // offset of s's update vector (nr entries) in the update buffer

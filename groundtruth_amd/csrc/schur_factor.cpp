@@ -5,17 +5,64 @@
 // symmetric mode). Here, with every strip alike, S = d_s (B + U C U^T) and only the real B is factorised, once per
 // stack; the frequency enters through the scalar d_s and the Woodbury matrix (the caller's part).
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "schur.hpp"
 
 namespace gt {
 
 namespace {
+
+// Worker threads of the host factorisation: the hardware's, at most 16 (one GPU's share of a host), fewer when
+// OMP_NUM_THREADS asks for fewer.
+int host_threads() {
+    int t = (int)std::thread::hardware_concurrency();
+    t = std::min(std::max(t, 1), 16);
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+        const int v = std::atoi(e);
+        if (v > 0) t = std::min(t, v);
+    }
+    return t;
+}
+
+// f(i, thread) for every i in [0, n) on up to host_threads() threads, the items taken in order from a shared
+// counter; the first exception stops the others and is rethrown once every thread has ended.
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+    const int nt = (int)std::min<int64_t>(host_threads(), n);
+    if (nt <= 1) {
+        for (int64_t i = 0; i < n; ++i) f(i, 0);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    std::atomic<bool> stop(false);
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&](int tid) {
+        try {
+            for (int64_t i = next.fetch_add(1); i < n && !stop.load(); i = next.fetch_add(1)) f(i, tid);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!err) err = std::current_exception();
+            stop = true;
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (std::thread& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
 
 
 struct NDTree {
@@ -231,7 +278,8 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     // ---- numeric: multifrontal partial Cholesky per supernode
     poff.assign(ns + 1, 0);
     for (int s = 0; s < ns; ++s) poff[s + 1] = poff[s] + (int64_t)(nc[s] + nr[s]) * nc[s];
-    panel.assign(poff[ns], 0.0);
+    std::vector<double, UninitAlloc<double>>().swap(panel);
+    panel.resize(poff[ns]);
     nnz_L = 0;
     max_cols = 0;
     for (int s = 0; s < ns; ++s) {
@@ -240,6 +288,7 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
     }
     std::vector<std::vector<double>> upd(ns);      // update matrices waiting for their parent (nr x nr, lower)
     std::vector<int> pos(n, -1);
+    // the device fronts' original entries, in postorder
     for (int s = 0; s < ns; ++s) {
         const int C = nc[s], m = nc[s] + nr[s];
         if (on_dev[s]) {
@@ -271,61 +320,89 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
             trip_off.push_back((int64_t)trip_v.size());
             for (int q = 0; q < C; ++q) pos[c0[s] + q] = -1;
             for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = -1;
-            continue;
         }
-        std::vector<double> F((size_t)m * m, 0.0);
-        for (int q = 0; q < C; ++q) pos[c0[s] + q] = q;
-        for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = C + a;
-        for (int q = 0; q < C; ++q) {
-            const int c = c0[s] + q, old = perm[c];
-            F[(size_t)q * m + q] += diag[old];
-            for (int e = aptr[old]; e < aptr[old + 1]; ++e) {
-                const int c2 = iperm[adj[e]];
-                if (c2 > c) F[(size_t)q * m + pos[c2]] -= 1.0;
-            }
-        }
-        for (int e = child_ptr[s]; e < child_ptr[s + 1]; ++e) {
-            const int k = child[e];
-            const int r = nr[k];
-            const std::vector<double>& U = upd[k];
-            for (int b = 0; b < r; ++b) {
-                const int pb = pos[rows[roff[k] + b]];
-                for (int a = b; a < r; ++a) F[(size_t)pb * m + pos[rows[roff[k] + a]]] += U[(size_t)b * r + a];
-            }
-            std::vector<double>().swap(upd[k]);
-        }
-        // right-looking partial Cholesky of the first C columns; the trailing block becomes the update matrix
-        for (int j = 0; j < C; ++j) {
-            double* cj = &F[(size_t)j * m];
-            const double d = cj[j];
-            if (!(d > 0.0) || !std::isfinite(d))
-                fail(GT_ERR_NUMERIC, "the Schur complement is not positive definite (a piece without a reference node?)");
-            const double sd = std::sqrt(d);
-            cj[j] = sd;
-            const double inv = 1.0 / sd;
-            for (int i = j + 1; i < m; ++i) cj[i] *= inv;
-            for (int k = j + 1; k < m; ++k) {
-                const double f = cj[k];
-                if (f == 0.0) continue;
-                double* ck = &F[(size_t)k * m];
-                for (int i = k; i < m; ++i) ck[i] -= cj[i] * f;
-            }
-        }
-        std::memcpy(&panel[poff[s]], F.data(), sizeof(double) * (size_t)m * C);
-        if (nr[s] > 0) {
-            const int r = nr[s];
-            std::vector<double> U((size_t)r * r);
-            for (int b = 0; b < r; ++b)
-                for (int a = 0; a < r; ++a) U[(size_t)b * r + a] = F[(size_t)(C + b) * m + C + a];
-            if (parent[s] >= 0 && on_dev[parent[s]]) upd_host[s] = std::move(U);
-            else upd[s] = std::move(U);
-        }
-        for (int q = 0; q < C; ++q) pos[c0[s] + q] = -1;
-        for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = -1;
     }
-    // ---- solve panels: Q = [L_CC^-1 ; -L_RC L_CC^-1] in place of each L panel
-    for (int sn = 0; sn < ns; ++sn) {
-        if (on_dev[sn]) continue;
+    // The host fronts form whole subtrees (a device front's ancestors are device fronts), each a contiguous range
+    // [s - size + 1, s] of the postorder: the maximal ones are independent tasks, run on worker threads, largest
+    // first, each in postorder with a position map of its own. Every front's arithmetic is the same in any task
+    // order, so the factor's bits do not depend on the threads.
+    std::vector<int> sub(ns, 1);
+    for (int s = 0; s < ns; ++s)
+        for (int e = child_ptr[s]; e < child_ptr[s + 1]; ++e) sub[s] += sub[child[e]];
+    std::vector<int> tasks;
+    for (int s = 0; s < ns; ++s)
+        if (!on_dev[s] && (parent[s] < 0 || on_dev[parent[s]])) tasks.push_back(s);
+    for (int s = 0; s < ns; ++s)
+        for (int e = child_ptr[s]; e < child_ptr[s + 1]; ++e)
+            if (child[e] >= s || child[e] < s - sub[s] + 1) fail(GT_ERR_ARG, "the assembly tree is not in postorder");
+    std::stable_sort(tasks.begin(), tasks.end(), [&](int a, int b) { return sub[a] > sub[b]; });
+    const int nthr = (int)std::min<int64_t>(host_threads(), std::max<size_t>(1, tasks.size()));
+    std::vector<std::vector<int>> tpos(nthr);
+    tpos[0].swap(pos);
+    parallel_for((int64_t)tasks.size(), [&](int64_t ti, int tid) {
+        std::vector<int>& pos = tpos[tid];
+        if (pos.empty()) pos.assign(n, -1);
+        const int root = tasks[ti];
+        for (int s = root - sub[root] + 1; s <= root; ++s) {
+            const int C = nc[s], m = nc[s] + nr[s];
+            std::vector<double> F((size_t)m * m, 0.0);
+            for (int q = 0; q < C; ++q) pos[c0[s] + q] = q;
+            for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = C + a;
+            for (int q = 0; q < C; ++q) {
+                const int c = c0[s] + q, old = perm[c];
+                F[(size_t)q * m + q] += diag[old];
+                for (int e = aptr[old]; e < aptr[old + 1]; ++e) {
+                    const int c2 = iperm[adj[e]];
+                    if (c2 > c) F[(size_t)q * m + pos[c2]] -= 1.0;
+                }
+            }
+            for (int e = child_ptr[s]; e < child_ptr[s + 1]; ++e) {
+                const int k = child[e];
+                const int r = nr[k];
+                const std::vector<double>& U = upd[k];
+                for (int b = 0; b < r; ++b) {
+                    const int pb = pos[rows[roff[k] + b]];
+                    for (int a = b; a < r; ++a) F[(size_t)pb * m + pos[rows[roff[k] + a]]] += U[(size_t)b * r + a];
+                }
+                std::vector<double>().swap(upd[k]);
+            }
+            // right-looking partial Cholesky of the first C columns; the trailing block becomes the update matrix
+            for (int j = 0; j < C; ++j) {
+                double* cj = &F[(size_t)j * m];
+                const double d = cj[j];
+                if (!(d > 0.0) || !std::isfinite(d))
+                    fail(GT_ERR_NUMERIC, "the Schur complement is not positive definite (a piece without a reference node?)");
+                const double sd = std::sqrt(d);
+                cj[j] = sd;
+                const double inv = 1.0 / sd;
+                for (int i = j + 1; i < m; ++i) cj[i] *= inv;
+                for (int k = j + 1; k < m; ++k) {
+                    const double f = cj[k];
+                    if (f == 0.0) continue;
+                    double* ck = &F[(size_t)k * m];
+                    for (int i = k; i < m; ++i) ck[i] -= cj[i] * f;
+                }
+            }
+            std::memcpy(&panel[poff[s]], F.data(), sizeof(double) * (size_t)m * C);
+            if (nr[s] > 0) {
+                const int r = nr[s];
+                std::vector<double> U((size_t)r * r);
+                for (int b = 0; b < r; ++b)
+                    for (int a = 0; a < r; ++a) U[(size_t)b * r + a] = F[(size_t)(C + b) * m + C + a];
+                if (parent[s] >= 0 && on_dev[parent[s]]) upd_host[s] = std::move(U);
+                else upd[s] = std::move(U);
+            }
+            for (int q = 0; q < C; ++q) pos[c0[s] + q] = -1;
+            for (int a = 0; a < nr[s]; ++a) pos[rows[roff[s] + a]] = -1;
+        }
+    });
+    std::vector<std::vector<int>>().swap(tpos);
+    // ---- solve panels: Q = [L_CC^-1 ; -L_RC L_CC^-1] in place of each L panel (each supernode on its own)
+    std::vector<int> hosts;
+    for (int sn = 0; sn < ns; ++sn)
+        if (!on_dev[sn]) hosts.push_back(sn);
+    parallel_for((int64_t)hosts.size(), [&](int64_t hi, int) {
+        const int sn = hosts[hi];
         const int C = nc[sn], m = nc[sn] + nr[sn];
         double* P = &panel[poff[sn]];
         std::vector<double> inv((size_t)C * C, 0.0);      // column-major, lower
@@ -352,7 +429,7 @@ void SchurHost::build(int n_, const std::vector<int>& ea, const std::vector<int>
             for (int i = 0; i < C; ++i) P[(size_t)j * m + i] = i >= j ? inv[(size_t)j * C + i] : 0.0;
             for (int a = 0; a < R; ++a) P[(size_t)j * m + C + a] = M[(size_t)j * R + a];
         }
-    }
+    });
     // ---- update vectors and the contributors of each front position (children ascending)
     uoff.assign(ns + 1, 0);
     for (int sn = 0; sn < ns; ++sn) uoff[sn + 1] = uoff[sn] + nr[sn];

@@ -601,7 +601,9 @@ void Schur::upload(hipStream_t st) {
     cidx.upload(h.cidx.empty() ? std::vector<int64_t>{0} : h.cidx, st);
     fwd.upload(h.fwd.empty() ? std::vector<SnItem>(1) : h.fwd, st);
     bwd.upload(h.bwd.empty() ? std::vector<SnItem>(1) : h.bwd, st);
-    panel.upload(h.panel.empty() ? std::vector<double>{0.0} : h.panel, st);
+    panel.alloc(std::max<size_t>(1, h.panel.size()));
+    if (h.panel.empty()) panel.upload(std::vector<double>{0.0}, st);
+    else panel.upload(h.panel.data(), h.panel.size(), st);
     part.alloc((size_t)std::max<int64_t>(1, h.slots) * SCHUR_TILE * KMAX);
     ffin.upload(h.ffin.empty() ? std::vector<SnItem>(1) : h.ffin, st);
     bfin.upload(h.bfin.empty() ? std::vector<SnItem>(1) : h.bfin, st);
