@@ -740,7 +740,15 @@ int gt_stack_lp_autotune(gt_stack* st, int32_t reps, int32_t* rw_out, int32_t* c
         GT_HIP(hipSetDevice(st->ctx->device));
         hipStream_t s = st->ctx->stream;
         const int nrs[3] = {1, 4, 8};
-        if (st->ns > 0) {
+        // the shapes choose among the one-wave-per-transform passes, which exist for transforms of 256 to 1024
+        // points: with every transform longer, every shape launches the same kernels and there is nothing to time
+        bool shaped = false;
+        {
+            const LpArgs<double> a0 = st->lp_args(0);
+            for (int oi = 0; oi < 2; ++oi)
+                if (a0.nblk_rows[oi] > 0 && (a0.o[oi].P0 <= 1024 || a0.o[oi].P1 <= 1024)) shaped = true;
+        }
+        if (st->ns > 0 && shaped) {
             const int cap0 = st->work.cap;          // the products' workspace goes back to this afterwards
             st->work.ensure(st, 8);
             DevBuf<zd> X((size_t)st->ns * 8), Y((size_t)st->ns * 8);
