@@ -266,71 +266,113 @@ __global__ __launch_bounds__(256) void dn_panel_cc(const double* __restrict__ X,
 void Schur::factor_device(hipStream_t st) {
     SchurHost& H = h;
     if (H.dev_order.empty()) return;
-    DevBuf<int> flag(1);
-    GT_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
-    std::map<int, DevBuf<double>> fronts;       // device fronts waiting for their parent
     const int nd = (int)H.dev_order.size();
+    const int ns = (int)H.nc.size();
+    // Everything the fronts read from the host goes up once, before the first front: the original entries, each
+    // device front's children's row positions in it, and the update matrices of its host children. The fronts then
+    // run back to back on the stream, with no wait between them.
+    std::vector<int64_t> map_at(ns, -1), u_at(ns, -1);
+    std::vector<int> maps;
+    int64_t utot = 0;
+    int cmax = 1, bmax = 1;
+    for (int di = 0; di < nd; ++di) {
+        const int s = H.dev_order[di];
+        const int C = H.nc[s], R = H.nr[s];
+        cmax = std::max(cmax, C);
+        bmax = std::max(bmax, (C + DB - 1) / DB);
+        for (int e = H.child_ptr[s]; e < H.child_ptr[s + 1]; ++e) {
+            const int k = H.child[e], r = H.nr[k];
+            if (r == 0) continue;
+            map_at[k] = (int64_t)maps.size();
+            for (int a = 0; a < r; ++a) {
+                const int c = H.rows[H.roff[k] + a];
+                if (c < H.c0[s] + C) {
+                    maps.push_back(c - H.c0[s]);
+                } else {
+                    const int* b0 = H.rows.data() + H.roff[s];
+                    const int* f = std::lower_bound(b0, b0 + R, c);
+                    if (f == b0 + R || *f != c) fail(GT_ERR_ARG, "a child's row is not in its parent's front");
+                    maps.push_back(C + (int)(f - b0));
+                }
+            }
+            if (!H.on_dev[k]) {
+                if (H.upd_host[k].size() != (size_t)r * r) fail(GT_ERR_ARG, "a host child's update matrix is missing");
+                u_at[k] = utot;
+                utot += (int64_t)r * r;
+            }
+        }
+    }
+    const int64_t nt = H.trip_off[nd];
+    DevBuf<int> flag(1), tr((size_t)std::max<int64_t>(1, nt)), tc((size_t)std::max<int64_t>(1, nt));
+    DevBuf<double> tv((size_t)std::max<int64_t>(1, nt));
+    DevBuf<int> mp(std::max<size_t>(1, maps.size()));
+    DevBuf<double> Uh((size_t)std::max<int64_t>(1, utot));
+    GT_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+    if (nt > 0) {
+        tr.upload(H.trip_r.data(), (size_t)nt, st);
+        tc.upload(H.trip_c.data(), (size_t)nt, st);
+        tv.upload(H.trip_v.data(), (size_t)nt, st);
+    }
+    if (!maps.empty()) mp.upload(maps.data(), maps.size(), st);
+    for (int k = 0; k < ns; ++k)
+        if (u_at[k] >= 0)
+            GT_HIP(hipMemcpyAsync(Uh.p + u_at[k], H.upd_host[k].data(), H.upd_host[k].size() * sizeof(double),
+                                  hipMemcpyHostToDevice, st));
+    GT_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < ns; ++k)
+        if (u_at[k] >= 0) std::vector<double>().swap(H.upd_host[k]);
+    // per-front scratch, sized for the largest front and reused in stream order
+    DevBuf<double> dinv((size_t)bmax * DB * DB), X((size_t)cmax * cmax);
+    // fronts waiting for their parent, and the buffers of consumed ones kept for reuse (stream order makes a
+    // buffer's next front start after its last reader)
+    std::map<int, DevBuf<double>> fronts;
+    std::multimap<size_t, DevBuf<double>> pool;
+    auto take = [&](size_t n) {
+        auto it = pool.lower_bound(n);
+        if (it != pool.end() && it->first <= 2 * n + (size_t(1) << 20)) {
+            DevBuf<double> b = std::move(it->second);
+            pool.erase(it);
+            return b;
+        }
+        return DevBuf<double>(n);
+    };
     for (int di = 0; di < nd; ++di) {
         const int s = H.dev_order[di];
         const int C = H.nc[s], R = H.nr[s], m = C + R;
-        DevBuf<double> F((size_t)m * m);
-        GT_HIP(hipMemsetAsync(F.p, 0, F.bytes(), st));
-        // 1. original entries
+        DevBuf<double> F = take((size_t)m * m);
+        GT_HIP(hipMemsetAsync(F.p, 0, (size_t)m * m * sizeof(double), st));
+        // 1. original entries, then the children's update matrices at their rows' positions in this front
         {
             const int64_t e0 = H.trip_off[di], e1 = H.trip_off[di + 1];
             const int n = (int)(e1 - e0);
             if (n > 0) {
-                DevBuf<int> tr(n), tc(n);
-                DevBuf<double> tv(n);
-                tr.upload(H.trip_r.data() + e0, n, st);
-                tc.upload(H.trip_c.data() + e0, n, st);
-                tv.upload(H.trip_v.data() + e0, n, st);
-                hipLaunchKernelGGL(dn_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, F.p, m, tr.p, tc.p, tv.p, n);
+                hipLaunchKernelGGL(dn_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, F.p, m, tr.p + e0,
+                                   tc.p + e0, tv.p + e0, n);
                 GT_LAUNCH("dn_scatter");
-                GT_HIP(hipStreamSynchronize(st));
             }
         }
-        //    children's update matrices, at their rows' positions in this front
         for (int e = H.child_ptr[s]; e < H.child_ptr[s + 1]; ++e) {
             const int k = H.child[e], r = H.nr[k];
             if (r == 0) continue;
-            std::vector<int> map(r);
-            for (int a = 0; a < r; ++a) {
-                const int c = H.rows[H.roff[k] + a];
-                if (c < H.c0[s] + C) {
-                    map[a] = c - H.c0[s];
-                } else {
-                    const int* b = H.rows.data() + H.roff[s];
-                    const int* f = std::lower_bound(b, b + R, c);
-                    if (f == b + R || *f != c) fail(GT_ERR_ARG, "a child's row is not in its parent's front");
-                    map[a] = C + (int)(f - b);
-                }
-            }
-            DevBuf<int> mp(r);
-            mp.upload(map.data(), r, st);
+            const int* map = mp.p + map_at[k];
             if (H.on_dev[k]) {
                 auto it = fronts.find(k);
                 if (it == fronts.end()) fail(GT_ERR_ARG, "a device child's front is missing");
                 const int mk = H.nc[k] + r;
                 const double* U = it->second.p + (size_t)H.nc[k] * mk + H.nc[k];
-                hipLaunchKernelGGL(dn_extend, dim3(blocks_for(r, 256), r), dim3(256), 0, st, F.p, m, U, mk, r, mp.p);
+                hipLaunchKernelGGL(dn_extend, dim3(blocks_for(r, 256), r), dim3(256), 0, st, F.p, m, U, mk, r, map);
                 GT_LAUNCH("dn_extend");
-                GT_HIP(hipStreamSynchronize(st));
+                DevBuf<double> done = std::move(it->second);
                 fronts.erase(it);
+                pool.emplace(done.n, std::move(done));
             } else {
-                const std::vector<double>& Uh = H.upd_host[k];
-                if (Uh.size() != (size_t)r * r) fail(GT_ERR_ARG, "a host child's update matrix is missing");
-                DevBuf<double> U((size_t)r * r);
-                U.upload(Uh.data(), Uh.size(), st);
-                hipLaunchKernelGGL(dn_extend, dim3(blocks_for(r, 256), r), dim3(256), 0, st, F.p, m, U.p, r, r, mp.p);
+                hipLaunchKernelGGL(dn_extend, dim3(blocks_for(r, 256), r), dim3(256), 0, st, F.p, m, Uh.p + u_at[k], r,
+                                   r, map);
                 GT_LAUNCH("dn_extend");
-                GT_HIP(hipStreamSynchronize(st));
-                std::vector<double>().swap(H.upd_host[k]);
             }
         }
         // 2. partial Cholesky of the first C columns
         const int nblk = (C + DB - 1) / DB;
-        DevBuf<double> dinv((size_t)std::max(1, nblk) * DB * DB);
         for (int b = 0; b < nblk; ++b) {
             const int j0 = b * DB, nb = std::min(DB, C - j0);
             hipLaunchKernelGGL(dn_potrf, dim3(1), dim3(256), 0, st, F.p, m, j0, nb, dinv.p + (size_t)b * DB * DB,
@@ -341,15 +383,14 @@ void Schur::factor_device(hipStream_t st) {
                 hipLaunchKernelGGL(dn_trsm, dim3(blocks_for(below, 256)), dim3(256), 0, st, F.p, m, j0, nb,
                                    dinv.p + (size_t)b * DB * DB);
                 GT_LAUNCH("dn_trsm");
-                const int nt = (below + DT - 1) / DT;
-                hipLaunchKernelGGL(dn_syrk, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, F.p, m, j0, nb);
+                const int ntl = (below + DT - 1) / DT;
+                hipLaunchKernelGGL(dn_syrk, dim3(ntl * (ntl + 1) / 2), dim3(256), 0, st, F.p, m, j0, nb);
                 GT_LAUNCH("dn_syrk");
             }
         }
         // 3. the solve panel into the factor's panel buffer
         // X above its diagonal blocks stays zero: the panel product's tiles read it from their first column on
-        DevBuf<double> X((size_t)C * C);
-        GT_HIP(hipMemsetAsync(X.p, 0, X.bytes(), st));
+        GT_HIP(hipMemsetAsync(X.p, 0, (size_t)C * C * sizeof(double), st));
         for (int bi = 0; bi < nblk; ++bi) {
             hipLaunchKernelGGL(dn_inv_row, dim3(bi + 1), dim3(256), 0, st, F.p, m, C, bi, dinv.p, X.p);
             GT_LAUNCH("dn_inv_row");
@@ -362,9 +403,10 @@ void Schur::factor_device(hipStream_t st) {
                                X.p, P);
             GT_LAUNCH("dn_panel_rc");
         }
-        GT_HIP(hipStreamSynchronize(st));
         if (H.parent[s] >= 0 && R > 0) fronts.emplace(s, std::move(F));
+        else pool.emplace(F.n, std::move(F));
     }
+    GT_HIP(hipStreamSynchronize(st));
     int bad = 0;
     GT_HIP(hipMemcpy(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) fail(GT_ERR_NUMERIC, "the Schur complement is not positive definite (a piece without a reference node?)");
