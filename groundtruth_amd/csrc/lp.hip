@@ -491,7 +491,7 @@ bool launch_rows_w_rw(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t 
     const int blocks = nb[0] + nb[1];
     if (blocks == 0) return false;
     const size_t lds = (size_t)RW * (P1 + P1 / 8) * sizeof(cpx<T>);
-    if (lds > 65536) return false;            // (P1 = 1024 in FP64 with 4 rows) the LDS kernels take it
+    if (lds > 65536) return false;            // (P1 = 1024 in FP64 with 4 rows) a smaller tile takes it
     const int grid = (blocks + 7) / 8 * 8;
     if (inverse)
         hipLaunchKernelGGL((lp_rows_inv_w<T, E, RW>), dim3(grid, nr_rhs), dim3(64 * RW), lds, st, a, nb[0], blocks);
@@ -500,12 +500,20 @@ bool launch_rows_w_rw(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t 
     return true;
 }
 
+// rw rows per workgroup, or fewer where rw rows do not fit 64 KB of LDS: a row length that has the register form
+// takes it for every plan and batch size (the LDS passes split the transform into other radices, so they would round
+// differently and a right-hand side would not get the same bits alone and batched)
 template <class T, int E>
 bool launch_rows_w_e(const LpArgs<T>& a, int nr_rhs, bool inverse, hipStream_t st, int P1, int rw) {
     switch (rw) {
     case 1: return launch_rows_w_rw<T, E, 1>(a, nr_rhs, inverse, st, P1);
-    case 2: return launch_rows_w_rw<T, E, 2>(a, nr_rhs, inverse, st, P1);
-    default: return launch_rows_w_rw<T, E, 4>(a, nr_rhs, inverse, st, P1);
+    case 2:
+        return launch_rows_w_rw<T, E, 2>(a, nr_rhs, inverse, st, P1) ||
+               launch_rows_w_rw<T, E, 1>(a, nr_rhs, inverse, st, P1);
+    default:
+        return launch_rows_w_rw<T, E, 4>(a, nr_rhs, inverse, st, P1) ||
+               launch_rows_w_rw<T, E, 2>(a, nr_rhs, inverse, st, P1) ||
+               launch_rows_w_rw<T, E, 1>(a, nr_rhs, inverse, st, P1);
     }
 }
 
