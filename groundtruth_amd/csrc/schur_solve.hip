@@ -601,9 +601,29 @@ void Schur::upload(hipStream_t st) {
     cidx.upload(h.cidx.empty() ? std::vector<int64_t>{0} : h.cidx, st);
     fwd.upload(h.fwd.empty() ? std::vector<SnItem>(1) : h.fwd, st);
     bwd.upload(h.bwd.empty() ? std::vector<SnItem>(1) : h.bwd, st);
+    // the host fronts' panels (the device fronts' are written on the device): runs of consecutive host fronts
     panel.alloc(std::max<size_t>(1, h.panel.size()));
-    if (h.panel.empty()) panel.upload(std::vector<double>{0.0}, st);
-    else panel.upload(h.panel.data(), h.panel.size(), st);
+    if (h.panel.empty()) {
+        panel.upload(std::vector<double>{0.0}, st);
+    } else if (h.dev_order.empty()) {
+        panel.upload(h.panel.data(), h.panel.size(), st);
+    } else {
+        const int ns = (int)h.nc.size();
+        for (int s = 0; s < ns;) {
+            if (h.on_dev[s]) {
+                ++s;
+                continue;
+            }
+            int e = s;
+            while (e < ns && !h.on_dev[e]) ++e;
+            const int64_t a = h.poff[s], b = h.poff[e];
+            if (b > a)
+                GT_HIP(hipMemcpyAsync(panel.p + a, h.panel.data() + a, (size_t)(b - a) * sizeof(double),
+                                      hipMemcpyHostToDevice, st));
+            s = e;
+        }
+        GT_HIP(hipStreamSynchronize(st));
+    }
     part.alloc((size_t)std::max<int64_t>(1, h.slots) * SCHUR_TILE * KMAX);
     ffin.upload(h.ffin.empty() ? std::vector<SnItem>(1) : h.ffin, st);
     bfin.upload(h.bfin.empty() ? std::vector<SnItem>(1) : h.bfin, st);
