@@ -704,7 +704,7 @@ void woodbury_extend(gt_stack* st, const gt_stack* base, int n_extra, const int*
             hipLaunchKernelGGL(wb_units, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, sb.p, nk, nr, u);
             GT_LAUNCH("wb_units");
             if (st->wb_root) {
-                schur_forward_device(*st->schur, sb.p, nr, s);
+                schur_forward_last_device(*st->schur, sb.p, nr, s);
                 hipLaunchKernelGGL(wb_take_root, dim3(blocks_for((size_t)st->yr_n * nr, 256)), dim3(256), 0, s,
                                    st->schur->zp.p, nr, st->yr_c0, st->yr_n, m0, st->Yr.p);
                 GT_LAUNCH("wb_take_root");

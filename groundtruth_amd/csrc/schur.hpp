@@ -219,5 +219,10 @@ void schur_solve_device(Schur& S, const zd* b, zd* x, int nr, hipStream_t s, boo
 // a caller may change zp (the Woodbury correction on the last supernode's rows).
 void schur_forward_device(Schur& S, const zd* b, int nr, hipStream_t s);
 void schur_backward_device(Schur& S, zd* x, int nr, hipStream_t s);
+// The forward pass for right-hand sides that vanish outside the last supernode's columns (the root Woodbury form's
+// unit columns): every other supernode's front is zero, so its update entries are cleared instead of computed and
+// only the last supernode's level runs (or the dense top, when it holds it). zp then holds, on the last supernode's
+// rows, the same bits as after schur_forward_device.
+void schur_forward_last_device(Schur& S, const zd* b, int nr, hipStream_t s);
 
 }  // namespace gt

@@ -12,6 +12,7 @@
 // split over several chunks writes partial sums, which a finalize launch after the level adds in chunk order. Every
 // sum runs in a fixed order that does not depend on the batch: a right-hand side gets the same
 // bits alone or in any batch.
+#include <cstdint>
 #include <type_traits>
 
 #include "lp.hpp"
@@ -543,13 +544,14 @@ void run_top_bwd(const Schur& S, zd* x, int nr, hipStream_t st) {
     }
 }
 
-// the forward levels of `ptr` / `items` (the tree below the dense top, or the top's own supernodes)
+// the forward levels [l0, l1) of `ptr` / `items` (the tree below the dense top, or the top's own supernodes)
 template <int KB>
 void run_fwd_items(const Schur& S, SolveArgs a, int nr, hipStream_t st, const std::vector<int>& fp,
-                   const std::vector<int>& cols, const std::vector<int>& finp, const SnItem* items, const SnItem* fins) {
+                   const std::vector<int>& cols, const std::vector<int>& finp, const SnItem* items, const SnItem* fins,
+                   size_t l0 = 0, size_t l1 = SIZE_MAX) {
     constexpr size_t row_bytes = (KB + 1) * sizeof(zd);
     a.item = items;
-    for (size_t l = 0; l + 1 < fp.size(); ++l) {
+    for (size_t l = l0; l + 1 < fp.size() && l < l1; ++l) {
         const int cnt = fp[l + 1] - fp[l];
         if (cnt == 0) continue;
         hipLaunchKernelGGL(schur_fwd<KB>, dim3(cnt), dim3(NTH), cols[l] * row_bytes, st, a, fp[l], nr);
@@ -752,6 +754,49 @@ void schur_forward_device(Schur& S, const zd* b, int nr, hipStream_t st) {
     if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
     S.ensure(nr);
     launch_passes(S, b, nullptr, nr, st, 1);
+}
+
+namespace {
+
+__global__ __launch_bounds__(NTH) void upd_zero(zd* v, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)NTH + threadIdx.x; e < n; e += (size_t)gridDim.x * NTH) v[e] = zd();
+}
+
+}  // namespace
+
+void schur_forward_last_device(Schur& S, const zd* b, int nr, hipStream_t st) {
+    if (nr < 1 || nr > KMAX) fail(GT_ERR_ARG, "Schur solve: 1 to 16 right-hand sides");
+    const int ns = (int)S.h.nc.size();
+    if (ns == 0) return;
+    S.ensure(nr);
+    // every supernode but the last has a zero front: its update entries are cleared, not computed
+    hipLaunchKernelGGL(upd_zero, dim3(1024), dim3(NTH), 0, st, S.ubuf.p, S.ubuf.n);
+    GT_LAUNCH("upd_zero");
+    if (S.h.top_T) {            // the last supernode is the top's root: the dense top alone
+        run_top_fwd(S, b, nr, st);
+        return;
+    }
+    size_t lvl = SIZE_MAX;      // the level holding the last supernode
+    const std::vector<int>& fp = S.h.fwd_ptr;
+    for (size_t l = 0; l + 1 < fp.size() && lvl == SIZE_MAX; ++l)
+        for (int i = fp[l]; i < fp[l + 1]; ++i)
+            if (S.h.fwd[i].s == ns - 1) {
+                lvl = l;
+                break;
+            }
+    if (lvl == SIZE_MAX) fail(GT_ERR_ARG, "the last supernode has no forward items");
+    SolveArgs a{nullptr, S.perm.p, S.rows.p, reinterpret_cast<const int2*>(S.ctab.p), S.cidx.p, S.panel.p, S.ubuf.p,
+                S.zp.p, S.xp.p, b, nullptr, S.part.p};
+    auto go = [&](auto kb) {
+        constexpr int KB = decltype(kb)::value;
+        run_fwd_items<KB>(S, a, nr, st, fp, S.h.fwd_cols, S.h.ffin_ptr, S.fwd.p, S.ffin.p, lvl, lvl + 1);
+    };
+    if (nr == 1) go(std::integral_constant<int, 1>());
+    else if (nr == 2) go(std::integral_constant<int, 2>());
+    else if (nr <= 4) go(std::integral_constant<int, 4>());
+    else if (nr <= 8) go(std::integral_constant<int, 8>());
+    else if (nr <= 12) go(std::integral_constant<int, 12>());
+    else go(std::integral_constant<int, 16>());
 }
 
 void schur_backward_device(Schur& S, zd* x, int nr, hipStream_t st) {
