@@ -69,15 +69,20 @@ __global__ __launch_bounds__(256) void dn_potrf(double* __restrict__ F, int m, i
         }
         __syncthreads();
     }
-    // inverse of the lower triangle, a column per thread
+    // inverse of the lower triangle, a column per thread, the column in registers (its entries above c are zero, so
+    // each row's sum over k < i adds zeros before the sum over c <= k < i) and every A read a broadcast
     if (tid < nb) {
         const int c = tid;
-        X[c][c] = 1.0 / A[c][c];
-        for (int i = c + 1; i < nb; ++i) {
+        double x[DB];
+#pragma unroll
+        for (int i = 0; i < DB; ++i) {
             double s = 0.0;
-            for (int k = c; k < i; ++k) s += A[i][k] * X[k][c];
-            X[i][c] = -s / A[i][i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) s += A[i][k] * x[k];
+            x[i] = i < c || i >= nb ? 0.0 : i == c ? 1.0 / A[i][i] : -s / A[i][i];
         }
+#pragma unroll
+        for (int i = 0; i < DB; ++i) X[i][c] = x[i];
     }
     __syncthreads();
     for (int e = tid; e < DB * DB; e += 256) {
