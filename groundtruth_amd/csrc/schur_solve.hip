@@ -45,6 +45,9 @@ struct SolveArgs {
 // the registers)
 template <int KB>
 constexpr int stage_depth() { return KB >= 8 ? 4 : 8; }
+// the forward pass at one right-hand side stages four, so that it fits 64 VGPRs and a CU holds 32 of its waves
+template <int KB>
+constexpr int fwd_stage_depth() { return KB == 1 ? 4 : stage_depth<KB>(); }
 
 // acc + the children's update entries at a front position (right-hand side p of nr), in the contributors' fixed order
 __device__ __forceinline__ zd add_contrib(const SolveArgs& a, int2 ct, int p, int nr, zd acc) {
@@ -80,7 +83,7 @@ __device__ __forceinline__ void mac_row(double (&are)[KB], double (&aim)[KB], do
 // rows t0 + 16 w ..), or rows [t0, t0 + 16) with the columns split 16 ways over the four waves (kw = 4), whose sums
 // meet in LDS in wave order; columns [k0, k1). Lane = (row rho = lane & 15, column phase lane >> 4).
 template <int KB>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void schur_fwd(SolveArgs a, int item0,
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(KB == 1 ? 8 : 4))) void schur_fwd(SolveArgs a, int item0,
                                                                                           int nr) {
     extern __shared__ double lds_raw[];
     zd* f = reinterpret_cast<zd*>(lds_raw);
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void s
     }
     const int2 cr = direct && (kw == 1 || w == 0) && r < m && r >= C ? ct[r] : int2{-1, -1};
     {
-        constexpr int U = stage_depth<KB>();
+        constexpr int U = fwd_stage_depth<KB>();
         const int tot = (k1 - k0) * KB;
         for (int e0 = threadIdx.x; e0 < tot; e0 += NTH * U) {
             int pm[U];
