@@ -60,24 +60,33 @@ struct PreArgs {
 // s = r2 - Ar^T (D^-1 r1): a kept node's incident strips lie on its own sheet (one admittance), its branches carry
 // their own
 template <class T>
-__global__ __launch_bounds__(256) void pre_schur_rhs(PreArgs<T> a) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= a.nk) return;
-    const int sh = a.node_sheet[n];
-    for (int p = 0; p < a.nr; ++p) {
-        const cpx<T>* r = a.r[p];
-        const int e0 = a.adj_ptr[p][n], e1 = a.adj_ptr[p][n + 1];
-        const int* adj = a.adj[p];
-        zd acc = ldz(r[a.nb[p] + n]);
-        const zd ds = a.dinv_sheet[p][sh];
-        for (int e = e0; e < e1; ++e) {
-            const int code = adj[e], b = code >> 1;
-            const zd v = (b < a.ns ? ds : a.dinv_tail[p][b - a.ns]) * ldz(r[b]);
-            if (code & 1) acc = acc + v;
-            else acc = acc - v;
+__global__ __launch_bounds__(64) void pre_schur_rhs(PreArgs<T> a) {
+    // a wave's 64 nodes: their right-hand sides staged in LDS, then written as one contiguous run of s (the
+    // interleaved rows of consecutive nodes) instead of one strided store per right-hand side
+    __shared__ double stg_raw[2 * 64 * KMAX];
+    zd* stg = reinterpret_cast<zd*>(stg_raw);
+    const int n0 = blockIdx.x * 64, n = n0 + threadIdx.x;
+    if (n < a.nk) {
+        const int sh = a.node_sheet[n];
+        for (int p = 0; p < a.nr; ++p) {
+            const cpx<T>* r = a.r[p];
+            const int e0 = a.adj_ptr[p][n], e1 = a.adj_ptr[p][n + 1];
+            const int* adj = a.adj[p];
+            zd acc = ldz(r[a.nb[p] + n]);
+            const zd ds = a.dinv_sheet[p][sh];
+            for (int e = e0; e < e1; ++e) {
+                const int code = adj[e], b = code >> 1;
+                const zd v = (b < a.ns ? ds : a.dinv_tail[p][b - a.ns]) * ldz(r[b]);
+                if (code & 1) acc = acc + v;
+                else acc = acc - v;
+            }
+            stg[threadIdx.x * a.nr + p] = a.s_in[p] ? a.s_in[p][sh] * acc : acc;
         }
-        a.s[(size_t)n * a.nr + p] = a.s_in[p] ? a.s_in[p][sh] * acc : acc;
     }
+    __syncthreads();
+    const int cnt = min(64, a.nk - n0) * a.nr;
+    zd* dst = a.s + (size_t)n0 * a.nr;
+    for (int e = threadIdx.x; e < cnt; e += 64) dst[e] = stg[e];
 }
 
 // t_p = Q_p (U_p^T z_p): one workgroup per right-hand side (k, the branch count, is small)
@@ -797,7 +806,7 @@ void ops_precond(gt_stack* st, int nr, gt_op* const* ops, const cpx<T>* const* r
     a.node_sheet = st->node_sheet.p;
     a.strip_sheet = st->strip_sheet.p;
     if (st->nk > 0) {
-        hipLaunchKernelGGL(pre_schur_rhs<T>, dim3(blocks_for(st->nk, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(pre_schur_rhs<T>, dim3(blocks_for(st->nk, 64)), dim3(64), 0, s, a);
         GT_LAUNCH("pre_schur_rhs");
         if (imp && a.kt > 0) {
             // the host stack's own right-hand sides in the root form, the others (what-ifs) with a second solve;
